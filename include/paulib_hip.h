@@ -1,0 +1,148 @@
+/*
+ * paulib_hip.h -- C ABI of the MI355X PauliSum engine (libpaulib_hip.so).
+ *
+ * Plain pointers, sizes and integer status codes; no C++ or torch types.
+ * Every call returns PLH_OK (0) or a negative PLH_ERR_* code; the text of
+ * the last error on the calling thread is plh_last_error().  Nothing throws
+ * across this boundary.
+ *
+ * Layout.  A Pauli sum of AS words per plane (AS = ceil(qubits/64)) is the
+ * reference's flat plane-major SoA (include/gpu/GpuPauliSum.h:8-12):
+ *   x[k*stride + t], z[k*stride + t]   word k of term t (qubit 64k+b = bit b)
+ *   flags[t]                           bit0 = i, bit1 = -1 (i^flags)
+ *   val[t]                             FP64 coefficient
+ * Host arrays passed to upload/download are packed (stride = length).
+ *
+ * Ordering.  Every result has a deterministic term order: products are
+ * out[i*n + j] = lhs[j] * rhs[i]; commutators list anticommuting pairs in
+ * (i over a, j over b) order; collated results are sorted by the
+ * reference's key (x words from word 0, z words, flags; unsigned).
+ *
+ * Streams.  Each context owns one HIP stream; all work of a context is
+ * ordered on it.  Calls that return a size synchronise that stream.  A
+ * context must not be used from two threads at once.
+ */
+#ifndef PAULIB_HIP_H
+#define PAULIB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PLH_OK = 0,
+    PLH_ERR_ARG = -1,   /* invalid argument (null, qubit out of range, ...) */
+    PLH_ERR_AS = -2,    /* unsupported words-per-plane (supported: 1..16) */
+    PLH_ERR_SIZE = -3,  /* size limit exceeded (see each call) */
+    PLH_ERR_HIP = -4,   /* HIP runtime error */
+    PLH_ERR_OOM = -5,   /* device allocation failed or admission refused */
+    PLH_ERR_NOGPU = -6  /* no usable GPU */
+};
+
+typedef struct plh_ctx plh_ctx; /* device + stream + scratch */
+typedef struct plh_sum plh_sum; /* device-resident PauliSum<N,double> */
+
+/* ------------------------------------------------------------------ misc */
+const char* plh_last_error(void);
+int plh_version(void);
+/* Engine build information (gfx target, flags); never fails. */
+const char* plh_build_info(void);
+
+/* ------------------------------------------------------------ contexts */
+/* stream == NULL creates a private stream at the default priority. */
+int plh_ctx_create(int device, void* stream, plh_ctx** out);
+int plh_ctx_destroy(plh_ctx* ctx);
+int plh_ctx_synchronize(plh_ctx* ctx);
+void* plh_ctx_stream(plh_ctx* ctx);
+/* Device scratch held by the context (bytes); plh_ctx_trim frees it. */
+int64_t plh_ctx_scratch_bytes(plh_ctx* ctx);
+int plh_ctx_trim(plh_ctx* ctx);
+
+/* --------------------------------------------------------- device sums */
+/* GpuPauliSum<N,double> (include/gpu/GpuPauliSum.h:34-148): device
+ * buffers owned by the engine.  capacity is a hint. */
+int plh_sum_create(plh_ctx* ctx, int array_size, int64_t num_qubits, int64_t capacity,
+                   plh_sum** out);
+int plh_sum_destroy(plh_sum* s);
+/* GpuPauliSum::upload (:75-92); host planes packed x[k*n + t]. */
+int plh_sum_upload(plh_sum* s, int64_t n, const uint64_t* x, const uint64_t* z,
+                   const uint8_t* flags, const double* val);
+/* GpuPauliSum::download (:95-114); arrays sized for plh_sum_length(). */
+int plh_sum_download(const plh_sum* s, uint64_t* x, uint64_t* z, uint8_t* flags, double* val);
+int64_t plh_sum_length(const plh_sum* s);
+int64_t plh_sum_num_qubits(const plh_sum* s);
+int plh_sum_set_num_qubits(plh_sum* s, int64_t num_qubits);
+int plh_sum_array_size(const plh_sum* s);
+/* Raw device planes (valid until the next call that resizes the sum). */
+int plh_sum_device_planes(plh_sum* s, uint64_t** x, uint64_t** z, uint8_t** flags, double** val,
+                          int64_t* stride);
+int plh_sum_copy(const plh_sum* src, plh_sum* dst);
+
+/* --------------------------------------------------- raw launchers */
+/* Replaces launch_gpu_mul_soa (include/gpu/paulib_gpu_api.h:25-33):
+ * out[i*n + j] = lhs[j] * rhs[i], caller-allocated device buffers of n*m
+ * terms, planes strided by n (lhs), m (rhs), n*m (out).  Asynchronous on
+ * `stream`.  array_size 1..16, else PLH_ERR_AS (the reference traps). */
+int plh_launch_mul_soa(int array_size, const uint64_t* lhs_x, const uint64_t* lhs_z,
+                       const uint8_t* lhs_flags, const double* lhs_val, const uint64_t* rhs_x,
+                       const uint64_t* rhs_z, const uint8_t* rhs_flags, const double* rhs_val,
+                       uint64_t* out_x, uint64_t* out_z, uint8_t* out_flags, double* out_val,
+                       int n, int m, void* stream);
+
+/* ------------------------------------------------------- bulk operations */
+/* PauliSum::multiply (include/PauliSumOperations.h:255-306): lhs <- lhs*rhs,
+ * uncombined, n*m terms; keeps lhs num_qubits.  n*m <= 2^32. */
+int plh_multiply(plh_sum* lhs, const plh_sum* rhs);
+/* canonicalize(lhs * rhs) without materialising the uncombined product
+ * (the fused product->collation path).  n*m <= 2^32 pairs. */
+int plh_multiply_canonical(const plh_sum* lhs, const plh_sum* rhs, plh_sum* out);
+/* commutator / i_commutator (include/PauliSumOperations.h:348-356), the
+ * uncombined terms in (i over a, j over b) order; num_qubits = max. */
+int plh_commutator(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out);
+/* canonicalize([a, b]) (or of i[a, b]) fused, without the uncombined list. */
+int plh_commutator_canonical(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out);
+/* PauliSum::sort_and_combine (include/PauliSumOperations.h:368-461). */
+int plh_sort_and_combine(plh_sum* s);
+/* PauliSum::canonicalize (include/PauliSumUtils.h:314-328). */
+int plh_canonicalize(plh_sum* s);
+/* operator+= (include/PauliSumCore.h:232-237): append, sort_and_combine. */
+int plh_add(plh_sum* lhs, const plh_sum* rhs);
+/* PauliSum::zero_state_expectation (include/PauliSumUtils.h:162-177). */
+int plh_zero_state_expectation(const plh_sum* s, double* out);
+/* PauliSum::truncate_weight / cutoff (include/PauliSumUtils.h:377-393). */
+int plh_truncate_weight(plh_sum* s, int64_t max_weight);
+int plh_cutoff(plh_sum* s, double eps);
+/* PauliSum::apply_H / apply_S / apply_CNOT (include/PauliSumClifford.h:
+ * 43-81, 212-239). */
+int plh_apply_H(plh_sum* s, int64_t q);
+int plh_apply_S(plh_sum* s, int64_t q);
+int plh_apply_CNOT(plh_sum* s, int64_t control, int64_t target);
+/* PauliSum::evolve (include/PauliSumOperations.h:106-183): rotation terms
+ * r = 0..nrot-1 in order, Pauli words px/pz[r*as + k], flags pflags[r],
+ * angle theta[r]; cos/sin are taken on the host (libm). */
+int plh_evolve(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,
+               const uint8_t* pflags, const double* theta);
+/* apply_truncation (include/PauliPropagation.h:35-47); max_weight < 0,
+ * cutoff <= 0, max_terms < 0 mean "not set".  max_terms is not supported
+ * yet (PLH_ERR_ARG when set). */
+int plh_apply_truncation(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms);
+/* propagate (include/PauliPropagation.h:53-63): gates in reverse order,
+ * each a one-term evolve then apply_truncation. */
+int plh_propagate(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,
+                  const uint8_t* pflags, const double* theta, int64_t max_weight, double cutoff,
+                  int64_t max_terms);
+/* Heisenberg run of a mixed program (application order, processed last to
+ * first): kind 0 = H(q0), 1 = S(q0), 2 = CNOT(q0, q1), 3 = rotation by
+ * theta about Pauli word px/pz[l*as..]; each rotation is evolve +
+ * apply_truncation as in propagate().  No reference counterpart: it
+ * composes the reference's methods the way a brickwork driver would. */
+int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t* q0,
+                    const int64_t* q1, const uint64_t* px, const uint64_t* pz,
+                    const double* theta, int64_t max_weight, double cutoff, int64_t max_terms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAULIB_HIP_H */

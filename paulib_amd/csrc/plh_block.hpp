@@ -1,0 +1,53 @@
+// plh_block.hpp -- wave/workgroup primitives (256-thread workgroups, wave64).
+#pragma once
+#include "plh_common.hpp"
+
+namespace plh {
+
+template <class T>
+PLH_DEV T wave_inclusive_scan(T v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const T t = __shfl_up(v, o, kWave);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
+template <class T>
+PLH_DEV T wave_sum(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Exclusive scan across the 256 threads of a workgroup; `sh` holds
+// kWavesPerBlock entries.  Contains barriers: call from uniform control flow.
+template <class T>
+PLH_DEV T block_exclusive_scan(T v, T& total, T* sh) {
+    const uint32_t lane = lane_id();
+    const int w = threadIdx.x / kWave;
+    const T inc = wave_inclusive_scan(v);
+    if (lane == kWave - 1) sh[w] = inc;
+    __syncthreads();
+    T pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+        const T s = sh[i];
+        if (i < w) pre += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + inc - v;
+}
+
+template <class T>
+PLH_DEV T block_sum(T v, T* sh) {
+    T total;
+    (void)block_exclusive_scan(v, total, sh);
+    return total;
+}
+
+}  // namespace plh

@@ -1,0 +1,531 @@
+// plh_capi.hip -- the extern "C" boundary (include/paulib_hip.h).
+//
+// Each entry point validates its arguments, runs stream-ordered work on the
+// context's stream and returns a status code; HIP errors become
+// PLH_ERR_HIP/PLH_ERR_OOM with text in plh_last_error() (the reference's
+// wrapper throws std::runtime_error("CUDA: ...") instead,
+// include/gpu/GpuPauliSum.h:25-30; the C++ mirror header turns these codes
+// back into that exception).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "plh_engine.hpp"
+#include "plh_sources.hpp"
+
+namespace plh {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+void sum_free_storage(plh_sum* s) {
+    if (s->x) (void)hipFree(s->x);
+    if (s->z) (void)hipFree(s->z);
+    if (s->f) (void)hipFree(s->f);
+    if (s->v) (void)hipFree(s->v);
+    s->x = s->z = nullptr;
+    s->f = nullptr;
+    s->v = nullptr;
+    s->cap = 0;
+}
+
+int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
+    if (cap <= s->cap && s->x) return PLH_OK;
+    int64_t nc = cap < 16 ? 16 : cap;
+    if (s->cap > 0 && nc < s->cap + s->cap / 2 && keep > 0) nc = s->cap + s->cap / 2;  // amortise appends
+    uint64_t *nx = nullptr, *nz = nullptr;
+    uint8_t* nf = nullptr;
+    double* nv = nullptr;
+    hipError_t e = hipMalloc((void**)&nx, (size_t)s->as * nc * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&nz, (size_t)s->as * nc * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&nf, (size_t)nc);
+    if (e == hipSuccess) e = hipMalloc((void**)&nv, (size_t)nc * 8);
+    if (e != hipSuccess) {
+        (void)hipFree(nx);
+        (void)hipFree(nz);
+        (void)hipFree(nf);
+        (void)hipFree(nv);
+        (void)hipGetLastError();
+        set_error("device allocation of " + std::to_string(nc) + " terms failed: " + hipGetErrorString(e));
+        return PLH_ERR_OOM;
+    }
+    hipStream_t st = s->ctx ? s->ctx->stream : nullptr;
+    if (keep > 0 && s->x) {
+        for (int k = 0; k < s->as; ++k) {
+            PLH_HIP(hipMemcpyAsync(nx + (int64_t)k * nc, s->x + (int64_t)k * s->cap, (size_t)keep * 8,
+                                   hipMemcpyDeviceToDevice, st));
+            PLH_HIP(hipMemcpyAsync(nz + (int64_t)k * nc, s->z + (int64_t)k * s->cap, (size_t)keep * 8,
+                                   hipMemcpyDeviceToDevice, st));
+        }
+        PLH_HIP(hipMemcpyAsync(nf, s->f, (size_t)keep, hipMemcpyDeviceToDevice, st));
+        PLH_HIP(hipMemcpyAsync(nv, s->v, (size_t)keep * 8, hipMemcpyDeviceToDevice, st));
+    }
+    if (s->x) {
+        PLH_HIP(hipStreamSynchronize(st));
+        sum_free_storage(s);
+    }
+    s->x = nx;
+    s->z = nz;
+    s->f = nf;
+    s->v = nv;
+    s->cap = nc;
+    return PLH_OK;
+}
+
+}  // namespace plh
+
+using namespace plh;
+
+namespace {
+
+bool valid_as(int as) { return as >= 1 && as <= 16; }
+
+int check_sum(const plh_sum* s) {
+    if (!s || !s->ctx) {
+        set_error("null sum");
+        return PLH_ERR_ARG;
+    }
+    return PLH_OK;
+}
+
+int check_pair(const plh_sum* a, const plh_sum* b) {
+    PLH_TRY(check_sum(a));
+    PLH_TRY(check_sum(b));
+    if (a->ctx != b->ctx) {
+        set_error("operands belong to different contexts");
+        return PLH_ERR_ARG;
+    }
+    if (a->as != b->as) {
+        set_error("operands have different array sizes");
+        return PLH_ERR_ARG;
+    }
+    return PLH_OK;
+}
+
+CollateSpec canon_spec() {
+    CollateSpec c;
+    c.mode = kFold | kDropZero;
+    return c;
+}
+
+CollateSpec trunc_spec(int64_t max_weight, double cutoff) {
+    CollateSpec c = canon_spec();
+    if (max_weight >= 0) {
+        c.mode |= kMaxWeight;
+        c.max_weight = max_weight;
+    }
+    if (cutoff > 0.0) {
+        c.mode |= kCutoff;
+        c.eps = cutoff;
+    }
+    return c;
+}
+
+// One rotation of evolve + apply_truncation (the body of propagate,
+// include/PauliPropagation.h:56-61).  When the sum is canonical and the
+// general path appended new terms, every (x, z, imag) key holds at most one
+// old and one new term, so evolve's sort_and_combine followed by
+// canonicalize equals a single fold-combine pass (IEEE addition of two
+// terms is commutative and sign-symmetric); the filters are per-term
+// predicates of the canonical terms, so they fuse into the same pass.
+int rotation_step(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf, double theta,
+                  bool truncate, int64_t max_weight, double cutoff) {
+    const double c = std::cos(theta), sn = std::sin(theta);
+    const bool was_canonical = s->canonical;
+    int appended = 0;
+    PLH_TRY(op_rotation(s->ctx, s, px, pz, pf, c, sn, &appended));
+    if (!truncate) {
+        if (appended) {
+            CollateSpec sc;
+            PLH_TRY(collate_terms(s->ctx, s, sc, s));
+            s->canonical = false;
+        }
+        return PLH_OK;
+    }
+    if (appended && !was_canonical) {
+        CollateSpec sc;
+        PLH_TRY(collate_terms(s->ctx, s, sc, s));
+    }
+    return collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* plh_last_error(void) { return last_error(); }
+int plh_version(void) { return 1; }
+const char* plh_build_info(void) {
+    return "libpaulib_hip: gfx950 (wave64), -O3 -ffp-contract=off, no packed-fp32";
+}
+
+int plh_ctx_create(int device, void* stream, plh_ctx** out) {
+    if (!out) return PLH_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device visible");
+        return PLH_ERR_NOGPU;
+    }
+    if (device < 0 || device >= ndev) {
+        set_error("device index out of range");
+        return PLH_ERR_ARG;
+    }
+    PLH_HIP(hipSetDevice(device));
+    plh_ctx* c = new plh_ctx();
+    c->device = device;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            set_error(std::string("HIP: ") + hipGetErrorString(e));
+            return PLH_ERR_HIP;
+        }
+        c->own_stream = true;
+    }
+    *out = c;
+    return PLH_OK;
+}
+
+int plh_ctx_destroy(plh_ctx* ctx) {
+    if (!ctx) return PLH_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ws.release();
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PLH_OK;
+}
+
+int plh_ctx_synchronize(plh_ctx* ctx) {
+    if (!ctx) return PLH_ERR_ARG;
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
+    return PLH_OK;
+}
+
+void* plh_ctx_stream(plh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int64_t plh_ctx_scratch_bytes(plh_ctx* ctx) { return ctx ? (int64_t)ctx->ws.bytes_held() : 0; }
+
+int plh_ctx_trim(plh_ctx* ctx) {
+    if (!ctx) return PLH_ERR_ARG;
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->ws.release();
+    return PLH_OK;
+}
+
+int plh_sum_create(plh_ctx* ctx, int array_size, int64_t num_qubits, int64_t capacity, plh_sum** out) {
+    if (!ctx || !out) return PLH_ERR_ARG;
+    *out = nullptr;
+    if (!valid_as(array_size)) {
+        set_error("unsupported array_size " + std::to_string(array_size) + " (supported: 1..16)");
+        return PLH_ERR_AS;
+    }
+    if (num_qubits < 0 || num_qubits > 64 * (int64_t)array_size) {
+        set_error("qubit count exceeds capacity");
+        return PLH_ERR_ARG;
+    }
+    plh_sum* s = new plh_sum();
+    s->ctx = ctx;
+    s->as = array_size;
+    s->nq = num_qubits;
+    int rc = sum_reserve(s, capacity > 0 ? capacity : 16, 0);
+    if (rc != PLH_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return PLH_OK;
+}
+
+int plh_sum_destroy(plh_sum* s) {
+    if (!s) return PLH_OK;
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    sum_free_storage(s);
+    delete s;
+    return PLH_OK;
+}
+
+int plh_sum_upload(plh_sum* s, int64_t n, const uint64_t* x, const uint64_t* z, const uint8_t* flags,
+                   const double* val) {
+    PLH_TRY(check_sum(s));
+    if (n < 0 || (n > 0 && (!x || !z || !flags || !val))) return PLH_ERR_ARG;
+    PLH_TRY(sum_reserve(s, n, 0));
+    hipStream_t st = s->ctx->stream;
+    for (int k = 0; k < s->as && n > 0; ++k) {
+        PLH_HIP(hipMemcpyAsync(s->x + (int64_t)k * s->cap, x + (int64_t)k * n, (size_t)n * 8,
+                               hipMemcpyHostToDevice, st));
+        PLH_HIP(hipMemcpyAsync(s->z + (int64_t)k * s->cap, z + (int64_t)k * n, (size_t)n * 8,
+                               hipMemcpyHostToDevice, st));
+    }
+    if (n > 0) {
+        PLH_HIP(hipMemcpyAsync(s->f, flags, (size_t)n, hipMemcpyHostToDevice, st));
+        PLH_HIP(hipMemcpyAsync(s->v, val, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    }
+    PLH_HIP(hipStreamSynchronize(st));
+    s->n = n;
+    s->canonical = false;
+    return PLH_OK;
+}
+
+int plh_sum_download(const plh_sum* s, uint64_t* x, uint64_t* z, uint8_t* flags, double* val) {
+    PLH_TRY(check_sum(s));
+    const int64_t n = s->n;
+    if (n > 0 && (!x || !z || !flags || !val)) return PLH_ERR_ARG;
+    hipStream_t st = s->ctx->stream;
+    for (int k = 0; k < s->as && n > 0; ++k) {
+        PLH_HIP(hipMemcpyAsync(x + (int64_t)k * n, s->x + (int64_t)k * s->cap, (size_t)n * 8,
+                               hipMemcpyDeviceToHost, st));
+        PLH_HIP(hipMemcpyAsync(z + (int64_t)k * n, s->z + (int64_t)k * s->cap, (size_t)n * 8,
+                               hipMemcpyDeviceToHost, st));
+    }
+    if (n > 0) {
+        PLH_HIP(hipMemcpyAsync(flags, s->f, (size_t)n, hipMemcpyDeviceToHost, st));
+        PLH_HIP(hipMemcpyAsync(val, s->v, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    }
+    PLH_HIP(hipStreamSynchronize(st));
+    return PLH_OK;
+}
+
+int64_t plh_sum_length(const plh_sum* s) { return s ? s->n : -1; }
+int64_t plh_sum_num_qubits(const plh_sum* s) { return s ? s->nq : -1; }
+int plh_sum_set_num_qubits(plh_sum* s, int64_t q) {
+    PLH_TRY(check_sum(s));
+    if (q < 0 || q > 64 * (int64_t)s->as) return PLH_ERR_ARG;
+    s->nq = q;
+    return PLH_OK;
+}
+int plh_sum_array_size(const plh_sum* s) { return s ? s->as : -1; }
+
+int plh_sum_device_planes(plh_sum* s, uint64_t** x, uint64_t** z, uint8_t** flags, double** val,
+                          int64_t* stride) {
+    PLH_TRY(check_sum(s));
+    if (x) *x = s->x;
+    if (z) *z = s->z;
+    if (flags) *flags = s->f;
+    if (val) *val = s->v;
+    if (stride) *stride = s->cap;
+    return PLH_OK;
+}
+
+int plh_sum_copy(const plh_sum* src, plh_sum* dst) {
+    PLH_TRY(check_pair(src, dst));
+    if (src == dst) return PLH_OK;
+    dst->n = 0;
+    PLH_TRY(op_append(dst->ctx, dst, src));
+    dst->nq = src->nq;
+    dst->canonical = src->canonical;
+    return PLH_OK;
+}
+
+int plh_launch_mul_soa(int array_size, const uint64_t* lhs_x, const uint64_t* lhs_z,
+                       const uint8_t* lhs_flags, const double* lhs_val, const uint64_t* rhs_x,
+                       const uint64_t* rhs_z, const uint8_t* rhs_flags, const double* rhs_val,
+                       uint64_t* out_x, uint64_t* out_z, uint8_t* out_flags, double* out_val, int n,
+                       int m, void* stream) {
+    if (!valid_as(array_size)) {
+        set_error("plh_launch_mul_soa: unsupported array_size=" + std::to_string(array_size) +
+                  " (supported: 1-16, up to 1024 qubits)");
+        return PLH_ERR_AS;
+    }
+    if (n < 0 || m < 0) return PLH_ERR_ARG;
+    const int64_t total = (int64_t)n * m;
+    return op_mul_raw(array_size, lhs_x, lhs_z, lhs_flags, lhs_val, n, rhs_x, rhs_z, rhs_flags, rhs_val, m,
+                      out_x, out_z, out_flags, out_val, total, n, m, (hipStream_t)stream);
+}
+
+int plh_multiply(plh_sum* lhs, const plh_sum* rhs) {
+    PLH_TRY(check_pair(lhs, rhs));
+    const int64_t n = lhs->n, m = rhs->n;
+    if (n > 0x7FFFFFFF || m > 0x7FFFFFFF) {
+        set_error("plh_multiply: input exceeds 2^31-1 terms");
+        return PLH_ERR_SIZE;
+    }
+    plh_sum out;
+    out.ctx = lhs->ctx;
+    out.as = lhs->as;
+    out.nq = lhs->nq;
+    PLH_TRY(sum_reserve(&out, n * m, 0));
+    int rc = op_mul_raw(lhs->as, lhs->x, lhs->z, lhs->f, lhs->v, lhs->cap, rhs->x, rhs->z, rhs->f, rhs->v,
+                        rhs->cap, out.x, out.z, out.f, out.v, out.cap, n, m, lhs->ctx->stream);
+    if (rc != PLH_OK) {
+        sum_free_storage(&out);
+        return rc;
+    }
+    PLH_HIP(hipStreamSynchronize(lhs->ctx->stream));
+    sum_free_storage(lhs);
+    lhs->x = out.x;
+    lhs->z = out.z;
+    lhs->f = out.f;
+    lhs->v = out.v;
+    lhs->cap = out.cap;
+    lhs->n = n * m;
+    lhs->canonical = false;
+    return PLH_OK;
+}
+
+int plh_multiply_canonical(const plh_sum* lhs, const plh_sum* rhs, plh_sum* out) {
+    PLH_TRY(check_pair(lhs, rhs));
+    PLH_TRY(check_sum(out));
+    out->nq = lhs->nq;
+    PLH_TRY(collate_pairs(lhs->ctx, lhs, rhs, 0, 0, canon_spec(), out));
+    out->nq = lhs->nq;
+    return plh_ctx_synchronize(lhs->ctx);
+}
+
+int plh_commutator(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out) {
+    PLH_TRY(check_pair(a, b));
+    PLH_TRY(check_sum(out));
+    if (out == a || out == b) return PLH_ERR_ARG;
+    if (a->n > 0x7FFFFFFF || b->n > 0x7FFFFFFF) {
+        set_error("plh_commutator: input exceeds 2^31-1 terms");
+        return PLH_ERR_SIZE;
+    }
+    PLH_TRY(op_commutator_raw(a->ctx, a, b, multiply_by_i ? 1 : 0, out));
+    return plh_ctx_synchronize(a->ctx);
+}
+
+int plh_commutator_canonical(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out) {
+    PLH_TRY(check_pair(a, b));
+    PLH_TRY(check_sum(out));
+    PLH_TRY(collate_pairs(a->ctx, a, b, 1, multiply_by_i ? 1 : 0, canon_spec(), out));
+    out->nq = a->nq > b->nq ? a->nq : b->nq;
+    return plh_ctx_synchronize(a->ctx);
+}
+
+int plh_sort_and_combine(plh_sum* s) {
+    PLH_TRY(check_sum(s));
+    CollateSpec sc;
+    PLH_TRY(collate_terms(s->ctx, s, sc, s));
+    s->canonical = false;
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_canonicalize(plh_sum* s) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(collate_terms(s->ctx, s, canon_spec(), s));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_add(plh_sum* lhs, const plh_sum* rhs) {
+    PLH_TRY(check_pair(lhs, rhs));
+    if (lhs == rhs) {
+        plh_sum tmp;
+        tmp.ctx = lhs->ctx;
+        tmp.as = lhs->as;
+        tmp.nq = lhs->nq;
+        int rc = op_append(lhs->ctx, &tmp, rhs);
+        if (rc == PLH_OK) rc = op_append(lhs->ctx, lhs, &tmp);
+        (void)hipStreamSynchronize(lhs->ctx->stream);
+        sum_free_storage(&tmp);
+        PLH_TRY(rc);
+    } else {
+        PLH_TRY(op_append(lhs->ctx, lhs, rhs));
+    }
+    return plh_sort_and_combine(lhs);
+}
+
+int plh_zero_state_expectation(const plh_sum* s, double* out) {
+    PLH_TRY(check_sum(s));
+    if (!out) return PLH_ERR_ARG;
+    return op_zero_state_expectation(s->ctx, s, out);
+}
+
+int plh_truncate_weight(plh_sum* s, int64_t max_weight) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(op_filter(s->ctx, s, kMaxWeight, max_weight, 0.0));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_cutoff(plh_sum* s, double eps) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(op_filter(s->ctx, s, kCutoff, 0, eps));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_apply_H(plh_sum* s, int64_t q) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(op_clifford(s->ctx, s, 0, q, 0));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_apply_S(plh_sum* s, int64_t q) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(op_clifford(s->ctx, s, 1, q, 0));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_apply_CNOT(plh_sum* s, int64_t control, int64_t target) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(op_clifford(s->ctx, s, 2, control, target));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_evolve(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz, const uint8_t* pflags,
+               const double* theta) {
+    PLH_TRY(check_sum(s));
+    if (nrot < 0 || (nrot > 0 && (!px || !pz || !theta))) return PLH_ERR_ARG;
+    for (int64_t r = 0; r < nrot; ++r) {
+        PLH_TRY(rotation_step(s, px + r * s->as, pz + r * s->as, pflags ? pflags[r] : 0, theta[r], false, -1,
+                              0.0));
+    }
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_apply_truncation(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms) {
+    PLH_TRY(check_sum(s));
+    if (max_terms >= 0) {
+        set_error("plh_apply_truncation: max_terms (trim) is not supported by this engine yet");
+        return PLH_ERR_ARG;
+    }
+    PLH_TRY(collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_propagate(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz, const uint8_t* pflags,
+                  const double* theta, int64_t max_weight, double cutoff, int64_t max_terms) {
+    PLH_TRY(check_sum(s));
+    if (max_terms >= 0) {
+        set_error("plh_propagate: max_terms (trim) is not supported by this engine yet");
+        return PLH_ERR_ARG;
+    }
+    if (nrot < 0 || (nrot > 0 && (!px || !pz || !theta))) return PLH_ERR_ARG;
+    for (int64_t l = nrot; l-- > 0;) {
+        PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, pflags ? pflags[l] : 0, theta[l], true,
+                              max_weight, cutoff));
+    }
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t* q0, const int64_t* q1,
+                    const uint64_t* px, const uint64_t* pz, const double* theta, int64_t max_weight,
+                    double cutoff, int64_t max_terms) {
+    PLH_TRY(check_sum(s));
+    if (max_terms >= 0) {
+        set_error("plh_run_program: max_terms (trim) is not supported by this engine yet");
+        return PLH_ERR_ARG;
+    }
+    if (nops < 0 || (nops > 0 && (!kind || !q0 || !q1 || !px || !pz || !theta))) return PLH_ERR_ARG;
+    for (int64_t l = nops; l-- > 0;) {
+        switch (kind[l]) {
+            case 0: PLH_TRY(op_clifford(s->ctx, s, 0, q0[l], 0)); break;
+            case 1: PLH_TRY(op_clifford(s->ctx, s, 1, q0[l], 0)); break;
+            case 2: PLH_TRY(op_clifford(s->ctx, s, 2, q0[l], q1[l])); break;
+            case 3:
+                PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, 0, theta[l], true, max_weight, cutoff));
+                break;
+            default: set_error("unknown op kind"); return PLH_ERR_ARG;
+        }
+    }
+    return plh_ctx_synchronize(s->ctx);
+}
+
+}  // extern "C"

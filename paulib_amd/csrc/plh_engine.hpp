@@ -1,0 +1,148 @@
+// plh_engine.hpp -- host-side engine state shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/paulib_hip.h"
+
+namespace plh {
+
+// Thread-local last-error text behind plh_last_error().
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define PLH_HIP(expr)                                                              \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            ::plh::set_error(std::string("HIP: ") + hipGetErrorString(_e) + " at " + \
+                             __FILE__ + ":" + std::to_string(__LINE__));           \
+            return _e == hipErrorOutOfMemory ? PLH_ERR_OOM : PLH_ERR_HIP;          \
+        }                                                                          \
+    } while (0)
+
+#define PLH_TRY(expr)            \
+    do {                         \
+        int _rc = (expr);        \
+        if (_rc != PLH_OK) return _rc; \
+    } while (0)
+
+// Grow-only device scratch owned by a context.  Slots are independent;
+// get() keeps the old contents only when no reallocation happens.
+class Workspace {
+   public:
+    ~Workspace() { release(); }
+    int get(int slot, size_t bytes, void** out);
+    void release();
+    size_t bytes_held() const;
+
+   private:
+    struct Buf {
+        void* p = nullptr;
+        size_t n = 0;
+    };
+    std::vector<Buf> bufs_;
+};
+
+// Small pinned host staging buffer for count read-backs.
+struct HostStage {
+    uint64_t* p = nullptr;
+    int init();
+    ~HostStage();
+};
+
+}  // namespace plh
+
+// Device-resident PauliSum<N,double>: plane-major SoA, x[k*cap + t].
+struct plh_sum {
+    plh_ctx* ctx = nullptr;
+    int as = 1;
+    int64_t nq = 64;
+    int64_t n = 0;
+    int64_t cap = 0;
+    uint64_t* x = nullptr;
+    uint64_t* z = nullptr;
+    uint8_t* f = nullptr;
+    double* v = nullptr;
+    bool canonical = false;  // (x, z, imag) keys distinct, flags in {0,1}, no zeros
+};
+
+struct plh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    plh::Workspace ws;
+    plh::HostStage stage;
+    // Sizes picked by the autotuner (0 = built-in default).
+    uint32_t tune_tile_rounds = 0;
+};
+
+namespace plh {
+
+// Workspace slots.
+enum : int {
+    kWsIds0 = 0,
+    kWsIds1,
+    kWsHist,
+    kWsListA,
+    kWsListB,
+    kWsBig,
+    kWsSub,
+    kWsScan0,
+    kWsScan1,
+    kWsScan2,
+    kWsKept,
+    kWsOffs,
+    kWsMisc,
+    kWsFlags,
+    kWsTmpSum,
+    kWsCount
+};
+
+// (Re)allocate a sum to hold at least cap terms; keeps the first keep
+// terms when growing.
+int sum_reserve(plh_sum* s, int64_t cap, int64_t keep);
+void sum_free_storage(plh_sum* s);
+
+// Device-wide exclusive scan of n u32 values into u64 offsets; the total
+// is written to *d_total (device).  Stream-ordered.
+int scan_u32_to_u64(plh_ctx* ctx, const uint32_t* d_in, uint64_t* d_out, uint64_t n,
+                    uint64_t* d_total);
+// Same for u64 inputs.
+int scan_u64(plh_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total);
+
+// Read one u64 from device memory (synchronises the stream).
+int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out);
+
+// Collation (plh_collate.hip).
+struct CollateSpec {
+    uint32_t mode = 0;      // kFold | kDropZero | kMaxWeight | kCutoff
+    int64_t max_weight = 0;
+    double eps = 0.0;
+};
+// out = collate(in); out may alias in (the result replaces it).
+int collate_terms(plh_ctx* ctx, plh_sum* in, const CollateSpec& spec, plh_sum* out);
+// out = collate(L x R) for kind 0 (product, out[i*n+j] = L[j]*R[i]) or
+// kind 1 (commutator of L with R, by_i adds the i factor).
+int collate_pairs(plh_ctx* ctx, const plh_sum* L, const plh_sum* R, int kind, int by_i,
+                  const CollateSpec& spec, plh_sum* out);
+
+// Ops (plh_ops.hip).
+int op_mul_raw(int as, const uint64_t* lx, const uint64_t* lz, const uint8_t* lf,
+               const double* lv, int64_t lstride, const uint64_t* rx, const uint64_t* rz,
+               const uint8_t* rf, const double* rv, int64_t rstride, uint64_t* ox, uint64_t* oz,
+               uint8_t* of, double* ov, int64_t ostride, int64_t n, int64_t m, hipStream_t stream);
+int op_commutator_raw(plh_ctx* ctx, const plh_sum* a, const plh_sum* b, int by_i, plh_sum* out);
+int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1);
+int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pflags,
+                double cos_t, double sin_t, int* appended_general);
+int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, double eps);
+int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out);
+int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or /* 2*as words */);
+int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src);
+
+}  // namespace plh

@@ -1,0 +1,651 @@
+// plh_ops.hip -- element-wise and pair kernels of the PauliSum engine.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "plh_block.hpp"
+#include "plh_engine.hpp"
+#include "plh_sources.hpp"
+
+namespace plh {
+
+namespace {
+
+constexpr int kOpsIpt = 16;  // items per thread in the ordered-compaction kernels
+constexpr uint64_t kOpsTile = (uint64_t)kBlock * kOpsIpt;
+
+int padded(int as) {
+    if (as <= 1) return 1;
+    if (as == 2) return 2;
+    if (as <= 4) return 4;
+    if (as <= 8) return 8;
+    if (as <= 16) return 16;
+    return -1;
+}
+
+// ------------------------------------------------------------ raw product
+// out[i*n + j] = lhs[j] * rhs[i] (include/PauliSumOperations.h:255-306).
+// One row i per block-row; the rhs term is wave-uniform, lhs reads and all
+// writes are coalesced; planes use non-temporal stores (streamed output).
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_mul_raw(
+    int as, const uint64_t* __restrict__ lx, const uint64_t* __restrict__ lz,
+    const uint8_t* __restrict__ lf, const double* __restrict__ lv, int64_t ls,
+    const uint64_t* __restrict__ rx, const uint64_t* __restrict__ rz,
+    const uint8_t* __restrict__ rf, const double* __restrict__ rv, int64_t rs,
+    uint64_t* __restrict__ ox, uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
+    double* __restrict__ ov, int64_t os, int64_t n, uint32_t jblocks) {
+    const uint64_t i = blockIdx.x / jblocks;
+    const uint64_t jb = blockIdx.x - i * jblocks;
+    uint64_t bx[AS], bz[AS];
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        bx[k] = k < as ? rx[(int64_t)k * rs + i] : 0ull;
+        bz[k] = k < as ? rz[(int64_t)k * rs + i] : 0ull;
+    }
+    const uint32_t bf = rf[i];
+    const double bv = rv[i];
+    const int64_t row = (int64_t)i * n;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int64_t j = (int64_t)jb * (4 * kBlock) + r * kBlock + threadIdx.x;
+        if (j >= n) break;
+        uint64_t c1 = 0, c2 = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            if (k < as) {
+                const uint64_t ax = lx[(int64_t)k * ls + j], az = lz[(int64_t)k * ls + j];
+                __builtin_nontemporal_store(ax ^ bx[k], &ox[(int64_t)k * os + row + j]);
+                __builtin_nontemporal_store(az ^ bz[k], &oz[(int64_t)k * os + row + j]);
+                phase_word(ax, az, bx[k], bz[k], c1, c2);
+            }
+        }
+        of[row + j] = (uint8_t)phase_of(lf[j], bf, c1, c2);
+        __builtin_nontemporal_store(lv[j] * bv, &ov[row + j]);
+    }
+}
+
+// ------------------------------------------------------- commutator (raw)
+// Anticommuting pairs of a (i) and b (j) in (i, j) order, compacted with a
+// counting pass and an ordered write pass (no atomics, deterministic).
+template <int AS>
+PLH_DEV bool anticommutes(int as, const uint64_t* ax, const uint64_t* az, int64_t as_,
+                          uint64_t ia, const uint64_t* bx, const uint64_t* bz, int64_t bs,
+                          uint64_t jb) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k)
+        if (k < as)
+            acc ^= (ax[(int64_t)k * as_ + ia] & bz[(int64_t)k * bs + jb]) ^
+                   (az[(int64_t)k * as_ + ia] & bx[(int64_t)k * bs + jb]);
+    return popc(acc) & 1;
+}
+
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_comm_count(int as, SumView A, SumView B, uint64_t m,
+                                                       uint64_t total, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t c = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= total) break;
+        const uint64_t i = t / m, j = t - i * m;
+        c += anticommutes<AS>(as, A.x, A.z, A.stride, i, B.x, B.z, B.stride, j) ? 1u : 0u;
+    }
+    const uint32_t s = block_sum(c, red);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = s;
+}
+
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_comm_write(int as, SumView A, SumView B, uint64_t m,
+                                                       uint64_t total, uint32_t by_i,
+                                                       const uint64_t* __restrict__ boff,
+                                                       uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
+                                                       uint8_t* __restrict__ of, double* __restrict__ ov,
+                                                       int64_t os) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t mask = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= total) break;
+        const uint64_t i = t / m, j = t - i * m;
+        if (anticommutes<AS>(as, A.x, A.z, A.stride, i, B.x, B.z, B.stride, j)) mask |= 1u << k;
+    }
+    uint32_t tot;
+    uint64_t o = boff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(mask), tot, red);
+    for (int k = 0; k < kOpsIpt; ++k) {
+        if (!(mask >> k & 1u)) continue;
+        const uint64_t t = base + k;
+        const uint64_t i = t / m, j = t - i * m;
+        uint64_t c1 = 0, c2 = 0;
+        for (int w = 0; w < as; ++w) {
+            const uint64_t ax = A.x[(int64_t)w * A.stride + i], az = A.z[(int64_t)w * A.stride + i];
+            const uint64_t bx = B.x[(int64_t)w * B.stride + j], bz = B.z[(int64_t)w * B.stride + j];
+            ox[(int64_t)w * os + o] = ax ^ bx;
+            oz[(int64_t)w * os + o] = az ^ bz;
+            phase_word(ax, az, bx, bz, c1, c2);
+        }
+        of[o] = (uint8_t)((phase_of(A.f[i], B.f[j], c1, c2) + by_i) & 3u);
+        ov[o] = (2.0 * A.v[i]) * B.v[j];
+        ++o;
+    }
+}
+
+// ----------------------------------------------------------- Clifford
+// H, S, CNOT exactly as include/PauliSumClifford.h:43-81, 212-239.
+__global__ void k_clifford(int gate, uint64_t* __restrict__ x, uint64_t* __restrict__ z,
+                           uint8_t* __restrict__ f, int64_t stride, int64_t n, int wc, uint64_t mc,
+                           int wt, uint64_t mt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (gate == 0) {  // H
+        uint64_t xw = x[(int64_t)wc * stride + i], zw = z[(int64_t)wc * stride + i];
+        const uint32_t xc = (xw & mc) != 0, zc = (zw & mc) != 0;
+        f[i] ^= (uint8_t)((xc & zc) << 1);
+        xw ^= zw & mc;
+        zw ^= xw & mc;
+        xw ^= zw & mc;
+        x[(int64_t)wc * stride + i] = xw;
+        z[(int64_t)wc * stride + i] = zw;
+    } else if (gate == 1) {  // S
+        const uint64_t xw = x[(int64_t)wc * stride + i], zw = z[(int64_t)wc * stride + i];
+        const uint32_t xc = (xw & mc) != 0, zc = (zw & mc) != 0;
+        f[i] ^= (uint8_t)((xc & zc) << 1);
+        z[(int64_t)wc * stride + i] = zw ^ (xw & mc);
+    } else {  // CNOT(control c, target t)
+        const uint64_t xc = (x[(int64_t)wc * stride + i] & mc) != 0;
+        const uint64_t zc = (z[(int64_t)wc * stride + i] & mc) != 0;
+        const uint64_t xt = (x[(int64_t)wt * stride + i] & mt) != 0;
+        const uint64_t zt = (z[(int64_t)wt * stride + i] & mt) != 0;
+        f[i] ^= (uint8_t)((xc & zt & (1ull ^ xt ^ zc)) << 1);
+        x[(int64_t)wt * stride + i] ^= (0ull - xc) & mt;
+        z[(int64_t)wc * stride + i] ^= (0ull - zt) & mc;
+    }
+}
+
+// -------------------------------------------------------------- rotation
+struct Rot {
+    uint64_t px[16];
+    uint64_t pz[16];
+    uint32_t psf;  // flags of ps = P with the i bit flipped
+    double c, s;
+};
+
+template <int AS>
+PLH_DEV bool term_anticommutes(int as, const uint64_t* x, const uint64_t* z, int64_t stride,
+                               uint64_t t, const Rot& r) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k)
+        if (k < as) acc ^= (x[(int64_t)k * stride + t] & r.pz[k]) ^ (z[(int64_t)k * stride + t] & r.px[k]);
+    return popc(acc) & 1;
+}
+
+// In-place term * ps (PauliSum::multiply(i, ps), :185-209).
+template <int AS>
+PLH_DEV void term_times_ps(int as, uint64_t* x, uint64_t* z, uint8_t* f, int64_t stride, uint64_t t,
+                           const Rot& r) {
+    uint64_t c1 = 0, c2 = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        if (k < as) {
+            const uint64_t ax = x[(int64_t)k * stride + t], az = z[(int64_t)k * stride + t];
+            x[(int64_t)k * stride + t] = ax ^ r.px[k];
+            z[(int64_t)k * stride + t] = az ^ r.pz[k];
+            phase_word(ax, az, r.px[k], r.pz[k], c1, c2);
+        }
+    }
+    f[t] = (uint8_t)phase_of(f[t], r.psf, c1, c2);
+}
+
+// Fast paths (:121-147): kind 1 flips the sign, 2 flips and multiplies,
+// 3 multiplies; only terms that anticommute with P.
+template <int AS>
+__global__ void k_rot_inplace(int as, int kind, uint64_t* __restrict__ x, uint64_t* __restrict__ z,
+                              uint8_t* __restrict__ f, int64_t stride, int64_t n, Rot r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (!term_anticommutes<AS>(as, x, z, stride, t, r)) return;
+    if (kind == 1 || kind == 2) f[t] ^= 2u;
+    if (kind >= 2) term_times_ps<AS>(as, x, z, f, stride, t, r);
+}
+
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rot_count(int as, const uint64_t* __restrict__ x,
+                                                      const uint64_t* __restrict__ z, int64_t stride,
+                                                      int64_t n, Rot r, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t c = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        c += term_anticommutes<AS>(as, x, z, stride, t, r) ? 1u : 0u;
+    }
+    const uint32_t s = block_sum(c, red);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = s;
+}
+
+// General path (:148-180): anticommuting term j is scaled by cos in place
+// and term_j * ps with value (-sin) * v_j is appended at n + rank(j).
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rot_general(int as, uint64_t* __restrict__ x,
+                                                        uint64_t* __restrict__ z, uint8_t* __restrict__ f,
+                                                        double* __restrict__ v, int64_t stride, int64_t n,
+                                                        Rot r, const uint64_t* __restrict__ boff) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t mask = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        if (term_anticommutes<AS>(as, x, z, stride, t, r)) mask |= 1u << k;
+    }
+    uint32_t tot;
+    uint64_t o = (uint64_t)n + boff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(mask), tot, red);
+    const double neg_s = -r.s;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        if (!(mask >> k & 1u)) continue;
+        const uint64_t t = base + k;
+        const double orig = v[t];
+        v[t] = orig * r.c;
+        uint64_t c1 = 0, c2 = 0;
+#pragma unroll
+        for (int w = 0; w < AS; ++w) {
+            if (w < as) {
+                const uint64_t ax = x[(int64_t)w * stride + t], az = z[(int64_t)w * stride + t];
+                x[(int64_t)w * stride + o] = ax ^ r.px[w];
+                z[(int64_t)w * stride + o] = az ^ r.pz[w];
+                phase_word(ax, az, r.px[w], r.pz[w], c1, c2);
+            }
+        }
+        f[o] = (uint8_t)phase_of(f[t], r.psf, c1, c2);
+        v[o] = neg_s * orig;
+        ++o;
+    }
+}
+
+// ---------------------------------------------------------------- filters
+template <int AS>
+PLH_DEV bool filter_keep(int as, const uint64_t* x, const uint64_t* z, const double* v, int64_t stride,
+                         uint64_t t, uint32_t mode, int64_t max_weight, double eps) {
+    if (mode & kMaxWeight) {
+        int64_t w = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k)
+            if (k < as) w += popc(x[(int64_t)k * stride + t] | z[(int64_t)k * stride + t]);
+        if (w > max_weight) return false;
+    }
+    if ((mode & kCutoff) && !(fabs(v[t]) >= eps)) return false;
+    return true;
+}
+
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_filter_count(int as, SumView s, int64_t n, uint32_t mode,
+                                                         int64_t max_weight, double eps,
+                                                         uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t c = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        c += filter_keep<AS>(as, s.x, s.z, s.v, s.stride, t, mode, max_weight, eps) ? 1u : 0u;
+    }
+    const uint32_t tot = block_sum(c, red);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_filter_write(int as, SumView s, int64_t n, uint32_t mode,
+                                                         int64_t max_weight, double eps,
+                                                         const uint64_t* __restrict__ boff,
+                                                         uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
+                                                         uint8_t* __restrict__ of, double* __restrict__ ov,
+                                                         int64_t os) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t mask = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        if (filter_keep<AS>(as, s.x, s.z, s.v, s.stride, t, mode, max_weight, eps)) mask |= 1u << k;
+    }
+    uint32_t tot;
+    uint64_t o = boff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(mask), tot, red);
+    for (int k = 0; k < kOpsIpt; ++k) {
+        if (!(mask >> k & 1u)) continue;
+        const uint64_t t = base + k;
+        for (int w = 0; w < as; ++w) {
+            ox[(int64_t)w * os + o] = s.x[(int64_t)w * s.stride + t];
+            oz[(int64_t)w * os + o] = s.z[(int64_t)w * s.stride + t];
+        }
+        of[o] = s.f[t];
+        ov[o] = s.v[t];
+        ++o;
+    }
+}
+
+// ------------------------------------------------------- expectation / OR
+// Deterministic two-level reduction: each block sums its contiguous range
+// with a fixed tree; block partials are summed in block order.
+__global__ __launch_bounds__(kBlock) void k_zse_partial(int as, SumView s, int64_t n,
+                                                        double* __restrict__ part) {
+    __shared__ double red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    double acc = 0.0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        bool has_xy = false;
+        for (int w = 0; w < as; ++w) has_xy |= s.x[(int64_t)w * s.stride + t] != 0;
+        const uint32_t fl = s.f[t];
+        if (has_xy || (fl & 1u)) continue;
+        acc += (fl & 2u) ? -s.v[t] : s.v[t];
+    }
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zse_final(const double* __restrict__ part, uint64_t nb,
+                                                      double* __restrict__ out) {
+    __shared__ double red[kWavesPerBlock];
+    double acc = 0.0;
+    // Each thread owns a contiguous slice so the order is fixed.
+    const uint64_t per = (nb + kBlock - 1) / kBlock;
+    for (uint64_t i = threadIdx.x * per; i < min(nb, (threadIdx.x + 1) * per); ++i) acc += part[i];
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) *out = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_or_words(const uint64_t* __restrict__ x,
+                                                     const uint64_t* __restrict__ z, int64_t stride,
+                                                     int as, int64_t n, unsigned long long* __restrict__ out) {
+    const int w = blockIdx.y;  // 0..2*as-1
+    const uint64_t* p = (w < as ? x : z) + (int64_t)(w < as ? w : w - as) * stride;
+    uint64_t acc = 0;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
+        acc |= p[t];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, kWave);
+    if (lane_id() == 0 && acc) atomicOr(&out[w], (unsigned long long)acc);
+}
+
+inline unsigned nblk(uint64_t n, uint64_t per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+// ====================================================================== API
+int op_mul_raw(int as, const uint64_t* lx, const uint64_t* lz, const uint8_t* lf, const double* lv,
+               int64_t lstride, const uint64_t* rx, const uint64_t* rz, const uint8_t* rf,
+               const double* rv, int64_t rstride, uint64_t* ox, uint64_t* oz, uint8_t* of, double* ov,
+               int64_t ostride, int64_t n, int64_t m, hipStream_t st) {
+    if (n == 0 || m == 0) return PLH_OK;
+    const uint32_t jblocks = (uint32_t)((n + 4 * kBlock - 1) / (4 * kBlock));
+    const uint64_t grid = (uint64_t)jblocks * (uint64_t)m;
+    if (grid > 0x7FFFFFFFull) {
+        set_error("raw product grid too large");
+        return PLH_ERR_SIZE;
+    }
+#define PLH_MUL(ASV)                                                                           \
+    case ASV:                                                                                  \
+        k_mul_raw<ASV><<<dim3((unsigned)grid), kBlock, 0, st>>>(as, lx, lz, lf, lv, lstride, rx, rz, \
+                                                               rf, rv, rstride, ox, oz, of, ov,     \
+                                                               ostride, n, jblocks);               \
+        break;
+    switch (padded(as)) {
+        PLH_MUL(1) PLH_MUL(2) PLH_MUL(4) PLH_MUL(8) PLH_MUL(16)
+        default: set_error("unsupported array_size"); return PLH_ERR_AS;
+    }
+#undef PLH_MUL
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
+int op_commutator_raw(plh_ctx* ctx, const plh_sum* a, const plh_sum* b, int by_i, plh_sum* out) {
+    hipStream_t st = ctx->stream;
+    const uint64_t n = (uint64_t)a->n, m = (uint64_t)b->n;
+    const uint64_t total = n * m;
+    out->as = a->as;
+    out->nq = a->nq > b->nq ? a->nq : b->nq;
+    out->n = 0;
+    if (total == 0) return PLH_OK;
+    const uint64_t nb = (total + kOpsTile - 1) / kOpsTile;
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsScan0, nb * 4 + 64, &p));
+    uint32_t* cnt = (uint32_t*)p;
+    PLH_TRY(ctx->ws.get(kWsScan1, nb * 8 + 64, &p));
+    uint64_t* off = (uint64_t*)p;
+    PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+    uint64_t* d_total = (uint64_t*)p + 40;
+    SumView A{a->x, a->z, a->f, a->v, a->cap}, B{b->x, b->z, b->f, b->v, b->cap};
+#define PLH_CC(ASV) \
+    case ASV: k_comm_count<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(a->as, A, B, m, total, cnt); break;
+    switch (padded(a->as)) {
+        PLH_CC(1) PLH_CC(2) PLH_CC(4) PLH_CC(8) PLH_CC(16)
+        default: set_error("unsupported array_size"); return PLH_ERR_AS;
+    }
+#undef PLH_CC
+    PLH_TRY(scan_u32_to_u64(ctx, cnt, off, nb, d_total));
+    uint64_t k = 0;
+    PLH_TRY(read_u64(ctx, d_total, &k));
+    PLH_TRY(sum_reserve(out, (int64_t)k, 0));
+    out->n = (int64_t)k;
+    if (k == 0) return PLH_OK;
+#define PLH_CW(ASV)                                                                                   \
+    case ASV:                                                                                         \
+        k_comm_write<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(a->as, A, B, m, total, (uint32_t)by_i, \
+                                                                off, out->x, out->z, out->f, out->v,   \
+                                                                out->cap);                            \
+        break;
+    switch (padded(a->as)) {
+        PLH_CW(1) PLH_CW(2) PLH_CW(4) PLH_CW(8) PLH_CW(16)
+        default: return PLH_ERR_AS;
+    }
+#undef PLH_CW
+    PLH_HIP(hipGetLastError());
+    out->canonical = false;
+    return PLH_OK;
+}
+
+int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1) {
+    if (q0 < 0 || q0 >= 64 * (int64_t)s->as || (gate == 2 && (q1 < 0 || q1 >= 64 * (int64_t)s->as))) {
+        set_error("qubit index outside the sum's words");
+        return PLH_ERR_ARG;
+    }
+    if (s->n == 0) return PLH_OK;
+    const int wc = (int)(q0 / 64), wt = gate == 2 ? (int)(q1 / 64) : 0;
+    const uint64_t mc = 1ull << (q0 % 64), mt = gate == 2 ? 1ull << (q1 % 64) : 0ull;
+    k_clifford<<<nblk((uint64_t)s->n, kBlock), kBlock, 0, ctx->stream>>>(gate, s->x, s->z, s->f, s->cap,
+                                                                          s->n, wc, mc, wt, mt);
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
+static bool close_one(double v) { return std::fabs(v - 1.0) <= 1e-14; }  // isCloseOne, :19-21
+
+int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pflags,
+                double cos_t, double sin_t, int* appended_general) {
+    hipStream_t st = ctx->stream;
+    *appended_general = 0;
+    Rot r{};
+    for (int k = 0; k < s->as; ++k) {
+        r.px[k] = px[k];
+        r.pz[k] = pz[k];
+    }
+    r.psf = (uint32_t)(pflags ^ 1u);
+    r.c = cos_t;
+    r.s = sin_t;
+    int kind;
+    if (close_one(cos_t)) return PLH_OK;
+    else if (close_one(-cos_t)) kind = 1;
+    else if (close_one(sin_t)) kind = 2;
+    else if (close_one(-sin_t)) kind = 3;
+    else kind = 4;
+    if (s->n == 0) return PLH_OK;
+    const int pa = padded(s->as);
+    if (pa < 0) return PLH_ERR_AS;
+    if (kind != 4) {
+#define PLH_RI(ASV)                                                                                 \
+    case ASV:                                                                                       \
+        k_rot_inplace<ASV><<<nblk((uint64_t)s->n, kBlock), kBlock, 0, st>>>(s->as, kind, s->x, s->z, \
+                                                                            s->f, s->cap, s->n, r); \
+        break;
+        switch (pa) { PLH_RI(1) PLH_RI(2) PLH_RI(4) PLH_RI(8) PLH_RI(16) }
+#undef PLH_RI
+        PLH_HIP(hipGetLastError());
+        s->canonical = s->canonical && false;
+        return PLH_OK;
+    }
+    const uint64_t nb = nblk((uint64_t)s->n, kOpsTile);
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsScan0, nb * 4 + 64, &p));
+    uint32_t* cnt = (uint32_t*)p;
+    PLH_TRY(ctx->ws.get(kWsScan1, nb * 8 + 64, &p));
+    uint64_t* off = (uint64_t*)p;
+    PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+    uint64_t* d_total = (uint64_t*)p + 40;
+#define PLH_RC(ASV)                                                                                      \
+    case ASV:                                                                                            \
+        k_rot_count<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, s->x, s->z, s->cap, s->n, r, cnt); \
+        break;
+    switch (pa) { PLH_RC(1) PLH_RC(2) PLH_RC(4) PLH_RC(8) PLH_RC(16) }
+#undef PLH_RC
+    PLH_TRY(scan_u32_to_u64(ctx, cnt, off, nb, d_total));
+    uint64_t k = 0;
+    PLH_TRY(read_u64(ctx, d_total, &k));
+    if (k == 0) return PLH_OK;
+    PLH_TRY(sum_reserve(s, s->n + (int64_t)k, s->n));
+#define PLH_RG(ASV)                                                                                    \
+    case ASV:                                                                                          \
+        k_rot_general<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, s->x, s->z, s->f, s->v, s->cap, \
+                                                                 s->n, r, off);                        \
+        break;
+    switch (pa) { PLH_RG(1) PLH_RG(2) PLH_RG(4) PLH_RG(8) PLH_RG(16) }
+#undef PLH_RG
+    PLH_HIP(hipGetLastError());
+    s->n += (int64_t)k;
+    s->canonical = false;
+    *appended_general = 1;
+    return PLH_OK;
+}
+
+int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, double eps) {
+    hipStream_t st = ctx->stream;
+    if (s->n == 0 || !(mode & (kMaxWeight | kCutoff))) return PLH_OK;
+    const uint64_t nb = nblk((uint64_t)s->n, kOpsTile);
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsScan0, nb * 4 + 64, &p));
+    uint32_t* cnt = (uint32_t*)p;
+    PLH_TRY(ctx->ws.get(kWsScan1, nb * 8 + 64, &p));
+    uint64_t* off = (uint64_t*)p;
+    PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+    uint64_t* d_total = (uint64_t*)p + 40;
+    SumView v{s->x, s->z, s->f, s->v, s->cap};
+    const int pa = padded(s->as);
+#define PLH_FC(ASV)                                                                                     \
+    case ASV:                                                                                           \
+        k_filter_count<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, v, s->n, mode, max_weight, eps, \
+                                                                  cnt);                                 \
+        break;
+    switch (pa) {
+        PLH_FC(1) PLH_FC(2) PLH_FC(4) PLH_FC(8) PLH_FC(16)
+        default: return PLH_ERR_AS;
+    }
+#undef PLH_FC
+    PLH_TRY(scan_u32_to_u64(ctx, cnt, off, nb, d_total));
+    uint64_t k = 0;
+    PLH_TRY(read_u64(ctx, d_total, &k));
+    if (k == (uint64_t)s->n) return PLH_OK;
+    plh_sum tmp;
+    tmp.ctx = ctx;
+    tmp.as = s->as;
+    tmp.nq = s->nq;
+    PLH_TRY(sum_reserve(&tmp, (int64_t)k, 0));
+    if (k) {
+#define PLH_FW(ASV)                                                                                     \
+    case ASV:                                                                                           \
+        k_filter_write<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, v, s->n, mode, max_weight, eps, \
+                                                                  off, tmp.x, tmp.z, tmp.f, tmp.v,      \
+                                                                  tmp.cap);                             \
+        break;
+        switch (pa) { PLH_FW(1) PLH_FW(2) PLH_FW(4) PLH_FW(8) PLH_FW(16) }
+#undef PLH_FW
+        PLH_HIP(hipGetLastError());
+    }
+    PLH_HIP(hipStreamSynchronize(st));
+    const bool canon = s->canonical;
+    sum_free_storage(s);
+    s->x = tmp.x;
+    s->z = tmp.z;
+    s->f = tmp.f;
+    s->v = tmp.v;
+    s->cap = tmp.cap;
+    s->n = (int64_t)k;
+    s->canonical = canon;  // a filter of a canonical sum stays canonical
+    return PLH_OK;
+}
+
+int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out) {
+    hipStream_t st = ctx->stream;
+    if (s->n == 0) {
+        *out = 0.0;
+        return PLH_OK;
+    }
+    const uint64_t nb = nblk((uint64_t)s->n, kOpsTile);
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsScan1, nb * 8 + 64, &p));
+    double* part = (double*)p;
+    PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+    double* d_out = (double*)p + 41;
+    SumView v{s->x, s->z, s->f, s->v, s->cap};
+    k_zse_partial<<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, v, s->n, part);
+    k_zse_final<<<1, kBlock, 0, st>>>(part, nb, d_out);
+    PLH_HIP(hipGetLastError());
+    uint64_t bits = 0;
+    PLH_TRY(read_u64(ctx, (const uint64_t*)d_out, &bits));
+    std::memcpy(out, &bits, sizeof(double));
+    return PLH_OK;
+}
+
+int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or) {
+    hipStream_t st = ctx->stream;
+    const int nw = 2 * s->as;
+    for (int w = 0; w < nw; ++w) h_or[w] = 0;
+    if (s->n == 0) return PLH_OK;
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsTmpSum, 64 * 8, &p));
+    unsigned long long* d = (unsigned long long*)p;
+    PLH_HIP(hipMemsetAsync(d, 0, (size_t)nw * 8, st));
+    const unsigned gx = (unsigned)std::min<uint64_t>(nblk((uint64_t)s->n, kBlock), 1024);
+    k_or_words<<<dim3(gx, (unsigned)nw), kBlock, 0, st>>>(s->x, s->z, s->cap, s->as, s->n, d);
+    PLH_HIP(hipGetLastError());
+    if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
+    PLH_HIP(hipMemcpyAsync(ctx->stage.p, d, (size_t)nw * 8, hipMemcpyDeviceToHost, st));
+    PLH_HIP(hipStreamSynchronize(st));
+    for (int w = 0; w < nw; ++w) h_or[w] = ctx->stage.p[w];
+    return PLH_OK;
+}
+
+int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src) {
+    hipStream_t st = ctx->stream;
+    if (src->n == 0) return PLH_OK;
+    const int64_t n0 = dst->n;
+    PLH_TRY(sum_reserve(dst, n0 + src->n, n0));
+    for (int k = 0; k < dst->as; ++k) {
+        PLH_HIP(hipMemcpyAsync(dst->x + (int64_t)k * dst->cap + n0, src->x + (int64_t)k * src->cap,
+                               (size_t)src->n * 8, hipMemcpyDeviceToDevice, st));
+        PLH_HIP(hipMemcpyAsync(dst->z + (int64_t)k * dst->cap + n0, src->z + (int64_t)k * src->cap,
+                               (size_t)src->n * 8, hipMemcpyDeviceToDevice, st));
+    }
+    PLH_HIP(hipMemcpyAsync(dst->f + n0, src->f, (size_t)src->n, hipMemcpyDeviceToDevice, st));
+    PLH_HIP(hipMemcpyAsync(dst->v + n0, src->v, (size_t)src->n * 8, hipMemcpyDeviceToDevice, st));
+    dst->n = n0 + src->n;
+    dst->canonical = false;
+    return PLH_OK;
+}
+
+}  // namespace plh

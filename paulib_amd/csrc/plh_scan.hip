@@ -1,0 +1,160 @@
+// plh_scan.hip -- device-wide exclusive scans and small host helpers.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+
+#include "plh_block.hpp"
+#include "plh_engine.hpp"
+
+namespace plh {
+
+namespace {
+
+constexpr int kScanIpt = 8;
+constexpr uint64_t kScanTile = (uint64_t)kBlock * kScanIpt;  // elements per block
+
+template <class TIn>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const TIn* __restrict__ in, uint64_t n,
+                                                        uint64_t* __restrict__ bsum) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanIpt; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i < n) acc += (uint64_t)in[i];
+    }
+    const uint64_t tot = block_sum(acc, sh);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of nb block sums in place; total out.
+__global__ __launch_bounds__(kBlock) void k_scan_bsum(uint64_t* __restrict__ bsum, uint64_t nb,
+                                                      uint64_t* __restrict__ total) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint64_t c0 = 0; c0 < nb; c0 += kBlock) {
+        const uint64_t i = c0 + threadIdx.x;
+        const uint64_t v = i < nb ? bsum[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_exclusive_scan(v, tot, sh);
+        const uint64_t c = carry;
+        if (i < nb) bsum[i] = c + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+template <class TIn>
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const TIn* __restrict__ in,
+                                                       uint64_t* __restrict__ out, uint64_t n,
+                                                       const uint64_t* __restrict__ bsum) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    // Blocked: thread t owns elements [t*ipt, t*ipt+ipt) of this block.
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanIpt;
+    uint64_t vals[kScanIpt];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanIpt; ++k) {
+        const uint64_t i = base + k;
+        vals[k] = i < n ? (uint64_t)in[i] : 0;
+        acc += vals[k];
+    }
+    uint64_t tot;
+    uint64_t run = block_exclusive_scan(acc, tot, sh) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanIpt; ++k) {
+        const uint64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += vals[k];
+    }
+}
+
+template <class TIn>
+int scan_impl(plh_ctx* ctx, const TIn* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total) {
+    hipStream_t st = ctx->stream;
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    void* p = nullptr;
+    PLH_TRY(ctx->ws.get(kWsScan2, (size_t)(nb ? nb : 1) * sizeof(uint64_t), &p));
+    uint64_t* bsum = (uint64_t*)p;
+    if (nb) {
+        k_scan_reduce<TIn><<<dim3((unsigned)nb), dim3(kBlock), 0, st>>>(d_in, n, bsum);
+    }
+    k_scan_bsum<<<1, kBlock, 0, st>>>(bsum, nb, d_total);
+    if (nb) {
+        k_scan_apply<TIn><<<dim3((unsigned)nb), dim3(kBlock), 0, st>>>(d_in, d_out, n, bsum);
+    }
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
+}  // namespace
+
+int scan_u32_to_u64(plh_ctx* ctx, const uint32_t* d_in, uint64_t* d_out, uint64_t n,
+                    uint64_t* d_total) {
+    return scan_impl<uint32_t>(ctx, d_in, d_out, n, d_total);
+}
+
+int scan_u64(plh_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total) {
+    return scan_impl<uint64_t>(ctx, d_in, d_out, n, d_total);
+}
+
+int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out) {
+    if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
+    PLH_HIP(hipMemcpyAsync(ctx->stage.p, d_src, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->stage.p[0];
+    return PLH_OK;
+}
+
+// ------------------------------------------------------------ workspace
+
+int Workspace::get(int slot, size_t bytes, void** out) {
+    if (slot < 0) return PLH_ERR_ARG;
+    if ((size_t)slot >= bufs_.size()) bufs_.resize((size_t)slot + 1);
+    Buf& b = bufs_[(size_t)slot];
+    if (bytes == 0) bytes = 1;
+    if (b.n < bytes) {
+        if (b.p) {
+            PLH_HIP(hipDeviceSynchronize());
+            PLH_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.n = 0;
+        }
+        // Grow geometrically so repeated small growth does not reallocate.
+        size_t want = bytes + bytes / 4;
+        PLH_HIP(hipMalloc(&b.p, want));
+        b.n = want;
+    }
+    *out = b.p;
+    return PLH_OK;
+}
+
+void Workspace::release() {
+    for (auto& b : bufs_) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+}
+
+size_t Workspace::bytes_held() const {
+    size_t t = 0;
+    for (auto& b : bufs_) t += b.n;
+    return t;
+}
+
+int HostStage::init() {
+    PLH_HIP(hipHostMalloc((void**)&p, 64 * sizeof(uint64_t), hipHostMallocDefault));
+    return PLH_OK;
+}
+HostStage::~HostStage() {
+    if (p) (void)hipHostFree(p);
+}
+
+}  // namespace plh

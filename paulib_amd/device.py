@@ -1,0 +1,282 @@
+"""Device-resident PauliSum over the C ABI -- the Python mirror of the
+reference's PauliSum<N,double> hot-path API.
+
+Method and function names follow the reference (include/PauliSumCore.h:58-133,
+include/PauliSumOperations.h:348-356, include/PauliPropagation.h:26-71); every
+call runs the HIP engine (libpaulib_hip.so) on the context's stream.  Errors
+from the engine raise EngineError (the reference throws std::runtime_error).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .host import HostSum
+
+
+def _u64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _u8(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def _f64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _i32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Context:
+    """One device and one HIP stream (plh_ctx)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        h = C.c_void_p()
+        _lib.call("plh_ctx_create", device, C.c_void_p(stream) if stream else None, C.byref(h))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return _lib.lib().plh_ctx_stream(self._h) or 0
+
+    def synchronize(self):
+        _lib.call("plh_ctx_synchronize", self._h)
+
+    def scratch_bytes(self) -> int:
+        return int(_lib.lib().plh_ctx_scratch_bytes(self._h))
+
+    def trim(self):
+        _lib.call("plh_ctx_trim", self._h)
+
+    def close(self):
+        if self._h:
+            _lib.lib().plh_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class TruncationOptions:
+    """TruncationOptions<double> (include/PauliPropagation.h:26-31); None = unset."""
+
+    max_weight: int | None = None
+    coefficient_cutoff: float = 0.0
+    max_terms: int | None = None
+
+    def args(self):
+        return (-1 if self.max_weight is None else int(self.max_weight),
+                float(self.coefficient_cutoff),
+                -1 if self.max_terms is None else int(self.max_terms))
+
+
+class PauliSum:
+    """Device-resident PauliSum<64*array_size, double> (plh_sum)."""
+
+    def __init__(self, ctx: Context, array_size: int, num_qubits: int | None = None, capacity: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        nq = 64 * array_size if num_qubits is None else num_qubits
+        _lib.call("plh_sum_create", ctx.handle, array_size, nq, capacity, C.byref(h))
+        self._h = h
+
+    # -------------------------------------------------------- lifetime
+    @classmethod
+    def from_host(cls, ctx: Context, host: HostSum) -> "PauliSum":
+        s = cls(ctx, host.array_size, host.num_qubits, max(len(host), 16))
+        s.upload(host)
+        return s
+
+    def upload(self, host: HostSum):
+        x = np.ascontiguousarray(host.x)
+        z = np.ascontiguousarray(host.z)
+        f = np.ascontiguousarray(host.flags)
+        v = np.ascontiguousarray(host.value)
+        _lib.call("plh_sum_upload", self._h, len(host), _u64(x), _u64(z), _u8(f), _f64(v))
+        _lib.call("plh_sum_set_num_qubits", self._h, host.num_qubits)
+
+    def to_host(self) -> HostSum:
+        n = len(self)
+        a = self.array_size
+        x = np.empty((a, n), np.uint64)
+        z = np.empty((a, n), np.uint64)
+        f = np.empty(n, np.uint8)
+        v = np.empty(n, np.float64)
+        _lib.call("plh_sum_download", self._h, _u64(x), _u64(z), _u8(f), _f64(v))
+        return HostSum(a, self.num_qubits(), x, z, f, v)
+
+    download = to_host
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().plh_sum_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self) -> int:
+        return int(_lib.lib().plh_sum_length(self._h))
+
+    get_length = __len__
+
+    def num_qubits(self) -> int:
+        return int(_lib.lib().plh_sum_num_qubits(self._h))
+
+    @property
+    def array_size(self) -> int:
+        return int(_lib.lib().plh_sum_array_size(self._h))
+
+    def device_planes(self):
+        """(x, z, flags, value) device addresses and the plane stride."""
+        ptrs = [C.c_void_p() for _ in range(4)]
+        stride = C.c_int64()
+        _lib.call("plh_sum_device_planes", self._h, *[C.byref(p) for p in ptrs], C.byref(stride))
+        return tuple(p.value for p in ptrs), stride.value
+
+    def copy(self) -> "PauliSum":
+        out = PauliSum(self.ctx, self.array_size, self.num_qubits(), max(len(self), 16))
+        _lib.call("plh_sum_copy", self._h, out._h)
+        return out
+
+    def _new(self) -> "PauliSum":
+        return PauliSum(self.ctx, self.array_size, self.num_qubits())
+
+    # ------------------------------------------------------ bulk ops
+    def multiply(self, rhs: "PauliSum"):
+        """PauliSum::multiply: self <- self * rhs, uncombined, out[i*n + j]."""
+        _lib.call("plh_multiply", self._h, rhs._h)
+        return self
+
+    def __imul__(self, rhs):
+        return self.multiply(rhs)
+
+    def multiply_canonical(self, rhs: "PauliSum") -> "PauliSum":
+        """canonicalize(self * rhs) via the fused product->collation path."""
+        out = self._new()
+        _lib.call("plh_multiply_canonical", self._h, rhs._h, out._h)
+        return out
+
+    def sort_and_combine(self):
+        _lib.call("plh_sort_and_combine", self._h)
+        return self
+
+    def canonicalize(self):
+        _lib.call("plh_canonicalize", self._h)
+        return self
+
+    def __iadd__(self, rhs: "PauliSum"):
+        _lib.call("plh_add", self._h, rhs._h)
+        return self
+
+    def zero_state_expectation(self) -> float:
+        out = C.c_double()
+        _lib.call("plh_zero_state_expectation", self._h, C.byref(out))
+        return out.value
+
+    def truncate_weight(self, max_weight: int):
+        _lib.call("plh_truncate_weight", self._h, int(max_weight))
+        return self
+
+    def cutoff(self, eps: float):
+        _lib.call("plh_cutoff", self._h, float(eps))
+        return self
+
+    def apply_H(self, c: int):
+        _lib.call("plh_apply_H", self._h, int(c))
+        return self
+
+    def apply_S(self, c: int):
+        _lib.call("plh_apply_S", self._h, int(c))
+        return self
+
+    def apply_CNOT(self, c: int, t: int):
+        _lib.call("plh_apply_CNOT", self._h, int(c), int(t))
+        return self
+
+    def evolve(self, rotations: HostSum):
+        """PauliSum::evolve: each rotation term in order; its coefficient is the angle."""
+        px = np.ascontiguousarray(rotations.x.T)
+        pz = np.ascontiguousarray(rotations.z.T)
+        pf = np.ascontiguousarray(rotations.flags)
+        th = np.ascontiguousarray(rotations.value)
+        _lib.call("plh_evolve", self._h, len(rotations), _u64(px), _u64(pz), _u8(pf), _f64(th))
+        return self
+
+
+def commutator(a: PauliSum, b: PauliSum) -> PauliSum:
+    """[A, B] uncombined, anticommuting pairs in (i over a, j over b) order."""
+    out = a._new()
+    _lib.call("plh_commutator", a._h, b._h, 0, out._h)
+    return out
+
+
+def i_commutator(a: PauliSum, b: PauliSum) -> PauliSum:
+    out = a._new()
+    _lib.call("plh_commutator", a._h, b._h, 1, out._h)
+    return out
+
+
+def commutator_canonical(a: PauliSum, b: PauliSum, multiply_by_i: bool = False) -> PauliSum:
+    """canonicalize([A, B]) (or of i[A, B]) without the uncombined list."""
+    out = a._new()
+    _lib.call("plh_commutator_canonical", a._h, b._h, 1 if multiply_by_i else 0, out._h)
+    return out
+
+
+def apply_truncation(O: PauliSum, opts: TruncationOptions):
+    _lib.call("plh_apply_truncation", O._h, *opts.args())
+    return O
+
+
+def propagate(O: PauliSum, rotations: HostSum, opts: TruncationOptions | None = None):
+    """propagate(O, rotations, opts): gates in reverse, evolve + apply_truncation each."""
+    opts = opts or TruncationOptions()
+    px = np.ascontiguousarray(rotations.x.T)
+    pz = np.ascontiguousarray(rotations.z.T)
+    pf = np.ascontiguousarray(rotations.flags)
+    th = np.ascontiguousarray(rotations.value)
+    _lib.call("plh_propagate", O._h, len(rotations), _u64(px), _u64(pz), _u8(pf), _f64(th), *opts.args())
+    return O
+
+
+def propagate_expectation(O: PauliSum, rotations: HostSum, opts: TruncationOptions | None = None) -> float:
+    work = O.copy()
+    propagate(work, rotations, opts)
+    r = work.zero_state_expectation()
+    work.close()
+    return r
+
+
+def run_program(O: PauliSum, prog, opts: TruncationOptions | None = None):
+    """Heisenberg run of a Clifford + rotation program (see workloads.Program)."""
+    opts = opts or TruncationOptions()
+    _lib.call("plh_run_program", O._h, len(prog.kind), _i32(prog.kind), _i64(prog.q0), _i64(prog.q1),
+              _u64(prog.px), _u64(prog.pz), _f64(prog.theta), *opts.args())
+    return O

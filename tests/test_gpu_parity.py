@@ -1,0 +1,262 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle.
+
+Strings and phases must match bit for bit.  Coefficients are compared bit
+for bit as well: every collated coefficient is a sequential sum over the
+run in original-index order on both sides, exactly as the reference's
+sort_and_combine computes it (include/PauliSumOperations.h:441-455).  The
+one reduction whose order differs is zero_state_expectation (a fixed tree
+on the GPU, sequential in the reference): it is checked to 1e-12 of the
+sum of the magnitudes of its addends.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import assert_terms_equal, cpu
+from paulib_amd import Context, PauliSum, TruncationOptions, commutator, commutator_canonical, i_commutator
+from paulib_amd import _lib, propagate, run_program
+from paulib_amd import workloads as W
+from paulib_amd.host import HostSum
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(ctx, host):
+    return PauliSum.from_host(ctx, host)
+
+
+def oracle_product(A, B, canonical):
+    s = cpu("oracle", A)
+    s.multiply(cpu("oracle", B))
+    if canonical:
+        s.canonicalize()
+    return s.to_host()
+
+
+SHAPES = [(1, 40, 13, 7), (1, 64, 3, 2), (2, 100, 13, 7), (2, 128, 31, 17), (8, 500, 10, 10),
+          (8, 500, 30, 20), (1, 64, 100, 100), (3, 150, 20, 11), (16, 1000, 5, 9)]
+
+
+@pytest.mark.parametrize("as_,nq,n,m", SHAPES)
+def test_multiply_raw(ctx, as_, nq, n, m):
+    A = W.random_words_sum(n, nq, 42, array_size=as_)
+    B = W.random_words_sum(m, nq, 99, array_size=as_)
+    d = dev(ctx, A)
+    d.multiply(dev(ctx, B))
+    got = d.to_host()
+    assert got.num_qubits == A.num_qubits
+    assert_terms_equal(got, oracle_product(A, B, False))
+
+
+def test_launch_mul_soa_raw_launcher(ctx):
+    """plh_launch_mul_soa with caller-allocated planes (GpuMulBench usage)."""
+    A = W.random_words_sum(57, 128, 3, array_size=2)
+    B = W.random_words_sum(33, 128, 4, array_size=2)
+    a, b = dev(ctx, A), dev(ctx, B)
+    out = PauliSum(ctx, 2, 128, capacity=57 * 33)
+    # Out planes must be strided by n*m: upload a zero sum of that size first.
+    zeros = HostSum(2, 128, np.zeros((2, 57 * 33), np.uint64), np.zeros((2, 57 * 33), np.uint64),
+                    np.zeros(57 * 33, np.uint8), np.zeros(57 * 33))
+    out.upload(zeros)
+    (ax, az, af, av), sa = a.device_planes()
+    (bx, bz, bf, bv), sb = b.device_planes()
+    (ox, oz, of, ov), so = out.device_planes()
+    assert sa >= 57 and sb >= 33
+    if sa != 57 or sb != 33 or so != 57 * 33:
+        pytest.skip("device planes not packed; raw launcher needs stride == length")
+    _lib.call("plh_launch_mul_soa", 2, ax, az, af, av, bx, bz, bf, bv, ox, oz, of, ov, 57, 33, ctx.stream)
+    ctx.synchronize()
+    assert_terms_equal(out.to_host(), oracle_product(A, B, False))
+
+
+def test_launch_mul_soa_rejects_bad_array_size(ctx):
+    with pytest.raises(_lib.EngineError) as e:
+        _lib.call("plh_launch_mul_soa", 17, None, None, None, None, None, None, None, None,
+                  None, None, None, None, 1, 1, None)
+    assert e.value.code == -2
+
+
+def test_multiply_empty_operand(ctx):
+    A = W.random_words_sum(5, 64, 1)
+    E = HostSum(1, 64)
+    d = dev(ctx, A)
+    d.multiply(dev(ctx, E))
+    assert len(d) == 0
+    d2 = dev(ctx, E)
+    d2.multiply(dev(ctx, A))
+    assert len(d2) == 0
+    assert len(dev(ctx, A).multiply_canonical(dev(ctx, E))) == 0
+
+
+@pytest.mark.parametrize("as_,nq,n,m", SHAPES)
+def test_multiply_canonical(ctx, as_, nq, n, m):
+    A = W.random_words_sum(n, nq, 7, array_size=as_)
+    B = W.random_words_sum(m, nq, 8, array_size=as_)
+    got = dev(ctx, A).multiply_canonical(dev(ctx, B)).to_host()
+    assert_terms_equal(got, oracle_product(A, B, True))
+
+
+def test_multiply_canonical_config_1000x1000_q32(ctx):
+    """The 1,000 x 1,000 product on 32 qubits (sites I w.p. 1/2, N(0,1))."""
+    A, B = W.product_1000x1000_q32(seed=1)
+    got = dev(ctx, A).multiply_canonical(dev(ctx, B)).to_host()
+    assert_terms_equal(got, oracle_product(A, B, True))
+
+
+@pytest.mark.parametrize("nq,n,m", [(3, 200, 300), (5, 2000, 900), (2, 64, 64), (8, 4096, 1500)])
+def test_multiply_canonical_heavy_collisions(ctx, nq, n, m):
+    """Few qubits: long runs of equal keys, sums of many addends in order."""
+    A = W.random_sum(n, nq, 0.3, "normal", 21)
+    B = W.random_sum(m, nq, 0.3, "normal", 22)
+    A.flags[:] = np.arange(n) % 4
+    got = dev(ctx, A).multiply_canonical(dev(ctx, B)).to_host()
+    assert_terms_equal(got, oracle_product(A, B, True))
+
+
+def test_multiply_canonical_identity_only(ctx):
+    """All keys equal (identity strings): one run of n*m addends."""
+    A = HostSum.from_terms([("II", 0.5 + i) for i in range(300)], array_size=1)
+    B = HostSum.from_terms([("II", -1.25 + 0.01 * i) for i in range(200)], array_size=1)
+    got = dev(ctx, A).multiply_canonical(dev(ctx, B)).to_host()
+    assert_terms_equal(got, oracle_product(A, B, True))
+
+
+def test_multiply_canonical_large_64q(ctx):
+    """65,536 x 96 on 64 qubits: several global radix passes."""
+    A, B = W.product_65536x65536_q64(seed=3, n=65536, m=96)
+    got = dev(ctx, A).multiply_canonical(dev(ctx, B)).to_host()
+    assert_terms_equal(got, oracle_product(A, B, True))
+
+
+@pytest.mark.parametrize("by_i", [False, True])
+@pytest.mark.parametrize("as_,nq,n,m", [(1, 40, 200, 150), (1, 30, 100, 100), (2, 128, 50, 40),
+                                         (8, 500, 20, 30)])
+def test_commutator_raw_and_canonical(ctx, by_i, as_, nq, n, m):
+    A = W.random_words_sum(n, nq, 11, array_size=as_)
+    B = W.random_words_sum(m, nq, 22, array_size=as_)
+    c = cpu("oracle", A).commutator(cpu("oracle", B), by_i)
+    want_raw = c.to_host()
+    a, b = dev(ctx, A), dev(ctx, B)
+    raw = (i_commutator(a, b) if by_i else commutator(a, b)).to_host()
+    assert_terms_equal(raw, want_raw)
+    c.canonicalize()
+    got = commutator_canonical(a, b, by_i).to_host()
+    assert_terms_equal(got, c.to_host())
+
+
+def test_commutator_commuting_inputs_give_empty(ctx):
+    a = dev(ctx, HostSum.from_terms([("ZI", 1.0)]))
+    b = dev(ctx, HostSum.from_terms([("IZ", 1.0)]))
+    assert len(commutator(a, b)) == 0
+    assert len(commutator_canonical(a, b)) == 0
+
+
+@pytest.mark.parametrize("nq,n", [(3, 500), (6, 5000), (64, 3000), (10, 40000), (130, 700)])
+def test_sort_and_combine_and_canonicalize(ctx, nq, n):
+    S = W.random_words_sum(n, nq, 5)
+    S.value[::7] = 0.0
+    for op in ("sort_and_combine", "canonicalize"):
+        d = dev(ctx, S)
+        getattr(d, op)()
+        o = cpu("oracle", S)
+        getattr(o, op)()
+        assert_terms_equal(d.to_host(), o.to_host())
+
+
+def test_add_operator(ctx):
+    A = W.random_words_sum(300, 6, 1)
+    B = W.random_words_sum(200, 6, 2)
+    d = dev(ctx, A)
+    d += dev(ctx, B)
+    o = cpu("oracle", A)
+    o.add(cpu("oracle", B))
+    assert_terms_equal(d.to_host(), o.to_host())
+    d += d  # self-add
+    o2 = cpu("oracle", o.to_host())
+    o2.add(cpu("oracle", o.to_host()))
+    assert_terms_equal(d.to_host(), o2.to_host())
+
+
+def test_clifford_gates(ctx):
+    for as_, nq in [(1, 64), (2, 128)]:
+        S = W.random_words_sum(777, nq, 9, array_size=as_)
+        d = dev(ctx, S)
+        o = cpu("oracle", S)
+        for g, args in [("apply_H", (3,)), ("apply_S", (17,)), ("apply_CNOT", (3, 9)),
+                        ("apply_CNOT", (9, 3)), ("apply_CNOT", (0, nq - 1)), ("apply_H", (nq - 1,)),
+                        ("apply_S", (0,)), ("apply_CNOT", (5, 6))]:
+            getattr(d, g)(*args)
+            getattr(o, g)(*args)
+            assert_terms_equal(d.to_host(), o.to_host())
+
+
+def test_clifford_rejects_out_of_range(ctx):
+    d = dev(ctx, W.random_words_sum(4, 64, 1))
+    with pytest.raises(_lib.EngineError):
+        d.apply_H(64)
+
+
+@pytest.mark.parametrize("theta", [0.0, math.pi, math.pi / 2, -math.pi / 2, math.pi / 3, 1.234, -2.5])
+def test_evolve(ctx, theta):
+    S = W.random_words_sum(500, 40, 13)
+    rot = HostSum.from_terms([("Z" * 3 + "X" + "I" * 36, theta), ("IYZ" + "I" * 37, 0.7)], array_size=1)
+    d = dev(ctx, S)
+    d.evolve(rot)
+    o = cpu("oracle", S)
+    o.evolve(rot)
+    assert_terms_equal(d.to_host(), o.to_host())
+
+
+def test_truncation_filters(ctx):
+    S = W.random_sum(3000, 20, 0.6, "normal", 4)
+    for op, arg in [("truncate_weight", 5), ("cutoff", 0.5)]:
+        d = dev(ctx, S)
+        getattr(d, op)(arg)
+        o = cpu("oracle", S)
+        getattr(o, op)(arg)
+        assert_terms_equal(d.to_host(), o.to_host())
+
+
+def test_zero_state_expectation(ctx):
+    S = W.random_sum(100000, 50, 0.9, "normal", 8)
+    S.flags[:] = np.arange(len(S)) % 4
+    got = dev(ctx, S).zero_state_expectation()
+    want = cpu("oracle", S).zero_state_expectation()
+    bound = 1e-12 * np.abs(S.value).sum()
+    assert abs(got - want) <= bound
+    assert dev(ctx, HostSum(1, 64)).zero_state_expectation() == 0.0
+
+
+def test_propagate_rotations(ctx):
+    rng = np.random.default_rng(3)
+    terms = []
+    for _ in range(40):
+        s = ["I"] * 24
+        for q in rng.choice(24, size=rng.integers(1, 3), replace=False):
+            s[q] = "XYZ"[rng.integers(0, 3)]
+        terms.append(("".join(s), float(rng.uniform(0, 2 * math.pi))))
+    rot = HostSum.from_terms(terms, array_size=1)
+    O = W.observable(24, [{0: "Z"}, {7: "X", 8: "X"}])
+    for opts in [TruncationOptions(), TruncationOptions(max_weight=4, coefficient_cutoff=1e-3)]:
+        d = dev(ctx, O)
+        propagate(d, rot, opts)
+        o = cpu("oracle", O)
+        o.propagate(rot, opts.max_weight, opts.coefficient_cutoff, opts.max_terms)
+        assert_terms_equal(d.to_host(), o.to_host())
+
+
+@pytest.mark.parametrize("nq,depth,seed", [(16, 4, 1), (24, 6, 3), (40, 5, 4), (100, 3, 5)])
+def test_run_program_brickwork(ctx, nq, depth, seed):
+    prog = W.brickwork_program(nq, depth, seed)
+    O = W.observable(nq, [{0: "Z"}, {nq // 2: "Z"}])
+    d = dev(ctx, O)
+    run_program(d, prog, TruncationOptions(max_weight=8, coefficient_cutoff=1e-4))
+    o = cpu("oracle", O)
+    o.run_program(prog, 8, 1e-4)
+    got, want = d.to_host(), o.to_host()
+    assert_terms_equal(got, want)
+    e_got, e_want = d.zero_state_expectation(), o.zero_state_expectation()
+    assert abs(e_got - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum())
