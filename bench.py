@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark: collated PauliSum product throughput + propagation-job latency.
+
+Step = canonicalize(A * B) for two 65,536-term sums on 64 qubits (sites
+non-identity w.p. 0.25, U[-1,1] coefficients, seeded), computed by the fused
+product->collation path with inputs and output resident in HBM.  value =
+n*m / (seconds per step) (collated product terms/s).  Propagation latency:
+sequential jobs of Z_0 + Z_31 through the depth-20 64-qubit brickwork
+(truncation |c| < 1e-4, weight > 8) on the same device, p50/p99 reported.
+
+roofline: the kernel with the largest share of the timed region, timed
+with HIP events on the engine's stream during the timed steps; achieved =
+its algorithmic bytes per launch / its mean launch time (the emit kernel's
+bytes are 25 B per collated term at one word per plane).
+cpu_baseline: the reference's own multiply + canonicalize (oracle/_ref, its
+OpenMP build) on a 65,536 x m_s slice of the same product, on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_profile(ctx):
+    from paulib_amd import _lib
+
+    n = 64
+    names = C.create_string_buffer(32 * n)
+    ms = (C.c_double * n)()
+    launches = (C.c_int64 * n)()
+    byt = (C.c_double * n)()
+    cnt = C.c_int()
+    _lib.call("plh_ctx_profile_read", ctx.handle, n, names, ms, launches, byt, C.byref(cnt))
+    out = {}
+    for i in range(cnt.value):
+        name = names.raw[32 * i:32 * i + 32].split(b"\0", 1)[0].decode()
+        out[name] = dict(ms=ms[i], launches=int(launches[i]), bytes=byt[i])
+    return out
+
+
+def cpu_baseline(A, B, target_s: float):
+    """Reference multiply + canonicalize on a rhs slice sized for ~target_s."""
+    import oracle
+
+    kind = "reference" if oracle.available("reference") else "oracle"
+    from paulib_amd.host import HostSum
+
+    def run(m):
+        Bs = HostSum(B.array_size, B.num_qubits, B.x[:, :m], B.z[:, :m], B.flags[:m], B.value[:m])
+        a = oracle.CpuSum.from_host(kind, A)
+        b = oracle.CpuSum.from_host(kind, Bs)
+        t0 = time.perf_counter()
+        a.multiply(b)
+        a.canonicalize()
+        return time.perf_counter() - t0
+
+    m = 8
+    t = run(m)  # warm-up and calibration
+    m = int(min(len(B), max(8, m * target_s / max(t, 1e-3))))
+    t = run(m)
+    threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
+    if kind == "reference":
+        import ctypes
+
+        h = ctypes.CDLL(str(oracle.REF_LIB))
+        threads = str(h.ref_num_threads())
+    return {
+        "value": len(A) * m / t,
+        "unit": "collated product terms/s",
+        "cores": int(threads),
+        "kind": "reference" if kind == "reference" else "port",
+        "sample": f"{len(A)} x {m} rhs slice of the 65536 x 65536 q64 product (multiply + canonicalize), {t:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--m", type=int, default=65536)
+    ap.add_argument("--prop-jobs", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-propagation", action="store_true")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_product_65536x65536_q64.json"))
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from paulib_amd import Context, PauliSum, TruncationOptions, _lib, run_program
+    from paulib_amd import workloads as W
+
+    log(f"inputs: {args.n} x {args.m} sums on 64 qubits")
+    A, B = W.product_65536x65536_q64(seed=7, n=args.n, m=args.m)
+    ctx = Context(0)
+    dA, dB = PauliSum.from_host(ctx, A), PauliSum.from_host(ctx, B)
+    out = PauliSum(ctx, 1, 64)
+    pairs = args.n * args.m
+
+    def step():
+        _lib.call("plh_multiply_canonical", dA.handle, dB.handle, out.handle)
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        ctx.synchronize()
+        log(f"warmup {i}: {time.perf_counter() - t0:.3f} s, {len(out)} terms")
+    _lib.call("plh_ctx_profile", ctx.handle, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    prof = kernel_profile(ctx)
+    _lib.call("plh_ctx_profile", ctx.handle, 0)
+    t_out = len(out)
+    log(f"timed: {dt * 1e3:.2f} ms/step, {t_out} collated terms")
+    for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"  {k:20s} {v['ms'] / args.steps:9.3f} ms/step  {v['launches'] / args.steps:6.1f} launches/step")
+
+    dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_s = dom["ms"] / dom["launches"] / 1e3
+    per_launch_bytes = dom["bytes"] / dom["launches"]
+    achieved = per_launch_bytes / per_launch_s / 1e9 if per_launch_bytes else 0.0
+    traffic = None
+    tp = Path(args.traffic)
+    if tp.exists():
+        try:
+            traffic = json.loads(tp.read_text()).get(dom_name)
+        except Exception:
+            traffic = None
+    b_alg_job = (args.n + args.m) * 25 + t_out * 25
+    result = {
+        "metric": "PauliSum product terms/s (collated) + p50/p99 propagation-job latency",
+        "value": pairs / dt,
+        "unit": "collated product terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 words + f64 coefficients",
+        "data": "synthetic, seeded (sites non-I w.p. 0.25, U[-1,1] FP64)",
+        "config": {"workload": f"product_{args.n}x{args.m}_q64 canonicalize(A*B)", "pairs": pairs,
+                   "collated_terms": t_out, "qubits": 64, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
+                     "launch_ms": per_launch_s * 1e3, "alg_bytes_per_launch": per_launch_bytes},
+        "job_roofline": {"alg_bytes": b_alg_job, "frac": b_alg_job / dt / HBM_PEAK},
+        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()},
+    }
+    dA.close(), dB.close(), out.close()
+    ctx.trim()
+
+    if not args.no_propagation and args.prop_jobs > 0:
+        O, prog, tr = W.brickwork_q64_d20()
+        lat = []
+        for j in range(args.prop_jobs + 1):
+            d = PauliSum.from_host(ctx, O)
+            t0 = time.perf_counter()
+            run_program(d, prog, TruncationOptions(max_weight=tr["max_weight"], coefficient_cutoff=tr["cutoff"]))
+            e = d.zero_state_expectation()
+            t = time.perf_counter() - t0
+            log(f"propagation job {j}: {t * 1e3:.1f} ms, {len(d)} terms, <O> = {e:.10f}")
+            if j > 0:  # the first job warms the workspace
+                lat.append(t)
+            d.close()
+        lat = np.array(lat) * 1e3
+        result["propagation"] = {"workload": "brickwork_q64_d20 (Z0+Z31, depth 20, w<=8, |c|>=1e-4)",
+                                 "mode": "sequential jobs, plain drop-in", "jobs": len(lat),
+                                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                                 "jobs_per_s": float(1e3 / lat.mean()), "expectation": e}
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

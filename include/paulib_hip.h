@@ -60,6 +60,17 @@ void* plh_ctx_stream(plh_ctx* ctx);
 int64_t plh_ctx_scratch_bytes(plh_ctx* ctx);
 int plh_ctx_trim(plh_ctx* ctx);
 
+/* Kernel timing: while enabled, the engine brackets its kernel launches
+ * with HIP events on the context's stream.  profile_read synchronises,
+ * aggregates per kernel name (names: 32-byte slots): total milliseconds,
+ * launch count and the algorithmic HBM bytes of those launches; returns the
+ * number of names in *count and clears the records.  No reference
+ * counterpart (the reference times its launcher with cudaEvent,
+ * bench/GpuMulBench.cpp:53-65). */
+int plh_ctx_profile(plh_ctx* ctx, int enable);
+int plh_ctx_profile_read(plh_ctx* ctx, int max_entries, char* names, double* total_ms,
+                         int64_t* launches, double* alg_bytes, int* count);
+
 /* --------------------------------------------------------- device sums */
 /* GpuPauliSum<N,double> (include/gpu/GpuPauliSum.h:34-148): device
  * buffers owned by the engine.  capacity is a hint. */

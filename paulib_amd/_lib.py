@@ -50,6 +50,8 @@ SIGNATURES = {
     "plh_ctx_stream": (_p, [_p]),
     "plh_ctx_scratch_bytes": (_i64, [_p]),
     "plh_ctx_trim": (_int, [_p]),
+    "plh_ctx_profile": (_int, [_p, _int]),
+    "plh_ctx_profile_read": (_int, [_p, _int, C.c_char_p, _f64p, _i64p, _f64p, C.POINTER(_int)]),
     "plh_sum_create": (_int, [_p, _int, _i64, _i64, C.POINTER(_p)]),
     "plh_sum_destroy": (_int, [_p]),
     "plh_sum_upload": (_int, [_p, _i64, _u64p, _u64p, _u8p, _f64p]),

@@ -199,6 +199,11 @@ int plh_ctx_create(int device, void* stream, plh_ctx** out) {
 int plh_ctx_destroy(plh_ctx* ctx) {
     if (!ctx) return PLH_OK;
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto& r : ctx->prof_recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
     ctx->ws.release();
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -219,6 +224,58 @@ int plh_ctx_trim(plh_ctx* ctx) {
     if (!ctx) return PLH_ERR_ARG;
     PLH_HIP(hipStreamSynchronize(ctx->stream));
     ctx->ws.release();
+    return PLH_OK;
+}
+
+int plh_ctx_profile(plh_ctx* ctx, int enable) {
+    if (!ctx) return PLH_ERR_ARG;
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& r : ctx->prof_recs) {
+        ctx->prof_pool.push_back(r.a);
+        ctx->prof_pool.push_back(r.b);
+    }
+    ctx->prof_recs.clear();
+    ctx->prof = enable != 0;
+    return PLH_OK;
+}
+
+int plh_ctx_profile_read(plh_ctx* ctx, int max_entries, char* names, double* total_ms, int64_t* launches,
+                         double* alg_bytes, int* count) {
+    if (!ctx || !count) return PLH_ERR_ARG;
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<std::string> keys;
+    std::vector<double> ms;
+    std::vector<int64_t> cnt;
+    std::vector<double> byt;
+    for (auto& r : ctx->prof_recs) {
+        float t = 0.f;
+        PLH_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        size_t k = 0;
+        while (k < keys.size() && keys[k] != r.name) ++k;
+        if (k == keys.size()) {
+            keys.push_back(r.name);
+            ms.push_back(0.0);
+            cnt.push_back(0);
+            byt.push_back(0.0);
+        }
+        ms[k] += t;
+        cnt[k] += 1;
+        byt[k] += r.bytes;
+        ctx->prof_pool.push_back(r.a);
+        ctx->prof_pool.push_back(r.b);
+    }
+    ctx->prof_recs.clear();
+    const int n = (int)keys.size() < max_entries ? (int)keys.size() : max_entries;
+    for (int i = 0; i < n; ++i) {
+        if (names) {
+            std::memset(names + 32 * i, 0, 32);
+            std::strncpy(names + 32 * i, keys[(size_t)i].c_str(), 31);
+        }
+        if (total_ms) total_ms[i] = ms[(size_t)i];
+        if (launches) launches[i] = cnt[(size_t)i];
+        if (alg_bytes) alg_bytes[i] = byt[(size_t)i];
+    }
+    *count = n;
     return PLH_OK;
 }
 

@@ -865,11 +865,20 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     PLH_TRY(ctx->ws.get(kWsSub, 2 * kDigits * 8, &p));
     uint64_t* sub_start = (uint64_t*)p;
     uint64_t* sub_size = sub_start + kDigits;
-    k_msd_hist<Src, true><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, nullptr, root_start, root_size,
-                                                                 root_tiles, 1u, pos, kTileImplicit, hist);
-    k_colscan<<<1, kBlock, 0, st>>>(root_tiles, 1u, t0, root_start, hist, sub_start, sub_size);
-    k_msd_scatter<Src, true><<<dim3((unsigned)t0), kBlock, 0, st>>>(
-        src, nullptr, ids0, root_start, root_size, root_tiles, 1u, pos, kTileImplicit, hist, sub_start);
+    {
+        ProfScope ps(ctx, "msd_hist_first");
+        k_msd_hist<Src, true><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, nullptr, root_start, root_size,
+                                                                     root_tiles, 1u, pos, kTileImplicit, hist);
+    }
+    {
+        ProfScope ps(ctx, "colscan");
+        k_colscan<<<1, kBlock, 0, st>>>(root_tiles, 1u, t0, root_start, hist, sub_start, sub_size);
+    }
+    {
+        ProfScope ps(ctx, "msd_scatter_first", 4.0 * (double)n_dense);
+        k_msd_scatter<Src, true><<<dim3((unsigned)t0), kBlock, 0, st>>>(
+            src, nullptr, ids0, root_start, root_size, root_tiles, 1u, pos, kTileImplicit, hist, sub_start);
+    }
     PLH_HIP(hipGetLastError());
     depth = pos + kDigitBits;
 
@@ -887,6 +896,8 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
         lz.push_back(hsz[d]);
         lm.push_back((uint32_t)depth | (0u << 16) | (uni0 << 17));
     }
+    uint64_t n_valid = 0;
+    for (uint64_t v : lz) n_valid += v;
     uint64_t L = ls.size();
     if (L == 0) {
         out->n = 0;
@@ -922,6 +933,12 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
         PLH_TRY(scan_u32_to_u64(ctx, ntiles, tbase, nbig, d_total));
         uint64_t T = 0;
         PLH_TRY(read_u64(ctx, d_total, &T));
+        uint64_t pass_items = 0;  // ids moved by this pass (for the timing records)
+        if (ctx->prof) {
+            PLH_TRY(ctx->ws.get(kWsScan1, nbig * 8 + 64, &p));
+            PLH_TRY(scan_u64(ctx, bsize, (uint64_t*)p, nbig, d_total));
+            PLH_TRY(read_u64(ctx, d_total, &pass_items));
+        }
         PLH_TRY(ctx->ws.get(kWsHist, T * kDigits * 4, &p));
         hist = (uint32_t*)p;
         PLH_TRY(ctx->ws.get(kWsSub, nbig * kDigits * 16, &p));
@@ -934,12 +951,21 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
         PLH_HIP(hipStreamSynchronize(st));
         const uint32_t* in_ids = ((hmeta >> 16) & 1u) ? ids1 : ids0;
         uint32_t* out_ids = ((hmeta >> 16) & 1u) ? ids0 : ids1;
-        k_msd_hist<Src, false><<<dim3((unsigned)T), kBlock, 0, st>>>(src, in_ids, bstart, bsize, tbase,
-                                                                     (uint32_t)nbig, pos, kTileExplicit, hist);
-        k_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(tbase, (uint32_t)nbig, T, bstart, hist,
-                                                           sub_start, sub_size);
-        k_msd_scatter<Src, false><<<dim3((unsigned)T), kBlock, 0, st>>>(
-            src, in_ids, out_ids, bstart, bsize, tbase, (uint32_t)nbig, pos, kTileExplicit, hist, sub_start);
+        {
+            ProfScope ps(ctx, "msd_hist", 4.0 * (double)pass_items);
+            k_msd_hist<Src, false><<<dim3((unsigned)T), kBlock, 0, st>>>(src, in_ids, bstart, bsize, tbase,
+                                                                         (uint32_t)nbig, pos, kTileExplicit, hist);
+        }
+        {
+            ProfScope ps(ctx, "colscan");
+            k_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(tbase, (uint32_t)nbig, T, bstart, hist,
+                                                               sub_start, sub_size);
+        }
+        {
+            ProfScope ps(ctx, "msd_scatter", 8.0 * (double)pass_items);
+            k_msd_scatter<Src, false><<<dim3((unsigned)T), kBlock, 0, st>>>(
+                src, in_ids, out_ids, bstart, bsize, tbase, (uint32_t)nbig, pos, kTileExplicit, hist, sub_start);
+        }
         PLH_HIP(hipGetLastError());
         depth = pos + kDigitBits;
         const uint32_t uni = next_live_host(h_live, kb, depth) >= kb ? 1u : 0u;
@@ -967,8 +993,11 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     uint32_t* kept = (uint32_t*)p;
     PLH_TRY(ctx->ws.get(kWsOffs, L * 8 + 64, &p));
     uint64_t* offs = (uint64_t*)p;
-    k_bucket_sort<AS, Src><<<dim3((unsigned)L), kBlock, 0, st>>>(src, cfg, A.start, A.size, A.meta,
-                                                                 ids0, ids1, kept);
+    {
+        ProfScope ps(ctx, "bucket_sort", 8.0 * (double)n_valid);
+        k_bucket_sort<AS, Src><<<dim3((unsigned)L), kBlock, 0, st>>>(src, cfg, A.start, A.size, A.meta,
+                                                                     ids0, ids1, kept);
+    }
     PLH_HIP(hipGetLastError());
     PLH_TRY(scan_u32_to_u64(ctx, kept, offs, L, d_total));
     uint64_t total = 0;
@@ -981,8 +1010,11 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     cfg.of = out->f;
     cfg.ov = out->v;
     cfg.ostride = out->cap;
-    k_emit<AS, Src><<<dim3((unsigned)L), kBlock, 0, st>>>(src, cfg, A.start, A.size, A.meta, ids0, ids1,
-                                                          offs);
+    {
+        ProfScope ps(ctx, "emit", (double)total * (16.0 * out_as + 9.0));
+        k_emit<AS, Src><<<dim3((unsigned)L), kBlock, 0, st>>>(src, cfg, A.start, A.size, A.meta, ids0, ids1,
+                                                              offs);
+    }
     PLH_HIP(hipGetLastError());
     return PLH_OK;
 }

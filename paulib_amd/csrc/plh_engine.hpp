@@ -71,15 +71,39 @@ struct plh_sum {
     bool canonical = false;  // (x, z, imag) keys distinct, flags in {0,1}, no zeros
 };
 
+namespace plh {
+// Kernel timing with HIP events on the context's stream (enabled by
+// plh_ctx_profile); records are aggregated per kernel name on read.
+struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+    double bytes;  // algorithmic HBM bytes of the bracketed launch(es)
+};
+}  // namespace plh
+
 struct plh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     plh::Workspace ws;
     plh::HostStage stage;
-    // Sizes picked by the autotuner (0 = built-in default).
-    uint32_t tune_tile_rounds = 0;
+    bool prof = false;
+    std::vector<plh::ProfRec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
 };
+
+namespace plh {
+// RAII: events around the launches issued inside the scope.
+class ProfScope {
+   public:
+    ProfScope(plh_ctx* c, const char* name, double alg_bytes = 0.0);
+    ~ProfScope();
+
+   private:
+    plh_ctx* c_;
+    int idx_ = -1;
+};
+}  // namespace plh
 
 namespace plh {
 

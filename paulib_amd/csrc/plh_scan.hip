@@ -149,6 +149,30 @@ size_t Workspace::bytes_held() const {
     return t;
 }
 
+static hipEvent_t take_event(plh_ctx* c) {
+    if (!c->prof_pool.empty()) {
+        hipEvent_t e = c->prof_pool.back();
+        c->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+ProfScope::ProfScope(plh_ctx* c, const char* name, double alg_bytes) : c_(c) {
+    if (!c_ || !c_->prof) return;
+    hipEvent_t a = take_event(c_), b = take_event(c_);
+    if (!a || !b) return;
+    if (hipEventRecord(a, c_->stream) != hipSuccess) return;
+    idx_ = (int)c_->prof_recs.size();
+    c_->prof_recs.push_back(ProfRec{name, a, b, alg_bytes});
+}
+
+ProfScope::~ProfScope() {
+    if (idx_ >= 0) (void)hipEventRecord(c_->prof_recs[(size_t)idx_].b, c_->stream);
+}
+
 int HostStage::init() {
     PLH_HIP(hipHostMalloc((void**)&p, 64 * sizeof(uint64_t), hipHostMallocDefault));
     return PLH_OK;

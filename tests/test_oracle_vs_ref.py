@@ -1,0 +1,115 @@
+"""The oracle's C restatement against the reference's own headers compiled as
+oracle/_ref (ref_bridge.cpp), on seeded random inputs.  Strings and phases
+must agree bit for bit; coefficients bit for bit except where a run sums
+three or more addends (the reference is built with -ffast-math, so its
+in-order loop may be re-associated), where 1e-12 relative applies."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import assert_terms_equal, cpu
+from paulib_amd import workloads as W
+from paulib_amd.host import HostSum
+
+pytestmark = pytest.mark.skipif(not oracle.available("reference"),
+                                reason="oracle/_ref not built (needs the reference tree)")
+
+
+def both(host, *ops):
+    res = []
+    for kind in ("oracle", "reference"):
+        s = cpu(kind, host)
+        for name, *args in ops:
+            getattr(s, name)(*args)
+        res.append(s.to_host())
+    return res
+
+
+@pytest.mark.parametrize("as_,nq,n,m", [(1, 64, 13, 7), (2, 128, 13, 7), (1, 64, 3, 2), (8, 500, 10, 10),
+                                         (4, 200, 17, 9), (16, 1024, 4, 5), (1, 40, 300, 200)])
+def test_multiply_and_canonicalize(as_, nq, n, m):
+    A = W.random_words_sum(n, nq, 42, array_size=as_)
+    B = W.random_words_sum(m, nq, 99, array_size=as_)
+    out = []
+    for kind in ("oracle", "reference"):
+        s = cpu(kind, A)
+        s.multiply(cpu(kind, B))
+        raw = s.to_host()
+        s.canonicalize()
+        out.append((raw, s.to_host()))
+    assert_terms_equal(out[0][0], out[1][0])
+    assert_terms_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("nq", [3, 5, 8])
+def test_collisions_sort_and_combine(nq):
+    S = W.random_sum(20000, nq, 0.3, "normal", 4)
+    S.flags[:] = np.arange(len(S)) % 4
+    for op in ("sort_and_combine", "canonicalize"):
+        o, r = both(S, (op,))
+        assert_terms_equal(o, r, values="rtol")
+
+
+@pytest.mark.parametrize("by_i", [False, True])
+def test_commutator(by_i):
+    A = W.random_words_sum(200, 40, 11)
+    B = W.random_words_sum(150, 40, 22)
+    res = []
+    for kind in ("oracle", "reference"):
+        c = cpu(kind, A).commutator(cpu(kind, B), by_i)
+        c.canonicalize()
+        res.append(c.to_host())
+    assert_terms_equal(res[0], res[1])
+
+
+@pytest.mark.parametrize("theta", [0.0, math.pi, math.pi / 2, -math.pi / 2, 0.4, 2.2])
+def test_evolve(theta):
+    S = W.random_words_sum(400, 30, 6)
+    rot = HostSum.from_terms([("XYZ" + "I" * 27, theta), ("I" * 10 + "ZZ" + "I" * 18, 1.1)], array_size=1)
+    o, r = both(S, ("evolve", rot))
+    assert_terms_equal(o, r)
+
+
+def test_gates_on_random_sum():
+    S = W.random_words_sum(1000, 128, 3, array_size=2)
+    o, r = both(S, ("apply_H", 5), ("apply_S", 70), ("apply_CNOT", 3, 100), ("apply_CNOT", 64, 2))
+    assert_terms_equal(o, r)
+
+
+def test_truncation():
+    S = W.random_sum(5000, 12, 0.5, "normal", 2)
+    o, r = both(S, ("apply_truncation", 4, 0.3, None))
+    assert_terms_equal(o, r, values="rtol")
+
+
+@pytest.mark.parametrize("nq,depth,seed,mw,cut", [(20, 5, 1, 8, 1e-4), (32, 4, 2, 6, 1e-3), (48, 3, 3, 8, 1e-5)])
+def test_brickwork_programs(nq, depth, seed, mw, cut):
+    prog = W.brickwork_program(nq, depth, seed)
+    O = W.observable(nq, [{0: "Z"}, {nq // 2: "Z"}])
+    res = []
+    for kind in ("oracle", "reference"):
+        s = cpu(kind, O)
+        s.run_program(prog, mw, cut)
+        res.append((s.to_host(), s.zero_state_expectation()))
+    assert_terms_equal(res[0][0], res[1][0])
+    assert abs(res[0][1] - res[1][1]) <= 1e-12 * max(1.0, np.abs(res[0][0].value).sum())
+
+
+def test_propagate_rotation_list():
+    rng = np.random.default_rng(5)
+    terms = []
+    for _ in range(60):
+        s = ["I"] * 16
+        for q in rng.choice(16, size=rng.integers(1, 3), replace=False):
+            s[q] = "XYZ"[rng.integers(0, 3)]
+        terms.append(("".join(s), float(rng.uniform(-math.pi, math.pi))))
+    rot = HostSum.from_terms(terms, array_size=1)
+    O = W.observable(16, [{3: "Z"}])
+    res = []
+    for kind in ("oracle", "reference"):
+        s = cpu(kind, O)
+        s.propagate(rot, 6, 1e-4)
+        res.append(s.to_host())
+    assert_terms_equal(res[0], res[1])
