@@ -56,5 +56,24 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
+ROOT = HERE.parent
+CPP_TEST = ROOT / "tests" / "cpp" / "mirror_test.cpp"
+CPP_TEST_BIN = ROOT / "tests" / "cpp" / "_build" / "mirror_test"
+
+
+def build_cpp_tests() -> Path:
+    """The C++ drop-in test (include/paulib_hip.hpp against the oracle)."""
+    CPP_TEST_BIN.parent.mkdir(parents=True, exist_ok=True)
+    oracle_dir = ROOT / "oracle" / "_build"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", str(CPP_TEST), "-o", str(CPP_TEST_BIN),
+           f"-L{OUT}", "-lpaulib_hip", f"-L{oracle_dir}", "-lpauli_oracle",
+           f"-Wl,-rpath,{OUT}", f"-Wl,-rpath,{oracle_dir}",
+           "-Wl,-rpath,$ORIGIN/../../../paulib_amd/_build", "-Wl,-rpath,$ORIGIN/../../../oracle/_build"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ mirror test build failed:\n{r.stdout}\n{r.stderr}")
+    return CPP_TEST_BIN
+
+
 if __name__ == "__main__":
     build()

@@ -1,4 +1,4 @@
-// plh_block.hpp -- wave/workgroup primitives (256-thread workgroups, wave64).
+// plh_block.hpp -- wave/workgroup primitives (wave64).
 #pragma once
 #include "plh_common.hpp"
 
@@ -22,9 +22,9 @@ PLH_DEV T wave_sum(T v) {
     return v;
 }
 
-// Exclusive scan across the 256 threads of a workgroup; `sh` holds
-// kWavesPerBlock entries.  Contains barriers: call from uniform control flow.
-template <class T>
+// Exclusive scan across the NW waves of a workgroup; `sh` holds NW entries.
+// Contains barriers: call from uniform control flow.
+template <int NW = kWavesPerBlock, class T>
 PLH_DEV T block_exclusive_scan(T v, T& total, T* sh) {
     const uint32_t lane = lane_id();
     const int w = threadIdx.x / kWave;
@@ -33,7 +33,7 @@ PLH_DEV T block_exclusive_scan(T v, T& total, T* sh) {
     __syncthreads();
     T pre = 0, tot = 0;
 #pragma unroll
-    for (int i = 0; i < kWavesPerBlock; ++i) {
+    for (int i = 0; i < NW; ++i) {
         const T s = sh[i];
         if (i < w) pre += s;
         tot += s;
@@ -43,10 +43,10 @@ PLH_DEV T block_exclusive_scan(T v, T& total, T* sh) {
     return pre + inc - v;
 }
 
-template <class T>
+template <int NW = kWavesPerBlock, class T>
 PLH_DEV T block_sum(T v, T* sh) {
     T total;
-    (void)block_exclusive_scan(v, total, sh);
+    (void)block_exclusive_scan<NW>(v, total, sh);
     return total;
 }
 

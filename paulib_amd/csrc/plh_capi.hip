@@ -25,38 +25,57 @@ void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
 
 void sum_free_storage(plh_sum* s) {
-    if (s->x) (void)hipFree(s->x);
-    if (s->z) (void)hipFree(s->z);
-    if (s->f) (void)hipFree(s->f);
-    if (s->v) (void)hipFree(s->v);
+    if (s->block) {
+        if (s->ctx) s->ctx->blocks.free(s->block);
+        else (void)hipFree(s->block);
+    }
+    s->block = nullptr;
     s->x = s->z = nullptr;
     s->f = nullptr;
     s->v = nullptr;
     s->cap = 0;
 }
 
+void sum_take_storage(plh_sum* dst, plh_sum* src) {
+    sum_free_storage(dst);
+    dst->block = src->block;
+    dst->x = src->x;
+    dst->z = src->z;
+    dst->f = src->f;
+    dst->v = src->v;
+    dst->cap = src->cap;
+    src->block = nullptr;
+    src->x = src->z = nullptr;
+    src->f = nullptr;
+    src->v = nullptr;
+    src->cap = 0;
+}
+
+// Layout of a sum's block: x planes, z planes, values, flags.
+static size_t sum_bytes(int as, int64_t cap) { return (size_t)cap * (16 * (size_t)as + 9); }
+
 int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
-    if (cap <= s->cap && s->x) return PLH_OK;
+    if (cap <= s->cap && s->block) return PLH_OK;
     int64_t nc = cap < 16 ? 16 : cap;
     if (s->cap > 0 && nc < s->cap + s->cap / 2 && keep > 0) nc = s->cap + s->cap / 2;  // amortise appends
-    uint64_t *nx = nullptr, *nz = nullptr;
-    uint8_t* nf = nullptr;
-    double* nv = nullptr;
-    hipError_t e = hipMalloc((void**)&nx, (size_t)s->as * nc * 8);
-    if (e == hipSuccess) e = hipMalloc((void**)&nz, (size_t)s->as * nc * 8);
-    if (e == hipSuccess) e = hipMalloc((void**)&nf, (size_t)nc);
-    if (e == hipSuccess) e = hipMalloc((void**)&nv, (size_t)nc * 8);
-    if (e != hipSuccess) {
-        (void)hipFree(nx);
-        (void)hipFree(nz);
-        (void)hipFree(nf);
-        (void)hipFree(nv);
-        (void)hipGetLastError();
-        set_error("device allocation of " + std::to_string(nc) + " terms failed: " + hipGetErrorString(e));
-        return PLH_ERR_OOM;
+    void* blk = nullptr;
+    if (s->ctx) {
+        PLH_TRY(s->ctx->blocks.alloc(sum_bytes(s->as, nc), &blk));
+    } else {
+        hipError_t e = hipMalloc(&blk, sum_bytes(s->as, nc));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error(std::string("device allocation failed: ") + hipGetErrorString(e));
+            return PLH_ERR_OOM;
+        }
     }
+    char* b = (char*)blk;
+    uint64_t* nx = (uint64_t*)b;
+    uint64_t* nz = nx + (size_t)s->as * nc;
+    double* nv = (double*)(nz + (size_t)s->as * nc);
+    uint8_t* nf = (uint8_t*)(nv + nc);
     hipStream_t st = s->ctx ? s->ctx->stream : nullptr;
-    if (keep > 0 && s->x) {
+    if (keep > 0 && s->block) {
         for (int k = 0; k < s->as; ++k) {
             PLH_HIP(hipMemcpyAsync(nx + (int64_t)k * nc, s->x + (int64_t)k * s->cap, (size_t)keep * 8,
                                    hipMemcpyDeviceToDevice, st));
@@ -66,10 +85,8 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
         PLH_HIP(hipMemcpyAsync(nf, s->f, (size_t)keep, hipMemcpyDeviceToDevice, st));
         PLH_HIP(hipMemcpyAsync(nv, s->v, (size_t)keep * 8, hipMemcpyDeviceToDevice, st));
     }
-    if (s->x) {
-        PLH_HIP(hipStreamSynchronize(st));
-        sum_free_storage(s);
-    }
+    sum_free_storage(s);  // stream-ordered reuse: no synchronisation needed
+    s->block = blk;
     s->x = nx;
     s->z = nz;
     s->f = nf;
@@ -199,6 +216,12 @@ int plh_ctx_create(int device, void* stream, plh_ctx** out) {
 int plh_ctx_destroy(plh_ctx* ctx) {
     if (!ctx) return PLH_OK;
     (void)hipStreamSynchronize(ctx->stream);
+    for (plh_sum* s : ctx->sums) {  // sums outliving their context become empty and unusable
+        sum_free_storage(s);
+        s->ctx = nullptr;
+        s->n = 0;
+    }
+    ctx->sums.clear();
     for (auto& r : ctx->prof_recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -218,12 +241,15 @@ int plh_ctx_synchronize(plh_ctx* ctx) {
 
 void* plh_ctx_stream(plh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-int64_t plh_ctx_scratch_bytes(plh_ctx* ctx) { return ctx ? (int64_t)ctx->ws.bytes_held() : 0; }
+int64_t plh_ctx_scratch_bytes(plh_ctx* ctx) {
+    return ctx ? (int64_t)(ctx->ws.bytes_held() + ctx->blocks.bytes_cached()) : 0;
+}
 
 int plh_ctx_trim(plh_ctx* ctx) {
     if (!ctx) return PLH_ERR_ARG;
     PLH_HIP(hipStreamSynchronize(ctx->stream));
     ctx->ws.release();
+    ctx->blocks.release_all();
     return PLH_OK;
 }
 
@@ -299,14 +325,23 @@ int plh_sum_create(plh_ctx* ctx, int array_size, int64_t num_qubits, int64_t cap
         delete s;
         return rc;
     }
+    ctx->sums.push_back(s);
     *out = s;
     return PLH_OK;
 }
 
 int plh_sum_destroy(plh_sum* s) {
     if (!s) return PLH_OK;
-    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
-    sum_free_storage(s);
+    if (s->ctx) {
+        auto& v = s->ctx->sums;
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i] == s) {
+                v[i] = v.back();
+                v.pop_back();
+                break;
+            }
+    }
+    sum_free_storage(s);  // back to the context's cache, stream-ordered
     delete s;
     return PLH_OK;
 }
@@ -417,13 +452,7 @@ int plh_multiply(plh_sum* lhs, const plh_sum* rhs) {
         sum_free_storage(&out);
         return rc;
     }
-    PLH_HIP(hipStreamSynchronize(lhs->ctx->stream));
-    sum_free_storage(lhs);
-    lhs->x = out.x;
-    lhs->z = out.z;
-    lhs->f = out.f;
-    lhs->v = out.v;
-    lhs->cap = out.cap;
+    sum_take_storage(lhs, &out);
     lhs->n = n * m;
     lhs->canonical = false;
     return PLH_OK;

@@ -48,6 +48,27 @@ class Workspace {
     std::vector<Buf> bufs_;
 };
 
+// Caching device allocator owned by a context: freed blocks are kept and
+// reused for later requests of up to twice their size.  All work of a
+// context is ordered on its one stream, so a block released by one call can
+// be handed to the next call without a synchronisation.
+class BlockCache {
+   public:
+    ~BlockCache() { release_all(); }
+    int alloc(size_t bytes, void** out);
+    void free(void* p);
+    void release_all();
+    size_t bytes_cached() const;
+
+   private:
+    struct Blk {
+        void* p;
+        size_t n;
+    };
+    std::vector<Blk> free_;
+    std::vector<Blk> used_;
+};
+
 // Small pinned host staging buffer for count read-backs.
 struct HostStage {
     uint64_t* p = nullptr;
@@ -60,6 +81,7 @@ struct HostStage {
 // Device-resident PauliSum<N,double>: plane-major SoA, x[k*cap + t].
 struct plh_sum {
     plh_ctx* ctx = nullptr;
+    void* block = nullptr;  // one allocation holding x, z, v, f
     int as = 1;
     int64_t nq = 64;
     int64_t n = 0;
@@ -86,8 +108,10 @@ struct plh_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     plh::Workspace ws;
+    plh::BlockCache blocks;
     plh::HostStage stage;
     bool prof = false;
+    std::vector<plh_sum*> sums;  // live sums; orphaned (ctx = null) when the context goes
     std::vector<plh::ProfRec> prof_recs;
     std::vector<hipEvent_t> prof_pool;
 };
@@ -131,6 +155,8 @@ enum : int {
 // terms when growing.
 int sum_reserve(plh_sum* s, int64_t cap, int64_t keep);
 void sum_free_storage(plh_sum* s);
+// dst takes src's storage (dst's old storage is released; src is left empty).
+void sum_take_storage(plh_sum* dst, plh_sum* src);
 
 // Device-wide exclusive scan of n u32 values into u64 offsets; the total
 // is written to *d_total (device).  Stream-ordered.

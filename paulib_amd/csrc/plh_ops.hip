@@ -576,14 +576,8 @@ int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, doubl
 #undef PLH_FW
         PLH_HIP(hipGetLastError());
     }
-    PLH_HIP(hipStreamSynchronize(st));
     const bool canon = s->canonical;
-    sum_free_storage(s);
-    s->x = tmp.x;
-    s->z = tmp.z;
-    s->f = tmp.f;
-    s->v = tmp.v;
-    s->cap = tmp.cap;
+    sum_take_storage(s, &tmp);
     s->n = (int64_t)k;
     s->canonical = canon;  // a filter of a canonical sum stays canonical
     return PLH_OK;

@@ -149,6 +149,62 @@ size_t Workspace::bytes_held() const {
     return t;
 }
 
+int BlockCache::alloc(size_t bytes, void** out) {
+    if (bytes == 0) bytes = 256;
+    size_t best = (size_t)-1;
+    for (size_t i = 0; i < free_.size(); ++i) {
+        const size_t n = free_[i].n;
+        if (n >= bytes && n <= 2 * bytes + (1u << 20) && (best == (size_t)-1 || n < free_[best].n)) best = i;
+    }
+    if (best != (size_t)-1) {
+        used_.push_back(free_[best]);
+        *out = free_[best].p;
+        free_.erase(free_.begin() + (long)best);
+        return PLH_OK;
+    }
+    const size_t rounded = (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, rounded);
+    if (e != hipSuccess) {
+        // Give the cached blocks back and retry once.
+        (void)hipGetLastError();
+        for (auto& b : free_) (void)hipFree(b.p);
+        free_.clear();
+        e = hipMalloc(&p, rounded);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+            return PLH_ERR_OOM;
+        }
+    }
+    used_.push_back(Blk{p, rounded});
+    *out = p;
+    return PLH_OK;
+}
+
+void BlockCache::free(void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < used_.size(); ++i) {
+        if (used_[i].p == p) {
+            free_.push_back(used_[i]);
+            used_.erase(used_.begin() + (long)i);
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+void BlockCache::release_all() {
+    for (auto& b : free_) (void)hipFree(b.p);
+    free_.clear();
+}
+
+size_t BlockCache::bytes_cached() const {
+    size_t t = 0;
+    for (auto& b : free_) t += b.n;
+    return t;
+}
+
 static hipEvent_t take_event(plh_ctx* c) {
     if (!c->prof_pool.empty()) {
         hipEvent_t e = c->prof_pool.back();

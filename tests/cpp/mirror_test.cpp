@@ -1,0 +1,157 @@
+// C++ drop-in check: a host sum with PauliSum<N,double>'s public fields goes
+// through include/paulib_hip.hpp (the GPU engine) and through the oracle's C
+// ABI; the results must match bit for bit.  Built by paulib_amd/build.py,
+// run by tests/test_gpu_cpp_mirror.py on the GPU box.
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "../../include/paulib_hip.hpp"
+
+extern "C" {
+void* orc_sum_new(int as, int64_t nq);
+void orc_sum_free(void*);
+int64_t orc_sum_len(const void*);
+int orc_sum_set(void*, int64_t, const uint64_t*, const uint64_t*, const uint8_t*, const double*);
+void orc_sum_get(const void*, uint64_t*, uint64_t*, uint8_t*, double*);
+int orc_multiply(void*, const void*);
+int orc_canonicalize(void*);
+int orc_apply_H(void*, int64_t);
+int orc_apply_CNOT(void*, int64_t, int64_t);
+int orc_evolve(void*, const void*);
+}
+
+// The public fields of paulib::PauliSum<N,double> (include/PauliSumCore.h:20-45).
+template <std::size_t N>
+struct HostPauliSum {
+    static constexpr std::size_t ArraySize = (N + 63) / 64;
+    std::array<std::vector<std::size_t>, ArraySize> x_data;
+    std::array<std::vector<std::size_t>, ArraySize> z_data;
+    std::vector<uint8_t> flags;
+    std::vector<double> value;
+    std::size_t num_qubits_ = N;
+    std::size_t num_qubits() const { return num_qubits_; }
+    std::size_t get_length() const { return value.size(); }
+};
+
+template <std::size_t N>
+HostPauliSum<N> random_sum(std::size_t n, unsigned seed) {
+    std::mt19937_64 rng(seed);
+    HostPauliSum<N> s;
+    for (std::size_t k = 0; k < s.ArraySize; ++k) {
+        s.x_data[k].resize(n);
+        s.z_data[k].resize(n);
+        for (std::size_t t = 0; t < n; ++t) {
+            s.x_data[k][t] = rng() & rng() & rng();
+            s.z_data[k][t] = rng() & rng() & rng();
+        }
+    }
+    for (std::size_t t = 0; t < n; ++t) {
+        s.flags.push_back((uint8_t)(rng() & 3));
+        s.value.push_back((t % 2 ? -1.0 : 1.0) * (0.5 + (double)t));
+    }
+    return s;
+}
+
+template <std::size_t N>
+void* to_oracle(const HostPauliSum<N>& h) {
+    const int64_t n = (int64_t)h.get_length();
+    constexpr int AS = (int)HostPauliSum<N>::ArraySize;
+    std::vector<uint64_t> x((std::size_t)AS * n), z((std::size_t)AS * n);
+    for (int k = 0; k < AS; ++k)
+        for (int64_t t = 0; t < n; ++t) {
+            x[(std::size_t)(k * n + t)] = h.x_data[k][(std::size_t)t];
+            z[(std::size_t)(k * n + t)] = h.z_data[k][(std::size_t)t];
+        }
+    void* o = orc_sum_new(AS, (int64_t)h.num_qubits());
+    orc_sum_set(o, n, x.data(), z.data(), h.flags.data(), h.value.data());
+    return o;
+}
+
+template <std::size_t N>
+bool same(const HostPauliSum<N>& g, void* o, const char* what) {
+    constexpr int AS = (int)HostPauliSum<N>::ArraySize;
+    const int64_t n = orc_sum_len(o);
+    if ((int64_t)g.get_length() != n) {
+        std::printf("FAIL %s: length %zu vs %lld\n", what, g.get_length(), (long long)n);
+        return false;
+    }
+    std::vector<uint64_t> x((std::size_t)AS * n), z((std::size_t)AS * n);
+    std::vector<uint8_t> f((std::size_t)n);
+    std::vector<double> v((std::size_t)n);
+    orc_sum_get(o, x.data(), z.data(), f.data(), v.data());
+    for (int64_t t = 0; t < n; ++t) {
+        for (int k = 0; k < AS; ++k)
+            if (g.x_data[k][(std::size_t)t] != x[(std::size_t)(k * n + t)] ||
+                g.z_data[k][(std::size_t)t] != z[(std::size_t)(k * n + t)]) {
+                std::printf("FAIL %s: string at %lld\n", what, (long long)t);
+                return false;
+            }
+        if (g.flags[(std::size_t)t] != f[(std::size_t)t] || std::memcmp(&g.value[(std::size_t)t], &v[(std::size_t)t], 8)) {
+            std::printf("FAIL %s: phase/value at %lld\n", what, (long long)t);
+            return false;
+        }
+    }
+    std::printf("ok   %s (%lld terms)\n", what, (long long)n);
+    return true;
+}
+
+template <std::size_t N>
+bool run_all() {
+    bool ok = true;
+    auto A = random_sum<N>(300, 1), B = random_sum<N>(200, 2);
+    // product + canonicalize through the device sum API
+    paulib_hip::DevicePauliSum<N> dA(A), dB(B);
+    auto prod = dA.multiply_canonical(dB).template download<HostPauliSum<N>>();
+    void* oa = to_oracle(A);
+    void* ob = to_oracle(B);
+    orc_multiply(oa, ob);
+    void* raw = to_oracle(A);
+    orc_multiply(raw, ob);
+    orc_canonicalize(oa);
+    ok &= same(prod, oa, "multiply_canonical");
+    // uncombined product in place on a host sum (the one-line drop-in)
+    auto H = A;
+    paulib_hip::multiply_on_gpu<N>(H, B);
+    ok &= same(H, raw, "multiply_on_gpu");
+    // canonicalize + Clifford + evolve on a device sum
+    paulib_hip::DevicePauliSum<N> d(A);
+    d.apply_H(3);
+    d.apply_CNOT(1, N - 1);
+    HostPauliSum<N> rot;
+    for (std::size_t k = 0; k < rot.ArraySize; ++k) {
+        rot.x_data[k] = {k == 0 ? 0x5ull : 0ull};
+        rot.z_data[k] = {k == 0 ? 0x6ull : 0ull};
+    }
+    rot.flags = {0};
+    rot.value = {0.7};
+    d.evolve(rot);
+    d.canonicalize();
+    void* oc = to_oracle(A);
+    orc_apply_H(oc, 3);
+    orc_apply_CNOT(oc, 1, (int64_t)N - 1);
+    void* orot = to_oracle(rot);
+    orc_evolve(oc, orot);
+    orc_canonicalize(oc);
+    ok &= same(d.template download<HostPauliSum<N>>(), oc, "H/CNOT/evolve/canonicalize");
+    for (void* p : {oa, ob, raw, oc, orot}) orc_sum_free(p);
+    return ok;
+}
+
+int main() {
+    bool ok = run_all<64>() && run_all<128>() && run_all<500>();
+    // errors surface as std::runtime_error like the reference's PAULIB_GPU_CK
+    try {
+        paulib_hip::DevicePauliSum<64> e;
+        e.apply_H(64);
+        std::printf("FAIL no exception for an out-of-range qubit\n");
+        ok = false;
+    } catch (const std::runtime_error& ex) {
+        std::printf("ok   exception: %s\n", ex.what());
+    }
+    std::printf(ok ? "ALL OK\n" : "FAILED\n");
+    return ok ? 0 : 1;
+}
