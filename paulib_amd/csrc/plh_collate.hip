@@ -57,8 +57,8 @@ template <int AS>
 struct LeafDims {
     // ids per leaf; sized so keys + values of a leaf fit ~56 KB of LDS
     static constexpr int kCap = AS == 1 ? 2048 : AS == 2 ? 1024 : AS <= 4 ? 512 : AS <= 8 ? 256 : 128;
-    static constexpr int kDl = AS == 1 ? 10 : AS == 2 ? 9 : AS <= 4 ? 8 : AS <= 8 ? 7 : 6;
-    static constexpr int kBins = 1 << kDl;  // counting-sort bins (kCap / 2)
+    static constexpr int kDl = AS == 1 ? 11 : AS == 2 ? 10 : AS <= 4 ? 9 : AS <= 8 ? 8 : 7;
+    static constexpr int kBins = 1 << kDl;  // counting-sort bins (= kCap)
     static constexpr int kPer = (kCap + kLeafThreads - 1) / kLeafThreads;
 };
 
@@ -398,11 +398,10 @@ struct LeafLds {
     uint64_t key[2 * AS][D::kCap];
     double val[D::kCap];               // folded coefficients; run sums at head slots
     uint16_t perm[D::kCap];            // sorted position -> slot
+    uint16_t perm2[D::kCap];           // bin order -> (key, id) order; later rslot
     uint8_t fl[D::kCap];               // raw flags
-    uint32_t bin[D::kBins + 1];        // counting sort; later u16 scratch (rank tmp, rslot)
+    uint32_t bin[D::kBins + 1];        // counting-sort bin starts
     unsigned long long diff[2 * AS + 1];
-    uint32_t bigbin[D::kCap / 33 + 2][2];
-    uint32_t nbigbin;
     int pos;
     uint32_t red[kLeafWaves];
 };
@@ -558,7 +557,6 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (int w = 0; w < 2 * AS + 1 && pos < 0; ++w)
             if (L.diff[w]) pos = 64 * w + __clzll(L.diff[w]);
         L.pos = pos;
-        L.nbigbin = 0;
     }
     for (int b = tid; b < D::kBins; b += kLeafThreads) L.bin[b] = 0;
     __syncthreads();
@@ -568,6 +566,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (uint32_t p = tid; p < n; p += kLeafThreads) L.perm[p] = (uint16_t)p;
         __syncthreads();
     } else {
+        // counting sort on the kDl bits where the keys first differ ...
         uint32_t dig[kPer], slot[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
@@ -607,40 +606,24 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             if (p < n) L.perm[L.bin[dig[k]] + slot[k]] = (uint16_t)p;
         }
         __syncthreads();
-        for (int b = tid; b < D::kBins; b += kLeafThreads) {
-            const uint32_t s0 = L.bin[b], s1 = L.bin[b + 1];
-            const uint32_t sz = s1 - s0;
-            if (sz >= 2 && sz <= 32) {
-                for (uint32_t p = s0 + 1; p < s1; ++p) {
-                    const uint16_t cur = L.perm[p];
-                    uint32_t q = p;
-                    while (q > s0 && slot_less<AS>(L, c.fmask, ids, cur, L.perm[q - 1])) {
-                        L.perm[q] = L.perm[q - 1];
-                        --q;
-                    }
-                    L.perm[q] = cur;
-                }
-            } else if (sz > 32) {
-                const uint32_t k = atomicAdd(&L.nbigbin, 1u);
-                L.bigbin[k][0] = s0;
-                L.bigbin[k][1] = sz;
+        // ... then each item's rank inside its bin on the full (key, id):
+        // O(bin size) per lane, every lane busy.
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t p = tid + k * kLeafThreads;
+            if (p < n) {
+                const uint32_t b = dig[k];
+                const uint32_t s0 = L.bin[b], s1 = L.bin[b + 1];
+                const int me = (int)p;
+                uint32_t r = s0;
+                if (s1 - s0 > 1)
+                    for (uint32_t q = s0; q < s1; ++q) r += slot_less<AS>(L, c.fmask, ids, L.perm[q], me) ? 1u : 0u;
+                L.perm2[r] = (uint16_t)me;
             }
         }
         __syncthreads();
-        const uint32_t nbb = L.nbigbin;
-        uint16_t* tmp = reinterpret_cast<uint16_t*>(L.bin);
-        for (uint32_t bb = 0; bb < nbb; ++bb) {
-            const uint32_t s0 = L.bigbin[bb][0], sz = L.bigbin[bb][1];
-            for (uint32_t p = tid; p < sz; p += kLeafThreads) {
-                const uint16_t me = L.perm[s0 + p];
-                uint32_t r = 0;
-                for (uint32_t q = 0; q < sz; ++q) r += slot_less<AS>(L, c.fmask, ids, L.perm[s0 + q], me) ? 1u : 0u;
-                tmp[r] = me;
-            }
-            __syncthreads();
-            for (uint32_t p = tid; p < sz; p += kLeafThreads) L.perm[s0 + p] = tmp[p];
-            __syncthreads();
-        }
+        for (uint32_t p = tid; p < n; p += kLeafThreads) L.perm[p] = L.perm2[p];
+        __syncthreads();
     }
     // ---- runs: thread t owns sorted positions [t*kPer, t*kPer + kPer)
     double sums[kPer];
@@ -673,7 +656,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     uint32_t total;
     const uint32_t r0 = block_exclusive_scan<kLeafWaves>(nk, total, L.red);
     // the scan's barriers order the sums above before these writes
-    uint16_t* rslot = reinterpret_cast<uint16_t*>(L.bin);
+    uint16_t* rslot = L.perm2;
     for (uint32_t k = 0; k < nk; ++k) {
         rslot[r0 + k] = heads[k];
         L.val[heads[k]] = sums[k];
