@@ -153,6 +153,47 @@ int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t
                     const int64_t* q1, const uint64_t* px, const uint64_t* pz,
                     const double* theta, int64_t max_weight, double cutoff, int64_t max_terms);
 
+/* ---- Batched propagation server (continuous batching).
+ * No reference counterpart: the reference runs one propagate() per call
+ * (include/PauliPropagation.h:53-71).  A server keeps every admitted job's
+ * terms in one device-resident sum tagged by job slot; each plh_server_step
+ * advances every running job by its Clifford gates up to and including its
+ * next rotation (evolve + apply_truncation with the job's own max_weight and
+ * cutoff, exactly as plh_run_program), in a fixed number of launches for all
+ * jobs together.  Jobs are admitted at step boundaries while a slot is free
+ * and the batch fits the HBM budget; a finished job's result is its
+ * zero_state_expectation (include/PauliSumUtils.h:162-177) and, with
+ * keep_terms, its final sum in canonical order.  A job's result does not
+ * depend on which other jobs share its steps. */
+typedef struct plh_server plh_server;
+/* array_size 1 or 2 (up to 128 qubits); max_jobs slots; hbm_budget_bytes <= 0
+ * takes half the device's free memory. */
+int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_budget_bytes,
+                      int keep_terms, plh_server** out);
+int plh_server_destroy(plh_server* s);
+/* Queue a job: observable (n terms, planes x[k*n+t] for k < ceil(nq/64),
+ * flags, values; its (x, z, imag) keys must be distinct, as after
+ * canonicalize) and a program as for plh_run_program.  max_weight < 0 and
+ * cutoff <= 0 disable those truncations. */
+int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_t* z,
+                      const uint8_t* flags, const double* values, int64_t num_qubits, int64_t nops,
+                      const int32_t* kind, const int64_t* q0, const int64_t* q1,
+                      const uint64_t* px, const uint64_t* pz, const double* theta,
+                      int64_t max_weight, double cutoff, int64_t* handle);
+/* One step: admit, advance every running job, finish jobs whose program is
+ * done.  Returns the running and queued job counts. */
+int plh_server_step(plh_server* s, int* running, int* queued);
+/* Finished jobs in completion order: handle, expectation, final term count,
+ * latency from submission (seconds). */
+int plh_server_poll(plh_server* s, int max_results, int64_t* handles, double* expectation,
+                    int64_t* terms, double* latency_s, int* count);
+/* Final terms of a finished job (keep_terms servers): planes for the job's
+ * own array size, `terms` entries each (as reported by poll). */
+int plh_server_take_terms(plh_server* s, int64_t handle, uint64_t* x, uint64_t* z,
+                          uint8_t* flags, double* values);
+int64_t plh_server_batch_terms(const plh_server* s);
+int64_t plh_server_steps(const plh_server* s);
+
 #ifdef __cplusplus
 }
 #endif

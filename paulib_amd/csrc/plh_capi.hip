@@ -33,6 +33,7 @@ void sum_free_storage(plh_sum* s) {
     s->x = s->z = nullptr;
     s->f = nullptr;
     s->v = nullptr;
+    s->j = nullptr;
     s->cap = 0;
 }
 
@@ -43,16 +44,21 @@ void sum_take_storage(plh_sum* dst, plh_sum* src) {
     dst->z = src->z;
     dst->f = src->f;
     dst->v = src->v;
+    dst->j = src->j;
     dst->cap = src->cap;
     src->block = nullptr;
     src->x = src->z = nullptr;
     src->f = nullptr;
     src->v = nullptr;
+    src->j = nullptr;
     src->cap = 0;
 }
 
-// Layout of a sum's block: x planes, z planes, values, flags.
-static size_t sum_bytes(int as, int64_t cap) { return (size_t)cap * (16 * (size_t)as + 9); }
+// Layout of a sum's block: x planes, z planes, values, job ids (batched
+// sums), flags.
+static size_t sum_bytes(int as, int64_t cap, bool job) {
+    return (size_t)cap * (16 * (size_t)as + 9 + (job ? 4 : 0));
+}
 
 int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
     if (cap <= s->cap && s->block) return PLH_OK;
@@ -60,9 +66,9 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
     if (s->cap > 0 && nc < s->cap + s->cap / 2 && keep > 0) nc = s->cap + s->cap / 2;  // amortise appends
     void* blk = nullptr;
     if (s->ctx) {
-        PLH_TRY(s->ctx->blocks.alloc(sum_bytes(s->as, nc), &blk));
+        PLH_TRY(s->ctx->blocks.alloc(sum_bytes(s->as, nc, s->has_job), &blk));
     } else {
-        hipError_t e = hipMalloc(&blk, sum_bytes(s->as, nc));
+        hipError_t e = hipMalloc(&blk, sum_bytes(s->as, nc, s->has_job));
         if (e != hipSuccess) {
             (void)hipGetLastError();
             set_error(std::string("device allocation failed: ") + hipGetErrorString(e));
@@ -73,7 +79,8 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
     uint64_t* nx = (uint64_t*)b;
     uint64_t* nz = nx + (size_t)s->as * nc;
     double* nv = (double*)(nz + (size_t)s->as * nc);
-    uint8_t* nf = (uint8_t*)(nv + nc);
+    uint32_t* nj = s->has_job ? (uint32_t*)(nv + nc) : nullptr;
+    uint8_t* nf = s->has_job ? (uint8_t*)(nj + nc) : (uint8_t*)(nv + nc);
     hipStream_t st = s->ctx ? s->ctx->stream : nullptr;
     if (keep > 0 && s->block) {
         for (int k = 0; k < s->as; ++k) {
@@ -84,9 +91,11 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
         }
         PLH_HIP(hipMemcpyAsync(nf, s->f, (size_t)keep, hipMemcpyDeviceToDevice, st));
         PLH_HIP(hipMemcpyAsync(nv, s->v, (size_t)keep * 8, hipMemcpyDeviceToDevice, st));
+        if (s->has_job && s->j) PLH_HIP(hipMemcpyAsync(nj, s->j, (size_t)keep * 4, hipMemcpyDeviceToDevice, st));
     }
     sum_free_storage(s);  // stream-ordered reuse: no synchronisation needed
     s->block = blk;
+    s->j = nj;
     s->x = nx;
     s->z = nz;
     s->f = nf;

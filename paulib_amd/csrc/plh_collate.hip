@@ -86,6 +86,10 @@ struct Cfg {
     uint8_t* of;
     double* ov;
     int64_t ostride;
+    uint32_t* oj;              // output job plane (batched sums) or null
+    const double* job_eps;     // per-job cutoff (batched sums) or null
+    const int64_t* job_maxw;   // per-job max weight (< 0: none) or null
+    const uint8_t* job_alive;  // per-job keep mask or null
 };
 
 PLH_DEV int next_live(const uint64_t* live, int kb, int d) {
@@ -392,29 +396,31 @@ __global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64
 }
 
 // --------------------------------------------------------------- leaves
-template <int AS>
+template <int AS, int KW>
 struct LeafLds {
     using D = LeafDims<AS>;
-    uint64_t key[2 * AS][D::kCap];
+    uint64_t key[KW][D::kCap];         // key words (job word first when present, then x, z)
     double val[D::kCap];               // folded coefficients; run sums at head slots
     uint16_t perm[D::kCap];            // sorted position -> slot
     uint16_t perm2[D::kCap];           // bin order -> (key, id) order; later rslot
+    uint16_t dig[D::kCap];             // counting-sort digit of each slot
+    uint32_t sec[D::kCap];             // the 32 key bits after the digit (cheap first compare)
     uint8_t fl[D::kCap];               // raw flags
     uint32_t bin[D::kBins + 1];        // counting-sort bin starts
-    unsigned long long diff[2 * AS + 1];
+    unsigned long long diff[KW + 1];
     int pos;
     uint32_t red[kLeafWaves];
 };
 
-template <int AS>
-PLH_DEV uint64_t lds_word(const LeafLds<AS>& L, uint32_t fmask, int i, int w) {
-    if (w < 2 * AS) return L.key[w][i];
-    if (w == 2 * AS) return (uint64_t)(L.fl[i] & fmask) << 56;
+template <int AS, int KW>
+PLH_DEV uint64_t lds_word(const LeafLds<AS, KW>& L, uint32_t fmask, int i, int w) {
+    if (w < KW) return L.key[w][i];
+    if (w == KW) return (uint64_t)(L.fl[i] & fmask) << 56;
     return 0ull;
 }
 
-template <int DB, int AS>
-PLH_DEV uint32_t lds_window(const LeafLds<AS>& L, uint32_t fmask, int i, int pos) {
+template <int DB, int AS, int KW>
+PLH_DEV uint32_t lds_window(const LeafLds<AS, KW>& L, uint32_t fmask, int i, int pos) {
     const int w = pos >> 6, o = pos & 63;
     uint64_t hi = lds_word(L, fmask, i, w) << o;
     if (o + DB > 64) hi |= lds_word(L, fmask, i, w + 1) >> (64 - o);
@@ -423,10 +429,10 @@ PLH_DEV uint32_t lds_window(const LeafLds<AS>& L, uint32_t fmask, int i, int pos
 
 // (key, masked flags, id) order; ids are re-read from global memory only on
 // a full key tie.
-template <int AS>
-PLH_DEV bool slot_less(const LeafLds<AS>& L, uint32_t fmask, const uint32_t* ids, int a, int b) {
+template <int AS, int KW>
+PLH_DEV bool slot_less(const LeafLds<AS, KW>& L, uint32_t fmask, const uint32_t* ids, int a, int b) {
 #pragma unroll
-    for (int w = 0; w < 2 * AS; ++w) {
+    for (int w = 0; w < KW; ++w) {
         const uint64_t p = L.key[w][a], q = L.key[w][b];
         if (p != q) return p < q;
     }
@@ -435,33 +441,42 @@ PLH_DEV bool slot_less(const LeafLds<AS>& L, uint32_t fmask, const uint32_t* ids
     return ids[a] < ids[b];
 }
 
-template <int AS>
-PLH_DEV bool slot_key_eq(const LeafLds<AS>& L, uint32_t fmask, int a, int b) {
+template <int AS, int KW>
+PLH_DEV bool slot_key_eq(const LeafLds<AS, KW>& L, uint32_t fmask, int a, int b) {
 #pragma unroll
-    for (int w = 0; w < 2 * AS; ++w)
+    for (int w = 0; w < KW; ++w)
         if (L.key[w][a] != L.key[w][b]) return false;
     return (L.fl[a] & fmask) == (L.fl[b] & fmask);
 }
 
-template <int AS>
-PLH_DEV bool keep_term(const Cfg& c, const uint64_t* xz, double acc) {
+// Filters of one output term; kw holds the key words (job word first when
+// XO = 1).  Per-job limits and the job-alive mask apply to batched sums.
+template <int AS, int XO>
+PLH_DEV bool keep_term(const Cfg& c, const uint64_t* kw, double acc) {
     if ((c.mode & kDropZero) && acc == 0.0) return false;
+    const uint32_t job = XO ? (uint32_t)kw[0] : 0u;
+    if (XO && c.job_alive && !c.job_alive[job]) return false;
     if (c.mode & kMaxWeight) {
         int64_t wgt = 0;
 #pragma unroll
-        for (int k = 0; k < AS; ++k) wgt += popc(xz[k] | xz[AS + k]);
-        if (wgt > c.max_weight) return false;
+        for (int k = 0; k < AS; ++k) wgt += popc(kw[XO + k] | kw[XO + AS + k]);
+        const int64_t mw = (XO && c.job_maxw) ? c.job_maxw[job] : c.max_weight;
+        if (mw >= 0 && wgt > mw) return false;
     }
-    if ((c.mode & kCutoff) && !(fabs(acc) >= c.eps)) return false;
+    if (c.mode & kCutoff) {
+        const double eps = (XO && c.job_eps) ? c.job_eps[job] : c.eps;
+        if (eps > 0.0 && !(fabs(acc) >= eps)) return false;  // cutoff() runs only for eps > 0
+    }
     return true;
 }
 
-template <int AS>
-PLH_DEV void write_term(const Cfg& c, uint64_t o, const uint64_t* xz, uint32_t fl, double v) {
+template <int AS, int XO>
+PLH_DEV void write_term(const Cfg& c, uint64_t o, const uint64_t* kw, uint32_t fl, double v) {
     for (int w = 0; w < c.out_as; ++w) {
-        c.ox[(int64_t)w * c.ostride + o] = xz[w];
-        c.oz[(int64_t)w * c.ostride + o] = xz[AS + w];
+        c.ox[(int64_t)w * c.ostride + o] = kw[XO + w];
+        c.oz[(int64_t)w * c.ostride + o] = kw[XO + AS + w];
     }
+    if (XO && c.oj) c.oj[o] = (uint32_t)kw[0];
     c.of[o] = (uint8_t)fl;
     c.ov[o] = v;
 }
@@ -478,7 +493,9 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
                                                        uint32_t* __restrict__ kept_out) {
     using D = LeafDims<AS>;
     constexpr int kPer = D::kPer;
-    __shared__ LeafLds<AS> L;
+    constexpr int KW = Src::kKW;   // key words before the flags word
+    constexpr int XO = Src::kXOff;  // 1 when the job word leads the key
+    __shared__ LeafLds<AS, KW> L;
     const int tid = threadIdx.x;
     const uint64_t e = blockIdx.x;
     const uint64_t start = Lstart[e];
@@ -495,17 +512,17 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
                 const uint32_t id = ids[i];
                 acc += fold_value(c, src.base_value(id), src.raw_flags(id));
             }
-            uint64_t xz[2 * AS];
+            uint64_t xz[KW];
             uint32_t fl;
             src.key(ids[0], xz, fl);
-            const bool keep = keep_term<AS>(c, xz, acc);
-            if (keep) write_term<AS>(c, out_base, xz, fl & c.fmask, acc);
+            const bool keep = keep_term<AS, XO>(c, xz, acc);
+            if (keep) write_term<AS, XO>(c, out_base, xz, fl & c.fmask, acc);
             kept_out[e] = keep ? 1u : 0u;
         }
         return;
     }
     const uint32_t n = (uint32_t)size;
-    if (tid < 2 * AS + 1) L.diff[tid] = 0;
+    if (tid < KW + 1) L.diff[tid] = 0;
     // ---- load: keys, raw flags, folded values (loads of kPer items batched)
     {
         uint32_t idv[kPer];
@@ -518,12 +535,12 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
             if (p < n) {
-                uint64_t xz[2 * AS];
+                uint64_t xz[KW];
                 uint32_t fl;
                 src.key(idv[k], xz, fl);
                 const double v = fold_value(c, src.base_value(idv[k]), fl);
 #pragma unroll
-                for (int w = 0; w < 2 * AS; ++w) L.key[w][p] = xz[w];
+                for (int w = 0; w < KW; ++w) L.key[w][p] = xz[w];
                 L.fl[p] = (uint8_t)fl;
                 L.val[p] = v;
             }
@@ -532,19 +549,19 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     __syncthreads();
     // ---- first bit where the keys differ
     {
-        uint64_t dl[2 * AS + 1];
+        uint64_t dl[KW + 1];
 #pragma unroll
-        for (int w = 0; w < 2 * AS + 1; ++w) dl[w] = 0;
+        for (int w = 0; w < KW + 1; ++w) dl[w] = 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
             if (p < n) {
 #pragma unroll
-                for (int w = 0; w < 2 * AS + 1; ++w) dl[w] |= lds_word(L, c.fmask, p, w) ^ lds_word(L, c.fmask, 0, w);
+                for (int w = 0; w < KW + 1; ++w) dl[w] |= lds_word(L, c.fmask, p, w) ^ lds_word(L, c.fmask, 0, w);
             }
         }
 #pragma unroll
-        for (int w = 0; w < 2 * AS + 1; ++w) {
+        for (int w = 0; w < KW + 1; ++w) {
             uint64_t v = dl[w];
 #pragma unroll
             for (int o = kWave / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
@@ -554,7 +571,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     __syncthreads();
     if (tid == 0) {
         int pos = -1;
-        for (int w = 0; w < 2 * AS + 1 && pos < 0; ++w)
+        for (int w = 0; w < KW + 1 && pos < 0; ++w)
             if (L.diff[w]) pos = 64 * w + __clzll(L.diff[w]);
         L.pos = pos;
     }
@@ -574,7 +591,9 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             dig[k] = 0;
             slot[k] = 0;
             if (p < n) {
-                dig[k] = lds_window<D::kDl, AS>(L, c.fmask, (int)p, pos);
+                dig[k] = lds_window<D::kDl, AS, KW>(L, c.fmask, (int)p, pos);
+                L.dig[p] = (uint16_t)dig[k];
+                L.sec[p] = lds_window<32, AS, KW>(L, c.fmask, (int)p, pos + D::kDl);
                 slot[k] = atomicAdd(&L.bin[dig[k]], 1u);
             }
         }
@@ -606,18 +625,26 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             if (p < n) L.perm[L.bin[dig[k]] + slot[k]] = (uint16_t)p;
         }
         __syncthreads();
-        // ... then each item's rank inside its bin on the full (key, id):
-        // O(bin size) per lane, every lane busy.
+        // ... then each item's rank inside its bin on the full (key, id).
+        // Work is assigned by bin-ordered position, so the lanes of a wave
+        // mostly share a bin; the 32 bits after the digit decide most
+        // comparisons before the full key is read.
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            const uint32_t p = tid + k * kLeafThreads;
-            if (p < n) {
-                const uint32_t b = dig[k];
+            const uint32_t q = tid + k * kLeafThreads;
+            if (q < n) {
+                const int me = L.perm[q];
+                const uint32_t b = L.dig[me];
                 const uint32_t s0 = L.bin[b], s1 = L.bin[b + 1];
-                const int me = (int)p;
                 uint32_t r = s0;
-                if (s1 - s0 > 1)
-                    for (uint32_t q = s0; q < s1; ++q) r += slot_less<AS>(L, c.fmask, ids, L.perm[q], me) ? 1u : 0u;
+                if (s1 - s0 > 1) {
+                    const uint32_t ms = L.sec[me];
+                    for (uint32_t t = s0; t < s1; ++t) {
+                        const int o = L.perm[t];
+                        const uint32_t os = L.sec[o];
+                        r += (os < ms || (os == ms && slot_less<AS, KW>(L, c.fmask, ids, o, me))) ? 1u : 0u;
+                    }
+                }
                 L.perm2[r] = (uint16_t)me;
             }
         }
@@ -635,18 +662,18 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (int k = 0; k < kPer; ++k) {
             const uint32_t q = q0 + k;
             if (q >= n) break;
-            if (q > 0 && slot_key_eq<AS>(L, c.fmask, L.perm[q], L.perm[q - 1])) continue;
+            if (q > 0 && slot_key_eq<AS, KW>(L, c.fmask, L.perm[q], L.perm[q - 1])) continue;
             double acc = 0.0;
             uint32_t r = q;
             do {
                 acc += L.val[L.perm[r]];
                 ++r;
-            } while (r < n && slot_key_eq<AS>(L, c.fmask, L.perm[r], L.perm[q]));
-            uint64_t xz[2 * AS];
+            } while (r < n && slot_key_eq<AS, KW>(L, c.fmask, L.perm[r], L.perm[q]));
+            uint64_t xz[KW];
             const int s = L.perm[q];
 #pragma unroll
-            for (int w = 0; w < 2 * AS; ++w) xz[w] = L.key[w][s];
-            if (keep_term<AS>(c, xz, acc)) {
+            for (int w = 0; w < KW; ++w) xz[w] = L.key[w][s];
+            if (keep_term<AS, XO>(c, xz, acc)) {
                 sums[nk] = acc;
                 heads[nk] = (uint16_t)s;
                 ++nk;
@@ -666,9 +693,10 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         const int s = rslot[r];
         const uint64_t o = out_base + r;
         for (int w = 0; w < c.out_as; ++w) {
-            c.ox[(int64_t)w * c.ostride + o] = L.key[w][s];
-            c.oz[(int64_t)w * c.ostride + o] = L.key[AS + w][s];
+            c.ox[(int64_t)w * c.ostride + o] = L.key[XO + w][s];
+            c.oz[(int64_t)w * c.ostride + o] = L.key[XO + AS + w][s];
         }
+        if (XO && c.oj) c.oj[o] = (uint32_t)L.key[0][s];
         c.of[o] = (uint8_t)(L.fl[s] & c.fmask);
         c.ov[o] = L.val[s];
     }
@@ -684,7 +712,8 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
                                                     const uint8_t* __restrict__ sf, const double* __restrict__ sv,
                                                     int64_t sstride, uint64_t* __restrict__ dx,
                                                     uint64_t* __restrict__ dz, uint8_t* __restrict__ df,
-                                                    double* __restrict__ dv, int64_t dstride) {
+                                                    double* __restrict__ dv, int64_t dstride,
+                                                    const uint32_t* __restrict__ sj, uint32_t* __restrict__ dj) {
     const uint64_t e = blockIdx.x;
     const uint64_t s0 = Lstart[e], d0 = offs[e];
     const uint32_t k = kept[e];
@@ -695,6 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
         }
         df[d0 + i] = sf[s0 + i];
         dv[d0 + i] = sv[s0 + i];
+        if (dj) dj[d0 + i] = sj[s0 + i];
     }
 }
 
@@ -720,10 +750,13 @@ inline unsigned grid1(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock)
 
 template <int AS, class Src>
 int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* h_live,
-                uint32_t mode, int64_t max_weight, double eps, plh_sum* out, int out_as) {
+                const CollateSpec& spec, plh_sum* out, int out_as) {
+    const uint32_t mode = spec.mode;
+    const int64_t max_weight = spec.max_weight;
+    const double eps = spec.eps;
     using D = LeafDims<AS>;
     hipStream_t st = ctx->stream;
-    constexpr int kWordsK = 2 * AS + 1;
+    constexpr int kWordsK = Src::kKW + 1;
     const int kb = 64 * kWordsK;
     const uint32_t cap = (uint32_t)D::kCap;
     out->n = 0;
@@ -756,6 +789,9 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     cfg.max_weight = max_weight;
     cfg.eps = eps;
     cfg.out_as = out_as;
+    cfg.job_eps = spec.job_eps;
+    cfg.job_maxw = spec.job_maxw;
+    cfg.job_alive = spec.job_alive;
 
     // ---- pass 0: implicit ids, one root bucket.
     int pos = next_live_host(h_live, kb, 0);
@@ -892,6 +928,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     cfg.oz = out->z;
     cfg.of = out->f;
     cfg.ov = out->v;
+    cfg.oj = out->j;
     cfg.ostride = out->cap;
     PLH_TRY(ctx->ws.get(kWsKept, L * 4 + 64, &p));
     uint32_t* kept = (uint32_t*)p;
@@ -912,16 +949,46 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
         tmp.ctx = ctx;
         tmp.as = out->as;
         tmp.nq = out->nq;
+        tmp.has_job = out->has_job;
         PLH_TRY(sum_reserve(&tmp, (int64_t)total, 0));
         if (total) {
             ProfScope ps(ctx, "compact", (double)total * 2.0 * (16.0 * out_as + 9.0));
             k_compact<<<dim3((unsigned)L), kBlock, 0, st>>>(A.start, kept, offs, out_as, out->x, out->z, out->f,
-                                                            out->v, out->cap, tmp.x, tmp.z, tmp.f, tmp.v, tmp.cap);
+                                                            out->v, out->cap, tmp.x, tmp.z, tmp.f, tmp.v, tmp.cap,
+                                                            out->j, tmp.j);
             PLH_HIP(hipGetLastError());
         }
         sum_take_storage(out, &tmp);
     }
     out->n = (int64_t)total;
+    return PLH_OK;
+}
+
+// Pack a sum into records of 2*AS + 2 words (x[AS], z[AS], value, flags);
+// words beyond the sum's own array size are zero.
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_pack(SumView s, int as, int64_t n, uint64_t* __restrict__ rec) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    uint64_t* r = rec + (size_t)t * (2 * AS + 2);
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        r[k] = k < as ? s.x[(int64_t)k * s.stride + t] : 0ull;
+        r[AS + k] = k < as ? s.z[(int64_t)k * s.stride + t] : 0ull;
+    }
+    r[2 * AS] = (uint64_t)__double_as_longlong(s.v[t]);
+    r[2 * AS + 1] = s.f[t];
+}
+
+template <int AS>
+int pack_sum(plh_ctx* ctx, const plh_sum* in, int slot, const uint64_t** out) {
+    void* p;
+    PLH_TRY(ctx->ws.get(slot, (size_t)(in->n ? in->n : 1) * (2 * AS + 2) * 8, &p));
+    if (in->n)
+        k_pack<AS><<<grid1((uint64_t)in->n), kBlock, 0, ctx->stream>>>(SumView{in->x, in->z, in->f, in->v, in->cap},
+                                                                        in->as, in->n, (uint64_t*)p);
+    PLH_HIP(hipGetLastError());
+    *out = (const uint64_t*)p;
     return PLH_OK;
 }
 
@@ -934,23 +1001,25 @@ int padded_as(int as) {
     return -1;
 }
 
-template <int AS>
+template <int AS, bool JOB>
 int collate_terms_as(plh_ctx* ctx, const plh_sum* in, const CollateSpec& spec, plh_sum* out) {
-    TermSource<AS> src;
+    TermSource<AS, JOB> src;
     src.s = SumView{in->x, in->z, in->f, in->v, in->cap};
+    src.j = in->j;
     src.as = in->as;
     src.mode = spec.mode;
     src.n = (uint32_t)in->n;
-    uint64_t orw[32];
+    uint64_t orw[40];
     PLH_TRY(op_or_words(ctx, in, orw));
-    uint64_t live[kMaxKeyWords] = {0};
+    constexpr int XO = TermSource<AS, JOB>::kXOff;
+    uint64_t live[kMaxKeyWords + 1] = {0};
+    if (JOB) live[0] = orw[2 * in->as];
     for (int k = 0; k < in->as; ++k) {
-        live[k] = orw[k];
-        live[AS + k] = orw[in->as + k];
+        live[XO + k] = orw[k];
+        live[XO + AS + k] = orw[in->as + k];
     }
-    live[2 * AS] = (spec.mode & kFold) ? (1ull << 56) : (3ull << 56);
-    return collate_run<AS>(ctx, src, (uint64_t)in->n, live, spec.mode, spec.max_weight, spec.eps, out,
-                           in->as);
+    live[XO + 2 * AS] = (spec.mode & kFold) ? (1ull << 56) : (3ull << 56);
+    return collate_run<AS>(ctx, src, (uint64_t)in->n, live, spec, out, in->as);
 }
 
 template <int AS, int KIND>
@@ -970,8 +1039,10 @@ int collate_pairs_as(plh_ctx* ctx, const plh_sum* Ls, const plh_sum* Rs, int by_
         return PLH_ERR_SIZE;
     }
     PairSource<AS, KIND> src;
-    src.L = SumView{Ls->x, Ls->z, Ls->f, Ls->v, Ls->cap};
-    src.R = SumView{Rs->x, Rs->z, Rs->f, Rs->v, Rs->cap};
+    PLH_TRY(pack_sum<AS>(ctx, Ls, kWsPackL, &src.L.r));
+    PLH_TRY(pack_sum<AS>(ctx, Rs, kWsPackR, &src.R.r));
+    src.Lp = SumView{Ls->x, Ls->z, Ls->f, Ls->v, Ls->cap};
+    src.Rp = SumView{Rs->x, Rs->z, Rs->f, Rs->v, Rs->cap};
     src.as = Ls->as;
     src.mode = spec.mode;
     src.lo_bits = lo_bits;
@@ -988,8 +1059,7 @@ int collate_pairs_as(plh_ctx* ctx, const plh_sum* Ls, const plh_sum* Rs, int by_
         live[AS + k] = ol[Ls->as + k] | orr[Ls->as + k];
     }
     live[2 * AS] = (spec.mode & kFold) ? (1ull << 56) : (3ull << 56);
-    return collate_run<AS>(ctx, src, n_lo * n_hi, live, spec.mode, spec.max_weight, spec.eps, out,
-                           Ls->as);
+    return collate_run<AS>(ctx, src, n_lo * n_hi, live, spec, out, Ls->as);
 }
 
 }  // namespace
@@ -1007,18 +1077,28 @@ int collate_terms(plh_ctx* ctx, plh_sum* in, const CollateSpec& spec, plh_sum* o
         tmp.ctx = ctx;
         tmp.as = in->as;
         tmp.nq = in->nq;
+        tmp.has_job = in->has_job;
         dst = &tmp;
     }
     dst->as = in->as;
     dst->nq = in->nq;
+    dst->has_job = in->has_job;
     int rc;
-    switch (padded_as(in->as)) {
-        case 1: rc = collate_terms_as<1>(ctx, in, spec, dst); break;
-        case 2: rc = collate_terms_as<2>(ctx, in, spec, dst); break;
-        case 4: rc = collate_terms_as<4>(ctx, in, spec, dst); break;
-        case 8: rc = collate_terms_as<8>(ctx, in, spec, dst); break;
-        case 16: rc = collate_terms_as<16>(ctx, in, spec, dst); break;
-        default: set_error("unsupported array_size"); return PLH_ERR_AS;
+    if (in->has_job) {
+        switch (padded_as(in->as)) {
+            case 1: rc = collate_terms_as<1, true>(ctx, in, spec, dst); break;
+            case 2: rc = collate_terms_as<2, true>(ctx, in, spec, dst); break;
+            default: set_error("batched sums support up to 128 qubits"); return PLH_ERR_AS;
+        }
+    } else {
+        switch (padded_as(in->as)) {
+            case 1: rc = collate_terms_as<1, false>(ctx, in, spec, dst); break;
+            case 2: rc = collate_terms_as<2, false>(ctx, in, spec, dst); break;
+            case 4: rc = collate_terms_as<4, false>(ctx, in, spec, dst); break;
+            case 8: rc = collate_terms_as<8, false>(ctx, in, spec, dst); break;
+            case 16: rc = collate_terms_as<16, false>(ctx, in, spec, dst); break;
+            default: set_error("unsupported array_size"); return PLH_ERR_AS;
+        }
     }
     if (rc != PLH_OK) {
         if (alias) sum_free_storage(&tmp);

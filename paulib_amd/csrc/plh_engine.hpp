@@ -90,6 +90,8 @@ struct plh_sum {
     uint64_t* z = nullptr;
     uint8_t* f = nullptr;
     double* v = nullptr;
+    uint32_t* j = nullptr;   // job plane of a batched sum (has_job), else null
+    bool has_job = false;
     bool canonical = false;  // (x, z, imag) keys distinct, flags in {0,1}, no zeros
 };
 
@@ -148,6 +150,8 @@ enum : int {
     kWsMisc,
     kWsFlags,
     kWsTmpSum,
+    kWsPackL,
+    kWsPackR,
     kWsCount
 };
 
@@ -173,6 +177,10 @@ struct CollateSpec {
     uint32_t mode = 0;      // kFold | kDropZero | kMaxWeight | kCutoff
     int64_t max_weight = 0;
     double eps = 0.0;
+    // batched sums: per-job limits and keep mask (device arrays) or null
+    const double* job_eps = nullptr;
+    const int64_t* job_maxw = nullptr;
+    const uint8_t* job_alive = nullptr;
 };
 // out = collate(in); out may alias in (the result replaces it).
 int collate_terms(plh_ctx* ctx, plh_sum* in, const CollateSpec& spec, plh_sum* out);
@@ -192,7 +200,7 @@ int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz
                 double cos_t, double sin_t, int* appended_general);
 int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, double eps);
 int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out);
-int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or /* 2*as words */);
+int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or /* 2*as words (+ job plane) */);
 int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src);
 
 }  // namespace plh

@@ -305,7 +305,8 @@ __global__ __launch_bounds__(kBlock) void k_filter_write(int as, SumView s, int6
                                                          const uint64_t* __restrict__ boff,
                                                          uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
                                                          uint8_t* __restrict__ of, double* __restrict__ ov,
-                                                         int64_t os) {
+                                                         int64_t os, const uint32_t* __restrict__ sj,
+                                                         uint32_t* __restrict__ oj) {
     __shared__ uint32_t red[kWavesPerBlock];
     const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
     uint32_t mask = 0;
@@ -325,6 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_write(int as, SumView s, int6
         }
         of[o] = s.f[t];
         ov[o] = s.v[t];
+        if (oj) oj[o] = sj[t];
         ++o;
     }
 }
@@ -362,13 +364,19 @@ __global__ __launch_bounds__(kBlock) void k_zse_final(const double* __restrict__
 }
 
 __global__ __launch_bounds__(kBlock) void k_or_words(const uint64_t* __restrict__ x,
-                                                     const uint64_t* __restrict__ z, int64_t stride,
+                                                     const uint64_t* __restrict__ z,
+                                                     const uint32_t* __restrict__ j, int64_t stride,
                                                      int as, int64_t n, unsigned long long* __restrict__ out) {
-    const int w = blockIdx.y;  // 0..2*as-1
-    const uint64_t* p = (w < as ? x : z) + (int64_t)(w < as ? w : w - as) * stride;
+    const int w = blockIdx.y;  // 0..2*as-1: planes; 2*as: job plane
     uint64_t acc = 0;
-    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
-        acc |= p[t];
+    if (w < 2 * as) {
+        const uint64_t* p = (w < as ? x : z) + (int64_t)(w < as ? w : w - as) * stride;
+        for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
+            acc |= p[t];
+    } else {
+        for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
+            acc |= j[t];
+    }
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, kWave);
     if (lane_id() == 0 && acc) atomicOr(&out[w], (unsigned long long)acc);
@@ -564,13 +572,14 @@ int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, doubl
     tmp.ctx = ctx;
     tmp.as = s->as;
     tmp.nq = s->nq;
+    tmp.has_job = s->has_job;
     PLH_TRY(sum_reserve(&tmp, (int64_t)k, 0));
     if (k) {
 #define PLH_FW(ASV)                                                                                     \
     case ASV:                                                                                           \
         k_filter_write<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, v, s->n, mode, max_weight, eps, \
                                                                   off, tmp.x, tmp.z, tmp.f, tmp.v,      \
-                                                                  tmp.cap);                             \
+                                                                  tmp.cap, s->j, tmp.j);                \
         break;
         switch (pa) { PLH_FW(1) PLH_FW(2) PLH_FW(4) PLH_FW(8) PLH_FW(16) }
 #undef PLH_FW
@@ -607,7 +616,7 @@ int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out) {
 
 int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or) {
     hipStream_t st = ctx->stream;
-    const int nw = 2 * s->as;
+    const int nw = 2 * s->as + (s->has_job ? 1 : 0);
     for (int w = 0; w < nw; ++w) h_or[w] = 0;
     if (s->n == 0) return PLH_OK;
     void* p;
@@ -615,7 +624,7 @@ int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or) {
     unsigned long long* d = (unsigned long long*)p;
     PLH_HIP(hipMemsetAsync(d, 0, (size_t)nw * 8, st));
     const unsigned gx = (unsigned)std::min<uint64_t>(nblk((uint64_t)s->n, kBlock), 1024);
-    k_or_words<<<dim3(gx, (unsigned)nw), kBlock, 0, st>>>(s->x, s->z, s->cap, s->as, s->n, d);
+    k_or_words<<<dim3(gx, (unsigned)nw), kBlock, 0, st>>>(s->x, s->z, s->j, s->cap, s->as, s->n, d);
     PLH_HIP(hipGetLastError());
     if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
     PLH_HIP(hipMemcpyAsync(ctx->stage.p, d, (size_t)nw * 8, hipMemcpyDeviceToHost, st));
@@ -637,6 +646,8 @@ int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src) {
     }
     PLH_HIP(hipMemcpyAsync(dst->f + n0, src->f, (size_t)src->n, hipMemcpyDeviceToDevice, st));
     PLH_HIP(hipMemcpyAsync(dst->v + n0, src->v, (size_t)src->n * 8, hipMemcpyDeviceToDevice, st));
+    if (dst->has_job && src->has_job)
+        PLH_HIP(hipMemcpyAsync(dst->j + n0, src->j, (size_t)src->n * 4, hipMemcpyDeviceToDevice, st));
     dst->n = n0 + src->n;
     dst->canonical = false;
     return PLH_OK;

@@ -1,0 +1,757 @@
+// plh_serve.hip -- batched propagation: many jobs in one device-resident sum.
+//
+// A batched sum is a plh_sum with a job plane: term t belongs to job slot
+// j[t].  The slot is the leading key word of the collation, so one
+// collation per step keeps every job's terms contiguous and ordered exactly
+// as that job alone would be, under its own cutoff and weight limit.  One
+// step advances every job in flight by one rotation (and the Clifford gates
+// before it, in Heisenberg order) with a fixed number of launches, so the
+// per-gate launch and synchronisation cost of the plain path is paid once
+// per step for all jobs instead of once per job.
+//
+// Per job the semantics are those of a Clifford + rotation program run by
+// propagate() (include/PauliPropagation.h:53-63) with the Clifford methods
+// (include/PauliSumClifford.h:43-81, 212-239): Clifford gates conjugate in
+// place; a rotation is evolve (include/PauliSumOperations.h:106-183) followed
+// by apply_truncation (include/PauliPropagation.h:35-47), fused into one
+// collation.  The fusion is exact while the job's (x, z, imag) keys are
+// distinct, which submission checks and every step preserves.  Steps with
+// Clifford gates only (the program's leading gates) still fold and order
+// the job's terms but apply no truncation, as propagate() would not.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "plh_block.hpp"
+#include "plh_engine.hpp"
+#include "plh_sources.hpp"
+
+namespace plh {
+
+namespace {
+
+constexpr int kIpt = 16;
+constexpr uint64_t kTile = (uint64_t)kBlock * kIpt;
+
+// Per-step job tables (device, one upload per step), one entry per slot.
+struct StepTables {
+    const uint32_t* cl_off;   // [slots + 1] range of the slot's Clifford ops
+    const uint32_t* cl_op;    // kind << 28 | q0 << 14 | q1 (kind 0 H, 1 S, 2 CNOT)
+    const uint8_t* rot_kind;  // 0 none, 1 sign flip, 2 flip + multiply, 3 multiply, 4 general
+    const uint8_t* rot_psf;   // flags of ps = P with the i bit flipped
+    const double* rot_c;
+    const double* rot_s;
+    const uint64_t* rot_p;    // per slot: x[AS] then z[AS]
+    double* eps;              // collation filters of this step
+    int64_t* maxw;
+    uint8_t* alive;
+};
+
+// One Clifford conjugation of a term held in registers: the update of the
+// plain k_clifford kernel (H, S, CNOT of include/PauliSumClifford.h).  Words
+// are selected by compare so the term stays in registers.
+template <int AS>
+PLH_DEV void apply_clifford_op(uint32_t op, uint64_t* x, uint64_t* z, uint32_t& f) {
+    const uint32_t kind = op >> 28, q0 = (op >> 14) & 0x3FFFu, q1 = op & 0x3FFFu;
+    const int wc = (int)(q0 >> 6);
+    const uint64_t mc = 1ull << (q0 & 63);
+    uint64_t xw = 0, zw = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k)
+        if (k == wc) { xw = x[k]; zw = z[k]; }
+    const uint32_t xc = (xw & mc) != 0, zc = (zw & mc) != 0;
+    if (kind == 0) {  // H: X <-> Z, Y -> -Y
+        f ^= (xc & zc) << 1;
+        const uint64_t d = (xw ^ zw) & mc;
+#pragma unroll
+        for (int k = 0; k < AS; ++k)
+            if (k == wc) { x[k] = xw ^ d; z[k] = zw ^ d; }
+    } else if (kind == 1) {  // S
+        f ^= (xc & zc) << 1;
+#pragma unroll
+        for (int k = 0; k < AS; ++k)
+            if (k == wc) z[k] = zw ^ (xw & mc);
+    } else {  // CNOT(q0 -> q1)
+        const int wt = (int)(q1 >> 6);
+        const uint64_t mt = 1ull << (q1 & 63);
+        uint64_t xtw = 0, ztw = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k)
+            if (k == wt) { xtw = x[k]; ztw = z[k]; }
+        const uint64_t xt = (xtw & mt) != 0, zt = (ztw & mt) != 0;
+        f ^= (uint32_t)(((uint64_t)xc & zt & (1ull ^ xt ^ (uint64_t)zc)) << 1);
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            if (k == wt) x[k] ^= (0ull - (uint64_t)xc) & mt;
+            if (k == wc) z[k] ^= (0ull - zt) & mc;
+        }
+    }
+}
+
+template <int AS>
+PLH_DEV bool anticommutes_p(const uint64_t* x, const uint64_t* z, const uint64_t* p) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) acc ^= (x[k] & p[AS + k]) ^ (z[k] & p[k]);
+    return popc(acc) & 1;
+}
+
+// term * ps in registers (PauliSum::multiply(i, ps), include/PauliSumOperations.h:185-209).
+template <int AS>
+PLH_DEV uint32_t times_p(uint64_t* x, uint64_t* z, uint32_t f, const uint64_t* p, uint32_t psf) {
+    uint64_t c1 = 0, c2 = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        phase_word(x[k], z[k], p[k], p[AS + k], c1, c2);
+        x[k] ^= p[k];
+        z[k] ^= p[AS + k];
+    }
+    return phase_of(f, psf, c1, c2);
+}
+
+// Clifford ops, then the fast-path rotations (:121-147) in place; counts
+// per tile the terms the general path will append.
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_batch_gates(uint64_t* __restrict__ X, uint64_t* __restrict__ Z,
+                                                        uint8_t* __restrict__ F, const uint32_t* __restrict__ J,
+                                                        int64_t stride, int64_t n, StepTables tb,
+                                                        uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint32_t c = 0;
+    for (int k = 0; k < kIpt; ++k) {
+        const uint64_t t = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (t >= (uint64_t)n) break;
+        const uint32_t job = J[t];
+        const uint32_t o0 = tb.cl_off[job], o1 = tb.cl_off[job + 1];
+        const uint32_t kind = tb.rot_kind[job];
+        if (o0 == o1 && kind == 0) continue;
+        uint64_t x[AS], z[AS];
+#pragma unroll
+        for (int w = 0; w < AS; ++w) {
+            x[w] = X[(int64_t)w * stride + t];
+            z[w] = Z[(int64_t)w * stride + t];
+        }
+        uint32_t f = F[t];
+        for (uint32_t o = o0; o < o1; ++o) apply_clifford_op<AS>(tb.cl_op[o], x, z, f);
+        if (kind != 0) {
+            const uint64_t* p = tb.rot_p + (size_t)job * 2 * AS;
+            if (anticommutes_p<AS>(x, z, p)) {
+                if (kind == 4) {
+                    ++c;
+                } else {
+                    if (kind == 1 || kind == 2) f ^= 2u;
+                    if (kind >= 2) f = times_p<AS>(x, z, f, p, tb.rot_psf[job]);
+                }
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < AS; ++w) {
+            X[(int64_t)w * stride + t] = x[w];
+            Z[(int64_t)w * stride + t] = z[w];
+        }
+        F[t] = (uint8_t)f;
+    }
+    const uint32_t s = block_sum(c, red);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = s;
+}
+
+// General path (:148-180): an anticommuting term t of a kind-4 job keeps
+// cos * v in place and term * ps with (-sin) * v is appended at n + rank;
+// ranks follow the tile's (k, thread) order, so the result is deterministic.
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_batch_rot_general(uint64_t* __restrict__ X, uint64_t* __restrict__ Z,
+                                                              uint8_t* __restrict__ F, double* __restrict__ V,
+                                                              uint32_t* __restrict__ J, int64_t stride, int64_t n,
+                                                              StepTables tb, const uint64_t* __restrict__ boff) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint64_t o_base = (uint64_t)n + boff[blockIdx.x];
+    for (int k = 0; k < kIpt; ++k) {
+        const uint64_t t = base + (uint64_t)k * kBlock + threadIdx.x;
+        uint32_t hit = 0, job = 0;
+        uint64_t x[AS], z[AS];
+        if (t < (uint64_t)n) {
+            job = J[t];
+            if (tb.rot_kind[job] == 4) {
+#pragma unroll
+                for (int w = 0; w < AS; ++w) {
+                    x[w] = X[(int64_t)w * stride + t];
+                    z[w] = Z[(int64_t)w * stride + t];
+                }
+                hit = anticommutes_p<AS>(x, z, tb.rot_p + (size_t)job * 2 * AS) ? 1u : 0u;
+            }
+        }
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(hit, tot, red);
+        if (hit) {
+            const uint64_t o = o_base + ex;
+            const double orig = V[t];
+            V[t] = orig * tb.rot_c[job];
+            const uint32_t f = times_p<AS>(x, z, F[t], tb.rot_p + (size_t)job * 2 * AS, tb.rot_psf[job]);
+#pragma unroll
+            for (int w = 0; w < AS; ++w) {
+                X[(int64_t)w * stride + o] = x[w];
+                Z[(int64_t)w * stride + o] = z[w];
+            }
+            F[o] = (uint8_t)f;
+            V[o] = (-tb.rot_s[job]) * orig;
+            J[o] = job;
+        }
+        o_base += tot;
+    }
+}
+
+// <0|O_job|0> (include/PauliSumUtils.h:162-177) for finished jobs of a
+// job-ordered batch: one workgroup per job, fixed-order reduction over the
+// job's contiguous range, whose bounds are also returned.
+__global__ __launch_bounds__(kBlock) void k_batch_zse(int as, SumView s, const uint32_t* __restrict__ J, int64_t n,
+                                                      const uint32_t* __restrict__ jobs, double* __restrict__ out,
+                                                      int64_t* __restrict__ range) {
+    __shared__ double red[kWavesPerBlock];
+    const uint32_t job = jobs[blockIdx.x];
+    int64_t lo = 0, hi = n;  // first index with J >= job
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (J[mid] < job) lo = mid + 1; else hi = mid;
+    }
+    const int64_t a = lo;
+    hi = n;
+    while (lo < hi) {  // first index with J > job
+        const int64_t mid = (lo + hi) >> 1;
+        if (J[mid] <= job) lo = mid + 1; else hi = mid;
+    }
+    const int64_t b = lo;
+    double acc = 0.0;
+    const int64_t per = (b - a + kBlock - 1) / kBlock;
+    const int64_t i0 = a + (int64_t)threadIdx.x * per, i1 = min(b, i0 + per);
+    for (int64_t i = i0; i < i1; ++i) {
+        bool has_xy = false;
+        for (int w = 0; w < as; ++w) has_xy |= s.x[(int64_t)w * s.stride + i] != 0;
+        const uint32_t fl = s.f[i];
+        if (has_xy || (fl & 1u)) continue;
+        acc += (fl & 2u) ? -s.v[i] : s.v[i];
+    }
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        out[blockIdx.x] = tot;
+        range[2 * blockIdx.x] = a;
+        range[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline unsigned blocks_for(uint64_t n, uint64_t per) { return (unsigned)((n + per - 1) / per); }
+
+bool close_one(double v) { return std::fabs(v - 1.0) <= 1e-14; }  // isCloseOne, :19-21
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Job {
+    int64_t handle = -1;
+    int job_as = 1;
+    int64_t n0 = 0;
+    std::vector<uint64_t> ox, oz;  // observable planes, x[k*n0 + t]
+    std::vector<uint8_t> of;
+    std::vector<double> ov;
+    std::vector<int32_t> kind;
+    std::vector<int64_t> q0, q1;
+    std::vector<uint64_t> px, pz;  // per op: job_as words
+    std::vector<double> theta;
+    int64_t next = -1;  // next op to apply (Heisenberg order: last to first)
+    int64_t max_weight = -1;
+    double cutoff = 0.0;
+    double t_submit = 0.0;
+};
+
+struct Result {
+    int64_t handle = -1;
+    double expectation = 0.0;
+    int64_t terms = 0;
+    double latency_s = 0.0;
+    std::vector<uint64_t> x, z;
+    std::vector<uint8_t> f;
+    std::vector<double> v;
+};
+
+}  // namespace
+}  // namespace plh
+
+struct plh_server {
+    plh_ctx* ctx = nullptr;
+    int as = 2;
+    int slots = 0;
+    int64_t budget = 0;
+    bool keep_terms = false;
+    plh_sum* batch = nullptr;  // job-plane sum holding every job in flight
+    std::deque<plh::Job> queue;
+    std::vector<plh::Job> active;  // by slot
+    std::vector<int> state;        // 0 free, 1 running, 2 draining (its terms leave at the next collation)
+    std::deque<plh::Result> done;
+    std::unordered_map<int64_t, plh::Result> kept;
+    int64_t next_handle = 0;
+    void* d_step = nullptr;
+    size_t step_cap = 0;
+    std::vector<uint8_t> h_step;
+    std::chrono::steady_clock::time_point t0;
+    int64_t steps = 0;
+};
+
+namespace plh {
+namespace {
+
+double now_s(const plh_server* s) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+// Device bytes per term in flight: the batch (x, z, v, j, f), the collation
+// output, its id arrays and packed key records.
+double bytes_per_term(int as) { return 2.0 * (16.0 * as + 13.0) + 8.0 + (2.0 * as + 2.0) * 8.0; }
+
+bool keys_distinct(const Job& j) {
+    const int a = j.job_as;
+    std::vector<std::vector<uint64_t>> keys((size_t)j.n0);
+    for (int64_t t = 0; t < j.n0; ++t) {
+        auto& k = keys[(size_t)t];
+        for (int w = 0; w < a; ++w) k.push_back(j.ox[(size_t)(w * j.n0 + t)]);
+        for (int w = 0; w < a; ++w) k.push_back(j.oz[(size_t)(w * j.n0 + t)]);
+        k.push_back(j.of[(size_t)t] & 1u);
+    }
+    std::sort(keys.begin(), keys.end());
+    return std::adjacent_find(keys.begin(), keys.end()) == keys.end();
+}
+
+// Append a job's observable to the batch under its slot.
+int admit(plh_server* s, const Job& j, int slot) {
+    const int64_t n = j.n0;
+    if (n == 0) return PLH_OK;
+    plh_sum tmp;
+    tmp.ctx = s->ctx;
+    tmp.as = s->as;
+    tmp.nq = 64 * s->as;
+    tmp.has_job = true;
+    PLH_TRY(sum_reserve(&tmp, n, 0));
+    hipStream_t st = s->ctx->stream;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < s->as; ++k) {
+        uint64_t* dx = tmp.x + (int64_t)k * tmp.cap;
+        uint64_t* dz = tmp.z + (int64_t)k * tmp.cap;
+        if (k < j.job_as) {
+            if (e == hipSuccess) e = hipMemcpyAsync(dx, j.ox.data() + (size_t)k * n, (size_t)n * 8, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(dz, j.oz.data() + (size_t)k * n, (size_t)n * 8, hipMemcpyHostToDevice, st);
+        } else {
+            if (e == hipSuccess) e = hipMemsetAsync(dx, 0, (size_t)n * 8, st);
+            if (e == hipSuccess) e = hipMemsetAsync(dz, 0, (size_t)n * 8, st);
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp.f, j.of.data(), (size_t)n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp.v, j.ov.data(), (size_t)n * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        k_fill_u32<<<blocks_for((uint64_t)n, 256), 256, 0, st>>>(tmp.j, n, (uint32_t)slot);
+        e = hipGetLastError();
+    }
+    int rc = PLH_OK;
+    if (e != hipSuccess) {
+        set_error(std::string("HIP: ") + hipGetErrorString(e) + " while admitting a job");
+        rc = PLH_ERR_HIP;
+    } else {
+        tmp.n = n;
+        rc = op_append(s->ctx, s->batch, &tmp);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && rc == PLH_OK) rc = PLH_ERR_HIP;  // staging is host memory of j
+    sum_free_storage(&tmp);
+    return rc;
+}
+
+}  // namespace
+}  // namespace plh
+
+using namespace plh;
+
+extern "C" {
+
+int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_budget_bytes, int keep_terms,
+                      plh_server** out) {
+    if (!ctx || !out || max_jobs <= 0 || max_jobs > (1 << 16)) {
+        set_error("plh_server_create: bad context, output or max_jobs (1..65536)");
+        return PLH_ERR_ARG;
+    }
+    if (array_size != 1 && array_size != 2) {
+        set_error("batched propagation supports 1 or 2 words per plane (up to 128 qubits)");
+        return PLH_ERR_AS;
+    }
+    PLH_HIP(hipSetDevice(ctx->device));
+    size_t free_b = 0, total_b = 0;
+    PLH_HIP(hipMemGetInfo(&free_b, &total_b));
+    plh_server* s = new plh_server();
+    s->ctx = ctx;
+    s->as = array_size;
+    s->slots = max_jobs;
+    s->keep_terms = keep_terms != 0;
+    s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)(free_b / 2);
+    s->active.resize((size_t)max_jobs);
+    s->state.assign((size_t)max_jobs, 0);
+    int rc = plh_sum_create(ctx, array_size, 64 * array_size, 0, &s->batch);
+    if (rc == PLH_OK) {
+        s->batch->has_job = true;
+        sum_free_storage(s->batch);
+        rc = sum_reserve(s->batch, 4096, 0);
+    }
+    if (rc != PLH_OK) {
+        if (s->batch) plh_sum_destroy(s->batch);
+        delete s;
+        return rc;
+    }
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return PLH_OK;
+}
+
+int plh_server_destroy(plh_server* s) {
+    if (!s) return PLH_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    plh_sum_destroy(s->batch);
+    (void)hipFree(s->d_step);
+    delete s;
+    return PLH_OK;
+}
+
+int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_t* z, const uint8_t* f,
+                      const double* v, int64_t num_qubits, int64_t nops, const int32_t* kind, const int64_t* q0,
+                      const int64_t* q1, const uint64_t* px, const uint64_t* pz, const double* theta,
+                      int64_t max_weight, double cutoff, int64_t* handle) {
+    if (!s || n < 0 || nops < 0 || !handle || (n > 0 && (!x || !z || !f || !v)) ||
+        (nops > 0 && (!kind || !q0 || !q1 || !px || !pz || !theta))) {
+        set_error("plh_server_submit: null pointer or negative size");
+        return PLH_ERR_ARG;
+    }
+    if (num_qubits < 1 || num_qubits > 64 * s->as) {
+        set_error("plh_server_submit: job qubit count exceeds the server's array size");
+        return PLH_ERR_ARG;
+    }
+    if (n > ((int64_t)1 << 31)) {
+        set_error("plh_server_submit: observable too large for a batched job");
+        return PLH_ERR_SIZE;
+    }
+    for (int64_t l = 0; l < nops; ++l) {
+        const bool clifford = kind[l] >= 0 && kind[l] <= 2;
+        if (kind[l] < 0 || kind[l] > 3 || (clifford && (q0[l] < 0 || q0[l] >= num_qubits)) ||
+            (kind[l] == 2 && (q1[l] < 0 || q1[l] >= num_qubits || q1[l] == q0[l]))) {
+            set_error("plh_server_submit: invalid op in job program");
+            return PLH_ERR_ARG;
+        }
+    }
+    Job j;
+    j.job_as = (int)((num_qubits + 63) / 64);
+    j.n0 = n;
+    const int a = j.job_as;
+    if (n > 0) {
+        j.ox.assign(x, x + (size_t)a * n);
+        j.oz.assign(z, z + (size_t)a * n);
+        j.of.assign(f, f + n);
+        j.ov.assign(v, v + n);
+    }
+    if (nops > 0) {
+        j.kind.assign(kind, kind + nops);
+        j.q0.assign(q0, q0 + nops);
+        j.q1.assign(q1, q1 + nops);
+        j.px.assign(px, px + (size_t)a * nops);
+        j.pz.assign(pz, pz + (size_t)a * nops);
+        j.theta.assign(theta, theta + nops);
+    }
+    if (!keys_distinct(j)) {
+        set_error("plh_server_submit: observable has repeated (x, z, imag) keys; canonicalize it first");
+        return PLH_ERR_ARG;
+    }
+    j.next = nops - 1;
+    j.max_weight = max_weight;
+    j.cutoff = cutoff;
+    j.handle = s->next_handle++;
+    j.t_submit = now_s(s);
+    *handle = j.handle;
+    s->queue.push_back(std::move(j));
+    return PLH_OK;
+}
+
+int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
+    if (!s) return PLH_ERR_ARG;
+    plh_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const int AS = s->as, S = s->slots;
+    plh_sum* B = s->batch;
+
+    // ---- admission: a free slot, and the batch with the new job fits the HBM budget
+    const double bpt = bytes_per_term(AS);
+    int running = 0;
+    for (int i = 0; i < S; ++i) running += s->state[(size_t)i] == 1;
+    int64_t in_flight = B->n;
+    while (!s->queue.empty()) {
+        int slot = -1;
+        for (int i = 0; i < S && slot < 0; ++i)
+            if (s->state[(size_t)i] == 0) slot = i;
+        if (slot < 0) break;
+        const Job& q = s->queue.front();
+        // room for the new job's terms to grow 64-fold
+        const double need = ((double)in_flight + 64.0 * (double)std::max<int64_t>(q.n0, 1)) * bpt;
+        if (running > 0 && need > (double)s->budget) break;
+        PLH_TRY(admit(s, q, slot));
+        in_flight += q.n0;
+        s->active[(size_t)slot] = std::move(s->queue.front());
+        s->queue.pop_front();
+        s->state[(size_t)slot] = 1;
+        ++running;
+    }
+
+    // ---- this step's tables: per running job, the Cliffords up to its next rotation and that rotation
+    std::vector<uint32_t> off((size_t)S + 1, 0), ops;
+    std::vector<uint8_t> rk((size_t)S, 0), rpsf((size_t)S, 1), alive((size_t)S, 0);
+    std::vector<double> rc((size_t)S, 0.0), rs((size_t)S, 0.0), eps((size_t)S, 0.0);
+    std::vector<int64_t> maxw((size_t)S, -1);
+    std::vector<uint64_t> rp((size_t)S * 2 * AS, 0);
+    std::vector<int> finishing;
+    for (int i = 0; i < S; ++i) {
+        off[(size_t)i] = (uint32_t)ops.size();
+        if (s->state[(size_t)i] != 1) continue;
+        alive[(size_t)i] = 1;
+        Job& j = s->active[(size_t)i];
+        while (j.next >= 0 && j.kind[(size_t)j.next] != 3) {
+            const int64_t l = j.next--;
+            const uint32_t k = (uint32_t)j.kind[(size_t)l];
+            ops.push_back((k << 28) | ((uint32_t)j.q0[(size_t)l] << 14) | (k == 2 ? (uint32_t)j.q1[(size_t)l] : 0u));
+        }
+        if (j.next >= 0) {
+            const int64_t l = j.next--;
+            const double c = std::cos(j.theta[(size_t)l]), sn = std::sin(j.theta[(size_t)l]);
+            uint8_t kind = 4;
+            if (close_one(c)) kind = 0;
+            else if (close_one(-c)) kind = 1;
+            else if (close_one(sn)) kind = 2;
+            else if (close_one(-sn)) kind = 3;
+            rk[(size_t)i] = kind;  // program generators carry flags 0, so ps has flags 1
+            rc[(size_t)i] = c;
+            rs[(size_t)i] = sn;
+            for (int k = 0; k < j.job_as; ++k) {
+                rp[(size_t)i * 2 * AS + k] = j.px[(size_t)(l * j.job_as + k)];
+                rp[(size_t)i * 2 * AS + AS + k] = j.pz[(size_t)(l * j.job_as + k)];
+            }
+            maxw[(size_t)i] = j.max_weight;  // apply_truncation follows every rotation
+            eps[(size_t)i] = j.cutoff;
+        }
+        if (j.next < 0) finishing.push_back(i);
+    }
+    off[(size_t)S] = (uint32_t)ops.size();
+
+    struct Part {
+        const void* src;
+        size_t bytes;
+    };
+    const Part parts[10] = {{off.data(), off.size() * 4}, {ops.data(), ops.size() * 4},
+                            {rk.data(), rk.size()},       {rpsf.data(), rpsf.size()},
+                            {rc.data(), rc.size() * 8},   {rs.data(), rs.size() * 8},
+                            {rp.data(), rp.size() * 8},   {eps.data(), eps.size() * 8},
+                            {maxw.data(), maxw.size() * 8}, {alive.data(), alive.size()}};
+    size_t total = 0;
+    for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
+    PLH_HIP(hipStreamSynchronize(st));  // the previous step's upload has been consumed
+    if (total > s->step_cap) {
+        PLH_HIP(hipFree(s->d_step));
+        s->d_step = nullptr;
+        s->step_cap = 0;
+        PLH_HIP(hipMalloc(&s->d_step, total * 2));
+        s->step_cap = total * 2;
+    }
+    s->h_step.assign(total, 0);
+    void* dptr[10];
+    size_t o = 0;
+    for (int i = 0; i < 10; ++i) {
+        if (parts[i].bytes) std::memcpy(s->h_step.data() + o, parts[i].src, parts[i].bytes);
+        dptr[i] = (char*)s->d_step + o;
+        o += align256(std::max<size_t>(parts[i].bytes, 1));
+    }
+    StepTables tb;
+    tb.cl_off = (const uint32_t*)dptr[0];
+    tb.cl_op = (const uint32_t*)dptr[1];
+    tb.rot_kind = (const uint8_t*)dptr[2];
+    tb.rot_psf = (const uint8_t*)dptr[3];
+    tb.rot_c = (const double*)dptr[4];
+    tb.rot_s = (const double*)dptr[5];
+    tb.rot_p = (const uint64_t*)dptr[6];
+    tb.eps = (double*)dptr[7];
+    tb.maxw = (int64_t*)dptr[8];
+    tb.alive = (uint8_t*)dptr[9];
+    PLH_HIP(hipMemcpyAsync(s->d_step, s->h_step.data(), total, hipMemcpyHostToDevice, st));
+
+    const int64_t n = B->n;
+    if (n > 0) {
+        // ---- gates: Cliffords + fast rotations in place; count the general appends
+        const uint64_t nb = blocks_for((uint64_t)n, kTile);
+        void* p;
+        PLH_TRY(ctx->ws.get(kWsScan0, nb * 4 + 64, &p));
+        uint32_t* cnt = (uint32_t*)p;
+        PLH_TRY(ctx->ws.get(kWsScan1, nb * 8 + 64, &p));
+        uint64_t* boff = (uint64_t*)p;
+        PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+        uint64_t* d_total = (uint64_t*)p + 40;
+        {
+            ProfScope ps(ctx, "batch_gates", (double)n * (2.0 * (16.0 * AS + 1.0) + 4.0));
+            if (AS == 1)
+                k_batch_gates<1><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb, cnt);
+            else
+                k_batch_gates<2><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb, cnt);
+            PLH_HIP(hipGetLastError());
+        }
+        PLH_TRY(scan_u32_to_u64(ctx, cnt, boff, nb, d_total));
+        uint64_t k = 0;
+        PLH_TRY(read_u64(ctx, d_total, &k));
+        if (k > 0) {
+            PLH_TRY(sum_reserve(B, n + (int64_t)k, n));
+            ProfScope ps(ctx, "batch_rot_general", (double)n * (16.0 * AS + 4.0) + (double)k * (16.0 * AS + 21.0));
+            if (AS == 1)
+                k_batch_rot_general<1><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->v, B->j, B->cap,
+                                                                             n, tb, boff);
+            else
+                k_batch_rot_general<2><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->v, B->j, B->cap,
+                                                                             n, tb, boff);
+            PLH_HIP(hipGetLastError());
+            B->n = n + (int64_t)k;
+        }
+        // ---- one collation for every job: fold, combine, zero-drop, per-job filters, drained jobs out
+        CollateSpec sc;
+        sc.mode = kFold | kDropZero | kMaxWeight | kCutoff;
+        sc.max_weight = -1;
+        sc.eps = 0.0;
+        sc.job_eps = tb.eps;
+        sc.job_maxw = tb.maxw;
+        sc.job_alive = tb.alive;
+        PLH_TRY(collate_terms(ctx, B, sc, B));
+    }
+    for (int i = 0; i < S; ++i)
+        if (s->state[(size_t)i] == 2) s->state[(size_t)i] = 0;  // their terms left in this collation
+
+    // ---- finished jobs: expectation on the job-ordered batch; their slots drain next step
+    if (!finishing.empty()) {
+        const size_t F = finishing.size();
+        std::vector<double> he(F, 0.0);
+        std::vector<int64_t> hr(2 * F, 0);
+        if (B->n > 0) {
+            void* p;
+            PLH_TRY(ctx->ws.get(kWsTmpSum, align256(F * 4) + F * 24 + 256, &p));
+            uint32_t* d_jobs = (uint32_t*)p;
+            double* d_e = (double*)((char*)p + align256(F * 4));
+            int64_t* d_rng = (int64_t*)(d_e + F);
+            std::vector<uint32_t> hj(F);
+            for (size_t i = 0; i < F; ++i) hj[i] = (uint32_t)finishing[i];
+            PLH_HIP(hipMemcpyAsync(d_jobs, hj.data(), F * 4, hipMemcpyHostToDevice, st));
+            k_batch_zse<<<dim3((unsigned)F), kBlock, 0, st>>>(B->as, SumView{B->x, B->z, B->f, B->v, B->cap}, B->j,
+                                                               B->n, d_jobs, d_e, d_rng);
+            PLH_HIP(hipGetLastError());
+            PLH_HIP(hipMemcpyAsync(he.data(), d_e, F * 8, hipMemcpyDeviceToHost, st));
+            PLH_HIP(hipMemcpyAsync(hr.data(), d_rng, F * 16, hipMemcpyDeviceToHost, st));
+            PLH_HIP(hipStreamSynchronize(st));
+        }
+        const double t = now_s(s);
+        for (size_t i = 0; i < F; ++i) {
+            const int slot = finishing[i];
+            Job& j = s->active[(size_t)slot];
+            Result r;
+            r.handle = j.handle;
+            r.expectation = he[i];
+            r.terms = hr[2 * i + 1] - hr[2 * i];
+            r.latency_s = t - j.t_submit;
+            if (s->keep_terms) {
+                const int64_t a = hr[2 * i], m = r.terms;
+                r.x.resize((size_t)j.job_as * m);
+                r.z.resize((size_t)j.job_as * m);
+                r.f.resize((size_t)m);
+                r.v.resize((size_t)m);
+                if (m > 0) {
+                    for (int k = 0; k < j.job_as; ++k) {
+                        PLH_HIP(hipMemcpyAsync(r.x.data() + (size_t)k * m, B->x + (int64_t)k * B->cap + a,
+                                               (size_t)m * 8, hipMemcpyDeviceToHost, st));
+                        PLH_HIP(hipMemcpyAsync(r.z.data() + (size_t)k * m, B->z + (int64_t)k * B->cap + a,
+                                               (size_t)m * 8, hipMemcpyDeviceToHost, st));
+                    }
+                    PLH_HIP(hipMemcpyAsync(r.f.data(), B->f + a, (size_t)m, hipMemcpyDeviceToHost, st));
+                    PLH_HIP(hipMemcpyAsync(r.v.data(), B->v + a, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+                    PLH_HIP(hipStreamSynchronize(st));
+                }
+            }
+            const int64_t h = r.handle;
+            if (s->keep_terms) {
+                Result meta;
+                meta.handle = h;
+                meta.expectation = r.expectation;
+                meta.terms = r.terms;
+                meta.latency_s = r.latency_s;
+                s->done.push_back(std::move(meta));
+                s->kept[h] = std::move(r);
+            } else {
+                s->done.push_back(std::move(r));
+            }
+            s->state[(size_t)slot] = 2;
+            s->active[(size_t)slot] = Job();
+        }
+    }
+    ++s->steps;
+    int act = 0;
+    for (int i = 0; i < S; ++i) act += s->state[(size_t)i] == 1;
+    if (act == 0) {
+        // nothing running: drained terms can go without a collation
+        for (int i = 0; i < S; ++i) s->state[(size_t)i] = 0;
+        B->n = 0;
+    }
+    if (active_out) *active_out = act;
+    if (queued_out) *queued_out = (int)s->queue.size();
+    return PLH_OK;
+}
+
+int plh_server_poll(plh_server* s, int max_results, int64_t* handles, double* expectation, int64_t* terms,
+                    double* latency_s, int* count) {
+    if (!s || !count || max_results < 0) return PLH_ERR_ARG;
+    int n = 0;
+    while (n < max_results && !s->done.empty()) {
+        const Result& r = s->done.front();
+        if (handles) handles[n] = r.handle;
+        if (expectation) expectation[n] = r.expectation;
+        if (terms) terms[n] = r.terms;
+        if (latency_s) latency_s[n] = r.latency_s;
+        ++n;
+        s->done.pop_front();
+    }
+    *count = n;
+    return PLH_OK;
+}
+
+int plh_server_take_terms(plh_server* s, int64_t handle, uint64_t* x, uint64_t* z, uint8_t* f, double* v) {
+    if (!s) return PLH_ERR_ARG;
+    auto it = s->kept.find(handle);
+    if (it == s->kept.end()) {
+        set_error("plh_server_take_terms: no kept terms for this handle (server made without keep_terms, "
+                  "job not finished, or terms already taken)");
+        return PLH_ERR_ARG;
+    }
+    const Result& r = it->second;
+    if (x && !r.x.empty()) std::memcpy(x, r.x.data(), r.x.size() * 8);
+    if (z && !r.z.empty()) std::memcpy(z, r.z.data(), r.z.size() * 8);
+    if (f && !r.f.empty()) std::memcpy(f, r.f.data(), r.f.size());
+    if (v && !r.v.empty()) std::memcpy(v, r.v.data(), r.v.size() * 8);
+    s->kept.erase(it);
+    return PLH_OK;
+}
+
+int64_t plh_server_batch_terms(const plh_server* s) { return s && s->batch ? s->batch->n : -1; }
+
+int64_t plh_server_steps(const plh_server* s) { return s ? s->steps : -1; }
+
+}  // extern "C"

@@ -34,13 +34,19 @@ struct SumView {  // plane-major SoA: x[k*stride + t]
     int64_t stride;
 };
 
-template <int AS>
+// JOB = true: a batched sum whose terms carry a job id (plane j[t]); the job
+// id is the leading key word, so one collation keeps every job's terms
+// contiguous and sorted exactly as that job alone would be.
+template <int AS, bool JOB = false>
 struct TermSource {
-    static constexpr int kWords = 2 * AS + 1;
+    static constexpr int kAS = AS;
+    static constexpr int kXOff = JOB ? 1 : 0;    // index of x[0] in the key
+    static constexpr int kKW = 2 * AS + kXOff;  // key words before the flags word
     SumView s;
-    int as;           // real words per plane (<= AS)
-    uint32_t mode;    // kFold ...
-    uint32_t n;       // items
+    const uint32_t* j;  // job plane (JOB only)
+    int as;             // real words per plane (<= AS)
+    uint32_t mode;      // kFold ...
+    uint32_t n;         // items
 
     PLH_DEV bool valid(uint32_t) const { return true; }
     PLH_DEV uint32_t dense_to_id(uint32_t t) const { return t; }
@@ -49,23 +55,38 @@ struct TermSource {
         return (mode & kFold) ? (fl & 1u) : fl;
     }
     PLH_DEV uint64_t word(uint32_t id, int w) const {
+        if (JOB) {
+            if (w == 0) return j[id];
+            w -= 1;
+        }
         if (w < AS) return w < as ? s.x[(int64_t)w * s.stride + id] : 0ull;
         if (w < 2 * AS) return (w - AS) < as ? s.z[(int64_t)(w - AS) * s.stride + id] : 0ull;
         if (w == 2 * AS) return (uint64_t)flag(id) << 56;
         return 0ull;
     }
     // Key words and the RAW flags byte (the engine masks the sign bit).
-    PLH_DEV void key(uint32_t id, uint64_t* xz, uint32_t& fl) const {
+    PLH_DEV void key(uint32_t id, uint64_t* kw, uint32_t& fl) const {
+        if (JOB) kw[0] = j[id];
 #pragma unroll
         for (int k = 0; k < AS; ++k) {
-            xz[k] = k < as ? s.x[(int64_t)k * s.stride + id] : 0ull;
-            xz[AS + k] = k < as ? s.z[(int64_t)k * s.stride + id] : 0ull;
+            kw[kXOff + k] = k < as ? s.x[(int64_t)k * s.stride + id] : 0ull;
+            kw[kXOff + AS + k] = k < as ? s.z[(int64_t)k * s.stride + id] : 0ull;
         }
         fl = s.f[id];
     }
     PLH_DEV uint32_t raw_flags(uint32_t id) const { return s.f[id]; }
     // Coefficient before the sign fold.
     PLH_DEV double base_value(uint32_t id) const { return s.v[id]; }
+};
+
+// A term packed as one record of 2*AS + 2 words: x[AS], z[AS], value,
+// flags -- a pair's key and coefficient come from two records (two cache
+// lines) instead of eight plane loads.
+template <int AS>
+struct PackedView {
+    static constexpr int kRec = 2 * AS + 2;
+    const uint64_t* r;
+    PLH_DEV const uint64_t* rec(uint32_t i) const { return r + (size_t)i * kRec; }
 };
 
 // KIND 0: product.  hi = rhs index i, lo = lhs index j, term = lhs[j]*rhs[i],
@@ -76,9 +97,12 @@ struct TermSource {
 //         (2*a.v[i])*b.v[j]  (commutator_impl, :315-346).
 template <int AS, int KIND>
 struct PairSource {
-    static constexpr int kWords = 2 * AS + 1;
-    SumView L;       // left factor of the product
-    SumView R;       // right factor
+    static constexpr int kAS = AS;
+    static constexpr int kXOff = 0;
+    static constexpr int kKW = 2 * AS;
+    PackedView<AS> L;  // left factor of the product (records: key + value)
+    PackedView<AS> R;  // right factor
+    SumView Lp, Rp;    // the same operands as planes (one key word per load)
     int as;
     uint32_t mode;
     uint32_t lo_bits;
@@ -100,46 +124,31 @@ struct PairSource {
         if (KIND == 0) return true;
         uint32_t li, ri;
         split(id, li, ri);
+        const uint64_t* a = L.rec(li);
+        const uint64_t* b = R.rec(ri);
         uint64_t acc = 0;
 #pragma unroll
-        for (int k = 0; k < AS; ++k) {
-            if (k < as) {
-                acc ^= (L.x[(int64_t)k * L.stride + li] & R.z[(int64_t)k * R.stride + ri]) ^
-                       (L.z[(int64_t)k * L.stride + li] & R.x[(int64_t)k * R.stride + ri]);
-            }
-        }
+        for (int k = 0; k < AS; ++k) acc ^= (a[k] & b[AS + k]) ^ (a[AS + k] & b[k]);
         return (popc(acc) & 1) != 0;
     }
-    PLH_DEV uint32_t raw_flags(uint32_t li, uint32_t ri) const {
+    PLH_DEV uint32_t raw_flags_of(const uint64_t* a, const uint64_t* b) const {
         uint64_t c1 = 0, c2 = 0;
 #pragma unroll
-        for (int k = 0; k < AS; ++k) {
-            if (k < as) {
-                phase_word(L.x[(int64_t)k * L.stride + li], L.z[(int64_t)k * L.stride + li],
-                           R.x[(int64_t)k * R.stride + ri], R.z[(int64_t)k * R.stride + ri], c1, c2);
-            }
-        }
-        uint32_t fl = phase_of(L.f[li], R.f[ri], c1, c2);
+        for (int k = 0; k < AS; ++k) phase_word(a[k], a[AS + k], b[k], b[AS + k], c1, c2);
+        uint32_t fl = phase_of((uint32_t)a[2 * AS + 1], (uint32_t)b[2 * AS + 1], c1, c2);
         if (KIND == 1) fl = (fl + by_i) & 3u;
         return fl;
-    }
-    PLH_DEV uint32_t flag(uint32_t id) const {
-        uint32_t li, ri;
-        split(id, li, ri);
-        const uint32_t fl = raw_flags(li, ri);
-        return (mode & kFold) ? (fl & 1u) : fl;
     }
     PLH_DEV uint64_t word(uint32_t id, int w) const {
         uint32_t li, ri;
         split(id, li, ri);
-        if (w < AS)
-            return w < as ? (L.x[(int64_t)w * L.stride + li] ^ R.x[(int64_t)w * R.stride + ri]) : 0ull;
+        if (w < AS) return w < as ? (Lp.x[(int64_t)w * Lp.stride + li] ^ Rp.x[(int64_t)w * Rp.stride + ri]) : 0ull;
         if (w < 2 * AS) {
             const int k = w - AS;
-            return k < as ? (L.z[(int64_t)k * L.stride + li] ^ R.z[(int64_t)k * R.stride + ri]) : 0ull;
+            return k < as ? (Lp.z[(int64_t)k * Lp.stride + li] ^ Rp.z[(int64_t)k * Rp.stride + ri]) : 0ull;
         }
         if (w == 2 * AS) {
-            const uint32_t fl = raw_flags(li, ri);
+            const uint32_t fl = raw_flags_of(L.rec(li), R.rec(ri));
             return (uint64_t)((mode & kFold) ? (fl & 1u) : fl) << 56;
         }
         return 0ull;
@@ -147,35 +156,25 @@ struct PairSource {
     PLH_DEV void key(uint32_t id, uint64_t* xz, uint32_t& fl) const {
         uint32_t li, ri;
         split(id, li, ri);
-        uint64_t c1 = 0, c2 = 0;
+        const uint64_t* a = L.rec(li);
+        const uint64_t* b = R.rec(ri);
 #pragma unroll
-        for (int k = 0; k < AS; ++k) {
-            uint64_t ax = 0, az = 0, bx = 0, bz = 0;
-            if (k < as) {
-                ax = L.x[(int64_t)k * L.stride + li];
-                az = L.z[(int64_t)k * L.stride + li];
-                bx = R.x[(int64_t)k * R.stride + ri];
-                bz = R.z[(int64_t)k * R.stride + ri];
-            }
-            xz[k] = ax ^ bx;
-            xz[AS + k] = az ^ bz;
-            phase_word(ax, az, bx, bz, c1, c2);
-        }
-        uint32_t f = phase_of(L.f[li], R.f[ri], c1, c2);
-        if (KIND == 1) f = (f + by_i) & 3u;
-        fl = f;  // raw flags; the engine masks the sign bit
+        for (int w = 0; w < 2 * AS; ++w) xz[w] = a[w] ^ b[w];
+        fl = raw_flags_of(a, b);  // raw flags; the engine masks the sign bit
     }
     PLH_DEV uint32_t raw_flags(uint32_t id) const {
         uint32_t li, ri;
         split(id, li, ri);
-        return raw_flags(li, ri);
+        return raw_flags_of(L.rec(li), R.rec(ri));
     }
     // Coefficient before the sign fold: lhs.v[j]*rhs.v[i] (product) or
     // (2*a.v[i])*b.v[j] (commutator), one IEEE multiply each.
     PLH_DEV double base_value(uint32_t id) const {
         uint32_t li, ri;
         split(id, li, ri);
-        return (KIND == 0) ? L.v[li] * R.v[ri] : (2.0 * L.v[li]) * R.v[ri];
+        const double va = __longlong_as_double((long long)L.rec(li)[2 * AS]);
+        const double vb = __longlong_as_double((long long)R.rec(ri)[2 * AS]);
+        return (KIND == 0) ? va * vb : (2.0 * va) * vb;
     }
 };
 

@@ -1,0 +1,163 @@
+"""Batched propagation server (plh_server_*) against the oracle and itself.
+
+Each job of the server must give what run_program gives for it alone:
+final strings and phases bit-exact and coefficients bit-exact against the
+oracle's run_program followed by canonicalize (the server keeps every job
+canonical), expectation to 1e-12 of the sum of |coefficients| (the server
+reduces in a different order).  And a job's result must not depend on the
+jobs that share its steps: alone and in a mix the bits are identical.
+"""
+import numpy as np
+import pytest
+
+from helpers import assert_terms_equal, cpu
+from paulib_amd import EngineError, PauliSum, TruncationOptions, run_program
+from paulib_amd import workloads as W
+from paulib_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def _job(rng, nq, depth):
+    nterms = int(rng.integers(1, 4))
+    paulis = []
+    seen = set()
+    while len(paulis) < nterms:
+        w = int(rng.integers(1, 4))
+        qs = tuple(sorted(int(q) for q in rng.choice(nq, size=min(w, nq), replace=False)))
+        ps = tuple("XYZ"[int(rng.integers(0, 3))] for _ in qs)
+        if (qs, ps) in seen:
+            continue
+        seen.add((qs, ps))
+        paulis.append(dict(zip(qs, ps)))
+    coeffs = list(rng.uniform(-1, 1, size=nterms))
+    O = W.observable(nq, paulis, coeffs)
+    prog = W.brickwork_program(nq, depth, int(rng.integers(1 << 30)))
+    mw = None if rng.random() < 0.3 else int(rng.integers(3, 7))
+    cut = float(np.exp(rng.uniform(np.log(1e-5), np.log(1e-3))))
+    return O, prog, TruncationOptions(max_weight=mw, coefficient_cutoff=cut)
+
+
+def _mixed_jobs(count, seed, qubits=(6, 20, 33, 64, 70, 100, 128), depths=(1, 4)):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return [_job(rng, int(rng.choice(qubits)), int(rng.integers(depths[0], depths[1] + 1)))
+            for _ in range(count)]
+
+
+def _oracle(O, prog, opts):
+    o = cpu("oracle", O)
+    o.run_program(prog, opts.max_weight, opts.coefficient_cutoff)
+    e = o.zero_state_expectation()
+    o.canonicalize()
+    return e, o.to_host()
+
+
+def _serve(ctx, jobs, max_jobs=64, stagger=0, budget=0):
+    """Run jobs through one server; with stagger > 0, submit that many jobs,
+    step twice, submit the next group, ... (arrivals while others run)."""
+    res = {}
+    terms = {}
+    with Server(ctx, array_size=2, max_jobs=max_jobs, keep_terms=True, hbm_budget_bytes=budget) as srv:
+        handles = []
+        group = stagger if stagger > 0 else len(jobs)
+        for g in range(0, len(jobs), group):
+            for O, prog, opts in jobs[g:g + group]:
+                handles.append(srv.submit(O, prog, opts))
+            if stagger > 0:
+                srv.step()
+                srv.step()
+                for r in srv.poll():
+                    res[r.handle] = r
+                    terms[r.handle] = srv.take_terms(r)
+        for r in srv.drain():
+            res[r.handle] = r
+            terms[r.handle] = srv.take_terms(r)
+        assert srv.batch_terms == 0
+    return [(res[h], terms[h]) for h in handles]
+
+
+def test_server_matches_oracle(ctx):
+    jobs = _mixed_jobs(14, seed=3)
+    out = _serve(ctx, jobs, max_jobs=6)  # more jobs than slots: some wait in the queue
+    for k, ((O, prog, opts), (r, got)) in enumerate(zip(jobs, out)):
+        e_want, want = _oracle(O, prog, opts)
+        assert r.terms == len(want), f"job {k}"
+        assert_terms_equal(got, want)
+        assert abs(r.expectation - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum()), f"job {k}"
+
+
+def test_server_alone_equals_mix(ctx):
+    jobs = _mixed_jobs(10, seed=8, depths=(2, 5))
+    mixed = _serve(ctx, jobs, max_jobs=32, stagger=3)
+    for k, job in enumerate(jobs):
+        (ra, ta), = _serve(ctx, [job], max_jobs=1)
+        rm, tm = mixed[k]
+        assert np.float64(ra.expectation).view(np.uint64) == np.float64(rm.expectation).view(np.uint64), f"job {k}"
+        assert ra.terms == rm.terms
+        assert_terms_equal(tm, ta)
+
+
+def test_server_matches_gpu_run_program(ctx):
+    jobs = _mixed_jobs(6, seed=21, qubits=(64, 128), depths=(2, 3))
+    out = _serve(ctx, jobs)
+    for (O, prog, opts), (r, got) in zip(jobs, out):
+        d = PauliSum.from_host(ctx, O)
+        run_program(d, prog, opts)
+        e_plain = d.zero_state_expectation()
+        d.canonicalize()
+        want = d.to_host()
+        assert_terms_equal(got, want)
+        assert abs(r.expectation - e_plain) <= 1e-12 * max(1.0, np.abs(want.value).sum())
+
+
+def test_server_budget_admits_one_at_a_time(ctx):
+    jobs = _mixed_jobs(5, seed=5, qubits=(20, 40), depths=(1, 2))
+    with Server(ctx, array_size=1, max_jobs=8, keep_terms=False, hbm_budget_bytes=1) as srv:
+        for O, prog, opts in jobs:
+            srv.submit(O, prog, opts)
+        results = []
+        while True:
+            running, queued = srv.step()
+            assert running <= 1
+            results.extend(srv.poll())
+            if running == 0 and queued == 0:
+                break
+    assert len(results) == len(jobs)
+    for (O, prog, opts), r in zip(jobs, sorted(results, key=lambda r: r.handle)):
+        e_want, want = _oracle(O, prog, opts)
+        assert r.terms == len(want)
+        assert abs(r.expectation - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum())
+
+
+def test_server_edge_jobs(ctx):
+    # empty program, Clifford-only program, empty observable
+    O = W.observable(10, [{0: "X", 3: "Y"}, {2: "Z"}], [0.5, -2.0])
+    b = W._ProgramBuilder(10)
+    empty = b.build()
+    b.clifford(0, 0)
+    b.clifford(2, 0, 3)
+    b.clifford(1, 2)
+    cliff = b.build()
+    E = W.observable(10, [])
+    jobs = [(O, empty, TruncationOptions()), (O, cliff, TruncationOptions(max_weight=1)),
+            (E, W.brickwork_program(10, 1, 3), TruncationOptions(coefficient_cutoff=1e-3))]
+    out = _serve(ctx, jobs)
+    for (O_, prog, opts), (r, got) in zip(jobs, out):
+        e_want, want = _oracle(O_, prog, opts)
+        assert_terms_equal(got, want)
+        assert r.expectation == pytest.approx(e_want, abs=1e-15)
+
+
+def test_server_rejects_bad_jobs(ctx):
+    with pytest.raises(EngineError):
+        Server(ctx, array_size=3)
+    with Server(ctx, array_size=1, max_jobs=2) as srv:
+        dup = W.observable(8, [{0: "Z"}, {0: "Z"}])
+        with pytest.raises(EngineError):
+            srv.submit(dup, W.brickwork_program(8, 1, 1))
+        big = W.observable(100, [{0: "Z"}])
+        with pytest.raises(EngineError):
+            srv.submit(big, W.brickwork_program(100, 1, 1))
+        with pytest.raises(ValueError):
+            srv.submit(W.observable(8, [{0: "Z"}]), W.brickwork_program(8, 1, 1),
+                       TruncationOptions(max_terms=10))
