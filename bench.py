@@ -10,7 +10,7 @@ sequential jobs of Z_0 + Z_31 through the depth-20 64-qubit brickwork
 
 roofline: the kernel with the largest share of the timed region, timed
 with HIP events on the engine's stream during the timed steps; achieved =
-its algorithmic bytes per launch / its mean launch time (the emit kernel's
+its algorithmic bytes per launch / its mean launch time (the leaf kernel's
 bytes are 25 B per collated term at one word per plane).
 cpu_baseline: the reference's own multiply + canonicalize (oracle/_ref, its
 OpenMP build) on a 65,536 x m_s slice of the same product, on this host.
