@@ -52,6 +52,7 @@ constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int kMaxKeyWords = 33;                    // 2*16 + 1
 constexpr int kLeafThreads = 512;
 constexpr int kLeafWaves = kLeafThreads / kWave;
+static_assert(kBlock == kDigits, "pass kernels map one thread to one digit");
 
 template <int AS>
 struct LeafDims {
@@ -174,60 +175,33 @@ PLH_DEV uint32_t round_pos(int s) {
     return (uint32_t)((threadIdx.x / kWave) * (kWave * kIpt) + s * kWave + lane_id());
 }
 
-// Which big bucket a tile belongs to (tile_base is an exclusive scan).
-PLH_DEV uint32_t find_bucket(const uint64_t* tile_base, uint32_t nbig, uint64_t t) {
-    uint32_t lo = 0, hi = nbig;  // largest b with tile_base[b] <= t
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (tile_base[mid] <= t) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-// --------------------------------------------------------- global pass
-template <class Src, bool IMPLICIT>
-__global__ __launch_bounds__(kBlock) void k_msd_hist(Src src, const uint32_t* __restrict__ ids,
-                                                     const uint64_t* __restrict__ bstart,
-                                                     const uint64_t* __restrict__ bsize,
-                                                     const uint64_t* __restrict__ tile_base,
-                                                     uint32_t nbig, int pos, uint32_t tile_items,
-                                                     uint32_t* __restrict__ hist) {
+// --------------------------------------------------------- global passes
+// First pass: the root bucket, ids enumerated implicitly (dense index i is
+// item i of the source), one digit position for the whole root.
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense, int pos, uint32_t tile_items,
+                                                      uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kDigits];
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t t = blockIdx.x;
-    const uint32_t bi = find_bucket(tile_base, nbig, t);
-    const uint64_t lt = t - tile_base[bi];
-    const uint64_t beg = bstart[bi] + lt * tile_items;
-    const uint64_t end = min(bstart[bi] + bsize[bi], beg + tile_items);
+    const uint64_t beg = t * tile_items;
+    const uint64_t end = min(n_dense, beg + tile_items);
     for (uint64_t i = beg + threadIdx.x; i < end; i += kBlock) {
-        uint32_t id;
-        if (IMPLICIT) {
-            id = src.dense_to_id((uint32_t)i);
-            if (!src.valid(id)) continue;
-        } else {
-            id = ids[i];
-        }
+        const uint32_t id = src.dense_to_id((uint32_t)i);
+        if (!src.valid(id)) continue;
         atomicAdd(&h[key_window<kDigitBits>(src, id, pos)], 1u);
     }
     __syncthreads();
     hist[t * kDigits + threadIdx.x] = h[threadIdx.x];
 }
 
-// One workgroup per big bucket: per-digit prefix over its tiles (rewritten
-// in place), sub-bucket starts and sizes.  Tiles are read 16 at a time so
-// the loads of a batch overlap.
-__global__ __launch_bounds__(kBlock) void k_colscan(const uint64_t* __restrict__ tile_base,
-                                                    uint32_t nbig, uint64_t ntiles,
-                                                    const uint64_t* __restrict__ bstart,
-                                                    uint32_t* __restrict__ hist,
-                                                    uint64_t* __restrict__ sub_start,
-                                                    uint64_t* __restrict__ sub_size) {
-    __shared__ uint64_t sh[kWavesPerBlock];
-    const uint32_t bi = blockIdx.x;
+// Per-digit prefix over a bucket's tiles (rewritten in place) and the
+// sub-bucket starts and sizes; tiles are read 16 at a time so the loads of
+// a batch overlap.  One workgroup of kDigits threads per bucket.
+PLH_DEV void column_scan(uint64_t t0, uint64_t t1, uint64_t bstart, uint32_t* __restrict__ hist,
+                         uint64_t* __restrict__ sub_start, uint64_t* __restrict__ sub_size, uint64_t* sh) {
     const int d = threadIdx.x;
-    const uint64_t t0 = tile_base[bi];
-    const uint64_t t1 = (bi + 1 < nbig) ? tile_base[bi + 1] : ntiles;
     uint64_t run = 0;
     constexpr int kB = 16;
     for (uint64_t t = t0; t < t1; t += kB) {
@@ -242,24 +216,28 @@ __global__ __launch_bounds__(kBlock) void k_colscan(const uint64_t* __restrict__
     }
     uint64_t tot;
     const uint64_t ex = block_exclusive_scan(run, tot, sh);
-    sub_start[(uint64_t)bi * kDigits + d] = bstart[bi] + ex;
-    sub_size[(uint64_t)bi * kDigits + d] = run;
+    sub_start[d] = bstart + ex;
+    sub_size[d] = run;
 }
 
-template <class Src, bool IMPLICIT>
-__global__ __launch_bounds__(kBlock) void k_msd_scatter(
-    Src src, const uint32_t* __restrict__ ids_in, uint32_t* __restrict__ ids_out,
-    const uint64_t* __restrict__ bstart, const uint64_t* __restrict__ bsize,
-    const uint64_t* __restrict__ tile_base, uint32_t nbig, int pos, uint32_t tile_items,
-    const uint32_t* __restrict__ hist, const uint64_t* __restrict__ sub_start) {
+__global__ __launch_bounds__(kBlock) void k_root_colscan(uint64_t ntiles, uint32_t* __restrict__ hist,
+                                                         uint64_t* __restrict__ sub_start,
+                                                         uint64_t* __restrict__ sub_size) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    column_scan(0, ntiles, 0, hist, sub_start, sub_size, sh);
+}
+
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, uint32_t* __restrict__ ids_out, uint64_t n_dense,
+                                                         int pos, uint32_t tile_items,
+                                                         const uint32_t* __restrict__ hist,
+                                                         const uint64_t* __restrict__ sub_start) {
     __shared__ MsplitShared sh;
     __shared__ uint64_t base[kDigits];
     const uint64_t t = blockIdx.x;
-    const uint32_t bi = find_bucket(tile_base, nbig, t);
-    const uint64_t lt = t - tile_base[bi];
-    const uint64_t beg = bstart[bi] + lt * tile_items;
-    const uint64_t end = min(bstart[bi] + bsize[bi], beg + tile_items);
-    base[threadIdx.x] = sub_start[(uint64_t)bi * kDigits + threadIdx.x] + hist[t * kDigits + threadIdx.x];
+    const uint64_t beg = t * tile_items;
+    const uint64_t end = min(n_dense, beg + tile_items);
+    base[threadIdx.x] = sub_start[threadIdx.x] + hist[t * kDigits + threadIdx.x];
     sh.running[threadIdx.x] = 0;
     __syncthreads();
     for (uint64_t r0 = beg; r0 < end; r0 += kRound) {
@@ -270,15 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_msd_scatter(
             dig[s] = kInvalid;
             idv[s] = 0;
             if (i < end) {
-                uint32_t id;
-                bool ok = true;
-                if (IMPLICIT) {
-                    id = src.dense_to_id((uint32_t)i);
-                    ok = src.valid(id);
-                } else {
-                    id = ids_in[i];
-                }
-                if (ok) {
+                const uint32_t id = src.dense_to_id((uint32_t)i);
+                if (src.valid(id)) {
                     idv[s] = id;
                     dig[s] = key_window<kDigitBits>(src, id, pos);
                 }
@@ -292,7 +263,8 @@ __global__ __launch_bounds__(kBlock) void k_msd_scatter(
 }
 
 // ------------------------------------------------------------ bucket lists
-// meta: bits 0-15 depth, bit 16 buffer, bit 17 uniform (no live bits left).
+// meta: bits 0-15 depth (key bits before it are equal across the bucket),
+// bit 16 id buffer, bit 17 uniform (no live bit at or after depth).
 PLH_HD uint32_t meta_depth(uint32_t m) { return m & 0xFFFFu; }
 PLH_HD uint32_t meta_buf(uint32_t m) { return (m >> 16) & 1u; }
 PLH_HD uint32_t meta_uniform(uint32_t m) { return (m >> 17) & 1u; }
@@ -300,30 +272,181 @@ PLH_HD uint32_t make_meta(uint32_t depth, uint32_t buf, uint32_t uni) {
     return depth | (buf << 16) | (uni << 17);
 }
 
+// The big buckets of one pass (device arrays of nbig entries each).
+struct PassBuckets {
+    uint64_t* start;
+    uint64_t* size;
+    uint64_t* tbase;  // exclusive scan of ntiles
+    uint64_t* kor;    // OR / AND over the bucket of the key bits from pos to its word's end
+    uint64_t* kand;
+    uint32_t* meta;
+    uint32_t* pos;     // digit position of this pass (first live bit >= depth)
+    uint32_t* ntiles;
+    uint32_t* skip;    // 0: split on the digit; else offset of the first differing bit (>= 8)
+};
+
+// The bucket list of a pass: entries at or beyond *dL are stale.  isbig
+// marks buckets to split; bigsz feeds the count of items moved this pass.
 __global__ void k_classify(const uint64_t* __restrict__ Lsize, const uint32_t* __restrict__ Lmeta,
-                           uint64_t L, uint32_t cap, uint32_t* __restrict__ isbig) {
+                           const uint64_t* __restrict__ dL, uint64_t Lb, uint32_t cap,
+                           uint32_t* __restrict__ isbig, uint64_t* __restrict__ bigsz) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= L) return;
-    isbig[e] = (Lsize[e] > cap && !meta_uniform(Lmeta[e])) ? 1u : 0u;
+    if (e >= Lb) return;
+    const bool big = e < *dL && Lsize[e] > cap && !meta_uniform(Lmeta[e]);
+    isbig[e] = big ? 1u : 0u;
+    bigsz[e] = big ? Lsize[e] : 0ull;
 }
 
 __global__ void k_gather_big(const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize,
                              const uint32_t* __restrict__ Lmeta, const uint32_t* __restrict__ isbig,
-                             const uint64_t* __restrict__ bigpos, uint64_t L,
-                             uint64_t* __restrict__ bstart, uint64_t* __restrict__ bsize,
-                             uint32_t* __restrict__ bmeta) {
+                             const uint64_t* __restrict__ bigpos, uint64_t Lb, PassBuckets B) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= L || !isbig[e]) return;
+    if (e >= Lb || !isbig[e]) return;
     const uint64_t b = bigpos[e];
-    bstart[b] = Lstart[e];
-    bsize[b] = Lsize[e];
-    bmeta[b] = Lmeta[e];
+    B.start[b] = Lstart[e];
+    B.size[b] = Lsize[e];
+    B.meta[b] = Lmeta[e];
 }
 
-__global__ void k_tile_count(const uint64_t* __restrict__ bsize, uint64_t nbig, uint32_t tile,
-                             uint32_t* __restrict__ ntiles) {
+__global__ void k_bucket_setup(PassBuckets B, uint64_t nbig, uint32_t tile, const uint64_t* __restrict__ live,
+                               int kb) {
     const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nbig) ntiles[b] = (uint32_t)((bsize[b] + tile - 1) / tile);
+    if (b >= nbig) return;
+    B.pos[b] = (uint32_t)next_live(live, kb, (int)meta_depth(B.meta[b]));
+    B.ntiles[b] = (uint32_t)((B.size[b] + tile - 1) / tile);
+    B.kor[b] = 0ull;
+    B.kand[b] = ~0ull;
+}
+
+// Which big bucket a tile belongs to (tbase is an exclusive scan).
+PLH_DEV uint32_t find_bucket(const uint64_t* tbase, uint32_t nbig, uint64_t t) {
+    uint32_t lo = 0, hi = nbig;  // largest b with tbase[b] <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tbase[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The digit at pos, and in `rest` the key bits from pos to the end of its
+// word, MSB-aligned and zero-filled (64 - pos % 64 bits are examined).
+template <class Src>
+PLH_DEV uint32_t digit_and_rest(const Src& src, uint32_t id, int pos, uint64_t& rest) {
+    const int w = pos >> 6, o = pos & 63;
+    const uint64_t hi = src.word(id, w) << o;
+    rest = hi;
+    uint64_t d = hi;
+    if (o + kDigitBits > 64) d |= src.word(id, w + 1) >> (64 - o);
+    return (uint32_t)(d >> (64 - kDigitBits));
+}
+
+// Digit histogram of a tile at its bucket's position, plus the OR and AND
+// of the key bits from there to the end of that word (so a bucket whose
+// digit is constant can jump straight to its first differing bit).  Grid
+// is an upper bound on the tile count; tiles at or beyond *dT exit at once.
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const uint32_t* __restrict__ ids0,
+                                                      const uint32_t* __restrict__ ids1, PassBuckets B,
+                                                      uint32_t nbig, const uint64_t* __restrict__ dT,
+                                                      uint32_t tile_items, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kDigits];
+    __shared__ unsigned long long sor, sand;
+    const uint64_t t = blockIdx.x;
+    if (t >= *dT) return;
+    h[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        sor = 0ull;
+        sand = ~0ull;
+    }
+    __syncthreads();
+    const uint32_t bi = find_bucket(B.tbase, nbig, t);
+    const uint64_t beg = B.start[bi] + (t - B.tbase[bi]) * tile_items;
+    const uint64_t end = min(B.start[bi] + B.size[bi], beg + tile_items);
+    const int pos = (int)B.pos[bi];
+    const uint32_t* ids = meta_buf(B.meta[bi]) ? ids1 : ids0;
+    uint64_t o = 0ull, a = ~0ull;
+    for (uint64_t i = beg + threadIdx.x; i < end; i += kBlock) {
+        uint64_t rest;
+        atomicAdd(&h[digit_and_rest(src, ids[i], pos, rest)], 1u);
+        o |= rest;
+        a &= rest;
+    }
+#pragma unroll
+    for (int s = kWave / 2; s > 0; s >>= 1) {
+        o |= __shfl_xor(o, s, kWave);
+        a &= __shfl_xor(a, s, kWave);
+    }
+    if (lane_id() == 0) {
+        atomicOr(&sor, (unsigned long long)o);
+        atomicAnd(&sand, (unsigned long long)a);
+    }
+    __syncthreads();
+    hist[t * kDigits + threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x == 0) {
+        atomicOr((unsigned long long*)&B.kor[bi], sor);
+        atomicAnd((unsigned long long*)&B.kand[bi], sand);
+    }
+}
+
+// One workgroup per big bucket: column scan, and whether the digit splits.
+__global__ __launch_bounds__(kBlock) void k_pass_colscan(PassBuckets B, uint32_t nbig,
+                                                         const uint64_t* __restrict__ dT,
+                                                         uint32_t* __restrict__ hist,
+                                                         uint64_t* __restrict__ sub_start,
+                                                         uint64_t* __restrict__ sub_size) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    const uint32_t bi = blockIdx.x;
+    const uint64_t t0 = B.tbase[bi];
+    const uint64_t t1 = (bi + 1 < nbig) ? B.tbase[bi + 1] : *dT;
+    column_scan(t0, t1, B.start[bi], hist, sub_start + (uint64_t)bi * kDigits, sub_size + (uint64_t)bi * kDigits, sh);
+    if (threadIdx.x == 0) {
+        // first differing bit among the examined ones, or the end of the word
+        const uint64_t diff = B.kor[bi] ^ B.kand[bi];
+        const uint32_t off = diff ? (uint32_t)__clzll(diff) : 64u - (B.pos[bi] & 63u);
+        B.skip[bi] = off >= (uint32_t)kDigitBits ? off : 0u;
+    }
+}
+
+// Stable scatter of a tile into its bucket's sub-buckets (other id buffer);
+// buckets whose digit is constant are left in place.
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, uint32_t* __restrict__ ids0,
+                                                         uint32_t* __restrict__ ids1, PassBuckets B, uint32_t nbig,
+                                                         const uint64_t* __restrict__ dT, uint32_t tile_items,
+                                                         const uint32_t* __restrict__ hist,
+                                                         const uint64_t* __restrict__ sub_start) {
+    __shared__ MsplitShared sh;
+    __shared__ uint64_t base[kDigits];
+    const uint64_t t = blockIdx.x;
+    if (t >= *dT) return;
+    const uint32_t bi = find_bucket(B.tbase, nbig, t);
+    if (B.skip[bi]) return;
+    const uint64_t beg = B.start[bi] + (t - B.tbase[bi]) * tile_items;
+    const uint64_t end = min(B.start[bi] + B.size[bi], beg + tile_items);
+    const int pos = (int)B.pos[bi];
+    const bool from1 = meta_buf(B.meta[bi]) != 0;
+    const uint32_t* ids_in = from1 ? ids1 : ids0;
+    uint32_t* ids_out = from1 ? ids0 : ids1;
+    base[threadIdx.x] = sub_start[(uint64_t)bi * kDigits + threadIdx.x] + hist[t * kDigits + threadIdx.x];
+    sh.running[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t r0 = beg; r0 < end; r0 += kRound) {
+        uint32_t dig[kIpt], rank[kIpt], idv[kIpt];
+#pragma unroll
+        for (int s = 0; s < kIpt; ++s) {
+            const uint64_t i = r0 + round_pos(s);
+            dig[s] = kInvalid;
+            idv[s] = 0;
+            if (i < end) {
+                idv[s] = ids_in[i];
+                dig[s] = key_window<kDigitBits>(src, idv[s], pos);
+            }
+        }
+        msplit_round(sh, dig, rank);
+#pragma unroll
+        for (int s = 0; s < kIpt; ++s)
+            if (dig[s] != kInvalid) ids_out[base[dig[s]] + rank[s]] = idv[s];
+    }
 }
 
 // The children of a split bucket, in digit order: sub-buckets larger than
@@ -359,22 +482,23 @@ struct ChildWalk {
 };
 
 __global__ void k_expand_count(const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos,
-                               const uint64_t* __restrict__ sub_start, const uint64_t* __restrict__ sub_size,
-                               uint64_t L, uint32_t cap, uint32_t* __restrict__ cnt) {
+                               PassBuckets B, const uint64_t* __restrict__ sub_start,
+                               const uint64_t* __restrict__ sub_size, uint64_t L, uint32_t cap,
+                               uint32_t* __restrict__ cnt) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= L) return;
-    if (!isbig[e]) { cnt[e] = 1; return; }
+    if (!isbig[e] || B.skip[bigpos[e]]) { cnt[e] = 1; return; }
     ChildWalk w{sub_start + bigpos[e] * kDigits, sub_size + bigpos[e] * kDigits, cap};
     cnt[e] = w.walk([](uint64_t, uint64_t, uint32_t) {});
 }
 
 __global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize,
                                const uint32_t* __restrict__ Lmeta, const uint32_t* __restrict__ isbig,
-                               const uint64_t* __restrict__ bigpos, const uint64_t* __restrict__ sub_start,
-                               const uint64_t* __restrict__ sub_size, const uint64_t* __restrict__ newpos,
-                               uint64_t L, uint32_t cap, int pos, int child_depth, uint32_t child_uniform,
-                               uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize,
-                               uint32_t* __restrict__ Nmeta) {
+                               const uint64_t* __restrict__ bigpos, PassBuckets B,
+                               const uint64_t* __restrict__ sub_start, const uint64_t* __restrict__ sub_size,
+                               const uint64_t* __restrict__ newpos, uint64_t L, uint32_t cap,
+                               const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ Nstart,
+                               uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= L) return;
     uint64_t o = newpos[e];
@@ -384,13 +508,25 @@ __global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64
         Nmeta[o] = Lmeta[e];
         return;
     }
+    const uint64_t b = bigpos[e];
+    const uint32_t pos = B.pos[b];
+    const uint32_t skip = B.skip[b];
+    if (skip) {
+        // the digit is constant: same ids, same buffer, deeper
+        const int depth = min((int)(pos + skip), kb);
+        Nstart[o] = Lstart[e];
+        Nsize[o] = Lsize[e];
+        Nmeta[o] = make_meta((uint32_t)depth, meta_buf(Lmeta[e]), next_live(live, kb, depth) >= kb ? 1u : 0u);
+        return;
+    }
     const uint32_t buf = meta_buf(Lmeta[e]) ^ 1u;
-    ChildWalk w{sub_start + bigpos[e] * kDigits, sub_size + bigpos[e] * kDigits, cap};
+    const int child_depth = min((int)pos + kDigitBits, kb);
+    const uint32_t child_uniform = next_live(live, kb, child_depth) >= kb ? 1u : 0u;
+    ChildWalk w{sub_start + b * kDigits, sub_size + b * kDigits, cap};
     w.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
         Nstart[o] = s0;
         Nsize[o] = n;
-        Nmeta[o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform)
-                              : make_meta((uint32_t)pos, buf, 0u);
+        Nmeta[o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
         ++o;
     });
 }
@@ -771,15 +907,10 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     PLH_TRY(ctx->ws.get(kWsIds1, n_dense * 4, &p));
     uint32_t* ids1 = (uint32_t*)p;
     PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
-    uint64_t* misc = (uint64_t*)p;  // [0..32] live, [40] total, [48..50] root bucket
+    uint64_t* misc = (uint64_t*)p;  // [0..33] live, [40] total, [56..59] pass counts
     uint64_t* d_live = misc;
     uint64_t* d_total = misc + 40;
-    uint64_t* root_start = misc + 48;
-    uint64_t* root_size = misc + 49;
-    uint64_t* root_tiles = misc + 50;
-    uint64_t hroot[3] = {0, n_dense, 0};
     PLH_HIP(hipMemcpyAsync(d_live, h_live, sizeof(uint64_t) * kWordsK, hipMemcpyHostToDevice, st));
-    PLH_HIP(hipMemcpyAsync(root_start, hroot, sizeof(hroot), hipMemcpyHostToDevice, st));
 
     Cfg cfg{};
     cfg.live = d_live;
@@ -805,17 +936,15 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     uint64_t* sub_size = sub_start + kDigits;
     {
         ProfScope ps(ctx, "msd_hist_first");
-        k_msd_hist<Src, true><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, nullptr, root_start, root_size,
-                                                                     root_tiles, 1u, pos, tile0, hist);
+        k_root_hist<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, n_dense, pos, tile0, hist);
     }
     {
         ProfScope ps(ctx, "colscan");
-        k_colscan<<<1, kBlock, 0, st>>>(root_tiles, 1u, t0, root_start, hist, sub_start, sub_size);
+        k_root_colscan<<<1, kBlock, 0, st>>>(t0, hist, sub_start, sub_size);
     }
     {
         ProfScope ps(ctx, "msd_scatter_first", 4.0 * (double)n_dense);
-        k_msd_scatter<Src, true><<<dim3((unsigned)t0), kBlock, 0, st>>>(
-            src, nullptr, ids0, root_start, root_size, root_tiles, 1u, pos, tile0, hist, sub_start);
+        k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, pos, tile0, hist, sub_start);
     }
     PLH_HIP(hipGetLastError());
     int depth = pos + kDigitBits;
@@ -838,88 +967,89 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     for (uint64_t v : lz) n_valid += v;
     uint64_t L = ls.size();
     if (L == 0) return PLH_OK;
+    // pass bookkeeping read back once per pass: list length, big buckets, their items, tiles
+    uint64_t* d_L = misc + 56;
+    uint64_t* d_nbig = misc + 57;
+    uint64_t* d_items = misc + 58;
+    uint64_t* d_T = misc + 59;
     Lists A, B;
     PLH_TRY(get_lists(ctx, kWsListA, L, &A));
     PLH_HIP(hipMemcpyAsync(A.start, ls.data(), L * 8, hipMemcpyHostToDevice, st));
     PLH_HIP(hipMemcpyAsync(A.size, lz.data(), L * 8, hipMemcpyHostToDevice, st));
     PLH_HIP(hipMemcpyAsync(A.meta, lm.data(), L * 4, hipMemcpyHostToDevice, st));
+    PLH_HIP(hipMemcpyAsync(d_L, &L, 8, hipMemcpyHostToDevice, st));
     int cur_slot = kWsListA;
+    uint64_t Lb = L;  // bound on the list length (exact on the first pass)
 
-    // ---- global passes until every non-uniform bucket fits a leaf.
+    // ---- global passes until every non-uniform bucket fits a leaf.  Grids
+    // of the list kernels use the bound Lb; the exact counts come back in
+    // one read per pass.
     for (int guard = 0; guard < 2 * kb; ++guard) {
-        PLH_TRY(ctx->ws.get(kWsFlags, L * 4 + 64, &p));
+        // list arrays: isbig u32, bigpos u64, bigsz u64, cnt u32, newpos u64
+        PLH_TRY(ctx->ws.get(kWsPassL, Lb * 32 + 256, &p));
         uint32_t* isbig = (uint32_t*)p;
-        PLH_TRY(ctx->ws.get(kWsOffs, L * 8 + 64, &p));
-        uint64_t* bigpos = (uint64_t*)p;
-        k_classify<<<grid1(L), kBlock, 0, st>>>(A.size, A.meta, L, cap, isbig);
-        PLH_TRY(scan_u32_to_u64(ctx, isbig, bigpos, L, d_total));
-        uint64_t nbig = 0;
-        PLH_TRY(read_u64(ctx, d_total, &nbig));
+        uint32_t* cnt = isbig + Lb;
+        uint64_t* bigpos = (uint64_t*)((char*)p + ((Lb * 8 + 15) & ~(uint64_t)15));
+        uint64_t* bigsz = bigpos + Lb;
+        uint64_t* newpos = bigsz + Lb;
+        k_classify<<<grid1(Lb), kBlock, 0, st>>>(A.size, A.meta, d_L, Lb, cap, isbig, bigsz);
+        PLH_TRY(scan_u32_to_u64(ctx, isbig, bigpos, Lb, d_nbig));
+        PLH_TRY(scan_u64(ctx, bigsz, newpos, Lb, d_items));
+        uint64_t info[3];
+        // the big-bucket arrays are sized by the bound Lb before the counts are known
+        PLH_TRY(ctx->ws.get(kWsPassB, Lb * 56 + 256, &p));
+        PassBuckets PB;
+        PB.start = (uint64_t*)p;
+        PB.size = PB.start + Lb;
+        PB.tbase = PB.size + Lb;
+        PB.kor = PB.tbase + Lb;
+        PB.kand = PB.kor + Lb;
+        PB.meta = (uint32_t*)(PB.kand + Lb);
+        PB.pos = PB.meta + Lb;
+        PB.ntiles = PB.pos + Lb;
+        PB.skip = PB.ntiles + Lb;
+        k_gather_big<<<grid1(Lb), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, Lb, PB);
+        PLH_HIP(hipGetLastError());
+        PLH_TRY(read_u64n(ctx, d_L, 3, info));
+        L = info[0];
+        const uint64_t nbig = info[1], pass_items = info[2];
         if (nbig == 0) break;
-        pos = next_live_host(h_live, kb, depth);
-        PLH_TRY(ctx->ws.get(kWsBig, nbig * (8 + 8 + 8 + 4 + 4) + 64, &p));
-        uint64_t* bstart = (uint64_t*)p;
-        uint64_t* bsize = bstart + nbig;
-        uint64_t* tbase = bsize + nbig;
-        uint32_t* bmeta = (uint32_t*)(tbase + nbig);
-        uint32_t* ntiles = bmeta + nbig;
-        k_gather_big<<<grid1(L), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, L, bstart, bsize,
-                                                  bmeta);
-        PLH_TRY(ctx->ws.get(kWsScan1, nbig * 8 + 64, &p));
-        PLH_TRY(scan_u64(ctx, bsize, (uint64_t*)p, nbig, d_total));
-        uint64_t pass_items = 0;
-        PLH_TRY(read_u64(ctx, d_total, &pass_items));
         const uint32_t tile = pick_tile(pass_items, kTileMaxExplicit);
-        k_tile_count<<<grid1(nbig), kBlock, 0, st>>>(bsize, nbig, tile, ntiles);
-        PLH_TRY(scan_u32_to_u64(ctx, ntiles, tbase, nbig, d_total));
-        uint64_t T = 0;
-        PLH_TRY(read_u64(ctx, d_total, &T));
-        PLH_TRY(ctx->ws.get(kWsHist, T * kDigits * 4, &p));
+        k_bucket_setup<<<grid1(nbig), kBlock, 0, st>>>(PB, nbig, tile, d_live, kb);
+        PLH_TRY(scan_u32_to_u64(ctx, PB.ntiles, PB.tbase, nbig, d_T));
+        const uint64_t Tb = pass_items / tile + nbig;  // >= sum of ceil(size / tile)
+        PLH_TRY(ctx->ws.get(kWsHist, Tb * kDigits * 4, &p));
         hist = (uint32_t*)p;
         PLH_TRY(ctx->ws.get(kWsSub, nbig * kDigits * 16, &p));
         sub_start = (uint64_t*)p;
         sub_size = sub_start + nbig * kDigits;
-        // All big buckets of a pass sit in one buffer: each was produced by
-        // the previous pass.
-        uint32_t hmeta = 0;
-        PLH_HIP(hipMemcpyAsync(&hmeta, bmeta, 4, hipMemcpyDeviceToHost, st));
-        PLH_HIP(hipStreamSynchronize(st));
-        const uint32_t* in_ids = meta_buf(hmeta) ? ids1 : ids0;
-        uint32_t* out_ids = meta_buf(hmeta) ? ids0 : ids1;
         {
             ProfScope ps(ctx, "msd_hist", 4.0 * (double)pass_items);
-            k_msd_hist<Src, false><<<dim3((unsigned)T), kBlock, 0, st>>>(src, in_ids, bstart, bsize, tbase,
-                                                                         (uint32_t)nbig, pos, tile, hist);
+            k_pass_hist<Src><<<dim3((unsigned)Tb), kBlock, 0, st>>>(src, ids0, ids1, PB, (uint32_t)nbig, d_T, tile,
+                                                                     hist);
         }
         {
             ProfScope ps(ctx, "colscan");
-            k_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(tbase, (uint32_t)nbig, T, bstart, hist,
-                                                               sub_start, sub_size);
+            k_pass_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(PB, (uint32_t)nbig, d_T, hist, sub_start, sub_size);
         }
         {
             ProfScope ps(ctx, "msd_scatter", 8.0 * (double)pass_items);
-            k_msd_scatter<Src, false><<<dim3((unsigned)T), kBlock, 0, st>>>(
-                src, in_ids, out_ids, bstart, bsize, tbase, (uint32_t)nbig, pos, tile, hist, sub_start);
+            k_pass_scatter<Src><<<dim3((unsigned)Tb), kBlock, 0, st>>>(src, ids0, ids1, PB, (uint32_t)nbig, d_T,
+                                                                        tile, hist, sub_start);
         }
         PLH_HIP(hipGetLastError());
-        depth = pos + kDigitBits;
-        const uint32_t uni = next_live_host(h_live, kb, depth) >= kb ? 1u : 0u;
-        PLH_TRY(ctx->ws.get(kWsScan0, L * 4 + 64, &p));
-        uint32_t* cnt = (uint32_t*)p;
-        PLH_TRY(ctx->ws.get(kWsScan1, L * 8 + 64, &p));
-        uint64_t* newpos = (uint64_t*)p;
-        k_expand_count<<<grid1(L), kBlock, 0, st>>>(isbig, bigpos, sub_start, sub_size, L, cap, cnt);
-        PLH_TRY(scan_u32_to_u64(ctx, cnt, newpos, L, d_total));
-        uint64_t L2 = 0;
-        PLH_TRY(read_u64(ctx, d_total, &L2));
+        k_expand_count<<<grid1(L), kBlock, 0, st>>>(isbig, bigpos, PB, sub_start, sub_size, L, cap, cnt);
+        PLH_TRY(scan_u32_to_u64(ctx, cnt, newpos, L, d_L));
+        // each split bucket has at most kDigits children; every entry holds >= 1 item
+        const uint64_t Lb2 = std::min<uint64_t>(L + (kDigits - 1) * nbig, n_valid);
         const int nslot = cur_slot == kWsListA ? kWsListB : kWsListA;
-        PLH_TRY(get_lists(ctx, nslot, L2, &B));
-        k_expand_write<<<grid1(L), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, sub_start, sub_size,
-                                                    newpos, L, cap, pos, depth, uni, B.start, B.size, B.meta);
+        PLH_TRY(get_lists(ctx, nslot, Lb2, &B));
+        k_expand_write<<<grid1(L), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, PB, sub_start, sub_size,
+                                                    newpos, L, cap, d_live, kb, B.start, B.size, B.meta);
         PLH_HIP(hipGetLastError());
         std::swap(A, B);
         cur_slot = nslot;
-        L = L2;
+        Lb = Lb2;
     }
 
     // ---- leaves: sort, sum, filter, write at each leaf's start.

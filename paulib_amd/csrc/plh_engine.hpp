@@ -152,6 +152,8 @@ enum : int {
     kWsTmpSum,
     kWsPackL,
     kWsPackR,
+    kWsPassL,   // per-pass arrays over the bucket list
+    kWsPassB,   // per-pass arrays over the big buckets
     kWsCount
 };
 
@@ -171,6 +173,8 @@ int scan_u64(plh_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n, ui
 
 // Read one u64 from device memory (synchronises the stream).
 int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out);
+// Read n <= 64 consecutive u64 from device memory (one synchronisation).
+int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out);
 
 // Collation (plh_collate.hip).
 struct CollateSpec {

@@ -75,9 +75,45 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TIn* __restrict__ i
     }
 }
 
+// One workgroup for short inputs: blocked chunks of kScanTile with a carry.
+template <class TIn>
+__global__ __launch_bounds__(kBlock) void k_scan_small(const TIn* __restrict__ in, uint64_t* __restrict__ out,
+                                                       uint64_t n, uint64_t* __restrict__ total) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    uint64_t carry = 0;
+    for (uint64_t c0 = 0; c0 < n; c0 += kScanTile) {
+        const uint64_t base = c0 + (uint64_t)threadIdx.x * kScanIpt;
+        uint64_t vals[kScanIpt];
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < kScanIpt; ++k) {
+            const uint64_t i = base + k;
+            vals[k] = i < n ? (uint64_t)in[i] : 0;
+            acc += vals[k];
+        }
+        uint64_t tot;
+        uint64_t run = carry + block_exclusive_scan(acc, tot, sh);
+#pragma unroll
+        for (int k = 0; k < kScanIpt; ++k) {
+            const uint64_t i = base + k;
+            if (i < n) out[i] = run;
+            run += vals[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+constexpr uint64_t kScanSmall = 16 * kScanTile;  // one workgroup up to this many elements
+
 template <class TIn>
 int scan_impl(plh_ctx* ctx, const TIn* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total) {
     hipStream_t st = ctx->stream;
+    if (n <= kScanSmall) {
+        k_scan_small<TIn><<<1, kBlock, 0, st>>>(d_in, d_out, n, d_total);
+        PLH_HIP(hipGetLastError());
+        return PLH_OK;
+    }
     const uint64_t nb = (n + kScanTile - 1) / kScanTile;
     void* p = nullptr;
     PLH_TRY(ctx->ws.get(kWsScan2, (size_t)(nb ? nb : 1) * sizeof(uint64_t), &p));
@@ -109,6 +145,15 @@ int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out) {
     PLH_HIP(hipMemcpyAsync(ctx->stage.p, d_src, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     PLH_HIP(hipStreamSynchronize(ctx->stream));
     *out = ctx->stage.p[0];
+    return PLH_OK;
+}
+
+int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out) {
+    if (n < 1 || n > 64) return PLH_ERR_ARG;
+    if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
+    PLH_HIP(hipMemcpyAsync(ctx->stage.p, d_src, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) out[i] = ctx->stage.p[i];
     return PLH_OK;
 }
 
