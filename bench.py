@@ -7,6 +7,11 @@ product->collation path with inputs and output resident in HBM.  value =
 n*m / (seconds per step) (collated product terms/s).  Propagation latency:
 sequential jobs of Z_0 + Z_31 through the depth-20 64-qubit brickwork
 (truncation |c| < 1e-4, weight > 8) on the same device, p50/p99 reported.
+Serving: an open-loop Poisson stream of propagation jobs (weight-1..3
+observables on 32..128 qubits, brickwork depth 5..40, cutoffs 1e-5..1e-3)
+through the continuous-batching server (plh_server), p50/p99 latency from
+arrival and jobs/s, beside the plain drop-in's jobs/s on the stream's first
+jobs run one after another.
 
 roofline: the kernel with the largest share of the timed region, timed
 with HIP events on the engine's stream during the timed steps; achieved =
@@ -56,6 +61,72 @@ def kernel_profile(ctx):
     return out
 
 
+def serving(ctx, n_jobs: int, rate: float, n_plain: int, max_jobs: int):
+    """Open-loop Poisson stream of propagation jobs through the batched server,
+    against the plain drop-in (run_program per job) on the stream's first jobs."""
+    from paulib_amd import PauliSum, TruncationOptions, run_program
+    from paulib_amd import workloads as W
+    from paulib_amd.serve import Server
+
+    jobs = W.serving_jobs(n_jobs, rate, seed=9)
+
+    def opts(cut):
+        return TruncationOptions(max_weight=8, coefficient_cutoff=cut)
+
+    t_plain = []
+    for k, (_, O, prog, cut) in enumerate(jobs[:n_plain]):
+        d = PauliSum.from_host(ctx, O)
+        t0 = time.perf_counter()
+        run_program(d, prog, opts(cut))
+        d.zero_state_expectation()
+        t_plain.append(time.perf_counter() - t0)
+        log(f"plain job {k}: {prog.num_rotations()} rotations on {O.num_qubits} qubits, {t_plain[-1]:.3f} s")
+        d.close()
+    plain_rate = len(t_plain) / sum(t_plain)
+
+    arrival, finish = {}, {}
+    in_flight_max = 0
+    with Server(ctx, array_size=2, max_jobs=max_jobs) as srv:
+        t0 = time.perf_counter()
+        i = 0
+        last_log = t0
+        while len(finish) < n_jobs:
+            now = time.perf_counter() - t0
+            while i < n_jobs and jobs[i][0] <= now:
+                _, O, prog, cut = jobs[i]
+                h = srv.submit(O, prog, opts(cut))
+                arrival[h] = t0 + jobs[i][0]
+                i += 1
+            running, queued = srv.step()
+            in_flight_max = max(in_flight_max, running)
+            t = time.perf_counter()
+            for r in srv.poll():
+                finish[r.handle] = t
+            if running == 0 and queued == 0 and i < n_jobs:
+                time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
+            if t - last_log > 5.0:
+                log(f"serving: {len(finish)}/{n_jobs} done, {running} running, {queued} queued, "
+                    f"batch {srv.batch_terms} terms, step {srv.steps}")
+                last_log = t
+        steps = srv.steps
+    lat = np.array([finish[h] - arrival[h] for h in finish]) * 1e3
+    span = max(finish.values()) - (t0 + jobs[0][0])
+    rate_served = n_jobs / span
+    rots = [p.num_rotations() for _, _, p, _ in jobs]
+    log(f"serving: {n_jobs} jobs in {span:.2f} s = {rate_served:.2f} jobs/s; plain loop {plain_rate:.2f} jobs/s; "
+        f"p50 {np.percentile(lat, 50):.0f} ms, p99 {np.percentile(lat, 99):.0f} ms")
+    return {
+        "workload": (f"{n_jobs} propagation jobs, Poisson arrivals at {rate:g}/s: weight-1..3 observable on "
+                     "32..128 qubits, brickwork depth 5..40, |c| cutoff 1e-5..1e-3 (log-uniform), weight <= 8"),
+        "mode": "continuous batching (plh_server), open loop",
+        "jobs": n_jobs, "rotations_mean": float(np.mean(rots)), "rotations_max": int(max(rots)),
+        "jobs_per_s": rate_served, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "steps": steps, "max_running": in_flight_max,
+        "plain_jobs_per_s": plain_rate, "plain_sample_jobs": len(t_plain),
+        "speedup_vs_plain": rate_served / plain_rate,
+    }
+
+
 def cpu_baseline(A, B, target_s: float):
     """Reference multiply + canonicalize on a rhs slice sized for ~target_s."""
     import oracle
@@ -99,6 +170,10 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--m", type=int, default=65536)
     ap.add_argument("--prop-jobs", type=int, default=5)
+    ap.add_argument("--serve-jobs", type=int, default=96)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-plain", type=int, default=6)
+    ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-propagation", action="store_true")
@@ -197,6 +272,8 @@ def main():
                                  "mode": "sequential jobs, plain drop-in", "jobs": len(lat),
                                  "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                                  "jobs_per_s": float(1e3 / lat.mean()), "expectation": e}
+    if args.serve_jobs > 0:
+        result["serving"] = serving(ctx, args.serve_jobs, args.serve_rate, args.serve_plain, args.serve_slots)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds)
