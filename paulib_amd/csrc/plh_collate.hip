@@ -539,12 +539,17 @@ struct LeafLds {
     double val[D::kCap];               // folded coefficients; run sums at head slots
     uint16_t perm[D::kCap];            // sorted position -> slot
     uint16_t perm2[D::kCap];           // bin order -> (key, id) order; later rslot
-    uint16_t dig[D::kCap];             // counting-sort digit of each slot
-    uint32_t sec[D::kCap];             // the 32 key bits after the digit (cheap first compare)
+    uint16_t bdig[D::kCap];            // counting-sort digit, in bin order
+    uint32_t bsec[D::kCap];            // the 32 key bits after the digit, in bin order
     uint8_t fl[D::kCap];               // raw flags
     uint32_t bin[D::kBins + 1];        // counting-sort bin starts
-    unsigned long long diff[KW + 1];
+    unsigned long long diff[KW + 1];   // varying bits of each key word
+    uint64_t cmask[KW + 1][6];         // compress masks of diff
+    int nvary[KW + 1];                 // varying bits per word
+    int crun[KW + 1];                  // 1 + shift when a word's varying bits are one run, else 0
     int pos;
+    uint32_t nvtot;                    // varying bits in the sort key (<= 64)
+    uint32_t pcs;                      // set sort-key bits, summed over the leaf
     uint32_t red[kLeafWaves];
 };
 
@@ -561,6 +566,88 @@ PLH_DEV uint32_t lds_window(const LeafLds<AS, KW>& L, uint32_t fmask, int i, int
     uint64_t hi = lds_word(L, fmask, i, w) << o;
     if (o + DB > 64) hi |= lds_word(L, fmask, i, w + 1) >> (64 - o);
     return (uint32_t)(hi >> (64 - DB));
+}
+
+template <int AS, int KW>
+PLH_DEV uint64_t lds_window64(const LeafLds<AS, KW>& L, uint32_t fmask, int i, int pos) {
+    const int w = pos >> 6, o = pos & 63;
+    uint64_t v = lds_word(L, fmask, i, w) << o;
+    if (o) v |= lds_word(L, fmask, i, w + 1) >> (64 - o);
+    return v;
+}
+
+// Bit compress (Hacker's Delight 7-4): the bits of x under mask m, moved to
+// the low end in their order.  The six shift masks depend on m only, so a
+// leaf computes them once per key word.
+PLH_HD void compress_masks(uint64_t m, uint64_t* mv) {
+    uint64_t mk = ~m << 1;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        uint64_t mp = mk ^ (mk << 1);
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        mp ^= mp << 32;
+        mv[i] = mp & m;
+        m = (m ^ mv[i]) | (mv[i] >> (1 << i));
+        mk &= ~mp;
+    }
+}
+
+PLH_HD uint64_t compress_bits(uint64_t x, uint64_t m, const uint64_t* mv) {
+    x &= m;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const uint64_t t = x & mv[i];
+        x = (x ^ t) | (t >> (1 << i));
+    }
+    return x;
+}
+
+// Sort key of item i: its key bits at the leaf's varying positions (bits
+// where some item differs), most significant first, the first 64 of them,
+// left-aligned.  Bits that do not vary inside a leaf do not order it, so
+// this is an order-preserving key that is complete when at most 64 bits vary.
+template <int AS, int KW>
+PLH_DEV uint64_t varying_key(const LeafLds<AS, KW>& L, uint32_t fmask, int i) {
+    uint64_t K = 0;
+    int used = 0;
+#pragma unroll
+    for (int w = 0; w < KW + 1; ++w) {
+        const int nv = L.nvary[w];
+        if (nv == 0 || used >= 64) continue;
+        const uint64_t m = L.diff[w];
+        const uint64_t x = lds_word(L, fmask, i, w);
+        // a contiguous run of varying bits needs only a mask and a shift
+        const uint64_t c = L.crun[w] ? (x & m) >> (L.crun[w] - 1) : compress_bits(x, m, L.cmask[w]);
+        const int room = 64 - used;
+        if (nv <= room) {
+            K = nv == 64 ? c : (K << nv) | c;
+            used += nv;
+        } else {
+            K = (K << room) | (c >> (nv - room));
+            used = 64;
+        }
+    }
+    return used == 0 ? 0ull : used < 64 ? K << (64 - used) : K;
+}
+
+// Sparse form of a sort key K: the positions of its first 9 set bits, 7 bits
+// each, most significant first, as 64 - position (absent bits: 0).  An
+// earlier set bit makes K larger, so these codes order as K does on all they
+// cover; they spread sparse keys (few set bits) over the counting-sort bins,
+// where K's own leading bits are mostly 0.
+PLH_DEV uint64_t sparse_code(uint64_t K) {
+    uint64_t S = 0;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+        if (!K) break;
+        const int b = __clzll(K);
+        S |= (uint64_t)(64 - b) << (57 - 7 * f);
+        K &= ~(1ull << (63 - b));
+    }
+    return S;
 }
 
 // (key, masked flags, id) order; ids are re-read from global memory only on
@@ -641,13 +728,21 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     const uint64_t out_base = start;
 
     if (size > (uint64_t)D::kCap) {
-        // identical keys, ids ascending (stable passes): one sequential run
-        if (tid == 0) {
-            double acc = 0.0;
-            for (uint64_t i = 0; i < size; ++i) {
-                const uint32_t id = ids[i];
-                acc += fold_value(c, src.base_value(id), src.raw_flags(id));
+        // identical keys, ids ascending (stable passes): one run, summed in
+        // id order by one thread from values all threads stage in LDS
+        double acc = 0.0;
+        for (uint64_t c0 = 0; c0 < size; c0 += D::kCap) {
+            const uint32_t m = (uint32_t)min<uint64_t>(size - c0, (uint64_t)D::kCap);
+            for (uint32_t i = tid; i < m; i += kLeafThreads) {
+                const uint32_t id = ids[c0 + i];
+                L.val[i] = fold_value(c, src.base_value(id), src.raw_flags(id));
             }
+            __syncthreads();
+            if (tid == 0)
+                for (uint32_t i = 0; i < m; ++i) acc += L.val[i];
+            __syncthreads();
+        }
+        if (tid == 0) {
             uint64_t xz[KW];
             uint32_t fl;
             src.key(ids[0], xz, fl);
@@ -710,6 +805,17 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (int w = 0; w < KW + 1 && pos < 0; ++w)
             if (L.diff[w]) pos = 64 * w + __clzll(L.diff[w]);
         L.pos = pos;
+        L.pcs = 0;
+        int tot = 0;
+        for (int w = 0; w < KW + 1; ++w) tot += popc(L.diff[w]);
+        L.nvtot = (uint32_t)min(tot, 64);
+    }
+    if (tid < KW + 1) {
+        const uint64_t m = L.diff[tid];
+        L.nvary[tid] = popc(m);
+        compress_masks(m, L.cmask[tid]);
+        const int tz = m ? __ffsll((long long)m) - 1 : 0;
+        L.crun[tid] = (m && (((m >> tz) + 1) & (m >> tz)) == 0) ? tz + 1 : 0;
     }
     for (int b = tid; b < D::kBins; b += kLeafThreads) L.bin[b] = 0;
     __syncthreads();
@@ -719,17 +825,34 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (uint32_t p = tid; p < n; p += kLeafThreads) L.perm[p] = (uint16_t)p;
         __syncthreads();
     } else {
-        // counting sort on the kDl bits where the keys first differ ...
-        uint32_t dig[kPer], slot[kPer];
+        // Sort key: the leaf's varying bits (see varying_key), or for sparse
+        // keys (under 1 bit in 8 set) their set-bit positions (sparse_code).
+        uint64_t K[kPer];
+        {
+            uint32_t pc = 0;
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const uint32_t p = tid + k * kLeafThreads;
+                K[k] = p < n ? varying_key<AS, KW>(L, c.fmask, (int)p) : 0ull;
+                pc += (uint32_t)popc(K[k]);
+            }
+            pc = wave_sum(pc);
+            if (lane_id() == 0) atomicAdd(&L.pcs, pc);
+        }
+        __syncthreads();
+        const bool sparse = 8u * L.pcs < L.nvtot * n;
+        // counting sort on the kDl leading bits of the sort key ...
+        uint32_t dig[kPer], slot[kPer], sec[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
             dig[k] = 0;
             slot[k] = 0;
+            sec[k] = 0;
             if (p < n) {
-                dig[k] = lds_window<D::kDl, AS, KW>(L, c.fmask, (int)p, pos);
-                L.dig[p] = (uint16_t)dig[k];
-                L.sec[p] = lds_window<32, AS, KW>(L, c.fmask, (int)p, pos + D::kDl);
+                const uint64_t S = sparse ? sparse_code(K[k]) : K[k];
+                dig[k] = (uint32_t)(S >> (64 - D::kDl));
+                sec[k] = (uint32_t)((S << D::kDl) >> 32);
                 slot[k] = atomicAdd(&L.bin[dig[k]], 1u);
             }
         }
@@ -758,27 +881,45 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
-            if (p < n) L.perm[L.bin[dig[k]] + slot[k]] = (uint16_t)p;
+            if (p < n) {
+                const uint32_t q = L.bin[dig[k]] + slot[k];
+                L.perm[q] = (uint16_t)p;
+                L.bdig[q] = (uint16_t)dig[k];
+                L.bsec[q] = sec[k];
+            }
         }
         __syncthreads();
         // ... then each item's rank inside its bin on the full (key, id).
         // Work is assigned by bin-ordered position, so the lanes of a wave
-        // mostly share a bin; the 32 bits after the digit decide most
-        // comparisons before the full key is read.
+        // mostly share a bin.  The 32 sort-key bits after the digit, read in
+        // bin order four at a time, decide every comparison but ties, which
+        // compare the whole key (and the id).
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t q = tid + k * kLeafThreads;
             if (q < n) {
                 const int me = L.perm[q];
-                const uint32_t b = L.dig[me];
+                const uint32_t b = L.bdig[q];
                 const uint32_t s0 = L.bin[b], s1 = L.bin[b + 1];
                 uint32_t r = s0;
                 if (s1 - s0 > 1) {
-                    const uint32_t ms = L.sec[me];
-                    for (uint32_t t = s0; t < s1; ++t) {
-                        const int o = L.perm[t];
-                        const uint32_t os = L.sec[o];
-                        r += (os < ms || (os == ms && slot_less<AS, KW>(L, c.fmask, ids, o, me))) ? 1u : 0u;
+                    const uint32_t ms = L.bsec[q];
+                    uint32_t lt = 0, eq = 0;
+                    uint32_t t = s0;
+                    for (; t + 4 <= s1; t += 4) {
+                        const uint32_t a0 = L.bsec[t], a1 = L.bsec[t + 1], a2 = L.bsec[t + 2], a3 = L.bsec[t + 3];
+                        lt += (uint32_t)(a0 < ms) + (uint32_t)(a1 < ms) + (uint32_t)(a2 < ms) + (uint32_t)(a3 < ms);
+                        eq += (uint32_t)(a0 == ms) + (uint32_t)(a1 == ms) + (uint32_t)(a2 == ms) + (uint32_t)(a3 == ms);
+                    }
+                    for (; t < s1; ++t) {
+                        const uint32_t a = L.bsec[t];
+                        lt += (uint32_t)(a < ms);
+                        eq += (uint32_t)(a == ms);
+                    }
+                    r += lt;
+                    if (eq > 1) {  // other items share these 32 bits (eq counts this one too)
+                        for (t = s0; t < s1; ++t)
+                            if (t != q && L.bsec[t] == ms && slot_less<AS, KW>(L, c.fmask, ids, L.perm[t], me)) ++r;
                     }
                 }
                 L.perm2[r] = (uint16_t)me;
@@ -1094,10 +1235,11 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     return PLH_OK;
 }
 
-// Pack a sum into records of 2*AS + 2 words (x[AS], z[AS], value, flags);
-// words beyond the sum's own array size are zero.
+// Pack a sum into records of 2*AS + 2 words (x[AS], z[AS], value,
+// flags | job << 32); words beyond the sum's own array size are zero.
 template <int AS>
-__global__ __launch_bounds__(kBlock) void k_pack(SumView s, int as, int64_t n, uint64_t* __restrict__ rec) {
+__global__ __launch_bounds__(kBlock) void k_pack(SumView s, const uint32_t* __restrict__ j, int as, int64_t n,
+                                                 uint64_t* __restrict__ rec) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
     uint64_t* r = rec + (size_t)t * (2 * AS + 2);
@@ -1107,7 +1249,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(SumView s, int as, int64_t n, u
         r[AS + k] = k < as ? s.z[(int64_t)k * s.stride + t] : 0ull;
     }
     r[2 * AS] = (uint64_t)__double_as_longlong(s.v[t]);
-    r[2 * AS + 1] = s.f[t];
+    r[2 * AS + 1] = (uint64_t)s.f[t] | (j ? (uint64_t)j[t] << 32 : 0ull);
 }
 
 template <int AS>
@@ -1116,11 +1258,16 @@ int pack_sum(plh_ctx* ctx, const plh_sum* in, int slot, const uint64_t** out) {
     PLH_TRY(ctx->ws.get(slot, (size_t)(in->n ? in->n : 1) * (2 * AS + 2) * 8, &p));
     if (in->n)
         k_pack<AS><<<grid1((uint64_t)in->n), kBlock, 0, ctx->stream>>>(SumView{in->x, in->z, in->f, in->v, in->cap},
-                                                                        in->as, in->n, (uint64_t*)p);
+                                                                        in->has_job ? in->j : nullptr, in->as, in->n,
+                                                                        (uint64_t*)p);
     PLH_HIP(hipGetLastError());
     *out = (const uint64_t*)p;
     return PLH_OK;
 }
+
+// Records of a collated sum beyond this many bytes are not made (the
+// leaf then reads the planes).
+constexpr uint64_t kMaxPackBytes = 16ull << 30;
 
 int padded_as(int as) {
     if (as <= 1) return 1;
@@ -1136,9 +1283,12 @@ int collate_terms_as(plh_ctx* ctx, const plh_sum* in, const CollateSpec& spec, p
     TermSource<AS, JOB> src;
     src.s = SumView{in->x, in->z, in->f, in->v, in->cap};
     src.j = in->j;
+    src.rec = nullptr;
     src.as = in->as;
     src.mode = spec.mode;
     src.n = (uint32_t)in->n;
+    if ((uint64_t)in->n * TermSource<AS, JOB>::kRec * 8 <= kMaxPackBytes)
+        PLH_TRY(pack_sum<AS>(ctx, in, kWsPackL, &src.rec));
     uint64_t orw[40];
     PLH_TRY(op_or_words(ctx, in, orw));
     constexpr int XO = TermSource<AS, JOB>::kXOff;

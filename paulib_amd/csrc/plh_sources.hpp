@@ -42,11 +42,15 @@ struct TermSource {
     static constexpr int kAS = AS;
     static constexpr int kXOff = JOB ? 1 : 0;    // index of x[0] in the key
     static constexpr int kKW = 2 * AS + kXOff;  // key words before the flags word
+    static constexpr int kRec = 2 * AS + 2;     // packed record words
     SumView s;
-    const uint32_t* j;  // job plane (JOB only)
-    int as;             // real words per plane (<= AS)
-    uint32_t mode;      // kFold ...
-    uint32_t n;         // items
+    const uint32_t* j;    // job plane (JOB only)
+    const uint64_t* rec;  // the terms packed as records (x[AS], z[AS], value,
+                          // flags | job << 32), or null: whole keys come from
+                          // one record instead of one load per plane
+    int as;               // real words per plane (<= AS)
+    uint32_t mode;        // kFold ...
+    uint32_t n;           // items
 
     PLH_DEV bool valid(uint32_t) const { return true; }
     PLH_DEV uint32_t dense_to_id(uint32_t t) const { return t; }
@@ -66,6 +70,18 @@ struct TermSource {
     }
     // Key words and the RAW flags byte (the engine masks the sign bit).
     PLH_DEV void key(uint32_t id, uint64_t* kw, uint32_t& fl) const {
+        if (rec) {
+            const uint64_t* r = rec + (size_t)id * kRec;
+            const uint64_t tail = r[kRec - 1];
+            if (JOB) kw[0] = tail >> 32;
+#pragma unroll
+            for (int k = 0; k < AS; ++k) {
+                kw[kXOff + k] = r[k];
+                kw[kXOff + AS + k] = r[AS + k];
+            }
+            fl = (uint32_t)tail & 0xFFu;
+            return;
+        }
         if (JOB) kw[0] = j[id];
 #pragma unroll
         for (int k = 0; k < AS; ++k) {
@@ -74,9 +90,13 @@ struct TermSource {
         }
         fl = s.f[id];
     }
-    PLH_DEV uint32_t raw_flags(uint32_t id) const { return s.f[id]; }
+    PLH_DEV uint32_t raw_flags(uint32_t id) const {
+        return rec ? (uint32_t)rec[(size_t)id * kRec + kRec - 1] & 0xFFu : s.f[id];
+    }
     // Coefficient before the sign fold.
-    PLH_DEV double base_value(uint32_t id) const { return s.v[id]; }
+    PLH_DEV double base_value(uint32_t id) const {
+        return rec ? __longlong_as_double((long long)rec[(size_t)id * kRec + 2 * AS]) : s.v[id];
+    }
 };
 
 // A term packed as one record of 2*AS + 2 words: x[AS], z[AS], value,
