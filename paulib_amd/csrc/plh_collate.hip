@@ -103,16 +103,6 @@ PLH_DEV int next_live(const uint64_t* live, int kb, int d) {
     return kb;
 }
 
-inline int next_live_host(const uint64_t* live, int kb, int d) {
-    while (d < kb) {
-        const int w = d >> 6, o = d & 63;
-        const uint64_t m = live[w] & (~0ull >> o);
-        if (m) return (w << 6) + __builtin_clzll(m);
-        d = (w + 1) << 6;
-    }
-    return kb;
-}
-
 PLH_DEV double fold_value(const Cfg& c, double base, uint32_t raw) {
     return ((c.mode & kFold) && (raw & 2u)) ? -base : base;
 }
@@ -179,9 +169,10 @@ PLH_DEV uint32_t round_pos(int s) {
 // First pass: the root bucket, ids enumerated implicitly (dense index i is
 // item i of the source), one digit position for the whole root.
 template <class Src>
-__global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense, int pos, uint32_t tile_items,
-                                                      uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense, const uint64_t* __restrict__ d_pos,
+                                                      uint32_t tile_items, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kDigits];
+    const int pos = (int)*d_pos;
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t t = blockIdx.x;
@@ -229,10 +220,11 @@ __global__ __launch_bounds__(kBlock) void k_root_colscan(uint64_t ntiles, uint32
 
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, uint32_t* __restrict__ ids_out, uint64_t n_dense,
-                                                         int pos, uint32_t tile_items,
+                                                         const uint64_t* __restrict__ d_pos, uint32_t tile_items,
                                                          const uint32_t* __restrict__ hist,
                                                          const uint64_t* __restrict__ sub_start) {
     __shared__ MsplitShared sh;
+    const int pos = (int)*d_pos;
     __shared__ uint64_t base[kDigits];
     const uint64_t t = blockIdx.x;
     const uint64_t beg = t * tile_items;
@@ -1005,6 +997,69 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
     }
 }
 
+// ------------------------------------------------------- live key bits
+// live[w] = OR over the items of key word w (a superset is fine: bits that
+// never vary cost at most a histogram pass).  The flags word's live bits
+// are preset by the mode; the rest is reduced on the device, nothing is
+// read back.
+__global__ void k_live_init(uint64_t* __restrict__ live, int nwords, int flags_word, uint64_t flags_bits) {
+    const int w = threadIdx.x;
+    if (w < nwords) live[w] = w == flags_word ? flags_bits : 0ull;
+}
+
+// One grid row per plane of a sum: x words go to key word xo + k, z words to
+// xo + AS + k, the job plane (when j is set) to key word 0.
+__global__ __launch_bounds__(kBlock) void k_live_or(SumView s, const uint32_t* __restrict__ j, int as, int key_as,
+                                                    int xo, int64_t n, unsigned long long* __restrict__ live) {
+    const int r = blockIdx.y;
+    uint64_t acc = 0;
+    int kw;
+    if (r < 2 * as) {
+        const bool isx = r < as;
+        const int k = isx ? r : r - as;
+        const uint64_t* p = (isx ? s.x : s.z) + (int64_t)k * s.stride;
+        for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
+            acc |= p[t];
+        kw = xo + (isx ? 0 : key_as) + k;
+    } else {
+        for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
+            acc |= j[t];
+        kw = 0;
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, kWave);
+    if (lane_id() == 0 && acc) atomicOr(&live[kw], (unsigned long long)acc);
+}
+
+// Root digit position: the first live bit (0 when none: all keys equal).
+__global__ void k_root_pos(const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ d_pos) {
+    const int pos = next_live(live, kb, 0);
+    *d_pos = (uint64_t)(pos >= kb ? 0 : pos);
+}
+
+// The root's children as the first bucket list (buffer 0), merged like
+// k_expand_write; also the list length and the number of valid items.
+__global__ void k_root_list(const uint64_t* __restrict__ sub_start, const uint64_t* __restrict__ sub_size,
+                            uint32_t cap, const uint64_t* __restrict__ live, int kb,
+                            const uint64_t* __restrict__ d_pos, uint64_t* __restrict__ Nstart,
+                            uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta,
+                            uint64_t* __restrict__ d_L, uint64_t* __restrict__ d_nvalid) {
+    const int pos = (int)*d_pos;
+    const int depth = min(pos + kDigitBits, kb);
+    const uint32_t uni = next_live(live, kb, depth) >= kb ? 1u : 0u;
+    uint64_t o = 0, nv = 0;
+    ChildWalk w{sub_start, sub_size, cap};
+    w.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
+        Nstart[o] = s0;
+        Nsize[o] = n;
+        Nmeta[o] = parts == 1 ? make_meta((uint32_t)depth, 0u, uni) : make_meta((uint32_t)pos, 0u, 0u);
+        ++o;
+        nv += n;
+    });
+    *d_L = o;
+    *d_nvalid = nv;
+}
+
 // ------------------------------------------------------------------ driver
 struct Lists {
     uint64_t* start;
@@ -1026,8 +1081,9 @@ int get_lists(plh_ctx* ctx, int slot, uint64_t cap, Lists* out) {
 inline unsigned grid1(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 template <int AS, class Src>
-int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* h_live,
-                const CollateSpec& spec, plh_sum* out, int out_as) {
+int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpec& spec, plh_sum* out,
+                int out_as) {
+    // The caller has filled the live mask: misc[0 .. kKW] of kWsMisc.
     const uint32_t mode = spec.mode;
     const int64_t max_weight = spec.max_weight;
     const double eps = spec.eps;
@@ -1048,10 +1104,17 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     PLH_TRY(ctx->ws.get(kWsIds1, n_dense * 4, &p));
     uint32_t* ids1 = (uint32_t*)p;
     PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
-    uint64_t* misc = (uint64_t*)p;  // [0..33] live, [40] total, [56..59] pass counts
+    uint64_t* misc = (uint64_t*)p;  // [0..33] live, [40] total, [56..61] pass counts, root position
     uint64_t* d_live = misc;
     uint64_t* d_total = misc + 40;
-    PLH_HIP(hipMemcpyAsync(d_live, h_live, sizeof(uint64_t) * kWordsK, hipMemcpyHostToDevice, st));
+    // pass bookkeeping read back once per pass: list length, big buckets,
+    // their items, valid items; then tiles and the root position
+    uint64_t* d_L = misc + 56;
+    uint64_t* d_nbig = misc + 57;
+    uint64_t* d_items = misc + 58;
+    uint64_t* d_nvalid = misc + 59;
+    uint64_t* d_T = misc + 60;
+    uint64_t* d_pos = misc + 61;
 
     Cfg cfg{};
     cfg.live = d_live;
@@ -1065,9 +1128,8 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     cfg.job_maxw = spec.job_maxw;
     cfg.job_alive = spec.job_alive;
 
-    // ---- pass 0: implicit ids, one root bucket.
-    int pos = next_live_host(h_live, kb, 0);
-    if (pos >= kb) pos = 0;  // no live bit: all keys equal, one uniform bucket
+    // ---- pass 0: implicit ids, one root bucket, position from the device.
+    k_root_pos<<<1, 1, 0, st>>>(d_live, kb, d_pos);
     const uint32_t tile0 = pick_tile(n_dense, kTileMaxImplicit);
     const uint64_t t0 = (n_dense + tile0 - 1) / tile0;
     PLH_TRY(ctx->ws.get(kWsHist, t0 * kDigits * 4, &p));
@@ -1077,7 +1139,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     uint64_t* sub_size = sub_start + kDigits;
     {
         ProfScope ps(ctx, "msd_hist_first");
-        k_root_hist<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, n_dense, pos, tile0, hist);
+        k_root_hist<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, n_dense, d_pos, tile0, hist);
     }
     {
         ProfScope ps(ctx, "colscan");
@@ -1085,42 +1147,16 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
     }
     {
         ProfScope ps(ctx, "msd_scatter_first", 4.0 * (double)n_dense);
-        k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, pos, tile0, hist, sub_start);
+        k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, d_pos, tile0, hist, sub_start);
     }
     PLH_HIP(hipGetLastError());
-    int depth = pos + kDigitBits;
-
-    // Initial list: the root's children (buffer 0), merged as in k_expand_write.
-    std::vector<uint64_t> hsz(kDigits), hst(kDigits);
-    PLH_HIP(hipMemcpyAsync(hsz.data(), sub_size, kDigits * 8, hipMemcpyDeviceToHost, st));
-    PLH_HIP(hipMemcpyAsync(hst.data(), sub_start, kDigits * 8, hipMemcpyDeviceToHost, st));
-    PLH_HIP(hipStreamSynchronize(st));
-    const uint32_t uni0 = next_live_host(h_live, kb, depth) >= kb ? 1u : 0u;
-    std::vector<uint64_t> ls, lz;
-    std::vector<uint32_t> lm;
-    ChildWalk w0{hst.data(), hsz.data(), cap};
-    w0.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
-        ls.push_back(s0);
-        lz.push_back(n);
-        lm.push_back(parts == 1 ? make_meta((uint32_t)depth, 0u, uni0) : make_meta((uint32_t)pos, 0u, 0u));
-    });
-    uint64_t n_valid = 0;
-    for (uint64_t v : lz) n_valid += v;
-    uint64_t L = ls.size();
-    if (L == 0) return PLH_OK;
-    // pass bookkeeping read back once per pass: list length, big buckets, their items, tiles
-    uint64_t* d_L = misc + 56;
-    uint64_t* d_nbig = misc + 57;
-    uint64_t* d_items = misc + 58;
-    uint64_t* d_T = misc + 59;
     Lists A, B;
-    PLH_TRY(get_lists(ctx, kWsListA, L, &A));
-    PLH_HIP(hipMemcpyAsync(A.start, ls.data(), L * 8, hipMemcpyHostToDevice, st));
-    PLH_HIP(hipMemcpyAsync(A.size, lz.data(), L * 8, hipMemcpyHostToDevice, st));
-    PLH_HIP(hipMemcpyAsync(A.meta, lm.data(), L * 4, hipMemcpyHostToDevice, st));
-    PLH_HIP(hipMemcpyAsync(d_L, &L, 8, hipMemcpyHostToDevice, st));
+    PLH_TRY(get_lists(ctx, kWsListA, kDigits, &A));
+    k_root_list<<<1, 1, 0, st>>>(sub_start, sub_size, cap, d_live, kb, d_pos, A.start, A.size, A.meta, d_L, d_nvalid);
+    PLH_HIP(hipGetLastError());
     int cur_slot = kWsListA;
-    uint64_t Lb = L;  // bound on the list length (exact on the first pass)
+    uint64_t Lb = kDigits;  // bound on the list length
+    uint64_t L = 0, n_valid = 0;
 
     // ---- global passes until every non-uniform bucket fits a leaf.  Grids
     // of the list kernels use the bound Lb; the exact counts come back in
@@ -1136,7 +1172,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
         k_classify<<<grid1(Lb), kBlock, 0, st>>>(A.size, A.meta, d_L, Lb, cap, isbig, bigsz);
         PLH_TRY(scan_u32_to_u64(ctx, isbig, bigpos, Lb, d_nbig));
         PLH_TRY(scan_u64(ctx, bigsz, newpos, Lb, d_items));
-        uint64_t info[3];
+        uint64_t info[4];
         // the big-bucket arrays are sized by the bound Lb before the counts are known
         PLH_TRY(ctx->ws.get(kWsPassB, Lb * 56 + 256, &p));
         PassBuckets PB;
@@ -1151,8 +1187,9 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
         PB.skip = PB.ntiles + Lb;
         k_gather_big<<<grid1(Lb), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, Lb, PB);
         PLH_HIP(hipGetLastError());
-        PLH_TRY(read_u64n(ctx, d_L, 3, info));
+        PLH_TRY(read_u64n(ctx, d_L, 4, info));
         L = info[0];
+        n_valid = info[3];
         const uint64_t nbig = info[1], pass_items = info[2];
         if (nbig == 0) break;
         const uint32_t tile = pick_tile(pass_items, kTileMaxExplicit);
@@ -1193,6 +1230,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const uint64_t* 
         Lb = Lb2;
     }
 
+    if (L == 0) return PLH_OK;  // no valid item (out->n is 0)
     // ---- leaves: sort, sum, filter, write at each leaf's start.
     PLH_TRY(sum_reserve(out, (int64_t)n_valid, 0));
     cfg.ox = out->x;
@@ -1278,6 +1316,30 @@ int padded_as(int as) {
     return -1;
 }
 
+// Live mask of a collation: zero every key word, preset the flags word.
+int live_init(plh_ctx* ctx, int key_words, uint32_t mode, uint64_t** live) {
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+    *live = (uint64_t*)p;
+    const uint64_t fbits = (mode & kFold) ? (1ull << 56) : (3ull << 56);
+    k_live_init<<<1, 64, 0, ctx->stream>>>(*live, key_words + 1, key_words, fbits);
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
+// OR a sum's planes into the live mask (x words at key word xo, z words at
+// xo + key_as, the job plane at key word 0 when the sum has one).
+int live_or(plh_ctx* ctx, const plh_sum* s, int key_as, int xo, uint64_t* live) {
+    if (s->n == 0) return PLH_OK;
+    const int rows = 2 * s->as + (s->has_job ? 1 : 0);
+    const unsigned gx = (unsigned)std::min<uint64_t>(grid1((uint64_t)s->n), 512);
+    k_live_or<<<dim3(gx, (unsigned)rows), kBlock, 0, ctx->stream>>>(SumView{s->x, s->z, s->f, s->v, s->cap},
+                                                                      s->has_job ? s->j : nullptr, s->as, key_as, xo,
+                                                                      s->n, (unsigned long long*)live);
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
 template <int AS, bool JOB>
 int collate_terms_as(plh_ctx* ctx, const plh_sum* in, const CollateSpec& spec, plh_sum* out) {
     TermSource<AS, JOB> src;
@@ -1289,17 +1351,10 @@ int collate_terms_as(plh_ctx* ctx, const plh_sum* in, const CollateSpec& spec, p
     src.n = (uint32_t)in->n;
     if ((uint64_t)in->n * TermSource<AS, JOB>::kRec * 8 <= kMaxPackBytes)
         PLH_TRY(pack_sum<AS>(ctx, in, kWsPackL, &src.rec));
-    uint64_t orw[40];
-    PLH_TRY(op_or_words(ctx, in, orw));
-    constexpr int XO = TermSource<AS, JOB>::kXOff;
-    uint64_t live[kMaxKeyWords + 1] = {0};
-    if (JOB) live[0] = orw[2 * in->as];
-    for (int k = 0; k < in->as; ++k) {
-        live[XO + k] = orw[k];
-        live[XO + AS + k] = orw[in->as + k];
-    }
-    live[XO + 2 * AS] = (spec.mode & kFold) ? (1ull << 56) : (3ull << 56);
-    return collate_run<AS>(ctx, src, (uint64_t)in->n, live, spec, out, in->as);
+    uint64_t* live;
+    PLH_TRY(live_init(ctx, TermSource<AS, JOB>::kKW, spec.mode, &live));
+    PLH_TRY(live_or(ctx, in, AS, TermSource<AS, JOB>::kXOff, live));
+    return collate_run<AS>(ctx, src, (uint64_t)in->n, spec, out, in->as);
 }
 
 template <int AS, int KIND>
@@ -1330,16 +1385,12 @@ int collate_pairs_as(plh_ctx* ctx, const plh_sum* Ls, const plh_sum* Rs, int by_
     src.n_hi = (uint32_t)n_hi;
     src.div_lo = make_fastdiv((uint32_t)n_lo);
     src.by_i = (uint32_t)by_i;
-    uint64_t ol[32], orr[32];
-    PLH_TRY(op_or_words(ctx, Ls, ol));
-    PLH_TRY(op_or_words(ctx, Rs, orr));
-    uint64_t live[kMaxKeyWords] = {0};
-    for (int k = 0; k < Ls->as; ++k) {
-        live[k] = ol[k] | orr[k];
-        live[AS + k] = ol[Ls->as + k] | orr[Ls->as + k];
-    }
-    live[2 * AS] = (spec.mode & kFold) ? (1ull << 56) : (3ull << 56);
-    return collate_run<AS>(ctx, src, n_lo * n_hi, live, spec, out, Ls->as);
+    // a product's words are XORs of the operands' words: OR(L) | OR(R) covers them
+    uint64_t* live;
+    PLH_TRY(live_init(ctx, 2 * AS, spec.mode, &live));
+    PLH_TRY(live_or(ctx, Ls, AS, 0, live));
+    PLH_TRY(live_or(ctx, Rs, AS, 0, live));
+    return collate_run<AS>(ctx, src, n_lo * n_hi, spec, out, Ls->as);
 }
 
 }  // namespace

@@ -204,7 +204,6 @@ int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz
                 double cos_t, double sin_t, int* appended_general);
 int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, double eps);
 int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out);
-int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or /* 2*as words (+ job plane) */);
 int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src);
 
 }  // namespace plh

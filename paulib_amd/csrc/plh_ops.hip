@@ -363,25 +363,6 @@ __global__ __launch_bounds__(kBlock) void k_zse_final(const double* __restrict__
     if (threadIdx.x == 0) *out = tot;
 }
 
-__global__ __launch_bounds__(kBlock) void k_or_words(const uint64_t* __restrict__ x,
-                                                     const uint64_t* __restrict__ z,
-                                                     const uint32_t* __restrict__ j, int64_t stride,
-                                                     int as, int64_t n, unsigned long long* __restrict__ out) {
-    const int w = blockIdx.y;  // 0..2*as-1: planes; 2*as: job plane
-    uint64_t acc = 0;
-    if (w < 2 * as) {
-        const uint64_t* p = (w < as ? x : z) + (int64_t)(w < as ? w : w - as) * stride;
-        for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
-            acc |= p[t];
-    } else {
-        for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
-            acc |= j[t];
-    }
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, kWave);
-    if (lane_id() == 0 && acc) atomicOr(&out[w], (unsigned long long)acc);
-}
-
 inline unsigned nblk(uint64_t n, uint64_t per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -614,24 +595,6 @@ int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out) {
     return PLH_OK;
 }
 
-int op_or_words(plh_ctx* ctx, const plh_sum* s, uint64_t* h_or) {
-    hipStream_t st = ctx->stream;
-    const int nw = 2 * s->as + (s->has_job ? 1 : 0);
-    for (int w = 0; w < nw; ++w) h_or[w] = 0;
-    if (s->n == 0) return PLH_OK;
-    void* p;
-    PLH_TRY(ctx->ws.get(kWsTmpSum, 64 * 8, &p));
-    unsigned long long* d = (unsigned long long*)p;
-    PLH_HIP(hipMemsetAsync(d, 0, (size_t)nw * 8, st));
-    const unsigned gx = (unsigned)std::min<uint64_t>(nblk((uint64_t)s->n, kBlock), 1024);
-    k_or_words<<<dim3(gx, (unsigned)nw), kBlock, 0, st>>>(s->x, s->z, s->j, s->cap, s->as, s->n, d);
-    PLH_HIP(hipGetLastError());
-    if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
-    PLH_HIP(hipMemcpyAsync(ctx->stage.p, d, (size_t)nw * 8, hipMemcpyDeviceToHost, st));
-    PLH_HIP(hipStreamSynchronize(st));
-    for (int w = 0; w < nw; ++w) h_or[w] = ctx->stage.p[w];
-    return PLH_OK;
-}
 
 int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src) {
     hipStream_t st = ctx->stream;
