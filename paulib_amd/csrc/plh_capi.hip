@@ -164,6 +164,19 @@ int rotation_step(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf
                   bool truncate, int64_t max_weight, double cutoff) {
     const double c = std::cos(theta), sn = std::sin(theta);
     const bool was_canonical = s->canonical;
+    if (rotation_kind(c, sn) == 4) {
+        // general path: evolve's new terms are produced inside the collation
+        // (append + sort_and_combine, and with distinct input keys also the
+        // truncation's canonicalize: every key then has at most two addends)
+        const uint32_t psf = (uint32_t)(pf ^ 1u);
+        if (truncate && was_canonical)
+            return collate_rotation(s->ctx, s, px, pz, psf, c, sn, trunc_spec(max_weight, cutoff), s);
+        CollateSpec sc;
+        PLH_TRY(collate_rotation(s->ctx, s, px, pz, psf, c, sn, sc, s));
+        s->canonical = false;
+        if (!truncate) return PLH_OK;
+        return collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s);
+    }
     int appended = 0;
     PLH_TRY(op_rotation(s->ctx, s, px, pz, pf, c, sn, &appended));
     if (!truncate) {

@@ -1031,6 +1031,15 @@ __global__ __launch_bounds__(kBlock) void k_live_or(SumView s, const uint32_t* _
     if (lane_id() == 0 && acc) atomicOr(&live[kw], (unsigned long long)acc);
 }
 
+struct LiveWords {
+    uint64_t w[kMaxKeyWords + 1];
+};
+
+__global__ void k_live_or_const(unsigned long long* __restrict__ live, LiveWords add, int nwords) {
+    const int w = threadIdx.x;
+    if (w < nwords && add.w[w]) atomicOr(&live[w], (unsigned long long)add.w[w]);
+}
+
 // Root digit position: the first live bit (0 when none: all keys equal).
 __global__ void k_root_pos(const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ d_pos) {
     const int pos = next_live(live, kb, 0);
@@ -1393,7 +1402,129 @@ int collate_pairs_as(plh_ctx* ctx, const plh_sum* Ls, const plh_sum* Rs, int by_
     return collate_run<AS>(ctx, src, n_lo * n_hi, spec, out, Ls->as);
 }
 
+// Collation of a sum after a rotation's general path (RotSource): 2n ids,
+// the new terms' bits are the old ones XOR P, so OR(sum) | P covers them.
+template <int AS, bool JOB>
+int collate_rotation_as(plh_ctx* ctx, const plh_sum* in, const RotParams<AS>& rp, const JobRotTables& jt,
+                        const uint64_t* p_or, const CollateSpec& spec, plh_sum* out) {
+    using S = RotSource<AS, JOB>;
+    S src;
+    src.s = SumView{in->x, in->z, in->f, in->v, in->cap};
+    src.j = in->j;
+    src.rec = nullptr;
+    src.as = in->as;
+    src.mode = spec.mode;
+    src.n = (uint32_t)in->n;
+    src.rp = rp;
+    src.jt = jt;
+    if ((uint64_t)in->n * S::kRec * 8 <= kMaxPackBytes) PLH_TRY(pack_sum<AS>(ctx, in, kWsPackL, &src.rec));
+    uint64_t* live;
+    PLH_TRY(live_init(ctx, S::kKW, spec.mode, &live));
+    PLH_TRY(live_or(ctx, in, AS, S::kXOff, live));
+    LiveWords add{};
+    for (int k = 0; k < 2 * AS; ++k) add.w[S::kXOff + k] = p_or[k];
+    k_live_or_const<<<1, 64, 0, ctx->stream>>>((unsigned long long*)live, add, S::kKW + 1);
+    PLH_HIP(hipGetLastError());
+    return collate_run<AS>(ctx, src, 2 * (uint64_t)in->n, spec, out, in->as);
+}
+
+// Runs f(dst) with dst = out, or a scratch sum when out aliases in, whose
+// storage then replaces in's.
+template <class F>
+int with_output(plh_ctx* ctx, plh_sum* in, plh_sum* out, F&& f) {
+    plh_sum tmp;
+    plh_sum* dst = out;
+    const bool alias = (out == in);
+    if (alias) {
+        tmp.ctx = ctx;
+        tmp.as = in->as;
+        tmp.nq = in->nq;
+        tmp.has_job = in->has_job;
+        dst = &tmp;
+    }
+    dst->as = in->as;
+    dst->nq = in->nq;
+    dst->has_job = in->has_job;
+    const int rc = f(dst);
+    if (rc != PLH_OK) {
+        if (alias) sum_free_storage(&tmp);
+        return rc;
+    }
+    dst->as = in->as;
+    dst->nq = in->nq;
+    if (alias) {
+        const int64_t n = tmp.n;
+        sum_take_storage(in, &tmp);
+        in->n = n;
+    }
+    return PLH_OK;
+}
+
 }  // namespace
+
+int collate_rotation(plh_ctx* ctx, plh_sum* in, const uint64_t* px, const uint64_t* pz, uint32_t psf, double c,
+                     double sn, const CollateSpec& spec, plh_sum* out) {
+    if (in->n > (int64_t)0x7FFFFFFFll) {
+        set_error("a fused rotation supports at most 2^31 - 1 terms");
+        return PLH_ERR_SIZE;
+    }
+    if (in->has_job) {
+        set_error("collate_rotation: batched sums rotate per job");
+        return PLH_ERR_ARG;
+    }
+    const int pa = padded_as(in->as);
+    uint64_t p_or[2 * 16] = {0};
+    int rc = with_output(ctx, in, out, [&](plh_sum* dst) -> int {
+        switch (pa) {
+#define PLH_ROT(ASV)                                                                              \
+    case ASV: {                                                                                   \
+        RotParams<ASV> rp{};                                                                      \
+        for (int k = 0; k < in->as; ++k) {                                                        \
+            rp.px[k] = px[k];                                                                     \
+            rp.pz[k] = pz[k];                                                                     \
+            p_or[k] = px[k];                                                                      \
+            p_or[ASV + k] = pz[k];                                                                \
+        }                                                                                         \
+        rp.psf = psf;                                                                             \
+        rp.c = c;                                                                                 \
+        rp.s = sn;                                                                                \
+        return collate_rotation_as<ASV, false>(ctx, in, rp, JobRotTables{}, p_or, spec, dst);   \
+    }
+            PLH_ROT(1)
+            PLH_ROT(2)
+            PLH_ROT(4)
+            PLH_ROT(8)
+            PLH_ROT(16)
+#undef PLH_ROT
+            default: set_error("unsupported array_size"); return PLH_ERR_AS;
+        }
+    });
+    if (rc != PLH_OK) return rc;
+    out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    return PLH_OK;
+}
+
+int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, const uint64_t* p_or,
+                          const CollateSpec& spec, plh_sum* out) {
+    if (in->n > (int64_t)0x7FFFFFFFll) {
+        set_error("a fused rotation supports at most 2^31 - 1 terms");
+        return PLH_ERR_SIZE;
+    }
+    if (!in->has_job) {
+        set_error("collate_rotation_jobs: the sum has no job plane");
+        return PLH_ERR_ARG;
+    }
+    const int pa = padded_as(in->as);
+    int rc = with_output(ctx, in, out, [&](plh_sum* dst) -> int {
+        if (pa == 1) return collate_rotation_as<1, true>(ctx, in, RotParams<1>{}, jt, p_or, spec, dst);
+        if (pa == 2) return collate_rotation_as<2, true>(ctx, in, RotParams<2>{}, jt, p_or, spec, dst);
+        set_error("batched sums support up to 128 qubits");
+        return PLH_ERR_AS;
+    });
+    if (rc != PLH_OK) return rc;
+    out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    return PLH_OK;
+}
 
 int collate_terms(plh_ctx* ctx, plh_sum* in, const CollateSpec& spec, plh_sum* out) {
     if (in->n > (int64_t)0xFFFFFFFFll) {

@@ -188,6 +188,17 @@ struct CollateSpec {
 };
 // out = collate(in); out may alias in (the result replaces it).
 int collate_terms(plh_ctx* ctx, plh_sum* in, const CollateSpec& spec, plh_sum* out);
+// out = collate(evolve's general path applied to in): the rotation by P
+// (psf = flags of P with the i bit flipped, c/sn its cos and sin) is folded
+// into the collation, nothing is materialised; out may alias in.
+int collate_rotation(plh_ctx* ctx, plh_sum* in, const uint64_t* px, const uint64_t* pz, uint32_t psf, double c,
+                     double sn, const CollateSpec& spec, plh_sum* out);
+// The same for a batched sum: each job rotates by its own P (device tables
+// by job slot; jobs whose kind is not 4 pass through); p_or = OR of every
+// job's P words (x then z, 2 * padded array size).
+struct JobRotTables;
+int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, const uint64_t* p_or,
+                          const CollateSpec& spec, plh_sum* out);
 // out = collate(L x R) for kind 0 (product, out[i*n+j] = L[j]*R[i]) or
 // kind 1 (commutator of L with R, by_i adds the i factor).
 int collate_pairs(plh_ctx* ctx, const plh_sum* L, const plh_sum* R, int kind, int by_i,
@@ -202,6 +213,9 @@ int op_commutator_raw(plh_ctx* ctx, const plh_sum* a, const plh_sum* b, int by_i
 int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1);
 int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pflags,
                 double cos_t, double sin_t, int* appended_general);
+// evolve's path for (cos, sin) (include/PauliSumOperations.h:121-180): 0 no
+// change, 1 sign flip, 2 sign flip and * ps, 3 * ps, 4 general (new terms).
+int rotation_kind(double cos_t, double sin_t);
 int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, double eps);
 int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out);
 int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src);

@@ -456,6 +456,14 @@ int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1) {
 
 static bool close_one(double v) { return std::fabs(v - 1.0) <= 1e-14; }  // isCloseOne, :19-21
 
+int rotation_kind(double cos_t, double sin_t) {
+    if (close_one(cos_t)) return 0;
+    if (close_one(-cos_t)) return 1;
+    if (close_one(sin_t)) return 2;
+    if (close_one(-sin_t)) return 3;
+    return 4;
+}
+
 int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pflags,
                 double cos_t, double sin_t, int* appended_general) {
     hipStream_t st = ctx->stream;
@@ -468,12 +476,8 @@ int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz
     r.psf = (uint32_t)(pflags ^ 1u);
     r.c = cos_t;
     r.s = sin_t;
-    int kind;
-    if (close_one(cos_t)) return PLH_OK;
-    else if (close_one(-cos_t)) kind = 1;
-    else if (close_one(sin_t)) kind = 2;
-    else if (close_one(-sin_t)) kind = 3;
-    else kind = 4;
+    const int kind = rotation_kind(cos_t, sin_t);
+    if (kind == 0) return PLH_OK;
     if (s->n == 0) return PLH_OK;
     const int pa = padded(s->as);
     if (pa < 0) return PLH_ERR_AS;

@@ -116,23 +116,20 @@ PLH_DEV uint32_t times_p(uint64_t* x, uint64_t* z, uint32_t f, const uint64_t* p
     return phase_of(f, psf, c1, c2);
 }
 
-// Clifford ops, then the fast-path rotations (:121-147) in place; counts
-// per tile the terms the general path will append.
+// Clifford ops, then the fast-path rotations (:121-147) in place; the
+// general path (kind 4) happens inside the step's collation.
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_batch_gates(uint64_t* __restrict__ X, uint64_t* __restrict__ Z,
                                                         uint8_t* __restrict__ F, const uint32_t* __restrict__ J,
-                                                        int64_t stride, int64_t n, StepTables tb,
-                                                        uint32_t* __restrict__ cnt) {
-    __shared__ uint32_t red[kWavesPerBlock];
+                                                        int64_t stride, int64_t n, StepTables tb) {
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
-    uint32_t c = 0;
     for (int k = 0; k < kIpt; ++k) {
         const uint64_t t = base + (uint64_t)k * kBlock + threadIdx.x;
         if (t >= (uint64_t)n) break;
         const uint32_t job = J[t];
         const uint32_t o0 = tb.cl_off[job], o1 = tb.cl_off[job + 1];
         const uint32_t kind = tb.rot_kind[job];
-        if (o0 == o1 && kind == 0) continue;
+        if (o0 == o1 && (kind == 0 || kind == 4)) continue;
         uint64_t x[AS], z[AS];
 #pragma unroll
         for (int w = 0; w < AS; ++w) {
@@ -141,15 +138,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_gates(uint64_t* __restrict__ X
         }
         uint32_t f = F[t];
         for (uint32_t o = o0; o < o1; ++o) apply_clifford_op<AS>(tb.cl_op[o], x, z, f);
-        if (kind != 0) {
+        if (kind >= 1 && kind <= 3) {
             const uint64_t* p = tb.rot_p + (size_t)job * 2 * AS;
             if (anticommutes_p<AS>(x, z, p)) {
-                if (kind == 4) {
-                    ++c;
-                } else {
-                    if (kind == 1 || kind == 2) f ^= 2u;
-                    if (kind >= 2) f = times_p<AS>(x, z, f, p, tb.rot_psf[job]);
-                }
+                if (kind == 1 || kind == 2) f ^= 2u;
+                if (kind >= 2) f = times_p<AS>(x, z, f, p, tb.rot_psf[job]);
             }
         }
 #pragma unroll
@@ -158,54 +151,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_gates(uint64_t* __restrict__ X
             Z[(int64_t)w * stride + t] = z[w];
         }
         F[t] = (uint8_t)f;
-    }
-    const uint32_t s = block_sum(c, red);
-    if (threadIdx.x == 0) cnt[blockIdx.x] = s;
-}
-
-// General path (:148-180): an anticommuting term t of a kind-4 job keeps
-// cos * v in place and term * ps with (-sin) * v is appended at n + rank;
-// ranks follow the tile's (k, thread) order, so the result is deterministic.
-template <int AS>
-__global__ __launch_bounds__(kBlock) void k_batch_rot_general(uint64_t* __restrict__ X, uint64_t* __restrict__ Z,
-                                                              uint8_t* __restrict__ F, double* __restrict__ V,
-                                                              uint32_t* __restrict__ J, int64_t stride, int64_t n,
-                                                              StepTables tb, const uint64_t* __restrict__ boff) {
-    __shared__ uint32_t red[kWavesPerBlock];
-    const uint64_t base = (uint64_t)blockIdx.x * kTile;
-    uint64_t o_base = (uint64_t)n + boff[blockIdx.x];
-    for (int k = 0; k < kIpt; ++k) {
-        const uint64_t t = base + (uint64_t)k * kBlock + threadIdx.x;
-        uint32_t hit = 0, job = 0;
-        uint64_t x[AS], z[AS];
-        if (t < (uint64_t)n) {
-            job = J[t];
-            if (tb.rot_kind[job] == 4) {
-#pragma unroll
-                for (int w = 0; w < AS; ++w) {
-                    x[w] = X[(int64_t)w * stride + t];
-                    z[w] = Z[(int64_t)w * stride + t];
-                }
-                hit = anticommutes_p<AS>(x, z, tb.rot_p + (size_t)job * 2 * AS) ? 1u : 0u;
-            }
-        }
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(hit, tot, red);
-        if (hit) {
-            const uint64_t o = o_base + ex;
-            const double orig = V[t];
-            V[t] = orig * tb.rot_c[job];
-            const uint32_t f = times_p<AS>(x, z, F[t], tb.rot_p + (size_t)job * 2 * AS, tb.rot_psf[job]);
-#pragma unroll
-            for (int w = 0; w < AS; ++w) {
-                X[(int64_t)w * stride + o] = x[w];
-                Z[(int64_t)w * stride + o] = z[w];
-            }
-            F[o] = (uint8_t)f;
-            V[o] = (-tb.rot_s[job]) * orig;
-            J[o] = job;
-        }
-        o_base += tot;
     }
 }
 
@@ -520,6 +465,8 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
     std::vector<int64_t> maxw((size_t)S, -1);
     std::vector<uint64_t> rp((size_t)S * 2 * AS, 0);
     std::vector<int> finishing;
+    std::vector<uint64_t> p_or((size_t)2 * AS, 0);
+    bool any_general = false;
     for (int i = 0; i < S; ++i) {
         off[(size_t)i] = (uint32_t)ops.size();
         if (s->state[(size_t)i] != 1) continue;
@@ -539,6 +486,13 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
             else if (close_one(sn)) kind = 2;
             else if (close_one(-sn)) kind = 3;
             rk[(size_t)i] = kind;  // program generators carry flags 0, so ps has flags 1
+            if (kind == 4) {
+                any_general = true;
+                for (int k = 0; k < j.job_as; ++k) {
+                    p_or[(size_t)k] |= j.px[(size_t)(l * j.job_as + k)];
+                    p_or[(size_t)AS + k] |= j.pz[(size_t)(l * j.job_as + k)];
+                }
+            }
             rc[(size_t)i] = c;
             rs[(size_t)i] = sn;
             for (int k = 0; k < j.job_as; ++k) {
@@ -594,39 +548,18 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
 
     const int64_t n = B->n;
     if (n > 0) {
-        // ---- gates: Cliffords + fast rotations in place; count the general appends
+        // ---- gates: Cliffords + fast rotations in place
         const uint64_t nb = blocks_for((uint64_t)n, kTile);
-        void* p;
-        PLH_TRY(ctx->ws.get(kWsScan0, nb * 4 + 64, &p));
-        uint32_t* cnt = (uint32_t*)p;
-        PLH_TRY(ctx->ws.get(kWsScan1, nb * 8 + 64, &p));
-        uint64_t* boff = (uint64_t*)p;
-        PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
-        uint64_t* d_total = (uint64_t*)p + 40;
         {
             ProfScope ps(ctx, "batch_gates", (double)n * (2.0 * (16.0 * AS + 1.0) + 4.0));
             if (AS == 1)
-                k_batch_gates<1><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb, cnt);
+                k_batch_gates<1><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb);
             else
-                k_batch_gates<2><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb, cnt);
+                k_batch_gates<2><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb);
             PLH_HIP(hipGetLastError());
         }
-        PLH_TRY(scan_u32_to_u64(ctx, cnt, boff, nb, d_total));
-        uint64_t k = 0;
-        PLH_TRY(read_u64(ctx, d_total, &k));
-        if (k > 0) {
-            PLH_TRY(sum_reserve(B, n + (int64_t)k, n));
-            ProfScope ps(ctx, "batch_rot_general", (double)n * (16.0 * AS + 4.0) + (double)k * (16.0 * AS + 21.0));
-            if (AS == 1)
-                k_batch_rot_general<1><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->v, B->j, B->cap,
-                                                                             n, tb, boff);
-            else
-                k_batch_rot_general<2><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->v, B->j, B->cap,
-                                                                             n, tb, boff);
-            PLH_HIP(hipGetLastError());
-            B->n = n + (int64_t)k;
-        }
-        // ---- one collation for every job: fold, combine, zero-drop, per-job filters, drained jobs out
+        // ---- one collation for every job: the general rotations' new terms,
+        // fold, combine, zero-drop, per-job filters, drained jobs out
         CollateSpec sc;
         sc.mode = kFold | kDropZero | kMaxWeight | kCutoff;
         sc.max_weight = -1;
@@ -634,7 +567,12 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
         sc.job_eps = tb.eps;
         sc.job_maxw = tb.maxw;
         sc.job_alive = tb.alive;
-        PLH_TRY(collate_terms(ctx, B, sc, B));
+        if (any_general) {
+            JobRotTables jt{tb.rot_kind, tb.rot_p, tb.rot_psf, tb.rot_c, tb.rot_s};
+            PLH_TRY(collate_rotation_jobs(ctx, B, jt, p_or.data(), sc, B));
+        } else {
+            PLH_TRY(collate_terms(ctx, B, sc, B));
+        }
     }
     for (int i = 0; i < S; ++i)
         if (s->state[(size_t)i] == 2) s->state[(size_t)i] = 0;  // their terms left in this collation

@@ -99,6 +99,187 @@ struct TermSource {
     }
 };
 
+// One rotation exp(-i theta/2 P) on the general path of evolve
+// (include/PauliSumOperations.h:148-180): ps = P with the i bit flipped,
+// c = cos(theta), s = sin(theta) (full angle, from the host).
+template <int AS>
+struct RotParams {
+    uint64_t px[AS], pz[AS];
+    uint32_t psf;  // flags of ps
+    double c, s;
+};
+
+// Per-job rotations of a batched sum (device tables indexed by job slot):
+// kind 4 = general path this step; P as x[AS] then z[AS] per slot.
+struct JobRotTables {
+    const uint8_t* kind;
+    const uint64_t* p;
+    const uint8_t* psf;
+    const double* c;
+    const double* s;
+};
+
+// A sum after evolve's general path, without materialising the new terms.
+// id < n: term id, its value times c when it anticommutes with P; id >= n:
+// the new term (term id - n) * ps with value (-s) * v, valid only when that
+// term anticommutes with P.  The numeric id order is the reference's order
+// after the new terms are appended (old terms, then new ones in old-index
+// order), so sort_and_combine over these ids sums exactly as it does.
+// JOB: a batched sum; each term rotates by its job's P when that job's
+// rotation is on the general path this step, else it passes through.
+template <int AS, bool JOB = false>
+struct RotSource {
+    static constexpr int kAS = AS;
+    static constexpr int kXOff = JOB ? 1 : 0;
+    static constexpr int kKW = 2 * AS + kXOff;
+    static constexpr int kRec = 2 * AS + 2;
+    SumView s;
+    const uint32_t* j;    // job plane (JOB only)
+    const uint64_t* rec;  // the old terms as records (x[AS], z[AS], value, flags | job << 32)
+    int as;
+    uint32_t mode;
+    uint32_t n;           // old terms; ids run over [0, 2n)
+    RotParams<AS> rp;     // single sum
+    JobRotTables jt;      // batched sum
+
+    PLH_DEV uint32_t dense_to_id(uint32_t t) const { return t; }
+    // The rotation that applies to old term t: false when none (batched sums).
+    PLH_DEV bool rot_of(uint32_t t, const uint64_t*& px, const uint64_t*& pz, uint32_t& psf, double& c,
+                        double& sn) const {
+        if (!JOB) {
+            px = rp.px;
+            pz = rp.pz;
+            psf = rp.psf;
+            c = rp.c;
+            sn = rp.s;
+            return true;
+        }
+        const uint32_t job = rec ? (uint32_t)(rec[(size_t)t * kRec + kRec - 1] >> 32) : j[t];
+        if (jt.kind[job] != 4) return false;
+        px = jt.p + (size_t)job * 2 * AS;
+        pz = px + AS;
+        psf = jt.psf[job];
+        c = jt.c[job];
+        sn = jt.s[job];
+        return true;
+    }
+    PLH_DEV void load(uint32_t t, uint64_t* x, uint64_t* z, uint32_t& fl, double& v, uint32_t& job) const {
+        if (rec) {
+            const uint64_t* r = rec + (size_t)t * kRec;
+#pragma unroll
+            for (int k = 0; k < AS; ++k) {
+                x[k] = r[k];
+                z[k] = r[AS + k];
+            }
+            v = __longlong_as_double((long long)r[2 * AS]);
+            fl = (uint32_t)r[kRec - 1] & 0xFFu;
+            job = (uint32_t)(r[kRec - 1] >> 32);
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            x[k] = k < as ? s.x[(int64_t)k * s.stride + t] : 0ull;
+            z[k] = k < as ? s.z[(int64_t)k * s.stride + t] : 0ull;
+        }
+        v = s.v[t];
+        fl = s.f[t];
+        job = JOB ? j[t] : 0u;
+    }
+    PLH_DEV static bool anti(const uint64_t* x, const uint64_t* z, const uint64_t* px, const uint64_t* pz) {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) acc ^= (x[k] & pz[k]) ^ (z[k] & px[k]);
+        return popc(acc) & 1;
+    }
+    // The item's key words (x, z) and raw flags, and its coefficient before
+    // the sign fold; false for a new-term id whose old term commutes.
+    PLH_DEV bool item(uint32_t id, uint64_t* x, uint64_t* z, uint32_t& fl, double& v, uint32_t& job) const {
+        const bool nw = id >= n;
+        const uint32_t t = nw ? id - n : id;
+        load(t, x, z, fl, v, job);
+        const uint64_t *px, *pz;
+        uint32_t psf;
+        double c, sn;
+        if (!rot_of(t, px, pz, psf, c, sn)) return !nw;
+        const bool a = anti(x, z, px, pz);
+        if (!nw) {
+            if (a) v = v * c;
+            return true;
+        }
+        if (!a) return false;
+        uint64_t c1 = 0, c2 = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            phase_word(x[k], z[k], px[k], pz[k], c1, c2);
+            x[k] ^= px[k];
+            z[k] ^= pz[k];
+        }
+        fl = phase_of(fl, psf, c1, c2);
+        v = (-sn) * v;
+        return true;
+    }
+    PLH_DEV bool valid(uint32_t id) const {
+        if (id < n) return true;
+        uint64_t x[AS], z[AS];
+        uint32_t fl, job;
+        double v;
+        return item(id, x, z, fl, v, job);
+    }
+    PLH_DEV uint64_t word(uint32_t id, int w) const {
+        if (JOB && w == 0) {
+            const uint32_t t = id >= n ? id - n : id;
+            return j[t];
+        }
+        const int kw = w - kXOff;
+        if (kw > 2 * AS) return 0ull;
+        const bool nw = id >= n;
+        const uint32_t t = nw ? id - n : id;
+        if (kw < 2 * AS) {
+            const bool isx = kw < AS;
+            const int k = isx ? kw : kw - AS;
+            uint64_t v = k < as ? (isx ? s.x : s.z)[(int64_t)k * s.stride + t] : 0ull;
+            if (nw) {
+                const uint64_t *px, *pz;
+                uint32_t psf;
+                double c, sn;
+                if (rot_of(t, px, pz, psf, c, sn)) v ^= isx ? px[k] : pz[k];
+            }
+            return v;
+        }
+        uint64_t x[AS], z[AS];
+        uint32_t fl, job;
+        double v;
+        item(id, x, z, fl, v, job);
+        return (uint64_t)((mode & kFold) ? (fl & 1u) : fl) << 56;
+    }
+    PLH_DEV void key(uint32_t id, uint64_t* kw, uint32_t& fl) const {
+        uint64_t x[AS], z[AS];
+        uint32_t job;
+        double v;
+        item(id, x, z, fl, v, job);
+        if (JOB) kw[0] = job;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            kw[kXOff + k] = x[k];
+            kw[kXOff + AS + k] = z[k];
+        }
+    }
+    PLH_DEV uint32_t raw_flags(uint32_t id) const {
+        uint64_t x[AS], z[AS];
+        uint32_t fl, job;
+        double v;
+        item(id, x, z, fl, v, job);
+        return fl;
+    }
+    PLH_DEV double base_value(uint32_t id) const {
+        uint64_t x[AS], z[AS];
+        uint32_t fl, job;
+        double v;
+        item(id, x, z, fl, v, job);
+        return v;
+    }
+};
+
 // A term packed as one record of 2*AS + 2 words: x[AS], z[AS], value,
 // flags -- a pair's key and coefficient come from two records (two cache
 // lines) instead of eight plane loads.
