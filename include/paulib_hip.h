@@ -82,6 +82,12 @@ int plh_sum_upload(plh_sum* s, int64_t n, const uint64_t* x, const uint64_t* z,
                    const uint8_t* flags, const double* val);
 /* GpuPauliSum::download (:95-114); arrays sized for plh_sum_length(). */
 int plh_sum_download(const plh_sum* s, uint64_t* x, uint64_t* z, uint8_t* flags, double* val);
+/* Terms [offset, offset + count) only, planes x[k*count + t]; any output may
+ * be null to skip it.  The reference's element accessors
+ * (get_pauli_string(i), value[i], include/PauliSumCore.h:20-134) for a
+ * device sum too large to download whole. */
+int plh_sum_download_range(const plh_sum* s, int64_t offset, int64_t count, uint64_t* x, uint64_t* z,
+                           uint8_t* flags, double* val);
 int64_t plh_sum_length(const plh_sum* s);
 int64_t plh_sum_num_qubits(const plh_sum* s);
 int plh_sum_set_num_qubits(plh_sum* s, int64_t num_qubits);

@@ -56,6 +56,7 @@ SIGNATURES = {
     "plh_sum_destroy": (_int, [_p]),
     "plh_sum_upload": (_int, [_p, _i64, _u64p, _u64p, _u8p, _f64p]),
     "plh_sum_download": (_int, [_p, _u64p, _u64p, _u8p, _f64p]),
+    "plh_sum_download_range": (_int, [_p, _i64, _i64, _u64p, _u64p, _u8p, _f64p]),
     "plh_sum_length": (_i64, [_p]),
     "plh_sum_num_qubits": (_i64, [_p]),
     "plh_sum_set_num_qubits": (_int, [_p, _i64]),

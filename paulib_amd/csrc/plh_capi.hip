@@ -409,6 +409,29 @@ int plh_sum_download(const plh_sum* s, uint64_t* x, uint64_t* z, uint8_t* flags,
     return PLH_OK;
 }
 
+int plh_sum_download_range(const plh_sum* s, int64_t offset, int64_t count, uint64_t* x, uint64_t* z,
+                           uint8_t* flags, double* val) {
+    PLH_TRY(check_sum(s));
+    if (offset < 0 || count < 0 || offset > s->n || count > s->n - offset) {
+        set_error("plh_sum_download_range: range outside the sum");
+        return PLH_ERR_ARG;
+    }
+    if (count == 0) return PLH_OK;
+    hipStream_t st = s->ctx->stream;
+    for (int k = 0; k < s->as; ++k) {
+        if (x)
+            PLH_HIP(hipMemcpyAsync(x + (int64_t)k * count, s->x + (int64_t)k * s->cap + offset, (size_t)count * 8,
+                                   hipMemcpyDeviceToHost, st));
+        if (z)
+            PLH_HIP(hipMemcpyAsync(z + (int64_t)k * count, s->z + (int64_t)k * s->cap + offset, (size_t)count * 8,
+                                   hipMemcpyDeviceToHost, st));
+    }
+    if (flags) PLH_HIP(hipMemcpyAsync(flags, s->f + offset, (size_t)count, hipMemcpyDeviceToHost, st));
+    if (val) PLH_HIP(hipMemcpyAsync(val, s->v + offset, (size_t)count * 8, hipMemcpyDeviceToHost, st));
+    PLH_HIP(hipStreamSynchronize(st));
+    return PLH_OK;
+}
+
 int64_t plh_sum_length(const plh_sum* s) { return s ? s->n : -1; }
 int64_t plh_sum_num_qubits(const plh_sum* s) { return s ? s->nq : -1; }
 int plh_sum_set_num_qubits(plh_sum* s, int64_t q) {

@@ -166,13 +166,19 @@ PLH_DEV uint32_t round_pos(int s) {
 }
 
 // --------------------------------------------------------- global passes
+// Root digit position: the first live bit (0 when none: all keys equal).
+PLH_DEV int root_pos(const uint64_t* live, int kb) {
+    const int pos = next_live(live, kb, 0);
+    return pos >= kb ? 0 : pos;
+}
+
 // First pass: the root bucket, ids enumerated implicitly (dense index i is
 // item i of the source), one digit position for the whole root.
 template <class Src>
-__global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense, const uint64_t* __restrict__ d_pos,
-                                                      uint32_t tile_items, uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense, const uint64_t* __restrict__ live,
+                                                      int kb, uint32_t tile_items, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kDigits];
-    const int pos = (int)*d_pos;
+    const int pos = root_pos(live, kb);
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t t = blockIdx.x;
@@ -220,11 +226,11 @@ __global__ __launch_bounds__(kBlock) void k_root_colscan(uint64_t ntiles, uint32
 
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, uint32_t* __restrict__ ids_out, uint64_t n_dense,
-                                                         const uint64_t* __restrict__ d_pos, uint32_t tile_items,
-                                                         const uint32_t* __restrict__ hist,
+                                                         const uint64_t* __restrict__ live, int kb,
+                                                         uint32_t tile_items, const uint32_t* __restrict__ hist,
                                                          const uint64_t* __restrict__ sub_start) {
     __shared__ MsplitShared sh;
-    const int pos = (int)*d_pos;
+    const int pos = root_pos(live, kb);
     __shared__ uint64_t base[kDigits];
     const uint64_t t = blockIdx.x;
     const uint64_t beg = t * tile_items;
@@ -473,27 +479,62 @@ struct ChildWalk {
     }
 };
 
-__global__ void k_expand_count(const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos,
-                               PassBuckets B, const uint64_t* __restrict__ sub_start,
-                               const uint64_t* __restrict__ sub_size, uint64_t L, uint32_t cap,
-                               uint32_t* __restrict__ cnt) {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= L) return;
-    if (!isbig[e] || B.skip[bigpos[e]]) { cnt[e] = 1; return; }
-    ChildWalk w{sub_start + bigpos[e] * kDigits, sub_size + bigpos[e] * kDigits, cap};
-    cnt[e] = w.walk([](uint64_t, uint64_t, uint32_t) {});
+// Children of each big bucket, from its 256 sub-buckets staged in LDS (one
+// workgroup per bucket; the walk is sequential, so it runs on LDS, not on
+// global loads): at most kDigits children at Cstart/Csize/Cmeta[b * kDigits],
+// their count in ccount[b].  A bucket whose digit was constant has one
+// child: itself, deeper, in the same id buffer.
+struct Children {
+    uint64_t* start;
+    uint64_t* size;
+    uint32_t* meta;
+    uint32_t* count;
+};
+
+__global__ __launch_bounds__(kDigits) void k_walk(PassBuckets B, const uint64_t* __restrict__ sub_start,
+                                                  const uint64_t* __restrict__ sub_size, uint32_t cap,
+                                                  const uint64_t* __restrict__ live, int kb, Children C) {
+    __shared__ uint64_t st[kDigits], ss[kDigits];
+    const uint64_t b = blockIdx.x;
+    const uint32_t pos = B.pos[b];
+    const uint32_t skip = B.skip[b];
+    const uint64_t base = b * kDigits;
+    if (skip) {
+        if (threadIdx.x == 0) {
+            const int depth = min((int)(pos + skip), kb);
+            C.start[base] = B.start[b];
+            C.size[base] = B.size[b];
+            C.meta[base] = make_meta((uint32_t)depth, meta_buf(B.meta[b]), next_live(live, kb, depth) >= kb ? 1u : 0u);
+            C.count[b] = 1;
+        }
+        return;
+    }
+    st[threadIdx.x] = sub_start[base + threadIdx.x];
+    ss[threadIdx.x] = sub_size[base + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const uint32_t buf = meta_buf(B.meta[b]) ^ 1u;
+    const int child_depth = min((int)pos + kDigitBits, kb);
+    const uint32_t child_uniform = next_live(live, kb, child_depth) >= kb ? 1u : 0u;
+    uint64_t o = base;
+    ChildWalk w{st, ss, cap};
+    w.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
+        C.start[o] = s0;
+        C.size[o] = n;
+        C.meta[o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
+        ++o;
+    });
+    C.count[b] = (uint32_t)(o - base);
 }
 
-__global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize,
-                               const uint32_t* __restrict__ Lmeta, const uint32_t* __restrict__ isbig,
-                               const uint64_t* __restrict__ bigpos, PassBuckets B,
-                               const uint64_t* __restrict__ sub_start, const uint64_t* __restrict__ sub_size,
-                               const uint64_t* __restrict__ newpos, uint64_t L, uint32_t cap,
-                               const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ Nstart,
-                               uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta) {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= L) return;
-    uint64_t o = newpos[e];
+PLH_DEV uint32_t expand_count_one(uint64_t e, const uint32_t* isbig, const uint64_t* bigpos, const Children& C) {
+    return isbig[e] ? C.count[bigpos[e]] : 1u;
+}
+
+// Writes entry e's children at o (the next pass's list).
+PLH_DEV void expand_write_one(uint64_t e, uint64_t o, const uint64_t* Lstart, const uint64_t* Lsize,
+                              const uint32_t* Lmeta, const uint32_t* isbig, const uint64_t* bigpos,
+                              const Children& C, uint64_t* Nstart, uint64_t* Nsize, uint32_t* Nmeta) {
     if (!isbig[e]) {
         Nstart[o] = Lstart[e];
         Nsize[o] = Lsize[e];
@@ -501,26 +542,129 @@ __global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64
         return;
     }
     const uint64_t b = bigpos[e];
-    const uint32_t pos = B.pos[b];
-    const uint32_t skip = B.skip[b];
-    if (skip) {
-        // the digit is constant: same ids, same buffer, deeper
-        const int depth = min((int)(pos + skip), kb);
-        Nstart[o] = Lstart[e];
-        Nsize[o] = Lsize[e];
-        Nmeta[o] = make_meta((uint32_t)depth, meta_buf(Lmeta[e]), next_live(live, kb, depth) >= kb ? 1u : 0u);
-        return;
+    const uint32_t k = C.count[b];
+    for (uint32_t c = 0; c < k; ++c) {
+        Nstart[o + c] = C.start[b * kDigits + c];
+        Nsize[o + c] = C.size[b * kDigits + c];
+        Nmeta[o + c] = C.meta[b * kDigits + c];
     }
-    const uint32_t buf = meta_buf(Lmeta[e]) ^ 1u;
-    const int child_depth = min((int)pos + kDigitBits, kb);
-    const uint32_t child_uniform = next_live(live, kb, child_depth) >= kb ? 1u : 0u;
-    ChildWalk w{sub_start + b * kDigits, sub_size + b * kDigits, cap};
-    w.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
-        Nstart[o] = s0;
-        Nsize[o] = n;
-        Nmeta[o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
-        ++o;
-    });
+}
+
+__global__ void k_expand_count(const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos, Children C,
+                               uint64_t L, uint32_t* __restrict__ cnt) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < L) cnt[e] = expand_count_one(e, isbig, bigpos, C);
+}
+
+__global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize,
+                               const uint32_t* __restrict__ Lmeta, const uint32_t* __restrict__ isbig,
+                               const uint64_t* __restrict__ bigpos, Children C,
+                               const uint64_t* __restrict__ newpos, uint64_t L, uint64_t* __restrict__ Nstart,
+                               uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < L) expand_write_one(e, newpos[e], Lstart, Lsize, Lmeta, isbig, bigpos, C, Nstart, Nsize, Nmeta);
+}
+
+// ---- short lists: one workgroup does a list step that otherwise takes a
+// kernel, a scan (one to three launches) and another kernel.
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanWaves = kPlanThreads / kWave;
+constexpr int kPlanPer = 8;
+constexpr uint64_t kPlanMax = (uint64_t)kPlanThreads * kPlanPer;  // list entries
+
+// classify + scan + gather of the big buckets; entry e is thread e / 8's.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __restrict__ Lstart,
+                                                             const uint64_t* __restrict__ Lsize,
+                                                             const uint32_t* __restrict__ Lmeta,
+                                                             const uint64_t* __restrict__ dL, uint32_t cap,
+                                                             uint32_t* __restrict__ isbig, uint64_t* __restrict__ bigpos,
+                                                             PassBuckets B, uint64_t* __restrict__ d_nbig,
+                                                             uint64_t* __restrict__ d_items) {
+    __shared__ uint64_t sh[kPlanWaves];
+    const uint64_t L = *dL;
+    const uint64_t e0 = (uint64_t)threadIdx.x * kPlanPer;
+    uint32_t big[kPlanPer];
+    uint64_t cnt = 0, items = 0;
+#pragma unroll
+    for (int k = 0; k < kPlanPer; ++k) {
+        const uint64_t e = e0 + k;
+        big[k] = (e < L && Lsize[e] > cap && !meta_uniform(Lmeta[e])) ? 1u : 0u;
+        cnt += big[k];
+        items += big[k] ? Lsize[e] : 0ull;
+    }
+    uint64_t nbig, tot_items;
+    uint64_t off = block_exclusive_scan<kPlanWaves>(cnt, nbig, sh);
+    (void)block_exclusive_scan<kPlanWaves>(items, tot_items, sh);
+#pragma unroll
+    for (int k = 0; k < kPlanPer; ++k) {
+        const uint64_t e = e0 + k;
+        if (e >= L) break;
+        isbig[e] = big[k];
+        bigpos[e] = off;
+        if (big[k]) {
+            B.start[off] = Lstart[e];
+            B.size[off] = Lsize[e];
+            B.meta[off] = Lmeta[e];
+            ++off;
+        }
+    }
+    if (threadIdx.x == 0) {
+        *d_nbig = nbig;
+        *d_items = tot_items;
+    }
+}
+
+// bucket setup + scan of the tile counts.
+__global__ __launch_bounds__(kPlanThreads) void k_setup_small(PassBuckets B, uint64_t nbig, uint32_t tile,
+                                                              const uint64_t* __restrict__ live, int kb,
+                                                              uint64_t* __restrict__ d_T) {
+    __shared__ uint64_t sh[kPlanWaves];
+    const uint64_t b0 = (uint64_t)threadIdx.x * kPlanPer;
+    uint32_t nt[kPlanPer];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kPlanPer; ++k) {
+        const uint64_t b = b0 + k;
+        nt[k] = 0;
+        if (b < nbig) {
+            B.pos[b] = (uint32_t)next_live(live, kb, (int)meta_depth(B.meta[b]));
+            nt[k] = (uint32_t)((B.size[b] + tile - 1) / tile);
+            B.ntiles[b] = nt[k];
+            B.kor[b] = 0ull;
+            B.kand[b] = ~0ull;
+        }
+        acc += nt[k];
+    }
+    uint64_t T;
+    uint64_t run = block_exclusive_scan<kPlanWaves>(acc, T, sh);
+#pragma unroll
+    for (int k = 0; k < kPlanPer; ++k) {
+        const uint64_t b = b0 + k;
+        if (b < nbig) B.tbase[b] = run;
+        run += nt[k];
+    }
+    if (threadIdx.x == 0) *d_T = T;
+}
+
+// child count + scan + child write; the new length goes to *d_L.  Entry e
+// belongs to thread e % kPlanThreads in round e / kPlanThreads, so a short
+// list spreads over the workgroup.
+__global__ __launch_bounds__(kPlanThreads) void k_expand_small(
+    const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize, const uint32_t* __restrict__ Lmeta,
+    const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos, Children C, uint64_t L,
+    uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta,
+    uint64_t* __restrict__ d_L) {
+    __shared__ uint64_t sh[kPlanWaves];
+    uint64_t carry = 0;
+    for (uint64_t r0 = 0; r0 < L; r0 += kPlanThreads) {
+        const uint64_t e = r0 + threadIdx.x;
+        const uint64_t c = e < L ? expand_count_one(e, isbig, bigpos, C) : 0ull;
+        uint64_t tot;
+        const uint64_t o = carry + block_exclusive_scan<kPlanWaves>(c, tot, sh);
+        if (e < L) expand_write_one(e, o, Lstart, Lsize, Lmeta, isbig, bigpos, C, Nstart, Nsize, Nmeta);
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *d_L = carry;
 }
 
 // --------------------------------------------------------------- leaves
@@ -1002,9 +1146,13 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
 // never vary cost at most a histogram pass).  The flags word's live bits
 // are preset by the mode; the rest is reduced on the device, nothing is
 // read back.
-__global__ void k_live_init(uint64_t* __restrict__ live, int nwords, int flags_word, uint64_t flags_bits) {
+struct LiveWords {
+    uint64_t w[kMaxKeyWords + 1];
+};
+
+__global__ void k_live_init(uint64_t* __restrict__ live, int nwords, LiveWords preset) {
     const int w = threadIdx.x;
-    if (w < nwords) live[w] = w == flags_word ? flags_bits : 0ull;
+    if (w < nwords) live[w] = preset.w[w];
 }
 
 // One grid row per plane of a sum: x words go to key word xo + k, z words to
@@ -1031,33 +1179,25 @@ __global__ __launch_bounds__(kBlock) void k_live_or(SumView s, const uint32_t* _
     if (lane_id() == 0 && acc) atomicOr(&live[kw], (unsigned long long)acc);
 }
 
-struct LiveWords {
-    uint64_t w[kMaxKeyWords + 1];
-};
-
-__global__ void k_live_or_const(unsigned long long* __restrict__ live, LiveWords add, int nwords) {
-    const int w = threadIdx.x;
-    if (w < nwords && add.w[w]) atomicOr(&live[w], (unsigned long long)add.w[w]);
-}
-
-// Root digit position: the first live bit (0 when none: all keys equal).
-__global__ void k_root_pos(const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ d_pos) {
-    const int pos = next_live(live, kb, 0);
-    *d_pos = (uint64_t)(pos >= kb ? 0 : pos);
-}
-
 // The root's children as the first bucket list (buffer 0), merged like
-// k_expand_write; also the list length and the number of valid items.
-__global__ void k_root_list(const uint64_t* __restrict__ sub_start, const uint64_t* __restrict__ sub_size,
-                            uint32_t cap, const uint64_t* __restrict__ live, int kb,
-                            const uint64_t* __restrict__ d_pos, uint64_t* __restrict__ Nstart,
-                            uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta,
-                            uint64_t* __restrict__ d_L, uint64_t* __restrict__ d_nvalid) {
-    const int pos = (int)*d_pos;
+// k_walk (sub-buckets staged in LDS); also the list length and the number
+// of valid items.
+__global__ __launch_bounds__(kDigits) void k_root_list(const uint64_t* __restrict__ sub_start,
+                                                       const uint64_t* __restrict__ sub_size, uint32_t cap,
+                                                       const uint64_t* __restrict__ live, int kb,
+                                                       uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize,
+                                                       uint32_t* __restrict__ Nmeta, uint64_t* __restrict__ d_L,
+                                                       uint64_t* __restrict__ d_nvalid) {
+    __shared__ uint64_t st[kDigits], ss[kDigits];
+    st[threadIdx.x] = sub_start[threadIdx.x];
+    ss[threadIdx.x] = sub_size[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int pos = root_pos(live, kb);
     const int depth = min(pos + kDigitBits, kb);
     const uint32_t uni = next_live(live, kb, depth) >= kb ? 1u : 0u;
     uint64_t o = 0, nv = 0;
-    ChildWalk w{sub_start, sub_size, cap};
+    ChildWalk w{st, ss, cap};
     w.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
         Nstart[o] = s0;
         Nsize[o] = n;
@@ -1123,7 +1263,6 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     uint64_t* d_items = misc + 58;
     uint64_t* d_nvalid = misc + 59;
     uint64_t* d_T = misc + 60;
-    uint64_t* d_pos = misc + 61;
 
     Cfg cfg{};
     cfg.live = d_live;
@@ -1137,8 +1276,8 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     cfg.job_maxw = spec.job_maxw;
     cfg.job_alive = spec.job_alive;
 
-    // ---- pass 0: implicit ids, one root bucket, position from the device.
-    k_root_pos<<<1, 1, 0, st>>>(d_live, kb, d_pos);
+    // ---- pass 0: implicit ids, one root bucket (its digit position is the
+    // first live bit, found by each kernel from the live mask).
     const uint32_t tile0 = pick_tile(n_dense, kTileMaxImplicit);
     const uint64_t t0 = (n_dense + tile0 - 1) / tile0;
     PLH_TRY(ctx->ws.get(kWsHist, t0 * kDigits * 4, &p));
@@ -1148,7 +1287,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     uint64_t* sub_size = sub_start + kDigits;
     {
         ProfScope ps(ctx, "msd_hist_first");
-        k_root_hist<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, n_dense, d_pos, tile0, hist);
+        k_root_hist<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, n_dense, d_live, kb, tile0, hist);
     }
     {
         ProfScope ps(ctx, "colscan");
@@ -1156,12 +1295,13 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     }
     {
         ProfScope ps(ctx, "msd_scatter_first", 4.0 * (double)n_dense);
-        k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, d_pos, tile0, hist, sub_start);
+        k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, d_live, kb, tile0, hist,
+                                                                    sub_start);
     }
     PLH_HIP(hipGetLastError());
     Lists A, B;
     PLH_TRY(get_lists(ctx, kWsListA, kDigits, &A));
-    k_root_list<<<1, 1, 0, st>>>(sub_start, sub_size, cap, d_live, kb, d_pos, A.start, A.size, A.meta, d_L, d_nvalid);
+    k_root_list<<<1, kDigits, 0, st>>>(sub_start, sub_size, cap, d_live, kb, A.start, A.size, A.meta, d_L, d_nvalid);
     PLH_HIP(hipGetLastError());
     int cur_slot = kWsListA;
     uint64_t Lb = kDigits;  // bound on the list length
@@ -1178,9 +1318,6 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         uint64_t* bigpos = (uint64_t*)((char*)p + ((Lb * 8 + 15) & ~(uint64_t)15));
         uint64_t* bigsz = bigpos + Lb;
         uint64_t* newpos = bigsz + Lb;
-        k_classify<<<grid1(Lb), kBlock, 0, st>>>(A.size, A.meta, d_L, Lb, cap, isbig, bigsz);
-        PLH_TRY(scan_u32_to_u64(ctx, isbig, bigpos, Lb, d_nbig));
-        PLH_TRY(scan_u64(ctx, bigsz, newpos, Lb, d_items));
         uint64_t info[4];
         // the big-bucket arrays are sized by the bound Lb before the counts are known
         PLH_TRY(ctx->ws.get(kWsPassB, Lb * 56 + 256, &p));
@@ -1194,7 +1331,15 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         PB.pos = PB.meta + Lb;
         PB.ntiles = PB.pos + Lb;
         PB.skip = PB.ntiles + Lb;
-        k_gather_big<<<grid1(Lb), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, Lb, PB);
+        if (Lb <= kPlanMax) {
+            k_plan_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, d_L, cap, isbig, bigpos, PB, d_nbig,
+                                                    d_items);
+        } else {
+            k_classify<<<grid1(Lb), kBlock, 0, st>>>(A.size, A.meta, d_L, Lb, cap, isbig, bigsz);
+            PLH_TRY(scan_u32_to_u64(ctx, isbig, bigpos, Lb, d_nbig));
+            PLH_TRY(scan_u64(ctx, bigsz, newpos, Lb, d_items));
+            k_gather_big<<<grid1(Lb), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, Lb, PB);
+        }
         PLH_HIP(hipGetLastError());
         PLH_TRY(read_u64n(ctx, d_L, 4, info));
         L = info[0];
@@ -1202,8 +1347,12 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         const uint64_t nbig = info[1], pass_items = info[2];
         if (nbig == 0) break;
         const uint32_t tile = pick_tile(pass_items, kTileMaxExplicit);
-        k_bucket_setup<<<grid1(nbig), kBlock, 0, st>>>(PB, nbig, tile, d_live, kb);
-        PLH_TRY(scan_u32_to_u64(ctx, PB.ntiles, PB.tbase, nbig, d_T));
+        if (nbig <= kPlanMax) {
+            k_setup_small<<<1, kPlanThreads, 0, st>>>(PB, nbig, tile, d_live, kb, d_T);
+        } else {
+            k_bucket_setup<<<grid1(nbig), kBlock, 0, st>>>(PB, nbig, tile, d_live, kb);
+            PLH_TRY(scan_u32_to_u64(ctx, PB.ntiles, PB.tbase, nbig, d_T));
+        }
         const uint64_t Tb = pass_items / tile + nbig;  // >= sum of ceil(size / tile)
         PLH_TRY(ctx->ws.get(kWsHist, Tb * kDigits * 4, &p));
         hist = (uint32_t*)p;
@@ -1225,14 +1374,26 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
                                                                         tile, hist, sub_start);
         }
         PLH_HIP(hipGetLastError());
-        k_expand_count<<<grid1(L), kBlock, 0, st>>>(isbig, bigpos, PB, sub_start, sub_size, L, cap, cnt);
-        PLH_TRY(scan_u32_to_u64(ctx, cnt, newpos, L, d_L));
         // each split bucket has at most kDigits children; every entry holds >= 1 item
         const uint64_t Lb2 = std::min<uint64_t>(L + (kDigits - 1) * nbig, n_valid);
         const int nslot = cur_slot == kWsListA ? kWsListB : kWsListA;
         PLH_TRY(get_lists(ctx, nslot, Lb2, &B));
-        k_expand_write<<<grid1(L), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, PB, sub_start, sub_size,
-                                                    newpos, L, cap, d_live, kb, B.start, B.size, B.meta);
+        PLH_TRY(ctx->ws.get(kWsChild, nbig * kDigits * 20 + nbig * 4 + 256, &p));
+        Children CH;
+        CH.start = (uint64_t*)p;
+        CH.size = CH.start + nbig * kDigits;
+        CH.meta = (uint32_t*)(CH.size + nbig * kDigits);
+        CH.count = CH.meta + nbig * kDigits;
+        k_walk<<<dim3((unsigned)nbig), kDigits, 0, st>>>(PB, sub_start, sub_size, cap, d_live, kb, CH);
+        if (L <= kPlanMax) {
+            k_expand_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, CH, L, B.start, B.size,
+                                                      B.meta, d_L);
+        } else {
+            k_expand_count<<<grid1(L), kBlock, 0, st>>>(isbig, bigpos, CH, L, cnt);
+            PLH_TRY(scan_u32_to_u64(ctx, cnt, newpos, L, d_L));
+            k_expand_write<<<grid1(L), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, CH, newpos, L, B.start,
+                                                        B.size, B.meta);
+        }
         PLH_HIP(hipGetLastError());
         std::swap(A, B);
         cur_slot = nslot;
@@ -1283,30 +1444,65 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
 }
 
 // Pack a sum into records of 2*AS + 2 words (x[AS], z[AS], value,
-// flags | job << 32); words beyond the sum's own array size are zero.
+// flags | job << 32); words beyond the sum's own array size are zero.  With
+// live set, the same pass ORs the sum's words into the collation's live
+// mask (x words at key word xo, z words at xo + AS, the job plane at 0).
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_pack(SumView s, const uint32_t* __restrict__ j, int as, int64_t n,
-                                                 uint64_t* __restrict__ rec) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
-    uint64_t* r = rec + (size_t)t * (2 * AS + 2);
+                                                 uint64_t* __restrict__ rec, unsigned long long* __restrict__ live,
+                                                 int xo) {
+    uint64_t ox[AS], oz[AS];
+    uint32_t oj = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) ox[k] = oz[k] = 0ull;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        uint64_t* r = rec + (size_t)t * (2 * AS + 2);
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            const uint64_t x = k < as ? s.x[(int64_t)k * s.stride + t] : 0ull;
+            const uint64_t z = k < as ? s.z[(int64_t)k * s.stride + t] : 0ull;
+            r[k] = x;
+            r[AS + k] = z;
+            ox[k] |= x;
+            oz[k] |= z;
+        }
+        const uint32_t jt = j ? j[t] : 0u;
+        oj |= jt;
+        r[2 * AS] = (uint64_t)__double_as_longlong(s.v[t]);
+        r[2 * AS + 1] = (uint64_t)s.f[t] | ((uint64_t)jt << 32);
+    }
+    if (!live) return;
 #pragma unroll
     for (int k = 0; k < AS; ++k) {
-        r[k] = k < as ? s.x[(int64_t)k * s.stride + t] : 0ull;
-        r[AS + k] = k < as ? s.z[(int64_t)k * s.stride + t] : 0ull;
+        uint64_t a = ox[k], b = oz[k];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            a |= __shfl_xor(a, o, kWave);
+            b |= __shfl_xor(b, o, kWave);
+        }
+        if (lane_id() == 0) {
+            if (a) atomicOr(&live[xo + k], (unsigned long long)a);
+            if (b) atomicOr(&live[xo + AS + k], (unsigned long long)b);
+        }
     }
-    r[2 * AS] = (uint64_t)__double_as_longlong(s.v[t]);
-    r[2 * AS + 1] = (uint64_t)s.f[t] | (j ? (uint64_t)j[t] << 32 : 0ull);
+    if (j) {
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) oj |= __shfl_xor(oj, o, kWave);
+        if (lane_id() == 0 && oj) atomicOr(&live[0], (unsigned long long)oj);
+    }
 }
 
 template <int AS>
-int pack_sum(plh_ctx* ctx, const plh_sum* in, int slot, const uint64_t** out) {
+int pack_sum(plh_ctx* ctx, const plh_sum* in, int slot, const uint64_t** out, uint64_t* live = nullptr,
+             int xo = 0) {
     void* p;
     PLH_TRY(ctx->ws.get(slot, (size_t)(in->n ? in->n : 1) * (2 * AS + 2) * 8, &p));
-    if (in->n)
-        k_pack<AS><<<grid1((uint64_t)in->n), kBlock, 0, ctx->stream>>>(SumView{in->x, in->z, in->f, in->v, in->cap},
-                                                                        in->has_job ? in->j : nullptr, in->as, in->n,
-                                                                        (uint64_t*)p);
+    if (in->n) {
+        const unsigned g = (unsigned)std::min<uint64_t>(grid1((uint64_t)in->n), 2048);
+        k_pack<AS><<<g, kBlock, 0, ctx->stream>>>(SumView{in->x, in->z, in->f, in->v, in->cap},
+                                                  in->has_job ? in->j : nullptr, in->as, in->n, (uint64_t*)p,
+                                                  (unsigned long long*)live, xo);
+    }
     PLH_HIP(hipGetLastError());
     *out = (const uint64_t*)p;
     return PLH_OK;
@@ -1325,13 +1521,16 @@ int padded_as(int as) {
     return -1;
 }
 
-// Live mask of a collation: zero every key word, preset the flags word.
-int live_init(plh_ctx* ctx, int key_words, uint32_t mode, uint64_t** live) {
+// Live mask of a collation: every key word preset from `preset` (or zero),
+// the flags word's bits set by the mode.
+int live_init(plh_ctx* ctx, int key_words, uint32_t mode, uint64_t** live, const LiveWords* preset = nullptr) {
     void* p;
     PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
     *live = (uint64_t*)p;
-    const uint64_t fbits = (mode & kFold) ? (1ull << 56) : (3ull << 56);
-    k_live_init<<<1, 64, 0, ctx->stream>>>(*live, key_words + 1, key_words, fbits);
+    LiveWords w{};
+    if (preset) w = *preset;
+    w.w[key_words] = (mode & kFold) ? (1ull << 56) : (3ull << 56);
+    k_live_init<<<1, 64, 0, ctx->stream>>>(*live, key_words + 1, w);
     PLH_HIP(hipGetLastError());
     return PLH_OK;
 }
@@ -1358,11 +1557,12 @@ int collate_terms_as(plh_ctx* ctx, const plh_sum* in, const CollateSpec& spec, p
     src.as = in->as;
     src.mode = spec.mode;
     src.n = (uint32_t)in->n;
-    if ((uint64_t)in->n * TermSource<AS, JOB>::kRec * 8 <= kMaxPackBytes)
-        PLH_TRY(pack_sum<AS>(ctx, in, kWsPackL, &src.rec));
     uint64_t* live;
     PLH_TRY(live_init(ctx, TermSource<AS, JOB>::kKW, spec.mode, &live));
-    PLH_TRY(live_or(ctx, in, AS, TermSource<AS, JOB>::kXOff, live));
+    if ((uint64_t)in->n * TermSource<AS, JOB>::kRec * 8 <= kMaxPackBytes)
+        PLH_TRY(pack_sum<AS>(ctx, in, kWsPackL, &src.rec, live, TermSource<AS, JOB>::kXOff));
+    else
+        PLH_TRY(live_or(ctx, in, AS, TermSource<AS, JOB>::kXOff, live));
     return collate_run<AS>(ctx, src, (uint64_t)in->n, spec, out, in->as);
 }
 
@@ -1383,8 +1583,11 @@ int collate_pairs_as(plh_ctx* ctx, const plh_sum* Ls, const plh_sum* Rs, int by_
         return PLH_ERR_SIZE;
     }
     PairSource<AS, KIND> src;
-    PLH_TRY(pack_sum<AS>(ctx, Ls, kWsPackL, &src.L.r));
-    PLH_TRY(pack_sum<AS>(ctx, Rs, kWsPackR, &src.R.r));
+    // a product's words are XORs of the operands' words: OR(L) | OR(R) covers them
+    uint64_t* live;
+    PLH_TRY(live_init(ctx, 2 * AS, spec.mode, &live));
+    PLH_TRY(pack_sum<AS>(ctx, Ls, kWsPackL, &src.L.r, live, 0));
+    PLH_TRY(pack_sum<AS>(ctx, Rs, kWsPackR, &src.R.r, live, 0));
     src.Lp = SumView{Ls->x, Ls->z, Ls->f, Ls->v, Ls->cap};
     src.Rp = SumView{Rs->x, Rs->z, Rs->f, Rs->v, Rs->cap};
     src.as = Ls->as;
@@ -1394,11 +1597,6 @@ int collate_pairs_as(plh_ctx* ctx, const plh_sum* Ls, const plh_sum* Rs, int by_
     src.n_hi = (uint32_t)n_hi;
     src.div_lo = make_fastdiv((uint32_t)n_lo);
     src.by_i = (uint32_t)by_i;
-    // a product's words are XORs of the operands' words: OR(L) | OR(R) covers them
-    uint64_t* live;
-    PLH_TRY(live_init(ctx, 2 * AS, spec.mode, &live));
-    PLH_TRY(live_or(ctx, Ls, AS, 0, live));
-    PLH_TRY(live_or(ctx, Rs, AS, 0, live));
     return collate_run<AS>(ctx, src, n_lo * n_hi, spec, out, Ls->as);
 }
 
@@ -1417,14 +1615,14 @@ int collate_rotation_as(plh_ctx* ctx, const plh_sum* in, const RotParams<AS>& rp
     src.n = (uint32_t)in->n;
     src.rp = rp;
     src.jt = jt;
-    if ((uint64_t)in->n * S::kRec * 8 <= kMaxPackBytes) PLH_TRY(pack_sum<AS>(ctx, in, kWsPackL, &src.rec));
-    uint64_t* live;
-    PLH_TRY(live_init(ctx, S::kKW, spec.mode, &live));
-    PLH_TRY(live_or(ctx, in, AS, S::kXOff, live));
     LiveWords add{};
     for (int k = 0; k < 2 * AS; ++k) add.w[S::kXOff + k] = p_or[k];
-    k_live_or_const<<<1, 64, 0, ctx->stream>>>((unsigned long long*)live, add, S::kKW + 1);
-    PLH_HIP(hipGetLastError());
+    uint64_t* live;
+    PLH_TRY(live_init(ctx, S::kKW, spec.mode, &live, &add));
+    if ((uint64_t)in->n * S::kRec * 8 <= kMaxPackBytes)
+        PLH_TRY(pack_sum<AS>(ctx, in, kWsPackL, &src.rec, live, S::kXOff));
+    else
+        PLH_TRY(live_or(ctx, in, AS, S::kXOff, live));
     return collate_run<AS>(ctx, src, 2 * (uint64_t)in->n, spec, out, in->as);
 }
 

@@ -48,8 +48,8 @@ class Workspace {
     std::vector<Buf> bufs_;
 };
 
-// Caching device allocator owned by a context: freed blocks are kept and
-// reused for later requests of up to twice their size.  All work of a
+// Caching device allocator owned by a context: blocks come in size classes
+// and freed blocks are kept for later requests of the same class.  All work of a
 // context is ordered on its one stream, so a block released by one call can
 // be handed to the next call without a synchronisation.
 class BlockCache {
@@ -154,6 +154,7 @@ enum : int {
     kWsPackR,
     kWsPassL,   // per-pass arrays over the bucket list
     kWsPassB,   // per-pass arrays over the big buckets
+    kWsChild,   // children of the big buckets of a pass
     kWsCount
 };
 

@@ -1,6 +1,7 @@
 // plh_scan.hip -- device-wide exclusive scans and small host helpers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -171,8 +172,9 @@ int Workspace::get(int slot, size_t bytes, void** out) {
             b.p = nullptr;
             b.n = 0;
         }
-        // Grow geometrically so repeated small growth does not reallocate.
-        size_t want = bytes + bytes / 4;
+        // Grow geometrically (x2, at least 1 MiB) so a sum that grows step by
+        // step (propagation) reallocates each slot only a few times.
+        size_t want = std::max<size_t>(2 * bytes, (size_t)1 << 20);
         PLH_HIP(hipMalloc(&b.p, want));
         b.n = want;
     }
@@ -194,35 +196,44 @@ size_t Workspace::bytes_held() const {
     return t;
 }
 
+// Size classes: powers of two up to 64 MiB (at least 2 MiB), above that
+// multiples of 1/16 of the next power of two (under 6.25% padding).  A
+// request is served only by a cached block of its own class, so a sum that
+// grows step by step reuses the blocks it freed instead of allocating anew.
+static size_t size_class(size_t bytes) {
+    size_t p2 = (size_t)2 << 20;
+    while (p2 < bytes) p2 <<= 1;
+    if (p2 <= ((size_t)64 << 20)) return p2;
+    const size_t step = p2 / 16;
+    return (bytes + step - 1) / step * step;
+}
+
 int BlockCache::alloc(size_t bytes, void** out) {
     if (bytes == 0) bytes = 256;
-    size_t best = (size_t)-1;
+    const size_t cls = size_class(bytes);
     for (size_t i = 0; i < free_.size(); ++i) {
-        const size_t n = free_[i].n;
-        if (n >= bytes && n <= 2 * bytes + (1u << 20) && (best == (size_t)-1 || n < free_[best].n)) best = i;
+        if (free_[i].n == cls) {
+            used_.push_back(free_[i]);
+            *out = free_[i].p;
+            free_.erase(free_.begin() + (long)i);
+            return PLH_OK;
+        }
     }
-    if (best != (size_t)-1) {
-        used_.push_back(free_[best]);
-        *out = free_[best].p;
-        free_.erase(free_.begin() + (long)best);
-        return PLH_OK;
-    }
-    const size_t rounded = (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, rounded);
+    hipError_t e = hipMalloc(&p, cls);
     if (e != hipSuccess) {
         // Give the cached blocks back and retry once.
         (void)hipGetLastError();
         for (auto& b : free_) (void)hipFree(b.p);
         free_.clear();
-        e = hipMalloc(&p, rounded);
+        e = hipMalloc(&p, cls);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             set_error("device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
             return PLH_ERR_OOM;
         }
     }
-    used_.push_back(Blk{p, rounded});
+    used_.push_back(Blk{p, cls});
     *out = p;
     return PLH_OK;
 }

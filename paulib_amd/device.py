@@ -126,6 +126,21 @@ class PauliSum:
 
     download = to_host
 
+    def range_to_host(self, offset: int, count: int) -> HostSum:
+        """Terms [offset, offset + count) (plh_sum_download_range)."""
+        a = self.array_size
+        x = np.empty((a, count), np.uint64)
+        z = np.empty((a, count), np.uint64)
+        f = np.empty(count, np.uint8)
+        v = np.empty(count, np.float64)
+        _lib.call("plh_sum_download_range", self._h, int(offset), int(count), _u64(x), _u64(z), _u8(f), _f64(v))
+        return HostSum(a, self.num_qubits(), x, z, f, v)
+
+    def word_at(self, t: int, plane: str = "x", k: int = 0) -> int:
+        """One key word of term t (x or z plane, word k)."""
+        h = self.range_to_host(t, 1)
+        return int((h.x if plane == "x" else h.z)[k, 0])
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().plh_sum_destroy(self._h)
