@@ -217,11 +217,52 @@ PLH_DEV void column_scan(uint64_t t0, uint64_t t1, uint64_t bstart, uint32_t* __
     sub_size[d] = run;
 }
 
-__global__ __launch_bounds__(kBlock) void k_root_colscan(uint64_t ntiles, uint32_t* __restrict__ hist,
-                                                         uint64_t* __restrict__ sub_start,
-                                                         uint64_t* __restrict__ sub_size) {
-    __shared__ uint64_t sh[kWavesPerBlock];
-    column_scan(0, ntiles, 0, hist, sub_start, sub_size, sh);
+// The root's column scan: 4 groups of kDigits threads take a quarter of the
+// tiles each (a sum pass, then a prefix-writing pass from the group's offset).
+constexpr int kRootGroups = 4;
+
+__global__ __launch_bounds__(kDigits * kRootGroups) void k_root_colscan(uint64_t ntiles, uint32_t* __restrict__ hist,
+                                                                       uint64_t* __restrict__ sub_start,
+                                                                       uint64_t* __restrict__ sub_size) {
+    __shared__ uint64_t part[kRootGroups][kDigits];
+    __shared__ uint64_t sh[kDigits * kRootGroups / kWave];
+    const int d = threadIdx.x % kDigits, g = threadIdx.x / kDigits;
+    const uint64_t per = (ntiles + kRootGroups - 1) / kRootGroups;
+    const uint64_t t0 = min(ntiles, (uint64_t)g * per), t1 = min(ntiles, t0 + per);
+    constexpr int kB = 16;
+    uint64_t sum = 0;
+    for (uint64_t t = t0; t < t1; t += kB) {
+        uint32_t c[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) c[k] = (t + k < t1) ? hist[(t + k) * kDigits + d] : 0u;
+#pragma unroll
+        for (int k = 0; k < kB; ++k) sum += c[k];
+    }
+    part[g][d] = sum;
+    __syncthreads();
+    uint64_t run = 0;
+    for (int q = 0; q < g; ++q) run += part[q][d];
+    for (uint64_t t = t0; t < t1; t += kB) {
+        uint32_t c[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) c[k] = (t + k < t1) ? hist[(t + k) * kDigits + d] : 0u;
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            if (t + k < t1) hist[(t + k) * kDigits + d] = (uint32_t)run;
+            run += c[k];
+        }
+    }
+    // digit totals in the first group's threads, a scan over the whole
+    // workgroup (the other threads add 0 after them)
+    uint64_t tot_d = 0;
+    if (g == 0)
+        for (int q = 0; q < kRootGroups; ++q) tot_d += part[q][d];
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan<kDigits * kRootGroups / kWave>(tot_d, tot, sh);
+    if (g == 0) {
+        sub_start[d] = ex;
+        sub_size[d] = tot_d;
+    }
 }
 
 template <class Src>
@@ -479,10 +520,10 @@ struct ChildWalk {
     }
 };
 
-// Children of each big bucket, from its 256 sub-buckets staged in LDS (one
-// workgroup per bucket; the walk is sequential, so it runs on LDS, not on
-// global loads): at most kDigits children at Cstart/Csize/Cmeta[b * kDigits],
-// their count in ccount[b].  A bucket whose digit was constant has one
+// Children of each big bucket, from its 256 sub-buckets (one wave per
+// bucket; the walk is sequential, so it runs on wave-uniform values, not on
+// global loads): at most kDigits children at start/size/meta[b * kDigits],
+// their count in count[b].  A bucket whose digit was constant has one
 // child: itself, deeper, in the same id buffer.
 struct Children {
     uint64_t* start;
@@ -491,16 +532,59 @@ struct Children {
     uint32_t* count;
 };
 
-__global__ __launch_bounds__(kDigits) void k_walk(PassBuckets B, const uint64_t* __restrict__ sub_start,
-                                                  const uint64_t* __restrict__ sub_size, uint32_t cap,
-                                                  const uint64_t* __restrict__ live, int kb, Children C) {
-    __shared__ uint64_t st[kDigits], ss[kDigits];
+PLH_DEV uint64_t readlane64(uint64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// ChildWalk over 256 sub-buckets held by one wave (lane l has digits 4l ..
+// 4l+3): the walk runs on wave-uniform values read with readlane, so each
+// step is a few scalar instructions.  emit(start, size, parts) is called in
+// digit order with wave-uniform arguments.
+template <class F>
+PLH_DEV void wave_walk(const uint64_t (&st)[4], const uint64_t (&ss)[4], uint32_t cap, F&& emit) {
+    uint64_t g0 = 0, gn = 0;
+    uint32_t gc = 0;
+    for (int l = 0; l < kWave; ++l) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t sz = readlane64(ss[k], l);
+            if (!sz) continue;
+            const uint64_t s0 = readlane64(st[k], l);
+            if (sz > cap) {
+                if (gn) {
+                    emit(g0, gn, gc);
+                    gn = 0;
+                    gc = 0;
+                }
+                emit(s0, sz, 1u);
+                continue;
+            }
+            if (gn + sz > cap) {
+                emit(g0, gn, gc);
+                gn = 0;
+                gc = 0;
+            }
+            if (!gn) g0 = s0;
+            gn += sz;
+            ++gc;
+        }
+    }
+    if (gn) emit(g0, gn, gc);
+}
+
+// One wave per big bucket.
+__global__ __launch_bounds__(kWave) void k_walk(PassBuckets B, const uint64_t* __restrict__ sub_start,
+                                                const uint64_t* __restrict__ sub_size, uint32_t cap,
+                                                const uint64_t* __restrict__ live, int kb, Children C) {
     const uint64_t b = blockIdx.x;
     const uint32_t pos = B.pos[b];
     const uint32_t skip = B.skip[b];
     const uint64_t base = b * kDigits;
+    const int lane = (int)threadIdx.x;
     if (skip) {
-        if (threadIdx.x == 0) {
+        if (lane == 0) {
             const int depth = min((int)(pos + skip), kb);
             C.start[base] = B.start[b];
             C.size[base] = B.size[b];
@@ -509,22 +593,25 @@ __global__ __launch_bounds__(kDigits) void k_walk(PassBuckets B, const uint64_t*
         }
         return;
     }
-    st[threadIdx.x] = sub_start[base + threadIdx.x];
-    ss[threadIdx.x] = sub_size[base + threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+    uint64_t st[4], ss[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        st[k] = sub_start[base + 4 * lane + k];
+        ss[k] = sub_size[base + 4 * lane + k];
+    }
     const uint32_t buf = meta_buf(B.meta[b]) ^ 1u;
     const int child_depth = min((int)pos + kDigitBits, kb);
     const uint32_t child_uniform = next_live(live, kb, child_depth) >= kb ? 1u : 0u;
     uint64_t o = base;
-    ChildWalk w{st, ss, cap};
-    w.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
-        C.start[o] = s0;
-        C.size[o] = n;
-        C.meta[o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
+    wave_walk(st, ss, cap, [&](uint64_t s0, uint64_t n, uint32_t parts) {
+        if (lane == 0) {
+            C.start[o] = s0;
+            C.size[o] = n;
+            C.meta[o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
+        }
         ++o;
     });
-    C.count[b] = (uint32_t)(o - base);
+    if (lane == 0) C.count[b] = (uint32_t)(o - base);
 }
 
 PLH_DEV uint32_t expand_count_one(uint64_t e, const uint32_t* isbig, const uint64_t* bigpos, const Children& C) {
@@ -1180,33 +1267,37 @@ __global__ __launch_bounds__(kBlock) void k_live_or(SumView s, const uint32_t* _
 }
 
 // The root's children as the first bucket list (buffer 0), merged like
-// k_walk (sub-buckets staged in LDS); also the list length and the number
-// of valid items.
-__global__ __launch_bounds__(kDigits) void k_root_list(const uint64_t* __restrict__ sub_start,
-                                                       const uint64_t* __restrict__ sub_size, uint32_t cap,
-                                                       const uint64_t* __restrict__ live, int kb,
-                                                       uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize,
-                                                       uint32_t* __restrict__ Nmeta, uint64_t* __restrict__ d_L,
-                                                       uint64_t* __restrict__ d_nvalid) {
-    __shared__ uint64_t st[kDigits], ss[kDigits];
-    st[threadIdx.x] = sub_start[threadIdx.x];
-    ss[threadIdx.x] = sub_size[threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+// k_walk (one wave); also the list length and the number of valid items.
+__global__ __launch_bounds__(kWave) void k_root_list(const uint64_t* __restrict__ sub_start,
+                                                     const uint64_t* __restrict__ sub_size, uint32_t cap,
+                                                     const uint64_t* __restrict__ live, int kb,
+                                                     uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize,
+                                                     uint32_t* __restrict__ Nmeta, uint64_t* __restrict__ d_L,
+                                                     uint64_t* __restrict__ d_nvalid) {
+    const int lane = (int)threadIdx.x;
+    uint64_t st[4], ss[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        st[k] = sub_start[4 * lane + k];
+        ss[k] = sub_size[4 * lane + k];
+    }
     const int pos = root_pos(live, kb);
     const int depth = min(pos + kDigitBits, kb);
     const uint32_t uni = next_live(live, kb, depth) >= kb ? 1u : 0u;
     uint64_t o = 0, nv = 0;
-    ChildWalk w{st, ss, cap};
-    w.walk([&](uint64_t s0, uint64_t n, uint32_t parts) {
-        Nstart[o] = s0;
-        Nsize[o] = n;
-        Nmeta[o] = parts == 1 ? make_meta((uint32_t)depth, 0u, uni) : make_meta((uint32_t)pos, 0u, 0u);
+    wave_walk(st, ss, cap, [&](uint64_t s0, uint64_t n, uint32_t parts) {
+        if (lane == 0) {
+            Nstart[o] = s0;
+            Nsize[o] = n;
+            Nmeta[o] = parts == 1 ? make_meta((uint32_t)depth, 0u, uni) : make_meta((uint32_t)pos, 0u, 0u);
+        }
         ++o;
         nv += n;
     });
-    *d_L = o;
-    *d_nvalid = nv;
+    if (lane == 0) {
+        *d_L = o;
+        *d_nvalid = nv;
+    }
 }
 
 // ------------------------------------------------------------------ driver
@@ -1291,7 +1382,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     }
     {
         ProfScope ps(ctx, "colscan");
-        k_root_colscan<<<1, kBlock, 0, st>>>(t0, hist, sub_start, sub_size);
+        k_root_colscan<<<1, kDigits * kRootGroups, 0, st>>>(t0, hist, sub_start, sub_size);
     }
     {
         ProfScope ps(ctx, "msd_scatter_first", 4.0 * (double)n_dense);
@@ -1301,7 +1392,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     PLH_HIP(hipGetLastError());
     Lists A, B;
     PLH_TRY(get_lists(ctx, kWsListA, kDigits, &A));
-    k_root_list<<<1, kDigits, 0, st>>>(sub_start, sub_size, cap, d_live, kb, A.start, A.size, A.meta, d_L, d_nvalid);
+    k_root_list<<<1, kWave, 0, st>>>(sub_start, sub_size, cap, d_live, kb, A.start, A.size, A.meta, d_L, d_nvalid);
     PLH_HIP(hipGetLastError());
     int cur_slot = kWsListA;
     uint64_t Lb = kDigits;  // bound on the list length
@@ -1384,7 +1475,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         CH.size = CH.start + nbig * kDigits;
         CH.meta = (uint32_t*)(CH.size + nbig * kDigits);
         CH.count = CH.meta + nbig * kDigits;
-        k_walk<<<dim3((unsigned)nbig), kDigits, 0, st>>>(PB, sub_start, sub_size, cap, d_live, kb, CH);
+        k_walk<<<dim3((unsigned)nbig), kWave, 0, st>>>(PB, sub_start, sub_size, cap, d_live, kb, CH);
         if (L <= kPlanMax) {
             k_expand_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, CH, L, B.start, B.size,
                                                       B.meta, d_L);
@@ -1472,6 +1563,11 @@ __global__ __launch_bounds__(kBlock) void k_pack(SumView s, const uint32_t* __re
         r[2 * AS + 1] = (uint64_t)s.f[t] | ((uint64_t)jt << 32);
     }
     if (!live) return;
+    // OR over the wave, then over the workgroup in LDS, then one global
+    // atomic per word per workgroup
+    __shared__ unsigned long long acc[2 * AS + 1];
+    if (threadIdx.x < 2 * AS + 1) acc[threadIdx.x] = 0ull;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < AS; ++k) {
         uint64_t a = ox[k], b = oz[k];
@@ -1481,14 +1577,20 @@ __global__ __launch_bounds__(kBlock) void k_pack(SumView s, const uint32_t* __re
             b |= __shfl_xor(b, o, kWave);
         }
         if (lane_id() == 0) {
-            if (a) atomicOr(&live[xo + k], (unsigned long long)a);
-            if (b) atomicOr(&live[xo + AS + k], (unsigned long long)b);
+            if (a) atomicOr(&acc[k], (unsigned long long)a);
+            if (b) atomicOr(&acc[AS + k], (unsigned long long)b);
         }
     }
-    if (j) {
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) oj |= __shfl_xor(oj, o, kWave);
-        if (lane_id() == 0 && oj) atomicOr(&live[0], (unsigned long long)oj);
+    for (int o = kWave / 2; o > 0; o >>= 1) oj |= __shfl_xor(oj, o, kWave);
+    if (lane_id() == 0 && oj) atomicOr(&acc[2 * AS], (unsigned long long)oj);
+    __syncthreads();
+    if (threadIdx.x < 2 * AS) {
+        const int w = (int)threadIdx.x;
+        const uint64_t v = acc[w];
+        if (v) atomicOr(&live[w < AS ? xo + w : xo + AS + (w - AS)], (unsigned long long)v);
+    } else if (threadIdx.x == 2 * AS && j && acc[2 * AS]) {
+        atomicOr(&live[0], acc[2 * AS]);
     }
 }
 
@@ -1498,7 +1600,7 @@ int pack_sum(plh_ctx* ctx, const plh_sum* in, int slot, const uint64_t** out, ui
     void* p;
     PLH_TRY(ctx->ws.get(slot, (size_t)(in->n ? in->n : 1) * (2 * AS + 2) * 8, &p));
     if (in->n) {
-        const unsigned g = (unsigned)std::min<uint64_t>(grid1((uint64_t)in->n), 2048);
+        const unsigned g = (unsigned)std::min<uint64_t>(grid1((uint64_t)in->n), 512);
         k_pack<AS><<<g, kBlock, 0, ctx->stream>>>(SumView{in->x, in->z, in->f, in->v, in->cap},
                                                   in->has_job ? in->j : nullptr, in->as, in->n, (uint64_t*)p,
                                                   (unsigned long long*)live, xo);
