@@ -115,6 +115,18 @@ int plh_multiply(plh_sum* lhs, const plh_sum* rhs);
 /* canonicalize(lhs * rhs) without materialising the uncombined product
  * (the fused product->collation path).  n*m <= 2^32 pairs. */
 int plh_multiply_canonical(const plh_sum* lhs, const plh_sum* rhs, plh_sum* out);
+/* A batch of independent collated products: for every k < njobs,
+ * canonicalize(lhs[k] * rhs[k]) exactly as plh_multiply_canonical gives it,
+ * all in one collation (one launch sequence for the whole batch).  out
+ * becomes a batched sum holding the jobs one after another; job k is terms
+ * offsets[k] .. offsets[k + 1] - 1 (offsets: njobs + 1 host entries; read
+ * them with plh_sum_download_range).  Operands share out's context and one
+ * array size (1 or 2: up to 128 qubits); the sum of n_k * m_k <= 2^32
+ * (PLH_ERR_SIZE otherwise: split the batch).  No reference counterpart:
+ * the reference multiplies one pair of sums per call
+ * (include/PauliSumOperations.h:255-306). */
+int plh_batch_multiply_canonical(int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
+                                 plh_sum* out, int64_t* offsets);
 /* commutator / i_commutator (include/PauliSumOperations.h:348-356), the
  * uncombined terms in (i over a, j over b) order; num_qubits = max. */
 int plh_commutator(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out);

@@ -11,9 +11,11 @@ from .device import (  # noqa: F401
     PauliSum,
     TruncationOptions,
     apply_truncation,
+    batch_multiply_canonical,
     commutator,
     commutator_canonical,
     i_commutator,
+    job_terms,
     propagate,
     propagate_expectation,
     run_program,

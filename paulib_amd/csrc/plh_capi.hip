@@ -8,6 +8,8 @@
 // back into that exception).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -64,6 +66,9 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
     if (cap <= s->cap && s->block) return PLH_OK;
     int64_t nc = cap < 16 ? 16 : cap;
     if (s->cap > 0 && nc < s->cap + s->cap / 2 && keep > 0) nc = s->cap + s->cap / 2;  // amortise appends
+    // nothing to keep: give the old block back first, so the two need not
+    // coexist (a sum near the HBM size can still be re-reserved)
+    if (keep <= 0) sum_free_storage(s);
     void* blk = nullptr;
     if (s->ctx) {
         PLH_TRY(s->ctx->blocks.alloc(sum_bytes(s->as, nc, s->has_job), &blk));
@@ -510,6 +515,31 @@ int plh_multiply_canonical(const plh_sum* lhs, const plh_sum* rhs, plh_sum* out)
     PLH_TRY(collate_pairs(lhs->ctx, lhs, rhs, 0, 0, canon_spec(), out));
     out->nq = lhs->nq;
     return plh_ctx_synchronize(lhs->ctx);
+}
+
+int plh_batch_multiply_canonical(int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs, plh_sum* out,
+                                 int64_t* offsets) {
+    if (njobs <= 0 || !lhs || !rhs || !offsets) {
+        set_error("plh_batch_multiply_canonical: bad job list");
+        return PLH_ERR_ARG;
+    }
+    PLH_TRY(check_sum(out));
+    int64_t nq = 0;
+    for (int k = 0; k < njobs; ++k) {
+        PLH_TRY(check_pair(lhs[k], rhs[k]));
+        if (lhs[k]->ctx != out->ctx) {
+            set_error("plh_batch_multiply_canonical: operands and output must share one context");
+            return PLH_ERR_ARG;
+        }
+        if (lhs[k]->n > 0x7FFFFFFF || rhs[k]->n > 0x7FFFFFFF) {
+            set_error("plh_batch_multiply_canonical: input exceeds 2^31-1 terms");
+            return PLH_ERR_SIZE;
+        }
+        nq = std::max<int64_t>(nq, lhs[k]->nq);
+    }
+    PLH_TRY(collate_products(out->ctx, njobs, lhs, rhs, canon_spec(), out, offsets));
+    out->nq = nq;
+    return plh_ctx_synchronize(out->ctx);
 }
 
 int plh_commutator(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out) {

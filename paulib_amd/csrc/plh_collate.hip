@@ -1762,6 +1762,149 @@ int with_output(plh_ctx* ctx, plh_sum* in, plh_sum* out, F&& f) {
 
 }  // namespace
 
+// ------------------------------------------------------- product batches
+// One concatenated record array for the lhs (or rhs) sums of a batch: term
+// t of the concatenation belongs to the sum whose prefix range holds it.
+struct PackJob {
+    const uint64_t* x;
+    const uint64_t* z;
+    const uint8_t* f;
+    const double* v;
+    int64_t stride;
+    uint32_t first;  // first term in the concatenation
+    uint32_t n;
+};
+
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_pack_multi(const PackJob* __restrict__ jobs, uint32_t njobs, int as,
+                                                       uint64_t total, uint64_t* __restrict__ rec,
+                                                       unsigned long long* __restrict__ live) {
+    uint64_t ox[AS], oz[AS];
+#pragma unroll
+    for (int k = 0; k < AS; ++k) ox[k] = oz[k] = 0ull;
+    for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (uint64_t)gridDim.x * kBlock) {
+        uint32_t lo = 0, hi = njobs;  // last job with first <= t
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (jobs[mid].first <= t) lo = mid; else hi = mid;
+        }
+        const PackJob J = jobs[lo];
+        const uint64_t i = t - J.first;
+        uint64_t* r = rec + t * (2 * AS + 2);
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            const uint64_t x = k < as ? J.x[(int64_t)k * J.stride + i] : 0ull;
+            const uint64_t z = k < as ? J.z[(int64_t)k * J.stride + i] : 0ull;
+            r[k] = x;
+            r[AS + k] = z;
+            ox[k] |= x;
+            oz[k] |= z;
+        }
+        r[2 * AS] = (uint64_t)__double_as_longlong(J.v[i]);
+        r[2 * AS + 1] = (uint64_t)J.f[i];
+    }
+    __shared__ unsigned long long acc[2 * AS];
+    if (threadIdx.x < 2 * AS) acc[threadIdx.x] = 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        uint64_t a = ox[k], b = oz[k];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            a |= __shfl_xor(a, o, kWave);
+            b |= __shfl_xor(b, o, kWave);
+        }
+        if (lane_id() == 0) {
+            if (a) atomicOr(&acc[k], (unsigned long long)a);
+            if (b) atomicOr(&acc[AS + k], (unsigned long long)b);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * AS && acc[threadIdx.x]) atomicOr(&live[1 + threadIdx.x], acc[threadIdx.x]);
+}
+
+// Job ranges of a job-ordered batched sum: offsets[k] = first term of job k.
+__global__ void k_job_offsets(const uint32_t* __restrict__ J, int64_t n, uint32_t njobs, int64_t* __restrict__ off) {
+    const uint32_t k = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k > njobs) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (J[mid] < k) lo = mid + 1; else hi = mid;
+    }
+    off[k] = lo;
+}
+
+template <int AS>
+int collate_products_as(plh_ctx* ctx, int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
+                        const CollateSpec& spec, plh_sum* out, uint64_t total_pairs) {
+    using S = MultiPairSource<AS>;
+    hipStream_t st = ctx->stream;
+    // host tables: jobs, pack descriptors, id chunks
+    std::vector<ProductJob> jobs((size_t)njobs);
+    std::vector<PackJob> pl((size_t)njobs), pr((size_t)njobs);
+    uint64_t off = 0, lt = 0, rt = 0;
+    for (int k = 0; k < njobs; ++k) {
+        const plh_sum* a = lhs[k];
+        const plh_sum* b = rhs[k];
+        jobs[(size_t)k] = ProductJob{off, (uint32_t)a->n, (uint32_t)lt, (uint32_t)rt,
+                                     make_fastdiv(a->n > 0 ? (uint32_t)a->n : 1u)};
+        pl[(size_t)k] = PackJob{a->x, a->z, a->f, a->v, a->cap, (uint32_t)lt, (uint32_t)a->n};
+        pr[(size_t)k] = PackJob{b->x, b->z, b->f, b->v, b->cap, (uint32_t)rt, (uint32_t)b->n};
+        off += (uint64_t)a->n * (uint64_t)b->n;
+        lt += (uint64_t)a->n;
+        rt += (uint64_t)b->n;
+    }
+    const uint64_t nchunks = (total_pairs >> S::kChunkBits) + 1;
+    std::vector<uint32_t> chunk((size_t)nchunks);
+    {
+        uint32_t k = 0;
+        for (uint64_t c = 0; c < nchunks; ++c) {
+            const uint64_t id = c << S::kChunkBits;
+            while (k + 1 < (uint32_t)njobs && jobs[(size_t)k + 1].off <= id) ++k;
+            chunk[(size_t)c] = k;
+        }
+    }
+    const size_t b_jobs = (size_t)njobs * sizeof(ProductJob), b_pack = (size_t)njobs * sizeof(PackJob);
+    const size_t o_pl = (b_jobs + 255) & ~(size_t)255, o_pr = o_pl + ((b_pack + 255) & ~(size_t)255),
+                 o_ch = o_pr + ((b_pack + 255) & ~(size_t)255);
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsJobs, o_ch + nchunks * 4 + 256, &p));
+    char* dj = (char*)p;
+    PLH_HIP(hipMemcpyAsync(dj, jobs.data(), b_jobs, hipMemcpyHostToDevice, st));
+    PLH_HIP(hipMemcpyAsync(dj + o_pl, pl.data(), b_pack, hipMemcpyHostToDevice, st));
+    PLH_HIP(hipMemcpyAsync(dj + o_pr, pr.data(), b_pack, hipMemcpyHostToDevice, st));
+    PLH_HIP(hipMemcpyAsync(dj + o_ch, chunk.data(), nchunks * 4, hipMemcpyHostToDevice, st));
+    // live mask: the job word's bits up to the last job, the operands' words, the mode's flags
+    LiveWords preset{};
+    uint64_t jm = 0;
+    while (jm < (uint64_t)(njobs - 1)) jm = (jm << 1) | 1ull;
+    preset.w[0] = jm;
+    uint64_t* live;
+    PLH_TRY(live_init(ctx, S::kKW, spec.mode, &live, &preset));
+    PLH_TRY(ctx->ws.get(kWsPackL, (size_t)(lt ? lt : 1) * S::kRec * 8, &p));
+    uint64_t* lrec = (uint64_t*)p;
+    PLH_TRY(ctx->ws.get(kWsPackR, (size_t)(rt ? rt : 1) * S::kRec * 8, &p));
+    uint64_t* rrec = (uint64_t*)p;
+    const int as = lhs[0]->as;
+    if (lt)
+        k_pack_multi<AS><<<(unsigned)std::min<uint64_t>(grid1(lt), 1024), kBlock, 0, st>>>(
+            (const PackJob*)(dj + o_pl), (uint32_t)njobs, as, lt, lrec, (unsigned long long*)live);
+    if (rt)
+        k_pack_multi<AS><<<(unsigned)std::min<uint64_t>(grid1(rt), 1024), kBlock, 0, st>>>(
+            (const PackJob*)(dj + o_pr), (uint32_t)njobs, as, rt, rrec, (unsigned long long*)live);
+    PLH_HIP(hipGetLastError());
+    S src;
+    src.L = lrec;
+    src.R = rrec;
+    src.jobs = (const ProductJob*)dj;
+    src.chunk_job = (const uint32_t*)(dj + o_ch);
+    src.njobs = (uint32_t)njobs;
+    src.mode = spec.mode;
+    // the host tables above are read by the copies before collate_run's first read-back returns
+    return collate_run<AS>(ctx, src, total_pairs, spec, out, as);
+}
+
 int collate_rotation(plh_ctx* ctx, plh_sum* in, const uint64_t* px, const uint64_t* pz, uint32_t psf, double c,
                      double sn, const CollateSpec& spec, plh_sum* out) {
     if (in->n > (int64_t)0x7FFFFFFFll) {
@@ -1823,6 +1966,60 @@ int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, con
     });
     if (rc != PLH_OK) return rc;
     out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    return PLH_OK;
+}
+
+int collate_products(plh_ctx* ctx, int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
+                     const CollateSpec& spec, plh_sum* out, int64_t* offsets) {
+    if (njobs <= 0) {
+        set_error("collate_products: no jobs");
+        return PLH_ERR_ARG;
+    }
+    const int as = lhs[0]->as;
+    uint64_t total = 0;
+    for (int k = 0; k < njobs; ++k) {
+        if (lhs[k]->as != as || rhs[k]->as != as) {
+            set_error("collate_products: every operand of a batch must have the same array size");
+            return PLH_ERR_ARG;
+        }
+        if (lhs[k] == out || rhs[k] == out) {
+            set_error("collate_products: the output must not alias an operand");
+            return PLH_ERR_ARG;
+        }
+        total += (uint64_t)lhs[k]->n * (uint64_t)rhs[k]->n;
+    }
+    if (total > (1ull << 32)) {
+        set_error("collate_products: more than 2^32 pairs in one batch (split it)");
+        return PLH_ERR_SIZE;
+    }
+    sum_free_storage(out);
+    out->as = as;
+    out->has_job = true;
+    out->n = 0;
+    int rc;
+    switch (padded_as(as)) {
+        case 1: rc = collate_products_as<1>(ctx, njobs, lhs, rhs, spec, out, total); break;
+        case 2: rc = collate_products_as<2>(ctx, njobs, lhs, rhs, spec, out, total); break;
+        default: set_error("product batches support up to 128 qubits"); return PLH_ERR_AS;
+    }
+    if (rc != PLH_OK) return rc;
+    if (out->n > 0 && !out->j) {
+        set_error("collate_products: output lost its job plane");
+        return PLH_ERR_HIP;
+    }
+    out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsTmpSum, ((size_t)njobs + 1) * 8 + 64, &p));
+    int64_t* doff = (int64_t*)p;
+    if (out->n > 0) {
+        k_job_offsets<<<(unsigned)((njobs + 1 + 255) / 256), 256, 0, ctx->stream>>>(out->j, out->n, (uint32_t)njobs,
+                                                                                    doff);
+        PLH_HIP(hipGetLastError());
+        PLH_HIP(hipMemcpyAsync(offsets, doff, ((size_t)njobs + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        PLH_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        for (int k = 0; k <= njobs; ++k) offsets[k] = 0;
+    }
     return PLH_OK;
 }
 

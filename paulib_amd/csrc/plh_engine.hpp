@@ -155,6 +155,7 @@ enum : int {
     kWsPassL,   // per-pass arrays over the bucket list
     kWsPassB,   // per-pass arrays over the big buckets
     kWsChild,   // children of the big buckets of a pass
+    kWsJobs,    // product batch: job table and id-chunk table
     kWsCount
 };
 
@@ -200,6 +201,11 @@ int collate_rotation(plh_ctx* ctx, plh_sum* in, const uint64_t* px, const uint64
 struct JobRotTables;
 int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, const uint64_t* p_or,
                           const CollateSpec& spec, plh_sum* out);
+// out = canonicalize(lhs[k] * rhs[k]) for k < njobs as one batched sum
+// (job plane = k, jobs in order); offsets[k] .. offsets[k + 1] is job k's
+// range (njobs + 1 entries, host).  Sum of n_k * m_k <= 2^32.
+int collate_products(plh_ctx* ctx, int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
+                     const CollateSpec& spec, plh_sum* out, int64_t* offsets);
 // out = collate(L x R) for kind 0 (product, out[i*n+j] = L[j]*R[i]) or
 // kind 1 (commutator of L with R, by_i adds the i factor).
 int collate_pairs(plh_ctx* ctx, const plh_sum* L, const plh_sum* R, int kind, int by_i,

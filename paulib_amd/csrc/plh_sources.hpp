@@ -379,6 +379,93 @@ struct PairSource {
     }
 };
 
+// Many independent products canonicalize(lhs_k * rhs_k) in one collation.
+// Ids are dense over the concatenated pair ranges: job k owns
+// [off_k, off_k + n_k * m_k), and inside it id - off_k = i * n_k + j is the
+// reference's order out[i*n + j] = lhs[j] * rhs[i].  The job index leads the
+// key, so each job's terms are collated exactly as they would be alone.
+// Operands are packed records (x[AS], z[AS], value, flags) concatenated over
+// the jobs; chunk_job[c] is the job that owns id c << kChunkBits.
+struct ProductJob {
+    uint64_t off;    // first id of the job
+    uint32_t n;      // lhs terms (the fast index j)
+    uint32_t lbase;  // first lhs record of the job
+    uint32_t rbase;  // first rhs record
+    FastDiv div;     // divides by n
+};
+
+template <int AS>
+struct MultiPairSource {
+    static constexpr int kAS = AS;
+    static constexpr int kXOff = 1;
+    static constexpr int kKW = 2 * AS + 1;
+    static constexpr int kRec = 2 * AS + 2;
+    static constexpr int kChunkBits = 12;
+    const uint64_t* L;   // lhs records of all jobs
+    const uint64_t* R;   // rhs records of all jobs
+    const ProductJob* jobs;
+    const uint32_t* chunk_job;
+    uint32_t njobs;
+    uint32_t mode;
+
+    PLH_DEV bool valid(uint32_t) const { return true; }
+    PLH_DEV uint32_t dense_to_id(uint32_t t) const { return t; }
+    // job of id: from the chunk table, then forward while the next job starts at or before id
+    PLH_DEV uint32_t job_of(uint32_t id) const {
+        uint32_t k = chunk_job[id >> kChunkBits];
+        while (k + 1 < njobs && jobs[k + 1].off <= id) ++k;
+        return k;
+    }
+    PLH_DEV void split(uint32_t id, uint32_t& k, const uint64_t*& a, const uint64_t*& b) const {
+        k = job_of(id);
+        const ProductJob J = jobs[k];
+        const uint32_t local = (uint32_t)(id - J.off);
+        const uint32_t i = fdiv(local, J.div);
+        const uint32_t jj = local - i * J.n;
+        a = L + (size_t)(J.lbase + jj) * kRec;
+        b = R + (size_t)(J.rbase + i) * kRec;
+    }
+    PLH_DEV static uint32_t flags_of(const uint64_t* a, const uint64_t* b) {
+        uint64_t c1 = 0, c2 = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) phase_word(a[k], a[AS + k], b[k], b[AS + k], c1, c2);
+        return phase_of((uint32_t)a[2 * AS + 1], (uint32_t)b[2 * AS + 1], c1, c2);
+    }
+    PLH_DEV uint64_t word(uint32_t id, int w) const {
+        uint32_t k;
+        const uint64_t *a, *b;
+        split(id, k, a, b);
+        if (w == 0) return k;
+        if (w <= 2 * AS) return a[w - 1] ^ b[w - 1];
+        if (w == 2 * AS + 1) {
+            const uint32_t fl = flags_of(a, b);
+            return (uint64_t)((mode & kFold) ? (fl & 1u) : fl) << 56;
+        }
+        return 0ull;
+    }
+    PLH_DEV void key(uint32_t id, uint64_t* kw, uint32_t& fl) const {
+        uint32_t k;
+        const uint64_t *a, *b;
+        split(id, k, a, b);
+        kw[0] = k;
+#pragma unroll
+        for (int q = 0; q < 2 * AS; ++q) kw[1 + q] = a[q] ^ b[q];
+        fl = flags_of(a, b);
+    }
+    PLH_DEV uint32_t raw_flags(uint32_t id) const {
+        uint32_t k;
+        const uint64_t *a, *b;
+        split(id, k, a, b);
+        return flags_of(a, b);
+    }
+    PLH_DEV double base_value(uint32_t id) const {
+        uint32_t k;
+        const uint64_t *a, *b;
+        split(id, k, a, b);
+        return __longlong_as_double((long long)a[2 * AS]) * __longlong_as_double((long long)b[2 * AS]);
+    }
+};
+
 // Window of DB bits starting at global bit position pos (MSB-first).
 template <int DB, class Src>
 PLH_DEV uint32_t key_window(const Src& src, uint32_t id, int pos) {

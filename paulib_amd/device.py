@@ -289,6 +289,28 @@ def propagate_expectation(O: PauliSum, rotations: HostSum, opts: TruncationOptio
     return r
 
 
+def batch_multiply_canonical(lhs: list, rhs: list, out: PauliSum | None = None):
+    """canonicalize(lhs[k] * rhs[k]) for every k in one collation.
+
+    Returns (out, offsets): out holds the jobs one after another, job k is
+    terms offsets[k] .. offsets[k + 1] - 1 (see job_terms)."""
+    if len(lhs) != len(rhs) or not lhs:
+        raise ValueError("lhs and rhs must be non-empty lists of equal length")
+    ctx = lhs[0].ctx
+    out = out or PauliSum(ctx, lhs[0].array_size, lhs[0].num_qubits())
+    k = len(lhs)
+    la = (C.c_void_p * k)(*[a._h.value for a in lhs])
+    ra = (C.c_void_p * k)(*[b._h.value for b in rhs])
+    off = np.zeros(k + 1, np.int64)
+    _lib.call("plh_batch_multiply_canonical", k, la, ra, out._h, _i64(off))
+    return out, off
+
+
+def job_terms(out: PauliSum, offsets, k: int) -> HostSum:
+    """Job k's terms of a batched result."""
+    return out.range_to_host(int(offsets[k]), int(offsets[k + 1] - offsets[k]))
+
+
 def run_program(O: PauliSum, prog, opts: TruncationOptions | None = None):
     """Heisenberg run of a Clifford + rotation program (see workloads.Program)."""
     opts = opts or TruncationOptions()
