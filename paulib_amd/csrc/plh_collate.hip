@@ -54,13 +54,18 @@ constexpr int kLeafThreads = 512;
 constexpr int kLeafWaves = kLeafThreads / kWave;
 static_assert(kBlock == kDigits, "pass kernels map one thread to one digit");
 
-template <int AS>
+// Leaf geometry for keys of KW words (plus flags): ids per leaf sized so a
+// leaf's LDS (key words, value, sort key, two permutations, flags per item,
+// plus the bins) stays under ~78 KB and two leaves fit one CU's 160 KB.
+template <int AS, int KW>
 struct LeafDims {
-    // ids per leaf; sized so keys + values of a leaf fit ~56 KB of LDS
-    static constexpr int kCap = AS == 1 ? 2048 : AS == 2 ? 1024 : AS <= 4 ? 512 : AS <= 8 ? 256 : 128;
     static constexpr int kDl = AS == 1 ? 11 : AS == 2 ? 10 : AS <= 4 ? 9 : AS <= 8 ? 8 : 7;
-    static constexpr int kBins = 1 << kDl;  // counting-sort bins (= kCap)
+    static constexpr int kBins = 1 << kDl;  // counting-sort bins
+    static constexpr int kPerItem = 8 * KW + 8 + 8 + 2 + 2 + 1;
+    static constexpr int kCap = ((78 * 1024 - 4 * (kBins + 1) - 1024) / kPerItem) / 256 * 256;
     static constexpr int kPer = (kCap + kLeafThreads - 1) / kLeafThreads;
+    static constexpr int kSlotBits = kCap <= 256 ? 8 : kCap <= 512 ? 9 : kCap <= 1024 ? 10 : 11;
+    static_assert(kCap >= 256 && kCap <= 2048, "leaf capacity");
 };
 
 // Tile size for a pass over `items` ids: enough tiles to fill the chip,
@@ -757,13 +762,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_expand_small(
 // --------------------------------------------------------------- leaves
 template <int AS, int KW>
 struct LeafLds {
-    using D = LeafDims<AS>;
+    using D = LeafDims<AS, KW>;
     uint64_t key[KW][D::kCap];         // key words (job word first when present, then x, z)
     double val[D::kCap];               // folded coefficients; run sums at head slots
+    uint64_t bkey[D::kCap];            // sort keys, in bin order
     uint16_t perm[D::kCap];            // sorted position -> slot
     uint16_t perm2[D::kCap];           // bin order -> (key, id) order; later rslot
-    uint16_t bdig[D::kCap];            // counting-sort digit, in bin order
-    uint32_t bsec[D::kCap];            // the 32 key bits after the digit, in bin order
     uint8_t fl[D::kCap];               // raw flags
     uint32_t bin[D::kBins + 1];        // counting-sort bin starts
     unsigned long long diff[KW + 1];   // varying bits of each key word
@@ -873,10 +877,10 @@ PLH_DEV uint64_t sparse_code(uint64_t K) {
     return S;
 }
 
-// (key, masked flags, id) order; ids are re-read from global memory only on
-// a full key tie.
+// (key, masked flags, id) order.  A leaf's ids ascend with its slots (the
+// passes are stable), so on a full key tie the slot decides.
 template <int AS, int KW>
-PLH_DEV bool slot_less(const LeafLds<AS, KW>& L, uint32_t fmask, const uint32_t* ids, int a, int b) {
+PLH_DEV bool slot_less(const LeafLds<AS, KW>& L, uint32_t fmask, int a, int b) {
 #pragma unroll
     for (int w = 0; w < KW; ++w) {
         const uint64_t p = L.key[w][a], q = L.key[w][b];
@@ -884,7 +888,7 @@ PLH_DEV bool slot_less(const LeafLds<AS, KW>& L, uint32_t fmask, const uint32_t*
     }
     const uint32_t fa = L.fl[a] & fmask, fb = L.fl[b] & fmask;
     if (fa != fb) return fa < fb;
-    return ids[a] < ids[b];
+    return a < b;
 }
 
 template <int AS, int KW>
@@ -937,10 +941,10 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
                                                        const uint32_t* __restrict__ ids0,
                                                        const uint32_t* __restrict__ ids1,
                                                        uint32_t* __restrict__ kept_out) {
-    using D = LeafDims<AS>;
-    constexpr int kPer = D::kPer;
     constexpr int KW = Src::kKW;   // key words before the flags word
     constexpr int XO = Src::kXOff;  // 1 when the job word leads the key
+    using D = LeafDims<AS, KW>;
+    constexpr int kPer = D::kPer;
     __shared__ LeafLds<AS, KW> L;
     const int tid = threadIdx.x;
     const uint64_t e = blockIdx.x;
@@ -991,8 +995,9 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             if (p < n) {
                 uint64_t xz[KW];
                 uint32_t fl;
-                src.key(idv[k], xz, fl);
-                const double v = fold_value(c, src.base_value(idv[k]), fl);
+                double bv;
+                src.key_value(idv[k], xz, fl, bv);
+                const double v = fold_value(c, bv, fl);
 #pragma unroll
                 for (int w = 0; w < KW; ++w) L.key[w][p] = xz[w];
                 L.fl[p] = (uint8_t)fl;
@@ -1064,18 +1069,22 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         }
         __syncthreads();
         const bool sparse = 8u * L.pcs < L.nvtot * n;
+        // With few varying bits the key is complete in the low-padded sort
+        // key, and the slot (= id order) fills the padding: every sort key
+        // is then distinct and already carries the (key, id) order.
+        const bool complete = !sparse && L.nvtot + D::kSlotBits <= 64;
         // counting sort on the kDl leading bits of the sort key ...
-        uint32_t dig[kPer], slot[kPer], sec[kPer];
+        uint32_t dig[kPer], slot[kPer];
+        uint64_t S[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
             dig[k] = 0;
             slot[k] = 0;
-            sec[k] = 0;
+            S[k] = 0;
             if (p < n) {
-                const uint64_t S = sparse ? sparse_code(K[k]) : K[k];
-                dig[k] = (uint32_t)(S >> (64 - D::kDl));
-                sec[k] = (uint32_t)((S << D::kDl) >> 32);
+                S[k] = sparse ? sparse_code(K[k]) : complete ? (K[k] | p) : K[k];
+                dig[k] = (uint32_t)(S[k] >> (64 - D::kDl));
                 slot[k] = atomicAdd(&L.bin[dig[k]], 1u);
             }
         }
@@ -1107,42 +1116,41 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             if (p < n) {
                 const uint32_t q = L.bin[dig[k]] + slot[k];
                 L.perm[q] = (uint16_t)p;
-                L.bdig[q] = (uint16_t)dig[k];
-                L.bsec[q] = sec[k];
+                L.bkey[q] = S[k];
             }
         }
         __syncthreads();
         // ... then each item's rank inside its bin on the full (key, id).
         // Work is assigned by bin-ordered position, so the lanes of a wave
-        // mostly share a bin.  The 32 sort-key bits after the digit, read in
-        // bin order four at a time, decide every comparison but ties, which
-        // compare the whole key (and the id).
+        // mostly share a bin.  The sort keys, read in bin order four at a
+        // time, decide every comparison but ties, which compare the whole
+        // key (and then the slot).
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t q = tid + k * kLeafThreads;
             if (q < n) {
                 const int me = L.perm[q];
-                const uint32_t b = L.bdig[q];
+                const uint64_t ms = L.bkey[q];
+                const uint32_t b = (uint32_t)(ms >> (64 - D::kDl));
                 const uint32_t s0 = L.bin[b], s1 = L.bin[b + 1];
                 uint32_t r = s0;
                 if (s1 - s0 > 1) {
-                    const uint32_t ms = L.bsec[q];
                     uint32_t lt = 0, eq = 0;
                     uint32_t t = s0;
                     for (; t + 4 <= s1; t += 4) {
-                        const uint32_t a0 = L.bsec[t], a1 = L.bsec[t + 1], a2 = L.bsec[t + 2], a3 = L.bsec[t + 3];
+                        const uint64_t a0 = L.bkey[t], a1 = L.bkey[t + 1], a2 = L.bkey[t + 2], a3 = L.bkey[t + 3];
                         lt += (uint32_t)(a0 < ms) + (uint32_t)(a1 < ms) + (uint32_t)(a2 < ms) + (uint32_t)(a3 < ms);
                         eq += (uint32_t)(a0 == ms) + (uint32_t)(a1 == ms) + (uint32_t)(a2 == ms) + (uint32_t)(a3 == ms);
                     }
                     for (; t < s1; ++t) {
-                        const uint32_t a = L.bsec[t];
+                        const uint64_t a = L.bkey[t];
                         lt += (uint32_t)(a < ms);
                         eq += (uint32_t)(a == ms);
                     }
                     r += lt;
-                    if (eq > 1) {  // other items share these 32 bits (eq counts this one too)
+                    if (eq > 1) {  // other items share this sort key (eq counts this one too)
                         for (t = s0; t < s1; ++t)
-                            if (t != q && L.bsec[t] == ms && slot_less<AS, KW>(L, c.fmask, ids, L.perm[t], me)) ++r;
+                            if (t != q && L.bkey[t] == ms && slot_less<AS, KW>(L, c.fmask, L.perm[t], me)) ++r;
                     }
                 }
                 L.perm2[r] = (uint16_t)me;
@@ -1327,7 +1335,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     const uint32_t mode = spec.mode;
     const int64_t max_weight = spec.max_weight;
     const double eps = spec.eps;
-    using D = LeafDims<AS>;
+    using D = LeafDims<AS, Src::kKW>;
     hipStream_t st = ctx->stream;
     constexpr int kWordsK = Src::kKW + 1;
     const int kb = 64 * kWordsK;

@@ -97,6 +97,10 @@ struct TermSource {
     PLH_DEV double base_value(uint32_t id) const {
         return rec ? __longlong_as_double((long long)rec[(size_t)id * kRec + 2 * AS]) : s.v[id];
     }
+    PLH_DEV void key_value(uint32_t id, uint64_t* kw, uint32_t& fl, double& v) const {
+        key(id, kw, fl);
+        v = base_value(id);
+    }
 };
 
 // One rotation exp(-i theta/2 P) on the general path of evolve
@@ -278,6 +282,17 @@ struct RotSource {
         item(id, x, z, fl, v, job);
         return v;
     }
+    PLH_DEV void key_value(uint32_t id, uint64_t* kw, uint32_t& fl, double& v) const {
+        uint64_t x[AS], z[AS];
+        uint32_t job;
+        item(id, x, z, fl, v, job);
+        if (JOB) kw[0] = job;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            kw[kXOff + k] = x[k];
+            kw[kXOff + AS + k] = z[k];
+        }
+    }
 };
 
 // A term packed as one record of 2*AS + 2 words: x[AS], z[AS], value,
@@ -377,6 +392,18 @@ struct PairSource {
         const double vb = __longlong_as_double((long long)R.rec(ri)[2 * AS]);
         return (KIND == 0) ? va * vb : (2.0 * va) * vb;
     }
+    PLH_DEV void key_value(uint32_t id, uint64_t* kw, uint32_t& fl, double& v) const {
+        uint32_t li, ri;
+        split(id, li, ri);
+        const uint64_t* a = L.rec(li);
+        const uint64_t* b = R.rec(ri);
+#pragma unroll
+        for (int k = 0; k < 2 * AS; ++k) kw[k] = a[k] ^ b[k];
+        fl = raw_flags_of(a, b);
+        const double va = __longlong_as_double((long long)a[2 * AS]);
+        const double vb = __longlong_as_double((long long)b[2 * AS]);
+        v = (KIND == 0) ? va * vb : (2.0 * va) * vb;
+    }
 };
 
 // Many independent products canonicalize(lhs_k * rhs_k) in one collation.
@@ -463,6 +490,16 @@ struct MultiPairSource {
         const uint64_t *a, *b;
         split(id, k, a, b);
         return __longlong_as_double((long long)a[2 * AS]) * __longlong_as_double((long long)b[2 * AS]);
+    }
+    PLH_DEV void key_value(uint32_t id, uint64_t* kw, uint32_t& fl, double& v) const {
+        uint32_t k;
+        const uint64_t *a, *b;
+        split(id, k, a, b);
+        kw[0] = k;
+#pragma unroll
+        for (int q = 0; q < 2 * AS; ++q) kw[1 + q] = a[q] ^ b[q];
+        fl = flags_of(a, b);
+        v = __longlong_as_double((long long)a[2 * AS]) * __longlong_as_double((long long)b[2 * AS]);
     }
 };
 
