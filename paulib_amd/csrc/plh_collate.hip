@@ -1160,7 +1160,18 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (uint32_t p = tid; p < n; p += kLeafThreads) L.perm[p] = L.perm2[p];
         __syncthreads();
     }
-    // ---- runs: thread t owns sorted positions [t*kPer, t*kPer + kPer)
+    // ---- values in sorted order and run heads, by all threads (the sort
+    // keys and the second permutation are free until the output step)
+    double* vs = reinterpret_cast<double*>(L.bkey);
+    uint16_t* head = L.perm2;
+    for (uint32_t q = tid; q < n; q += kLeafThreads) {
+        const int sq = L.perm[q];
+        vs[q] = L.val[sq];
+        head[q] = (q == 0 || !slot_key_eq<AS, KW>(L, c.fmask, sq, L.perm[q - 1])) ? 1 : 0;
+    }
+    __syncthreads();
+    // ---- runs: thread t owns sorted positions [t*kPer, t*kPer + kPer);
+    // each run is summed acc = 0; acc += v in (key, id) order
     double sums[kPer];
     uint16_t heads[kPer];
     uint32_t nk = 0;
@@ -1170,13 +1181,28 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (int k = 0; k < kPer; ++k) {
             const uint32_t q = q0 + k;
             if (q >= n) break;
-            if (q > 0 && slot_key_eq<AS, KW>(L, c.fmask, L.perm[q], L.perm[q - 1])) continue;
+            if (!head[q]) continue;
             double acc = 0.0;
-            uint32_t r = q;
-            do {
-                acc += L.val[L.perm[r]];
-                ++r;
-            } while (r < n && slot_key_eq<AS, KW>(L, c.fmask, L.perm[r], L.perm[q]));
+            acc += vs[q];
+            for (uint32_t r = q + 1; r < n;) {
+                if (r + 4 <= n) {
+                    const uint16_t h0 = head[r], h1 = head[r + 1], h2 = head[r + 2], h3 = head[r + 3];
+                    const double v0 = vs[r], v1 = vs[r + 1], v2 = vs[r + 2], v3 = vs[r + 3];
+                    if (h0) break;
+                    acc += v0;
+                    if (h1) break;
+                    acc += v1;
+                    if (h2) break;
+                    acc += v2;
+                    if (h3) break;
+                    acc += v3;
+                    r += 4;
+                } else {
+                    if (head[r]) break;
+                    acc += vs[r];
+                    ++r;
+                }
+            }
             uint64_t xz[KW];
             const int s = L.perm[q];
 #pragma unroll
@@ -1190,7 +1216,8 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     }
     uint32_t total;
     const uint32_t r0 = block_exclusive_scan<kLeafWaves>(nk, total, L.red);
-    // the scan's barriers order the sums above before these writes
+    // the scan's barriers order the run sums (and their reads of head[])
+    // before these writes
     uint16_t* rslot = L.perm2;
     for (uint32_t k = 0; k < nk; ++k) {
         rslot[r0 + k] = heads[k];
