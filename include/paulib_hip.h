@@ -54,9 +54,13 @@ const char* plh_build_info(void);
 /* stream == NULL creates a private stream at the default priority. */
 int plh_ctx_create(int device, void* stream, plh_ctx** out);
 int plh_ctx_destroy(plh_ctx* ctx);
+/* Waits for the context's stream; scratch slots of 256 MiB and more go back
+ * to the context's device heap (which keeps its memory mapped). */
 int plh_ctx_synchronize(plh_ctx* ctx);
 void* plh_ctx_stream(plh_ctx* ctx);
-/* Device scratch held by the context (bytes); plh_ctx_trim frees it. */
+/* Device memory the context holds beyond its live sums: scratch slots plus
+ * the free part of its heap's mapped memory (bytes).  plh_ctx_trim gives
+ * the scratch, and every heap chunk without a live sum, back to the device. */
 int64_t plh_ctx_scratch_bytes(plh_ctx* ctx);
 int plh_ctx_trim(plh_ctx* ctx);
 

@@ -27,10 +27,7 @@ void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
 
 void sum_free_storage(plh_sum* s) {
-    if (s->block) {
-        if (s->ctx) s->ctx->blocks.free(s->block);
-        else (void)hipFree(s->block);
-    }
+    if (s->block && s->ctx) s->ctx->heap.free(s->block);
     s->block = nullptr;
     s->x = s->z = nullptr;
     s->f = nullptr;
@@ -69,17 +66,12 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
     // nothing to keep: give the old block back first, so the two need not
     // coexist (a sum near the HBM size can still be re-reserved)
     if (keep <= 0) sum_free_storage(s);
-    void* blk = nullptr;
-    if (s->ctx) {
-        PLH_TRY(s->ctx->blocks.alloc(sum_bytes(s->as, nc, s->has_job), &blk));
-    } else {
-        hipError_t e = hipMalloc(&blk, sum_bytes(s->as, nc, s->has_job));
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            set_error(std::string("device allocation failed: ") + hipGetErrorString(e));
-            return PLH_ERR_OOM;
-        }
+    if (!s->ctx) {
+        set_error("the sum's context has been destroyed");
+        return PLH_ERR_ARG;
     }
+    void* blk = nullptr;
+    PLH_TRY(s->ctx->heap.alloc(sum_bytes(s->as, nc, s->has_job), &blk));
     char* b = (char*)blk;
     uint64_t* nx = (uint64_t*)b;
     uint64_t* nz = nx + (size_t)s->as * nc;
@@ -225,6 +217,7 @@ int plh_ctx_create(int device, void* stream, plh_ctx** out) {
     PLH_HIP(hipSetDevice(device));
     plh_ctx* c = new plh_ctx();
     c->device = device;
+    c->heap.set_device(device);
     if (stream) {
         c->stream = (hipStream_t)stream;
     } else {
@@ -263,20 +256,24 @@ int plh_ctx_destroy(plh_ctx* ctx) {
 int plh_ctx_synchronize(plh_ctx* ctx) {
     if (!ctx) return PLH_ERR_ARG;
     PLH_HIP(hipStreamSynchronize(ctx->stream));
+    // Every blocking entry point ends here: large scratch goes back to the
+    // context's heap (its memory stays mapped), small scratch stays in its
+    // slots.
+    ctx->ws.release_large();
     return PLH_OK;
 }
 
 void* plh_ctx_stream(plh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int64_t plh_ctx_scratch_bytes(plh_ctx* ctx) {
-    return ctx ? (int64_t)(ctx->ws.bytes_held() + ctx->blocks.bytes_cached()) : 0;
+    return ctx ? (int64_t)(ctx->ws.bytes_held() + ctx->heap.bytes_held() - ctx->heap.bytes_live()) : 0;
 }
 
 int plh_ctx_trim(plh_ctx* ctx) {
     if (!ctx) return PLH_ERR_ARG;
     PLH_HIP(hipStreamSynchronize(ctx->stream));
     ctx->ws.release();
-    ctx->blocks.release_all();
+    ctx->heap.trim();
     return PLH_OK;
 }
 
@@ -318,6 +315,20 @@ int plh_ctx_profile_read(plh_ctx* ctx, int max_entries, char* names, double* tot
         ctx->prof_pool.push_back(r.b);
     }
     ctx->prof_recs.clear();
+    AllocClock& ac = alloc_clock();
+    if (ac.mallocs) {
+        keys.push_back("host:heap_map");
+        ms.push_back(ac.malloc_ms);
+        cnt.push_back(ac.mallocs);
+        byt.push_back(ac.malloc_bytes);
+    }
+    if (ac.frees) {
+        keys.push_back("host:heap_unmap");
+        ms.push_back(ac.free_ms);
+        cnt.push_back(ac.frees);
+        byt.push_back(0.0);
+    }
+    ac = AllocClock{};
     const int n = (int)keys.size() < max_entries ? (int)keys.size() : max_entries;
     for (int i = 0; i < n; ++i) {
         if (names) {

@@ -4,7 +4,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/paulib_hip.h"
@@ -31,13 +33,73 @@ const char* last_error();
         if (_rc != PLH_OK) return _rc; \
     } while (0)
 
-// Grow-only device scratch owned by a context.  Slots are independent;
-// get() keeps the old contents only when no reallocation happens.
+// Host time spent mapping and unmapping device memory (and the bytes of
+// physical memory created), reported with the kernel profile.
+struct AllocClock {
+    double malloc_ms = 0.0, free_ms = 0.0, malloc_bytes = 0.0;
+    int64_t mallocs = 0, frees = 0;
+};
+AllocClock& alloc_clock();
+
+// Device memory of a context: one reserved virtual address range, backed
+// chunk by chunk (256 MiB) with physical memory from hipMemCreate, and
+// carved into byte ranges.  A chunk's physical memory is cleared once, when
+// it is first created: new physical memory on this device costs about
+// 1 s per 60 GB, so a chunk whose ranges are all released stays mapped for
+// later requests, and when a request needs backing that is not there,
+// unused chunks are unmapped and mapped again where the request lies (a
+// remap keeps the memory and costs well under a millisecond).  Ranges are
+// placed where they need the fewest new chunks (then best fit), split on
+// allocation and merged with free neighbours on release, so the physical
+// memory held is the live bytes plus partly used chunks, whatever the
+// order of requests.  Every kernel of a context runs on its one stream, so
+// a range released by one call is handed to the next without a
+// synchronisation; only unmapping waits for the device.
+class DeviceHeap {
+   public:
+    ~DeviceHeap() { release(); }
+    void set_device(int device) { device_ = device; }
+    int alloc(size_t bytes, void** out);
+    void free(void* p);
+    // Unmaps the chunks without live ranges and gives their physical
+    // memory back to the device (synchronises).
+    void trim();
+    size_t bytes_held() const;
+    size_t bytes_live() const { return live_; }
+    static constexpr size_t kChunk = (size_t)256 << 20;
+
+   private:
+    int reserve();
+    int back(uintptr_t lo, uintptr_t hi);
+    int map_slot(size_t i, uintptr_t lo, uintptr_t hi);
+    void reclaim(uintptr_t keep_lo, uintptr_t keep_hi);
+    void release();
+    size_t slot_of(uintptr_t a) const { return (size_t)((a - (uintptr_t)va_) / kChunk); }
+    void add_live(uintptr_t lo, uintptr_t hi, bool add);
+    int device_ = 0;
+    char* va_ = nullptr;
+    size_t va_size_ = 0;
+    std::vector<hipMemGenericAllocationHandle_t> slot_;  // handle mapped at chunk i, or null
+    std::vector<size_t> slot_live_;                      // live bytes inside chunk i
+    std::vector<hipMemGenericAllocationHandle_t> pool_;  // created, not mapped
+    std::map<uintptr_t, size_t> free_;                   // free address ranges: start -> size
+    std::unordered_map<uintptr_t, size_t> used_;         // live ranges: start -> size
+    size_t live_ = 0, mapped_ = 0;
+};
+
+// Grow-only device scratch owned by a context, drawn from its heap.  Slots
+// are independent; get() keeps the old contents only when no reallocation
+// happens.
 class Workspace {
    public:
+    explicit Workspace(DeviceHeap* heap) : heap_(heap) {}
     ~Workspace() { release(); }
     int get(int slot, size_t bytes, void** out);
     void release();
+    // Returns the slots of 256 MiB and more to the heap, so that the
+    // regions under them can serve other shapes (sums, other slots) between
+    // calls; later calls take them back without a hipMalloc.
+    void release_large();
     size_t bytes_held() const;
 
    private:
@@ -45,28 +107,8 @@ class Workspace {
         void* p = nullptr;
         size_t n = 0;
     };
+    DeviceHeap* heap_;
     std::vector<Buf> bufs_;
-};
-
-// Caching device allocator owned by a context: blocks come in size classes
-// and freed blocks are kept for later requests of the same class.  All work of a
-// context is ordered on its one stream, so a block released by one call can
-// be handed to the next call without a synchronisation.
-class BlockCache {
-   public:
-    ~BlockCache() { release_all(); }
-    int alloc(size_t bytes, void** out);
-    void free(void* p);
-    void release_all();
-    size_t bytes_cached() const;
-
-   private:
-    struct Blk {
-        void* p;
-        size_t n;
-    };
-    std::vector<Blk> free_;
-    std::vector<Blk> used_;
 };
 
 // Small pinned host staging buffer for count read-backs.
@@ -109,8 +151,8 @@ struct plh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    plh::Workspace ws;
-    plh::BlockCache blocks;
+    plh::DeviceHeap heap;  // declared before ws: ws returns its ranges on destruction
+    plh::Workspace ws{&heap};
     plh::HostStage stage;
     bool prof = false;
     std::vector<plh_sum*> sums;  // live sums; orphaned (ctx = null) when the context goes

@@ -158,109 +158,6 @@ int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out) {
     return PLH_OK;
 }
 
-// ------------------------------------------------------------ workspace
-
-int Workspace::get(int slot, size_t bytes, void** out) {
-    if (slot < 0) return PLH_ERR_ARG;
-    if ((size_t)slot >= bufs_.size()) bufs_.resize((size_t)slot + 1);
-    Buf& b = bufs_[(size_t)slot];
-    if (bytes == 0) bytes = 1;
-    if (b.n < bytes) {
-        if (b.p) {
-            PLH_HIP(hipDeviceSynchronize());
-            PLH_HIP(hipFree(b.p));
-            b.p = nullptr;
-            b.n = 0;
-        }
-        // Grow geometrically (x2, at least 1 MiB) so a sum that grows step by
-        // step (propagation) reallocates each slot only a few times.
-        size_t want = std::max<size_t>(2 * bytes, (size_t)1 << 20);
-        PLH_HIP(hipMalloc(&b.p, want));
-        b.n = want;
-    }
-    *out = b.p;
-    return PLH_OK;
-}
-
-void Workspace::release() {
-    for (auto& b : bufs_) {
-        if (b.p) (void)hipFree(b.p);
-        b.p = nullptr;
-        b.n = 0;
-    }
-}
-
-size_t Workspace::bytes_held() const {
-    size_t t = 0;
-    for (auto& b : bufs_) t += b.n;
-    return t;
-}
-
-// Size classes: powers of two up to 64 MiB (at least 2 MiB), above that
-// multiples of 1/16 of the next power of two (under 6.25% padding).  A
-// request is served only by a cached block of its own class, so a sum that
-// grows step by step reuses the blocks it freed instead of allocating anew.
-static size_t size_class(size_t bytes) {
-    size_t p2 = (size_t)2 << 20;
-    while (p2 < bytes) p2 <<= 1;
-    if (p2 <= ((size_t)64 << 20)) return p2;
-    const size_t step = p2 / 16;
-    return (bytes + step - 1) / step * step;
-}
-
-int BlockCache::alloc(size_t bytes, void** out) {
-    if (bytes == 0) bytes = 256;
-    const size_t cls = size_class(bytes);
-    for (size_t i = 0; i < free_.size(); ++i) {
-        if (free_[i].n == cls) {
-            used_.push_back(free_[i]);
-            *out = free_[i].p;
-            free_.erase(free_.begin() + (long)i);
-            return PLH_OK;
-        }
-    }
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, cls);
-    if (e != hipSuccess) {
-        // Give the cached blocks back and retry once.
-        (void)hipGetLastError();
-        for (auto& b : free_) (void)hipFree(b.p);
-        free_.clear();
-        e = hipMalloc(&p, cls);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            set_error("device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
-            return PLH_ERR_OOM;
-        }
-    }
-    used_.push_back(Blk{p, cls});
-    *out = p;
-    return PLH_OK;
-}
-
-void BlockCache::free(void* p) {
-    if (!p) return;
-    for (size_t i = 0; i < used_.size(); ++i) {
-        if (used_[i].p == p) {
-            free_.push_back(used_[i]);
-            used_.erase(used_.begin() + (long)i);
-            return;
-        }
-    }
-    (void)hipFree(p);
-}
-
-void BlockCache::release_all() {
-    for (auto& b : free_) (void)hipFree(b.p);
-    free_.clear();
-}
-
-size_t BlockCache::bytes_cached() const {
-    size_t t = 0;
-    for (auto& b : free_) t += b.n;
-    return t;
-}
-
 static hipEvent_t take_event(plh_ctx* c) {
     if (!c->prof_pool.empty()) {
         hipEvent_t e = c->prof_pool.back();
