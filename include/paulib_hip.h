@@ -131,6 +131,17 @@ int plh_multiply_canonical(const plh_sum* lhs, const plh_sum* rhs, plh_sum* out)
  * (include/PauliSumOperations.h:255-306). */
 int plh_batch_multiply_canonical(int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
                                  plh_sum* out, int64_t* offsets);
+/* A batch of independent collated products, each into its own output sum:
+ * outs[k] = canonicalize(lhs[k] * rhs[k]), exactly as
+ * plh_multiply_canonical gives it.  Products of at least solo_pairs pairs
+ * (0: 2^22), and those over 128 qubits, run alone, straight into their
+ * output; the rest are collated together per array size (up to 2^31 pairs
+ * per collation, job plane leading the key) and split into their outputs
+ * in one launch.  Operands and outputs share one context; outputs are
+ * distinct and alias no operand.  No reference counterpart: the reference
+ * multiplies one pair of sums per call (include/PauliSumOperations.h:255-306). */
+int plh_batch_multiply_canonical_each(int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
+                                      plh_sum* const* outs, int64_t solo_pairs);
 /* commutator / i_commutator (include/PauliSumOperations.h:348-356), the
  * uncombined terms in (i over a, j over b) order; num_qubits = max. */
 int plh_commutator(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out);

@@ -12,6 +12,7 @@ from .device import (  # noqa: F401
     TruncationOptions,
     apply_truncation,
     batch_multiply_canonical,
+    batch_multiply_canonical_each,
     commutator,
     commutator_canonical,
     i_commutator,

@@ -67,6 +67,7 @@ SIGNATURES = {
     "plh_multiply": (_int, [_p, _p]),
     "plh_multiply_canonical": (_int, [_p, _p, _p]),
     "plh_batch_multiply_canonical": (_int, [_int, C.POINTER(_p), C.POINTER(_p), _p, _i64p]),
+    "plh_batch_multiply_canonical_each": (_int, [_int, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), _i64]),
     "plh_commutator": (_int, [_p, _p, _int, _p]),
     "plh_commutator_canonical": (_int, [_p, _p, _int, _p]),
     "plh_sort_and_combine": (_int, [_p]),

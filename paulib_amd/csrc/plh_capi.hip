@@ -39,6 +39,8 @@ void sum_free_storage(plh_sum* s) {
 void sum_take_storage(plh_sum* dst, plh_sum* src) {
     sum_free_storage(dst);
     dst->block = src->block;
+    dst->block_as = src->block_as;
+    dst->block_job = src->block_job;
     dst->x = src->x;
     dst->z = src->z;
     dst->f = src->f;
@@ -60,7 +62,14 @@ static size_t sum_bytes(int as, int64_t cap, bool job) {
 }
 
 int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
-    if (cap <= s->cap && s->block) return PLH_OK;
+    // the block's planes were laid out for block_as words and a job plane
+    // or not: a sum whose array size or job plane changed needs a new block
+    const bool same_layout = s->block_as == s->as && s->block_job == s->has_job;
+    if (cap <= s->cap && s->block && same_layout) return PLH_OK;
+    if (keep > 0 && s->block && !same_layout) {
+        set_error("sum_reserve: cannot keep terms across an array-size or job-plane change");
+        return PLH_ERR_ARG;
+    }
     int64_t nc = cap < 16 ? 16 : cap;
     if (s->cap > 0 && nc < s->cap + s->cap / 2 && keep > 0) nc = s->cap + s->cap / 2;  // amortise appends
     // nothing to keep: give the old block back first, so the two need not
@@ -92,6 +101,8 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
     }
     sum_free_storage(s);  // stream-ordered reuse: no synchronisation needed
     s->block = blk;
+    s->block_as = s->as;
+    s->block_job = s->has_job;
     s->j = nj;
     s->x = nx;
     s->z = nz;
@@ -286,6 +297,7 @@ int plh_ctx_profile(plh_ctx* ctx, int enable) {
     }
     ctx->prof_recs.clear();
     ctx->prof = enable != 0;
+    alloc_clock() = AllocClock{};
     return PLH_OK;
 }
 
@@ -528,6 +540,13 @@ int plh_multiply_canonical(const plh_sum* lhs, const plh_sum* rhs, plh_sum* out)
     return plh_ctx_synchronize(lhs->ctx);
 }
 
+// Products of at least this many pairs run alone in
+// plh_batch_multiply_canonical_each: from there a product fills the device
+// by itself and batching only adds the job word to its keys.
+constexpr int64_t kSoloPairsDefault = (int64_t)1 << 22;
+// The batched collation's 32-bit ids bound the pairs of one collation.
+constexpr int64_t kBatchPairsMax = (int64_t)1 << 31;
+
 int plh_batch_multiply_canonical(int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs, plh_sum* out,
                                  int64_t* offsets) {
     if (njobs <= 0 || !lhs || !rhs || !offsets) {
@@ -551,6 +570,93 @@ int plh_batch_multiply_canonical(int njobs, const plh_sum* const* lhs, const plh
     PLH_TRY(collate_products(out->ctx, njobs, lhs, rhs, canon_spec(), out, offsets));
     out->nq = nq;
     return plh_ctx_synchronize(out->ctx);
+}
+
+int plh_batch_multiply_canonical_each(int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
+                                      plh_sum* const* outs, int64_t solo_pairs) {
+    if (njobs <= 0 || !lhs || !rhs || !outs || solo_pairs < 0) {
+        set_error("plh_batch_multiply_canonical_each: bad job list or threshold");
+        return PLH_ERR_ARG;
+    }
+    if (solo_pairs == 0) solo_pairs = kSoloPairsDefault;
+    plh_ctx* ctx = nullptr;
+    for (int k = 0; k < njobs; ++k) {
+        PLH_TRY(check_pair(lhs[k], rhs[k]));
+        PLH_TRY(check_sum(outs[k]));
+        if (!ctx) ctx = outs[k]->ctx;
+        if (lhs[k]->ctx != ctx || outs[k]->ctx != ctx) {
+            set_error("plh_batch_multiply_canonical_each: operands and outputs must share one context");
+            return PLH_ERR_ARG;
+        }
+        if (lhs[k]->n > 0x7FFFFFFF || rhs[k]->n > 0x7FFFFFFF) {
+            set_error("plh_batch_multiply_canonical_each: input exceeds 2^31-1 terms");
+            return PLH_ERR_SIZE;
+        }
+    }
+    {  // every output distinct, and none an operand of any job
+        std::vector<const plh_sum*> o(outs, outs + njobs), ops;
+        for (int k = 0; k < njobs; ++k) {
+            ops.push_back(lhs[k]);
+            ops.push_back(rhs[k]);
+        }
+        std::sort(o.begin(), o.end());
+        std::sort(ops.begin(), ops.end());
+        bool bad = std::adjacent_find(o.begin(), o.end()) != o.end();
+        for (const plh_sum* x : o) bad |= std::binary_search(ops.begin(), ops.end(), x);
+        if (bad) {
+            set_error("plh_batch_multiply_canonical_each: outputs must be distinct and must not alias an operand");
+            return PLH_ERR_ARG;
+        }
+    }
+    const CollateSpec spec = canon_spec();
+    // large products (and those over 128 qubits) alone: one collation each,
+    // written straight into their output
+    auto solo = [&](int k) { return lhs[k]->n * rhs[k]->n >= solo_pairs || lhs[k]->as > 2; };
+    for (int k = 0; k < njobs; ++k) {
+        if (!solo(k)) continue;
+        PLH_TRY(collate_pairs(ctx, lhs[k], rhs[k], 0, 0, spec, outs[k]));
+        outs[k]->nq = std::max(lhs[k]->nq, rhs[k]->nq);
+    }
+    // the rest batched: per array size, in job order, up to kBatchPairsMax pairs per collation
+    plh_sum tmp;
+    tmp.ctx = ctx;
+    tmp.has_job = true;
+    int rc = PLH_OK;
+    for (int as = 1; as <= 2 && rc == PLH_OK; ++as) {
+        std::vector<int> group;
+        int64_t gp = 0;
+        auto flush = [&]() -> int {
+            if (group.empty()) return PLH_OK;
+            const int g = (int)group.size();
+            std::vector<const plh_sum*> L((size_t)g), R((size_t)g);
+            std::vector<plh_sum*> O((size_t)g);
+            for (int i = 0; i < g; ++i) {
+                L[(size_t)i] = lhs[group[(size_t)i]];
+                R[(size_t)i] = rhs[group[(size_t)i]];
+                O[(size_t)i] = outs[group[(size_t)i]];
+            }
+            std::vector<int64_t> off((size_t)g + 1, 0);
+            tmp.as = as;
+            PLH_TRY(collate_products(ctx, g, L.data(), R.data(), spec, &tmp, off.data()));
+            PLH_TRY(split_jobs(ctx, &tmp, g, off.data(), O.data()));
+            for (int i = 0; i < g; ++i) O[(size_t)i]->nq = std::max(L[(size_t)i]->nq, R[(size_t)i]->nq);
+            group.clear();
+            gp = 0;
+            return PLH_OK;
+        };
+        for (int k = 0; k < njobs && rc == PLH_OK; ++k) {
+            if (lhs[k]->as != as || solo(k)) continue;
+            const int64_t pairs = lhs[k]->n * rhs[k]->n;
+            if (gp + pairs > kBatchPairsMax) rc = flush();
+            group.push_back(k);
+            gp += pairs;
+        }
+        if (rc == PLH_OK) rc = flush();
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    sum_free_storage(&tmp);
+    PLH_TRY(rc);
+    return plh_ctx_synchronize(ctx);
 }
 
 int plh_commutator(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out) {

@@ -1870,6 +1870,31 @@ __global__ void k_job_offsets(const uint32_t* __restrict__ J, int64_t n, uint32_
     off[k] = lo;
 }
 
+// Destination of one job when a batched sum is split into per-job sums.
+struct SplitDst {
+    uint64_t* x;
+    uint64_t* z;
+    uint8_t* f;
+    double* v;
+    int64_t stride;
+    int64_t off;  // the job's first term in the batch
+};
+
+// Term t of a job-ordered batch goes to its job's sum at t - off.
+__global__ __launch_bounds__(kBlock) void k_split_jobs(SumView b, const uint32_t* __restrict__ J, int as, int64_t n,
+                                                       const SplitDst* __restrict__ dst) {
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        const SplitDst d = dst[J[t]];
+        const int64_t i = t - d.off;
+        for (int w = 0; w < as; ++w) {
+            d.x[(int64_t)w * d.stride + i] = b.x[(int64_t)w * b.stride + t];
+            d.z[(int64_t)w * d.stride + i] = b.z[(int64_t)w * b.stride + t];
+        }
+        d.f[i] = b.f[t];
+        d.v[i] = b.v[t];
+    }
+}
+
 template <int AS>
 int collate_products_as(plh_ctx* ctx, int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
                         const CollateSpec& spec, plh_sum* out, uint64_t total_pairs) {
@@ -2106,6 +2131,44 @@ int collate_terms(plh_ctx* ctx, plh_sum* in, const CollateSpec& spec, plh_sum* o
         in->n = n;
     }
     out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    return PLH_OK;
+}
+
+int split_jobs(plh_ctx* ctx, const plh_sum* batch, int njobs, const int64_t* offsets, plh_sum* const* outs) {
+    if (!batch->has_job || (batch->n > 0 && !batch->j)) {
+        set_error("split_jobs: the batch has no job plane");
+        return PLH_ERR_ARG;
+    }
+    if (offsets[0] != 0 || offsets[njobs] != batch->n) {
+        set_error("split_jobs: offsets do not cover the batch");
+        return PLH_ERR_ARG;
+    }
+    std::vector<SplitDst> d((size_t)njobs);
+    for (int k = 0; k < njobs; ++k) {
+        const int64_t cnt = offsets[k + 1] - offsets[k];
+        if (cnt < 0) {
+            set_error("split_jobs: offsets decrease");
+            return PLH_ERR_ARG;
+        }
+        plh_sum* o = outs[k];
+        o->as = batch->as;
+        o->has_job = false;
+        PLH_TRY(sum_reserve(o, cnt, 0));
+        o->n = cnt;
+        o->canonical = batch->canonical;
+        d[(size_t)k] = SplitDst{o->x, o->z, o->f, o->v, o->cap, offsets[k]};
+    }
+    if (batch->n == 0) return PLH_OK;
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsJobs, (size_t)njobs * sizeof(SplitDst) + 256, &p));
+    PLH_HIP(hipMemcpyAsync(p, d.data(), (size_t)njobs * sizeof(SplitDst), hipMemcpyHostToDevice, ctx->stream));
+    const uint64_t n = (uint64_t)batch->n;
+    k_split_jobs<<<(unsigned)std::min<uint64_t>(grid1(n), 4096), kBlock, 0, ctx->stream>>>(
+        SumView{batch->x, batch->z, batch->f, batch->v, batch->cap}, batch->j, batch->as, batch->n,
+        (const SplitDst*)p);
+    PLH_HIP(hipGetLastError());
+    // the host table is read by the copy before the caller's synchronisation returns
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
     return PLH_OK;
 }
 

@@ -42,7 +42,7 @@ struct AllocClock {
 AllocClock& alloc_clock();
 
 // Device memory of a context: one reserved virtual address range, backed
-// chunk by chunk (256 MiB) with physical memory from hipMemCreate, and
+// chunk by chunk (2 GiB) with physical memory from hipMemCreate, and
 // carved into byte ranges.  A chunk's physical memory is cleared once, when
 // it is first created: new physical memory on this device costs about
 // 1 s per 60 GB, so a chunk whose ranges are all released stays mapped for
@@ -66,7 +66,7 @@ class DeviceHeap {
     void trim();
     size_t bytes_held() const;
     size_t bytes_live() const { return live_; }
-    static constexpr size_t kChunk = (size_t)256 << 20;
+    static constexpr size_t kChunk = (size_t)2 << 30;
 
    private:
     int reserve();
@@ -134,6 +134,8 @@ struct plh_sum {
     double* v = nullptr;
     uint32_t* j = nullptr;   // job plane of a batched sum (has_job), else null
     bool has_job = false;
+    int block_as = 0;        // layout of the block: words per plane and job plane
+    bool block_job = false;
     bool canonical = false;  // (x, z, imag) keys distinct, flags in {0,1}, no zeros
 };
 
@@ -248,6 +250,9 @@ int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, con
 // range (njobs + 1 entries, host).  Sum of n_k * m_k <= 2^32.
 int collate_products(plh_ctx* ctx, int njobs, const plh_sum* const* lhs, const plh_sum* const* rhs,
                      const CollateSpec& spec, plh_sum* out, int64_t* offsets);
+// outs[k] = job k's range offsets[k] .. offsets[k + 1] (host) of a
+// job-ordered batched sum, for k < njobs, copied in one launch.
+int split_jobs(plh_ctx* ctx, const plh_sum* batch, int njobs, const int64_t* offsets, plh_sum* const* outs);
 // out = collate(L x R) for kind 0 (product, out[i*n+j] = L[j]*R[i]) or
 // kind 1 (commutator of L with R, by_i adds the i factor).
 int collate_pairs(plh_ctx* ctx, const plh_sum* L, const plh_sum* R, int kind, int by_i,

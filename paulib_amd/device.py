@@ -306,6 +306,24 @@ def batch_multiply_canonical(lhs: list, rhs: list, out: PauliSum | None = None):
     return out, off
 
 
+def batch_multiply_canonical_each(lhs: list, rhs: list, outs: list | None = None, solo_pairs: int = 0) -> list:
+    """canonicalize(lhs[k] * rhs[k]) into outs[k] for every k: products of at
+    least solo_pairs pairs (0: the engine's default) run alone, the rest are
+    collated together and split.  Returns the output sums."""
+    if len(lhs) != len(rhs) or not lhs:
+        raise ValueError("lhs and rhs must be non-empty lists of equal length")
+    k = len(lhs)
+    if outs is None:
+        outs = [PauliSum(a.ctx, a.array_size, a.num_qubits()) for a in lhs]
+    if len(outs) != k:
+        raise ValueError("one output per job")
+    la = (C.c_void_p * k)(*[a._h.value for a in lhs])
+    ra = (C.c_void_p * k)(*[b._h.value for b in rhs])
+    oa = (C.c_void_p * k)(*[o._h.value for o in outs])
+    _lib.call("plh_batch_multiply_canonical_each", k, la, ra, oa, int(solo_pairs))
+    return outs
+
+
 def job_terms(out: PauliSum, offsets, k: int) -> HostSum:
     """Job k's terms of a batched result."""
     return out.range_to_host(int(offsets[k]), int(offsets[k + 1] - offsets[k]))

@@ -23,7 +23,7 @@
 // ------------------------------------------------------------ fake device
 namespace sim {
 constexpr uintptr_t kBase = 0x100000000000ull;
-constexpr size_t kTotal = (size_t)8 << 30;  // "device memory": 32 chunks of 256 MiB
+constexpr size_t kTotal = 32 * plh::DeviceHeap::kChunk;  // "device memory": 32 chunks
 size_t cap_chunks = 32;
 size_t created = 0;
 std::set<uintptr_t> mapped;                // chunk addresses

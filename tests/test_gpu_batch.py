@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from helpers import assert_terms_equal, cpu
-from paulib_amd import EngineError, PauliSum, batch_multiply_canonical, job_terms
+from paulib_amd import EngineError, PauliSum, batch_multiply_canonical, batch_multiply_canonical_each, job_terms
 from paulib_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -70,3 +70,47 @@ def test_batch_rejects_mixed_array_sizes(ctx):
     b = PauliSum.from_host(ctx, W.random_sum(10, 128, 0.5, "uniform", 2))
     with pytest.raises(EngineError):
         batch_multiply_canonical([a, a], [a, b])
+
+
+def _check_each(ctx, jobs, solo_pairs):
+    L = [PauliSum.from_host(ctx, a) for a, _ in jobs]
+    R = [PauliSum.from_host(ctx, b) for _, b in jobs]
+    outs = batch_multiply_canonical_each(L, R, solo_pairs=solo_pairs)
+    for k, (a, b) in enumerate(jobs):
+        got = outs[k].to_host()
+        assert_terms_equal(got, _oracle(a, b))
+        assert_terms_equal(got, L[k].multiply_canonical(R[k]).to_host())
+    return outs
+
+
+@pytest.mark.parametrize("solo_pairs", [1, 50_000, 1 << 40])
+def test_batch_each_matches_plain(ctx, solo_pairs):
+    # mixed array sizes, uneven sizes, an empty operand and heavy collisions;
+    # solo_pairs 1: every job alone; 1 << 40: every job batched; else a mix
+    jobs = _jobs(6, 16, [16, 64, 100, 128], hi=1200)
+    jobs.insert(3, (W.random_sum(0, 32, 0.5, "uniform", 1), W.random_sum(40, 32, 0.5, "uniform", 2)))
+    jobs.insert(7, (W.random_sum(900, 6, 1.0, "uniform", 5), W.random_sum(700, 6, 1.0, "uniform", 6)))
+    jobs.insert(9, (W.random_sum(60, 200, 0.5, "uniform", 7), W.random_sum(50, 200, 0.5, "uniform", 8)))
+    _check_each(ctx, jobs, solo_pairs)
+
+
+def test_batch_each_reuses_outputs(ctx):
+    # the same outputs twice, the second batch with smaller jobs
+    jobs = _jobs(9, 6, [32, 64], lo=500, hi=2000)
+    outs = _check_each(ctx, jobs, 200_000)
+    small = _jobs(10, 6, [32, 64], lo=1, hi=50)
+    L = [PauliSum.from_host(ctx, a) for a, _ in small]
+    R = [PauliSum.from_host(ctx, b) for _, b in small]
+    batch_multiply_canonical_each(L, R, outs, solo_pairs=200_000)
+    for k, (a, b) in enumerate(small):
+        assert_terms_equal(outs[k].to_host(), _oracle(a, b))
+
+
+def test_batch_each_rejects_aliasing(ctx):
+    a = PauliSum.from_host(ctx, W.random_sum(10, 64, 0.5, "uniform", 1))
+    b = PauliSum.from_host(ctx, W.random_sum(10, 64, 0.5, "uniform", 2))
+    o = PauliSum(ctx, 1, 64)
+    with pytest.raises(EngineError):
+        batch_multiply_canonical_each([a, a], [b, b], [o, o])
+    with pytest.raises(EngineError):
+        batch_multiply_canonical_each([a], [b], [a])
