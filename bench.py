@@ -129,12 +129,18 @@ def serving(ctx, n_jobs: int, rate: float, n_plain: int, max_jobs: int):
     }
 
 
-def batched_products(ctx, count: int, budget_pairs: int):
+def batched_products(ctx, count: int, budget_pairs: int, solo_pairs: int):
     """C3: `count` independent collated products of uneven sizes (terms
-    log-uniform in [64, 65536], qubits from {16, 32, 64, 128}), batched per
-    array size into collations of at most budget_pairs pairs, against the
-    plain drop-in (plh_multiply_canonical) looped over the same jobs."""
-    from paulib_amd import PauliSum, batch_multiply_canonical
+    log-uniform in [64, 65536], qubits from {16, 32, 64, 128}), three ways:
+    - batched, one output per job (plh_batch_multiply_canonical_each): jobs
+      in groups whose outputs fit at once (budget_pairs pairs); inside a
+      group products of at least solo_pairs pairs run alone and the rest
+      share one collation;
+    - one collation per group (plh_batch_multiply_canonical, per array
+      size): every product of the group in one job-keyed collation;
+    - the plain drop-in (plh_multiply_canonical) looped over the jobs.
+    Outputs are reused from group to group (results are discarded)."""
+    from paulib_amd import PauliSum, batch_multiply_canonical, batch_multiply_canonical_each
     from paulib_amd import workloads as W
 
     spec = W.batch_products(count)
@@ -145,43 +151,56 @@ def batched_products(ctx, count: int, budget_pairs: int):
         R.append(PauliSum.from_host(ctx, W.random_sum(m, nq, 0.75, "uniform", seed + 1)))
         pairs.append(n * m)
     log(f"batch: {count} jobs, {sum(pairs):.4g} pairs, inputs built in {time.perf_counter() - t0:.1f} s")
-    groups = []
-    for a in (1, 2):
-        idx = [k for k in range(count) if L[k].array_size == a]
-        cur, tot = [], 0
-        for k in idx:
-            if cur and tot + pairs[k] > budget_pairs:
+
+    def grouped(keys):
+        groups = []
+        for a in keys:
+            idx = [k for k in range(count) if a is None or L[k].array_size == a]
+            cur, tot = [], 0
+            for k in idx:
+                if cur and tot + pairs[k] > budget_pairs:
+                    groups.append(cur)
+                    cur, tot = [], 0
+                cur.append(k)
+                tot += pairs[k]
+            if cur:
                 groups.append(cur)
-                cur, tot = [], 0
-            cur.append(k)
-            tot += pairs[k]
-        if cur:
-            groups.append(cur)
+        return groups
+
+    each_groups = grouped([None])
+    one_groups = grouped([1, 2])
     terms = 0
+
+    # a group's outputs are released before the next group runs (their
+    # memory stays mapped in the context's heap for the next outputs)
+    def run_each():
+        nonlocal terms
+        terms = 0
+        for g in each_groups:
+            o = batch_multiply_canonical_each([L[k] for k in g], [R[k] for k in g], solo_pairs=solo_pairs)
+            terms += sum(len(x) for x in o)
+            for x in o:
+                x.close()
 
     # One output sum at a time: the largest 128-qubit jobs alone need most of
     # the HBM, so the output of one array size goes back to the context's
-    # heap before the other's jobs run.  The heap keeps its regions, so both
-    # loops run on warm device memory (the warm-up pass grows it).
-    def run_batched():
-        nonlocal terms
-        terms = 0
+    # heap before the other's jobs run.
+    def run_one():
         out, cur = None, None
-        for g in groups:
+        for g in one_groups:
             a = L[g[0]].array_size
             if a != cur:
                 if out is not None:
                     out.close()
                 out, cur = PauliSum(ctx, a, 64 * a), a
             out, off = batch_multiply_canonical([L[k] for k in g], [R[k] for k in g], out)
-            terms += int(off[-1])
         out.close()
 
-    def run_plain():
+    def run_plain(order):
         from paulib_amd import _lib
 
         out, cur = None, None
-        for k in range(count):
+        for k in order:
             a = L[k].array_size
             if a != cur:
                 if out is not None:
@@ -190,30 +209,33 @@ def batched_products(ctx, count: int, budget_pairs: int):
             _lib.call("plh_multiply_canonical", L[k].handle, R[k].handle, out.handle)
         out.close()
 
-    run_batched()  # warm-up
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    run_batched()
-    ctx.synchronize()
-    t_batch = time.perf_counter() - t0
-    order = sorted(range(count), key=lambda k: L[k].array_size)
-    L, R = [L[k] for k in order], [R[k] for k in order]
-    t0 = time.perf_counter()
-    run_plain()
-    ctx.synchronize()
-    t_plain = time.perf_counter() - t0
+    def timed(f, *a):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        f(*a)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    run_each()  # warm-up: grows the context's heap to the workload's working set
+    t_each = timed(run_each)
+    t_one = timed(run_one)
+    t_plain = timed(run_plain, sorted(range(count), key=lambda k: L[k].array_size))
     for s_ in L + R:
         s_.close()
     ctx.trim()
     total = sum(pairs)
-    log(f"batch: {len(groups)} collations, {t_batch:.3f} s ({total / t_batch:.4g} pairs/s); "
-        f"plain loop {t_plain:.3f} s ({total / t_plain:.4g} pairs/s)")
+    solo = sum(1 for k in range(count) if pairs[k] >= solo_pairs or L[k].array_size > 2)
+    log(f"batch: {len(each_groups)} groups ({solo} products alone, {count - solo} batched), {t_each:.3f} s "
+        f"({total / t_each:.4g} pairs/s); one collation per group {t_one:.3f} s; plain loop {t_plain:.3f} s "
+        f"({total / t_plain:.4g} pairs/s)")
     return {
         "workload": (f"{count} independent collated products, terms per sum log-uniform in [64, 65536], "
                      "qubits from {16, 32, 64, 128}, sites non-I w.p. 0.25, U[-1,1] FP64"),
-        "jobs": count, "pairs": total, "collated_terms": terms, "collations": len(groups),
-        "batch_s": t_batch, "pairs_per_s": total / t_batch, "jobs_per_s": count / t_batch,
-        "plain_loop_s": t_plain, "plain_pairs_per_s": total / t_plain, "speedup_vs_plain": t_plain / t_batch,
+        "jobs": count, "pairs": total, "collated_terms": terms, "groups": len(each_groups),
+        "solo_pairs": solo_pairs, "products_alone": solo,
+        "batch_s": t_each, "pairs_per_s": total / t_each, "jobs_per_s": count / t_each,
+        "one_collation_per_group_s": t_one, "one_collation_groups": len(one_groups),
+        "plain_loop_s": t_plain, "plain_pairs_per_s": total / t_plain, "speedup_vs_plain": t_plain / t_each,
     }
 
 
@@ -266,6 +288,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--batch-jobs", type=int, default=512)
     ap.add_argument("--batch-pairs", type=int, default=1 << 31)
+    ap.add_argument("--solo-pairs", type=int, default=1 << 22)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-propagation", action="store_true")
@@ -365,7 +388,7 @@ def main():
                                  "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                                  "jobs_per_s": float(1e3 / lat.mean()), "expectation": e}
     if args.batch_jobs > 0:
-        result["batched_products"] = batched_products(ctx, args.batch_jobs, args.batch_pairs)
+        result["batched_products"] = batched_products(ctx, args.batch_jobs, args.batch_pairs, args.solo_pairs)
     if args.serve_jobs > 0:
         result["serving"] = serving(ctx, args.serve_jobs, args.serve_rate, args.serve_plain, args.serve_slots)
     if not args.no_cpu_baseline:
