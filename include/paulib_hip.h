@@ -54,8 +54,6 @@ const char* plh_build_info(void);
 /* stream == NULL creates a private stream at the default priority. */
 int plh_ctx_create(int device, void* stream, plh_ctx** out);
 int plh_ctx_destroy(plh_ctx* ctx);
-/* Waits for the context's stream; scratch slots of 256 MiB and more go back
- * to the context's device heap (which keeps its memory mapped). */
 int plh_ctx_synchronize(plh_ctx* ctx);
 void* plh_ctx_stream(plh_ctx* ctx);
 /* Device memory the context holds beyond its live sums: scratch slots plus

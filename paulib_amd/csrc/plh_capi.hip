@@ -267,10 +267,6 @@ int plh_ctx_destroy(plh_ctx* ctx) {
 int plh_ctx_synchronize(plh_ctx* ctx) {
     if (!ctx) return PLH_ERR_ARG;
     PLH_HIP(hipStreamSynchronize(ctx->stream));
-    // Every blocking entry point ends here: large scratch goes back to the
-    // context's heap (its memory stays mapped), small scratch stays in its
-    // slots.
-    ctx->ws.release_large();
     return PLH_OK;
 }
 

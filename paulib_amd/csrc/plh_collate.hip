@@ -177,6 +177,8 @@ PLH_DEV int root_pos(const uint64_t* live, int kb) {
     return pos >= kb ? 0 : pos;
 }
 
+constexpr int kHistU = 8;  // items per thread per round of the histogram kernels
+
 // First pass: the root bucket, ids enumerated implicitly (dense index i is
 // item i of the source), one digit position for the whole root.
 template <class Src>
@@ -189,10 +191,20 @@ __global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense,
     const uint64_t t = blockIdx.x;
     const uint64_t beg = t * tile_items;
     const uint64_t end = min(n_dense, beg + tile_items);
-    for (uint64_t i = beg + threadIdx.x; i < end; i += kBlock) {
-        const uint32_t id = src.dense_to_id((uint32_t)i);
-        if (!src.valid(id)) continue;
-        atomicAdd(&h[key_window<kDigitBits>(src, id, pos)], 1u);
+    for (uint64_t i0 = beg + threadIdx.x; i0 < end; i0 += (uint64_t)kBlock * kHistU) {
+        uint32_t dg[kHistU];
+#pragma unroll
+        for (int u = 0; u < kHistU; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kBlock;
+            dg[u] = kInvalid;
+            if (i < end) {
+                const uint32_t id = src.dense_to_id((uint32_t)i);
+                if (src.valid(id)) dg[u] = key_window<kDigitBits>(src, id, pos);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kHistU; ++u)
+            if (dg[u] != kInvalid) atomicAdd(&h[dg[u]], 1u);
     }
     __syncthreads();
     hist[t * kDigits + threadIdx.x] = h[threadIdx.x];
@@ -409,11 +421,28 @@ __global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const uint32_t* _
     const int pos = (int)B.pos[bi];
     const uint32_t* ids = meta_buf(B.meta[bi]) ? ids1 : ids0;
     uint64_t o = 0ull, a = ~0ull;
-    for (uint64_t i = beg + threadIdx.x; i < end; i += kBlock) {
-        uint64_t rest;
-        atomicAdd(&h[digit_and_rest(src, ids[i], pos, rest)], 1u);
-        o |= rest;
-        a &= rest;
+    // kHistU items per thread per round: their id loads, then their key
+    // gathers, are all in flight before the first LDS atomic
+    for (uint64_t i0 = beg + threadIdx.x; i0 < end; i0 += (uint64_t)kBlock * kHistU) {
+        uint32_t idv[kHistU], dg[kHistU];
+#pragma unroll
+        for (int u = 0; u < kHistU; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kBlock;
+            idv[u] = i < end ? ids[i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kHistU; ++u) {
+            dg[u] = kInvalid;
+            if (i0 + (uint64_t)u * kBlock < end) {
+                uint64_t rest;
+                dg[u] = digit_and_rest(src, idv[u], pos, rest);
+                o |= rest;
+                a &= rest;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kHistU; ++u)
+            if (dg[u] != kInvalid) atomicAdd(&h[dg[u]], 1u);
     }
 #pragma unroll
     for (int s = kWave / 2; s > 0; s >>= 1) {

@@ -88,18 +88,15 @@ class DeviceHeap {
 };
 
 // Grow-only device scratch owned by a context, drawn from its heap.  Slots
-// are independent; get() keeps the old contents only when no reallocation
-// happens.
+// are independent and stay until plh_ctx_trim, so a repeated call of one
+// shape allocates nothing; get() keeps the old contents only when no
+// reallocation happens.
 class Workspace {
    public:
     explicit Workspace(DeviceHeap* heap) : heap_(heap) {}
     ~Workspace() { release(); }
     int get(int slot, size_t bytes, void** out);
     void release();
-    // Returns the slots of 256 MiB and more to the heap, so that the
-    // regions under them can serve other shapes (sums, other slots) between
-    // calls; later calls take them back without a hipMalloc.
-    void release_large();
     size_t bytes_held() const;
 
    private:

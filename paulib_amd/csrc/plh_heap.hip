@@ -13,7 +13,7 @@ namespace plh {
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr size_t kLargeMin = (size_t)256 << 20;  // workspace slots from here on are returned after each call
+constexpr size_t kSlotDoubleMax = (size_t)256 << 20;  // workspace slots grow x2 up to here, then by 1/8
 
 size_t round_up(size_t v, size_t q) { return (v + q - 1) / q * q; }
 
@@ -245,22 +245,12 @@ int Workspace::get(int slot, size_t bytes, void** out) {
         // Grow geometrically so a sum that grows step by step (propagation)
         // moves each slot only a few times: x2 (at least 1 MiB) up to 256 MiB,
         // then by an eighth.
-        const size_t want = bytes <= ((size_t)256 << 20) ? std::max<size_t>(2 * bytes, (size_t)1 << 20)
-                                                         : bytes + bytes / 8;
+        const size_t want = bytes <= kSlotDoubleMax ? std::max<size_t>(2 * bytes, (size_t)1 << 20) : bytes + bytes / 8;
         PLH_TRY(heap_->alloc(want, &b.p));
         b.n = want;
     }
     *out = b.p;
     return PLH_OK;
-}
-
-void Workspace::release_large() {
-    for (auto& b : bufs_) {
-        if (b.n < kLargeMin) continue;
-        heap_->free(b.p);
-        b.p = nullptr;
-        b.n = 0;
-    }
 }
 
 void Workspace::release() {
