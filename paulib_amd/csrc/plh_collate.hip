@@ -505,16 +505,15 @@ __global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, uint32_t* __re
     __syncthreads();
     for (uint64_t r0 = beg; r0 < end; r0 += kRound) {
         uint32_t dig[kIpt], rank[kIpt], idv[kIpt];
+        // every id load of the round, then every key gather, before the split
 #pragma unroll
         for (int s = 0; s < kIpt; ++s) {
             const uint64_t i = r0 + round_pos(s);
-            dig[s] = kInvalid;
-            idv[s] = 0;
-            if (i < end) {
-                idv[s] = ids_in[i];
-                dig[s] = key_window<kDigitBits>(src, idv[s], pos);
-            }
+            idv[s] = i < end ? ids_in[i] : 0u;
         }
+#pragma unroll
+        for (int s = 0; s < kIpt; ++s)
+            dig[s] = r0 + round_pos(s) < end ? key_window<kDigitBits>(src, idv[s], pos) : kInvalid;
         msplit_round(sh, dig, rank);
 #pragma unroll
         for (int s = 0; s < kIpt; ++s)
@@ -1018,19 +1017,24 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             const uint32_t p = tid + k * kLeafThreads;
             idv[k] = p < n ? ids[p] : 0u;
         }
+        // every gather into registers first, then the LDS stores: the
+        // stores cannot then hold back the next item's loads
+        uint64_t xz[kPer][KW];
+        uint32_t fl[kPer];
+        double bv[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t p = tid + k * kLeafThreads;
+            if (p < n) src.key_value(idv[k], xz[k], fl[k], bv[k]);
+        }
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
             if (p < n) {
-                uint64_t xz[KW];
-                uint32_t fl;
-                double bv;
-                src.key_value(idv[k], xz, fl, bv);
-                const double v = fold_value(c, bv, fl);
 #pragma unroll
-                for (int w = 0; w < KW; ++w) L.key[w][p] = xz[w];
-                L.fl[p] = (uint8_t)fl;
-                L.val[p] = v;
+                for (int w = 0; w < KW; ++w) L.key[w][p] = xz[k][w];
+                L.fl[p] = (uint8_t)fl[k];
+                L.val[p] = fold_value(c, bv[k], fl[k]);
             }
         }
     }
@@ -1201,16 +1205,19 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     __syncthreads();
     // ---- runs: thread t owns sorted positions [t*kPer, t*kPer + kPer);
     // each run is summed acc = 0; acc += v in (key, id) order
+    // Slots are indexed by k (compile-time after unrolling), never by a
+    // running count, so sums[] and kept[] stay in registers.
     double sums[kPer];
-    uint16_t heads[kPer];
+    bool kept[kPer];
     uint32_t nk = 0;
+    const uint32_t q0 = (uint32_t)tid * kPer;
     {
-        const uint32_t q0 = (uint32_t)tid * kPer;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t q = q0 + k;
-            if (q >= n) break;
-            if (!head[q]) continue;
+            kept[k] = false;
+            sums[k] = 0.0;
+            if (q >= n || !head[q]) continue;
             double acc = 0.0;
             acc += vs[q];
             for (uint32_t r = q + 1; r < n;) {
@@ -1237,8 +1244,8 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
 #pragma unroll
             for (int w = 0; w < KW; ++w) xz[w] = L.key[w][s];
             if (keep_term<AS, XO>(c, xz, acc)) {
-                sums[nk] = acc;
-                heads[nk] = (uint16_t)s;
+                sums[k] = acc;
+                kept[k] = true;
                 ++nk;
             }
         }
@@ -1246,11 +1253,17 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     uint32_t total;
     const uint32_t r0 = block_exclusive_scan<kLeafWaves>(nk, total, L.red);
     // the scan's barriers order the run sums (and their reads of head[])
-    // before these writes
+    // before these writes; perm is unchanged since the sort
     uint16_t* rslot = L.perm2;
-    for (uint32_t k = 0; k < nk; ++k) {
-        rslot[r0 + k] = heads[k];
-        L.val[heads[k]] = sums[k];
+    {
+        uint32_t r = r0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (!kept[k]) continue;
+            const int s = L.perm[q0 + k];
+            rslot[r++] = (uint16_t)s;
+            L.val[s] = sums[k];
+        }
     }
     __syncthreads();
     for (uint32_t r = tid; r < total; r += kLeafThreads) {
