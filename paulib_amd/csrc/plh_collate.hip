@@ -119,9 +119,16 @@ PLH_DEV double fold_value(const Cfg& c, double base, uint32_t raw) {
 struct MsplitShared {
     uint32_t wave_cnt[kWavesPerBlock][kDigits];
     uint32_t running[kDigits];
+    uint32_t rbefore[kDigits];       // running[] before this round
+    uint32_t roff[kDigits];          // this round's items of digit d start at roff[d] in stage
+    uint32_t scan[kWavesPerBlock];
+    uint32_t stage_id[kRound];       // this round's ids in (digit, rank) order
+    uint8_t stage_dig[kRound];
 };
+static_assert(kDigits <= 256, "stage_dig holds a digit in one byte");
 
-PLH_DEV void msplit_round(MsplitShared& sh, const uint32_t (&dig)[kIpt], uint32_t (&rank)[kIpt]) {
+// Returns the number of valid items of the round (uniform).
+PLH_DEV uint32_t msplit_round(MsplitShared& sh, const uint32_t (&dig)[kIpt], uint32_t (&rank)[kIpt]) {
     const int w = threadIdx.x / kWave;
     uint32_t* flat = &sh.wave_cnt[0][0];
     for (int i = threadIdx.x; i < kWavesPerBlock * kDigits; i += kBlock) flat[i] = 0;
@@ -148,21 +155,51 @@ PLH_DEV void msplit_round(MsplitShared& sh, const uint32_t (&dig)[kIpt], uint32_
         rank[s] = r;
     }
     __syncthreads();
-    for (int dd = threadIdx.x; dd < kDigits; dd += kBlock) {
-        uint32_t run = sh.running[dd];
+    // one digit per thread (kBlock == kDigits): wave offsets, the running
+    // count, and this round's count of the digit
+    const int dd = threadIdx.x;
+    uint32_t run = sh.running[dd];
+    sh.rbefore[dd] = run;
+    const uint32_t r_before = run;
 #pragma unroll
-        for (int ww = 0; ww < kWavesPerBlock; ++ww) {
-            const uint32_t c = sh.wave_cnt[ww][dd];
-            sh.wave_cnt[ww][dd] = run;
-            run += c;
-        }
-        sh.running[dd] = run;
+    for (int ww = 0; ww < kWavesPerBlock; ++ww) {
+        const uint32_t c = sh.wave_cnt[ww][dd];
+        sh.wave_cnt[ww][dd] = run;
+        run += c;
     }
+    sh.running[dd] = run;
+    uint32_t total;
+    const uint32_t off = block_exclusive_scan(run - r_before, total, sh.scan);  // has barriers
+    sh.roff[dd] = off;
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < kIpt; ++s)
         if (dig[s] != kInvalid) rank[s] += sh.wave_cnt[w][dig[s]];
     __syncthreads();
+    return total;
+}
+
+// The round's ids staged in LDS in (digit, rank) order, then written out by
+// consecutive threads: each digit's run of the round goes to consecutive
+// addresses from base[d] + rank, so a wave's stores fall into a few runs
+// instead of 64 scattered words.  Targets are those of the unstaged write.
+PLH_DEV void msplit_write(MsplitShared& sh, uint32_t total, const uint32_t (&dig)[kIpt],
+                          const uint32_t (&rank)[kIpt], const uint32_t (&idv)[kIpt], const uint64_t* base,
+                          uint32_t* __restrict__ ids_out) {
+#pragma unroll
+    for (int s = 0; s < kIpt; ++s) {
+        const uint32_t d = dig[s];
+        if (d == kInvalid) continue;
+        const uint32_t slot = sh.roff[d] + (rank[s] - sh.rbefore[d]);
+        sh.stage_id[slot] = idv[s];
+        sh.stage_dig[slot] = (uint8_t)d;
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < total; j += kBlock) {
+        const uint32_t d = sh.stage_dig[j];
+        ids_out[base[d] + sh.rbefore[d] + (j - sh.roff[d])] = sh.stage_id[j];
+    }
+    // the next round's first barrier orders these reads before its writes
 }
 
 // Position of item (round base r0, wave w, sub s, lane l) inside a round.
@@ -311,10 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, uint32_t* __re
                 }
             }
         }
-        msplit_round(sh, dig, rank);
-#pragma unroll
-        for (int s = 0; s < kIpt; ++s)
-            if (dig[s] != kInvalid) ids_out[base[dig[s]] + rank[s]] = idv[s];
+        const uint32_t total = msplit_round(sh, dig, rank);
+        msplit_write(sh, total, dig, rank, idv, base, ids_out);
     }
 }
 
@@ -514,10 +549,8 @@ __global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, uint32_t* __re
 #pragma unroll
         for (int s = 0; s < kIpt; ++s)
             dig[s] = r0 + round_pos(s) < end ? key_window<kDigitBits>(src, idv[s], pos) : kInvalid;
-        msplit_round(sh, dig, rank);
-#pragma unroll
-        for (int s = 0; s < kIpt; ++s)
-            if (dig[s] != kInvalid) ids_out[base[dig[s]] + rank[s]] = idv[s];
+        const uint32_t total = msplit_round(sh, dig, rank);
+        msplit_write(sh, total, dig, rank, idv, base, ids_out);
     }
 }
 
