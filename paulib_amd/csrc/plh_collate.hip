@@ -214,16 +214,17 @@ PLH_DEV int root_pos(const uint64_t* live, int kb) {
     return pos >= kb ? 0 : pos;
 }
 
-constexpr int kHistU = 8;  // items per thread per round of the histogram kernels
+constexpr int kHistU = 16;  // items per thread per round of the histogram kernels
 
 // First pass: the root bucket, ids enumerated implicitly (dense index i is
 // item i of the source), one digit position for the whole root.
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense, const uint64_t* __restrict__ live,
                                                       int kb, uint32_t tile_items, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kDigits];
+    __shared__ uint32_t hw[kWavesPerBlock][kDigits];  // one histogram per wave
     const int pos = root_pos(live, kb);
-    h[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < kWavesPerBlock * kDigits; i += kBlock) (&hw[0][0])[i] = 0;
+    uint32_t* h = hw[threadIdx.x / kWave];
     __syncthreads();
     const uint64_t t = blockIdx.x;
     const uint64_t beg = t * tile_items;
@@ -244,7 +245,10 @@ __global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense,
             if (dg[u] != kInvalid) atomicAdd(&h[dg[u]], 1u);
     }
     __syncthreads();
-    hist[t * kDigits + threadIdx.x] = h[threadIdx.x];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) sum += hw[w][threadIdx.x];
+    hist[t * kDigits + threadIdx.x] = sum;
 }
 
 // Per-digit prefix over a bucket's tiles (rewritten in place) and the
@@ -440,11 +444,12 @@ __global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const uint32_t* _
                                                       const uint32_t* __restrict__ ids1, PassBuckets B,
                                                       uint32_t nbig, const uint64_t* __restrict__ dT,
                                                       uint32_t tile_items, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kDigits];
+    __shared__ uint32_t hw[kWavesPerBlock][kDigits];  // one histogram per wave: fewer colliding atomics
     __shared__ unsigned long long sor, sand;
     const uint64_t t = blockIdx.x;
     if (t >= *dT) return;
-    h[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < kWavesPerBlock * kDigits; i += kBlock) (&hw[0][0])[i] = 0;
+    uint32_t* h = hw[threadIdx.x / kWave];
     if (threadIdx.x == 0) {
         sor = 0ull;
         sand = ~0ull;
@@ -489,7 +494,12 @@ __global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const uint32_t* _
         atomicAnd(&sand, (unsigned long long)a);
     }
     __syncthreads();
-    hist[t * kDigits + threadIdx.x] = h[threadIdx.x];
+    {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) sum += hw[w][threadIdx.x];
+        hist[t * kDigits + threadIdx.x] = sum;
+    }
     if (threadIdx.x == 0) {
         atomicOr((unsigned long long*)&B.kor[bi], sor);
         atomicAnd((unsigned long long*)&B.kand[bi], sand);
