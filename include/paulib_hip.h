@@ -167,9 +167,13 @@ int plh_apply_CNOT(plh_sum* s, int64_t control, int64_t target);
 int plh_evolve(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,
                const uint8_t* pflags, const double* theta);
 /* apply_truncation (include/PauliPropagation.h:35-47); max_weight < 0,
- * cutoff <= 0, max_terms < 0 mean "not set".  max_terms is not supported
- * yet (PLH_ERR_ARG when set). */
+ * cutoff <= 0, max_terms < 0 mean "not set". */
 int plh_apply_truncation(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms);
+/* PauliSum::trim(max_terms) (include/PauliSumUtils.h:330-343): keep the
+ * max_terms terms of largest |c|, in term order.  Among equal magnitudes the
+ * reference leaves the choice to std::nth_element; here the lower index
+ * is kept. */
+int plh_trim(plh_sum* s, int64_t max_terms);
 /* propagate (include/PauliPropagation.h:53-63): gates in reverse order,
  * each a one-term evolve then apply_truncation. */
 int plh_propagate(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,

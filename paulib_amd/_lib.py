@@ -81,6 +81,7 @@ SIGNATURES = {
     "plh_apply_CNOT": (_int, [_p, _i64, _i64]),
     "plh_evolve": (_int, [_p, _i64, _u64p, _u64p, _u8p, _f64p]),
     "plh_apply_truncation": (_int, [_p, _i64, C.c_double, _i64]),
+    "plh_trim": (_int, [_p, _i64]),
     "plh_propagate": (_int, [_p, _i64, _u64p, _u64p, _u8p, _f64p, _i64, C.c_double, _i64]),
     "plh_run_program": (_int, [_p, _i64, _i32p, _i64p, _i64p, _u64p, _u64p, _f64p, _i64, C.c_double, _i64]),
     "plh_server_create": (_int, [_p, _int, _int, _i64, _int, C.POINTER(_p)]),

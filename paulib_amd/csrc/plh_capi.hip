@@ -168,8 +168,8 @@ CollateSpec trunc_spec(int64_t max_weight, double cutoff) {
 // canonicalize equals a single fold-combine pass (IEEE addition of two
 // terms is commutative and sign-symmetric); the filters are per-term
 // predicates of the canonical terms, so they fuse into the same pass.
-int rotation_step(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf, double theta,
-                  bool truncate, int64_t max_weight, double cutoff) {
+int rotation_step_core(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf, double theta,
+                       bool truncate, int64_t max_weight, double cutoff) {
     const double c = std::cos(theta), sn = std::sin(theta);
     const bool was_canonical = s->canonical;
     if (rotation_kind(c, sn) == 4) {
@@ -200,6 +200,15 @@ int rotation_step(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf
         PLH_TRY(collate_terms(s->ctx, s, sc, s));
     }
     return collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s);
+}
+
+// apply_truncation's last step, trim (when max_terms >= 0), follows the
+// collation (include/PauliPropagation.h:35-47).
+int rotation_step(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf, double theta, bool truncate,
+                  int64_t max_weight, double cutoff, int64_t max_terms) {
+    PLH_TRY(rotation_step_core(s, px, pz, pf, theta, truncate, max_weight, cutoff));
+    if (truncate && max_terms >= 0) PLH_TRY(op_trim(s->ctx, s, max_terms));
+    return PLH_OK;
 }
 
 }  // namespace
@@ -749,32 +758,35 @@ int plh_evolve(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,
     if (nrot < 0 || (nrot > 0 && (!px || !pz || !theta))) return PLH_ERR_ARG;
     for (int64_t r = 0; r < nrot; ++r) {
         PLH_TRY(rotation_step(s, px + r * s->as, pz + r * s->as, pflags ? pflags[r] : 0, theta[r], false, -1,
-                              0.0));
+                              0.0, -1));
     }
     return plh_ctx_synchronize(s->ctx);
 }
 
 int plh_apply_truncation(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms) {
     PLH_TRY(check_sum(s));
-    if (max_terms >= 0) {
-        set_error("plh_apply_truncation: max_terms (trim) is not supported by this engine yet");
+    PLH_TRY(collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s));
+    if (max_terms >= 0) PLH_TRY(op_trim(s->ctx, s, max_terms));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_trim(plh_sum* s, int64_t max_terms) {
+    PLH_TRY(check_sum(s));
+    if (max_terms < 0) {
+        set_error("plh_trim: max_terms must be >= 0");
         return PLH_ERR_ARG;
     }
-    PLH_TRY(collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s));
+    PLH_TRY(op_trim(s->ctx, s, max_terms));
     return plh_ctx_synchronize(s->ctx);
 }
 
 int plh_propagate(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz, const uint8_t* pflags,
                   const double* theta, int64_t max_weight, double cutoff, int64_t max_terms) {
     PLH_TRY(check_sum(s));
-    if (max_terms >= 0) {
-        set_error("plh_propagate: max_terms (trim) is not supported by this engine yet");
-        return PLH_ERR_ARG;
-    }
     if (nrot < 0 || (nrot > 0 && (!px || !pz || !theta))) return PLH_ERR_ARG;
     for (int64_t l = nrot; l-- > 0;) {
         PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, pflags ? pflags[l] : 0, theta[l], true,
-                              max_weight, cutoff));
+                              max_weight, cutoff, max_terms));
     }
     return plh_ctx_synchronize(s->ctx);
 }
@@ -783,10 +795,6 @@ int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t
                     const uint64_t* px, const uint64_t* pz, const double* theta, int64_t max_weight,
                     double cutoff, int64_t max_terms) {
     PLH_TRY(check_sum(s));
-    if (max_terms >= 0) {
-        set_error("plh_run_program: max_terms (trim) is not supported by this engine yet");
-        return PLH_ERR_ARG;
-    }
     if (nops < 0 || (nops > 0 && (!kind || !q0 || !q1 || !px || !pz || !theta))) return PLH_ERR_ARG;
     for (int64_t l = nops; l-- > 0;) {
         switch (kind[l]) {
@@ -794,7 +802,8 @@ int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t
             case 1: PLH_TRY(op_clifford(s->ctx, s, 1, q0[l], 0)); break;
             case 2: PLH_TRY(op_clifford(s->ctx, s, 2, q0[l], q1[l])); break;
             case 3:
-                PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, 0, theta[l], true, max_weight, cutoff));
+                PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, 0, theta[l], true, max_weight, cutoff,
+                                      max_terms));
                 break;
             default: set_error("unknown op kind"); return PLH_ERR_ARG;
         }

@@ -268,6 +268,9 @@ int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz
 // change, 1 sign flip, 2 sign flip and * ps, 3 * ps, 4 general (new terms).
 int rotation_kind(double cos_t, double sin_t);
 int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, double eps);
+// PauliSum::trim(max_terms): keep the max_terms largest |c| in term order
+// (equal magnitudes: lower index first).
+int op_trim(plh_ctx* ctx, plh_sum* s, int64_t max_terms);
 int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out);
 int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src);
 

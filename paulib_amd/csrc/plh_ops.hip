@@ -1,6 +1,7 @@
 // plh_ops.hip -- element-wise and pair kernels of the PauliSum engine.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -574,6 +575,183 @@ int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, doubl
     sum_take_storage(s, &tmp);
     s->n = (int64_t)k;
     s->canonical = canon;  // a filter of a canonical sum stays canonical
+    return PLH_OK;
+}
+
+// ------------------------------------------------------------------ trim
+// PauliSum::trim(max_terms) (include/PauliSumUtils.h:330-343) keeps the
+// max_terms terms of largest |c| and compacts them in term order.  The
+// magnitude's IEEE bits order like the magnitude, so the k-th largest is
+// found by an 8-round radix select on those 63 bits (device-side, no
+// read-back); a term is kept when its key is above that threshold, or equal
+// to it and among the first k_rem such terms by index.
+namespace {
+
+PLH_DEV uint64_t mag_key(double v) { return (uint64_t)__double_as_longlong(v) & 0x7FFFFFFFFFFFFFFFull; }
+
+// sel: [0] key prefix found so far, [1] its mask, [2] terms still to take among keys with that prefix
+__global__ void k_trim_init(uint64_t* __restrict__ sel, uint32_t* __restrict__ ghist, uint64_t k) {
+    if (threadIdx.x < 3) sel[threadIdx.x] = threadIdx.x == 2 ? k : 0ull;
+    ghist[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_trim_hist(const double* __restrict__ v, int64_t n,
+                                                      const uint64_t* __restrict__ sel, int shift,
+                                                      uint32_t* __restrict__ ghist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t prefix = sel[0], mask = sel[1];
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        const uint64_t k = mag_key(v[t]);
+        if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&ghist[threadIdx.x], h[threadIdx.x]);
+}
+
+// The digit holding the k_rem-th largest key with the prefix (from the top).
+__global__ void k_trim_pick(uint64_t* __restrict__ sel, uint32_t* __restrict__ ghist, int shift) {
+    if (threadIdx.x == 0) {
+        const uint64_t k = sel[2];
+        uint64_t run = 0;
+        int d = 255;
+        for (; d > 0; --d) {
+            if (run + ghist[d] >= k) break;
+            run += ghist[d];
+        }
+        sel[2] = k - run;
+        sel[0] |= (uint64_t)d << shift;
+        sel[1] |= 255ull << shift;
+    }
+    __syncthreads();
+    ghist[threadIdx.x] = 0;
+}
+
+// Per block of kOpsTile terms: keys above the threshold, and keys equal to it.
+__global__ __launch_bounds__(kBlock) void k_trim_count(const double* __restrict__ v, int64_t n,
+                                                       const uint64_t* __restrict__ sel, uint32_t* __restrict__ cgt,
+                                                       uint32_t* __restrict__ ctie) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t T = sel[0];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t g = 0, e = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        const uint64_t key = mag_key(v[t]);
+        g += key > T;
+        e += key == T;
+    }
+    const uint32_t tg = block_sum(g, red);
+    const uint32_t te = block_sum(e, red);
+    if (threadIdx.x == 0) {
+        cgt[blockIdx.x] = tg;
+        ctie[blockIdx.x] = te;
+    }
+}
+
+// Kept terms per block: every key above the threshold, and the block's share
+// of the first sel[2] ties (toff = ties in earlier blocks).
+__global__ void k_trim_keepcnt(const uint32_t* __restrict__ cgt, const uint32_t* __restrict__ ctie,
+                               const uint64_t* __restrict__ toff, const uint64_t* __restrict__ sel, uint64_t nb,
+                               uint32_t* __restrict__ ckeep) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint64_t kr = sel[2];
+    const uint64_t take = toff[b] >= kr ? 0 : min((uint64_t)ctie[b], kr - toff[b]);
+    ckeep[b] = cgt[b] + (uint32_t)take;
+}
+
+__global__ __launch_bounds__(kBlock) void k_trim_write(int as, SumView s, int64_t n, const uint64_t* __restrict__ sel,
+                                                       const uint64_t* __restrict__ toff,
+                                                       const uint64_t* __restrict__ koff, uint64_t* __restrict__ ox,
+                                                       uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
+                                                       double* __restrict__ ov, int64_t os, uint64_t kmax,
+                                                       const uint32_t* __restrict__ sj, uint32_t* __restrict__ oj) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const uint64_t T = sel[0], kr = sel[2];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    uint32_t gt = 0, tie = 0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        const uint64_t key = mag_key(s.v[t]);
+        if (key > T) gt |= 1u << k;
+        if (key == T) tie |= 1u << k;
+    }
+    uint32_t tt;
+    uint64_t r = toff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(tie), tt, red);  // rank of my first tie
+    uint32_t keep = gt;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        if (!(tie >> k & 1u)) continue;
+        if (r < kr) keep |= 1u << k;
+        ++r;
+    }
+    uint32_t tk;
+    uint64_t o = koff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(keep), tk, red);
+    for (int k = 0; k < kOpsIpt; ++k) {
+        if (!(keep >> k & 1u)) continue;
+        if (o >= kmax) break;  // the select keeps exactly kmax terms; never write past them
+        const uint64_t t = base + k;
+        for (int w = 0; w < as; ++w) {
+            ox[(int64_t)w * os + o] = s.x[(int64_t)w * s.stride + t];
+            oz[(int64_t)w * os + o] = s.z[(int64_t)w * s.stride + t];
+        }
+        of[o] = s.f[t];
+        ov[o] = s.v[t];
+        if (oj) oj[o] = sj[t];
+        ++o;
+    }
+}
+
+}  // namespace
+
+int op_trim(plh_ctx* ctx, plh_sum* s, int64_t max_terms) {
+    if (max_terms < 0) return PLH_ERR_ARG;
+    if (s->n <= max_terms) return PLH_OK;
+    hipStream_t st = ctx->stream;
+    const uint64_t n = (uint64_t)s->n, k = (uint64_t)max_terms;
+    plh_sum tmp;
+    tmp.ctx = ctx;
+    tmp.as = s->as;
+    tmp.nq = s->nq;
+    tmp.has_job = s->has_job;
+    PLH_TRY(sum_reserve(&tmp, (int64_t)k, 0));
+    if (k > 0) {
+        const uint64_t nb = nblk(n, kOpsTile);
+        void* p;
+        PLH_TRY(ctx->ws.get(kWsScan0, nb * 12 + 256, &p));
+        uint32_t* cgt = (uint32_t*)p;
+        uint32_t* ctie = cgt + nb;
+        uint32_t* ckeep = ctie + nb;
+        PLH_TRY(ctx->ws.get(kWsScan1, nb * 16 + 256, &p));
+        uint64_t* toff = (uint64_t*)p;
+        uint64_t* koff = toff + nb;
+        PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+        uint64_t* sel = (uint64_t*)p + 64;            // 3 words
+        uint32_t* ghist = (uint32_t*)((uint64_t*)p + 128);  // 256 counters
+        uint64_t* d_total = (uint64_t*)p + 40;
+        const unsigned grid = (unsigned)std::min<uint64_t>(nblk(n, kBlock), 2048);
+        k_trim_init<<<1, 256, 0, st>>>(sel, ghist, k);
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            k_trim_hist<<<grid, kBlock, 0, st>>>(s->v, s->n, sel, shift, ghist);
+            k_trim_pick<<<1, 256, 0, st>>>(sel, ghist, shift);
+        }
+        k_trim_count<<<dim3((unsigned)nb), kBlock, 0, st>>>(s->v, s->n, sel, cgt, ctie);
+        PLH_HIP(hipGetLastError());
+        PLH_TRY(scan_u32_to_u64(ctx, ctie, toff, nb, d_total));
+        k_trim_keepcnt<<<(unsigned)nblk(nb, 256), 256, 0, st>>>(cgt, ctie, toff, sel, nb, ckeep);
+        PLH_TRY(scan_u32_to_u64(ctx, ckeep, koff, nb, d_total));
+        k_trim_write<<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, SumView{s->x, s->z, s->f, s->v, s->cap}, s->n,
+                                                            sel, toff, koff, tmp.x, tmp.z, tmp.f, tmp.v, tmp.cap, k,
+                                                            s->j, tmp.j);
+        PLH_HIP(hipGetLastError());
+    }
+    const bool canon = s->canonical;
+    sum_take_storage(s, &tmp);
+    s->n = (int64_t)k;
+    s->canonical = canon;  // a subset in term order of a canonical sum stays canonical
     return PLH_OK;
 }
 

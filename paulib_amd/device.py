@@ -223,6 +223,11 @@ class PauliSum:
         _lib.call("plh_cutoff", self._h, float(eps))
         return self
 
+    def trim(self, max_terms: int):
+        """PauliSum::trim: keep the max_terms largest |c| in term order (ties: lower index)."""
+        _lib.call("plh_trim", self._h, int(max_terms))
+        return self
+
     def apply_H(self, c: int):
         _lib.call("plh_apply_H", self._h, int(c))
         return self
