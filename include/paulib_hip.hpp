@@ -170,6 +170,8 @@ class DevicePauliSum {
     }
     void truncate_weight(std::size_t w) { check(plh_truncate_weight(h_, (int64_t)w), "plh_truncate_weight"); }
     void cutoff(T eps) { check(plh_cutoff(h_, eps), "plh_cutoff"); }
+    // PauliSum::trim(max_terms) (include/PauliSumUtils.h:330-343); ties keep the lower index.
+    void trim(std::size_t max_terms) { check(plh_trim(h_, (int64_t)max_terms), "plh_trim"); }
     void apply_H(std::size_t c) { check(plh_apply_H(h_, (int64_t)c), "plh_apply_H"); }
     void apply_S(std::size_t c) { check(plh_apply_S(h_, (int64_t)c), "plh_apply_S"); }
     void apply_CNOT(std::size_t c, std::size_t t) { check(plh_apply_CNOT(h_, (int64_t)c, (int64_t)t), "plh_apply_CNOT"); }

@@ -22,6 +22,7 @@ int orc_canonicalize(void*);
 int orc_apply_H(void*, int64_t);
 int orc_apply_CNOT(void*, int64_t, int64_t);
 int orc_evolve(void*, const void*);
+int orc_trim(void*, int64_t);
 }
 
 // The public fields of paulib::PauliSum<N,double> (include/PauliSumCore.h:20-45).
@@ -137,6 +138,10 @@ bool run_all() {
     orc_evolve(oc, orot);
     orc_canonicalize(oc);
     ok &= same(d.template download<HostPauliSum<N>>(), oc, "H/CNOT/evolve/canonicalize");
+    // trim: the largest |c| in term order
+    d.trim(77);
+    orc_trim(oc, 77);
+    ok &= same(d.template download<HostPauliSum<N>>(), oc, "trim");
     for (void* p : {oa, ob, raw, oc, orot}) orc_sum_free(p);
     return ok;
 }
