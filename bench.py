@@ -288,10 +288,13 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--batch-jobs", type=int, default=512)
     ap.add_argument("--batch-pairs", type=int, default=1 << 31)
-    ap.add_argument("--solo-pairs", type=int, default=1 << 22)
+    ap.add_argument("--solo-pairs", type=int, default=0)  # 0: the context's tuned threshold
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-propagation", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune-cache", default=os.environ.get("PLH_TUNE_CACHE",
+                                                          str(ROOT / "paulib_amd" / "_build" / "tune_cache.txt")))
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_product_65536x65536_q64.json"))
     args = ap.parse_args()
 
@@ -307,6 +310,11 @@ def main():
     log(f"inputs: {args.n} x {args.m} sums on 64 qubits")
     A, B = W.product_65536x65536_q64(seed=7, n=args.n, m=args.m)
     ctx = Context(0)
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = ctx.autotune(args.tune_cache)
+        log(f"autotune: {tuning} ({time.perf_counter() - t0:.1f} s, cache {args.tune_cache})")
     dA, dB = PauliSum.from_host(ctx, A), PauliSum.from_host(ctx, B)
     out = PauliSum(ctx, 1, 64)
     pairs = args.n * args.m
@@ -365,6 +373,7 @@ def main():
                      "launch_ms": per_launch_s * 1e3, "alg_bytes_per_launch": per_launch_bytes},
         "job_roofline": {"alg_bytes": b_alg_job, "frac": b_alg_job / dt / HBM_PEAK},
         "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()},
+        "tuning": tuning,
     }
     dA.close(), dB.close(), out.close()
     ctx.trim()

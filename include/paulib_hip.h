@@ -61,6 +61,17 @@ void* plh_ctx_stream(plh_ctx* ctx);
  * the scratch, and every heap chunk without a live sum, back to the device. */
 int64_t plh_ctx_scratch_bytes(plh_ctx* ctx);
 int plh_ctx_trim(plh_ctx* ctx);
+/* Per-device autotuning (no reference counterpart): the radix passes' tile
+ * caps (root pass and later passes) from timed sweeps of 32768 x 32768 and
+ * 65536 x 65536 products on 64 qubits, and the threshold from which
+ * plh_batch_multiply_canonical_each runs a product alone.  With cache_path
+ * set, a cached entry for this device (architecture, CU count, memory) is
+ * loaded instead of sweeping unless force != 0, and a new sweep's result is
+ * written there.  The sweep takes ~10 s, needs ~150 GB of device memory and
+ * trims the context. */
+int plh_ctx_autotune(plh_ctx* ctx, const char* cache_path, int force);
+/* The context's tuned values (0 = built-in default). */
+int plh_ctx_tuning(const plh_ctx* ctx, int64_t* tile_root_max, int64_t* tile_pass_max, int64_t* solo_pairs);
 
 /* Kernel timing: while enabled, the engine brackets its kernel launches
  * with HIP events on the context's stream.  profile_read synchronises,

@@ -49,6 +49,8 @@ SIGNATURES = {
     "plh_ctx_synchronize": (_int, [_p]),
     "plh_ctx_stream": (_p, [_p]),
     "plh_ctx_scratch_bytes": (_i64, [_p]),
+    "plh_ctx_autotune": (_int, [_p, C.c_char_p, _int]),
+    "plh_ctx_tuning": (_int, [_p, _i64p, _i64p, _i64p]),
     "plh_ctx_trim": (_int, [_p]),
     "plh_ctx_profile": (_int, [_p, _int]),
     "plh_ctx_profile_read": (_int, [_p, _int, C.c_char_p, _f64p, _i64p, _f64p, C.POINTER(_int)]),

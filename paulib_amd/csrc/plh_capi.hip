@@ -583,7 +583,6 @@ int plh_batch_multiply_canonical_each(int njobs, const plh_sum* const* lhs, cons
         set_error("plh_batch_multiply_canonical_each: bad job list or threshold");
         return PLH_ERR_ARG;
     }
-    if (solo_pairs == 0) solo_pairs = kSoloPairsDefault;
     plh_ctx* ctx = nullptr;
     for (int k = 0; k < njobs; ++k) {
         PLH_TRY(check_pair(lhs[k], rhs[k]));
@@ -613,6 +612,7 @@ int plh_batch_multiply_canonical_each(int njobs, const plh_sum* const* lhs, cons
             return PLH_ERR_ARG;
         }
     }
+    if (solo_pairs == 0) solo_pairs = ctx->tune.solo_pairs > 0 ? ctx->tune.solo_pairs : kSoloPairsDefault;
     const CollateSpec spec = canon_spec();
     // large products (and those over 128 qubits) alone: one collation each,
     // written straight into their output

@@ -69,12 +69,13 @@ struct LeafDims {
 };
 
 // Tile size for a pass over `items` ids: enough tiles to fill the chip,
-// whole rounds, capped.
+// whole rounds, capped (the cap is rounded down to whole rounds).
 inline uint32_t pick_tile(uint64_t items, uint32_t cap) {
     uint64_t t = items / 4096;
     t = (t + kRound - 1) / kRound * kRound;
+    const uint64_t c = cap < kRound ? kRound : (uint64_t)cap / kRound * kRound;
     if (t < kRound) t = kRound;
-    if (t > cap) t = cap;
+    if (t > c) t = c;
     return (uint32_t)t;
 }
 
@@ -1489,7 +1490,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
 
     // ---- pass 0: implicit ids, one root bucket (its digit position is the
     // first live bit, found by each kernel from the live mask).
-    const uint32_t tile0 = pick_tile(n_dense, kTileMaxImplicit);
+    const uint32_t tile0 = pick_tile(n_dense, ctx->tune.tile_root_max ? ctx->tune.tile_root_max : kTileMaxImplicit);
     const uint64_t t0 = (n_dense + tile0 - 1) / tile0;
     PLH_TRY(ctx->ws.get(kWsHist, t0 * kDigits * 4, &p));
     uint32_t* hist = (uint32_t*)p;
@@ -1557,7 +1558,8 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         n_valid = info[3];
         const uint64_t nbig = info[1], pass_items = info[2];
         if (nbig == 0) break;
-        const uint32_t tile = pick_tile(pass_items, kTileMaxExplicit);
+        const uint32_t tile =
+            pick_tile(pass_items, ctx->tune.tile_pass_max ? ctx->tune.tile_pass_max : kTileMaxExplicit);
         if (nbig <= kPlanMax) {
             k_setup_small<<<1, kPlanThreads, 0, st>>>(PB, nbig, tile, d_live, kb, d_T);
         } else {

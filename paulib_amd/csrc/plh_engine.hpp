@@ -146,8 +146,21 @@ struct ProfRec {
 };
 }  // namespace plh
 
+namespace plh {
+// Engine parameters picked per device by plh_ctx_autotune (0 = built-in
+// default): the tile caps of the root pass and of the later radix passes
+// (items per workgroup), and the batch threshold of
+// plh_batch_multiply_canonical_each.
+struct Tuning {
+    uint32_t tile_root_max = 0;
+    uint32_t tile_pass_max = 0;
+    int64_t solo_pairs = 0;
+};
+}  // namespace plh
+
 struct plh_ctx {
     int device = 0;
+    plh::Tuning tune;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     plh::DeviceHeap heap;  // declared before ws: ws returns its ranges on destruction

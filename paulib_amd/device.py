@@ -60,6 +60,18 @@ class Context:
     def scratch_bytes(self) -> int:
         return int(_lib.lib().plh_ctx_scratch_bytes(self._h))
 
+    def autotune(self, cache_path: str | None = None, force: bool = False) -> dict:
+        """Tune the engine for this device (tile caps, batch threshold); see
+        plh_ctx_autotune.  Returns the tuned values."""
+        path = None if cache_path is None else str(cache_path).encode()
+        _lib.call("plh_ctx_autotune", self._h, path, 1 if force else 0)
+        return self.tuning()
+
+    def tuning(self) -> dict:
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.call("plh_ctx_tuning", self._h, C.byref(a), C.byref(b), C.byref(c))
+        return {"tile_root_max": a.value, "tile_pass_max": b.value, "solo_pairs": c.value}
+
     def trim(self):
         _lib.call("plh_ctx_trim", self._h)
 
