@@ -167,6 +167,8 @@ def batched_products(ctx, count: int, budget_pairs: int, solo_pairs: int):
                 groups.append(cur)
         return groups
 
+    if solo_pairs <= 0:  # the engine's threshold: the context's tuned one, else its default
+        solo_pairs = ctx.tuning()["solo_pairs"] or (1 << 22)
     each_groups = grouped([None])
     one_groups = grouped([1, 2])
     terms = 0
