@@ -117,13 +117,23 @@ PLH_DEV double fold_value(const Cfg& c, double base, uint32_t raw) {
 // Stable rank of each item among same-digit items of the workgroup, in the
 // item order (round, wave, sub-round, lane); running[] carries per-digit
 // counts over the rounds of a tile.
+// An item id with the 32 key bits that follow the root digit (the carry
+// window [W0, W0 + 32), MSB-first).  The ids move through the passes with
+// their window, so a pass whose digit lies inside it reads the digit here
+// instead of gathering the key from the source.
+struct IdKey {
+    uint32_t id;
+    uint32_t bits;
+};
+constexpr int kCarryBits = 32;
+
 struct MsplitShared {
     uint32_t wave_cnt[kWavesPerBlock][kDigits];
     uint32_t running[kDigits];
     uint32_t rbefore[kDigits];       // running[] before this round
     uint32_t roff[kDigits];          // this round's items of digit d start at roff[d] in stage
     uint32_t scan[kWavesPerBlock];
-    uint32_t stage_id[kRound];       // this round's ids in (digit, rank) order
+    IdKey stage_id[kRound];          // this round's items in (digit, rank) order
     uint8_t stage_dig[kRound];
 };
 static_assert(kDigits <= 256, "stage_dig holds a digit in one byte");
@@ -185,8 +195,8 @@ PLH_DEV uint32_t msplit_round(MsplitShared& sh, const uint32_t (&dig)[kIpt], uin
 // addresses from base[d] + rank, so a wave's stores fall into a few runs
 // instead of 64 scattered words.  Targets are those of the unstaged write.
 PLH_DEV void msplit_write(MsplitShared& sh, uint32_t total, const uint32_t (&dig)[kIpt],
-                          const uint32_t (&rank)[kIpt], const uint32_t (&idv)[kIpt], const uint64_t* base,
-                          uint32_t* __restrict__ ids_out) {
+                          const uint32_t (&rank)[kIpt], const IdKey (&idv)[kIpt], const uint64_t* base,
+                          IdKey* __restrict__ ids_out) {
 #pragma unroll
     for (int s = 0; s < kIpt; ++s) {
         const uint32_t d = dig[s];
@@ -214,6 +224,16 @@ PLH_DEV int root_pos(const uint64_t* live, int kb) {
     const int pos = next_live(live, kb, 0);
     return pos >= kb ? 0 : pos;
 }
+
+// Start of the carry window: the first key bit after the root digit.
+PLH_DEV int carry_start(const uint64_t* live, int kb) { return root_pos(live, kb) + kDigitBits; }
+// A pass reads its digit from the carry window when the digit lies inside it.
+PLH_DEV bool in_carry(int pos, int w0) { return pos >= w0 && pos + kDigitBits <= w0 + kCarryBits; }
+PLH_DEV uint32_t carry_digit(uint32_t bits, int pos, int w0) {
+    return (uint32_t)(bits << (pos - w0)) >> (32 - kDigitBits);
+}
+// The window's bits from pos on, MSB-aligned; bits past the window read 0.
+PLH_DEV uint64_t carry_rest(uint32_t bits, int pos, int w0) { return ((uint64_t)bits << 32) << (pos - w0); }
 
 constexpr int kHistU = 16;  // items per thread per round of the histogram kernels
 
@@ -325,7 +345,7 @@ __global__ __launch_bounds__(kDigits * kRootGroups) void k_root_colscan(uint64_t
 }
 
 template <class Src>
-__global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, uint32_t* __restrict__ ids_out, uint64_t n_dense,
+__global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, IdKey* __restrict__ ids_out, uint64_t n_dense,
                                                          const uint64_t* __restrict__ live, int kb,
                                                          uint32_t tile_items, const uint32_t* __restrict__ hist,
                                                          const uint64_t* __restrict__ sub_start) {
@@ -339,17 +359,20 @@ __global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, uint32_t* __re
     sh.running[threadIdx.x] = 0;
     __syncthreads();
     for (uint64_t r0 = beg; r0 < end; r0 += kRound) {
-        uint32_t dig[kIpt], rank[kIpt], idv[kIpt];
+        uint32_t dig[kIpt], rank[kIpt];
+        IdKey idv[kIpt];
 #pragma unroll
         for (int s = 0; s < kIpt; ++s) {
             const uint64_t i = r0 + round_pos(s);
             dig[s] = kInvalid;
-            idv[s] = 0;
+            idv[s] = IdKey{0u, 0u};
             if (i < end) {
                 const uint32_t id = src.dense_to_id((uint32_t)i);
                 if (src.valid(id)) {
-                    idv[s] = id;
-                    dig[s] = key_window<kDigitBits>(src, id, pos);
+                    // the root digit and the carry window after it, in one read
+                    const uint64_t win = key_bits<kDigitBits + kCarryBits>(src, id, pos);
+                    idv[s] = IdKey{id, (uint32_t)win};
+                    dig[s] = (uint32_t)(win >> kCarryBits);
                 }
             }
         }
@@ -441,10 +464,11 @@ PLH_DEV uint32_t digit_and_rest(const Src& src, uint32_t id, int pos, uint64_t& 
 // digit is constant can jump straight to its first differing bit).  Grid
 // is an upper bound on the tile count; tiles at or beyond *dT exit at once.
 template <class Src>
-__global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const uint32_t* __restrict__ ids0,
-                                                      const uint32_t* __restrict__ ids1, PassBuckets B,
+__global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const IdKey* __restrict__ ids0,
+                                                      const IdKey* __restrict__ ids1, PassBuckets B,
                                                       uint32_t nbig, const uint64_t* __restrict__ dT,
-                                                      uint32_t tile_items, uint32_t* __restrict__ hist) {
+                                                      uint32_t tile_items, uint32_t* __restrict__ hist,
+                                                      const uint64_t* __restrict__ live, int kb) {
     __shared__ uint32_t hw[kWavesPerBlock][kDigits];  // one histogram per wave: fewer colliding atomics
     __shared__ unsigned long long sor, sand;
     const uint64_t t = blockIdx.x;
@@ -460,23 +484,32 @@ __global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const uint32_t* _
     const uint64_t beg = B.start[bi] + (t - B.tbase[bi]) * tile_items;
     const uint64_t end = min(B.start[bi] + B.size[bi], beg + tile_items);
     const int pos = (int)B.pos[bi];
-    const uint32_t* ids = meta_buf(B.meta[bi]) ? ids1 : ids0;
+    const int w0 = carry_start(live, kb);
+    const bool carried = in_carry(pos, w0);  // uniform over the tile
+    const IdKey* ids = meta_buf(B.meta[bi]) ? ids1 : ids0;
     uint64_t o = 0ull, a = ~0ull;
     // kHistU items per thread per round: their id loads, then their key
-    // gathers, are all in flight before the first LDS atomic
+    // gathers (when the digit is not carried), are all in flight before the
+    // first LDS atomic
     for (uint64_t i0 = beg + threadIdx.x; i0 < end; i0 += (uint64_t)kBlock * kHistU) {
-        uint32_t idv[kHistU], dg[kHistU];
+        IdKey idv[kHistU];
+        uint32_t dg[kHistU];
 #pragma unroll
         for (int u = 0; u < kHistU; ++u) {
             const uint64_t i = i0 + (uint64_t)u * kBlock;
-            idv[u] = i < end ? ids[i] : 0u;
+            idv[u] = i < end ? ids[i] : IdKey{0u, 0u};
         }
 #pragma unroll
         for (int u = 0; u < kHistU; ++u) {
             dg[u] = kInvalid;
             if (i0 + (uint64_t)u * kBlock < end) {
                 uint64_t rest;
-                dg[u] = digit_and_rest(src, idv[u], pos, rest);
+                if (carried) {
+                    dg[u] = carry_digit(idv[u].bits, pos, w0);
+                    rest = carry_rest(idv[u].bits, pos, w0);
+                } else {
+                    dg[u] = digit_and_rest(src, idv[u].id, pos, rest);
+                }
                 o |= rest;
                 a &= rest;
             }
@@ -512,16 +545,20 @@ __global__ __launch_bounds__(kBlock) void k_pass_colscan(PassBuckets B, uint32_t
                                                          const uint64_t* __restrict__ dT,
                                                          uint32_t* __restrict__ hist,
                                                          uint64_t* __restrict__ sub_start,
-                                                         uint64_t* __restrict__ sub_size) {
+                                                         uint64_t* __restrict__ sub_size,
+                                                         const uint64_t* __restrict__ live, int kb) {
     __shared__ uint64_t sh[kWavesPerBlock];
     const uint32_t bi = blockIdx.x;
     const uint64_t t0 = B.tbase[bi];
     const uint64_t t1 = (bi + 1 < nbig) ? B.tbase[bi + 1] : *dT;
     column_scan(t0, t1, B.start[bi], hist, sub_start + (uint64_t)bi * kDigits, sub_size + (uint64_t)bi * kDigits, sh);
     if (threadIdx.x == 0) {
-        // first differing bit among the examined ones, or the end of the word
+        // first differing bit among the examined ones, or the end of what was
+        // examined: the carry window for a carried digit, else the word
+        const int pos = (int)B.pos[bi], w0 = carry_start(live, kb);
         const uint64_t diff = B.kor[bi] ^ B.kand[bi];
-        const uint32_t off = diff ? (uint32_t)__clzll(diff) : 64u - (B.pos[bi] & 63u);
+        const uint32_t end = in_carry(pos, w0) ? (uint32_t)(w0 + kCarryBits - pos) : 64u - ((uint32_t)pos & 63u);
+        const uint32_t off = diff ? (uint32_t)__clzll(diff) : end;
         B.skip[bi] = off >= (uint32_t)kDigitBits ? off : 0u;
     }
 }
@@ -529,11 +566,12 @@ __global__ __launch_bounds__(kBlock) void k_pass_colscan(PassBuckets B, uint32_t
 // Stable scatter of a tile into its bucket's sub-buckets (other id buffer);
 // buckets whose digit is constant are left in place.
 template <class Src>
-__global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, uint32_t* __restrict__ ids0,
-                                                         uint32_t* __restrict__ ids1, PassBuckets B, uint32_t nbig,
+__global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, IdKey* __restrict__ ids0,
+                                                         IdKey* __restrict__ ids1, PassBuckets B, uint32_t nbig,
                                                          const uint64_t* __restrict__ dT, uint32_t tile_items,
                                                          const uint32_t* __restrict__ hist,
-                                                         const uint64_t* __restrict__ sub_start) {
+                                                         const uint64_t* __restrict__ sub_start,
+                                                         const uint64_t* __restrict__ live, int kb) {
     __shared__ MsplitShared sh;
     __shared__ uint64_t base[kDigits];
     const uint64_t t = blockIdx.x;
@@ -543,23 +581,29 @@ __global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, uint32_t* __re
     const uint64_t beg = B.start[bi] + (t - B.tbase[bi]) * tile_items;
     const uint64_t end = min(B.start[bi] + B.size[bi], beg + tile_items);
     const int pos = (int)B.pos[bi];
+    const int w0 = carry_start(live, kb);
+    const bool carried = in_carry(pos, w0);
     const bool from1 = meta_buf(B.meta[bi]) != 0;
-    const uint32_t* ids_in = from1 ? ids1 : ids0;
-    uint32_t* ids_out = from1 ? ids0 : ids1;
+    const IdKey* ids_in = from1 ? ids1 : ids0;
+    IdKey* ids_out = from1 ? ids0 : ids1;
     base[threadIdx.x] = sub_start[(uint64_t)bi * kDigits + threadIdx.x] + hist[t * kDigits + threadIdx.x];
     sh.running[threadIdx.x] = 0;
     __syncthreads();
     for (uint64_t r0 = beg; r0 < end; r0 += kRound) {
-        uint32_t dig[kIpt], rank[kIpt], idv[kIpt];
+        uint32_t dig[kIpt], rank[kIpt];
+        IdKey idv[kIpt];
         // every id load of the round, then every key gather, before the split
 #pragma unroll
         for (int s = 0; s < kIpt; ++s) {
             const uint64_t i = r0 + round_pos(s);
-            idv[s] = i < end ? ids_in[i] : 0u;
+            idv[s] = i < end ? ids_in[i] : IdKey{0u, 0u};
         }
 #pragma unroll
-        for (int s = 0; s < kIpt; ++s)
-            dig[s] = r0 + round_pos(s) < end ? key_window<kDigitBits>(src, idv[s], pos) : kInvalid;
+        for (int s = 0; s < kIpt; ++s) {
+            dig[s] = kInvalid;
+            if (r0 + round_pos(s) < end)
+                dig[s] = carried ? carry_digit(idv[s].bits, pos, w0) : key_window<kDigitBits>(src, idv[s].id, pos);
+        }
         const uint32_t total = msplit_round(sh, dig, rank);
         msplit_write(sh, total, dig, rank, idv, base, ids_out);
     }
@@ -1010,8 +1054,8 @@ template <int AS, class Src>
 __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uint64_t* __restrict__ Lstart,
                                                        const uint64_t* __restrict__ Lsize,
                                                        const uint32_t* __restrict__ Lmeta,
-                                                       const uint32_t* __restrict__ ids0,
-                                                       const uint32_t* __restrict__ ids1,
+                                                       const IdKey* __restrict__ ids0,
+                                                       const IdKey* __restrict__ ids1,
                                                        uint32_t* __restrict__ kept_out) {
     constexpr int KW = Src::kKW;   // key words before the flags word
     constexpr int XO = Src::kXOff;  // 1 when the job word leads the key
@@ -1023,7 +1067,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     const uint64_t start = Lstart[e];
     const uint64_t size = Lsize[e];
     const uint32_t meta = Lmeta[e];
-    const uint32_t* ids = (meta_buf(meta) ? ids1 : ids0) + start;
+    const IdKey* ids = (meta_buf(meta) ? ids1 : ids0) + start;
     const uint64_t out_base = start;
 
     if (size > (uint64_t)D::kCap) {
@@ -1033,7 +1077,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         for (uint64_t c0 = 0; c0 < size; c0 += D::kCap) {
             const uint32_t m = (uint32_t)min<uint64_t>(size - c0, (uint64_t)D::kCap);
             for (uint32_t i = tid; i < m; i += kLeafThreads) {
-                const uint32_t id = ids[c0 + i];
+                const uint32_t id = ids[c0 + i].id;
                 L.val[i] = fold_value(c, src.base_value(id), src.raw_flags(id));
             }
             __syncthreads();
@@ -1044,7 +1088,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         if (tid == 0) {
             uint64_t xz[KW];
             uint32_t fl;
-            src.key(ids[0], xz, fl);
+            src.key(ids[0].id, xz, fl);
             const bool keep = keep_term<AS, XO>(c, xz, acc);
             if (keep) write_term<AS, XO>(c, out_base, xz, fl & c.fmask, acc);
             kept_out[e] = keep ? 1u : 0u;
@@ -1059,7 +1103,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
-            idv[k] = p < n ? ids[p] : 0u;
+            idv[k] = p < n ? ids[p].id : 0u;
         }
         // every gather into registers first, then the LDS stores: the
         // stores cannot then hold back the next item's loads
@@ -1460,10 +1504,10 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         return PLH_ERR_SIZE;
     }
     void* p;
-    PLH_TRY(ctx->ws.get(kWsIds0, n_dense * 4, &p));
-    uint32_t* ids0 = (uint32_t*)p;
-    PLH_TRY(ctx->ws.get(kWsIds1, n_dense * 4, &p));
-    uint32_t* ids1 = (uint32_t*)p;
+    PLH_TRY(ctx->ws.get(kWsIds0, n_dense * sizeof(IdKey), &p));
+    IdKey* ids0 = (IdKey*)p;
+    PLH_TRY(ctx->ws.get(kWsIds1, n_dense * sizeof(IdKey), &p));
+    IdKey* ids1 = (IdKey*)p;
     PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
     uint64_t* misc = (uint64_t*)p;  // [0..33] live, [40] total, [56..61] pass counts, root position
     uint64_t* d_live = misc;
@@ -1506,7 +1550,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         k_root_colscan<<<1, kDigits * kRootGroups, 0, st>>>(t0, hist, sub_start, sub_size);
     }
     {
-        ProfScope ps(ctx, "msd_scatter_first", 4.0 * (double)n_dense);
+        ProfScope ps(ctx, "msd_scatter_first", 8.0 * (double)n_dense);
         k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, d_live, kb, tile0, hist,
                                                                     sub_start);
     }
@@ -1573,18 +1617,19 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         sub_start = (uint64_t*)p;
         sub_size = sub_start + nbig * kDigits;
         {
-            ProfScope ps(ctx, "msd_hist", 4.0 * (double)pass_items);
+            ProfScope ps(ctx, "msd_hist", 8.0 * (double)pass_items);
             k_pass_hist<Src><<<dim3((unsigned)Tb), kBlock, 0, st>>>(src, ids0, ids1, PB, (uint32_t)nbig, d_T, tile,
-                                                                     hist);
+                                                                     hist, d_live, kb);
         }
         {
             ProfScope ps(ctx, "colscan");
-            k_pass_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(PB, (uint32_t)nbig, d_T, hist, sub_start, sub_size);
+            k_pass_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(PB, (uint32_t)nbig, d_T, hist, sub_start, sub_size,
+                                                                    d_live, kb);
         }
         {
-            ProfScope ps(ctx, "msd_scatter", 8.0 * (double)pass_items);
+            ProfScope ps(ctx, "msd_scatter", 16.0 * (double)pass_items);
             k_pass_scatter<Src><<<dim3((unsigned)Tb), kBlock, 0, st>>>(src, ids0, ids1, PB, (uint32_t)nbig, d_T,
-                                                                        tile, hist, sub_start);
+                                                                        tile, hist, sub_start, d_live, kb);
         }
         PLH_HIP(hipGetLastError());
         // each split bucket has at most kDigits children; every entry holds >= 1 item

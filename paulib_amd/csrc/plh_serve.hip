@@ -261,8 +261,9 @@ double now_s(const plh_server* s) {
 }
 
 // Device bytes per term in flight: the batch (x, z, v, j, f), the collation
-// output, its id arrays and packed key records.
-double bytes_per_term(int as) { return 2.0 * (16.0 * as + 13.0) + 8.0 + (2.0 * as + 2.0) * 8.0; }
+// output, its two id arrays (an id and 32 carried key bits each) and packed
+// key records.
+double bytes_per_term(int as) { return 2.0 * (16.0 * as + 13.0) + 16.0 + (2.0 * as + 2.0) * 8.0; }
 
 bool keys_distinct(const Job& j) {
     const int a = j.job_as;

@@ -503,14 +503,21 @@ struct MultiPairSource {
     }
 };
 
-// Window of DB bits starting at global bit position pos (MSB-first).
+// Window of DB <= 64 bits starting at global bit position pos (MSB-first);
+// bits past the key's last word read 0.
 template <int DB, class Src>
-PLH_DEV uint32_t key_window(const Src& src, uint32_t id, int pos) {
+PLH_DEV uint64_t key_bits(const Src& src, uint32_t id, int pos) {
+    static_assert(DB > 0 && DB <= 64, "window wider than a word");
     const int w = pos >> 6;
     const int o = pos & 63;
     uint64_t hi = src.word(id, w) << o;
     if (o + DB > 64) hi |= src.word(id, w + 1) >> (64 - o);
-    return (uint32_t)(hi >> (64 - DB));
+    return hi >> (64 - DB);
+}
+template <int DB, class Src>
+PLH_DEV uint32_t key_window(const Src& src, uint32_t id, int pos) {
+    static_assert(DB <= 32, "use key_bits for windows wider than 32 bits");
+    return (uint32_t)key_bits<DB>(src, id, pos);
 }
 
 }  // namespace plh
