@@ -56,13 +56,17 @@ static_assert(kBlock == kDigits, "pass kernels map one thread to one digit");
 
 // Leaf geometry for keys of KW words (plus flags): ids per leaf sized so a
 // leaf's LDS (key words, value, sort key, two permutations, flags per item,
-// plus the bins) stays under ~78 KB and two leaves fit one CU's 160 KB.
+// plus the bins) fits its budget.  Single-word keys take ~52 KB, so three
+// leaves share a CU's 160 KB (the leaf is latency-bound: on the 65536 x 65536
+// product on 64 qubits three resident leaves of 1280 ids beat two of 1792 by
+// 17%, and four of 896 lose to three); wider keys take ~78 KB, two per CU.
 template <int AS, int KW>
 struct LeafDims {
-    static constexpr int kDl = AS == 1 ? 11 : AS == 2 ? 10 : AS <= 4 ? 9 : AS <= 8 ? 8 : 7;
+    static constexpr int kLdsBudget = (AS == 1 ? 52 : 78) * 1024;
+    static constexpr int kDl = AS <= 2 ? 10 : AS <= 4 ? 9 : AS <= 8 ? 8 : 7;
     static constexpr int kBins = 1 << kDl;  // counting-sort bins
     static constexpr int kPerItem = 8 * KW + 8 + 8 + 2 + 2 + 1;
-    static constexpr int kCap = ((78 * 1024 - 4 * (kBins + 1) - 1024) / kPerItem) / 256 * 256;
+    static constexpr int kCap = ((kLdsBudget - 4 * (kBins + 1) - 1024) / kPerItem) / 256 * 256;
     static constexpr int kPer = (kCap + kLeafThreads - 1) / kLeafThreads;
     static constexpr int kSlotBits = kCap <= 256 ? 8 : kCap <= 512 ? 9 : kCap <= 1024 ? 10 : 11;
     static_assert(kCap >= 256 && kCap <= 2048, "leaf capacity");
