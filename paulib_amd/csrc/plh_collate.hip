@@ -615,36 +615,11 @@ __global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, IdKey* __restr
 
 // The children of a split bucket, in digit order: sub-buckets larger than
 // `cap` stay single (depth pos + 8); runs of consecutive small ones are
-// merged while their total fits `cap` (a run of several sub-buckets keeps
-// the parent's depth, pos).  Count and write use the same walk.
-struct ChildWalk {
-    const uint64_t* st;
-    const uint64_t* ss;
-    uint32_t cap;
-    template <class F>
-    PLH_HD uint32_t walk(F&& emit) const {
-        uint32_t n = 0;
-        uint64_t g0 = 0, gn = 0;
-        uint32_t gc = 0;
-        for (int d = 0; d < kDigits; ++d) {
-            const uint64_t sz = ss[d];
-            if (!sz) continue;
-            if (sz > cap) {
-                if (gn) { emit(g0, gn, gc); ++n; gn = 0; gc = 0; }
-                emit(st[d], sz, 1u);
-                ++n;
-                continue;
-            }
-            if (gn + sz > cap) { emit(g0, gn, gc); ++n; gn = 0; gc = 0; }
-            if (!gn) g0 = st[d];
-            gn += sz;
-            ++gc;
-        }
-        if (gn) { emit(g0, gn, gc); ++n; }
-        return n;
-    }
-};
-
+// merged greedily while their total fits `cap` (a run of several
+// sub-buckets keeps the parent's depth, pos).  Empty sub-buckets are
+// skipped.  Read sequentially: a group starts at a non-empty sub-bucket h
+// and ends before the first non-empty sub-bucket that is large or would
+// take the total over `cap`; the next group starts there.
 // Children of each big bucket, from its 256 sub-buckets (one wave per
 // bucket; the walk is sequential, so it runs on wave-uniform values, not on
 // global loads): at most kDigits children at start/size/meta[b * kDigits],
@@ -657,46 +632,133 @@ struct Children {
     uint32_t* count;
 };
 
-PLH_DEV uint64_t readlane64(uint64_t v, int lane) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
-    return ((uint64_t)hi << 32) | lo;
+// LDS of one wave's grouping (group_children).
+struct GroupShared {
+    uint64_t P[kDigits + 1];   // exclusive prefix of the sizes
+    uint32_t NZ[kDigits + 1];  // exclusive count of non-empty sub-buckets
+    uint32_t J[kDigits + 1];   // successor under pointer doubling
+    uint32_t M[kDigits + 1];   // reached from the first group's head
+};
+
+// Index of the first set flag among bits k+1..3 of a lane's 4-bit mask,
+// else of the first flagged element in a later lane, else kDigits.
+PLH_DEV uint32_t next_flagged(uint32_t mine, int k, uint32_t later) {
+    const uint32_t rest = mine & (0xEu << k) & 0xFu;
+    return rest ? 4u * lane_id() + (uint32_t)__builtin_ctz(rest) : later;
+}
+PLH_DEV uint32_t first_flagged_after_lane(uint32_t mine) {
+    const uint32_t l = lane_id();
+    const uint64_t any = ballot(mine != 0);
+    const uint64_t above = l == kWave - 1 ? 0ull : any & (~0ull << (l + 1));
+    const int l2 = above ? __ffsll((long long)above) - 1 : (int)l;
+    const uint32_t first = mine ? (uint32_t)__builtin_ctz(mine) : 0u;
+    const uint32_t f2 = (uint32_t)__shfl((int)first, l2, kWave);
+    return above ? 4u * (uint32_t)l2 + f2 : (uint32_t)kDigits;
 }
 
-// ChildWalk over 256 sub-buckets held by one wave (lane l has digits 4l ..
-// 4l+3): the walk runs on wave-uniform values read with readlane, so each
-// step is a few scalar instructions.  emit(start, size, parts) is called in
-// digit order with wave-uniform arguments.
+// The grouping of 256 sub-buckets held by one wave (lane l has digits 4l ..
+// 4l+3), computed in parallel: each sub-bucket d gets the end nxt[d] of a
+// group that would start there (a binary search on the size prefix, capped
+// by the next large sub-bucket; a large one ends at the next non-empty
+// one), and the groups actually formed are the heads reached from the
+// first non-empty sub-bucket by following nxt (8 rounds of pointer
+// doubling cover chains of up to 256).  emit(o, start, size, parts) is
+// called by the lane owning group o's head (o counts groups in digit
+// order).  Returns the number of groups (uniform).  One wave per
+// workgroup; contains barriers.
 template <class F>
-PLH_DEV void wave_walk(const uint64_t (&st)[4], const uint64_t (&ss)[4], uint32_t cap, F&& emit) {
-    uint64_t g0 = 0, gn = 0;
-    uint32_t gc = 0;
-    for (int l = 0; l < kWave; ++l) {
+PLH_DEV uint32_t group_children(const uint64_t (&st)[4], const uint64_t (&ss)[4], uint32_t cap, GroupShared& g,
+                                F&& emit) {
+    const uint32_t l = lane_id();
+    uint32_t bigm = 0, nzm = 0;
+    uint64_t lp[4];
+    uint32_t lnz[4];
+    uint64_t run = 0;
+    uint32_t nrun = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        lp[k] = run;
+        lnz[k] = nrun;
+        run += ss[k];
+        nrun += ss[k] != 0;
+        bigm |= (ss[k] > cap ? 1u : 0u) << k;
+        nzm |= (ss[k] != 0 ? 1u : 0u) << k;
+    }
+    const uint64_t pex = wave_inclusive_scan(run) - run;
+    const uint32_t nex = wave_inclusive_scan(nrun) - nrun;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        g.P[4 * l + k] = pex + lp[k];
+        g.NZ[4 * l + k] = nex + lnz[k];
+    }
+    if (l == kWave - 1) {
+        g.P[kDigits] = pex + run;
+        g.NZ[kDigits] = nex + nrun;
+        g.J[kDigits] = kDigits;
+        g.M[kDigits] = 0;
+    }
+    const uint32_t big_later = first_flagged_after_lane(bigm);
+    const uint32_t nz_later = first_flagged_after_lane(nzm);
+    const uint64_t nzl = ballot(nzm != 0);
+    if (!nzl) return 0;  // every sub-bucket empty (uniform)
+    const int lc = __ffsll((long long)nzl) - 1;
+    const uint32_t c0 = 4u * (uint32_t)lc + (uint32_t)__builtin_ctz((uint32_t)__shfl((int)nzm, lc, kWave));
+    __syncthreads();
+    uint32_t nxt[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t d = 4 * l + k;
+        if ((bigm >> k) & 1u) {
+            nxt[k] = next_flagged(nzm, k, nz_later);
+        } else {
+            // first m in [d+1, 256] with P[m] > P[d] + cap (257: none)
+            const uint64_t t = g.P[d] + cap;
+            uint32_t lo = d + 1, n = kDigits + 1 - lo;
+            while (n > 0) {
+                const uint32_t half = n / 2, m = lo + half;
+                if (g.P[m] <= t) {
+                    lo = m + 1;
+                    n -= half + 1;
+                } else {
+                    n = half;
+                }
+            }
+            nxt[k] = min(next_flagged(bigm, k, big_later), lo - 1);
+        }
+        g.J[d] = nxt[k];
+        g.M[d] = d == c0 ? 1u : 0u;
+    }
+    __syncthreads();
+    for (int r = 0; r < 8; ++r) {
+        uint32_t j[4], jj[4], m[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint64_t sz = readlane64(ss[k], l);
-            if (!sz) continue;
-            const uint64_t s0 = readlane64(st[k], l);
-            if (sz > cap) {
-                if (gn) {
-                    emit(g0, gn, gc);
-                    gn = 0;
-                    gc = 0;
-                }
-                emit(s0, sz, 1u);
-                continue;
-            }
-            if (gn + sz > cap) {
-                emit(g0, gn, gc);
-                gn = 0;
-                gc = 0;
-            }
-            if (!gn) g0 = s0;
-            gn += sz;
-            ++gc;
+            j[k] = g.J[4 * l + k];
+            jj[k] = g.J[j[k]];
+            m[k] = g.M[4 * l + k];
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (m[k]) g.M[j[k]] = 1u;
+            g.J[4 * l + k] = jj[k];
+        }
+        __syncthreads();
     }
-    if (gn) emit(g0, gn, gc);
+    uint32_t head = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) head |= (g.M[4 * l + k] ? 1u : 0u) << k;
+    const uint32_t hc = (uint32_t)__builtin_popcount(head);
+    const uint32_t hinc = wave_inclusive_scan(hc);
+    uint32_t o = hinc - hc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!((head >> k) & 1u)) continue;
+        const uint32_t d = 4 * l + k;
+        emit(o, st[k], g.P[nxt[k]] - g.P[d], g.NZ[nxt[k]] - g.NZ[d]);
+        ++o;
+    }
+    return (uint32_t)__shfl((int)hinc, kWave - 1, kWave);
 }
 
 // One wave per big bucket.
@@ -727,16 +789,13 @@ __global__ __launch_bounds__(kWave) void k_walk(PassBuckets B, const uint64_t* _
     const uint32_t buf = meta_buf(B.meta[b]) ^ 1u;
     const int child_depth = min((int)pos + kDigitBits, kb);
     const uint32_t child_uniform = next_live(live, kb, child_depth) >= kb ? 1u : 0u;
-    uint64_t o = base;
-    wave_walk(st, ss, cap, [&](uint64_t s0, uint64_t n, uint32_t parts) {
-        if (lane == 0) {
-            C.start[o] = s0;
-            C.size[o] = n;
-            C.meta[o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
-        }
-        ++o;
+    __shared__ GroupShared g;
+    const uint32_t n = group_children(st, ss, cap, g, [&](uint32_t o, uint64_t s0, uint64_t sz, uint32_t parts) {
+        C.start[base + o] = s0;
+        C.size[base + o] = sz;
+        C.meta[base + o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
     });
-    if (lane == 0) C.count[b] = (uint32_t)(o - base);
+    if (lane == 0) C.count[b] = n;
 }
 
 PLH_DEV uint32_t expand_count_one(uint64_t e, const uint32_t* isbig, const uint64_t* bigpos, const Children& C) {
@@ -1453,19 +1512,15 @@ __global__ __launch_bounds__(kWave) void k_root_list(const uint64_t* __restrict_
     const int pos = root_pos(live, kb);
     const int depth = min(pos + kDigitBits, kb);
     const uint32_t uni = next_live(live, kb, depth) >= kb ? 1u : 0u;
-    uint64_t o = 0, nv = 0;
-    wave_walk(st, ss, cap, [&](uint64_t s0, uint64_t n, uint32_t parts) {
-        if (lane == 0) {
-            Nstart[o] = s0;
-            Nsize[o] = n;
-            Nmeta[o] = parts == 1 ? make_meta((uint32_t)depth, 0u, uni) : make_meta((uint32_t)pos, 0u, 0u);
-        }
-        ++o;
-        nv += n;
+    __shared__ GroupShared g;
+    const uint32_t n = group_children(st, ss, cap, g, [&](uint32_t o, uint64_t s0, uint64_t sz, uint32_t parts) {
+        Nstart[o] = s0;
+        Nsize[o] = sz;
+        Nmeta[o] = parts == 1 ? make_meta((uint32_t)depth, 0u, uni) : make_meta((uint32_t)pos, 0u, 0u);
     });
     if (lane == 0) {
-        *d_L = o;
-        *d_nvalid = nv;
+        *d_L = n;
+        *d_nvalid = n ? g.P[kDigits] : 0ull;
     }
 }
 
