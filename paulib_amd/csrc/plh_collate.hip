@@ -74,7 +74,7 @@ struct LeafDims {
 
 // Tile size for a pass over `items` ids: enough tiles to fill the chip,
 // whole rounds, capped (the cap is rounded down to whole rounds).
-inline uint32_t pick_tile(uint64_t items, uint32_t cap) {
+PLH_HD uint32_t pick_tile(uint64_t items, uint32_t cap) {
     uint64_t t = items / 4096;
     t = (t + kRound - 1) / kRound * kRound;
     const uint64_t c = cap < kRound ? kRound : (uint64_t)cap / kRound * kRound;
@@ -280,7 +280,8 @@ __global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense,
 // sub-bucket starts and sizes; tiles are read 16 at a time so the loads of
 // a batch overlap.  One workgroup of kDigits threads per bucket.
 PLH_DEV void column_scan(uint64_t t0, uint64_t t1, uint64_t bstart, uint32_t* __restrict__ hist,
-                         uint64_t* __restrict__ sub_start, uint64_t* __restrict__ sub_size, uint64_t* sh) {
+                         uint64_t* __restrict__ sub_start, uint64_t* __restrict__ sub_size, uint64_t* sh,
+                         uint64_t* lst, uint64_t* lss) {
     const int d = threadIdx.x;
     uint64_t run = 0;
     constexpr int kB = 16;
@@ -298,6 +299,8 @@ PLH_DEV void column_scan(uint64_t t0, uint64_t t1, uint64_t bstart, uint32_t* __
     const uint64_t ex = block_exclusive_scan(run, tot, sh);
     sub_start[d] = bstart + ex;
     sub_size[d] = run;
+    lst[d] = bstart + ex;
+    lss[d] = run;
 }
 
 // The root's column scan: 4 groups of kDigits threads take a quarter of the
@@ -545,26 +548,221 @@ __global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const IdKey* __re
 }
 
 // One workgroup per big bucket: column scan, and whether the digit splits.
+// The children of a split bucket, in digit order: sub-buckets larger than
+// `cap` stay single (depth pos + 8); runs of consecutive small ones are
+// merged greedily while their total fits `cap` (a run of several
+// sub-buckets keeps the parent's depth, pos).  Empty sub-buckets are
+// skipped.  Read sequentially: a group starts at a non-empty sub-bucket h
+// and ends before the first non-empty sub-bucket that is large or would
+// take the total over `cap`; the next group starts there.
+struct Children {
+    uint64_t* start;
+    uint64_t* size;
+    uint32_t* meta;
+    uint32_t* count;
+};
+
+// Orders one wave's LDS accesses around it: a wave issues its LDS
+// instructions in order, so lanes see each other's earlier writes once the
+// compiler keeps the order (no workgroup barrier needed).
+PLH_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS of one wave's grouping (group_children).
+struct GroupShared {
+    uint64_t P[kDigits + 1];   // exclusive prefix of the sizes
+    uint32_t NZ[kDigits + 1];  // exclusive count of non-empty sub-buckets
+    uint32_t J[kDigits + 1];   // successor under pointer doubling
+    uint32_t M[kDigits + 1];   // reached from the first group's head
+};
+
+// Index of the first set flag among bits k+1..3 of a lane's 4-bit mask,
+// else of the first flagged element in a later lane, else kDigits.
+PLH_DEV uint32_t next_flagged(uint32_t mine, int k, uint32_t later) {
+    const uint32_t rest = mine & (0xEu << k) & 0xFu;
+    return rest ? 4u * lane_id() + (uint32_t)__builtin_ctz(rest) : later;
+}
+PLH_DEV uint32_t first_flagged_after_lane(uint32_t mine) {
+    const uint32_t l = lane_id();
+    const uint64_t any = ballot(mine != 0);
+    const uint64_t above = l == kWave - 1 ? 0ull : any & (~0ull << (l + 1));
+    const int l2 = above ? __ffsll((long long)above) - 1 : (int)l;
+    const uint32_t first = mine ? (uint32_t)__builtin_ctz(mine) : 0u;
+    const uint32_t f2 = (uint32_t)__shfl((int)first, l2, kWave);
+    return above ? 4u * (uint32_t)l2 + f2 : (uint32_t)kDigits;
+}
+
+// The grouping of 256 sub-buckets held by one wave (lane l has digits 4l ..
+// 4l+3), computed in parallel: each sub-bucket d gets the end nxt[d] of a
+// group that would start there (a binary search on the size prefix, capped
+// by the next large sub-bucket; a large one ends at the next non-empty
+// one), and the groups actually formed are the heads reached from the
+// first non-empty sub-bucket by following nxt (8 rounds of pointer
+// doubling cover chains of up to 256).  emit(o, start, size, parts) is
+// called by the lane owning group o's head (o counts groups in digit
+// order).  Returns the number of groups (uniform).  Runs on one wave (the
+// other waves of its workgroup need not take part) with wave-local syncs.
+template <class F>
+PLH_DEV uint32_t group_children(const uint64_t (&st)[4], const uint64_t (&ss)[4], uint32_t cap, GroupShared& g,
+                                F&& emit) {
+    const uint32_t l = lane_id();
+    uint32_t bigm = 0, nzm = 0;
+    uint64_t lp[4];
+    uint32_t lnz[4];
+    uint64_t run = 0;
+    uint32_t nrun = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        lp[k] = run;
+        lnz[k] = nrun;
+        run += ss[k];
+        nrun += ss[k] != 0;
+        bigm |= (ss[k] > cap ? 1u : 0u) << k;
+        nzm |= (ss[k] != 0 ? 1u : 0u) << k;
+    }
+    const uint64_t pex = wave_inclusive_scan(run) - run;
+    const uint32_t nex = wave_inclusive_scan(nrun) - nrun;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        g.P[4 * l + k] = pex + lp[k];
+        g.NZ[4 * l + k] = nex + lnz[k];
+    }
+    if (l == kWave - 1) {
+        g.P[kDigits] = pex + run;
+        g.NZ[kDigits] = nex + nrun;
+        g.J[kDigits] = kDigits;
+        g.M[kDigits] = 0;
+    }
+    const uint32_t big_later = first_flagged_after_lane(bigm);
+    const uint32_t nz_later = first_flagged_after_lane(nzm);
+    const uint64_t nzl = ballot(nzm != 0);
+    if (!nzl) return 0;  // every sub-bucket empty (uniform)
+    const int lc = __ffsll((long long)nzl) - 1;
+    const uint32_t c0 = 4u * (uint32_t)lc + (uint32_t)__builtin_ctz((uint32_t)__shfl((int)nzm, lc, kWave));
+    wave_sync();
+    uint32_t nxt[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t d = 4 * l + k;
+        if ((bigm >> k) & 1u) {
+            nxt[k] = next_flagged(nzm, k, nz_later);
+        } else {
+            // first m in [d+1, 256] with P[m] > P[d] + cap (257: none)
+            const uint64_t t = g.P[d] + cap;
+            uint32_t lo = d + 1, n = kDigits + 1 - lo;
+            while (n > 0) {
+                const uint32_t half = n / 2, m = lo + half;
+                if (g.P[m] <= t) {
+                    lo = m + 1;
+                    n -= half + 1;
+                } else {
+                    n = half;
+                }
+            }
+            nxt[k] = min(next_flagged(bigm, k, big_later), lo - 1);
+        }
+        g.J[d] = nxt[k];
+        g.M[d] = d == c0 ? 1u : 0u;
+    }
+    wave_sync();
+    for (int r = 0; r < 8; ++r) {
+        uint32_t j[4], jj[4], m[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            j[k] = g.J[4 * l + k];
+            jj[k] = g.J[j[k]];
+            m[k] = g.M[4 * l + k];
+        }
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (m[k]) g.M[j[k]] = 1u;
+            g.J[4 * l + k] = jj[k];
+        }
+        wave_sync();
+    }
+    uint32_t head = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) head |= (g.M[4 * l + k] ? 1u : 0u) << k;
+    const uint32_t hc = (uint32_t)__builtin_popcount(head);
+    const uint32_t hinc = wave_inclusive_scan(hc);
+    uint32_t o = hinc - hc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!((head >> k) & 1u)) continue;
+        const uint32_t d = 4 * l + k;
+        emit(o, st[k], g.P[nxt[k]] - g.P[d], g.NZ[nxt[k]] - g.NZ[d]);
+        ++o;
+    }
+    return (uint32_t)__shfl((int)hinc, kWave - 1, kWave);
+}
+
+// Per big bucket (one workgroup each): the column scan of its tiles'
+// histograms, whether its digit is constant (skip), and its children for
+// the next list (group_children on wave 0).  A bucket whose digit was
+// constant has one child: itself, deeper, in the same id buffer.
 __global__ __launch_bounds__(kBlock) void k_pass_colscan(PassBuckets B, uint32_t nbig,
                                                          const uint64_t* __restrict__ dT,
                                                          uint32_t* __restrict__ hist,
                                                          uint64_t* __restrict__ sub_start,
                                                          uint64_t* __restrict__ sub_size,
-                                                         const uint64_t* __restrict__ live, int kb) {
+                                                         const uint64_t* __restrict__ live, int kb, uint32_t cap,
+                                                         Children C) {
     __shared__ uint64_t sh[kWavesPerBlock];
+    __shared__ uint64_t gst[kDigits], gss[kDigits];
+    __shared__ uint32_t skip_sh;
+    __shared__ GroupShared g;
     const uint32_t bi = blockIdx.x;
     const uint64_t t0 = B.tbase[bi];
     const uint64_t t1 = (bi + 1 < nbig) ? B.tbase[bi + 1] : *dT;
-    column_scan(t0, t1, B.start[bi], hist, sub_start + (uint64_t)bi * kDigits, sub_size + (uint64_t)bi * kDigits, sh);
+    const uint64_t base = (uint64_t)bi * kDigits;
+    column_scan(t0, t1, B.start[bi], hist, sub_start + base, sub_size + base, sh, gst, gss);
+    const uint32_t pos = B.pos[bi];
     if (threadIdx.x == 0) {
         // first differing bit among the examined ones, or the end of what was
         // examined: the carry window for a carried digit, else the word
-        const int pos = (int)B.pos[bi], w0 = carry_start(live, kb);
+        const int w0 = carry_start(live, kb);
         const uint64_t diff = B.kor[bi] ^ B.kand[bi];
-        const uint32_t end = in_carry(pos, w0) ? (uint32_t)(w0 + kCarryBits - pos) : 64u - ((uint32_t)pos & 63u);
+        const uint32_t end =
+            in_carry((int)pos, w0) ? (uint32_t)(w0 + kCarryBits - (int)pos) : 64u - (pos & 63u);
         const uint32_t off = diff ? (uint32_t)__clzll(diff) : end;
-        B.skip[bi] = off >= (uint32_t)kDigitBits ? off : 0u;
+        const uint32_t skip = off >= (uint32_t)kDigitBits ? off : 0u;
+        B.skip[bi] = skip;
+        skip_sh = skip;
     }
+    __syncthreads();
+    if (threadIdx.x >= kWave) return;
+    const int lane = (int)threadIdx.x;
+    const uint32_t skip = skip_sh;
+    if (skip) {
+        if (lane == 0) {
+            const int depth = min((int)(pos + skip), kb);
+            C.start[base] = B.start[bi];
+            C.size[base] = B.size[bi];
+            C.meta[base] = make_meta((uint32_t)depth, meta_buf(B.meta[bi]), next_live(live, kb, depth) >= kb ? 1u : 0u);
+            C.count[bi] = 1;
+        }
+        return;
+    }
+    uint64_t st[4], ss[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        st[k] = gst[4 * lane + k];
+        ss[k] = gss[4 * lane + k];
+    }
+    const uint32_t buf = meta_buf(B.meta[bi]) ^ 1u;
+    const int child_depth = min((int)pos + kDigitBits, kb);
+    const uint32_t child_uniform = next_live(live, kb, child_depth) >= kb ? 1u : 0u;
+    const uint32_t n = group_children(st, ss, cap, g, [&](uint32_t o, uint64_t s0, uint64_t sz, uint32_t parts) {
+        C.start[base + o] = s0;
+        C.size[base + o] = sz;
+        C.meta[base + o] =
+            parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
+    });
+    if (lane == 0) C.count[bi] = n;
 }
 
 // Stable scatter of a tile into its bucket's sub-buckets (other id buffer);
@@ -613,191 +811,6 @@ __global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, IdKey* __restr
     }
 }
 
-// The children of a split bucket, in digit order: sub-buckets larger than
-// `cap` stay single (depth pos + 8); runs of consecutive small ones are
-// merged greedily while their total fits `cap` (a run of several
-// sub-buckets keeps the parent's depth, pos).  Empty sub-buckets are
-// skipped.  Read sequentially: a group starts at a non-empty sub-bucket h
-// and ends before the first non-empty sub-bucket that is large or would
-// take the total over `cap`; the next group starts there.
-// Children of each big bucket, from its 256 sub-buckets (one wave per
-// bucket; the walk is sequential, so it runs on wave-uniform values, not on
-// global loads): at most kDigits children at start/size/meta[b * kDigits],
-// their count in count[b].  A bucket whose digit was constant has one
-// child: itself, deeper, in the same id buffer.
-struct Children {
-    uint64_t* start;
-    uint64_t* size;
-    uint32_t* meta;
-    uint32_t* count;
-};
-
-// LDS of one wave's grouping (group_children).
-struct GroupShared {
-    uint64_t P[kDigits + 1];   // exclusive prefix of the sizes
-    uint32_t NZ[kDigits + 1];  // exclusive count of non-empty sub-buckets
-    uint32_t J[kDigits + 1];   // successor under pointer doubling
-    uint32_t M[kDigits + 1];   // reached from the first group's head
-};
-
-// Index of the first set flag among bits k+1..3 of a lane's 4-bit mask,
-// else of the first flagged element in a later lane, else kDigits.
-PLH_DEV uint32_t next_flagged(uint32_t mine, int k, uint32_t later) {
-    const uint32_t rest = mine & (0xEu << k) & 0xFu;
-    return rest ? 4u * lane_id() + (uint32_t)__builtin_ctz(rest) : later;
-}
-PLH_DEV uint32_t first_flagged_after_lane(uint32_t mine) {
-    const uint32_t l = lane_id();
-    const uint64_t any = ballot(mine != 0);
-    const uint64_t above = l == kWave - 1 ? 0ull : any & (~0ull << (l + 1));
-    const int l2 = above ? __ffsll((long long)above) - 1 : (int)l;
-    const uint32_t first = mine ? (uint32_t)__builtin_ctz(mine) : 0u;
-    const uint32_t f2 = (uint32_t)__shfl((int)first, l2, kWave);
-    return above ? 4u * (uint32_t)l2 + f2 : (uint32_t)kDigits;
-}
-
-// The grouping of 256 sub-buckets held by one wave (lane l has digits 4l ..
-// 4l+3), computed in parallel: each sub-bucket d gets the end nxt[d] of a
-// group that would start there (a binary search on the size prefix, capped
-// by the next large sub-bucket; a large one ends at the next non-empty
-// one), and the groups actually formed are the heads reached from the
-// first non-empty sub-bucket by following nxt (8 rounds of pointer
-// doubling cover chains of up to 256).  emit(o, start, size, parts) is
-// called by the lane owning group o's head (o counts groups in digit
-// order).  Returns the number of groups (uniform).  One wave per
-// workgroup; contains barriers.
-template <class F>
-PLH_DEV uint32_t group_children(const uint64_t (&st)[4], const uint64_t (&ss)[4], uint32_t cap, GroupShared& g,
-                                F&& emit) {
-    const uint32_t l = lane_id();
-    uint32_t bigm = 0, nzm = 0;
-    uint64_t lp[4];
-    uint32_t lnz[4];
-    uint64_t run = 0;
-    uint32_t nrun = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        lp[k] = run;
-        lnz[k] = nrun;
-        run += ss[k];
-        nrun += ss[k] != 0;
-        bigm |= (ss[k] > cap ? 1u : 0u) << k;
-        nzm |= (ss[k] != 0 ? 1u : 0u) << k;
-    }
-    const uint64_t pex = wave_inclusive_scan(run) - run;
-    const uint32_t nex = wave_inclusive_scan(nrun) - nrun;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        g.P[4 * l + k] = pex + lp[k];
-        g.NZ[4 * l + k] = nex + lnz[k];
-    }
-    if (l == kWave - 1) {
-        g.P[kDigits] = pex + run;
-        g.NZ[kDigits] = nex + nrun;
-        g.J[kDigits] = kDigits;
-        g.M[kDigits] = 0;
-    }
-    const uint32_t big_later = first_flagged_after_lane(bigm);
-    const uint32_t nz_later = first_flagged_after_lane(nzm);
-    const uint64_t nzl = ballot(nzm != 0);
-    if (!nzl) return 0;  // every sub-bucket empty (uniform)
-    const int lc = __ffsll((long long)nzl) - 1;
-    const uint32_t c0 = 4u * (uint32_t)lc + (uint32_t)__builtin_ctz((uint32_t)__shfl((int)nzm, lc, kWave));
-    __syncthreads();
-    uint32_t nxt[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t d = 4 * l + k;
-        if ((bigm >> k) & 1u) {
-            nxt[k] = next_flagged(nzm, k, nz_later);
-        } else {
-            // first m in [d+1, 256] with P[m] > P[d] + cap (257: none)
-            const uint64_t t = g.P[d] + cap;
-            uint32_t lo = d + 1, n = kDigits + 1 - lo;
-            while (n > 0) {
-                const uint32_t half = n / 2, m = lo + half;
-                if (g.P[m] <= t) {
-                    lo = m + 1;
-                    n -= half + 1;
-                } else {
-                    n = half;
-                }
-            }
-            nxt[k] = min(next_flagged(bigm, k, big_later), lo - 1);
-        }
-        g.J[d] = nxt[k];
-        g.M[d] = d == c0 ? 1u : 0u;
-    }
-    __syncthreads();
-    for (int r = 0; r < 8; ++r) {
-        uint32_t j[4], jj[4], m[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            j[k] = g.J[4 * l + k];
-            jj[k] = g.J[j[k]];
-            m[k] = g.M[4 * l + k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (m[k]) g.M[j[k]] = 1u;
-            g.J[4 * l + k] = jj[k];
-        }
-        __syncthreads();
-    }
-    uint32_t head = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) head |= (g.M[4 * l + k] ? 1u : 0u) << k;
-    const uint32_t hc = (uint32_t)__builtin_popcount(head);
-    const uint32_t hinc = wave_inclusive_scan(hc);
-    uint32_t o = hinc - hc;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (!((head >> k) & 1u)) continue;
-        const uint32_t d = 4 * l + k;
-        emit(o, st[k], g.P[nxt[k]] - g.P[d], g.NZ[nxt[k]] - g.NZ[d]);
-        ++o;
-    }
-    return (uint32_t)__shfl((int)hinc, kWave - 1, kWave);
-}
-
-// One wave per big bucket.
-__global__ __launch_bounds__(kWave) void k_walk(PassBuckets B, const uint64_t* __restrict__ sub_start,
-                                                const uint64_t* __restrict__ sub_size, uint32_t cap,
-                                                const uint64_t* __restrict__ live, int kb, Children C) {
-    const uint64_t b = blockIdx.x;
-    const uint32_t pos = B.pos[b];
-    const uint32_t skip = B.skip[b];
-    const uint64_t base = b * kDigits;
-    const int lane = (int)threadIdx.x;
-    if (skip) {
-        if (lane == 0) {
-            const int depth = min((int)(pos + skip), kb);
-            C.start[base] = B.start[b];
-            C.size[base] = B.size[b];
-            C.meta[base] = make_meta((uint32_t)depth, meta_buf(B.meta[b]), next_live(live, kb, depth) >= kb ? 1u : 0u);
-            C.count[b] = 1;
-        }
-        return;
-    }
-    uint64_t st[4], ss[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        st[k] = sub_start[base + 4 * lane + k];
-        ss[k] = sub_size[base + 4 * lane + k];
-    }
-    const uint32_t buf = meta_buf(B.meta[b]) ^ 1u;
-    const int child_depth = min((int)pos + kDigitBits, kb);
-    const uint32_t child_uniform = next_live(live, kb, child_depth) >= kb ? 1u : 0u;
-    __shared__ GroupShared g;
-    const uint32_t n = group_children(st, ss, cap, g, [&](uint32_t o, uint64_t s0, uint64_t sz, uint32_t parts) {
-        C.start[base + o] = s0;
-        C.size[base + o] = sz;
-        C.meta[base + o] = parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
-    });
-    if (lane == 0) C.count[b] = n;
-}
-
 PLH_DEV uint32_t expand_count_one(uint64_t e, const uint32_t* isbig, const uint64_t* bigpos, const Children& C) {
     return isbig[e] ? C.count[bigpos[e]] : 1u;
 }
@@ -843,29 +856,38 @@ constexpr int kPlanWaves = kPlanThreads / kWave;
 constexpr int kPlanPer = 8;
 constexpr uint64_t kPlanMax = (uint64_t)kPlanThreads * kPlanPer;  // list entries
 
-// classify + scan + gather of the big buckets; entry e is thread e / 8's.
+// classify + scan + gather of the big buckets, then their setup (digit
+// position, tile count, tile base) for tiles of pick_tile(items, tile_cap)
+// ids, the size the host derives from the same item count; entry e is
+// thread e / 8's.
 __global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __restrict__ Lstart,
                                                              const uint64_t* __restrict__ Lsize,
                                                              const uint32_t* __restrict__ Lmeta,
                                                              const uint64_t* __restrict__ dL, uint32_t cap,
                                                              uint32_t* __restrict__ isbig, uint64_t* __restrict__ bigpos,
                                                              PassBuckets B, uint64_t* __restrict__ d_nbig,
-                                                             uint64_t* __restrict__ d_items) {
+                                                             uint64_t* __restrict__ d_items, uint32_t tile_cap,
+                                                             const uint64_t* __restrict__ live, int kb,
+                                                             uint64_t* __restrict__ d_T) {
     __shared__ uint64_t sh[kPlanWaves];
     const uint64_t L = *dL;
     const uint64_t e0 = (uint64_t)threadIdx.x * kPlanPer;
     uint32_t big[kPlanPer];
+    uint64_t sz[kPlanPer];
     uint64_t cnt = 0, items = 0;
 #pragma unroll
     for (int k = 0; k < kPlanPer; ++k) {
         const uint64_t e = e0 + k;
-        big[k] = (e < L && Lsize[e] > cap && !meta_uniform(Lmeta[e])) ? 1u : 0u;
+        sz[k] = e < L ? Lsize[e] : 0ull;
+        big[k] = (e < L && sz[k] > cap && !meta_uniform(Lmeta[e])) ? 1u : 0u;
         cnt += big[k];
-        items += big[k] ? Lsize[e] : 0ull;
+        items += big[k] ? sz[k] : 0ull;
     }
     uint64_t nbig, tot_items;
-    uint64_t off = block_exclusive_scan<kPlanWaves>(cnt, nbig, sh);
+    const uint64_t off0 = block_exclusive_scan<kPlanWaves>(cnt, nbig, sh);
     (void)block_exclusive_scan<kPlanWaves>(items, tot_items, sh);
+    const uint32_t tile = pick_tile(tot_items, tile_cap);
+    uint64_t off = off0, nt_sum = 0;
 #pragma unroll
     for (int k = 0; k < kPlanPer; ++k) {
         const uint64_t e = e0 + k;
@@ -873,15 +895,29 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __r
         isbig[e] = big[k];
         bigpos[e] = off;
         if (big[k]) {
+            const uint32_t meta = Lmeta[e];
+            const uint32_t nt = (uint32_t)((sz[k] + tile - 1) / tile);
             B.start[off] = Lstart[e];
-            B.size[off] = Lsize[e];
-            B.meta[off] = Lmeta[e];
+            B.size[off] = sz[k];
+            B.meta[off] = meta;
+            B.pos[off] = (uint32_t)next_live(live, kb, (int)meta_depth(meta));
+            B.ntiles[off] = nt;
+            B.kor[off] = 0ull;
+            B.kand[off] = ~0ull;
+            nt_sum += nt;
             ++off;
         }
+    }
+    uint64_t T;
+    uint64_t tb = block_exclusive_scan<kPlanWaves>(nt_sum, T, sh);
+    for (uint64_t b = off0; b < off; ++b) {
+        B.tbase[b] = tb;
+        tb += B.ntiles[b];  // written by this thread above
     }
     if (threadIdx.x == 0) {
         *d_nbig = nbig;
         *d_items = tot_items;
+        *d_T = T;
     }
 }
 
@@ -1495,7 +1531,7 @@ __global__ __launch_bounds__(kBlock) void k_live_or(SumView s, const uint32_t* _
 }
 
 // The root's children as the first bucket list (buffer 0), merged like
-// k_walk (one wave); also the list length and the number of valid items.
+// a split bucket's (one wave); also the list length and the number of valid items.
 __global__ __launch_bounds__(kWave) void k_root_list(const uint64_t* __restrict__ sub_start,
                                                      const uint64_t* __restrict__ sub_size, uint32_t cap,
                                                      const uint64_t* __restrict__ live, int kb,
@@ -1646,9 +1682,11 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         PB.pos = PB.meta + Lb;
         PB.ntiles = PB.pos + Lb;
         PB.skip = PB.ntiles + Lb;
-        if (Lb <= kPlanMax) {
+        const uint32_t tile_cap = ctx->tune.tile_pass_max ? ctx->tune.tile_pass_max : kTileMaxExplicit;
+        const bool plan_small = Lb <= kPlanMax;  // the plan kernel also sets the buckets up
+        if (plan_small) {
             k_plan_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, d_L, cap, isbig, bigpos, PB, d_nbig,
-                                                    d_items);
+                                                    d_items, tile_cap, d_live, kb, d_T);
         } else {
             k_classify<<<grid1(Lb), kBlock, 0, st>>>(A.size, A.meta, d_L, Lb, cap, isbig, bigsz);
             PLH_TRY(scan_u32_to_u64(ctx, isbig, bigpos, Lb, d_nbig));
@@ -1661,9 +1699,10 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         n_valid = info[3];
         const uint64_t nbig = info[1], pass_items = info[2];
         if (nbig == 0) break;
-        const uint32_t tile =
-            pick_tile(pass_items, ctx->tune.tile_pass_max ? ctx->tune.tile_pass_max : kTileMaxExplicit);
-        if (nbig <= kPlanMax) {
+        const uint32_t tile = pick_tile(pass_items, tile_cap);
+        if (plan_small) {
+            // set up by k_plan_small
+        } else if (nbig <= kPlanMax) {
             k_setup_small<<<1, kPlanThreads, 0, st>>>(PB, nbig, tile, d_live, kb, d_T);
         } else {
             k_bucket_setup<<<grid1(nbig), kBlock, 0, st>>>(PB, nbig, tile, d_live, kb);
@@ -1680,10 +1719,16 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
             k_pass_hist<Src><<<dim3((unsigned)Tb), kBlock, 0, st>>>(src, ids0, ids1, PB, (uint32_t)nbig, d_T, tile,
                                                                      hist, d_live, kb);
         }
+        PLH_TRY(ctx->ws.get(kWsChild, nbig * kDigits * 20 + nbig * 4 + 256, &p));
+        Children CH;
+        CH.start = (uint64_t*)p;
+        CH.size = CH.start + nbig * kDigits;
+        CH.meta = (uint32_t*)(CH.size + nbig * kDigits);
+        CH.count = CH.meta + nbig * kDigits;
         {
             ProfScope ps(ctx, "colscan");
             k_pass_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(PB, (uint32_t)nbig, d_T, hist, sub_start, sub_size,
-                                                                    d_live, kb);
+                                                                    d_live, kb, cap, CH);
         }
         {
             ProfScope ps(ctx, "msd_scatter", 16.0 * (double)pass_items);
@@ -1695,13 +1740,6 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         const uint64_t Lb2 = std::min<uint64_t>(L + (kDigits - 1) * nbig, n_valid);
         const int nslot = cur_slot == kWsListA ? kWsListB : kWsListA;
         PLH_TRY(get_lists(ctx, nslot, Lb2, &B));
-        PLH_TRY(ctx->ws.get(kWsChild, nbig * kDigits * 20 + nbig * 4 + 256, &p));
-        Children CH;
-        CH.start = (uint64_t*)p;
-        CH.size = CH.start + nbig * kDigits;
-        CH.meta = (uint32_t*)(CH.size + nbig * kDigits);
-        CH.count = CH.meta + nbig * kDigits;
-        k_walk<<<dim3((unsigned)nbig), kWave, 0, st>>>(PB, sub_start, sub_size, cap, d_live, kb, CH);
         if (L <= kPlanMax) {
             k_expand_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, CH, L, B.start, B.size,
                                                       B.meta, d_L);
