@@ -109,8 +109,13 @@ class Workspace {
 };
 
 // Small pinned host staging buffer for count read-backs.
+// Host-pinned, GPU-coherent mailbox for small read-backs: word 0 is a
+// sequence number, words 1..64 the values (written by a kernel on the
+// context's stream, polled by the host).
 struct HostStage {
-    uint64_t* p = nullptr;
+    uint64_t* p = nullptr;   // host address
+    uint64_t* dp = nullptr;  // device address of the same memory
+    uint64_t seq = 0;
     int init();
     ~HostStage();
 };

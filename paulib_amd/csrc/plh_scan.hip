@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 
@@ -130,6 +131,16 @@ int scan_impl(plh_ctx* ctx, const TIn* d_in, uint64_t* d_out, uint64_t n, uint64
     return PLH_OK;
 }
 
+// Copies n <= 64 device words into the pinned mailbox (words 1..n), then,
+// after a system-scope fence, the sequence number into word 0 (one wave).
+__global__ __launch_bounds__(kWave) void k_post(const uint64_t* __restrict__ src, int n, uint64_t* mbox,
+                                                uint64_t seq) {
+    const int i = (int)threadIdx.x;
+    if (i < n) __hip_atomic_store(&mbox[1 + i], src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __threadfence_system();
+    if (i == 0) __hip_atomic_store(&mbox[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 }  // namespace
 
 int scan_u32_to_u64(plh_ctx* ctx, const uint32_t* d_in, uint64_t* d_out, uint64_t n,
@@ -141,20 +152,32 @@ int scan_u64(plh_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n, ui
     return scan_impl<uint64_t>(ctx, d_in, d_out, n, d_total);
 }
 
-int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out) {
-    if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
-    PLH_HIP(hipMemcpyAsync(ctx->stage.p, d_src, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-    PLH_HIP(hipStreamSynchronize(ctx->stream));
-    *out = ctx->stage.p[0];
-    return PLH_OK;
-}
+int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out) { return read_u64n(ctx, d_src, 1, out); }
 
+// A read-back posted by a one-wave kernel into the pinned mailbox and polled
+// by the host: the host sees the values as soon as they land, instead of
+// waking from a stream synchronisation after a copy.  After 2 ms of polling
+// it synchronises the stream once (a long-running predecessor, or an error,
+// which the synchronisation reports); the values must be there after it.
 int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out) {
     if (n < 1 || n > 64) return PLH_ERR_ARG;
     if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
-    PLH_HIP(hipMemcpyAsync(ctx->stage.p, d_src, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-    PLH_HIP(hipStreamSynchronize(ctx->stream));
-    for (int i = 0; i < n; ++i) out[i] = ctx->stage.p[i];
+    const uint64_t seq = ++ctx->stage.seq;
+    k_post<<<1, kWave, 0, ctx->stream>>>(d_src, n, ctx->stage.dp, seq);
+    PLH_HIP(hipGetLastError());
+    const auto t0 = std::chrono::steady_clock::now();
+    bool synced = false;
+    while (__atomic_load_n(&ctx->stage.p[0], __ATOMIC_ACQUIRE) != seq) {
+        if (synced) {
+            set_error("read-back: the posting kernel completed without writing the mailbox");
+            return PLH_ERR_HIP;
+        }
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+            PLH_HIP(hipStreamSynchronize(ctx->stream));
+            synced = true;
+        }
+    }
+    for (int i = 0; i < n; ++i) out[i] = __atomic_load_n(&ctx->stage.p[1 + i], __ATOMIC_RELAXED);
     return PLH_OK;
 }
 
@@ -183,7 +206,10 @@ ProfScope::~ProfScope() {
 }
 
 int HostStage::init() {
-    PLH_HIP(hipHostMalloc((void**)&p, 64 * sizeof(uint64_t), hipHostMallocDefault));
+    PLH_HIP(hipHostMalloc((void**)&p, 72 * sizeof(uint64_t), hipHostMallocCoherent | hipHostMallocMapped));
+    for (int i = 0; i < 72; ++i) p[i] = 0;
+    seq = 0;
+    PLH_HIP(hipHostGetDevicePointer((void**)&dp, p, 0));
     return PLH_OK;
 }
 HostStage::~HostStage() {
