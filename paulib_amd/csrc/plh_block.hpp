@@ -50,4 +50,13 @@ PLH_DEV T block_sum(T v, T* sh) {
     return total;
 }
 
+// One thread posts n values to a host-pinned coherent mailbox: words
+// 1..n, then, after a system-scope fence, the sequence number in word 0
+// (the host polls word 0).
+PLH_DEV void post_write(uint64_t* mbox, uint64_t seq, int n, const uint64_t* v) {
+    for (int i = 0; i < n; ++i) __hip_atomic_store(&mbox[1 + i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __threadfence_system();
+    __hip_atomic_store(&mbox[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 }  // namespace plh

@@ -72,10 +72,12 @@ struct LeafDims {
     static_assert(kCap >= 256 && kCap <= 2048, "leaf capacity");
 };
 
-// Tile size for a pass over `items` ids: enough tiles to fill the chip,
-// whole rounds, capped (the cap is rounded down to whole rounds).
+// Tile size for a pass over `items` ids: about kTileTarget tiles (four per
+// CU: the chip fills while the column scans stay short), whole rounds,
+// capped (the cap is rounded down to whole rounds).
+constexpr uint64_t kTileTarget = 1024;
 PLH_HD uint32_t pick_tile(uint64_t items, uint32_t cap) {
-    uint64_t t = items / 4096;
+    uint64_t t = items / kTileTarget;
     t = (t + kRound - 1) / kRound * kRound;
     const uint64_t c = cap < kRound ? kRound : (uint64_t)cap / kRound * kRound;
     if (t < kRound) t = kRound;
@@ -276,78 +278,63 @@ __global__ __launch_bounds__(kBlock) void k_root_hist(Src src, uint64_t n_dense,
     hist[t * kDigits + threadIdx.x] = sum;
 }
 
-// Per-digit prefix over a bucket's tiles (rewritten in place) and the
-// sub-bucket starts and sizes; tiles are read 16 at a time so the loads of
-// a batch overlap.  One workgroup of kDigits threads per bucket.
-PLH_DEV void column_scan(uint64_t t0, uint64_t t1, uint64_t bstart, uint32_t* __restrict__ hist,
-                         uint64_t* __restrict__ sub_start, uint64_t* __restrict__ sub_size, uint64_t* sh,
-                         uint64_t* lst, uint64_t* lss) {
-    const int d = threadIdx.x;
-    uint64_t run = 0;
-    constexpr int kB = 16;
-    for (uint64_t t = t0; t < t1; t += kB) {
-        uint32_t c[kB];
-#pragma unroll
-        for (int k = 0; k < kB; ++k) c[k] = (t + k < t1) ? hist[(t + k) * kDigits + d] : 0u;
-#pragma unroll
-        for (int k = 0; k < kB; ++k) {
-            if (t + k < t1) hist[(t + k) * kDigits + d] = (uint32_t)run;
-            run += c[k];
-        }
-    }
-    uint64_t tot;
-    const uint64_t ex = block_exclusive_scan(run, tot, sh);
-    sub_start[d] = bstart + ex;
-    sub_size[d] = run;
-    lst[d] = bstart + ex;
-    lss[d] = run;
-}
+// Column scans of per-tile digit histograms (in place: each count becomes
+// the exclusive prefix of its digit over the earlier tiles).  kColGroups
+// groups of kDigits threads take consecutive slices of the tiles: a sum
+// pass per slice, then a prefix-writing pass from the slice's offset, with
+// kColChunk loads in flight per thread.  Returns the digit total in group
+// 0's thread d (0 in the others).
+constexpr int kColGroups = 4;
+constexpr int kColThreads = kDigits * kColGroups;
+constexpr int kColChunk = 32;
 
-// The root's column scan: 4 groups of kDigits threads take a quarter of the
-// tiles each (a sum pass, then a prefix-writing pass from the group's offset).
-constexpr int kRootGroups = 4;
-
-__global__ __launch_bounds__(kDigits * kRootGroups) void k_root_colscan(uint64_t ntiles, uint32_t* __restrict__ hist,
-                                                                       uint64_t* __restrict__ sub_start,
-                                                                       uint64_t* __restrict__ sub_size) {
-    __shared__ uint64_t part[kRootGroups][kDigits];
-    __shared__ uint64_t sh[kDigits * kRootGroups / kWave];
-    const int d = threadIdx.x % kDigits, g = threadIdx.x / kDigits;
-    const uint64_t per = (ntiles + kRootGroups - 1) / kRootGroups;
-    const uint64_t t0 = min(ntiles, (uint64_t)g * per), t1 = min(ntiles, t0 + per);
-    constexpr int kB = 16;
+template <int G>
+PLH_DEV uint64_t column_scan(uint64_t t0, uint64_t t1, uint32_t* __restrict__ hist, uint64_t (&part)[G][kDigits]) {
+    const int d = (int)threadIdx.x % kDigits, g = (int)threadIdx.x / kDigits;
+    const uint64_t n = t1 - t0, per = (n + G - 1) / G;
+    const uint64_t a = t0 + min(n, (uint64_t)g * per), b = t0 + min(n, (uint64_t)(g + 1) * per);
     uint64_t sum = 0;
-    for (uint64_t t = t0; t < t1; t += kB) {
-        uint32_t c[kB];
+    for (uint64_t t = a; t < b; t += kColChunk) {
+        uint32_t c[kColChunk];
 #pragma unroll
-        for (int k = 0; k < kB; ++k) c[k] = (t + k < t1) ? hist[(t + k) * kDigits + d] : 0u;
+        for (int k = 0; k < kColChunk; ++k) c[k] = (t + k < b) ? hist[(t + k) * kDigits + d] : 0u;
 #pragma unroll
-        for (int k = 0; k < kB; ++k) sum += c[k];
+        for (int k = 0; k < kColChunk; ++k) sum += c[k];
     }
     part[g][d] = sum;
     __syncthreads();
     uint64_t run = 0;
     for (int q = 0; q < g; ++q) run += part[q][d];
-    for (uint64_t t = t0; t < t1; t += kB) {
-        uint32_t c[kB];
+    for (uint64_t t = a; t < b; t += kColChunk) {
+        uint32_t c[kColChunk];
 #pragma unroll
-        for (int k = 0; k < kB; ++k) c[k] = (t + k < t1) ? hist[(t + k) * kDigits + d] : 0u;
+        for (int k = 0; k < kColChunk; ++k) c[k] = (t + k < b) ? hist[(t + k) * kDigits + d] : 0u;
 #pragma unroll
-        for (int k = 0; k < kB; ++k) {
-            if (t + k < t1) hist[(t + k) * kDigits + d] = (uint32_t)run;
+        for (int k = 0; k < kColChunk; ++k) {
+            if (t + k < b) hist[(t + k) * kDigits + d] = (uint32_t)run;
             run += c[k];
         }
     }
-    // digit totals in the first group's threads, a scan over the whole
-    // workgroup (the other threads add 0 after them)
     uint64_t tot_d = 0;
     if (g == 0)
-        for (int q = 0; q < kRootGroups; ++q) tot_d += part[q][d];
+        for (int q = 0; q < G; ++q) tot_d += part[q][d];
+    return tot_d;
+}
+
+// The root's column scan and its digit offsets (one workgroup).
+__global__ __launch_bounds__(kColThreads) void k_root_colscan(uint64_t ntiles, uint32_t* __restrict__ hist,
+                                                              uint64_t* __restrict__ sub_start,
+                                                              uint64_t* __restrict__ sub_size) {
+    __shared__ uint64_t part[kColGroups][kDigits];
+    __shared__ uint64_t sh[kColThreads / kWave];
+    const uint64_t tot_d = column_scan<kColGroups>(0, ntiles, hist, part);
+    // digit totals in the first group's threads, a scan over the whole
+    // workgroup (the other threads add 0 after them)
     uint64_t tot;
-    const uint64_t ex = block_exclusive_scan<kDigits * kRootGroups / kWave>(tot_d, tot, sh);
-    if (g == 0) {
-        sub_start[d] = ex;
-        sub_size[d] = tot_d;
+    const uint64_t ex = block_exclusive_scan<kColThreads / kWave>(tot_d, tot, sh);
+    if (threadIdx.x < kDigits) {
+        sub_start[threadIdx.x] = ex;
+        sub_size[threadIdx.x] = tot_d;
     }
 }
 
@@ -704,14 +691,18 @@ PLH_DEV uint32_t group_children(const uint64_t (&st)[4], const uint64_t (&ss)[4]
 // histograms, whether its digit is constant (skip), and its children for
 // the next list (group_children on wave 0).  A bucket whose digit was
 // constant has one child: itself, deeper, in the same id buffer.
-__global__ __launch_bounds__(kBlock) void k_pass_colscan(PassBuckets B, uint32_t nbig,
-                                                         const uint64_t* __restrict__ dT,
-                                                         uint32_t* __restrict__ hist,
-                                                         uint64_t* __restrict__ sub_start,
-                                                         uint64_t* __restrict__ sub_size,
-                                                         const uint64_t* __restrict__ live, int kb, uint32_t cap,
-                                                         Children C) {
-    __shared__ uint64_t sh[kWavesPerBlock];
+// G groups of kDigits threads per bucket: 4 when buckets hold many tiles
+// (the scan's serial part shrinks), 1 when they hold few.
+template <int G>
+__global__ __launch_bounds__(kDigits * G) void k_pass_colscan(PassBuckets B, uint32_t nbig,
+                                                             const uint64_t* __restrict__ dT,
+                                                             uint32_t* __restrict__ hist,
+                                                             uint64_t* __restrict__ sub_start,
+                                                             uint64_t* __restrict__ sub_size,
+                                                             const uint64_t* __restrict__ live, int kb,
+                                                             uint32_t cap, Children C) {
+    __shared__ uint64_t part[G][kDigits];
+    __shared__ uint64_t sh[kDigits * G / kWave];
     __shared__ uint64_t gst[kDigits], gss[kDigits];
     __shared__ uint32_t skip_sh;
     __shared__ GroupShared g;
@@ -719,7 +710,16 @@ __global__ __launch_bounds__(kBlock) void k_pass_colscan(PassBuckets B, uint32_t
     const uint64_t t0 = B.tbase[bi];
     const uint64_t t1 = (bi + 1 < nbig) ? B.tbase[bi + 1] : *dT;
     const uint64_t base = (uint64_t)bi * kDigits;
-    column_scan(t0, t1, B.start[bi], hist, sub_start + base, sub_size + base, sh, gst, gss);
+    const uint64_t tot_d = column_scan<G>(t0, t1, hist, part);
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan<kDigits * G / kWave>(tot_d, tot, sh);
+    if (threadIdx.x < kDigits) {
+        const uint64_t s0 = B.start[bi] + ex;
+        sub_start[base + threadIdx.x] = s0;
+        sub_size[base + threadIdx.x] = tot_d;
+        gst[threadIdx.x] = s0;
+        gss[threadIdx.x] = tot_d;
+    }
     const uint32_t pos = B.pos[bi];
     if (threadIdx.x == 0) {
         // first differing bit among the examined ones, or the end of what was
@@ -853,7 +853,7 @@ __global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64
 // kernel, a scan (one to three launches) and another kernel.
 constexpr int kPlanThreads = 1024;
 constexpr int kPlanWaves = kPlanThreads / kWave;
-constexpr int kPlanPer = 8;
+constexpr int kPlanPer = 16;
 constexpr uint64_t kPlanMax = (uint64_t)kPlanThreads * kPlanPer;  // list entries
 
 // classify + scan + gather of the big buckets, then their setup (digit
@@ -868,37 +868,41 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __r
                                                              PassBuckets B, uint64_t* __restrict__ d_nbig,
                                                              uint64_t* __restrict__ d_items, uint32_t tile_cap,
                                                              const uint64_t* __restrict__ live, int kb,
-                                                             uint64_t* __restrict__ d_T) {
+                                                             uint64_t* __restrict__ d_T,
+                                                             const uint64_t* __restrict__ d_nvalid, uint64_t* mbox,
+                                                             uint64_t seq) {
     __shared__ uint64_t sh[kPlanWaves];
     const uint64_t L = *dL;
     const uint64_t e0 = (uint64_t)threadIdx.x * kPlanPer;
-    uint32_t big[kPlanPer];
-    uint64_t sz[kPlanPer];
+    static_assert(kPlanPer <= 32, "big flags in one word");
+    uint32_t big = 0;
     uint64_t cnt = 0, items = 0;
 #pragma unroll
     for (int k = 0; k < kPlanPer; ++k) {
         const uint64_t e = e0 + k;
-        sz[k] = e < L ? Lsize[e] : 0ull;
-        big[k] = (e < L && sz[k] > cap && !meta_uniform(Lmeta[e])) ? 1u : 0u;
-        cnt += big[k];
-        items += big[k] ? sz[k] : 0ull;
+        const uint64_t sz = e < L ? Lsize[e] : 0ull;
+        const bool b = e < L && sz > cap && !meta_uniform(Lmeta[e]);
+        big |= (b ? 1u : 0u) << k;
+        cnt += b;
+        items += b ? sz : 0ull;
     }
     uint64_t nbig, tot_items;
     const uint64_t off0 = block_exclusive_scan<kPlanWaves>(cnt, nbig, sh);
     (void)block_exclusive_scan<kPlanWaves>(items, tot_items, sh);
     const uint32_t tile = pick_tile(tot_items, tile_cap);
     uint64_t off = off0, nt_sum = 0;
-#pragma unroll
     for (int k = 0; k < kPlanPer; ++k) {
         const uint64_t e = e0 + k;
         if (e >= L) break;
-        isbig[e] = big[k];
+        const uint32_t bk = (big >> k) & 1u;
+        isbig[e] = bk;
         bigpos[e] = off;
-        if (big[k]) {
+        if (bk) {
             const uint32_t meta = Lmeta[e];
-            const uint32_t nt = (uint32_t)((sz[k] + tile - 1) / tile);
+            const uint64_t sz = Lsize[e];
+            const uint32_t nt = (uint32_t)((sz + tile - 1) / tile);
             B.start[off] = Lstart[e];
-            B.size[off] = sz[k];
+            B.size[off] = sz;
             B.meta[off] = meta;
             B.pos[off] = (uint32_t)next_live(live, kb, (int)meta_depth(meta));
             B.ntiles[off] = nt;
@@ -918,6 +922,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __r
         *d_nbig = nbig;
         *d_items = tot_items;
         *d_T = T;
+        // the pass's read-back: list length, big buckets, their items, valid items
+        const uint64_t info[4] = {L, nbig, tot_items, *d_nvalid};
+        post_write(mbox, seq, 4, info);
     }
 }
 
@@ -1642,7 +1649,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     }
     {
         ProfScope ps(ctx, "colscan");
-        k_root_colscan<<<1, kDigits * kRootGroups, 0, st>>>(t0, hist, sub_start, sub_size);
+        k_root_colscan<<<1, kColThreads, 0, st>>>(t0, hist, sub_start, sub_size);
     }
     {
         ProfScope ps(ctx, "msd_scatter_first", 8.0 * (double)n_dense);
@@ -1684,9 +1691,12 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         PB.skip = PB.ntiles + Lb;
         const uint32_t tile_cap = ctx->tune.tile_pass_max ? ctx->tune.tile_pass_max : kTileMaxExplicit;
         const bool plan_small = Lb <= kPlanMax;  // the plan kernel also sets the buckets up
+        Post post;
+        PLH_TRY(post_prepare(ctx, &post));
         if (plan_small) {
             k_plan_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, d_L, cap, isbig, bigpos, PB, d_nbig,
-                                                    d_items, tile_cap, d_live, kb, d_T);
+                                                    d_items, tile_cap, d_live, kb, d_T, d_nvalid, post.mbox,
+                                                    post.seq);
         } else {
             k_classify<<<grid1(Lb), kBlock, 0, st>>>(A.size, A.meta, d_L, Lb, cap, isbig, bigsz);
             PLH_TRY(scan_u32_to_u64(ctx, isbig, bigpos, Lb, d_nbig));
@@ -1694,7 +1704,10 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
             k_gather_big<<<grid1(Lb), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, Lb, PB);
         }
         PLH_HIP(hipGetLastError());
-        PLH_TRY(read_u64n(ctx, d_L, 4, info));
+        if (plan_small)
+            PLH_TRY(post_wait(ctx, post, 4, info));
+        else
+            PLH_TRY(read_u64n(ctx, d_L, 4, info));
         L = info[0];
         n_valid = info[3];
         const uint64_t nbig = info[1], pass_items = info[2];
@@ -1727,8 +1740,12 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         CH.count = CH.meta + nbig * kDigits;
         {
             ProfScope ps(ctx, "colscan");
-            k_pass_colscan<<<dim3((unsigned)nbig), kBlock, 0, st>>>(PB, (uint32_t)nbig, d_T, hist, sub_start, sub_size,
-                                                                    d_live, kb, cap, CH);
+            if (pass_items / tile >= 64 * nbig)  // many tiles per bucket
+                k_pass_colscan<kColGroups><<<dim3((unsigned)nbig), kColThreads, 0, st>>>(
+                    PB, (uint32_t)nbig, d_T, hist, sub_start, sub_size, d_live, kb, cap, CH);
+            else
+                k_pass_colscan<1><<<dim3((unsigned)nbig), kDigits, 0, st>>>(PB, (uint32_t)nbig, d_T, hist, sub_start,
+                                                                            sub_size, d_live, kb, cap, CH);
         }
         {
             ProfScope ps(ctx, "msd_scatter", 16.0 * (double)pass_items);
@@ -1774,9 +1791,8 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     PLH_HIP(hipGetLastError());
     PLH_TRY(ctx->ws.get(kWsOffs, L * 8 + 64, &p));
     uint64_t* offs = (uint64_t*)p;
-    PLH_TRY(scan_u32_to_u64(ctx, kept, offs, L, d_total));
     uint64_t total = 0;
-    PLH_TRY(read_u64(ctx, d_total, &total));
+    PLH_TRY(scan_u32_to_u64_read(ctx, kept, offs, L, d_total, &total));
     if (total != n_valid) {
         // Runs collapsed or terms were dropped: compact out of place.
         plh_sum tmp;

@@ -236,6 +236,18 @@ int scan_u64(plh_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n, ui
 int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out);
 // Read n <= 64 consecutive u64 from device memory (one synchronisation).
 int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out);
+// A read-back posted by a kernel of the caller's: post_prepare gives the
+// mailbox address and sequence number to hand to the kernel (post_write),
+// post_wait polls for it after the launch.
+struct Post {
+    uint64_t* mbox = nullptr;
+    uint64_t seq = 0;
+};
+int post_prepare(plh_ctx* ctx, Post* p);
+int post_wait(plh_ctx* ctx, const Post& p, int n, uint64_t* out);
+// scan_u32_to_u64 whose total also comes back to the host (*host_total).
+int scan_u32_to_u64_read(plh_ctx* ctx, const uint32_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total,
+                         uint64_t* host_total);
 
 // Collation (plh_collate.hip).
 struct CollateSpec {

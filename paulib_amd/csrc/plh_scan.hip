@@ -80,7 +80,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TIn* __restrict__ i
 // One workgroup for short inputs: blocked chunks of kScanTile with a carry.
 template <class TIn>
 __global__ __launch_bounds__(kBlock) void k_scan_small(const TIn* __restrict__ in, uint64_t* __restrict__ out,
-                                                       uint64_t n, uint64_t* __restrict__ total) {
+                                                       uint64_t n, uint64_t* __restrict__ total, uint64_t* mbox,
+                                                       uint64_t seq) {
     __shared__ uint64_t sh[kWavesPerBlock];
     uint64_t carry = 0;
     for (uint64_t c0 = 0; c0 < n; c0 += kScanTile) {
@@ -103,17 +104,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_small(const TIn* __restrict__ i
         }
         carry += tot;
     }
-    if (threadIdx.x == 0) *total = carry;
+    if (threadIdx.x == 0) {
+        *total = carry;
+        if (mbox) post_write(mbox, seq, 1, &carry);
+    }
 }
 
 constexpr uint64_t kScanSmall = 16 * kScanTile;  // one workgroup up to this many elements
 
+// With `post`, a one-workgroup scan also posts the total to the mailbox
+// (*posted set); a longer one leaves that to the caller.
 template <class TIn>
-int scan_impl(plh_ctx* ctx, const TIn* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total) {
+int scan_impl(plh_ctx* ctx, const TIn* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total,
+              const Post* post = nullptr, bool* posted = nullptr) {
     hipStream_t st = ctx->stream;
     if (n <= kScanSmall) {
-        k_scan_small<TIn><<<1, kBlock, 0, st>>>(d_in, d_out, n, d_total);
+        k_scan_small<TIn><<<1, kBlock, 0, st>>>(d_in, d_out, n, d_total, post ? post->mbox : nullptr,
+                                                post ? post->seq : 0ull);
         PLH_HIP(hipGetLastError());
+        if (posted) *posted = post != nullptr;
         return PLH_OK;
     }
     const uint64_t nb = (n + kScanTile - 1) / kScanTile;
@@ -154,20 +163,20 @@ int scan_u64(plh_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n, ui
 
 int read_u64(plh_ctx* ctx, const uint64_t* d_src, uint64_t* out) { return read_u64n(ctx, d_src, 1, out); }
 
-// A read-back posted by a one-wave kernel into the pinned mailbox and polled
-// by the host: the host sees the values as soon as they land, instead of
-// waking from a stream synchronisation after a copy.  After 2 ms of polling
-// it synchronises the stream once (a long-running predecessor, or an error,
-// which the synchronisation reports); the values must be there after it.
-int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out) {
-    if (n < 1 || n > 64) return PLH_ERR_ARG;
+int post_prepare(plh_ctx* ctx, Post* p) {
     if (!ctx->stage.p) PLH_TRY(ctx->stage.init());
-    const uint64_t seq = ++ctx->stage.seq;
-    k_post<<<1, kWave, 0, ctx->stream>>>(d_src, n, ctx->stage.dp, seq);
-    PLH_HIP(hipGetLastError());
+    p->mbox = ctx->stage.dp;
+    p->seq = ++ctx->stage.seq;
+    return PLH_OK;
+}
+
+// Polls the mailbox for p.seq; after 2 ms it synchronises the stream once (a
+// long-running predecessor, or an error, which the synchronisation
+// reports), after which the values must be there.
+int post_wait(plh_ctx* ctx, const Post& p, int n, uint64_t* out) {
     const auto t0 = std::chrono::steady_clock::now();
     bool synced = false;
-    while (__atomic_load_n(&ctx->stage.p[0], __ATOMIC_ACQUIRE) != seq) {
+    while (__atomic_load_n(&ctx->stage.p[0], __ATOMIC_ACQUIRE) != p.seq) {
         if (synced) {
             set_error("read-back: the posting kernel completed without writing the mailbox");
             return PLH_ERR_HIP;
@@ -179,6 +188,28 @@ int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out) {
     }
     for (int i = 0; i < n; ++i) out[i] = __atomic_load_n(&ctx->stage.p[1 + i], __ATOMIC_RELAXED);
     return PLH_OK;
+}
+
+int scan_u32_to_u64_read(plh_ctx* ctx, const uint32_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total,
+                         uint64_t* host_total) {
+    Post post;
+    PLH_TRY(post_prepare(ctx, &post));
+    bool posted = false;
+    PLH_TRY(scan_impl<uint32_t>(ctx, d_in, d_out, n, d_total, &post, &posted));
+    if (posted) return post_wait(ctx, post, 1, host_total);
+    return read_u64(ctx, d_total, host_total);
+}
+
+// A read-back posted by a one-wave kernel into the pinned mailbox and polled
+// by the host: the host sees the values as soon as they land, instead of
+// waking from a stream synchronisation after a copy.
+int read_u64n(plh_ctx* ctx, const uint64_t* d_src, int n, uint64_t* out) {
+    if (n < 1 || n > 64) return PLH_ERR_ARG;
+    Post post;
+    PLH_TRY(post_prepare(ctx, &post));
+    k_post<<<1, kWave, 0, ctx->stream>>>(d_src, n, post.mbox, post.seq);
+    PLH_HIP(hipGetLastError());
+    return post_wait(ctx, post, n, out);
 }
 
 static hipEvent_t take_event(plh_ctx* c) {
