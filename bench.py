@@ -284,8 +284,8 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--m", type=int, default=65536)
     ap.add_argument("--prop-jobs", type=int, default=5)
-    ap.add_argument("--serve-jobs", type=int, default=200)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-jobs", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-plain", type=int, default=8)
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--batch-jobs", type=int, default=512)
