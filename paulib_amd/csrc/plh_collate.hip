@@ -856,23 +856,16 @@ constexpr int kPlanWaves = kPlanThreads / kWave;
 constexpr int kPlanPer = 16;
 constexpr uint64_t kPlanMax = (uint64_t)kPlanThreads * kPlanPer;  // list entries
 
-// classify + scan + gather of the big buckets, then their setup (digit
-// position, tile count, tile base) for tiles of pick_tile(items, tile_cap)
-// ids, the size the host derives from the same item count; entry e is
-// thread e / 8's.
-__global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __restrict__ Lstart,
-                                                             const uint64_t* __restrict__ Lsize,
-                                                             const uint32_t* __restrict__ Lmeta,
-                                                             const uint64_t* __restrict__ dL, uint32_t cap,
-                                                             uint32_t* __restrict__ isbig, uint64_t* __restrict__ bigpos,
-                                                             PassBuckets B, uint64_t* __restrict__ d_nbig,
-                                                             uint64_t* __restrict__ d_items, uint32_t tile_cap,
-                                                             const uint64_t* __restrict__ live, int kb,
-                                                             uint64_t* __restrict__ d_T,
-                                                             const uint64_t* __restrict__ d_nvalid, uint64_t* mbox,
-                                                             uint64_t seq) {
-    __shared__ uint64_t sh[kPlanWaves];
-    const uint64_t L = *dL;
+// classify + scan + gather of the big buckets of a list of L entries, then
+// their setup (digit position, tile count, tile base) for tiles of
+// pick_tile(items, tile_cap) ids, the size the host derives from the same
+// item count; thread 0 posts the pass's read-back {L, big buckets, their
+// items, valid items}.  Entry e is thread e / kPlanPer's.  One workgroup of
+// kPlanThreads; L <= kPlanMax.
+PLH_DEV void plan_list(const uint64_t* Lstart, const uint64_t* Lsize, const uint32_t* Lmeta, uint64_t L, uint32_t cap,
+                       uint32_t* isbig, uint64_t* bigpos, const PassBuckets& B, uint64_t* d_nbig,
+                       uint64_t* d_items, uint32_t tile_cap, const uint64_t* live, int kb, uint64_t* d_T,
+                       uint64_t n_valid, uint64_t* mbox, uint64_t seq, uint64_t* sh) {
     const uint64_t e0 = (uint64_t)threadIdx.x * kPlanPer;
     static_assert(kPlanPer <= 32, "big flags in one word");
     uint32_t big = 0;
@@ -922,10 +915,25 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __r
         *d_nbig = nbig;
         *d_items = tot_items;
         *d_T = T;
-        // the pass's read-back: list length, big buckets, their items, valid items
-        const uint64_t info[4] = {L, nbig, tot_items, *d_nvalid};
+        const uint64_t info[4] = {L, nbig, tot_items, n_valid};
         post_write(mbox, seq, 4, info);
     }
+}
+
+__global__ __launch_bounds__(kPlanThreads) void k_plan_small(const uint64_t* __restrict__ Lstart,
+                                                             const uint64_t* __restrict__ Lsize,
+                                                             const uint32_t* __restrict__ Lmeta,
+                                                             const uint64_t* __restrict__ dL, uint32_t cap,
+                                                             uint32_t* __restrict__ isbig, uint64_t* __restrict__ bigpos,
+                                                             PassBuckets B, uint64_t* __restrict__ d_nbig,
+                                                             uint64_t* __restrict__ d_items, uint32_t tile_cap,
+                                                             const uint64_t* __restrict__ live, int kb,
+                                                             uint64_t* __restrict__ d_T,
+                                                             const uint64_t* __restrict__ d_nvalid, uint64_t* mbox,
+                                                             uint64_t seq) {
+    __shared__ uint64_t sh[kPlanWaves];
+    plan_list(Lstart, Lsize, Lmeta, *dL, cap, isbig, bigpos, B, d_nbig, d_items, tile_cap, live, kb, d_T, *d_nvalid,
+              mbox, seq, sh);
 }
 
 // bucket setup + scan of the tile counts.
@@ -960,15 +968,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_setup_small(PassBuckets B, uin
     if (threadIdx.x == 0) *d_T = T;
 }
 
-// child count + scan + child write; the new length goes to *d_L.  Entry e
-// belongs to thread e % kPlanThreads in round e / kPlanThreads, so a short
-// list spreads over the workgroup.
-__global__ __launch_bounds__(kPlanThreads) void k_expand_small(
-    const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize, const uint32_t* __restrict__ Lmeta,
-    const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos, Children C, uint64_t L,
-    uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta,
-    uint64_t* __restrict__ d_L) {
-    __shared__ uint64_t sh[kPlanWaves];
+// child count + scan + child write; the new length goes to *d_L and is
+// returned.  Entry e belongs to thread e % kPlanThreads in round e /
+// kPlanThreads, so a short list spreads over the workgroup.
+PLH_DEV uint64_t expand_list(const uint64_t* Lstart, const uint64_t* Lsize, const uint32_t* Lmeta,
+                             const uint32_t* isbig, const uint64_t* bigpos, const Children& C, uint64_t L,
+                             uint64_t* Nstart, uint64_t* Nsize, uint32_t* Nmeta, uint64_t* d_L, uint64_t* sh) {
     uint64_t carry = 0;
     for (uint64_t r0 = 0; r0 < L; r0 += kPlanThreads) {
         const uint64_t e = r0 + threadIdx.x;
@@ -979,6 +984,36 @@ __global__ __launch_bounds__(kPlanThreads) void k_expand_small(
         carry += tot;
     }
     if (threadIdx.x == 0) *d_L = carry;
+    return carry;
+}
+
+__global__ __launch_bounds__(kPlanThreads) void k_expand_small(
+    const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize, const uint32_t* __restrict__ Lmeta,
+    const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos, Children C, uint64_t L,
+    uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta,
+    uint64_t* __restrict__ d_L) {
+    __shared__ uint64_t sh[kPlanWaves];
+    (void)expand_list(Lstart, Lsize, Lmeta, isbig, bigpos, C, L, Nstart, Nsize, Nmeta, d_L, sh);
+}
+
+// The expansion into the next list and that list's plan in one workgroup
+// (the new list's entries, written by all threads, are read back after a
+// workgroup barrier).  The plan's big-bucket flags go to a second set of
+// list arrays (isbig2, bigpos2): the expansion still reads the first.
+// Needs L <= kPlanMax and a next list of at most kPlanMax entries.
+__global__ __launch_bounds__(kPlanThreads) void k_expand_plan_small(
+    const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize, const uint32_t* __restrict__ Lmeta,
+    const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos, Children C, uint64_t L,
+    uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta,
+    uint64_t* __restrict__ d_L, uint32_t cap, uint32_t* __restrict__ isbig2, uint64_t* __restrict__ bigpos2,
+    PassBuckets B2, uint64_t* __restrict__ d_nbig, uint64_t* __restrict__ d_items, uint32_t tile_cap,
+    const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ d_T, const uint64_t* __restrict__ d_nvalid,
+    uint64_t* mbox, uint64_t seq) {
+    __shared__ uint64_t sh[kPlanWaves];
+    const uint64_t L2 = expand_list(Lstart, Lsize, Lmeta, isbig, bigpos, C, L, Nstart, Nsize, Nmeta, d_L, sh);
+    __syncthreads();  // the new list's entries, written by other threads
+    plan_list(Nstart, Nsize, Nmeta, L2, cap, isbig2, bigpos2, B2, d_nbig, d_items, tile_cap, live, kb, d_T,
+              *d_nvalid, mbox, seq, sh);
 }
 
 // --------------------------------------------------------------- leaves
@@ -1668,32 +1703,61 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     // ---- global passes until every non-uniform bucket fits a leaf.  Grids
     // of the list kernels use the bound Lb; the exact counts come back in
     // one read per pass.
+    // list arrays (isbig u32, cnt u32, bigpos u64, bigsz u64, newpos u64) for a
+    // list bound of lb entries, in one of two alternating slots
+    struct ListArrays {
+        uint32_t* isbig;
+        uint32_t* cnt;
+        uint64_t* bigpos;
+        uint64_t* bigsz;
+        uint64_t* newpos;
+    };
+    auto list_arrays = [&](int slot, uint64_t lb, ListArrays* la) -> int {
+        void* q = nullptr;
+        PLH_TRY(ctx->ws.get(slot, lb * 32 + 256, &q));
+        la->isbig = (uint32_t*)q;
+        la->cnt = la->isbig + lb;
+        la->bigpos = (uint64_t*)((char*)q + ((lb * 8 + 15) & ~(uint64_t)15));
+        la->bigsz = la->bigpos + lb;
+        la->newpos = la->bigsz + lb;
+        return PLH_OK;
+    };
+    // the big-bucket arrays, sized by the bound lb before the counts are known
+    auto pass_buckets = [&](uint64_t lb, PassBuckets* pb) -> int {
+        void* q = nullptr;
+        PLH_TRY(ctx->ws.get(kWsPassB, lb * 56 + 256, &q));
+        pb->start = (uint64_t*)q;
+        pb->size = pb->start + lb;
+        pb->tbase = pb->size + lb;
+        pb->kor = pb->tbase + lb;
+        pb->kand = pb->kor + lb;
+        pb->meta = (uint32_t*)(pb->kand + lb);
+        pb->pos = pb->meta + lb;
+        pb->ntiles = pb->pos + lb;
+        pb->skip = pb->ntiles + lb;
+        return PLH_OK;
+    };
+    const uint32_t tile_cap = ctx->tune.tile_pass_max ? ctx->tune.tile_pass_max : kTileMaxExplicit;
+    int pl_slot = kWsPassL;
+    bool planned = false;  // the pass's plan ran inside the previous expansion
+    Post post;
     for (int guard = 0; guard < 2 * kb; ++guard) {
-        // list arrays: isbig u32, bigpos u64, bigsz u64, cnt u32, newpos u64
-        PLH_TRY(ctx->ws.get(kWsPassL, Lb * 32 + 256, &p));
-        uint32_t* isbig = (uint32_t*)p;
-        uint32_t* cnt = isbig + Lb;
-        uint64_t* bigpos = (uint64_t*)((char*)p + ((Lb * 8 + 15) & ~(uint64_t)15));
-        uint64_t* bigsz = bigpos + Lb;
-        uint64_t* newpos = bigsz + Lb;
+        ListArrays LA;
+        PLH_TRY(list_arrays(pl_slot, Lb, &LA));
+        uint32_t* isbig = LA.isbig;
+        uint32_t* cnt = LA.cnt;
+        uint64_t* bigpos = LA.bigpos;
+        uint64_t* bigsz = LA.bigsz;
+        uint64_t* newpos = LA.newpos;
         uint64_t info[4];
-        // the big-bucket arrays are sized by the bound Lb before the counts are known
-        PLH_TRY(ctx->ws.get(kWsPassB, Lb * 56 + 256, &p));
         PassBuckets PB;
-        PB.start = (uint64_t*)p;
-        PB.size = PB.start + Lb;
-        PB.tbase = PB.size + Lb;
-        PB.kor = PB.tbase + Lb;
-        PB.kand = PB.kor + Lb;
-        PB.meta = (uint32_t*)(PB.kand + Lb);
-        PB.pos = PB.meta + Lb;
-        PB.ntiles = PB.pos + Lb;
-        PB.skip = PB.ntiles + Lb;
-        const uint32_t tile_cap = ctx->tune.tile_pass_max ? ctx->tune.tile_pass_max : kTileMaxExplicit;
-        const bool plan_small = Lb <= kPlanMax;  // the plan kernel also sets the buckets up
-        Post post;
-        PLH_TRY(post_prepare(ctx, &post));
-        if (plan_small) {
+        PLH_TRY(pass_buckets(Lb, &PB));
+        // the plan kernel also sets the buckets up (and so does the fused expansion)
+        const bool plan_small = planned || Lb <= kPlanMax;
+        if (planned) {
+            // plan and read-back already issued
+        } else if (plan_small) {
+            PLH_TRY(post_prepare(ctx, &post));
             k_plan_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, d_L, cap, isbig, bigpos, PB, d_nbig,
                                                     d_items, tile_cap, d_live, kb, d_T, d_nvalid, post.mbox,
                                                     post.seq);
@@ -1708,6 +1772,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
             PLH_TRY(post_wait(ctx, post, 4, info));
         else
             PLH_TRY(read_u64n(ctx, d_L, 4, info));
+        planned = false;
         L = info[0];
         n_valid = info[3];
         const uint64_t nbig = info[1], pass_items = info[2];
@@ -1756,8 +1821,21 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         // each split bucket has at most kDigits children; every entry holds >= 1 item
         const uint64_t Lb2 = std::min<uint64_t>(L + (kDigits - 1) * nbig, n_valid);
         const int nslot = cur_slot == kWsListA ? kWsListB : kWsListA;
+        const int npl = pl_slot == kWsPassL ? kWsPassL2 : kWsPassL;
         PLH_TRY(get_lists(ctx, nslot, Lb2, &B));
-        if (L <= kPlanMax) {
+        if (L <= kPlanMax && Lb2 <= kPlanMax) {
+            // expansion + the next pass's plan (its read-back posted)
+            ListArrays LA2;
+            PLH_TRY(list_arrays(npl, Lb2, &LA2));
+            PassBuckets PB2;
+            PLH_TRY(pass_buckets(Lb2, &PB2));
+            PLH_TRY(post_prepare(ctx, &post));
+            k_expand_plan_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, CH, L, B.start,
+                                                           B.size, B.meta, d_L, cap, LA2.isbig, LA2.bigpos, PB2,
+                                                           d_nbig, d_items, tile_cap, d_live, kb, d_T, d_nvalid,
+                                                           post.mbox, post.seq);
+            planned = true;
+        } else if (L <= kPlanMax) {
             k_expand_small<<<1, kPlanThreads, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, CH, L, B.start, B.size,
                                                       B.meta, d_L);
         } else {
@@ -1769,6 +1847,7 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         PLH_HIP(hipGetLastError());
         std::swap(A, B);
         cur_slot = nslot;
+        pl_slot = npl;
         Lb = Lb2;
     }
 

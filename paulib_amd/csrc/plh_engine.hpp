@@ -211,7 +211,8 @@ enum : int {
     kWsTmpSum,
     kWsPackL,
     kWsPackR,
-    kWsPassL,   // per-pass arrays over the bucket list
+    kWsPassL,   // per-pass arrays over the bucket list (alternating with kWsPassL2)
+    kWsPassL2,
     kWsPassB,   // per-pass arrays over the big buckets
     kWsChild,   // children of the big buckets of a pass
     kWsJobs,    // product batch: job table and id-chunk table
