@@ -1177,15 +1177,18 @@ PLH_DEV bool keep_term(const Cfg& c, const uint64_t* kw, double acc) {
     return true;
 }
 
+// Output stores are non-temporal: the output is not read again by the
+// collation, and streaming it past the caches keeps the operand records
+// the leaves gather from resident.
 template <int AS, int XO>
 PLH_DEV void write_term(const Cfg& c, uint64_t o, const uint64_t* kw, uint32_t fl, double v) {
     for (int w = 0; w < c.out_as; ++w) {
-        c.ox[(int64_t)w * c.ostride + o] = kw[XO + w];
-        c.oz[(int64_t)w * c.ostride + o] = kw[XO + AS + w];
+        __builtin_nontemporal_store(kw[XO + w], &c.ox[(int64_t)w * c.ostride + o]);
+        __builtin_nontemporal_store(kw[XO + AS + w], &c.oz[(int64_t)w * c.ostride + o]);
     }
-    if (XO && c.oj) c.oj[o] = (uint32_t)kw[0];
-    c.of[o] = (uint8_t)fl;
-    c.ov[o] = v;
+    if (XO && c.oj) __builtin_nontemporal_store((uint32_t)kw[0], &c.oj[o]);
+    __builtin_nontemporal_store((uint8_t)fl, &c.of[o]);
+    __builtin_nontemporal_store(v, &c.ov[o]);
 }
 
 // One leaf per workgroup: sort, sum runs, filter, write at out_base.
@@ -1499,12 +1502,12 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         const int s = rslot[r];
         const uint64_t o = out_base + r;
         for (int w = 0; w < c.out_as; ++w) {
-            c.ox[(int64_t)w * c.ostride + o] = L.key[XO + w][s];
-            c.oz[(int64_t)w * c.ostride + o] = L.key[XO + AS + w][s];
+            __builtin_nontemporal_store(L.key[XO + w][s], &c.ox[(int64_t)w * c.ostride + o]);
+            __builtin_nontemporal_store(L.key[XO + AS + w][s], &c.oz[(int64_t)w * c.ostride + o]);
         }
-        if (XO && c.oj) c.oj[o] = (uint32_t)L.key[0][s];
-        c.of[o] = (uint8_t)(L.fl[s] & c.fmask);
-        c.ov[o] = L.val[s];
+        if (XO && c.oj) __builtin_nontemporal_store((uint32_t)L.key[0][s], &c.oj[o]);
+        __builtin_nontemporal_store((uint8_t)(L.fl[s] & c.fmask), &c.of[o]);
+        __builtin_nontemporal_store(L.val[s], &c.ov[o]);
     }
     if (tid == 0) kept_out[e] = total;
 }
