@@ -971,17 +971,49 @@ __global__ __launch_bounds__(kPlanThreads) void k_setup_small(PassBuckets B, uin
 // child count + scan + child write; the new length goes to *d_L and is
 // returned.  Entry e belongs to thread e % kPlanThreads in round e /
 // kPlanThreads, so a short list spreads over the workgroup.
+// The round's output slots are written by all threads: slot q of the round
+// belongs to the entry whose offset is the last one <= q (every entry has at
+// least one child, so the offsets in `roff` increase strictly), so a big
+// bucket's up to 256 children are copied in parallel, not by its thread.
 PLH_DEV uint64_t expand_list(const uint64_t* Lstart, const uint64_t* Lsize, const uint32_t* Lmeta,
                              const uint32_t* isbig, const uint64_t* bigpos, const Children& C, uint64_t L,
-                             uint64_t* Nstart, uint64_t* Nsize, uint32_t* Nmeta, uint64_t* d_L, uint64_t* sh) {
+                             uint64_t* Nstart, uint64_t* Nsize, uint32_t* Nmeta, uint64_t* d_L, uint64_t* sh,
+                             uint32_t* roff) {
     uint64_t carry = 0;
     for (uint64_t r0 = 0; r0 < L; r0 += kPlanThreads) {
         const uint64_t e = r0 + threadIdx.x;
         const uint64_t c = e < L ? expand_count_one(e, isbig, bigpos, C) : 0ull;
         uint64_t tot;
-        const uint64_t o = carry + block_exclusive_scan<kPlanWaves>(c, tot, sh);
-        if (e < L) expand_write_one(e, o, Lstart, Lsize, Lmeta, isbig, bigpos, C, Nstart, Nsize, Nmeta);
+        const uint64_t o = block_exclusive_scan<kPlanWaves>(c, tot, sh);
+        roff[threadIdx.x] = (uint32_t)o;  // < kPlanThreads * kDigits
+        __syncthreads();
+        const uint32_t nent = (uint32_t)min<uint64_t>(L - r0, (uint64_t)kPlanThreads);
+        for (uint32_t q = threadIdx.x; q < (uint32_t)tot; q += kPlanThreads) {
+            uint32_t lo = 0, n = nent;  // last t with roff[t] <= q
+            while (n > 1) {
+                const uint32_t half = n / 2;
+                if (roff[lo + half] <= q) {
+                    lo += half;
+                    n -= half;
+                } else {
+                    n = half;
+                }
+            }
+            const uint64_t ee = r0 + lo, out = carry + q;
+            const uint32_t k = q - roff[lo];
+            if (!isbig[ee]) {
+                Nstart[out] = Lstart[ee];
+                Nsize[out] = Lsize[ee];
+                Nmeta[out] = Lmeta[ee];
+            } else {
+                const uint64_t src = bigpos[ee] * kDigits + k;
+                Nstart[out] = C.start[src];
+                Nsize[out] = C.size[src];
+                Nmeta[out] = C.meta[src];
+            }
+        }
         carry += tot;
+        __syncthreads();  // roff is rewritten by the next round
     }
     if (threadIdx.x == 0) *d_L = carry;
     return carry;
@@ -993,7 +1025,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_expand_small(
     uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta,
     uint64_t* __restrict__ d_L) {
     __shared__ uint64_t sh[kPlanWaves];
-    (void)expand_list(Lstart, Lsize, Lmeta, isbig, bigpos, C, L, Nstart, Nsize, Nmeta, d_L, sh);
+    __shared__ uint32_t roff[kPlanThreads];
+    (void)expand_list(Lstart, Lsize, Lmeta, isbig, bigpos, C, L, Nstart, Nsize, Nmeta, d_L, sh, roff);
 }
 
 // The expansion into the next list and that list's plan in one workgroup
@@ -1010,7 +1043,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_expand_plan_small(
     const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ d_T, const uint64_t* __restrict__ d_nvalid,
     uint64_t* mbox, uint64_t seq) {
     __shared__ uint64_t sh[kPlanWaves];
-    const uint64_t L2 = expand_list(Lstart, Lsize, Lmeta, isbig, bigpos, C, L, Nstart, Nsize, Nmeta, d_L, sh);
+    __shared__ uint32_t roff[kPlanThreads];
+    const uint64_t L2 =
+        expand_list(Lstart, Lsize, Lmeta, isbig, bigpos, C, L, Nstart, Nsize, Nmeta, d_L, sh, roff);
     __syncthreads();  // the new list's entries, written by other threads
     plan_list(Nstart, Nsize, Nmeta, L2, cap, isbig2, bigpos2, B2, d_nbig, d_items, tile_cap, live, kb, d_T,
               *d_nvalid, mbox, seq, sh);
