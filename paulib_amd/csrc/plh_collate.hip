@@ -815,39 +815,13 @@ PLH_DEV uint32_t expand_count_one(uint64_t e, const uint32_t* isbig, const uint6
     return isbig[e] ? C.count[bigpos[e]] : 1u;
 }
 
-// Writes entry e's children at o (the next pass's list).
-PLH_DEV void expand_write_one(uint64_t e, uint64_t o, const uint64_t* Lstart, const uint64_t* Lsize,
-                              const uint32_t* Lmeta, const uint32_t* isbig, const uint64_t* bigpos,
-                              const Children& C, uint64_t* Nstart, uint64_t* Nsize, uint32_t* Nmeta) {
-    if (!isbig[e]) {
-        Nstart[o] = Lstart[e];
-        Nsize[o] = Lsize[e];
-        Nmeta[o] = Lmeta[e];
-        return;
-    }
-    const uint64_t b = bigpos[e];
-    const uint32_t k = C.count[b];
-    for (uint32_t c = 0; c < k; ++c) {
-        Nstart[o + c] = C.start[b * kDigits + c];
-        Nsize[o + c] = C.size[b * kDigits + c];
-        Nmeta[o + c] = C.meta[b * kDigits + c];
-    }
-}
-
 __global__ void k_expand_count(const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos, Children C,
                                uint64_t L, uint32_t* __restrict__ cnt) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < L) cnt[e] = expand_count_one(e, isbig, bigpos, C);
 }
 
-__global__ void k_expand_write(const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize,
-                               const uint32_t* __restrict__ Lmeta, const uint32_t* __restrict__ isbig,
-                               const uint64_t* __restrict__ bigpos, Children C,
-                               const uint64_t* __restrict__ newpos, uint64_t L, uint64_t* __restrict__ Nstart,
-                               uint64_t* __restrict__ Nsize, uint32_t* __restrict__ Nmeta) {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < L) expand_write_one(e, newpos[e], Lstart, Lsize, Lmeta, isbig, bigpos, C, Nstart, Nsize, Nmeta);
-}
+
 
 // ---- short lists: one workgroup does a list step that otherwise takes a
 // kernel, a scan (one to three launches) and another kernel.
@@ -971,6 +945,58 @@ __global__ __launch_bounds__(kPlanThreads) void k_setup_small(PassBuckets B, uin
 // child count + scan + child write; the new length goes to *d_L and is
 // returned.  Entry e belongs to thread e % kPlanThreads in round e /
 // kPlanThreads, so a short list spreads over the workgroup.
+// Copies the children of entries r0 .. r0+nent-1 (their output offsets
+// relative to out0 in roff, in LDS) to out0 .. out0+tot-1, one output slot
+// per thread: slot q belongs to the entry whose offset is the last one <= q
+// (every entry has at least one child, so the offsets increase strictly).
+PLH_DEV void expand_copy(const uint32_t* roff, uint32_t nent, uint64_t r0, uint64_t out0, uint32_t tot,
+                         const uint64_t* Lstart, const uint64_t* Lsize, const uint32_t* Lmeta, const uint32_t* isbig,
+                         const uint64_t* bigpos, const Children& C, uint64_t* Nstart, uint64_t* Nsize,
+                         uint32_t* Nmeta) {
+    for (uint32_t q = threadIdx.x; q < tot; q += blockDim.x) {
+        uint32_t lo = 0, n = nent;  // last t with roff[t] <= q
+        while (n > 1) {
+            const uint32_t half = n / 2;
+            if (roff[lo + half] <= q) {
+                lo += half;
+                n -= half;
+            } else {
+                n = half;
+            }
+        }
+        const uint64_t ee = r0 + lo, out = out0 + q;
+        const uint32_t k = q - roff[lo];
+        if (!isbig[ee]) {
+            Nstart[out] = Lstart[ee];
+            Nsize[out] = Lsize[ee];
+            Nmeta[out] = Lmeta[ee];
+        } else {
+            const uint64_t src = bigpos[ee] * kDigits + k;
+            Nstart[out] = C.start[src];
+            Nsize[out] = C.size[src];
+            Nmeta[out] = C.meta[src];
+        }
+    }
+}
+
+// Long lists: one workgroup per kPlanThreads consecutive entries, their
+// children copied by all its threads (newpos: exclusive scan of the counts).
+__global__ __launch_bounds__(kPlanThreads) void k_expand_write(
+    const uint64_t* __restrict__ Lstart, const uint64_t* __restrict__ Lsize, const uint32_t* __restrict__ Lmeta,
+    const uint32_t* __restrict__ isbig, const uint64_t* __restrict__ bigpos, Children C,
+    const uint64_t* __restrict__ newpos, uint64_t L, uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize,
+    uint32_t* __restrict__ Nmeta) {
+    __shared__ uint32_t roff[kPlanThreads];
+    const uint64_t r0 = (uint64_t)blockIdx.x * kPlanThreads;
+    const uint32_t nent = (uint32_t)min<uint64_t>(L - r0, (uint64_t)kPlanThreads);
+    const uint64_t base = newpos[r0];
+    if (threadIdx.x < nent) roff[threadIdx.x] = (uint32_t)(newpos[r0 + threadIdx.x] - base);  // < 1024 * 256
+    __syncthreads();
+    const uint64_t last = r0 + nent - 1;
+    const uint32_t tot = roff[nent - 1] + expand_count_one(last, isbig, bigpos, C);
+    expand_copy(roff, nent, r0, base, tot, Lstart, Lsize, Lmeta, isbig, bigpos, C, Nstart, Nsize, Nmeta);
+}
+
 // The round's output slots are written by all threads: slot q of the round
 // belongs to the entry whose offset is the last one <= q (every entry has at
 // least one child, so the offsets in `roff` increase strictly), so a big
@@ -988,30 +1014,8 @@ PLH_DEV uint64_t expand_list(const uint64_t* Lstart, const uint64_t* Lsize, cons
         roff[threadIdx.x] = (uint32_t)o;  // < kPlanThreads * kDigits
         __syncthreads();
         const uint32_t nent = (uint32_t)min<uint64_t>(L - r0, (uint64_t)kPlanThreads);
-        for (uint32_t q = threadIdx.x; q < (uint32_t)tot; q += kPlanThreads) {
-            uint32_t lo = 0, n = nent;  // last t with roff[t] <= q
-            while (n > 1) {
-                const uint32_t half = n / 2;
-                if (roff[lo + half] <= q) {
-                    lo += half;
-                    n -= half;
-                } else {
-                    n = half;
-                }
-            }
-            const uint64_t ee = r0 + lo, out = carry + q;
-            const uint32_t k = q - roff[lo];
-            if (!isbig[ee]) {
-                Nstart[out] = Lstart[ee];
-                Nsize[out] = Lsize[ee];
-                Nmeta[out] = Lmeta[ee];
-            } else {
-                const uint64_t src = bigpos[ee] * kDigits + k;
-                Nstart[out] = C.start[src];
-                Nsize[out] = C.size[src];
-                Nmeta[out] = C.meta[src];
-            }
-        }
+        expand_copy(roff, nent, r0, carry, (uint32_t)tot, Lstart, Lsize, Lmeta, isbig, bigpos, C, Nstart, Nsize,
+                    Nmeta);
         carry += tot;
         __syncthreads();  // roff is rewritten by the next round
     }
@@ -1879,8 +1883,8 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
         } else {
             k_expand_count<<<grid1(L), kBlock, 0, st>>>(isbig, bigpos, CH, L, cnt);
             PLH_TRY(scan_u32_to_u64(ctx, cnt, newpos, L, d_L));
-            k_expand_write<<<grid1(L), kBlock, 0, st>>>(A.start, A.size, A.meta, isbig, bigpos, CH, newpos, L, B.start,
-                                                        B.size, B.meta);
+            k_expand_write<<<dim3((unsigned)((L + kPlanThreads - 1) / kPlanThreads)), kPlanThreads, 0, st>>>(
+                A.start, A.size, A.meta, isbig, bigpos, CH, newpos, L, B.start, B.size, B.meta);
         }
         PLH_HIP(hipGetLastError());
         std::swap(A, B);
