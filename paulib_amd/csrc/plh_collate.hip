@@ -1614,34 +1614,46 @@ __global__ __launch_bounds__(kBlock) void k_live_or(SumView s, const uint32_t* _
     if (lane_id() == 0 && acc) atomicOr(&live[kw], (unsigned long long)acc);
 }
 
-// The root's children as the first bucket list (buffer 0), merged like
-// a split bucket's (one wave); also the list length and the number of valid items.
-__global__ __launch_bounds__(kWave) void k_root_list(const uint64_t* __restrict__ sub_start,
-                                                     const uint64_t* __restrict__ sub_size, uint32_t cap,
-                                                     const uint64_t* __restrict__ live, int kb,
-                                                     uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize,
-                                                     uint32_t* __restrict__ Nmeta, uint64_t* __restrict__ d_L,
-                                                     uint64_t* __restrict__ d_nvalid) {
-    const int lane = (int)threadIdx.x;
-    uint64_t st[4], ss[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        st[k] = sub_start[4 * lane + k];
-        ss[k] = sub_size[4 * lane + k];
-    }
-    const int pos = root_pos(live, kb);
-    const int depth = min(pos + kDigitBits, kb);
-    const uint32_t uni = next_live(live, kb, depth) >= kb ? 1u : 0u;
+// The root's children as the first bucket list (buffer 0), grouped like a
+// split bucket's, and that list's plan, in one workgroup: wave 0 groups the
+// root's sub-buckets (wave-local) and writes the list length and the number
+// of valid items, then all threads plan the list.
+__global__ __launch_bounds__(kPlanThreads) void k_root_list_plan(
+    const uint64_t* __restrict__ sub_start, const uint64_t* __restrict__ sub_size, uint32_t cap,
+    const uint64_t* __restrict__ live, int kb, uint64_t* __restrict__ Nstart, uint64_t* __restrict__ Nsize,
+    uint32_t* __restrict__ Nmeta, uint64_t* __restrict__ d_L, uint64_t* __restrict__ d_nvalid,
+    uint32_t* __restrict__ isbig, uint64_t* __restrict__ bigpos, PassBuckets B, uint64_t* __restrict__ d_nbig,
+    uint64_t* __restrict__ d_items, uint32_t tile_cap, uint64_t* __restrict__ d_T, uint64_t* mbox, uint64_t seq) {
     __shared__ GroupShared g;
-    const uint32_t n = group_children(st, ss, cap, g, [&](uint32_t o, uint64_t s0, uint64_t sz, uint32_t parts) {
-        Nstart[o] = s0;
-        Nsize[o] = sz;
-        Nmeta[o] = parts == 1 ? make_meta((uint32_t)depth, 0u, uni) : make_meta((uint32_t)pos, 0u, 0u);
-    });
-    if (lane == 0) {
-        *d_L = n;
-        *d_nvalid = n ? g.P[kDigits] : 0ull;
+    __shared__ uint64_t sh[kPlanWaves];
+    __shared__ uint64_t s_len, s_valid;
+    if (threadIdx.x < kWave) {
+        const int lane = (int)threadIdx.x;
+        uint64_t st[4], ss[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            st[k] = sub_start[4 * lane + k];
+            ss[k] = sub_size[4 * lane + k];
+        }
+        const int pos = root_pos(live, kb);
+        const int depth = min(pos + kDigitBits, kb);
+        const uint32_t uni = next_live(live, kb, depth) >= kb ? 1u : 0u;
+        const uint32_t n = group_children(st, ss, cap, g, [&](uint32_t o, uint64_t s0, uint64_t sz, uint32_t parts) {
+            Nstart[o] = s0;
+            Nsize[o] = sz;
+            Nmeta[o] = parts == 1 ? make_meta((uint32_t)depth, 0u, uni) : make_meta((uint32_t)pos, 0u, 0u);
+        });
+        if (lane == 0) {
+            const uint64_t nv = n ? g.P[kDigits] : 0ull;
+            *d_L = n;
+            *d_nvalid = nv;
+            s_len = n;
+            s_valid = nv;
+        }
     }
+    __syncthreads();  // the list, written by wave 0
+    plan_list(Nstart, Nsize, Nmeta, s_len, cap, isbig, bigpos, B, d_nbig, d_items, tile_cap, live, kb, d_T, s_valid,
+              mbox, seq, sh);
 }
 
 // ------------------------------------------------------------------ driver
@@ -1734,17 +1746,6 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
                                                                     sub_start);
     }
     PLH_HIP(hipGetLastError());
-    Lists A, B;
-    PLH_TRY(get_lists(ctx, kWsListA, kDigits, &A));
-    k_root_list<<<1, kWave, 0, st>>>(sub_start, sub_size, cap, d_live, kb, A.start, A.size, A.meta, d_L, d_nvalid);
-    PLH_HIP(hipGetLastError());
-    int cur_slot = kWsListA;
-    uint64_t Lb = kDigits;  // bound on the list length
-    uint64_t L = 0, n_valid = 0;
-
-    // ---- global passes until every non-uniform bucket fits a leaf.  Grids
-    // of the list kernels use the bound Lb; the exact counts come back in
-    // one read per pass.
     // list arrays (isbig u32, cnt u32, bigpos u64, bigsz u64, newpos u64) for a
     // list bound of lb entries, in one of two alternating slots
     struct ListArrays {
@@ -1781,8 +1782,29 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
     };
     const uint32_t tile_cap = ctx->tune.tile_pass_max ? ctx->tune.tile_pass_max : kTileMaxExplicit;
     int pl_slot = kWsPassL;
-    bool planned = false;  // the pass's plan ran inside the previous expansion
     Post post;
+    // ---- the root's children as the first list, and that list's plan
+    Lists A, B;
+    PLH_TRY(get_lists(ctx, kWsListA, kDigits, &A));
+    {
+        ListArrays LA0;
+        PLH_TRY(list_arrays(pl_slot, kDigits, &LA0));
+        PassBuckets PB0;
+        PLH_TRY(pass_buckets(kDigits, &PB0));
+        PLH_TRY(post_prepare(ctx, &post));
+        k_root_list_plan<<<1, kPlanThreads, 0, st>>>(sub_start, sub_size, cap, d_live, kb, A.start, A.size, A.meta,
+                                                     d_L, d_nvalid, LA0.isbig, LA0.bigpos, PB0, d_nbig, d_items,
+                                                     tile_cap, d_T, post.mbox, post.seq);
+    }
+    PLH_HIP(hipGetLastError());
+    int cur_slot = kWsListA;
+    uint64_t Lb = kDigits;  // bound on the list length
+    uint64_t L = 0, n_valid = 0;
+
+    // ---- global passes until every non-uniform bucket fits a leaf.  Grids
+    // of the list kernels use the bound Lb; the exact counts come back in
+    // one read per pass.
+    bool planned = true;  // the pass's plan ran inside the previous kernel (root list or expansion)
     for (int guard = 0; guard < 2 * kb; ++guard) {
         ListArrays LA;
         PLH_TRY(list_arrays(pl_slot, Lb, &LA));
