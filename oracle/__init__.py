@@ -48,6 +48,9 @@ _SIG = {
     "apply_S": (C.c_int, [_p, _i64]),
     "apply_CNOT": (C.c_int, [_p, _i64, _i64]),
     "evolve": (C.c_int, [_p, _p]),
+    "apply_gate": (C.c_int, [_p, C.c_int, _i64, _i64]),
+    "apply_rot_axis": (C.c_int, [_p, C.c_int, _i64, C.c_double]),
+    "elementwise_multiply": (C.c_int, [_p, _p]),
     "apply_truncation": (C.c_int, [_p, _i64, C.c_double, _i64]),
     "propagate": (C.c_int, [_p, _p, _i64, C.c_double, _i64]),
     "run_program": (C.c_int, [_p, _i64, _i32p, _i64p, _i64p, _u64p, _u64p, _f64p, _i64, C.c_double, _i64]),
@@ -181,6 +184,20 @@ class CpuSum:
 
     def apply_CNOT(self, c, t):
         return self._rc(self._fn("apply_CNOT")(self.handle, int(c), int(t)), "apply_CNOT")
+
+    def apply_gate(self, gate, c, t=0):
+        """Any gate of the extended Clifford set by name ('X', 'CZ', 'iSWAP', ...)."""
+        from paulib_amd.gates import gate_id
+
+        return self._rc(self._fn("apply_gate")(self.handle, gate_id(gate), int(c), int(t)), f"apply_{gate}")
+
+    def apply_rot(self, axis, c, angle):
+        """apply_RX / apply_RY / apply_RZ (axis 'X', 'Y' or 'Z')."""
+        a = {"X": 1, "Y": 2, "Z": 3}[axis]
+        return self._rc(self._fn("apply_rot_axis")(self.handle, a, int(c), float(angle)), f"apply_R{axis}")
+
+    def elementwise_multiply(self, rhs):
+        return self._rc(self._fn("elementwise_multiply")(self.handle, rhs.handle), "elementwise_multiply")
 
     def evolve(self, rot_host):
         r = CpuSum.from_host(self.kind, rot_host)

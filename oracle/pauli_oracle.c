@@ -593,3 +593,137 @@ int orc_run_program(orc_sum *s, int64_t nops, const int32_t *kind, const int64_t
     }
     return 0;
 }
+
+/* ------------------------------------------------- extended Clifford set */
+
+/* Gate ids shared with the engine's plh_apply_clifford and ref_bridge.cpp:
+ *   0 I  1 X  2 Y  3 Z  4 H  5 S  6 H_YZ  7 H_XY  8 sqrt_X  9 sqrt_Y
+ *  10 CNOT 11 CZ 12 CY 13 XC_X 14 XC_Z 15 XC_Y 16 YC_X 17 YC_Z 18 YC_Y
+ *  19 SWAP 20 iSWAP
+ * Each case restates the reference's per-term bit update; the qubit bits of
+ * one term are read before any is written, so control and target may share
+ * a word (as in the reference, which takes no __restrict__ for them). */
+#define ORC_NGATES 21
+
+static int gate_is_two(int g) { return g >= 10; }
+
+static void gate_term(orc_sum *s, int g, int64_t i, int wc, uint64_t mc, int wt, uint64_t mt) {
+    const uint64_t xc = (X(s, wc, i) & mc) != 0, zc = (Z(s, wc, i) & mc) != 0;
+    uint64_t xt = 0, zt = 0;
+    if (gate_is_two(g)) { xt = (X(s, wt, i) & mt) != 0; zt = (Z(s, wt, i) & mt) != 0; }
+    uint64_t sg = 0;                 /* sign flip (flags bit 1) */
+    uint64_t dxc = 0, dzc = 0, dxt = 0, dzt = 0;  /* bits toggled */
+    switch (g) {
+        case 0: return;                                        /* apply_I :208-210 */
+        case 1: sg = zc; break;                                /* apply_X :162-174 */
+        case 2: sg = xc ^ zc; break;                           /* apply_Y :177-190 */
+        case 3: sg = xc; break;                                /* apply_Z :193-205 */
+        case 4: sg = xc & zc; dxc = xc ^ zc; dzc = xc ^ zc; break;   /* apply_H :44-62 (swap) */
+        case 5: sg = xc & zc; dzc = xc; break;                 /* apply_S :65-81 */
+        case 6: sg = xc & (1 ^ zc); dxc = zc; break;           /* apply_H_YZ :84-100 */
+        case 7: sg = (1 ^ xc) & zc; dzc = xc; break;           /* apply_H_XY :103-119 */
+        case 8: sg = (1 ^ xc) & zc; dxc = zc; break;           /* apply_sqrt_X :122-138 */
+        case 9: sg = xc & (1 ^ zc); dxc = xc ^ zc; dzc = xc ^ zc; break; /* apply_sqrt_Y :141-159 */
+        case 10: sg = xc & zt & (1 ^ xt ^ zc); dxt = xc; dzc = zt; break; /* apply_CNOT :213-239 */
+        case 11: sg = xc & xt & (zc ^ zt); dzt = xc; dzc = xt; break;     /* apply_CZ :242-266 */
+        case 12: /* apply_CY :269-294 */
+            sg = (xc & zc & zt & (1 ^ xt)) | (xc & xt & (1 ^ zc) & (1 ^ zt));
+            dzc = xt ^ zt; dxt = xc; dzt = xc; break;
+        case 13: /* apply_XC_X :297-321 */
+            sg = (xc & zc & zt & (1 ^ xt)) | (xt & zc & zt & (1 ^ xc));
+            dxc = zt; dxt = zc; break;
+        case 14: /* apply_XC_Z :324-348 */
+            sg = (xc & xt & zc & zt) | (xt & zc & (1 ^ xc) & (1 ^ zt));
+            dxc = xt; dzt = zc; break;
+        case 15: /* apply_XC_Y :351-376 */
+            sg = (xc & xt & zc & (1 ^ zt)) | (zc & zt & (1 ^ xc) & (1 ^ xt));
+            dxc = xt ^ zt; dxt = zc; dzt = zc; break;
+        case 16: /* apply_YC_X :379-404 */
+            sg = (xc & xt & zt & (1 ^ zc)) | (zc & zt & (1 ^ xc) & (1 ^ xt));
+            dxc = zt; dzc = zt; dxt = xc ^ zc; break;
+        case 17: /* apply_YC_Z :407-432 */
+            sg = (xt & zc & zt & (1 ^ xc)) | (xc & xt & (1 ^ zc) & (1 ^ zt));
+            dxc = xt; dzc = xt; dzt = xc ^ zc; break;
+        case 18: /* apply_YC_Y :435-462 */
+            sg = (xc & zt & (1 ^ xt) & (1 ^ zc)) | (xt & zc & (1 ^ xc) & (1 ^ zt));
+            dxc = xt ^ zt; dzc = xt ^ zt; dxt = xc ^ zc; dzt = xc ^ zc; break;
+        case 19: /* apply_SWAP :465-490 */
+            dxc = xc ^ xt; dxt = xc ^ xt; dzc = zc ^ zt; dzt = zc ^ zt; break;
+        default: return;
+    }
+    s->f[i] ^= (uint8_t)(sg << 1);
+    X(s, wc, i) ^= (0 - dxc) & mc;
+    Z(s, wc, i) ^= (0 - dzc) & mc;
+    if (gate_is_two(g)) {
+        X(s, wt, i) ^= (0 - dxt) & mt;
+        Z(s, wt, i) ^= (0 - dzt) & mt;
+    }
+}
+
+/* PauliSum::apply_<gate>, include/PauliSumClifford.h:44-498.  iSWAP is the
+ * reference's literal sequence SWAP, CZ, S(c), S(t) (:493-498).  Returns -2
+ * for an unknown gate, a qubit outside the words, or c == t. */
+int orc_apply_gate(orc_sum *s, int g, int64_t c, int64_t t) {
+    if (g < 0 || g >= ORC_NGATES || c < 0 || c >= 64 * (int64_t)s->as) return -2;
+    if (gate_is_two(g) && (t < 0 || t >= 64 * (int64_t)s->as || t == c)) return -2;
+    if (g == 20) {
+        int rc = orc_apply_gate(s, 19, c, t);
+        if (!rc) rc = orc_apply_gate(s, 11, c, t);
+        if (!rc) rc = orc_apply_gate(s, 5, c, 0);
+        if (!rc) rc = orc_apply_gate(s, 5, t, 0);
+        return rc;
+    }
+    const int wc = (int)(c / 64), wt = gate_is_two(g) ? (int)(t / 64) : 0;
+    const uint64_t mc = 1ull << (c % 64), mt = gate_is_two(g) ? 1ull << (t % 64) : 0;
+    for (int64_t i = 0; i < s->n; ++i) gate_term(s, g, i, wc, mc, wt, mt);
+    return 0;
+}
+
+/* PauliSum::apply_RZ / apply_RX / apply_RY, include/PauliSumOperations.h:
+ * 28-104 (axis 3 = Z, 1 = X, 2 = Y): half-angle cos/sin; each term that
+ * anticommutes with the axis Pauli at c keeps value*cos_half, and a copy
+ * with value*sin_half, left-multiplied by i*P_c (multiply_iZ / iX / iY,
+ * include/PauliSumClifford.h:15-40), is appended in term order; then
+ * operator+= (append + sort_and_combine). */
+int orc_apply_rot_axis(orc_sum *s, int axis, int64_t c, double angle) {
+    if (axis < 1 || axis > 3 || c < 0 || c >= 64 * (int64_t)s->as) return -2;
+    const double cos_half = cos(angle / 2), sin_half = sin(angle / 2);
+    const int w = (int)(c / 64);
+    const uint64_t m = 1ull << (c % 64);
+    orc_sum *nw = orc_sum_new(s->as, s->nq);
+    if (!nw) return -1;
+    uint64_t xw[16], zw[16];
+    for (int64_t i = 0; i < s->n; ++i) {
+        const uint64_t xc = (X(s, w, i) & m) != 0, zc = (Z(s, w, i) & m) != 0;
+        const uint64_t anti = axis == 3 ? xc : axis == 1 ? zc : (xc ^ zc);
+        if (!anti) continue;
+        const double t_val = s->v[i] * sin_half;
+        s->v[i] *= cos_half;
+        load_term(s, i, xw, zw);
+        uint8_t f = s->f[i];
+        if (axis == 1) { f ^= (uint8_t)(xc << 1); xw[w] ^= m; }            /* multiply_iX */
+        else if (axis == 3) { f ^= (uint8_t)((1 ^ zc) << 1); zw[w] ^= m; } /* multiply_iZ */
+        else { f ^= (uint8_t)((1 ^ xc) << 1); xw[w] ^= m; zw[w] ^= m; }    /* multiply_iY */
+        if (push(nw, xw, zw, f, t_val)) { orc_sum_free(nw); return -1; }
+    }
+    const int rc = orc_add(s, nw);
+    orc_sum_free(nw);
+    return rc;
+}
+
+/* PauliSum::elementwise_multiply, include/PauliSumOperations.h:220-250:
+ * term i <- term_i * rhs_i (phase as PauliString::multiply), value
+ * v_i * rhs.v_i.  The reference reads rhs[i] for every i < n; here a rhs of
+ * another length or word count is refused (-2). */
+int orc_elementwise_multiply(orc_sum *s, const orc_sum *rhs) {
+    if (s->as != rhs->as || s->n != rhs->n) return -2;
+    uint64_t ax[16], az[16], bx[16], bz[16], ox[16], oz[16];
+    for (int64_t i = 0; i < s->n; ++i) {
+        load_term(s, i, ax, az);
+        load_term(rhs, i, bx, bz);
+        s->f[i] = pair_product(s->as, ax, az, s->f[i], bx, bz, rhs->f[i], ox, oz);
+        for (int k = 0; k < s->as; ++k) { X(s, k, i) = ox[k]; Z(s, k, i) = oz[k]; }
+        s->v[i] *= rhs->v[i];
+    }
+    return 0;
+}

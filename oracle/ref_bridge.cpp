@@ -241,6 +241,61 @@ int ref_run_program(void* p, int64_t nops, const int32_t* kind, const int64_t* q
     });
 }
 
+// The extended gate set by the oracle's gate ids (0 I .. 20 iSWAP); the
+// argument checks are the oracle's (the reference checks none).
+int ref_apply_gate(void* p, int g, int64_t c, int64_t t) {
+    auto* b = static_cast<RefSumBase*>(p);
+    const int64_t nqb = 64 * (int64_t)b->as;
+    if (g < 0 || g > 20 || c < 0 || c >= nqb) return -2;
+    if (g >= 10 && (t < 0 || t >= nqb || t == c)) return -2;
+    const std::size_t q = (std::size_t)c, r = (std::size_t)t;
+    return dispatch(b, [&](auto& s) {
+        switch (g) {
+            case 0: s.apply_I(q); break;
+            case 1: s.apply_X(q); break;
+            case 2: s.apply_Y(q); break;
+            case 3: s.apply_Z(q); break;
+            case 4: s.apply_H(q); break;
+            case 5: s.apply_S(q); break;
+            case 6: s.apply_H_YZ(q); break;
+            case 7: s.apply_H_XY(q); break;
+            case 8: s.apply_sqrt_X(q); break;
+            case 9: s.apply_sqrt_Y(q); break;
+            case 10: s.apply_CNOT(q, r); break;
+            case 11: s.apply_CZ(q, r); break;
+            case 12: s.apply_CY(q, r); break;
+            case 13: s.apply_XC_X(q, r); break;
+            case 14: s.apply_XC_Z(q, r); break;
+            case 15: s.apply_XC_Y(q, r); break;
+            case 16: s.apply_YC_X(q, r); break;
+            case 17: s.apply_YC_Z(q, r); break;
+            case 18: s.apply_YC_Y(q, r); break;
+            case 19: s.apply_SWAP(q, r); break;
+            default: s.apply_iSWAP(q, r); break;
+        }
+        return 0;
+    });
+}
+
+// apply_RX / apply_RY / apply_RZ by axis (1 X, 2 Y, 3 Z).
+int ref_apply_rot_axis(void* p, int axis, int64_t c, double angle) {
+    auto* b = static_cast<RefSumBase*>(p);
+    if (axis < 1 || axis > 3 || c < 0 || c >= 64 * (int64_t)b->as) return -2;
+    return dispatch(b, [&](auto& s) {
+        if (axis == 1) s.apply_RX((std::size_t)c, angle);
+        else if (axis == 2) s.apply_RY((std::size_t)c, angle);
+        else s.apply_RZ((std::size_t)c, angle);
+        return 0;
+    });
+}
+
+int ref_elementwise_multiply(void* lhs, const void* rhs) {
+    auto* a = static_cast<RefSumBase*>(lhs);
+    auto* b = const_cast<RefSumBase*>(static_cast<const RefSumBase*>(rhs));
+    if (a->as != b->as || a->len() != b->len()) return -2;
+    return dispatch(a, [&](auto& s) { s.elementwise_multiply(same<std::decay_t<decltype(s)>>(b)); return 0; });
+}
+
 int ref_num_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -1,0 +1,221 @@
+"""The extended Clifford gate set, apply_RX/RY/RZ and elementwise_multiply in
+the oracle (and the compiled reference where built), pinned three ways:
+  * the reference tests' truth tables (tests/CliffordExtendedTests.cpp:282-518),
+  * dense-matrix conjugation U P U^dag of every gate on every 1- and 2-qubit
+    Pauli with every phase, and U^dag O U for the half-angle rotations,
+  * the compiled reference on seeded random sums (flags != 0, several words,
+    control and target in one word and in different words).
+A wrong sign term, a swapped control/target or a wrong angle fails one of them."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import assert_terms_equal, cpu, oracle_kinds
+from paulib_amd import workloads as W
+from paulib_amd.gates import GATES, SINGLE, TWO
+from paulib_amd.host import HostSum
+from test_oracle_math import I2, MAT, PX, PY, PZ, all_terms, sum_matrix, term_matrix
+
+KINDS = oracle_kinds()
+
+# tests/CliffordExtendedTests.cpp:282-378 (single qubit) and :378-518 (two qubit)
+KAT1 = {
+    "H_YZ": {"I": "I", "X": "-X", "Y": "Z", "Z": "Y"},
+    "H_XY": {"I": "I", "X": "Y", "Y": "X", "Z": "-Z"},
+    "sqrt_X": {"I": "I", "X": "X", "Y": "Z", "Z": "-Y"},
+    "sqrt_Y": {"I": "I", "X": "-Z", "Y": "Y", "Z": "X"},
+    "X": {"I": "I", "X": "X", "Y": "-Y", "Z": "-Z"},
+    "Y": {"I": "I", "X": "-X", "Y": "Y", "Z": "-Z"},
+    "Z": {"I": "I", "X": "-X", "Y": "-Y", "Z": "Z"},
+    "I": {"I": "I", "X": "X", "Y": "Y", "Z": "Z"},
+}
+
+
+def _two(rows):
+    keys = [a + b for a in "IXYZ" for b in "IXYZ"]
+    return dict(zip(keys, rows.split()))
+
+
+KAT2 = {
+    "CZ": _two("II ZX ZY IZ XZ YY -YX XI YZ -XY XX YI ZI IX IY ZZ"),
+    "CY": _two("II ZX IY ZZ XY -YZ XI YX YY XZ YI -XX ZI IX ZY IZ"),
+    "XC_X": _two("II IX XY XZ XI XX IY IZ YX YI ZZ -ZY ZX ZI -YZ YY"),
+    "XC_Z": _two("II XX XY IZ XI IX IY XZ YZ ZY -ZX YI ZZ -YY YX ZI"),
+    "XC_Y": _two("II XX IY XZ XI IX XY IZ YY -ZZ YI ZX ZY YZ ZI -YX"),
+    "YC_X": _two("II IX YY YZ XX XI -ZZ ZY YI YX IY IZ ZX ZI XZ -XY"),
+    "YC_Z": _two("II YX YY IZ XZ -ZY ZX XI YI IX IY YZ ZZ XY -XX ZI"),
+    "YC_Y": _two("II YX IY YZ XY ZZ XI -ZX YI IX YY IZ ZY -XZ ZI XX"),
+    "SWAP": _two("II XI YI ZI IX XX YX ZX IY XY YY ZY IZ XZ YZ ZZ"),
+}
+
+
+def S(terms, nq=None, as_=1):
+    return HostSum.from_terms(terms, array_size=as_, num_qubits=nq)
+
+
+def gate(kind, host, name, c, t=0):
+    s = cpu(kind, host)
+    s.apply_gate(name, c, t)
+    return s.to_host()
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_single_qubit_truth_tables(kind):
+    for name, table in KAT1.items():
+        for src, dst in table.items():
+            assert gate(kind, S([(src, 1.0)]), name, 0).get_str() == f"1.000000 * {dst}\n", (name, src)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_two_qubit_truth_tables(kind):
+    for name, table in KAT2.items():
+        for src, dst in table.items():
+            assert gate(kind, S([(src, 1.0)]), name, 0, 1).get_str() == f"1.000000 * {dst}\n", (name, src)
+    # :515-517: iSWAP on XI gives ZY
+    assert gate(kind, S([("XI", 1.0)]), "iSWAP", 0, 1).get_str() == "1.000000 * ZY\n"
+
+
+def _ctrl(P, Q):
+    """P-controlled Q: (II + P(x)I + I(x)Q - P(x)Q) / 2, control on qubit 0."""
+    return (np.kron(I2, I2) + np.kron(P, I2) + np.kron(I2, Q) - np.kron(P, Q)) / 2
+
+
+SWAP_M = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=complex)
+S_M = np.array([[1, 0], [0, 1j]], dtype=complex)
+UNITARY = {
+    "I": I2, "X": PX, "Y": PY, "Z": PZ,
+    "H": np.array([[1, 1], [1, -1]], dtype=complex) / math.sqrt(2),
+    "S": S_M,
+    "H_YZ": (PY + PZ) / math.sqrt(2), "H_XY": (PX + PY) / math.sqrt(2),
+    "sqrt_X": (I2 - 1j * PX) / math.sqrt(2), "sqrt_Y": (I2 - 1j * PY) / math.sqrt(2),
+    "CNOT": _ctrl(PZ, PX), "CZ": _ctrl(PZ, PZ), "CY": _ctrl(PZ, PY),
+    "XC_X": _ctrl(PX, PX), "XC_Z": _ctrl(PX, PZ), "XC_Y": _ctrl(PX, PY),
+    "YC_X": _ctrl(PY, PX), "YC_Z": _ctrl(PY, PZ), "YC_Y": _ctrl(PY, PY),
+    "SWAP": SWAP_M,
+    # iSWAP := SWAP, then CZ, then S on each qubit (PauliSumClifford.h:493-498)
+    "iSWAP": np.kron(I2, S_M) @ np.kron(S_M, I2) @ _ctrl(PZ, PZ) @ SWAP_M,
+}
+
+
+@pytest.mark.parametrize("name", GATES)
+def test_gates_are_clifford_conjugations(name):
+    """Every gate maps i^f P to U (i^f P) U^dag, for every 2-qubit P and phase f."""
+    two = name in TWO
+    U = UNITARY[name] if two else np.kron(UNITARY[name], I2)
+    for p in all_terms(2):
+        for f in range(4):
+            h = S([(p, 1.0)], nq=2)
+            h.flags[:] = f
+            out = gate("oracle", h, name, 0, 1)
+            m = term_matrix(h, 0)
+            np.testing.assert_allclose(term_matrix(out, 0), U @ m @ U.conj().T, atol=1e-12,
+                                       err_msg=f"{name} on {p} flags {f}")
+
+
+def _take(h, idx):
+    idx = np.asarray(idx)
+    return HostSum(h.array_size, h.num_qubits, h.x[:, idx].copy(), h.z[:, idx].copy(),
+                   h.flags[idx].copy(), h.value[idx].copy())
+
+
+def test_target_before_control_and_cross_word():
+    """A gate with control and target in different words, or in reversed
+    order, acts on each term as the same gate on a lone 2-qubit string, and
+    leaves every other qubit alone."""
+    for name in TWO:
+        h = W.random_words_sum(64, 128, 40 + TWO.index(name))
+        for c, t in [(5, 70), (70, 5), (63, 64), (9, 3)]:
+            got = gate("oracle", h, name, c, t)
+            for i in range(0, 64, 5):
+                src = h.term_string(i).lstrip("-i")
+                loc = src[c] + src[t]
+                want = gate("oracle", S([(loc, 1.0)], nq=2), name, 0, 1).term_string(0)
+                out = got.term_string(i).lstrip("-i")
+                assert out[c] + out[t] == want.lstrip("-"), (name, c, t, i)
+                rest = [q for q in range(128) if q not in (c, t)]
+                assert "".join(out[q] for q in rest) == "".join(src[q] for q in rest)
+                flip = int(want.startswith("-"))
+                assert int(got.flags[i]) == int(h.flags[i]) ^ (flip << 1), (name, c, t, i)
+
+
+def test_bad_gate_arguments_are_refused():
+    h = S([("XY", 1.0)])
+    s = cpu("oracle", h)
+    with pytest.raises(RuntimeError):
+        s.apply_gate("CZ", 1, 1)
+    with pytest.raises(RuntimeError):
+        s.apply_gate("H", 64, 0)
+
+
+def _expm(P, theta):
+    return math.cos(theta / 2) * np.eye(P.shape[0]) - 1j * math.sin(theta / 2) * P
+
+
+@pytest.mark.parametrize("angle", [0.0, 0.4, -1.3, math.pi, 2.5])
+@pytest.mark.parametrize("axis", ["X", "Y", "Z"])
+def test_half_angle_rotations_are_conjugations(axis, angle):
+    """apply_R<axis>(c, angle) is U^dag O U with U = exp(-i (angle/2)/2 P_c):
+    evolve's convention at half the angle (the new term is cos*Q + sin*(iP)Q)."""
+    O = W.random_words_sum(10, 3, 11)
+    O.flags[:] = 0
+    for c in range(3):
+        P = np.eye(1, dtype=complex)
+        for q in range(3):
+            P = np.kron(P, MAT[axis] if q == c else I2)
+        U = _expm(P, angle / 2)
+        s = cpu("oracle", O)
+        s.apply_rot(axis, c, angle)
+        np.testing.assert_allclose(sum_matrix(s.to_host()), U.conj().T @ sum_matrix(O) @ U, atol=1e-12)
+
+
+def test_rotation_keeps_zero_terms_and_signs():
+    """operator+= is sort_and_combine: at angle 0 every anticommuting term
+    gains a zero-valued partner, kept (no zero drop, no sign fold)."""
+    h = S([("XZ", 1.0), ("ZI", 2.0)], nq=2)
+    s = cpu("oracle", h)
+    s.apply_rot("Z", 0, 0.0)
+    out = s.to_host()
+    assert len(out) == 3 and (out.value == 0.0).sum() == 1
+
+
+def test_elementwise_multiply_against_products():
+    """Term i of A.elementwise_multiply(B) is the 1x1 product A_i * B_i."""
+    A = W.random_words_sum(40, 100, 5)
+    B = W.random_words_sum(40, 100, 6)
+    s = cpu("oracle", A)
+    s.elementwise_multiply(cpu("oracle", B))
+    got = s.to_host()
+    for i in range(40):
+        a = cpu("oracle", _take(A, [i]))
+        a.multiply(cpu("oracle", _take(B, [i])))
+        w = a.to_host()
+        assert (got.x[:, i] == w.x[:, 0]).all() and (got.z[:, i] == w.z[:, 0]).all()
+        assert got.flags[i] == w.flags[0] and got.value[i] == w.value[0]
+    with pytest.raises(RuntimeError):
+        cpu("oracle", A).elementwise_multiply(cpu("oracle", _take(B, list(range(39)))))
+
+
+@pytest.mark.skipif("reference" not in KINDS, reason="oracle/_ref not built")
+@pytest.mark.parametrize("as_", [1, 2])
+def test_against_reference_random_sums(as_):
+    rng = np.random.default_rng(as_)
+    nq = 64 * as_
+    h = W.random_words_sum(300, nq, 21)
+    pairs = [(3, 9), (9, 3), (0, 63)] + ([(5, 100), (100, 5), (63, 64)] if as_ == 2 else [])
+    for name in GATES:
+        qs = pairs if name in TWO else [(0, 0), (17, 0), (nq - 1, 0)]
+        for c, t in qs:
+            assert_terms_equal(gate("oracle", h, name, c, t), gate("reference", h, name, c, t))
+    for axis in "XYZ":
+        for c in [0, nq - 1, int(rng.integers(nq))]:
+            for angle in [0.0, 0.7, -2.2, math.pi]:
+                a, b = cpu("oracle", h), cpu("reference", h)
+                a.apply_rot(axis, c, angle)
+                b.apply_rot(axis, c, angle)
+                assert_terms_equal(a.to_host(), b.to_host(), values="tol")
+    B = W.random_words_sum(300, nq, 22)
+    a, b = cpu("oracle", h), cpu("reference", h)
+    a.elementwise_multiply(cpu("oracle", B))
+    b.elementwise_multiply(cpu("reference", B))
+    assert_terms_equal(a.to_host(), b.to_host())
