@@ -241,6 +241,38 @@ def batched_products(ctx, count: int, budget_pairs: int, solo_pairs: int):
     }
 
 
+def clifford_gates(ctx, log2_terms: int, reps: int):
+    """The table-driven Clifford kernel on a resident 64-qubit sum: per gate
+    kind, the kernel time from HIP events on the engine's stream and the
+    algorithmic bytes (touched words read and written, flags read and
+    written) against the HBM peak."""
+    from paulib_amd import PauliSum, _lib
+    from paulib_amd.host import HostSum
+
+    n = 1 << log2_terms
+    rng = np.random.default_rng(12)
+    x = rng.integers(0, 1 << 63, size=(1, n), dtype=np.uint64) << np.uint64(1)
+    z = rng.integers(0, 1 << 63, size=(1, n), dtype=np.uint64)
+    h = HostSum(1, 64, x, z, np.zeros(n, np.uint8), rng.uniform(-1, 1, n))
+    d = PauliSum.from_host(ctx, h)
+    out = {"workload": f"{n} terms on 64 qubits, resident", "terms": n}
+    for name, c, t in [("H", 5, 0), ("X", 7, 0), ("CZ", 3, 40), ("iSWAP", 12, 50)]:
+        d.apply_gate(name, c, t)
+        ctx.synchronize()
+        _lib.call("plh_ctx_profile", ctx.handle, 1)
+        for _ in range(reps):
+            d.apply_gate(name, c, t)
+        ctx.synchronize()
+        prof = kernel_profile(ctx)["clifford_table"]
+        _lib.call("plh_ctx_profile", ctx.handle, 0)
+        ms = prof["ms"] / prof["launches"]
+        gbs = prof["bytes"] / prof["launches"] / (ms / 1e3) / 1e9
+        out[name] = {"ms": ms, "GB/s": gbs, "frac": gbs * 1e9 / HBM_PEAK}
+        log(f"gate {name}: {ms:.3f} ms per launch, {gbs:.0f} GB/s")
+    d.close()
+    return out
+
+
 def cpu_baseline(A, B, target_s: float):
     """Reference multiply + canonicalize on a rhs slice sized for ~target_s."""
     import oracle
@@ -292,6 +324,7 @@ def main():
     ap.add_argument("--batch-pairs", type=int, default=1 << 31)
     ap.add_argument("--solo-pairs", type=int, default=0)  # 0: the context's tuned threshold
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--gate-log2-terms", type=int, default=26)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-propagation", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -398,6 +431,8 @@ def main():
                                  "mode": "sequential jobs, plain drop-in", "jobs": len(lat),
                                  "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                                  "jobs_per_s": float(1e3 / lat.mean()), "expectation": e}
+    if args.gate_log2_terms > 0:
+        result["clifford_gates"] = clifford_gates(ctx, args.gate_log2_terms, 20)
     if args.batch_jobs > 0:
         result["batched_products"] = batched_products(ctx, args.batch_jobs, args.batch_pairs, args.solo_pairs)
     if args.serve_jobs > 0:

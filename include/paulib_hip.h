@@ -172,6 +172,78 @@ int plh_cutoff(plh_sum* s, double eps);
 int plh_apply_H(plh_sum* s, int64_t q);
 int plh_apply_S(plh_sum* s, int64_t q);
 int plh_apply_CNOT(plh_sum* s, int64_t control, int64_t target);
+/* The whole gate set of include/PauliSumClifford.h (:44-498), one
+ * table-driven launch per gate.  Gate ids for plh_apply_clifford:
+ *   0 I  1 X  2 Y  3 Z  4 H  5 S  6 H_YZ  7 H_XY  8 sqrt_X  9 sqrt_Y
+ *  10 CNOT 11 CZ 12 CY 13 XC_X 14 XC_Z 15 XC_Y 16 YC_X 17 YC_Z 18 YC_Y
+ *  19 SWAP 20 iSWAP (= SWAP, CZ, S(c), S(t) as :493-498, one launch).
+ * q1 is ignored for ids 0-9.  A qubit outside the sum's words, or the same
+ * qubit as control and target, is PLH_ERR_ARG (the reference checks
+ * neither).  The named entry points below replace PauliSum::apply_<name>
+ * one for one (apply_H/S/CNOT are declared above). */
+#define PLH_GATE_I 0
+#define PLH_GATE_X 1
+#define PLH_GATE_Y 2
+#define PLH_GATE_Z 3
+#define PLH_GATE_H 4
+#define PLH_GATE_S 5
+#define PLH_GATE_H_YZ 6
+#define PLH_GATE_H_XY 7
+#define PLH_GATE_SQRT_X 8
+#define PLH_GATE_SQRT_Y 9
+#define PLH_GATE_CNOT 10
+#define PLH_GATE_CZ 11
+#define PLH_GATE_CY 12
+#define PLH_GATE_XC_X 13
+#define PLH_GATE_XC_Z 14
+#define PLH_GATE_XC_Y 15
+#define PLH_GATE_YC_X 16
+#define PLH_GATE_YC_Z 17
+#define PLH_GATE_YC_Y 18
+#define PLH_GATE_SWAP 19
+#define PLH_GATE_ISWAP 20
+int plh_apply_clifford(plh_sum* s, int gate, int64_t q0, int64_t q1);
+/* apply_I :208-210, apply_X :162-174, apply_Y :177-190, apply_Z :193-205,
+ * apply_H_YZ :84-100, apply_H_XY :103-119, apply_sqrt_X :122-138,
+ * apply_sqrt_Y :141-159. */
+int plh_apply_I(plh_sum* s, int64_t q);
+int plh_apply_X(plh_sum* s, int64_t q);
+int plh_apply_Y(plh_sum* s, int64_t q);
+int plh_apply_Z(plh_sum* s, int64_t q);
+int plh_apply_H_YZ(plh_sum* s, int64_t q);
+int plh_apply_H_XY(plh_sum* s, int64_t q);
+int plh_apply_sqrt_X(plh_sum* s, int64_t q);
+int plh_apply_sqrt_Y(plh_sum* s, int64_t q);
+/* apply_CZ :242-266, apply_CY :269-294, apply_XC_X :297-321,
+ * apply_XC_Z :324-348, apply_XC_Y :351-376, apply_YC_X :379-404,
+ * apply_YC_Z :407-432, apply_YC_Y :435-462, apply_SWAP :465-490,
+ * apply_iSWAP :493-498 (c = control / first qubit, t = target). */
+int plh_apply_CZ(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_CY(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_XC_X(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_XC_Z(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_XC_Y(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_YC_X(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_YC_Z(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_YC_Y(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_SWAP(plh_sum* s, int64_t c, int64_t t);
+int plh_apply_iSWAP(plh_sum* s, int64_t c, int64_t t);
+/* Host-only, no GPU needed (no counterpart in the reference): the gate's
+ * 16-entry table, new local bits xc | zc<<1 | xt<<2 | zt<<3 as nibble b of
+ * *table for input bits b, and the sign flip as bit b of *sign.  For tests
+ * and tools that check the gate set without a device. */
+int plh_clifford_table(int gate, uint64_t* table, uint32_t* sign);
+/* PauliSum::apply_RX / apply_RY / apply_RZ (include/PauliSumOperations.h:
+ * 28-104): half-angle cos/sin (host libm), the new terms (iP_q) Q appended
+ * in term order, then sort_and_combine; no fast paths, zeros kept. */
+int plh_apply_RX(plh_sum* s, int64_t q, double angle);
+int plh_apply_RY(plh_sum* s, int64_t q, double angle);
+int plh_apply_RZ(plh_sum* s, int64_t q, double angle);
+/* PauliSum::elementwise_multiply (include/PauliSumOperations.h:220-250):
+ * term i <- term_i * rhs_i, value v_i * rhs.v_i.  rhs must have the same
+ * length and word count (PLH_ERR_ARG otherwise; the reference reads rhs[i]
+ * unchecked) and the same context; rhs may be s itself. */
+int plh_elementwise_multiply(plh_sum* s, const plh_sum* rhs);
 /* PauliSum::evolve (include/PauliSumOperations.h:106-183): rotation terms
  * r = 0..nrot-1 in order, Pauli words px/pz[r*as + k], flags pflags[r],
  * angle theta[r]; cos/sin are taken on the host (libm). */

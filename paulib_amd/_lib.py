@@ -81,6 +81,15 @@ SIGNATURES = {
     "plh_apply_H": (_int, [_p, _i64]),
     "plh_apply_S": (_int, [_p, _i64]),
     "plh_apply_CNOT": (_int, [_p, _i64, _i64]),
+    "plh_apply_clifford": (_int, [_p, C.c_int, _i64, _i64]),
+    **{f"plh_apply_{g}": (_int, [_p, _i64]) for g in ("I", "X", "Y", "Z", "H_YZ", "H_XY", "sqrt_X", "sqrt_Y")},
+    **{f"plh_apply_{g}": (_int, [_p, _i64, _i64])
+       for g in ("CZ", "CY", "XC_X", "XC_Z", "XC_Y", "YC_X", "YC_Z", "YC_Y", "SWAP", "iSWAP")},
+    "plh_clifford_table": (_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "plh_apply_RX": (_int, [_p, _i64, C.c_double]),
+    "plh_apply_RY": (_int, [_p, _i64, C.c_double]),
+    "plh_apply_RZ": (_int, [_p, _i64, C.c_double]),
+    "plh_elementwise_multiply": (_int, [_p, _p]),
     "plh_evolve": (_int, [_p, _i64, _u64p, _u64p, _u8p, _f64p]),
     "plh_apply_truncation": (_int, [_p, _i64, C.c_double, _i64]),
     "plh_trim": (_int, [_p, _i64]),

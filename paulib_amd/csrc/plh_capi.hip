@@ -752,6 +752,88 @@ int plh_apply_CNOT(plh_sum* s, int64_t control, int64_t target) {
     return plh_ctx_synchronize(s->ctx);
 }
 
+int plh_apply_clifford(plh_sum* s, int gate, int64_t q0, int64_t q1) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(op_clifford_table(s->ctx, s, gate, q0, q1));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+#define PLH_GATE1(NAME, ID) \
+    int plh_apply_##NAME(plh_sum* s, int64_t q) { return plh_apply_clifford(s, ID, q, 0); }
+#define PLH_GATE2(NAME, ID) \
+    int plh_apply_##NAME(plh_sum* s, int64_t c, int64_t t) { return plh_apply_clifford(s, ID, c, t); }
+PLH_GATE1(I, 0)
+PLH_GATE1(X, 1)
+PLH_GATE1(Y, 2)
+PLH_GATE1(Z, 3)
+PLH_GATE1(H_YZ, 6)
+PLH_GATE1(H_XY, 7)
+PLH_GATE1(sqrt_X, 8)
+PLH_GATE1(sqrt_Y, 9)
+PLH_GATE2(CZ, 11)
+PLH_GATE2(CY, 12)
+PLH_GATE2(XC_X, 13)
+PLH_GATE2(XC_Z, 14)
+PLH_GATE2(XC_Y, 15)
+PLH_GATE2(YC_X, 16)
+PLH_GATE2(YC_Z, 17)
+PLH_GATE2(YC_Y, 18)
+PLH_GATE2(SWAP, 19)
+PLH_GATE2(iSWAP, 20)
+#undef PLH_GATE1
+#undef PLH_GATE2
+
+int plh_clifford_table(int gate, uint64_t* table, uint32_t* sign) {
+    if (!table || !sign) return PLH_ERR_ARG;
+    const int rc = clifford_table_of(gate, table, sign);
+    if (rc) set_error("unknown Clifford gate id");
+    return rc;
+}
+
+// apply_RX / apply_RY / apply_RZ (include/PauliSumOperations.h:28-104):
+// every term Q that anticommutes with P_q keeps cos(angle/2)*v and gains
+// (iP_q) Q with sin(angle/2)*v, appended in term order, then
+// sort_and_combine.  For such Q, (iP) Q = Q (-iP), so this is the fused
+// rotation collation with ps = -iP (flags 3) and s = -sin(angle/2): its new
+// coefficient (-s) * v is sin(angle/2) * v exactly.  There are no fast
+// paths: at angle 0 the zero-valued new terms are kept, as in the reference.
+static int rotation_axis(plh_sum* s, int axis, int64_t q, double angle) {
+    PLH_TRY(check_sum(s));
+    if (q < 0 || q >= 64 * (int64_t)s->as) {
+        set_error("qubit index outside the sum's words");
+        return PLH_ERR_ARG;
+    }
+    if (s->has_job) {
+        set_error("apply_R*: batched sums are not supported");
+        return PLH_ERR_ARG;
+    }
+    if (s->n == 0) return plh_ctx_synchronize(s->ctx);
+    uint64_t px[16] = {0}, pz[16] = {0};
+    const uint64_t m = 1ull << (q % 64);
+    if (axis != 3) px[q / 64] = m;  // X or Y
+    if (axis != 1) pz[q / 64] = m;  // Z or Y
+    const double ch = std::cos(angle / 2), sh = std::sin(angle / 2);
+    CollateSpec sc;
+    PLH_TRY(collate_rotation(s->ctx, s, px, pz, 3u, ch, -sh, sc, s));
+    s->canonical = false;
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_apply_RX(plh_sum* s, int64_t q, double angle) { return rotation_axis(s, 1, q, angle); }
+int plh_apply_RY(plh_sum* s, int64_t q, double angle) { return rotation_axis(s, 2, q, angle); }
+int plh_apply_RZ(plh_sum* s, int64_t q, double angle) { return rotation_axis(s, 3, q, angle); }
+
+int plh_elementwise_multiply(plh_sum* s, const plh_sum* rhs) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(check_sum(rhs));
+    if (rhs->ctx != s->ctx) {
+        set_error("elementwise_multiply: both sums must belong to one context");
+        return PLH_ERR_ARG;
+    }
+    PLH_TRY(op_elementwise(s->ctx, s, rhs));
+    return plh_ctx_synchronize(s->ctx);
+}
+
 int plh_evolve(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz, const uint8_t* pflags,
                const double* theta) {
     PLH_TRY(check_sum(s));

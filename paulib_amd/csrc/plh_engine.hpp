@@ -293,6 +293,13 @@ int op_mul_raw(int as, const uint64_t* lx, const uint64_t* lz, const uint8_t* lf
                uint8_t* of, double* ov, int64_t ostride, int64_t n, int64_t m, hipStream_t stream);
 int op_commutator_raw(plh_ctx* ctx, const plh_sum* a, const plh_sum* b, int by_i, plh_sum* out);
 int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1);
+// The whole gate set of include/PauliSumClifford.h by id (0 I .. 9 sqrt_Y
+// single-qubit, 10 CNOT .. 20 iSWAP two-qubit), one table-driven launch.
+constexpr int kNumGates = 21;
+constexpr int kFirstTwoQubitGate = 10;
+int op_clifford_table(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1);
+int clifford_table_of(int gate, uint64_t* tbl, uint32_t* sgn);
+int op_elementwise(plh_ctx* ctx, plh_sum* s, const plh_sum* rhs);
 int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pflags,
                 double cos_t, double sin_t, int* appended_general);
 // evolve's path for (cos, sin) (include/PauliSumOperations.h:121-180): 0 no

@@ -167,6 +167,73 @@ __global__ void k_clifford(int gate, uint64_t* __restrict__ x, uint64_t* __restr
     }
 }
 
+// ------------------------------------------------ table-driven Clifford
+// Every gate of include/PauliSumClifford.h acts on a term only through the
+// bits of its one or two qubits, b = xc | zc<<1 | xt<<2 | zt<<3.  A 16-entry
+// table gives the new local bits (one nibble per entry, `tbl`) and whether
+// the sign flips (bit b of `sgn`).  A wave reads 64 consecutive words of each
+// touched plane; the control and target words are loaded once when they are
+// the same word.  Gates that only flip signs (X, Y, Z) write only flags.
+__global__ void k_clifford_table(uint64_t* __restrict__ x, uint64_t* __restrict__ z,
+                                 uint8_t* __restrict__ f, int64_t stride, int64_t n, int wc,
+                                 uint64_t mc, int wt, uint64_t mt, uint64_t tbl, uint32_t sgn,
+                                 int two, int writes_bits) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t oc = (int64_t)wc * stride + i, ot = (int64_t)wt * stride + i;
+    uint64_t xcw = x[oc], zcw = z[oc];
+    uint32_t b = (uint32_t)((xcw & mc) != 0) | ((uint32_t)((zcw & mc) != 0) << 1);
+    const bool same = two && wt == wc;
+    uint64_t xtw = xcw, ztw = zcw;
+    if (two) {
+        if (!same) {
+            xtw = x[ot];
+            ztw = z[ot];
+        }
+        b |= ((uint32_t)((xtw & mt) != 0) << 2) | ((uint32_t)((ztw & mt) != 0) << 3);
+    }
+    if ((sgn >> b) & 1u) f[i] ^= (uint8_t)2;
+    if (!writes_bits) return;
+    const uint32_t e = (uint32_t)(tbl >> (4 * b)) & 15u;
+    xcw = (xcw & ~mc) | ((e & 1u) ? mc : 0ull);
+    zcw = (zcw & ~mc) | ((e & 2u) ? mc : 0ull);
+    if (two) {
+        if (same) {
+            xcw = (xcw & ~mt) | ((e & 4u) ? mt : 0ull);
+            zcw = (zcw & ~mt) | ((e & 8u) ? mt : 0ull);
+        } else {
+            x[ot] = (xtw & ~mt) | ((e & 4u) ? mt : 0ull);
+            z[ot] = (ztw & ~mt) | ((e & 8u) ? mt : 0ull);
+        }
+    }
+    x[oc] = xcw;
+    z[oc] = zcw;
+}
+
+// elementwise_multiply: term i <- term_i * rhs_i with the pair product's
+// phase (include/PauliSumOperations.h:220-250).  Each thread reads both
+// terms' words before writing its own, so rhs may be the sum itself.
+template <int AS>
+__global__ void k_elementwise(int as, uint64_t* x, uint64_t* z, uint8_t* f, double* v, int64_t stride,
+                              const uint64_t* bx, const uint64_t* bz, const uint8_t* bf, const double* bv,
+                              int64_t bstride, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t c1 = 0, c2 = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        if (k < as) {
+            const uint64_t ax = x[(int64_t)k * stride + i], az = z[(int64_t)k * stride + i];
+            const uint64_t rx = bx[(int64_t)k * bstride + i], rz = bz[(int64_t)k * bstride + i];
+            phase_word(ax, az, rx, rz, c1, c2);
+            x[(int64_t)k * stride + i] = ax ^ rx;
+            z[(int64_t)k * stride + i] = az ^ rz;
+        }
+    }
+    f[i] = (uint8_t)phase_of(f[i], bf[i], c1, c2);
+    v[i] = v[i] * bv[i];
+}
+
 // -------------------------------------------------------------- rotation
 struct Rot {
     uint64_t px[16];
@@ -452,6 +519,175 @@ int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1) {
     k_clifford<<<nblk((uint64_t)s->n, kBlock), kBlock, 0, ctx->stream>>>(gate, s->x, s->z, s->f, s->cap,
                                                                           s->n, wc, mc, wt, mt);
     PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
+// Clifford tables.  A local Pauli on (c, t) is i^k * P(b), b = xc | zc<<1 |
+// xt<<2 | zt<<3, with bits (1,1) on a qubit meaning Y (the flags convention
+// of include/PauliStringCore.h:34-35).  A gate is given by its images of the
+// generators Xc, Zc (and Xt, Zt); conjugation is multiplicative, so the image
+// of i^(xc&zc + xt&zt) Xc^xc Zc^zc Xt^xt Zt^zt is the ordered product of the
+// images, and its phase is 0 or 2 (the sign flip).
+namespace {
+struct Loc {
+    uint32_t b, k;
+};
+
+// i^k of sigma_a sigma_b on one qubit, codes x | z<<1 (1 X, 2 Z, 3 Y):
+// XY = iZ, YZ = iX, ZX = iY, and the reversed products take -i.
+uint32_t site_phase(uint32_t a, uint32_t b) {
+    if (a == 0 || b == 0 || a == b) return 0;
+    const bool cyclic = (a == 1 && b == 3) || (a == 3 && b == 2) || (a == 2 && b == 1);
+    return cyclic ? 1u : 3u;
+}
+
+Loc loc_mul(Loc a, Loc b) {
+    uint32_t k = a.k + b.k;
+    for (int q = 0; q < 2; ++q) k += site_phase((a.b >> (2 * q)) & 3u, (b.b >> (2 * q)) & 3u);
+    return Loc{a.b ^ b.b, k & 3u};
+}
+
+// "-ZY": optional sign, then the Pauli on c and (for two-qubit images) on t.
+Loc loc_parse(const char* s) {
+    Loc r{0, 0};
+    if (*s == '-') {
+        r.k = 2;
+        ++s;
+    }
+    for (int q = 0; s[q]; ++q) {
+        const char ch = s[q];
+        const uint32_t code = ch == 'X' ? 1u : ch == 'Z' ? 2u : ch == 'Y' ? 3u : 0u;
+        r.b |= code << (2 * q);
+    }
+    return r;
+}
+
+struct CliffTable {
+    uint64_t tbl;
+    uint32_t sgn;
+};
+
+CliffTable table_from_images(const char* const img[4]) {
+    Loc g[4];
+    for (int j = 0; j < 4; ++j) g[j] = loc_parse(img[j]);
+    CliffTable t{0, 0};
+    for (uint32_t b = 0; b < 16; ++b) {
+        Loc acc{0, ((b & 1u) & (b >> 1 & 1u)) + ((b >> 2 & 1u) & (b >> 3 & 1u))};
+        for (int j = 0; j < 4; ++j)
+            if (b >> j & 1u) acc = loc_mul(acc, g[j]);
+        t.tbl |= (uint64_t)acc.b << (4 * b);
+        t.sgn |= (acc.k >> 1 & 1u) << b;
+    }
+    return t;
+}
+
+// Applies a, then b.
+CliffTable table_then(CliffTable a, CliffTable b) {
+    CliffTable t{0, 0};
+    for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t m = (uint32_t)(a.tbl >> (4 * i)) & 15u;
+        t.tbl |= (uint64_t)((b.tbl >> (4 * m)) & 15u) << (4 * i);
+        t.sgn |= (((a.sgn >> i) ^ (b.sgn >> m)) & 1u) << i;
+    }
+    return t;
+}
+
+// Images of Xc, Zc, Xt, Zt (qubit c first) for the gate ids of
+// plh_apply_clifford, read off the per-term updates of
+// include/PauliSumClifford.h:44-490.
+const char* const kImages[20][4] = {
+    {"X", "Z", "IX", "IZ"},     // I
+    {"X", "-Z", "IX", "IZ"},    // X
+    {"-X", "-Z", "IX", "IZ"},   // Y
+    {"-X", "Z", "IX", "IZ"},    // Z
+    {"Z", "X", "IX", "IZ"},     // H
+    {"Y", "Z", "IX", "IZ"},     // S
+    {"-X", "Y", "IX", "IZ"},    // H_YZ
+    {"Y", "-Z", "IX", "IZ"},    // H_XY
+    {"X", "-Y", "IX", "IZ"},    // sqrt_X
+    {"-Z", "X", "IX", "IZ"},    // sqrt_Y
+    {"XX", "ZI", "IX", "ZZ"},   // CNOT
+    {"XZ", "ZI", "ZX", "IZ"},   // CZ
+    {"XY", "ZI", "ZX", "ZZ"},   // CY
+    {"XI", "ZX", "IX", "XZ"},   // XC_X
+    {"XI", "ZZ", "XX", "IZ"},   // XC_Z
+    {"XI", "ZY", "XX", "XZ"},   // XC_Y
+    {"XX", "ZX", "IX", "YZ"},   // YC_X
+    {"XZ", "ZZ", "YX", "IZ"},   // YC_Z
+    {"XY", "ZY", "YX", "YZ"},   // YC_Y
+    {"IX", "IZ", "XI", "ZI"},   // SWAP
+};
+
+// gate 20, iSWAP: SWAP, then CZ, then S on c, then S on t
+// (include/PauliSumClifford.h:493-498), composed into one table.
+CliffTable clifford_table(int gate) {
+    if (gate < 20) return table_from_images(kImages[gate]);
+    static const char* const s_t[4] = {"X", "Z", "IY", "IZ"};
+    CliffTable t = table_then(table_from_images(kImages[19]), table_from_images(kImages[11]));
+    t = table_then(t, table_from_images(kImages[5]));
+    return table_then(t, table_from_images(s_t));
+}
+}  // namespace
+
+int clifford_table_of(int gate, uint64_t* tbl, uint32_t* sgn) {
+    if (gate < 0 || gate >= kNumGates) return PLH_ERR_ARG;
+    const CliffTable t = clifford_table(gate);
+    *tbl = t.tbl;
+    *sgn = t.sgn;
+    return PLH_OK;
+}
+
+int op_clifford_table(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1) {
+    if (gate < 0 || gate >= kNumGates) {
+        set_error("unknown Clifford gate id");
+        return PLH_ERR_ARG;
+    }
+    const bool two = gate >= kFirstTwoQubitGate;
+    const int64_t nqb = 64 * (int64_t)s->as;
+    if (q0 < 0 || q0 >= nqb || (two && (q1 < 0 || q1 >= nqb))) {
+        set_error("qubit index outside the sum's words");
+        return PLH_ERR_ARG;
+    }
+    if (two && q0 == q1) {
+        set_error("a two-qubit gate needs two different qubits");
+        return PLH_ERR_ARG;
+    }
+    if (gate == 0 || s->n == 0) return PLH_OK;  // apply_I is a no-op
+    const CliffTable t = clifford_table(gate);
+    const bool sign_only = gate >= 1 && gate <= 3;
+    const int wc = (int)(q0 / 64), wt = two ? (int)(q1 / 64) : wc;
+    const uint64_t mc = 1ull << (q0 % 64), mt = two ? 1ull << (q1 % 64) : 0ull;
+    // algorithmic bytes: the touched words read (and written unless the gate
+    // only flips signs), the flags read and written
+    const double words = (two && wt != wc) ? 4.0 : 2.0;
+    ProfScope ps(ctx, "clifford_table", (double)s->n * (words * 8.0 * (sign_only ? 1.0 : 2.0) + 2.0));
+    k_clifford_table<<<nblk((uint64_t)s->n, kBlock), kBlock, 0, ctx->stream>>>(
+        s->x, s->z, s->f, s->cap, s->n, wc, mc, wt, mt, t.tbl, t.sgn, two ? 1 : 0, sign_only ? 0 : 1);
+    PLH_HIP(hipGetLastError());
+    s->canonical = false;
+    return PLH_OK;
+}
+
+int op_elementwise(plh_ctx* ctx, plh_sum* s, const plh_sum* r) {
+    if (r->as != s->as || r->n != s->n || r->has_job || s->has_job) {
+        set_error("elementwise_multiply needs two sums of the same length and word count");
+        return PLH_ERR_ARG;
+    }
+    if (s->n == 0) return PLH_OK;
+    const int pa = padded(s->as);
+    if (pa < 0) return PLH_ERR_AS;
+    hipStream_t st = ctx->stream;
+    // both terms read (words, flags, value), the left one written
+    ProfScope ps(ctx, "elementwise", (double)s->n * 3.0 * (16.0 * s->as + 9.0));
+#define PLH_EW(ASV)                                                                                       \
+    case ASV:                                                                                             \
+        k_elementwise<ASV><<<nblk((uint64_t)s->n, kBlock), kBlock, 0, st>>>(                              \
+            s->as, s->x, s->z, s->f, s->v, s->cap, r->x, r->z, r->f, r->v, r->cap, s->n);                 \
+        break;
+    switch (pa) { PLH_EW(1) PLH_EW(2) PLH_EW(4) PLH_EW(8) PLH_EW(16) }
+#undef PLH_EW
+    PLH_HIP(hipGetLastError());
+    s->canonical = false;
     return PLH_OK;
 }
 

@@ -252,6 +252,29 @@ class PauliSum:
         _lib.call("plh_apply_CNOT", self._h, int(c), int(t))
         return self
 
+    def apply_gate(self, gate, c: int, t: int = 0):
+        """Any gate of include/PauliSumClifford.h by name ('X', 'CZ', 'iSWAP', ...) or id."""
+        from .gates import gate_id
+
+        _lib.call("plh_apply_clifford", self._h, gate_id(gate), int(c), int(t))
+        return self
+
+    def apply_RX(self, c: int, angle: float):
+        _lib.call("plh_apply_RX", self._h, int(c), float(angle))
+        return self
+
+    def apply_RY(self, c: int, angle: float):
+        _lib.call("plh_apply_RY", self._h, int(c), float(angle))
+        return self
+
+    def apply_RZ(self, c: int, angle: float):
+        _lib.call("plh_apply_RZ", self._h, int(c), float(angle))
+        return self
+
+    def elementwise_multiply(self, rhs: "PauliSum"):
+        _lib.call("plh_elementwise_multiply", self._h, rhs._h)
+        return self
+
     def evolve(self, rotations: HostSum):
         """PauliSum::evolve: each rotation term in order; its coefficient is the angle."""
         px = np.ascontiguousarray(rotations.x.T)
@@ -352,3 +375,23 @@ def run_program(O: PauliSum, prog, opts: TruncationOptions | None = None):
     _lib.call("plh_run_program", O._h, len(prog.kind), _i32(prog.kind), _i64(prog.q0), _i64(prog.q1),
               _u64(prog.px), _u64(prog.pz), _f64(prog.theta), *opts.args())
     return O
+
+
+def _named_gate(name, two):
+    if two:
+        def f(self, c: int, t: int):
+            _lib.call(f"plh_apply_{name}", self._h, int(c), int(t))
+            return self
+    else:
+        def f(self, c: int):
+            _lib.call(f"plh_apply_{name}", self._h, int(c))
+            return self
+    f.__name__ = f"apply_{name}"
+    f.__doc__ = f"PauliSum::apply_{name} (include/PauliSumClifford.h)."
+    return f
+
+
+for _g in ("I", "X", "Y", "Z", "H_YZ", "H_XY", "sqrt_X", "sqrt_Y"):
+    setattr(PauliSum, f"apply_{_g}", _named_gate(_g, False))
+for _g in ("CZ", "CY", "XC_X", "XC_Z", "XC_Y", "YC_X", "YC_Z", "YC_Y", "SWAP", "iSWAP"):
+    setattr(PauliSum, f"apply_{_g}", _named_gate(_g, True))

@@ -19,7 +19,9 @@ def test_header_declares_the_expected_surface():
     syms = declared_symbols()
     for must in ["plh_launch_mul_soa", "plh_multiply", "plh_multiply_canonical", "plh_commutator",
                  "plh_canonicalize", "plh_sort_and_combine", "plh_evolve", "plh_propagate",
-                 "plh_apply_H", "plh_apply_S", "plh_apply_CNOT", "plh_zero_state_expectation"]:
+                 "plh_apply_H", "plh_apply_S", "plh_apply_CNOT", "plh_zero_state_expectation",
+                 "plh_apply_clifford", "plh_apply_iSWAP", "plh_apply_YC_Y", "plh_apply_sqrt_X",
+                 "plh_apply_RX", "plh_apply_RY", "plh_apply_RZ", "plh_elementwise_multiply"]:
         assert must in syms
 
 
