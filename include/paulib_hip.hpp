@@ -11,7 +11,8 @@
 //     std::runtime_error like PAULIB_GPU_CK (:25-30).
 //   * the PauliSum member operations of the path as DevicePauliSum methods
 //     with the reference's names (multiply, sort_and_combine, canonicalize,
-//     evolve, apply_H/S/CNOT, truncate_weight, cutoff,
+//     evolve, apply_H/S/CNOT and every other gate of PauliSumClifford.h,
+//     apply_RX/RY/RZ, elementwise_multiply, truncate_weight, cutoff,
 //     zero_state_expectation) and the free functions commutator,
 //     i_commutator, apply_truncation, propagate, propagate_expectation
 //     (include/PauliSumOperations.h:348-356, include/PauliPropagation.h).
@@ -175,6 +176,41 @@ class DevicePauliSum {
     void apply_H(std::size_t c) { check(plh_apply_H(h_, (int64_t)c), "plh_apply_H"); }
     void apply_S(std::size_t c) { check(plh_apply_S(h_, (int64_t)c), "plh_apply_S"); }
     void apply_CNOT(std::size_t c, std::size_t t) { check(plh_apply_CNOT(h_, (int64_t)c, (int64_t)t), "plh_apply_CNOT"); }
+    // The rest of include/PauliSumClifford.h (:84-498), one launch each.
+#define PLH_MIRROR_G1(NAME) \
+    void apply_##NAME(std::size_t c) { check(plh_apply_##NAME(h_, (int64_t)c), "plh_apply_" #NAME); }
+#define PLH_MIRROR_G2(NAME)                                                              \
+    void apply_##NAME(std::size_t c, std::size_t t) {                                    \
+        check(plh_apply_##NAME(h_, (int64_t)c, (int64_t)t), "plh_apply_" #NAME);         \
+    }
+    PLH_MIRROR_G1(I)
+    PLH_MIRROR_G1(X)
+    PLH_MIRROR_G1(Y)
+    PLH_MIRROR_G1(Z)
+    PLH_MIRROR_G1(H_YZ)
+    PLH_MIRROR_G1(H_XY)
+    PLH_MIRROR_G1(sqrt_X)
+    PLH_MIRROR_G1(sqrt_Y)
+    PLH_MIRROR_G2(CZ)
+    PLH_MIRROR_G2(CY)
+    PLH_MIRROR_G2(XC_X)
+    PLH_MIRROR_G2(XC_Z)
+    PLH_MIRROR_G2(XC_Y)
+    PLH_MIRROR_G2(YC_X)
+    PLH_MIRROR_G2(YC_Z)
+    PLH_MIRROR_G2(YC_Y)
+    PLH_MIRROR_G2(SWAP)
+    PLH_MIRROR_G2(iSWAP)
+#undef PLH_MIRROR_G1
+#undef PLH_MIRROR_G2
+    // apply_RX/RY/RZ (include/PauliSumOperations.h:28-104), half-angle convention.
+    void apply_RX(std::size_t c, T angle) { check(plh_apply_RX(h_, (int64_t)c, angle), "plh_apply_RX"); }
+    void apply_RY(std::size_t c, T angle) { check(plh_apply_RY(h_, (int64_t)c, angle), "plh_apply_RY"); }
+    void apply_RZ(std::size_t c, T angle) { check(plh_apply_RZ(h_, (int64_t)c, angle), "plh_apply_RZ"); }
+    // elementwise_multiply (include/PauliSumOperations.h:220-250).
+    void elementwise_multiply(const DevicePauliSum& rhs) {
+        check(plh_elementwise_multiply(h_, rhs.h_), "plh_elementwise_multiply");
+    }
     template <class HostSum>
     void evolve(const HostSum& rotations) {
         const int64_t n = (int64_t)rotations.get_length();

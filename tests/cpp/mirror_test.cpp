@@ -23,6 +23,9 @@ int orc_apply_H(void*, int64_t);
 int orc_apply_CNOT(void*, int64_t, int64_t);
 int orc_evolve(void*, const void*);
 int orc_trim(void*, int64_t);
+int orc_apply_gate(void*, int, int64_t, int64_t);
+int orc_apply_rot_axis(void*, int, int64_t, double);
+int orc_elementwise_multiply(void*, const void*);
 }
 
 // The public fields of paulib::PauliSum<N,double> (include/PauliSumCore.h:20-45).
@@ -142,6 +145,27 @@ bool run_all() {
     d.trim(77);
     orc_trim(oc, 77);
     ok &= same(d.template download<HostPauliSum<N>>(), oc, "trim");
+    // the extended gate set, half-angle rotations and elementwise_multiply
+    d.apply_sqrt_X(2);
+    d.apply_YC_Z(0, N - 2);
+    d.apply_iSWAP(N - 1, 4);
+    d.apply_RY(5, 0.3);
+    d.apply_RZ(N - 1, -1.1);
+    orc_apply_gate(oc, 8, 2, 0);
+    orc_apply_gate(oc, 17, 0, (int64_t)N - 2);
+    orc_apply_gate(oc, 20, (int64_t)N - 1, 4);
+    orc_apply_rot_axis(oc, 2, 5, 0.3);
+    orc_apply_rot_axis(oc, 3, (int64_t)N - 1, -1.1);
+    ok &= same(d.template download<HostPauliSum<N>>(), oc, "gates/RY/RZ");
+    auto B3 = random_sum<N>(300, 3);
+    paulib_hip::DevicePauliSum<N> ea(A), eb(B3);
+    ea.elementwise_multiply(eb);
+    void* oe = to_oracle(A);
+    void* ob3 = to_oracle(B3);
+    orc_elementwise_multiply(oe, ob3);
+    ok &= same(ea.template download<HostPauliSum<N>>(), oe, "elementwise_multiply");
+    orc_sum_free(oe);
+    orc_sum_free(ob3);
     for (void* p : {oa, ob, raw, oc, orot}) orc_sum_free(p);
     return ok;
 }
