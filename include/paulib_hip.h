@@ -264,7 +264,8 @@ int plh_propagate(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* 
                   int64_t max_terms);
 /* Heisenberg run of a mixed program (application order, processed last to
  * first): kind 0 = H(q0), 1 = S(q0), 2 = CNOT(q0, q1), 3 = rotation by
- * theta about Pauli word px/pz[l*as..]; each rotation is evolve +
+ * theta about Pauli word px/pz[l*as..], 16 + g = Clifford gate id g of
+ * plh_apply_clifford on (q0, q1); each rotation is evolve +
  * apply_truncation as in propagate().  No reference counterpart: it
  * composes the reference's methods the way a brickwork driver would. */
 int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t* q0,

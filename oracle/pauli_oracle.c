@@ -457,6 +457,9 @@ int orc_apply_CNOT(orc_sum *s, int64_t c, int64_t t) {
     return 0;
 }
 
+int orc_apply_gate(orc_sum *s, int g, int64_t c, int64_t t);
+#define ORC_NGATES 21
+
 /* ------------------------------------------------------------- rotation */
 
 static int is_close_one(double v) { return fabs(v - 1.0) <= 1e-14; } /* :19-21 */
@@ -587,7 +590,11 @@ int orc_run_program(orc_sum *s, int64_t nops, const int32_t *kind, const int64_t
                 rc = orc_evolve_cs(s, px + l * as, pz + l * as, 0, cos(theta[l]), sin(theta[l]));
                 if (!rc) rc = orc_apply_truncation(s, max_weight, cutoff, max_terms);
                 break;
-            default: return -2;
+            default:
+                /* kinds 16 + gate id: any gate of orc_apply_gate */
+                if (kind[l] < 16 || kind[l] >= 16 + ORC_NGATES) return -2;
+                rc = orc_apply_gate(s, kind[l] - 16, q0[l], q1[l]);
+                break;
         }
         if (rc) return rc;
     }
@@ -603,8 +610,6 @@ int orc_run_program(orc_sum *s, int64_t nops, const int32_t *kind, const int64_t
  * Each case restates the reference's per-term bit update; the qubit bits of
  * one term are read before any is written, so control and target may share
  * a word (as in the reference, which takes no __restrict__ for them). */
-#define ORC_NGATES 21
-
 static int gate_is_two(int g) { return g >= 10; }
 
 static void gate_term(orc_sum *s, int g, int64_t i, int wc, uint64_t mc, int wt, uint64_t mt) {

@@ -104,6 +104,33 @@ paulib::TruncationOptions<double> opts_of(int64_t max_weight, double cutoff, int
     return o;
 }
 
+template <class S>
+void apply_gate_to(S& s, int g, std::size_t q, std::size_t r) {
+    switch (g) {
+        case 0: s.apply_I(q); break;
+        case 1: s.apply_X(q); break;
+        case 2: s.apply_Y(q); break;
+        case 3: s.apply_Z(q); break;
+        case 4: s.apply_H(q); break;
+        case 5: s.apply_S(q); break;
+        case 6: s.apply_H_YZ(q); break;
+        case 7: s.apply_H_XY(q); break;
+        case 8: s.apply_sqrt_X(q); break;
+        case 9: s.apply_sqrt_Y(q); break;
+        case 10: s.apply_CNOT(q, r); break;
+        case 11: s.apply_CZ(q, r); break;
+        case 12: s.apply_CY(q, r); break;
+        case 13: s.apply_XC_X(q, r); break;
+        case 14: s.apply_XC_Z(q, r); break;
+        case 15: s.apply_XC_Y(q, r); break;
+        case 16: s.apply_YC_X(q, r); break;
+        case 17: s.apply_YC_Z(q, r); break;
+        case 18: s.apply_YC_Y(q, r); break;
+        case 19: s.apply_SWAP(q, r); break;
+        default: s.apply_iSWAP(q, r); break;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -234,7 +261,11 @@ int ref_run_program(void* p, int64_t nops, const int32_t* kind, const int64_t* q
                     paulib::apply_truncation(s, opts);
                     break;
                 }
-                default: return -2;
+                default:
+                    // kinds 16 + gate id: the extended gate set
+                    if (kind[l] < 16 || kind[l] > 36) return -2;
+                    apply_gate_to(s, kind[l] - 16, (std::size_t)q0[l], (std::size_t)q1[l]);
+                    break;
             }
         }
         return 0;
@@ -249,32 +280,7 @@ int ref_apply_gate(void* p, int g, int64_t c, int64_t t) {
     if (g < 0 || g > 20 || c < 0 || c >= nqb) return -2;
     if (g >= 10 && (t < 0 || t >= nqb || t == c)) return -2;
     const std::size_t q = (std::size_t)c, r = (std::size_t)t;
-    return dispatch(b, [&](auto& s) {
-        switch (g) {
-            case 0: s.apply_I(q); break;
-            case 1: s.apply_X(q); break;
-            case 2: s.apply_Y(q); break;
-            case 3: s.apply_Z(q); break;
-            case 4: s.apply_H(q); break;
-            case 5: s.apply_S(q); break;
-            case 6: s.apply_H_YZ(q); break;
-            case 7: s.apply_H_XY(q); break;
-            case 8: s.apply_sqrt_X(q); break;
-            case 9: s.apply_sqrt_Y(q); break;
-            case 10: s.apply_CNOT(q, r); break;
-            case 11: s.apply_CZ(q, r); break;
-            case 12: s.apply_CY(q, r); break;
-            case 13: s.apply_XC_X(q, r); break;
-            case 14: s.apply_XC_Z(q, r); break;
-            case 15: s.apply_XC_Y(q, r); break;
-            case 16: s.apply_YC_X(q, r); break;
-            case 17: s.apply_YC_Z(q, r); break;
-            case 18: s.apply_YC_Y(q, r); break;
-            case 19: s.apply_SWAP(q, r); break;
-            default: s.apply_iSWAP(q, r); break;
-        }
-        return 0;
-    });
+    return dispatch(b, [&](auto& s) { apply_gate_to(s, g, q, r); return 0; });
 }
 
 // apply_RX / apply_RY / apply_RZ by axis (1 X, 2 Y, 3 Z).

@@ -887,7 +887,13 @@ int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t
                 PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, 0, theta[l], true, max_weight, cutoff,
                                       max_terms));
                 break;
-            default: set_error("unknown op kind"); return PLH_ERR_ARG;
+            default:
+                if (kind[l] < kProgramGateBase || kind[l] >= kProgramGateBase + kNumGates) {
+                    set_error("unknown op kind");
+                    return PLH_ERR_ARG;
+                }
+                PLH_TRY(op_clifford_table(s->ctx, s, kind[l] - kProgramGateBase, q0[l], q1[l]));
+                break;
         }
     }
     return plh_ctx_synchronize(s->ctx);

@@ -297,6 +297,9 @@ int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1);
 // single-qubit, 10 CNOT .. 20 iSWAP two-qubit), one table-driven launch.
 constexpr int kNumGates = 21;
 constexpr int kFirstTwoQubitGate = 10;
+// Program op kinds (plh_run_program, plh_server_submit): 0 H, 1 S, 2 CNOT,
+// 3 rotation, and kProgramGateBase + id for any gate of the table set.
+constexpr int kProgramGateBase = 16;
 int op_clifford_table(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1);
 int clifford_table_of(int gate, uint64_t* tbl, uint32_t* sgn);
 int op_elementwise(plh_ctx* ctx, plh_sum* s, const plh_sum* rhs);

@@ -43,7 +43,7 @@ constexpr uint64_t kTile = (uint64_t)kBlock * kIpt;
 // Per-step job tables (device, one upload per step), one entry per slot.
 struct StepTables {
     const uint32_t* cl_off;   // [slots + 1] range of the slot's Clifford ops
-    const uint32_t* cl_op;    // kind << 28 | q0 << 14 | q1 (kind 0 H, 1 S, 2 CNOT)
+    const uint32_t* cl_op;    // kind << 28 | q0 << 14 | q1 (kind 0 H, 1 S, 2 CNOT); kind 3: apply_table_op
     const uint8_t* rot_kind;  // 0 none, 1 sign flip, 2 flip + multiply, 3 multiply, 4 general
     const uint8_t* rot_psf;   // flags of ps = P with the i bit flipped
     const double* rot_c;
@@ -54,12 +54,51 @@ struct StepTables {
     uint8_t* alive;
 };
 
+// Tables of the gate set (16 nibbles of new local bits, 16 sign bits per
+// gate, as k_clifford_table takes them), written once per device when a
+// server is created.
+__constant__ uint64_t c_gate_tbl[kNumGates];
+__constant__ uint32_t c_gate_sgn[kNumGates];
+
+// A table gate on a term held in registers: op = 3 << 28 | gate << 20 |
+// q0 << 10 | q1 (qubits below 1024).
+template <int AS>
+PLH_DEV void apply_table_op(uint32_t op, uint64_t* x, uint64_t* z, uint32_t& f) {
+    const uint32_t g = (op >> 20) & 0xFFu, q0 = (op >> 10) & 0x3FFu, q1 = op & 0x3FFu;
+    const bool two = g >= (uint32_t)kFirstTwoQubitGate;
+    const int wc = (int)(q0 >> 6), wt = (int)(q1 >> 6);
+    const uint64_t mc = 1ull << (q0 & 63), mt = two ? 1ull << (q1 & 63) : 0ull;
+    uint32_t b = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        if (k == wc) b |= (uint32_t)((x[k] & mc) != 0) | ((uint32_t)((z[k] & mc) != 0) << 1);
+        if (two && k == wt) b |= ((uint32_t)((x[k] & mt) != 0) << 2) | ((uint32_t)((z[k] & mt) != 0) << 3);
+    }
+    const uint32_t e = (uint32_t)(c_gate_tbl[g] >> (4 * b)) & 15u;
+    f ^= ((c_gate_sgn[g] >> b) & 1u) << 1;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        if (k == wc) {
+            x[k] = (x[k] & ~mc) | ((e & 1u) ? mc : 0ull);
+            z[k] = (z[k] & ~mc) | ((e & 2u) ? mc : 0ull);
+        }
+        if (two && k == wt) {
+            x[k] = (x[k] & ~mt) | ((e & 4u) ? mt : 0ull);
+            z[k] = (z[k] & ~mt) | ((e & 8u) ? mt : 0ull);
+        }
+    }
+}
+
 // One Clifford conjugation of a term held in registers: the update of the
-// plain k_clifford kernel (H, S, CNOT of include/PauliSumClifford.h).  Words
-// are selected by compare so the term stays in registers.
+// plain k_clifford kernel (H, S, CNOT of include/PauliSumClifford.h), or a
+// table gate.  Words are selected by compare so the term stays in registers.
 template <int AS>
 PLH_DEV void apply_clifford_op(uint32_t op, uint64_t* x, uint64_t* z, uint32_t& f) {
     const uint32_t kind = op >> 28, q0 = (op >> 14) & 0x3FFFu, q1 = op & 0x3FFFu;
+    if (kind == 3) {
+        apply_table_op<AS>(op, x, z, f);
+        return;
+    }
     const int wc = (int)(q0 >> 6);
     const uint64_t mc = 1ull << (q0 & 63);
     uint64_t xw = 0, zw = 0;
@@ -338,6 +377,13 @@ int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_bu
         return PLH_ERR_AS;
     }
     PLH_HIP(hipSetDevice(ctx->device));
+    {
+        uint64_t tbl[kNumGates];
+        uint32_t sgn[kNumGates];
+        for (int g = 0; g < kNumGates; ++g) PLH_TRY(clifford_table_of(g, &tbl[g], &sgn[g]));
+        PLH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gate_tbl), tbl, sizeof(tbl), 0, hipMemcpyHostToDevice));
+        PLH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gate_sgn), sgn, sizeof(sgn), 0, hipMemcpyHostToDevice));
+    }
     size_t free_b = 0, total_b = 0;
     PLH_HIP(hipMemGetInfo(&free_b, &total_b));
     plh_server* s = new plh_server();
@@ -391,9 +437,11 @@ int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_
         return PLH_ERR_SIZE;
     }
     for (int64_t l = 0; l < nops; ++l) {
-        const bool clifford = kind[l] >= 0 && kind[l] <= 2;
-        if (kind[l] < 0 || kind[l] > 3 || (clifford && (q0[l] < 0 || q0[l] >= num_qubits)) ||
-            (kind[l] == 2 && (q1[l] < 0 || q1[l] >= num_qubits || q1[l] == q0[l]))) {
+        const bool table = kind[l] >= kProgramGateBase && kind[l] < kProgramGateBase + kNumGates;
+        const bool two = kind[l] == 2 || (table && kind[l] - kProgramGateBase >= kFirstTwoQubitGate);
+        const bool clifford = (kind[l] >= 0 && kind[l] <= 2) || table;
+        if (((kind[l] < 0 || kind[l] > 3) && !table) || (clifford && (q0[l] < 0 || q0[l] >= num_qubits)) ||
+            (two && (q1[l] < 0 || q1[l] >= num_qubits || q1[l] == q0[l]))) {
             set_error("plh_server_submit: invalid op in job program");
             return PLH_ERR_ARG;
         }
@@ -476,7 +524,13 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
         while (j.next >= 0 && j.kind[(size_t)j.next] != 3) {
             const int64_t l = j.next--;
             const uint32_t k = (uint32_t)j.kind[(size_t)l];
-            ops.push_back((k << 28) | ((uint32_t)j.q0[(size_t)l] << 14) | (k == 2 ? (uint32_t)j.q1[(size_t)l] : 0u));
+            if (k >= (uint32_t)kProgramGateBase) {  // table gate: qubits < 1024 (at most 16 words)
+                const uint32_t g = k - (uint32_t)kProgramGateBase;
+                const uint32_t t = g >= (uint32_t)kFirstTwoQubitGate ? (uint32_t)j.q1[(size_t)l] : 0u;
+                ops.push_back((3u << 28) | (g << 20) | ((uint32_t)j.q0[(size_t)l] << 10) | t);
+            } else {
+                ops.push_back((k << 28) | ((uint32_t)j.q0[(size_t)l] << 14) | (k == 2 ? (uint32_t)j.q1[(size_t)l] : 0u));
+            }
         }
         if (j.next >= 0) {
             const int64_t l = j.next--;

@@ -104,7 +104,8 @@ def product_65536x65536_q64(seed: int = 7, n: int = 65536, m: int | None = None)
 @dataclass
 class Program:
     """Ops in application order: kind 0 = H(q0), 1 = S(q0), 2 = CNOT(q0, q1),
-    3 = rotation by theta about the Pauli word px/pz[l*AS : l*AS + AS]."""
+    3 = rotation by theta about the Pauli word px/pz[l*AS : l*AS + AS],
+    16 + g = gate id g of the extended set (paulib_amd.gates) on (q0, q1)."""
 
     array_size: int
     num_qubits: int
@@ -165,6 +166,33 @@ def brickwork_program(num_qubits: int, depth: int, seed: int) -> Program:
         off = layer % 2
         for q in range(off, num_qubits - 1, 2):
             b.clifford(2, q, q + 1)
+        axes = rng.integers(0, 3, size=num_qubits)
+        th = rng.random(num_qubits) * 2 * np.pi
+        for q in range(num_qubits):
+            b.rotation({q: "XYZ"[axes[q]]}, th[q])
+        th2 = rng.random(num_qubits) * 2 * np.pi
+        for q in range(off, num_qubits - 1, 2):
+            b.rotation({q: "Z", q + 1: "Z"}, th2[q])
+    return b.build()
+
+
+def mixed_gate_program(num_qubits: int, depth: int, seed: int) -> Program:
+    """Brickwork layers whose Cliffords are drawn from the whole gate set
+    (single-qubit gates on random qubits, two-qubit gates on neighbour pairs
+    in either order), each layer followed by single-qubit and ZZ rotations."""
+    from .gates import SINGLE, TWO
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = _ProgramBuilder(num_qubits)
+    for layer in range(depth):
+        for q in range(num_qubits):
+            if rng.random() < 0.5:
+                b.clifford(16 + int(rng.integers(len(SINGLE))), q)
+        off = layer % 2
+        for q in range(off, num_qubits - 1, 2):
+            g = 16 + len(SINGLE) + int(rng.integers(len(TWO)))
+            c, t = (q, q + 1) if rng.random() < 0.5 else (q + 1, q)
+            b.clifford(g, c, t)
         axes = rng.integers(0, 3, size=num_qubits)
         th = rng.random(num_qubits) * 2 * np.pi
         for q in range(num_qubits):

@@ -123,3 +123,20 @@ def test_elementwise_length_mismatch(ctx):
     d = dev(ctx, W.random_words_sum(10, 64, 1))
     with pytest.raises(_lib.EngineError):
         d.elementwise_multiply(dev(ctx, W.random_words_sum(9, 64, 2)))
+
+
+@pytest.mark.parametrize("nq,depth,seed", [(12, 5, 1), (64, 3, 2), (100, 2, 3)])
+def test_run_program_with_gate_set(ctx, nq, depth, seed):
+    """run_program with the whole gate set (kinds 16 + id) and truncated
+    rotations against the oracle's run_program."""
+    from paulib_amd import TruncationOptions, run_program
+
+    prog = W.mixed_gate_program(nq, depth, seed)
+    O = W.observable(nq, [{0: "Z"}, {nq // 2: "X", nq // 2 + 1: "Y"}])
+    d = dev(ctx, O)
+    run_program(d, prog, TruncationOptions(max_weight=6, coefficient_cutoff=1e-4))
+    o = cpu("oracle", O)
+    o.run_program(prog, 6, 1e-4)
+    assert_terms_equal(d.to_host(), o.to_host())
+    e_got, e_want = d.zero_state_expectation(), o.zero_state_expectation()
+    assert abs(e_got - e_want) <= 1e-12 * max(1.0, np.abs(o.to_host().value).sum())

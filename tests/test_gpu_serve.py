@@ -164,3 +164,24 @@ def test_server_rejects_bad_jobs(ctx):
         with pytest.raises(ValueError):
             srv.submit(W.observable(8, [{0: "Z"}]), W.brickwork_program(8, 1, 1),
                        TruncationOptions(max_terms=10))
+
+
+def test_server_gate_set_matches_run_program(ctx):
+    """Programs with the whole gate set (kinds 16 + id, applied from the
+    server's constant gate tables) give what the plain run_program gives
+    (strings, phases and coefficients bit for bit) and the oracle's
+    expectation."""
+    rng = np.random.Generator(np.random.PCG64(77))
+    jobs = []
+    for nq in (9, 40, 64, 100, 128):
+        O = W.observable(nq, [{0: "Z"}, {nq // 2: "X", nq - 1: "Y"}], [0.8, -0.3])
+        prog = W.mixed_gate_program(nq, int(rng.integers(2, 4)), int(rng.integers(1 << 30)))
+        jobs.append((O, prog, TruncationOptions(max_weight=6, coefficient_cutoff=1e-4)))
+    out = _serve(ctx, jobs)
+    for (O, prog, opts), (r, got) in zip(jobs, out):
+        d = PauliSum.from_host(ctx, O)
+        run_program(d, prog, opts)
+        d.canonicalize()
+        assert_terms_equal(got, d.to_host())
+        e_want, want = _oracle(O, prog, opts)
+        assert abs(r.expectation - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum())

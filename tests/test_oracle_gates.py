@@ -219,3 +219,16 @@ def test_against_reference_random_sums(as_):
     a.elementwise_multiply(cpu("oracle", B))
     b.elementwise_multiply(cpu("reference", B))
     assert_terms_equal(a.to_host(), b.to_host())
+
+
+@pytest.mark.skipif("reference" not in KINDS, reason="oracle/_ref not built")
+@pytest.mark.parametrize("nq,seed", [(12, 1), (70, 2)])
+def test_mixed_gate_program_against_reference(nq, seed):
+    """Programs with the whole gate set (kinds 16 + id) and truncated
+    rotations: the oracle's run_program against the reference's methods."""
+    prog = W.mixed_gate_program(nq, 4, seed)
+    O = W.observable(nq, [{0: "Z"}, {nq // 2: "X", nq // 2 + 1: "Y"}])
+    a, b = cpu("oracle", O), cpu("reference", O)
+    a.run_program(prog, 6, 1e-4)
+    b.run_program(prog, 6, 1e-4)
+    assert_terms_equal(a.to_host(), b.to_host(), values="tol")
