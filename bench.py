@@ -299,7 +299,27 @@ def cpu_baseline(A, B, target_s: float):
 
         h = ctypes.CDLL(str(oracle.REF_LIB))
         threads = str(h.ref_num_threads())
+    # the 1000 x 1000, 32-qubit case in full: best of 5 after 3 warm-ups
+    from paulib_amd import workloads as W
+
+    A0, B0 = W.product_1000x1000_q32(seed=1)
+    c0 = {}
+    for op in ("product", "commutator"):
+        ts = []
+        for r in range(8):
+            a = oracle.CpuSum.from_host(kind, A0)
+            b = oracle.CpuSum.from_host(kind, B0)
+            t0 = time.perf_counter()
+            if op == "product":
+                a.multiply(b)
+                a.canonicalize()
+            else:
+                a.commutator(b).canonicalize()
+            if r >= 3:
+                ts.append(time.perf_counter() - t0)
+        c0[op + "_ms_best"] = min(ts) * 1e3
     return {
+        "c0": c0,
         "value": len(A) * m / t,
         "unit": "collated product terms/s",
         "cores": int(threads),
@@ -325,6 +345,7 @@ def main():
     ap.add_argument("--solo-pairs", type=int, default=0)  # 0: the context's tuned threshold
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--gate-log2-terms", type=int, default=26)
+    ap.add_argument("--commutator-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-propagation", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -350,7 +371,13 @@ def main():
         t0 = time.perf_counter()
         tuning = ctx.autotune(args.tune_cache)
         log(f"autotune: {tuning} ({time.perf_counter() - t0:.1f} s, cache {args.tune_cache})")
+    dA, dB = PauliSum.from_host(ctx, A), PauliSum.from_host(ctx, B)  # first upload: allocation
+    dA.close(), dB.close()
+    ctx.synchronize()
+    t0 = time.perf_counter()
     dA, dB = PauliSum.from_host(ctx, A), PauliSum.from_host(ctx, B)
+    ctx.synchronize()
+    t_upload = time.perf_counter() - t0
     out = PauliSum(ctx, 1, 64)
     pairs = args.n * args.m
 
@@ -410,8 +437,69 @@ def main():
         "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()},
         "tuning": tuning,
     }
-    dA.close(), dB.close(), out.close()
+    # PCIe-inclusive rate of the host-buffer drop-in (multiply_canonical_on_gpu):
+    # measured upload of both operands, the device time, and the output's
+    # download at the rate measured on a range of 2^27 terms (pageable host
+    # memory, as the drop-in writes into the caller's vectors).
+    cnt = min(1 << 27, t_out)
+    out.range_to_host(0, min(1 << 20, t_out))
+    t0 = time.perf_counter()
+    out.range_to_host(0, cnt)
+    t_rng = time.perf_counter() - t0
+    d2h = cnt * 25 / t_rng
+    t_pcie = t_upload + dt + t_out * 25 / d2h
+    result["pcie_inclusive"] = {
+        "value": pairs / t_pcie, "unit": "collated product terms/s", "upload_s": t_upload, "device_s": dt,
+        "download_s_est": t_out * 25 / d2h, "d2h_GBps": d2h / 1e9,
+        "note": f"download time from a measured {cnt}-term range download; not the headline value"}
+    log(f"pcie-inclusive: upload {t_upload:.3f} s, D2H {d2h / 1e9:.1f} GB/s, {pairs / t_pcie:.3e} terms/s")
+    out.close()
+    if args.commutator_steps > 0:
+        cout = PauliSum(ctx, 1, 64)
+        _lib.call("plh_commutator_canonical", dA.handle, dB.handle, 0, cout.handle)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.commutator_steps):
+            _lib.call("plh_commutator_canonical", dA.handle, dB.handle, 0, cout.handle)
+        ctx.synchronize()
+        dt_c = (time.perf_counter() - t0) / args.commutator_steps
+        result["commutator"] = {"workload": f"canonicalize([A, B]) at {args.n} x {args.m}, 64 qubits",
+                                "ms_per_step": dt_c * 1e3, "pairs_per_s": pairs / dt_c,
+                                "collated_terms": len(cout), "steps": args.commutator_steps}
+        log(f"commutator: {dt_c * 1e3:.1f} ms, {len(cout)} collated terms")
+        cout.close()
+    dA.close(), dB.close()
     ctx.trim()
+
+    # the 1000 x 1000, 32-qubit case (A*B and [A, B], canonicalized): device
+    # time with resident operands and the time with upload and download
+    A0, B0 = W.product_1000x1000_q32(seed=1)
+    c0 = {"workload": "product_1000x1000_q32 canonicalize(A*B) and canonicalize([A,B])"}
+    for name, fn in [("product", "plh_multiply_canonical"), ("commutator", "plh_commutator_canonical")]:
+        dev_t, inc_t = [], []
+        for r in range(8):
+            t0 = time.perf_counter()
+            a, b = PauliSum.from_host(ctx, A0), PauliSum.from_host(ctx, B0)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            o = PauliSum(ctx, 1, 32)
+            if fn == "plh_multiply_canonical":
+                _lib.call(fn, a.handle, b.handle, o.handle)
+            else:
+                _lib.call(fn, a.handle, b.handle, 0, o.handle)
+            ctx.synchronize()
+            t2 = time.perf_counter()
+            h = o.to_host()
+            t3 = time.perf_counter()
+            if r >= 3:
+                dev_t.append(t2 - t1)
+                inc_t.append(t3 - t0)
+            a.close(), b.close(), o.close()
+        c0[name] = {"device_ms_best": min(dev_t) * 1e3, "device_ms_median": float(np.median(dev_t)) * 1e3,
+                    "with_transfers_ms_median": float(np.median(inc_t)) * 1e3, "collated_terms": len(h),
+                    "pairs_per_s": 1e6 / float(np.median(dev_t))}
+        log(f"c0 {name}: {min(dev_t) * 1e3:.3f} ms device, {float(np.median(inc_t)) * 1e3:.3f} ms with transfers")
+    result["c0"] = c0
 
     if not args.no_propagation and args.prop_jobs > 0:
         O, prog, tr = W.brickwork_q64_d20()
