@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include <cmath>
+#include <thread>
 #include <cstring>
 #include <string>
 
@@ -211,7 +212,75 @@ int rotation_step(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf
     return PLH_OK;
 }
 
+// Host copy of len bytes spread over up to 8 threads (the destination's
+// first-touch page faults are spread with it).
+void parallel_memcpy(void* dst, const void* src, size_t len) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t T = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, len >> 20));
+    if (T <= 1) {
+        std::memcpy(dst, src, len);
+        return;
+    }
+    const size_t part = (len + T - 1) / T;
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (size_t t = 1; t < T; ++t) {
+        const size_t o = t * part;
+        if (o >= len) break;
+        th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, std::min(part, len - o)); });
+    }
+    std::memcpy(dst, src, std::min(part, len));
+    for (auto& t : th) t.join();
+}
+
+// Device-to-host copy into pageable memory, ordered after the context's
+// earlier work.  Large copies go through the pinned XferStage chunks: the DMA
+// of chunk k+1 is issued before the host copies chunk k out, and a chunk's
+// buffer is reused only after its host copy has finished.  Small ones are
+// one direct async copy (the caller synchronises the stream).
+int copy_to_host(plh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    hipStream_t st = ctx->stream;
+    if (bytes < XferStage::kMinStaged) {
+        PLH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        return PLH_OK;
+    }
+    PLH_TRY(ctx->xfer.init());
+    const size_t C = XferStage::kChunk;
+    const size_t nch = (bytes + C - 1) / C;
+    auto issue = [&](size_t k) -> int {
+        const size_t off = k * C, len = std::min(C, bytes - off);
+        PLH_HIP(hipMemcpyAsync(ctx->xfer.buf[k & 1], (const char*)src + off, len, hipMemcpyDeviceToHost, st));
+        PLH_HIP(hipEventRecord(ctx->xfer.ev[k & 1], st));
+        return PLH_OK;
+    };
+    PLH_TRY(issue(0));
+    for (size_t k = 0; k < nch; ++k) {
+        if (k + 1 < nch) PLH_TRY(issue(k + 1));  // its buffer's last chunk (k - 1) is already copied out
+        PLH_HIP(hipEventSynchronize(ctx->xfer.ev[k & 1]));
+        const size_t off = k * C;
+        parallel_memcpy((char*)dst + off, ctx->xfer.buf[k & 1], std::min(C, bytes - off));
+    }
+    return PLH_OK;
+}
+
 }  // namespace
+
+namespace plh {
+int XferStage::init() {
+    for (int b = 0; b < 2; ++b) {
+        if (!buf[b]) PLH_HIP(hipHostMalloc(&buf[b], kChunk, hipHostMallocDefault));
+        if (!ev[b]) PLH_HIP(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+    }
+    return PLH_OK;
+}
+
+XferStage::~XferStage() {
+    for (int b = 0; b < 2; ++b) {
+        if (buf[b]) (void)hipHostFree(buf[b]);
+        if (ev[b]) (void)hipEventDestroy(ev[b]);
+    }
+}
+}  // namespace plh
 
 extern "C" {
 
@@ -427,19 +496,8 @@ int plh_sum_download(const plh_sum* s, uint64_t* x, uint64_t* z, uint8_t* flags,
     PLH_TRY(check_sum(s));
     const int64_t n = s->n;
     if (n > 0 && (!x || !z || !flags || !val)) return PLH_ERR_ARG;
-    hipStream_t st = s->ctx->stream;
-    for (int k = 0; k < s->as && n > 0; ++k) {
-        PLH_HIP(hipMemcpyAsync(x + (int64_t)k * n, s->x + (int64_t)k * s->cap, (size_t)n * 8,
-                               hipMemcpyDeviceToHost, st));
-        PLH_HIP(hipMemcpyAsync(z + (int64_t)k * n, s->z + (int64_t)k * s->cap, (size_t)n * 8,
-                               hipMemcpyDeviceToHost, st));
-    }
-    if (n > 0) {
-        PLH_HIP(hipMemcpyAsync(flags, s->f, (size_t)n, hipMemcpyDeviceToHost, st));
-        PLH_HIP(hipMemcpyAsync(val, s->v, (size_t)n * 8, hipMemcpyDeviceToHost, st));
-    }
-    PLH_HIP(hipStreamSynchronize(st));
-    return PLH_OK;
+    if (n == 0) return PLH_OK;
+    return plh_sum_download_range(s, 0, n, x, z, flags, val);
 }
 
 int plh_sum_download_range(const plh_sum* s, int64_t offset, int64_t count, uint64_t* x, uint64_t* z,
@@ -450,18 +508,18 @@ int plh_sum_download_range(const plh_sum* s, int64_t offset, int64_t count, uint
         return PLH_ERR_ARG;
     }
     if (count == 0) return PLH_OK;
-    hipStream_t st = s->ctx->stream;
+    plh_ctx* ctx = s->ctx;
     for (int k = 0; k < s->as; ++k) {
         if (x)
-            PLH_HIP(hipMemcpyAsync(x + (int64_t)k * count, s->x + (int64_t)k * s->cap + offset, (size_t)count * 8,
-                                   hipMemcpyDeviceToHost, st));
+            PLH_TRY(copy_to_host(ctx, x + (int64_t)k * count, s->x + (int64_t)k * s->cap + offset,
+                                 (size_t)count * 8));
         if (z)
-            PLH_HIP(hipMemcpyAsync(z + (int64_t)k * count, s->z + (int64_t)k * s->cap + offset, (size_t)count * 8,
-                                   hipMemcpyDeviceToHost, st));
+            PLH_TRY(copy_to_host(ctx, z + (int64_t)k * count, s->z + (int64_t)k * s->cap + offset,
+                                 (size_t)count * 8));
     }
-    if (flags) PLH_HIP(hipMemcpyAsync(flags, s->f + offset, (size_t)count, hipMemcpyDeviceToHost, st));
-    if (val) PLH_HIP(hipMemcpyAsync(val, s->v + offset, (size_t)count * 8, hipMemcpyDeviceToHost, st));
-    PLH_HIP(hipStreamSynchronize(st));
+    if (flags) PLH_TRY(copy_to_host(ctx, flags, s->f + offset, (size_t)count));
+    if (val) PLH_TRY(copy_to_host(ctx, val, s->v + offset, (size_t)count * 8));
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
     return PLH_OK;
 }
 

@@ -120,6 +120,18 @@ struct HostStage {
     ~HostStage();
 };
 
+// Pinned staging for large device-to-host copies into pageable memory (a
+// caller's vectors): two host-pinned chunks, so the DMA of chunk k+1 runs
+// while the host copies chunk k out, spread over a few threads.
+struct XferStage {
+    static constexpr size_t kChunk = 64ull << 20;
+    static constexpr size_t kMinStaged = 1ull << 20;  // smaller copies go direct
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int init();
+    ~XferStage();
+};
+
 }  // namespace plh
 
 // Device-resident PauliSum<N,double>: plane-major SoA, x[k*cap + t].
@@ -171,6 +183,7 @@ struct plh_ctx {
     plh::DeviceHeap heap;  // declared before ws: ws returns its ranges on destruction
     plh::Workspace ws{&heap};
     plh::HostStage stage;
+    plh::XferStage xfer;
     bool prof = false;
     std::vector<plh_sum*> sums;  // live sums; orphaned (ctx = null) when the context goes
     std::vector<plh::ProfRec> prof_recs;
