@@ -263,6 +263,29 @@ int copy_to_host(plh_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return PLH_OK;
 }
 
+// Host-to-device copy from pageable memory, ordered on the context's stream.
+// Large copies go through the pinned chunks: the host copy of chunk k+1 into
+// one chunk runs while chunk k's DMA runs from the other, and a chunk is
+// refilled only after its previous DMA has finished (its event).  The caller
+// synchronises the stream before the source may change.
+int copy_to_device(plh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    hipStream_t st = ctx->stream;
+    if (bytes < XferStage::kMinStaged) {
+        PLH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return PLH_OK;
+    }
+    PLH_TRY(ctx->xfer.init());
+    const size_t C = XferStage::kChunk;
+    for (size_t k = 0, off = 0; off < bytes; ++k, off += C) {
+        const size_t len = std::min(C, bytes - off);
+        PLH_HIP(hipEventSynchronize(ctx->xfer.ev[k & 1]));  // the chunk's previous DMA is done
+        parallel_memcpy(ctx->xfer.buf[k & 1], (const char*)src + off, len);
+        PLH_HIP(hipMemcpyAsync((char*)dst + off, ctx->xfer.buf[k & 1], len, hipMemcpyHostToDevice, st));
+        PLH_HIP(hipEventRecord(ctx->xfer.ev[k & 1], st));
+    }
+    return PLH_OK;
+}
+
 }  // namespace
 
 namespace plh {
@@ -475,18 +498,16 @@ int plh_sum_upload(plh_sum* s, int64_t n, const uint64_t* x, const uint64_t* z, 
     PLH_TRY(check_sum(s));
     if (n < 0 || (n > 0 && (!x || !z || !flags || !val))) return PLH_ERR_ARG;
     PLH_TRY(sum_reserve(s, n, 0));
-    hipStream_t st = s->ctx->stream;
+    plh_ctx* ctx = s->ctx;
     for (int k = 0; k < s->as && n > 0; ++k) {
-        PLH_HIP(hipMemcpyAsync(s->x + (int64_t)k * s->cap, x + (int64_t)k * n, (size_t)n * 8,
-                               hipMemcpyHostToDevice, st));
-        PLH_HIP(hipMemcpyAsync(s->z + (int64_t)k * s->cap, z + (int64_t)k * n, (size_t)n * 8,
-                               hipMemcpyHostToDevice, st));
+        PLH_TRY(copy_to_device(ctx, s->x + (int64_t)k * s->cap, x + (int64_t)k * n, (size_t)n * 8));
+        PLH_TRY(copy_to_device(ctx, s->z + (int64_t)k * s->cap, z + (int64_t)k * n, (size_t)n * 8));
     }
     if (n > 0) {
-        PLH_HIP(hipMemcpyAsync(s->f, flags, (size_t)n, hipMemcpyHostToDevice, st));
-        PLH_HIP(hipMemcpyAsync(s->v, val, (size_t)n * 8, hipMemcpyHostToDevice, st));
+        PLH_TRY(copy_to_device(ctx, s->f, flags, (size_t)n));
+        PLH_TRY(copy_to_device(ctx, s->v, val, (size_t)n * 8));
     }
-    PLH_HIP(hipStreamSynchronize(st));
+    PLH_HIP(hipStreamSynchronize(ctx->stream));
     s->n = n;
     s->canonical = false;
     return PLH_OK;
