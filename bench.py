@@ -273,6 +273,48 @@ def clifford_gates(ctx, log2_terms: int, reps: int):
     return out
 
 
+def raw_products(ctx, reps: int):
+    """The uncombined all-pairs product (PauliSum::multiply, k_mul_raw) at the
+    shapes of the reference's own GPU benchmark (500 qubits, 8 words per
+    plane; bench/GpuMulBench.cpp) and at 65536 x 65536 on 64 qubits: kernel
+    time from HIP events on the engine's stream, pairs/s and the HBM roofline
+    (output written, operands read: 16 * words + 9 bytes per term)."""
+    from paulib_amd import PauliSum, _lib
+    from paulib_amd import workloads as W
+
+    shapes = [(1000, 100, 500), (1000, 500, 500), (1000, 1000, 500), (1000, 2000, 500), (1000, 5000, 500),
+              (10000, 10000, 500), (65536, 65536, 64)]
+    out = {}
+    for n, m, nq in shapes:
+        if nq == 64:
+            A, B = W.product_65536x65536_q64(seed=7, n=n, m=m)
+        else:
+            A, B = W.random_words_sum(n, nq, 101), W.random_words_sum(m, nq, 102)
+        a, b = PauliSum.from_host(ctx, A), PauliSum.from_host(ctx, B)
+        r = max(1, reps if n * m <= 10 ** 8 else 2)
+        lhs = a.copy()
+        lhs.multiply(b)  # warm-up: output storage
+        lhs.close()
+        ctx.synchronize()
+        _lib.call("plh_ctx_profile", ctx.handle, 1)
+        for _ in range(r):
+            lhs = a.copy()
+            lhs.multiply(b)
+            lhs.close()
+        ctx.synchronize()
+        prof = kernel_profile(ctx)["mul_raw"]
+        _lib.call("plh_ctx_profile", ctx.handle, 0)
+        ms = prof["ms"] / prof["launches"]
+        gbs = prof["bytes"] / prof["launches"] / (ms / 1e3) / 1e9
+        key = f"{n}x{m}_q{nq}"
+        out[key] = {"ms": ms, "pairs_per_s": n * m / (ms / 1e3), "GB/s": gbs, "frac": gbs * 1e9 / HBM_PEAK,
+                    "words": A.array_size}
+        log(f"raw product {key}: {ms:.4f} ms, {n * m / (ms / 1e3):.3e} pairs/s, {gbs:.0f} GB/s")
+        a.close(), b.close()
+    ctx.trim()
+    return out
+
+
 def cpu_baseline(A, B, target_s: float):
     """Reference multiply + canonicalize on a rhs slice sized for ~target_s."""
     import oracle
@@ -318,6 +360,18 @@ def cpu_baseline(A, B, target_s: float):
             if r >= 3:
                 ts.append(time.perf_counter() - t0)
         c0[op + "_ms_best"] = min(ts) * 1e3
+    # the uncombined product at one shape of the reference's GPU benchmark
+    # (bench/GpuMulBench.cpp: 1000 x 1000 on 500 qubits), best of 5
+    Ar, Br = W.random_words_sum(1000, 500, 101), W.random_words_sum(1000, 500, 102)
+    ts = []
+    for r in range(8):
+        a = oracle.CpuSum.from_host(kind, Ar)
+        b = oracle.CpuSum.from_host(kind, Br)
+        t0 = time.perf_counter()
+        a.multiply(b)
+        if r >= 3:
+            ts.append(time.perf_counter() - t0)
+    c0["raw_product_1000x1000_q500_ms_best"] = min(ts) * 1e3
     return {
         "c0": c0,
         "value": len(A) * m / t,
@@ -346,6 +400,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--gate-log2-terms", type=int, default=26)
     ap.add_argument("--commutator-steps", type=int, default=2)
+    ap.add_argument("--raw-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-propagation", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -528,6 +583,8 @@ def main():
                                  "mode": "sequential jobs, plain drop-in", "jobs": len(lat),
                                  "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                                  "jobs_per_s": float(1e3 / lat.mean()), "expectation": e}
+    if args.raw_reps > 0:
+        result["raw_products"] = raw_products(ctx, args.raw_reps)
     if args.gate_log2_terms > 0:
         result["clifford_gates"] = clifford_gates(ctx, args.gate_log2_terms, 20)
     if args.batch_jobs > 0:

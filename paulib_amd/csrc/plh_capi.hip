@@ -603,8 +603,14 @@ int plh_multiply(plh_sum* lhs, const plh_sum* rhs) {
     out.as = lhs->as;
     out.nq = lhs->nq;
     PLH_TRY(sum_reserve(&out, n * m, 0));
-    int rc = op_mul_raw(lhs->as, lhs->x, lhs->z, lhs->f, lhs->v, lhs->cap, rhs->x, rhs->z, rhs->f, rhs->v,
+    int rc;
+    {
+        // algorithmic bytes: the n*m output terms written, both operands read
+        const double term_bytes = 16.0 * lhs->as + 9.0;
+        ProfScope ps(lhs->ctx, "mul_raw", term_bytes * ((double)n * (double)m + (double)(n + m)));
+        rc = op_mul_raw(lhs->as, lhs->x, lhs->z, lhs->f, lhs->v, lhs->cap, rhs->x, rhs->z, rhs->f, rhs->v,
                         rhs->cap, out.x, out.z, out.f, out.v, out.cap, n, m, lhs->ctx->stream);
+    }
     if (rc != PLH_OK) {
         sum_free_storage(&out);
         return rc;
