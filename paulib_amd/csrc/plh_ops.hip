@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_mul_raw(
     const uint64_t* __restrict__ rx, const uint64_t* __restrict__ rz,
     const uint8_t* __restrict__ rf, const double* __restrict__ rv, int64_t rs,
     uint64_t* __restrict__ ox, uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
-    double* __restrict__ ov, int64_t os, int64_t n, uint32_t jblocks) {
+    double* __restrict__ ov, int64_t os, int64_t n, uint32_t jblocks, int reps) {
     const uint64_t i = blockIdx.x / jblocks;
     const uint64_t jb = blockIdx.x - i * jblocks;
     uint64_t bx[AS], bz[AS];
@@ -50,7 +50,8 @@ __global__ __launch_bounds__(kBlock) void k_mul_raw(
     const int64_t row = (int64_t)i * n;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int64_t j = (int64_t)jb * (4 * kBlock) + r * kBlock + threadIdx.x;
+        if (r >= reps) break;
+        const int64_t j = (int64_t)jb * (reps * kBlock) + r * kBlock + threadIdx.x;
         if (j >= n) break;
         uint64_t c1 = 0, c2 = 0;
 #pragma unroll
@@ -441,7 +442,10 @@ int op_mul_raw(int as, const uint64_t* lx, const uint64_t* lz, const uint8_t* lf
                const double* rv, int64_t rstride, uint64_t* ox, uint64_t* oz, uint8_t* of, double* ov,
                int64_t ostride, int64_t n, int64_t m, hipStream_t st) {
     if (n == 0 || m == 0) return PLH_OK;
-    const uint32_t jblocks = (uint32_t)((n + 4 * kBlock - 1) / (4 * kBlock));
+    // 4 x 256 lhs terms per workgroup; 1 x 256 when that would leave the
+    // chip underfilled (fewer than 8192 workgroups, e.g. 1000 x 1000)
+    const int reps = ((uint64_t)((n + 4 * kBlock - 1) / (4 * kBlock)) * (uint64_t)m >= 8192) ? 4 : 1;
+    const uint32_t jblocks = (uint32_t)((n + reps * kBlock - 1) / (reps * kBlock));
     const uint64_t grid = (uint64_t)jblocks * (uint64_t)m;
     if (grid > 0x7FFFFFFFull) {
         set_error("raw product grid too large");
@@ -451,7 +455,7 @@ int op_mul_raw(int as, const uint64_t* lx, const uint64_t* lz, const uint8_t* lf
     case ASV:                                                                                  \
         k_mul_raw<ASV><<<dim3((unsigned)grid), kBlock, 0, st>>>(as, lx, lz, lf, lv, lstride, rx, rz, \
                                                                rf, rv, rstride, ox, oz, of, ov,     \
-                                                               ostride, n, jblocks);               \
+                                                               ostride, n, jblocks, reps);         \
         break;
     switch (padded(as)) {
         PLH_MUL(1) PLH_MUL(2) PLH_MUL(4) PLH_MUL(8) PLH_MUL(16)

@@ -260,3 +260,14 @@ def test_run_program_brickwork(ctx, nq, depth, seed):
     assert_terms_equal(got, want)
     e_got, e_want = d.zero_state_expectation(), o.zero_state_expectation()
     assert abs(e_got - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum())
+
+
+@pytest.mark.parametrize("as_,nq,n,m", [(1, 64, 4096, 2100), (2, 100, 3000, 700), (8, 500, 1000, 100)])
+def test_multiply_raw_tiles(ctx, as_, nq, n, m):
+    """Both workgroup tiles of the raw product: 4 x 256 lhs terms when the grid
+    has at least 8192 workgroups, 1 x 256 below."""
+    A = W.random_words_sum(n, nq, 61, array_size=as_)
+    B = W.random_words_sum(m, nq, 62, array_size=as_)
+    d = dev(ctx, A)
+    d.multiply(dev(ctx, B))
+    assert_terms_equal(d.to_host(), oracle_product(A, B, False))
