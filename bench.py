@@ -315,6 +315,61 @@ def raw_products(ctx, reps: int):
     return out
 
 
+def collate_leg(ctx, reps: int):
+    """sort_and_combine of a materialised 1000 x m product on 100 qubits
+    (2 words per plane), timed alone as the reference's
+    bench/CollateBench.cpp times it; operands as random_pauli_sum (uniform
+    words, random flags)."""
+    from paulib_amd import PauliSum
+    from paulib_amd import workloads as W
+
+    A = W.random_words_sum(1000, 100, 42)
+    a = PauliSum.from_host(ctx, A)
+    out = {}
+    for m in (100, 500, 1000, 2000):
+        b = PauliSum.from_host(ctx, W.random_words_sum(m, 100, 123))
+        ts = []
+        for r in range(reps + 1):
+            d = a.copy()
+            d.multiply(b)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            d.sort_and_combine()  # synchronises
+            if r > 0:
+                ts.append(time.perf_counter() - t0)
+            d.close()
+        out[f"1000x{m}_q100"] = {"ms_best": min(ts) * 1e3, "ms_median": float(np.median(ts)) * 1e3,
+                                 "terms_per_s": 1000 * m / float(np.median(ts))}
+        log(f"collate 1000x{m}: {min(ts) * 1e3:.3f} ms best")
+        b.close()
+    a.close()
+    return out
+
+
+def propagation_trim_leg(ctx):
+    """propagate(Z_0, mixed-field Ising circuit n = 40, 10 layers) with
+    max_terms 2^14, 2^16, 2^18 (the reference's bench/PropagationBench.cpp):
+    ms per gate."""
+    from paulib_amd import PauliSum, TruncationOptions, propagate
+    from paulib_amd import workloads as W
+
+    O, circ = W.mixed_field_ising(40, 10)
+    out = {"workload": "mixed-field Ising, 40 qubits, 10 layers, 1190 rotations, observable Z_0"}
+    for log2m in (14, 16, 18):
+        for r in range(2):  # the first run warms the workspace
+            d = PauliSum.from_host(ctx, O)
+            t0 = time.perf_counter()
+            propagate(d, circ, TruncationOptions(max_terms=1 << log2m))
+            e = d.zero_state_expectation()
+            t = time.perf_counter() - t0
+            terms = len(d)
+            d.close()
+        out[f"trim_2^{log2m}"] = {"s": t, "ms_per_gate": 1e3 * t / len(circ), "final_terms": terms,
+                                  "expectation": e}
+        log(f"propagation trim 2^{log2m}: {t:.3f} s, {1e3 * t / len(circ):.3f} ms/gate, {terms} terms")
+    return out
+
+
 def cpu_baseline(A, B, target_s: float):
     """Reference multiply + canonicalize on a rhs slice sized for ~target_s."""
     import oracle
@@ -372,6 +427,25 @@ def cpu_baseline(A, B, target_s: float):
         if r >= 3:
             ts.append(time.perf_counter() - t0)
     c0["raw_product_1000x1000_q500_ms_best"] = min(ts) * 1e3
+    # sort_and_combine of the 1000 x 1000 product on 100 qubits (CollateBench)
+    Ac, Bc = W.random_words_sum(1000, 100, 42), W.random_words_sum(1000, 100, 123)
+    ts = []
+    for r in range(4):
+        a = oracle.CpuSum.from_host(kind, Ac)
+        a.multiply(oracle.CpuSum.from_host(kind, Bc))
+        t0 = time.perf_counter()
+        a.sort_and_combine()
+        if r >= 1:
+            ts.append(time.perf_counter() - t0)
+    c0["collate_1000x1000_q100_ms_best"] = min(ts) * 1e3
+    # propagate with max_terms 2^14 on the mixed-field Ising circuit (PropagationBench)
+    O, circ = W.mixed_field_ising(40, 10)
+    for log2m in (14, 16, 18):
+        o = oracle.CpuSum.from_host(kind, O)
+        t0 = time.perf_counter()
+        o.propagate(circ, None, 0.0, 1 << log2m)
+        t = time.perf_counter() - t0
+        c0[f"propagation_trim_2^{log2m}_ms_per_gate"] = 1e3 * t / len(circ)
     return {
         "c0": c0,
         "value": len(A) * m / t,
@@ -585,6 +659,9 @@ def main():
                                  "jobs_per_s": float(1e3 / lat.mean()), "expectation": e}
     if args.raw_reps > 0:
         result["raw_products"] = raw_products(ctx, args.raw_reps)
+        result["collate"] = collate_leg(ctx, 5)
+    if not args.no_propagation:
+        result["propagation_trim"] = propagation_trim_leg(ctx)
     if args.gate_log2_terms > 0:
         result["clifford_gates"] = clifford_gates(ctx, args.gate_log2_terms, 20)
     if args.batch_jobs > 0:

@@ -203,6 +203,29 @@ def mixed_gate_program(num_qubits: int, depth: int, seed: int) -> Program:
     return b.build()
 
 
+def mixed_field_ising(n: int = 40, layers: int = 10, seed: int = 7):
+    """Trotterised mixed-field Ising circuit of the reference's
+    bench/PropagationBench.cpp:30-42, as a rotation list (the angle is the
+    coefficient): per layer ZZ on (q, q+1), then X on every qubit, then Z on
+    every qubit, angles uniform on [-0.4, 0.4) (drawn here with numpy's PCG64,
+    not the reference's mt19937), and the observable Z_0."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    terms = []
+    for _ in range(layers):
+        for q in range(n - 1):
+            s = ["I"] * n
+            s[q] = s[q + 1] = "Z"
+            terms.append(("".join(s), float(rng.uniform(-0.4, 0.4))))
+        for a in "XZ":
+            for q in range(n):
+                s = ["I"] * n
+                s[q] = a
+                terms.append(("".join(s), float(rng.uniform(-0.4, 0.4))))
+    circ = HostSum.from_terms(terms, array_size=words_for(n), num_qubits=n)
+    O = observable(n, [{0: "Z"}])
+    return O, circ
+
+
 def observable(num_qubits: int, paulis: list[dict], coeffs=None) -> HostSum:
     a = words_for(num_qubits)
     terms = []
