@@ -123,7 +123,7 @@ class Program:
         return int((self.kind == 3).sum())
 
 
-class _ProgramBuilder:
+class ProgramBuilder:
     def __init__(self, nq: int):
         self.nq = nq
         self.a = words_for(nq)
@@ -131,6 +131,12 @@ class _ProgramBuilder:
 
     def clifford(self, kind: int, q0: int, q1: int = 0):
         self.ops.append((kind, q0, q1, [0] * self.a, [0] * self.a, 0.0))
+
+    def gate(self, name, c: int, t: int = 0):
+        """Any gate of the extended set by name ('sqrt_X', 'YC_Z', 'iSWAP', ...): kind 16 + id."""
+        from .gates import gate_id
+
+        self.clifford(16 + gate_id(name), c, t)
 
     def rotation(self, paulis: dict, theta: float):
         xw = [0] * self.a
@@ -157,7 +163,7 @@ class _ProgramBuilder:
 
 def brickwork_program(num_qubits: int, depth: int, seed: int) -> Program:
     rng = np.random.Generator(np.random.PCG64(seed))
-    b = _ProgramBuilder(num_qubits)
+    b = ProgramBuilder(num_qubits)
     for layer in range(depth):
         cl = rng.integers(0, 3, size=num_qubits)  # 0 = H, 1 = S, 2 = none
         for q in range(num_qubits):
@@ -183,7 +189,7 @@ def mixed_gate_program(num_qubits: int, depth: int, seed: int) -> Program:
     from .gates import SINGLE, TWO
 
     rng = np.random.Generator(np.random.PCG64(seed))
-    b = _ProgramBuilder(num_qubits)
+    b = ProgramBuilder(num_qubits)
     for layer in range(depth):
         for q in range(num_qubits):
             if rng.random() < 0.5:

@@ -135,7 +135,7 @@ def test_server_budget_admits_one_at_a_time(ctx):
 def test_server_edge_jobs(ctx):
     # empty program, Clifford-only program, empty observable
     O = W.observable(10, [{0: "X", 3: "Y"}, {2: "Z"}], [0.5, -2.0])
-    b = W._ProgramBuilder(10)
+    b = W.ProgramBuilder(10)
     empty = b.build()
     b.clifford(0, 0)
     b.clifford(2, 0, 3)
