@@ -84,3 +84,19 @@ def test_apply_truncation_and_programs_with_max_terms(ctx):
         o = cpu("oracle", O)
         o.run_program(prog, 8, 1e-5, mt)
         assert_terms_equal(d.to_host(), o.to_host())
+
+
+@pytest.mark.parametrize("n,layers,log2m", [(12, 4, 6), (20, 5, 8), (16, 6, 10)])
+def test_propagate_mixed_field_ising_with_max_terms(ctx, n, layers, log2m):
+    """propagate with only max_terms set (the reference's
+    bench/PropagationBench.cpp workload, smaller), against the oracle."""
+    from paulib_amd import propagate
+
+    O, circ = W.mixed_field_ising(n, layers)
+    d = PauliSum.from_host(ctx, O)
+    propagate(d, circ, TruncationOptions(max_terms=1 << log2m))
+    o = cpu("oracle", O)
+    o.propagate(circ, None, 0.0, 1 << log2m)
+    assert_terms_equal(d.to_host(), o.to_host())
+    e_got, e_want = d.zero_state_expectation(), o.zero_state_expectation()
+    assert abs(e_got - e_want) <= 1e-12 * max(1.0, np.abs(o.to_host().value).sum())
