@@ -113,3 +113,14 @@ def test_propagate_rotation_list():
         s.propagate(rot, 6, 1e-4)
         res.append(s.to_host())
     assert_terms_equal(res[0], res[1])
+
+
+@pytest.mark.parametrize("n,layers,log2m", [(12, 4, 6), (20, 5, 8), (16, 6, 10), (24, 4, 9)])
+def test_propagate_mixed_field_ising_trim_matches_reference(n, layers, log2m):
+    """The reference's bench/PropagationBench.cpp circuit (smaller) with only
+    max_terms set: the oracle's propagate equals the reference's bit for bit."""
+    O, circ = W.mixed_field_ising(n, layers)
+    a, b = cpu("oracle", O), cpu("reference", O)
+    a.propagate(circ, None, 0.0, 1 << log2m)
+    b.propagate(circ, None, 0.0, 1 << log2m)
+    assert_terms_equal(a.to_host(), b.to_host())
