@@ -9,11 +9,13 @@ sequential jobs of Z_0 + Z_31 through the depth-20 64-qubit brickwork
 (truncation |c| < 1e-4, weight > 8) on the same device, p50/p99 reported.
 Batched products: 512 uneven independent products (C3 distribution) in a
 few collations, against the plain drop-in looped over the same jobs.
-Serving: an open-loop Poisson stream of propagation jobs (weight-1..3
+Serving: open-loop Poisson streams of propagation jobs (weight-1..3
 observables on 32..128 qubits, brickwork depth 5..40, cutoffs 1e-5..1e-3)
-through the continuous-batching server (plh_server), p50/p99 latency from
-arrival and jobs/s, beside the plain drop-in's jobs/s on the stream's first
-jobs run one after another.
+through the continuous-batching server (plh_server), once above capacity
+and once at 70% of the measured capacity: p50/p99 latency from arrival and
+jobs/s, beside the plain drop-in's jobs/s on 100 jobs drawn at random from
+the stream and run one after another (their results compared with the
+served ones).
 
 roofline: the kernel with the largest share of the timed region, timed
 with HIP events on the engine's stream during the timed steps; achieved =
@@ -63,30 +65,16 @@ def kernel_profile(ctx):
     return out
 
 
-def serving(ctx, n_jobs: int, rate: float, n_plain: int, max_jobs: int):
-    """Open-loop Poisson stream of propagation jobs through the batched server,
-    against the plain drop-in (run_program per job) on the stream's first jobs."""
-    from paulib_amd import PauliSum, TruncationOptions, run_program
-    from paulib_amd import workloads as W
+def serve_stream(ctx, jobs, max_jobs: int, label: str):
+    """An open-loop stream through the batched server: jobs are submitted at
+    their arrival times while the server steps; per job its latency from
+    arrival, its expectation and final term count."""
+    from paulib_amd import TruncationOptions
     from paulib_amd.serve import Server
 
-    jobs = W.serving_jobs(n_jobs, rate, seed=9)
-
-    def opts(cut):
-        return TruncationOptions(max_weight=8, coefficient_cutoff=cut)
-
-    t_plain = []
-    for k, (_, O, prog, cut) in enumerate(jobs[:n_plain]):
-        d = PauliSum.from_host(ctx, O)
-        t0 = time.perf_counter()
-        run_program(d, prog, opts(cut))
-        d.zero_state_expectation()
-        t_plain.append(time.perf_counter() - t0)
-        log(f"plain job {k}: {prog.num_rotations()} rotations on {O.num_qubits} qubits, {t_plain[-1]:.3f} s")
-        d.close()
-    plain_rate = len(t_plain) / sum(t_plain)
-
-    arrival, finish = {}, {}
+    n_jobs = len(jobs)
+    arrival, finish, res = {}, {}, {}
+    order = {}
     in_flight_max = 0
     with Server(ctx, array_size=2, max_jobs=max_jobs) as srv:
         t0 = time.perf_counter()
@@ -96,36 +84,86 @@ def serving(ctx, n_jobs: int, rate: float, n_plain: int, max_jobs: int):
             now = time.perf_counter() - t0
             while i < n_jobs and jobs[i][0] <= now:
                 _, O, prog, cut = jobs[i]
-                h = srv.submit(O, prog, opts(cut))
+                h = srv.submit(O, prog, TruncationOptions(max_weight=8, coefficient_cutoff=cut))
                 arrival[h] = t0 + jobs[i][0]
+                order[h] = i
                 i += 1
             running, queued = srv.step()
             in_flight_max = max(in_flight_max, running)
             t = time.perf_counter()
             for r in srv.poll():
                 finish[r.handle] = t
+                res[order[r.handle]] = (r.expectation, r.terms)
             if running == 0 and queued == 0 and i < n_jobs:
                 time.sleep(max(0.0, jobs[i][0] - (time.perf_counter() - t0)))
             if t - last_log > 5.0:
-                log(f"serving: {len(finish)}/{n_jobs} done, {running} running, {queued} queued, "
+                log(f"serving {label}: {len(finish)}/{n_jobs} done, {running} running, {queued} queued, "
                     f"batch {srv.batch_terms} terms, step {srv.steps}")
                 last_log = t
         steps = srv.steps
     lat = np.array([finish[h] - arrival[h] for h in finish]) * 1e3
     span = max(finish.values()) - (t0 + jobs[0][0])
-    rate_served = n_jobs / span
-    rots = [p.num_rotations() for _, _, p, _ in jobs]
-    log(f"serving: {n_jobs} jobs in {span:.2f} s = {rate_served:.2f} jobs/s; plain loop {plain_rate:.2f} jobs/s; "
-        f"p50 {np.percentile(lat, 50):.0f} ms, p99 {np.percentile(lat, 99):.0f} ms")
+    return {"jobs": n_jobs, "jobs_per_s": n_jobs / span, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "steps": steps, "max_running": in_flight_max}, res
+
+
+def serving(ctx, n_jobs: int, rate: float, n_plain: int, max_jobs: int, sustain_frac: float, sustain_jobs: int):
+    """C4: an open-loop Poisson stream of propagation jobs through the
+    batched server at `rate` (above capacity: all slots busy, which measures
+    the server's throughput), the same kind of stream at sustain_frac of the
+    measured capacity (latency at a load the server keeps up with), and the
+    plain drop-in (run_program per job, one after another) on n_plain jobs
+    drawn at random from the first stream, whose expectations and term
+    counts are compared with the served ones."""
+    from paulib_amd import PauliSum, TruncationOptions, run_program
+    from paulib_amd import workloads as W
+
+    jobs = W.serving_jobs(n_jobs, rate, seed=9)
+    over, served = serve_stream(ctx, jobs, max_jobs, "overload")
+    log(f"serving at {rate:g}/s: {over['jobs_per_s']:.2f} jobs/s, p50 {over['p50_ms']:.0f} ms, "
+        f"p99 {over['p99_ms']:.0f} ms")
+    rate_s = sustain_frac * over["jobs_per_s"]
+    jobs_s = W.serving_jobs(sustain_jobs, rate_s, seed=10)
+    sus, _ = serve_stream(ctx, jobs_s, max_jobs, "sustained")
+    log(f"serving at {rate_s:.2f}/s: {sus['jobs_per_s']:.2f} jobs/s, p50 {sus['p50_ms']:.0f} ms, "
+        f"p99 {sus['p99_ms']:.0f} ms")
+
+    rng = np.random.Generator(np.random.PCG64(2024))
+    sample = sorted(rng.choice(n_jobs, size=min(n_plain, n_jobs), replace=False).tolist())
+    t_plain, max_diff, term_mismatch = [], 0.0, 0
+    for k in sample:
+        _, O, prog, cut = jobs[k]
+        d = PauliSum.from_host(ctx, O)
+        t0 = time.perf_counter()
+        run_program(d, prog, TruncationOptions(max_weight=8, coefficient_cutoff=cut))
+        e = d.zero_state_expectation()
+        t_plain.append(time.perf_counter() - t0)
+        e_s, n_s = served[k]
+        max_diff = max(max_diff, abs(e - e_s))
+        term_mismatch += int(n_s != len(d))
+        d.close()
+    plain_rate = len(t_plain) / sum(t_plain)
+    rots_all = [p.num_rotations() for _, _, p, _ in jobs]
+    rots_smp = [jobs[k][2].num_rotations() for k in sample]
+    # the plain loop's rate over the whole stream, from its per-rotation time on the sample
+    plain_rate_stream = plain_rate * float(np.mean(rots_smp)) / float(np.mean(rots_all))
+    log(f"serving: plain loop {plain_rate:.2f} jobs/s on {len(sample)} sampled jobs "
+        f"({plain_rate_stream:.2f} scaled to the stream's rotations); served vs plain: max |<O> diff| {max_diff:.2e}, "
+        f"{term_mismatch} term-count mismatches")
     return {
         "workload": (f"{n_jobs} propagation jobs, Poisson arrivals at {rate:g}/s: weight-1..3 observable on "
                      "32..128 qubits, brickwork depth 5..40, |c| cutoff 1e-5..1e-3 (log-uniform), weight <= 8"),
         "mode": "continuous batching (plh_server), open loop",
-        "jobs": n_jobs, "rotations_mean": float(np.mean(rots)), "rotations_max": int(max(rots)),
-        "jobs_per_s": rate_served, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-        "steps": steps, "max_running": in_flight_max,
-        "plain_jobs_per_s": plain_rate, "plain_sample_jobs": len(t_plain),
-        "speedup_vs_plain": rate_served / plain_rate,
+        "jobs": n_jobs, "rotations_mean": float(np.mean(rots_all)), "rotations_max": int(max(rots_all)),
+        "jobs_per_s": over["jobs_per_s"], "p50_ms": over["p50_ms"], "p99_ms": over["p99_ms"],
+        "steps": over["steps"], "max_running": over["max_running"],
+        "sustained": {"rate_per_s": rate_s, "fraction_of_capacity": sustain_frac, **sus},
+        "plain_jobs_per_s": plain_rate, "plain_sample_jobs": len(sample),
+        "plain_sample_rotations_mean": float(np.mean(rots_smp)),
+        "plain_jobs_per_s_stream": plain_rate_stream,
+        "speedup_vs_plain": over["jobs_per_s"] / plain_rate,
+        "speedup_vs_plain_stream": over["jobs_per_s"] / plain_rate_stream,
+        "served_vs_plain": {"max_abs_expectation_diff": max_diff, "term_count_mismatches": term_mismatch},
     }
 
 
@@ -370,8 +408,21 @@ def propagation_trim_leg(ctx):
     return out
 
 
-def cpu_baseline(A, B, target_s: float):
-    """Reference multiply + canonicalize on a rhs slice sized for ~target_s."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(A, B, target_s: float, gpu_c2=None, gpu_trim=None):
+    """Reference multiply + canonicalize on a rhs slice sized for ~target_s,
+    and the reference's propagation legs in full (the depth-20 brickwork
+    configuration and the PropagationBench trims), each compared with the
+    GPU's result of the same workload when given."""
     import oracle
 
     kind = "reference" if oracle.available("reference") else "oracle"
@@ -438,20 +489,43 @@ def cpu_baseline(A, B, target_s: float):
         if r >= 1:
             ts.append(time.perf_counter() - t0)
     c0["collate_1000x1000_q100_ms_best"] = min(ts) * 1e3
-    # propagate with max_terms 2^14 on the mixed-field Ising circuit (PropagationBench)
+    # propagate with max_terms 2^14..2^18 on the mixed-field Ising circuit (PropagationBench)
     O, circ = W.mixed_field_ising(40, 10)
     for log2m in (14, 16, 18):
         o = oracle.CpuSum.from_host(kind, O)
         t0 = time.perf_counter()
         o.propagate(circ, None, 0.0, 1 << log2m)
-        t = time.perf_counter() - t0
-        c0[f"propagation_trim_2^{log2m}_ms_per_gate"] = 1e3 * t / len(circ)
+        tp = time.perf_counter() - t0
+        c0[f"propagation_trim_2^{log2m}_ms_per_gate"] = 1e3 * tp / len(circ)
+        g = (gpu_trim or {}).get(f"trim_2^{log2m}")
+        if g:
+            e = o.zero_state_expectation()
+            c0[f"propagation_trim_2^{log2m}_agrees"] = {
+                "terms_cpu": len(o), "terms_gpu": g["final_terms"], "expectation_cpu": e,
+                "expectation_gpu": g["expectation"], "abs_diff": abs(e - g["expectation"])}
+    # the depth-20 brickwork propagation (Z0 + Z31, 64 qubits) in full
+    Oc, prog, tr = W.brickwork_q64_d20()
+    o = oracle.CpuSum.from_host(kind, Oc)
+    t0 = time.perf_counter()
+    o.run_program(prog, tr["max_weight"], tr["cutoff"])
+    e = o.zero_state_expectation()
+    tc2 = time.perf_counter() - t0
+    c2 = {"ms": tc2 * 1e3, "terms": len(o), "expectation": e}
+    if gpu_c2:
+        c2.update(terms_gpu=gpu_c2["terms"], expectation_gpu=gpu_c2["expectation"],
+                  abs_diff=abs(e - gpu_c2["expectation"]), speedup_gpu=tc2 * 1e3 / gpu_c2["p50_ms"])
+    log(f"cpu baseline: C2 {tc2:.1f} s, {len(o)} terms, <O> = {e:.10e}")
     return {
         "c0": c0,
+        "c2": c2,
         "value": len(A) * m / t,
         "unit": "collated product terms/s",
         "cores": int(threads),
         "kind": "reference" if kind == "reference" else "port",
+        "cpu": cpu_model(),
+        "build": ("reference headers via oracle/ref_bridge.cpp: g++ -std=c++20 -O3 -march=x86-64-v3 -ffast-math "
+                  "-fopenmp (the reference's flags, -march=native replaced so the library built elsewhere runs here)"
+                  if kind == "reference" else "oracle/pauli_oracle.c: gcc -O2 -ffp-contract=off, one thread"),
         "sample": f"{len(A)} x {m} rhs slice of the 65536 x 65536 q64 product (multiply + canonicalize), {t:.2f} s",
     }
 
@@ -466,7 +540,9 @@ def main():
     ap.add_argument("--prop-jobs", type=int, default=5)
     ap.add_argument("--serve-jobs", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=40.0)
-    ap.add_argument("--serve-plain", type=int, default=8)
+    ap.add_argument("--serve-plain", type=int, default=100)
+    ap.add_argument("--serve-sustain", type=float, default=0.7)
+    ap.add_argument("--serve-sustain-jobs", type=int, default=200)
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--batch-jobs", type=int, default=512)
     ap.add_argument("--batch-pairs", type=int, default=1 << 31)
@@ -648,7 +724,8 @@ def main():
             run_program(d, prog, TruncationOptions(max_weight=tr["max_weight"], coefficient_cutoff=tr["cutoff"]))
             e = d.zero_state_expectation()
             t = time.perf_counter() - t0
-            log(f"propagation job {j}: {t * 1e3:.1f} ms, {len(d)} terms, <O> = {e:.10f}")
+            terms_c2 = len(d)
+            log(f"propagation job {j}: {t * 1e3:.1f} ms, {terms_c2} terms, <O> = {e:.10f}")
             if j > 0:  # the first job warms the workspace
                 lat.append(t)
             d.close()
@@ -656,7 +733,7 @@ def main():
         result["propagation"] = {"workload": "brickwork_q64_d20 (Z0+Z31, depth 20, w<=8, |c|>=1e-4)",
                                  "mode": "sequential jobs, plain drop-in", "jobs": len(lat),
                                  "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                                 "jobs_per_s": float(1e3 / lat.mean()), "expectation": e}
+                                 "jobs_per_s": float(1e3 / lat.mean()), "expectation": e, "terms": terms_c2}
     if args.raw_reps > 0:
         result["raw_products"] = raw_products(ctx, args.raw_reps)
         result["collate"] = collate_leg(ctx, 5)
@@ -667,10 +744,12 @@ def main():
     if args.batch_jobs > 0:
         result["batched_products"] = batched_products(ctx, args.batch_jobs, args.batch_pairs, args.solo_pairs)
     if args.serve_jobs > 0:
-        result["serving"] = serving(ctx, args.serve_jobs, args.serve_rate, args.serve_plain, args.serve_slots)
+        result["serving"] = serving(ctx, args.serve_jobs, args.serve_rate, args.serve_plain, args.serve_slots,
+                                    args.serve_sustain, args.serve_sustain_jobs)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds, result.get("propagation"),
+                                              result.get("propagation_trim"))
     print(json.dumps(result), flush=True)
 
 

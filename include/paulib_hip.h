@@ -38,7 +38,8 @@ enum {
     PLH_ERR_SIZE = -3,  /* size limit exceeded (see each call) */
     PLH_ERR_HIP = -4,   /* HIP runtime error */
     PLH_ERR_OOM = -5,   /* device allocation failed or admission refused */
-    PLH_ERR_NOGPU = -6  /* no usable GPU */
+    PLH_ERR_NOGPU = -6, /* no usable GPU */
+    PLH_ERR_STATE = -7  /* object unusable after an earlier failure (see the call) */
 };
 
 typedef struct plh_ctx plh_ctx; /* device + stream + scratch */
@@ -300,7 +301,10 @@ int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_
                       const uint64_t* px, const uint64_t* pz, const double* theta,
                       int64_t max_weight, double cutoff, int64_t* handle);
 /* One step: admit, advance every running job, finish jobs whose program is
- * done.  Returns the running and queued job counts. */
+ * done.  Returns the running and queued job counts.  A job that cannot be
+ * admitted (its upload fails) stays queued and the call returns the error;
+ * a failure in the step's device work leaves the batch with half-applied
+ * gates, so every later step and submit returns PLH_ERR_STATE. */
 int plh_server_step(plh_server* s, int* running, int* queued);
 /* Finished jobs in completion order: handle, expectation, final term count,
  * latency from submission (seconds). */

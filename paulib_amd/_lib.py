@@ -19,6 +19,7 @@ ERRORS = {
     -4: "PLH_ERR_HIP",
     -5: "PLH_ERR_OOM",
     -6: "PLH_ERR_NOGPU",
+    -7: "PLH_ERR_STATE",
 }
 
 

@@ -75,20 +75,5 @@ def build_cpp_tests() -> Path:
     return CPP_TEST_BIN
 
 
-COPY_PROBE = HERE / "probe" / "copy_probe.hip"
-COPY_PROBE_BIN = OUT / "copy_probe"
-
-
-def build_probes() -> Path:
-    """copy_probe: the plain copy whose counters scale rocprofv3 --pmc reads."""
-    OUT.mkdir(parents=True, exist_ok=True)
-    cmd = [HIPCC, *[f for f in CFLAGS if f != "-fPIC"], str(COPY_PROBE), "-o", str(COPY_PROBE_BIN)]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"copy probe build failed:\n{r.stdout}\n{r.stderr}")
-    return COPY_PROBE_BIN
-
-
 if __name__ == "__main__":
     build()
-    build_probes()

@@ -66,6 +66,9 @@ class DeviceHeap {
     void trim();
     size_t bytes_held() const;
     size_t bytes_live() const { return live_; }
+    // Whether [p, p + n) lies inside one live range (a check for host-side
+    // bookkeeping tests: device work may only touch live ranges).
+    bool holds(const void* p, size_t n) const;
     static constexpr size_t kChunk = (size_t)2 << 30;
 
    private:

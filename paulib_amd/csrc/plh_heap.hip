@@ -52,6 +52,13 @@ int DeviceHeap::reserve() {
 
 size_t DeviceHeap::bytes_held() const { return (mapped_ + pool_.size()) * kChunk; }
 
+bool DeviceHeap::holds(const void* p, size_t n) const {
+    const uintptr_t a = (uintptr_t)p;
+    for (const auto& kv : used_)
+        if (a >= kv.first && a + n <= kv.first + kv.second) return true;
+    return false;
+}
+
 void DeviceHeap::add_live(uintptr_t lo, uintptr_t hi, bool add) {
     for (size_t i = slot_of(lo); i <= slot_of(hi - 1); ++i) {
         const uintptr_t c0 = (uintptr_t)va_ + i * kChunk;
@@ -239,14 +246,26 @@ int Workspace::get(int slot, size_t bytes, void** out) {
     Buf& b = bufs_[(size_t)slot];
     if (bytes == 0) bytes = 1;
     if (b.n < bytes) {
-        heap_->free(b.p);  // stream-ordered: later work on the stream may reuse it
-        b.p = nullptr;
-        b.n = 0;
         // Grow geometrically so a sum that grows step by step (propagation)
         // moves each slot only a few times: x2 (at least 1 MiB) up to 256 MiB,
-        // then by an eighth.
+        // then by an eighth.  The new range is taken before the old one is
+        // released, so a failed request leaves the slot as it was; only when
+        // the two cannot coexist is the old one released first (its contents
+        // are not kept either way).  Release is stream-ordered: later work on
+        // the stream may reuse the range.
         const size_t want = bytes <= kSlotDoubleMax ? std::max<size_t>(2 * bytes, (size_t)1 << 20) : bytes + bytes / 8;
-        PLH_TRY(heap_->alloc(want, &b.p));
+        void* np = nullptr;
+        const int rc = heap_->alloc(want, &np);
+        if (rc != PLH_OK) {
+            if (!b.p) return rc;
+            heap_->free(b.p);
+            b.p = nullptr;
+            b.n = 0;
+            PLH_TRY(heap_->alloc(want, &np));
+        } else {
+            heap_->free(b.p);
+        }
+        b.p = np;
         b.n = want;
     }
     *out = b.p;

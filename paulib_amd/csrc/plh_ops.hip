@@ -1018,24 +1018,4 @@ int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out) {
 }
 
 
-int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src) {
-    hipStream_t st = ctx->stream;
-    if (src->n == 0) return PLH_OK;
-    const int64_t n0 = dst->n;
-    PLH_TRY(sum_reserve(dst, n0 + src->n, n0));
-    for (int k = 0; k < dst->as; ++k) {
-        PLH_HIP(hipMemcpyAsync(dst->x + (int64_t)k * dst->cap + n0, src->x + (int64_t)k * src->cap,
-                               (size_t)src->n * 8, hipMemcpyDeviceToDevice, st));
-        PLH_HIP(hipMemcpyAsync(dst->z + (int64_t)k * dst->cap + n0, src->z + (int64_t)k * src->cap,
-                               (size_t)src->n * 8, hipMemcpyDeviceToDevice, st));
-    }
-    PLH_HIP(hipMemcpyAsync(dst->f + n0, src->f, (size_t)src->n, hipMemcpyDeviceToDevice, st));
-    PLH_HIP(hipMemcpyAsync(dst->v + n0, src->v, (size_t)src->n * 8, hipMemcpyDeviceToDevice, st));
-    if (dst->has_job && src->has_job)
-        PLH_HIP(hipMemcpyAsync(dst->j + n0, src->j, (size_t)src->n * 4, hipMemcpyDeviceToDevice, st));
-    dst->n = n0 + src->n;
-    dst->canonical = false;
-    return PLH_OK;
-}
-
 }  // namespace plh

@@ -1,0 +1,216 @@
+// plh_sched.hpp -- host-side bookkeeping of the batched propagation server:
+// the job queue, the slots and their states, admission by HBM budget, and
+// the per-step tables (Clifford ops up to each job's next rotation, that
+// rotation's fast-path kind, the step's truncation).  Pure C++ with no HIP
+// calls, so the same code runs in the server (plh_serve.hip) and in the
+// CPU replay of its bookkeeping (tests/cpp/sched_sim.cpp).
+//
+// A step is planned without changing any job (plan), the device work runs,
+// and only then is the plan committed (commit): a step whose device work
+// fails leaves every job where it was, and the server refuses further
+// steps, since the batch may hold half-applied gates.
+//
+// Per job the program order is that of propagate()
+// (include/PauliPropagation.h:53-63): ops from the last to the first, a
+// rotation (evolve, include/PauliSumOperations.h:106-183) followed by
+// apply_truncation (include/PauliPropagation.h:35-47); Clifford ops alone
+// are not followed by a truncation.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <deque>
+#include <vector>
+
+namespace plh {
+namespace sched {
+
+// Program op kinds: 0 H, 1 S, 2 CNOT, 3 rotation, kGateBase + id for a gate
+// of the table set.
+constexpr int kGateBase = 16;
+constexpr int kGates = 21;
+constexpr int kFirstTwoQubit = 10;
+
+struct Job {
+    int64_t handle = -1;
+    int job_as = 1;
+    int64_t n0 = 0;
+    std::vector<uint64_t> ox, oz;  // observable planes, x[k*n0 + t]
+    std::vector<uint8_t> of;
+    std::vector<double> ov;
+    std::vector<int32_t> kind;
+    std::vector<int64_t> q0, q1;
+    std::vector<uint64_t> px, pz;  // per op: job_as words
+    std::vector<double> theta;
+    int64_t next = -1;  // next op to apply (Heisenberg order: last to first)
+    int64_t max_weight = -1;
+    double cutoff = 0.0;
+    double t_submit = 0.0;
+};
+
+enum SlotState : int { kFree = 0, kRunning = 1, kDraining = 2 };
+
+// One step's device tables (one entry per slot) and its effect on the jobs.
+struct StepPlan {
+    std::vector<uint32_t> off, ops;  // ops[off[i] .. off[i+1]) are slot i's Clifford ops
+    std::vector<uint8_t> rk, rpsf, alive;
+    std::vector<double> rc, rs, eps;
+    std::vector<int64_t> maxw;
+    std::vector<uint64_t> rp;     // per slot: x[AS] then z[AS] of its rotation
+    std::vector<uint64_t> p_or;   // OR of the general-path rotations' words (x then z)
+    std::vector<int64_t> next;    // per slot: the job's next op after this step
+    std::vector<int> finishing;   // slots whose job ends with this step
+    bool any_general = false;
+};
+
+inline bool close_one(double v) { return std::fabs(v - 1.0) <= 1e-14; }  // isCloseOne, :19-21
+
+// evolve's path for (cos, sin) (include/PauliSumOperations.h:121-147): 0 no
+// change, 1 sign flip, 2 sign flip and * ps, 3 * ps, 4 general.
+inline uint8_t rotation_kind_of(double c, double s) {
+    if (close_one(c)) return 0;
+    if (close_one(-c)) return 1;
+    if (close_one(s)) return 2;
+    if (close_one(-s)) return 3;
+    return 4;
+}
+
+// Device bytes per term in flight: the batch (x, z, v, j, f), the collation
+// output, its two id arrays (an id and 32 carried key bits each) and packed
+// key records.
+inline double bytes_per_term(int as) { return 2.0 * (16.0 * as + 13.0) + 16.0 + (2.0 * as + 2.0) * 8.0; }
+
+class Scheduler {
+   public:
+    Scheduler(int as, int slots, int64_t budget)
+        : as_(as), slots_(slots), budget_(budget), active_((size_t)slots), state_((size_t)slots, kFree) {}
+
+    int as() const { return as_; }
+    int slots() const { return slots_; }
+    int64_t budget() const { return budget_; }
+    std::deque<Job>& queue() { return queue_; }
+    const std::deque<Job>& queue() const { return queue_; }
+    Job& job(int slot) { return active_[(size_t)slot]; }
+    int state(int slot) const { return state_[(size_t)slot]; }
+    int running() const {
+        int r = 0;
+        for (int s : state_) r += s == kRunning;
+        return r;
+    }
+
+    // The slot the queue's first job would be admitted to, or -1: a free
+    // slot, and (unless nothing runs) room for the batch plus 64 times the
+    // new job's terms within the budget.
+    int admission_slot(int64_t terms_in_flight) const {
+        if (queue_.empty()) return -1;
+        int slot = -1;
+        for (int i = 0; i < slots_ && slot < 0; ++i)
+            if (state_[(size_t)i] == kFree) slot = i;
+        if (slot < 0) return -1;
+        const Job& q = queue_.front();
+        const double need =
+            ((double)terms_in_flight + 64.0 * (double)std::max<int64_t>(q.n0, 1)) * bytes_per_term(as_);
+        if (running() > 0 && need > (double)budget_) return -1;
+        return slot;
+    }
+    // The queue's first job, whose terms the batch now holds, runs in slot.
+    void admitted(int slot) {
+        active_[(size_t)slot] = std::move(queue_.front());
+        queue_.pop_front();
+        state_[(size_t)slot] = kRunning;
+    }
+
+    // This step's tables; no job changes.
+    void plan(StepPlan* p) const {
+        const size_t S = (size_t)slots_, A = (size_t)as_;
+        p->off.assign(S + 1, 0);
+        p->ops.clear();
+        p->rk.assign(S, 0);
+        p->rpsf.assign(S, 1);  // program generators carry flags 0, so ps has flags 1
+        p->alive.assign(S, 0);
+        p->rc.assign(S, 0.0);
+        p->rs.assign(S, 0.0);
+        p->eps.assign(S, 0.0);
+        p->maxw.assign(S, -1);
+        p->rp.assign(S * 2 * A, 0);
+        p->p_or.assign(2 * A, 0);
+        p->next.assign(S, -1);
+        p->finishing.clear();
+        p->any_general = false;
+        for (size_t i = 0; i < S; ++i) {
+            p->off[i] = (uint32_t)p->ops.size();
+            if (state_[i] != kRunning) continue;
+            p->alive[i] = 1;
+            const Job& j = active_[i];
+            int64_t next = j.next;
+            while (next >= 0 && j.kind[(size_t)next] != 3) {
+                const int64_t l = next--;
+                const uint32_t k = (uint32_t)j.kind[(size_t)l];
+                if (k >= (uint32_t)kGateBase) {  // table gate: qubits < 1024 (at most 16 words)
+                    const uint32_t g = k - (uint32_t)kGateBase;
+                    const uint32_t t = g >= (uint32_t)kFirstTwoQubit ? (uint32_t)j.q1[(size_t)l] : 0u;
+                    p->ops.push_back((3u << 28) | (g << 20) | ((uint32_t)j.q0[(size_t)l] << 10) | t);
+                } else {
+                    p->ops.push_back((k << 28) | ((uint32_t)j.q0[(size_t)l] << 14) |
+                                     (k == 2 ? (uint32_t)j.q1[(size_t)l] : 0u));
+                }
+            }
+            if (next >= 0) {
+                const int64_t l = next--;
+                const double c = std::cos(j.theta[(size_t)l]), sn = std::sin(j.theta[(size_t)l]);
+                const uint8_t kind = rotation_kind_of(c, sn);
+                p->rk[i] = kind;
+                const size_t ja = (size_t)j.job_as;
+                if (kind == 4) {
+                    p->any_general = true;
+                    for (size_t k = 0; k < ja; ++k) {
+                        p->p_or[k] |= j.px[(size_t)l * ja + k];
+                        p->p_or[A + k] |= j.pz[(size_t)l * ja + k];
+                    }
+                }
+                p->rc[i] = c;
+                p->rs[i] = sn;
+                for (size_t k = 0; k < ja; ++k) {
+                    p->rp[i * 2 * A + k] = j.px[(size_t)l * ja + k];
+                    p->rp[i * 2 * A + A + k] = j.pz[(size_t)l * ja + k];
+                }
+                p->maxw[i] = j.max_weight;  // apply_truncation follows every rotation
+                p->eps[i] = j.cutoff;
+            }
+            p->next[i] = next;
+            if (next < 0) p->finishing.push_back((int)i);
+        }
+        p->off[S] = (uint32_t)p->ops.size();
+    }
+
+    // The step's device work is done: draining slots (their terms left in
+    // this step's collation) become free, every running job moves to its
+    // planned next op, and finishing jobs' slots start draining.  Returns
+    // true when nothing runs any more (the batch can then be emptied).
+    bool commit(const StepPlan& p) {
+        for (auto& s : state_)
+            if (s == kDraining) s = kFree;
+        for (size_t i = 0; i < (size_t)slots_; ++i)
+            if (state_[i] == kRunning) active_[i].next = p.next[i];
+        for (int slot : p.finishing) {
+            state_[(size_t)slot] = kDraining;
+            active_[(size_t)slot] = Job();
+        }
+        if (running() == 0) {
+            for (auto& s : state_) s = kFree;
+            return true;
+        }
+        return false;
+    }
+
+   private:
+    int as_, slots_;
+    int64_t budget_;
+    std::deque<Job> queue_;
+    std::vector<Job> active_;  // by slot
+    std::vector<int> state_;
+};
+
+}  // namespace sched
+}  // namespace plh

@@ -31,6 +31,7 @@
 
 #include "plh_block.hpp"
 #include "plh_engine.hpp"
+#include "plh_sched.hpp"
 #include "plh_sources.hpp"
 
 namespace plh {
@@ -238,26 +239,9 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
 
 inline unsigned blocks_for(uint64_t n, uint64_t per) { return (unsigned)((n + per - 1) / per); }
 
-bool close_one(double v) { return std::fabs(v - 1.0) <= 1e-14; }  // isCloseOne, :19-21
-
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-struct Job {
-    int64_t handle = -1;
-    int job_as = 1;
-    int64_t n0 = 0;
-    std::vector<uint64_t> ox, oz;  // observable planes, x[k*n0 + t]
-    std::vector<uint8_t> of;
-    std::vector<double> ov;
-    std::vector<int32_t> kind;
-    std::vector<int64_t> q0, q1;
-    std::vector<uint64_t> px, pz;  // per op: job_as words
-    std::vector<double> theta;
-    int64_t next = -1;  // next op to apply (Heisenberg order: last to first)
-    int64_t max_weight = -1;
-    double cutoff = 0.0;
-    double t_submit = 0.0;
-};
+using sched::Job;
 
 struct Result {
     int64_t handle = -1;
@@ -275,13 +259,9 @@ struct Result {
 struct plh_server {
     plh_ctx* ctx = nullptr;
     int as = 2;
-    int slots = 0;
-    int64_t budget = 0;
     bool keep_terms = false;
     plh_sum* batch = nullptr;  // job-plane sum holding every job in flight
-    std::deque<plh::Job> queue;
-    std::vector<plh::Job> active;  // by slot
-    std::vector<int> state;        // 0 free, 1 running, 2 draining (its terms leave at the next collation)
+    plh::sched::Scheduler* sched = nullptr;
     std::deque<plh::Result> done;
     std::unordered_map<int64_t, plh::Result> kept;
     int64_t next_handle = 0;
@@ -290,6 +270,10 @@ struct plh_server {
     std::vector<uint8_t> h_step;
     std::chrono::steady_clock::time_point t0;
     int64_t steps = 0;
+    // set when a step's device work failed part-way: the batch may hold
+    // half-applied gates, so the server refuses further steps
+    bool failed = false;
+    std::string failure;
 };
 
 namespace plh {
@@ -298,11 +282,6 @@ namespace {
 double now_s(const plh_server* s) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
 }
-
-// Device bytes per term in flight: the batch (x, z, v, j, f), the collation
-// output, its two id arrays (an id and 32 carried key bits each) and packed
-// key records.
-double bytes_per_term(int as) { return 2.0 * (16.0 * as + 13.0) + 16.0 + (2.0 * as + 2.0) * 8.0; }
 
 bool keys_distinct(const Job& j) {
     const int a = j.job_as;
@@ -317,7 +296,8 @@ bool keys_distinct(const Job& j) {
     return std::adjacent_find(keys.begin(), keys.end()) == keys.end();
 }
 
-// Append a job's observable to the batch under its slot.
+// Append a job's observable to the batch under its slot.  On failure the
+// batch holds what it held before (its storage may have grown).
 int admit(plh_server* s, const Job& j, int slot) {
     const int64_t n = j.n0;
     if (n == 0) return PLH_OK;
@@ -359,217 +339,23 @@ int admit(plh_server* s, const Job& j, int slot) {
     return rc;
 }
 
-}  // namespace
-}  // namespace plh
-
-using namespace plh;
-
-extern "C" {
-
-int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_budget_bytes, int keep_terms,
-                      plh_server** out) {
-    if (!ctx || !out || max_jobs <= 0 || max_jobs > (1 << 16)) {
-        set_error("plh_server_create: bad context, output or max_jobs (1..65536)");
-        return PLH_ERR_ARG;
-    }
-    if (array_size != 1 && array_size != 2) {
-        set_error("batched propagation supports 1 or 2 words per plane (up to 128 qubits)");
-        return PLH_ERR_AS;
-    }
-    PLH_HIP(hipSetDevice(ctx->device));
-    {
-        uint64_t tbl[kNumGates];
-        uint32_t sgn[kNumGates];
-        for (int g = 0; g < kNumGates; ++g) PLH_TRY(clifford_table_of(g, &tbl[g], &sgn[g]));
-        PLH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gate_tbl), tbl, sizeof(tbl), 0, hipMemcpyHostToDevice));
-        PLH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gate_sgn), sgn, sizeof(sgn), 0, hipMemcpyHostToDevice));
-    }
-    size_t free_b = 0, total_b = 0;
-    PLH_HIP(hipMemGetInfo(&free_b, &total_b));
-    plh_server* s = new plh_server();
-    s->ctx = ctx;
-    s->as = array_size;
-    s->slots = max_jobs;
-    s->keep_terms = keep_terms != 0;
-    s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)(free_b / 2);
-    s->active.resize((size_t)max_jobs);
-    s->state.assign((size_t)max_jobs, 0);
-    int rc = plh_sum_create(ctx, array_size, 64 * array_size, 0, &s->batch);
-    if (rc == PLH_OK) {
-        s->batch->has_job = true;
-        sum_free_storage(s->batch);
-        rc = sum_reserve(s->batch, 4096, 0);
-    }
-    if (rc != PLH_OK) {
-        if (s->batch) plh_sum_destroy(s->batch);
-        delete s;
-        return rc;
-    }
-    s->t0 = std::chrono::steady_clock::now();
-    *out = s;
-    return PLH_OK;
-}
-
-int plh_server_destroy(plh_server* s) {
-    if (!s) return PLH_OK;
-    (void)hipStreamSynchronize(s->ctx->stream);
-    plh_sum_destroy(s->batch);
-    (void)hipFree(s->d_step);
-    delete s;
-    return PLH_OK;
-}
-
-int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_t* z, const uint8_t* f,
-                      const double* v, int64_t num_qubits, int64_t nops, const int32_t* kind, const int64_t* q0,
-                      const int64_t* q1, const uint64_t* px, const uint64_t* pz, const double* theta,
-                      int64_t max_weight, double cutoff, int64_t* handle) {
-    if (!s || n < 0 || nops < 0 || !handle || (n > 0 && (!x || !z || !f || !v)) ||
-        (nops > 0 && (!kind || !q0 || !q1 || !px || !pz || !theta))) {
-        set_error("plh_server_submit: null pointer or negative size");
-        return PLH_ERR_ARG;
-    }
-    if (num_qubits < 1 || num_qubits > 64 * s->as) {
-        set_error("plh_server_submit: job qubit count exceeds the server's array size");
-        return PLH_ERR_ARG;
-    }
-    if (n > ((int64_t)1 << 31)) {
-        set_error("plh_server_submit: observable too large for a batched job");
-        return PLH_ERR_SIZE;
-    }
-    for (int64_t l = 0; l < nops; ++l) {
-        const bool table = kind[l] >= kProgramGateBase && kind[l] < kProgramGateBase + kNumGates;
-        const bool two = kind[l] == 2 || (table && kind[l] - kProgramGateBase >= kFirstTwoQubitGate);
-        const bool clifford = (kind[l] >= 0 && kind[l] <= 2) || table;
-        if (((kind[l] < 0 || kind[l] > 3) && !table) || (clifford && (q0[l] < 0 || q0[l] >= num_qubits)) ||
-            (two && (q1[l] < 0 || q1[l] >= num_qubits || q1[l] == q0[l]))) {
-            set_error("plh_server_submit: invalid op in job program");
-            return PLH_ERR_ARG;
-        }
-    }
-    Job j;
-    j.job_as = (int)((num_qubits + 63) / 64);
-    j.n0 = n;
-    const int a = j.job_as;
-    if (n > 0) {
-        j.ox.assign(x, x + (size_t)a * n);
-        j.oz.assign(z, z + (size_t)a * n);
-        j.of.assign(f, f + n);
-        j.ov.assign(v, v + n);
-    }
-    if (nops > 0) {
-        j.kind.assign(kind, kind + nops);
-        j.q0.assign(q0, q0 + nops);
-        j.q1.assign(q1, q1 + nops);
-        j.px.assign(px, px + (size_t)a * nops);
-        j.pz.assign(pz, pz + (size_t)a * nops);
-        j.theta.assign(theta, theta + nops);
-    }
-    if (!keys_distinct(j)) {
-        set_error("plh_server_submit: observable has repeated (x, z, imag) keys; canonicalize it first");
-        return PLH_ERR_ARG;
-    }
-    j.next = nops - 1;
-    j.max_weight = max_weight;
-    j.cutoff = cutoff;
-    j.handle = s->next_handle++;
-    j.t_submit = now_s(s);
-    *handle = j.handle;
-    s->queue.push_back(std::move(j));
-    return PLH_OK;
-}
-
-int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
-    if (!s) return PLH_ERR_ARG;
+// The device part of a step: table upload, gates, one collation for every
+// job, and the finished jobs' expectations (and terms); their results go
+// to *res.  Nothing of the scheduler changes here.
+int step_device(plh_server* s, const sched::StepPlan& pl, std::vector<Result>* res) {
     plh_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
-    const int AS = s->as, S = s->slots;
+    const int AS = s->as;
     plh_sum* B = s->batch;
-
-    // ---- admission: a free slot, and the batch with the new job fits the HBM budget
-    const double bpt = bytes_per_term(AS);
-    int running = 0;
-    for (int i = 0; i < S; ++i) running += s->state[(size_t)i] == 1;
-    int64_t in_flight = B->n;
-    while (!s->queue.empty()) {
-        int slot = -1;
-        for (int i = 0; i < S && slot < 0; ++i)
-            if (s->state[(size_t)i] == 0) slot = i;
-        if (slot < 0) break;
-        const Job& q = s->queue.front();
-        // room for the new job's terms to grow 64-fold
-        const double need = ((double)in_flight + 64.0 * (double)std::max<int64_t>(q.n0, 1)) * bpt;
-        if (running > 0 && need > (double)s->budget) break;
-        PLH_TRY(admit(s, q, slot));
-        in_flight += q.n0;
-        s->active[(size_t)slot] = std::move(s->queue.front());
-        s->queue.pop_front();
-        s->state[(size_t)slot] = 1;
-        ++running;
-    }
-
-    // ---- this step's tables: per running job, the Cliffords up to its next rotation and that rotation
-    std::vector<uint32_t> off((size_t)S + 1, 0), ops;
-    std::vector<uint8_t> rk((size_t)S, 0), rpsf((size_t)S, 1), alive((size_t)S, 0);
-    std::vector<double> rc((size_t)S, 0.0), rs((size_t)S, 0.0), eps((size_t)S, 0.0);
-    std::vector<int64_t> maxw((size_t)S, -1);
-    std::vector<uint64_t> rp((size_t)S * 2 * AS, 0);
-    std::vector<int> finishing;
-    std::vector<uint64_t> p_or((size_t)2 * AS, 0);
-    bool any_general = false;
-    for (int i = 0; i < S; ++i) {
-        off[(size_t)i] = (uint32_t)ops.size();
-        if (s->state[(size_t)i] != 1) continue;
-        alive[(size_t)i] = 1;
-        Job& j = s->active[(size_t)i];
-        while (j.next >= 0 && j.kind[(size_t)j.next] != 3) {
-            const int64_t l = j.next--;
-            const uint32_t k = (uint32_t)j.kind[(size_t)l];
-            if (k >= (uint32_t)kProgramGateBase) {  // table gate: qubits < 1024 (at most 16 words)
-                const uint32_t g = k - (uint32_t)kProgramGateBase;
-                const uint32_t t = g >= (uint32_t)kFirstTwoQubitGate ? (uint32_t)j.q1[(size_t)l] : 0u;
-                ops.push_back((3u << 28) | (g << 20) | ((uint32_t)j.q0[(size_t)l] << 10) | t);
-            } else {
-                ops.push_back((k << 28) | ((uint32_t)j.q0[(size_t)l] << 14) | (k == 2 ? (uint32_t)j.q1[(size_t)l] : 0u));
-            }
-        }
-        if (j.next >= 0) {
-            const int64_t l = j.next--;
-            const double c = std::cos(j.theta[(size_t)l]), sn = std::sin(j.theta[(size_t)l]);
-            uint8_t kind = 4;
-            if (close_one(c)) kind = 0;
-            else if (close_one(-c)) kind = 1;
-            else if (close_one(sn)) kind = 2;
-            else if (close_one(-sn)) kind = 3;
-            rk[(size_t)i] = kind;  // program generators carry flags 0, so ps has flags 1
-            if (kind == 4) {
-                any_general = true;
-                for (int k = 0; k < j.job_as; ++k) {
-                    p_or[(size_t)k] |= j.px[(size_t)(l * j.job_as + k)];
-                    p_or[(size_t)AS + k] |= j.pz[(size_t)(l * j.job_as + k)];
-                }
-            }
-            rc[(size_t)i] = c;
-            rs[(size_t)i] = sn;
-            for (int k = 0; k < j.job_as; ++k) {
-                rp[(size_t)i * 2 * AS + k] = j.px[(size_t)(l * j.job_as + k)];
-                rp[(size_t)i * 2 * AS + AS + k] = j.pz[(size_t)(l * j.job_as + k)];
-            }
-            maxw[(size_t)i] = j.max_weight;  // apply_truncation follows every rotation
-            eps[(size_t)i] = j.cutoff;
-        }
-        if (j.next < 0) finishing.push_back(i);
-    }
-    off[(size_t)S] = (uint32_t)ops.size();
-
     struct Part {
         const void* src;
         size_t bytes;
     };
-    const Part parts[10] = {{off.data(), off.size() * 4}, {ops.data(), ops.size() * 4},
-                            {rk.data(), rk.size()},       {rpsf.data(), rpsf.size()},
-                            {rc.data(), rc.size() * 8},   {rs.data(), rs.size() * 8},
-                            {rp.data(), rp.size() * 8},   {eps.data(), eps.size() * 8},
-                            {maxw.data(), maxw.size() * 8}, {alive.data(), alive.size()}};
+    const Part parts[10] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
+                            {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
+                            {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
+                            {pl.rp.data(), pl.rp.size() * 8},   {pl.eps.data(), pl.eps.size() * 8},
+                            {pl.maxw.data(), pl.maxw.size() * 8}, {pl.alive.data(), pl.alive.size()}};
     size_t total = 0;
     for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
     PLH_HIP(hipStreamSynchronize(st));  // the previous step's upload has been consumed
@@ -622,90 +408,243 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
         sc.job_eps = tb.eps;
         sc.job_maxw = tb.maxw;
         sc.job_alive = tb.alive;
-        if (any_general) {
+        if (pl.any_general) {
             JobRotTables jt{tb.rot_kind, tb.rot_p, tb.rot_psf, tb.rot_c, tb.rot_s};
-            PLH_TRY(collate_rotation_jobs(ctx, B, jt, p_or.data(), sc, B));
+            PLH_TRY(collate_rotation_jobs(ctx, B, jt, pl.p_or.data(), sc, B));
         } else {
             PLH_TRY(collate_terms(ctx, B, sc, B));
         }
     }
-    for (int i = 0; i < S; ++i)
-        if (s->state[(size_t)i] == 2) s->state[(size_t)i] = 0;  // their terms left in this collation
 
-    // ---- finished jobs: expectation on the job-ordered batch; their slots drain next step
-    if (!finishing.empty()) {
-        const size_t F = finishing.size();
-        std::vector<double> he(F, 0.0);
-        std::vector<int64_t> hr(2 * F, 0);
-        if (B->n > 0) {
-            void* p;
-            PLH_TRY(ctx->ws.get(kWsTmpSum, align256(F * 4) + F * 24 + 256, &p));
-            uint32_t* d_jobs = (uint32_t*)p;
-            double* d_e = (double*)((char*)p + align256(F * 4));
-            int64_t* d_rng = (int64_t*)(d_e + F);
-            std::vector<uint32_t> hj(F);
-            for (size_t i = 0; i < F; ++i) hj[i] = (uint32_t)finishing[i];
-            PLH_HIP(hipMemcpyAsync(d_jobs, hj.data(), F * 4, hipMemcpyHostToDevice, st));
-            k_batch_zse<<<dim3((unsigned)F), kBlock, 0, st>>>(B->as, SumView{B->x, B->z, B->f, B->v, B->cap}, B->j,
-                                                               B->n, d_jobs, d_e, d_rng);
-            PLH_HIP(hipGetLastError());
-            PLH_HIP(hipMemcpyAsync(he.data(), d_e, F * 8, hipMemcpyDeviceToHost, st));
-            PLH_HIP(hipMemcpyAsync(hr.data(), d_rng, F * 16, hipMemcpyDeviceToHost, st));
+    // ---- finished jobs: expectation on the job-ordered batch
+    if (pl.finishing.empty()) return PLH_OK;
+    const size_t F = pl.finishing.size();
+    std::vector<double> he(F, 0.0);
+    std::vector<int64_t> hr(2 * F, 0);
+    if (B->n > 0) {
+        void* p;
+        PLH_TRY(ctx->ws.get(kWsTmpSum, align256(F * 4) + F * 24 + 256, &p));
+        uint32_t* d_jobs = (uint32_t*)p;
+        double* d_e = (double*)((char*)p + align256(F * 4));
+        int64_t* d_rng = (int64_t*)(d_e + F);
+        std::vector<uint32_t> hj(F);
+        for (size_t i = 0; i < F; ++i) hj[i] = (uint32_t)pl.finishing[i];
+        PLH_HIP(hipMemcpyAsync(d_jobs, hj.data(), F * 4, hipMemcpyHostToDevice, st));
+        k_batch_zse<<<dim3((unsigned)F), kBlock, 0, st>>>(B->as, SumView{B->x, B->z, B->f, B->v, B->cap}, B->j,
+                                                           B->n, d_jobs, d_e, d_rng);
+        PLH_HIP(hipGetLastError());
+        PLH_HIP(hipMemcpyAsync(he.data(), d_e, F * 8, hipMemcpyDeviceToHost, st));
+        PLH_HIP(hipMemcpyAsync(hr.data(), d_rng, F * 16, hipMemcpyDeviceToHost, st));
+        PLH_HIP(hipStreamSynchronize(st));
+    }
+    const double t = now_s(s);
+    res->resize(F);
+    for (size_t i = 0; i < F; ++i) {
+        const int slot = pl.finishing[i];
+        const Job& j = s->sched->job(slot);
+        Result& r = (*res)[i];
+        r.handle = j.handle;
+        r.expectation = he[i];
+        r.terms = hr[2 * i + 1] - hr[2 * i];
+        r.latency_s = t - j.t_submit;
+        if (!s->keep_terms) continue;
+        const int64_t a = hr[2 * i], m = r.terms;
+        if (a < 0 || m < 0 || a + m > B->n) {
+            set_error("plh_server_step: a finished job's range lies outside the batch");
+            return PLH_ERR_HIP;
+        }
+        r.x.resize((size_t)j.job_as * m);
+        r.z.resize((size_t)j.job_as * m);
+        r.f.resize((size_t)m);
+        r.v.resize((size_t)m);
+        if (m > 0) {
+            for (int k = 0; k < j.job_as; ++k) {
+                PLH_HIP(hipMemcpyAsync(r.x.data() + (size_t)k * m, B->x + (int64_t)k * B->cap + a, (size_t)m * 8,
+                                       hipMemcpyDeviceToHost, st));
+                PLH_HIP(hipMemcpyAsync(r.z.data() + (size_t)k * m, B->z + (int64_t)k * B->cap + a, (size_t)m * 8,
+                                       hipMemcpyDeviceToHost, st));
+            }
+            PLH_HIP(hipMemcpyAsync(r.f.data(), B->f + a, (size_t)m, hipMemcpyDeviceToHost, st));
+            PLH_HIP(hipMemcpyAsync(r.v.data(), B->v + a, (size_t)m * 8, hipMemcpyDeviceToHost, st));
             PLH_HIP(hipStreamSynchronize(st));
         }
-        const double t = now_s(s);
-        for (size_t i = 0; i < F; ++i) {
-            const int slot = finishing[i];
-            Job& j = s->active[(size_t)slot];
-            Result r;
-            r.handle = j.handle;
-            r.expectation = he[i];
-            r.terms = hr[2 * i + 1] - hr[2 * i];
-            r.latency_s = t - j.t_submit;
-            if (s->keep_terms) {
-                const int64_t a = hr[2 * i], m = r.terms;
-                r.x.resize((size_t)j.job_as * m);
-                r.z.resize((size_t)j.job_as * m);
-                r.f.resize((size_t)m);
-                r.v.resize((size_t)m);
-                if (m > 0) {
-                    for (int k = 0; k < j.job_as; ++k) {
-                        PLH_HIP(hipMemcpyAsync(r.x.data() + (size_t)k * m, B->x + (int64_t)k * B->cap + a,
-                                               (size_t)m * 8, hipMemcpyDeviceToHost, st));
-                        PLH_HIP(hipMemcpyAsync(r.z.data() + (size_t)k * m, B->z + (int64_t)k * B->cap + a,
-                                               (size_t)m * 8, hipMemcpyDeviceToHost, st));
-                    }
-                    PLH_HIP(hipMemcpyAsync(r.f.data(), B->f + a, (size_t)m, hipMemcpyDeviceToHost, st));
-                    PLH_HIP(hipMemcpyAsync(r.v.data(), B->v + a, (size_t)m * 8, hipMemcpyDeviceToHost, st));
-                    PLH_HIP(hipStreamSynchronize(st));
-                }
-            }
-            const int64_t h = r.handle;
-            if (s->keep_terms) {
-                Result meta;
-                meta.handle = h;
-                meta.expectation = r.expectation;
-                meta.terms = r.terms;
-                meta.latency_s = r.latency_s;
-                s->done.push_back(std::move(meta));
-                s->kept[h] = std::move(r);
-            } else {
-                s->done.push_back(std::move(r));
-            }
-            s->state[(size_t)slot] = 2;
-            s->active[(size_t)slot] = Job();
+    }
+    return PLH_OK;
+}
+
+}  // namespace
+}  // namespace plh
+
+using namespace plh;
+
+extern "C" {
+
+int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_budget_bytes, int keep_terms,
+                      plh_server** out) {
+    static_assert(sched::kGateBase == kProgramGateBase && sched::kGates == kNumGates &&
+                      sched::kFirstTwoQubit == kFirstTwoQubitGate,
+                  "scheduler and engine agree on the program op kinds");
+    if (!ctx || !out || max_jobs <= 0 || max_jobs > (1 << 16)) {
+        set_error("plh_server_create: bad context, output or max_jobs (1..65536)");
+        return PLH_ERR_ARG;
+    }
+    if (array_size != 1 && array_size != 2) {
+        set_error("batched propagation supports 1 or 2 words per plane (up to 128 qubits)");
+        return PLH_ERR_AS;
+    }
+    PLH_HIP(hipSetDevice(ctx->device));
+    {
+        uint64_t tbl[kNumGates];
+        uint32_t sgn[kNumGates];
+        for (int g = 0; g < kNumGates; ++g) PLH_TRY(clifford_table_of(g, &tbl[g], &sgn[g]));
+        PLH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gate_tbl), tbl, sizeof(tbl), 0, hipMemcpyHostToDevice));
+        PLH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gate_sgn), sgn, sizeof(sgn), 0, hipMemcpyHostToDevice));
+    }
+    size_t free_b = 0, total_b = 0;
+    PLH_HIP(hipMemGetInfo(&free_b, &total_b));
+    plh_server* s = new plh_server();
+    s->ctx = ctx;
+    s->as = array_size;
+    s->keep_terms = keep_terms != 0;
+    s->sched = new sched::Scheduler(array_size, max_jobs,
+                                    hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)(free_b / 2));
+    int rc = plh_sum_create(ctx, array_size, 64 * array_size, 0, &s->batch);
+    if (rc == PLH_OK) {
+        s->batch->has_job = true;
+        sum_free_storage(s->batch);
+        rc = sum_reserve(s->batch, 4096, 0);
+    }
+    if (rc != PLH_OK) {
+        if (s->batch) plh_sum_destroy(s->batch);
+        delete s->sched;
+        delete s;
+        return rc;
+    }
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return PLH_OK;
+}
+
+int plh_server_destroy(plh_server* s) {
+    if (!s) return PLH_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    plh_sum_destroy(s->batch);
+    (void)hipFree(s->d_step);
+    delete s->sched;
+    delete s;
+    return PLH_OK;
+}
+
+int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_t* z, const uint8_t* f,
+                      const double* v, int64_t num_qubits, int64_t nops, const int32_t* kind, const int64_t* q0,
+                      const int64_t* q1, const uint64_t* px, const uint64_t* pz, const double* theta,
+                      int64_t max_weight, double cutoff, int64_t* handle) {
+    if (!s || n < 0 || nops < 0 || !handle || (n > 0 && (!x || !z || !f || !v)) ||
+        (nops > 0 && (!kind || !q0 || !q1 || !px || !pz || !theta))) {
+        set_error("plh_server_submit: null pointer or negative size");
+        return PLH_ERR_ARG;
+    }
+    if (s->failed) {
+        set_error("plh_server_submit: the server failed in an earlier step (" + s->failure + ")");
+        return PLH_ERR_STATE;
+    }
+    if (num_qubits < 1 || num_qubits > 64 * s->as) {
+        set_error("plh_server_submit: job qubit count exceeds the server's array size");
+        return PLH_ERR_ARG;
+    }
+    if (n > ((int64_t)1 << 31)) {
+        set_error("plh_server_submit: observable too large for a batched job");
+        return PLH_ERR_SIZE;
+    }
+    for (int64_t l = 0; l < nops; ++l) {
+        const bool table = kind[l] >= kProgramGateBase && kind[l] < kProgramGateBase + kNumGates;
+        const bool two = kind[l] == 2 || (table && kind[l] - kProgramGateBase >= kFirstTwoQubitGate);
+        const bool clifford = (kind[l] >= 0 && kind[l] <= 2) || table;
+        if (((kind[l] < 0 || kind[l] > 3) && !table) || (clifford && (q0[l] < 0 || q0[l] >= num_qubits)) ||
+            (two && (q1[l] < 0 || q1[l] >= num_qubits || q1[l] == q0[l]))) {
+            set_error("plh_server_submit: invalid op in job program");
+            return PLH_ERR_ARG;
         }
     }
-    ++s->steps;
-    int act = 0;
-    for (int i = 0; i < S; ++i) act += s->state[(size_t)i] == 1;
-    if (act == 0) {
-        // nothing running: drained terms can go without a collation
-        for (int i = 0; i < S; ++i) s->state[(size_t)i] = 0;
-        B->n = 0;
+    Job j;
+    j.job_as = (int)((num_qubits + 63) / 64);
+    j.n0 = n;
+    const int a = j.job_as;
+    if (n > 0) {
+        j.ox.assign(x, x + (size_t)a * n);
+        j.oz.assign(z, z + (size_t)a * n);
+        j.of.assign(f, f + n);
+        j.ov.assign(v, v + n);
     }
-    if (active_out) *active_out = act;
-    if (queued_out) *queued_out = (int)s->queue.size();
+    if (nops > 0) {
+        j.kind.assign(kind, kind + nops);
+        j.q0.assign(q0, q0 + nops);
+        j.q1.assign(q1, q1 + nops);
+        j.px.assign(px, px + (size_t)a * nops);
+        j.pz.assign(pz, pz + (size_t)a * nops);
+        j.theta.assign(theta, theta + nops);
+    }
+    if (!keys_distinct(j)) {
+        set_error("plh_server_submit: observable has repeated (x, z, imag) keys; canonicalize it first");
+        return PLH_ERR_ARG;
+    }
+    j.next = nops - 1;
+    j.max_weight = max_weight;
+    j.cutoff = cutoff;
+    j.handle = s->next_handle++;
+    j.t_submit = now_s(s);
+    *handle = j.handle;
+    s->sched->queue().push_back(std::move(j));
+    return PLH_OK;
+}
+
+int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
+    if (!s) return PLH_ERR_ARG;
+    if (s->failed) {
+        set_error("plh_server_step: the server failed in an earlier step (" + s->failure + ")");
+        return PLH_ERR_STATE;
+    }
+    sched::Scheduler& sc = *s->sched;
+    plh_sum* B = s->batch;
+
+    // ---- admission: a free slot, and the batch with the new job fits the HBM
+    // budget.  A job that cannot be appended stays queued; the server stays
+    // usable (the batch holds what it held).
+    int64_t in_flight = B->n;
+    for (int slot; (slot = sc.admission_slot(in_flight)) >= 0;) {
+        const int64_t n0 = sc.queue().front().n0;
+        PLH_TRY(admit(s, sc.queue().front(), slot));
+        sc.admitted(slot);
+        in_flight += n0;
+    }
+
+    // ---- the step: planned, run on the device, then committed
+    sched::StepPlan pl;
+    sc.plan(&pl);
+    std::vector<Result> res;
+    const int rc = step_device(s, pl, &res);
+    if (rc != PLH_OK) {
+        s->failed = true;
+        s->failure = last_error();
+        return rc;
+    }
+    for (Result& r : res) {
+        if (s->keep_terms) {
+            Result meta;
+            meta.handle = r.handle;
+            meta.expectation = r.expectation;
+            meta.terms = r.terms;
+            meta.latency_s = r.latency_s;
+            s->done.push_back(std::move(meta));
+            s->kept[r.handle] = std::move(r);
+        } else {
+            s->done.push_back(std::move(r));
+        }
+    }
+    if (sc.commit(pl)) B->n = 0;  // nothing running: drained terms go without a collation
+    ++s->steps;
+    if (active_out) *active_out = sc.running();
+    if (queued_out) *queued_out = (int)sc.queue().size();
     return PLH_OK;
 }
 

@@ -1,4 +1,6 @@
 """Comparison helpers shared by the tests."""
+import hashlib
+
 import numpy as np
 
 import oracle
@@ -32,3 +34,21 @@ def assert_terms_equal(got, want, values="exact", rtol=1e-12):
 
 def cpu(kind, host):
     return oracle.CpuSum.from_host(kind, host)
+
+
+def digests(h) -> dict:
+    """SHA-256 of a sum's planes (x words, z words, flags bytes) and of its
+    coefficient bits, little-endian: a size-independent fingerprint of every
+    bit of a result."""
+    planes = hashlib.sha256()
+    planes.update(np.ascontiguousarray(h.x, "<u8").tobytes())
+    planes.update(np.ascontiguousarray(h.z, "<u8").tobytes())
+    planes.update(np.ascontiguousarray(h.flags, "u1").tobytes())
+    vals = hashlib.sha256(np.ascontiguousarray(h.value, "<f8").view("<u8").tobytes())
+    return {"planes_sha256": planes.hexdigest(), "values_sha256": vals.hexdigest()}
+
+
+def terms_of(h, idx) -> list:
+    """Terms of a one-word sum as [x hex, z hex, flags, value hex] rows."""
+    return [[f"{int(h.x[0, t]):016x}", f"{int(h.z[0, t]):016x}", int(h.flags[t]), float(h.value[t]).hex()]
+            for t in idx]
