@@ -28,11 +28,20 @@
  *                           oracle/_ref (with truncation).
  *   trim                    reference KATs with distinct magnitudes; the
  *                           tie order of std::nth_element is parity unpinned.
+ *
+ * Angles: evolve and apply_RX/RY/RZ take std::cos and std::sin of one angle,
+ * which the reference's compilers merge into one call of the C library's
+ * sincos (its results can differ from separate cos and sin calls in the
+ * last bit); the oracle calls sincos itself (orc_cos_sin).
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* cos and sin of an angle as the reference's build computes them */
+static void orc_cos_sin(double th, double *c, double *s) { sincos(th, s, c); }
 
 #define ORC_SIGN 0x02u
 #define ORC_PHASE 0x01u
@@ -534,8 +543,9 @@ int orc_evolve(orc_sum *s, const orc_sum *rot) {
     uint64_t px[16], pz[16];
     for (int64_t r = 0; r < rot->n; ++r) {
         load_term(rot, r, px, pz);
-        const double th = rot->v[r];
-        const int rc = orc_evolve_cs(s, px, pz, rot->f[r], cos(th), sin(th));
+        double c, sn;
+        orc_cos_sin(rot->v[r], &c, &sn);
+        const int rc = orc_evolve_cs(s, px, pz, rot->f[r], c, sn);
         if (rc) return rc;
     }
     return 0;
@@ -559,8 +569,9 @@ int orc_propagate(orc_sum *s, const orc_sum *rot, int64_t max_weight, double cut
     uint64_t px[16], pz[16];
     for (int64_t l = rot->n; l-- > 0;) {
         load_term(rot, l, px, pz);
-        const double th = rot->v[l];
-        int rc = orc_evolve_cs(s, px, pz, rot->f[l], cos(th), sin(th));
+        double c, sn;
+        orc_cos_sin(rot->v[l], &c, &sn);
+        int rc = orc_evolve_cs(s, px, pz, rot->f[l], c, sn);
         if (!rc) rc = orc_apply_truncation(s, max_weight, cutoff, max_terms);
         if (rc) return rc;
     }
@@ -587,7 +598,11 @@ int orc_run_program(orc_sum *s, int64_t nops, const int32_t *kind, const int64_t
             case 1: rc = orc_apply_S(s, q0[l]); break;
             case 2: rc = orc_apply_CNOT(s, q0[l], q1[l]); break;
             case 3:
-                rc = orc_evolve_cs(s, px + l * as, pz + l * as, 0, cos(theta[l]), sin(theta[l]));
+            {
+                double c, sn;
+                orc_cos_sin(theta[l], &c, &sn);
+                rc = orc_evolve_cs(s, px + l * as, pz + l * as, 0, c, sn);
+            }
                 if (!rc) rc = orc_apply_truncation(s, max_weight, cutoff, max_terms);
                 break;
             default:
@@ -692,7 +707,8 @@ int orc_apply_gate(orc_sum *s, int g, int64_t c, int64_t t) {
  * operator+= (append + sort_and_combine). */
 int orc_apply_rot_axis(orc_sum *s, int axis, int64_t c, double angle) {
     if (axis < 1 || axis > 3 || c < 0 || c >= 64 * (int64_t)s->as) return -2;
-    const double cos_half = cos(angle / 2), sin_half = sin(angle / 2);
+    double cos_half, sin_half;
+    orc_cos_sin(angle / 2, &cos_half, &sin_half);
     const int w = (int)(c / 64);
     const uint64_t m = 1ull << (c % 64);
     orc_sum *nw = orc_sum_new(s->as, s->nq);

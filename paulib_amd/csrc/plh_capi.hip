@@ -15,6 +15,7 @@
 #include <cstring>
 #include <string>
 
+#include "plh_angles.hpp"
 #include "plh_engine.hpp"
 #include "plh_sources.hpp"
 
@@ -85,7 +86,8 @@ CollateSpec trunc_spec(int64_t max_weight, double cutoff) {
 // predicates of the canonical terms, so they fuse into the same pass.
 int rotation_step_core(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf, double theta,
                        bool truncate, int64_t max_weight, double cutoff) {
-    const double c = std::cos(theta), sn = std::sin(theta);
+    double c, sn;
+    angle_cos_sin(theta, &c, &sn);
     const bool was_canonical = s->canonical;
     if (rotation_kind(c, sn) == 4) {
         // general path: evolve's new terms are produced inside the collation
@@ -811,7 +813,8 @@ static int rotation_axis(plh_sum* s, int axis, int64_t q, double angle) {
     const uint64_t m = 1ull << (q % 64);
     if (axis != 3) px[q / 64] = m;  // X or Y
     if (axis != 1) pz[q / 64] = m;  // Z or Y
-    const double ch = std::cos(angle / 2), sh = std::sin(angle / 2);
+    double ch, sh;
+    angle_cos_sin(angle / 2, &ch, &sh);
     CollateSpec sc;
     PLH_TRY(collate_rotation(s->ctx, s, px, pz, 3u, ch, -sh, sc, s));
     s->canonical = false;

@@ -23,6 +23,8 @@
 #include <deque>
 #include <vector>
 
+#include "plh_angles.hpp"
+
 namespace plh {
 namespace sched {
 
@@ -158,7 +160,8 @@ class Scheduler {
             }
             if (next >= 0) {
                 const int64_t l = next--;
-                const double c = std::cos(j.theta[(size_t)l]), sn = std::sin(j.theta[(size_t)l]);
+                double c, sn;
+                angle_cos_sin(j.theta[(size_t)l], &c, &sn);
                 const uint8_t kind = rotation_kind_of(c, sn);
                 p->rk[i] = kind;
                 const size_t ja = (size_t)j.job_as;

@@ -341,7 +341,9 @@ struct Replay {
             if (l >= 0) {
                 if (j.kind[(size_t)l] != 3) fail("job %lld's step stops at a non-rotation op %lld", j.handle, l);
                 if (pl.next[(size_t)i] != l - 1) fail("job %lld's next op after the step is %lld", j.handle, pl.next[(size_t)i]);
-                const uint8_t want = plh::sched::rotation_kind_of(std::cos(j.theta[(size_t)l]), std::sin(j.theta[(size_t)l]));
+                double c, sn;
+                plh::angle_cos_sin(j.theta[(size_t)l], &c, &sn);
+                const uint8_t want = plh::sched::rotation_kind_of(c, sn);
                 if (pl.rk[(size_t)i] != want) fail("rotation kind of job %lld", j.handle);
                 if (pl.maxw[(size_t)i] != j.max_weight || pl.eps[(size_t)i] != j.cutoff) fail("truncation of job %lld", j.handle);
             } else {

@@ -209,7 +209,7 @@ def test_against_reference_random_sums(as_):
             assert_terms_equal(gate("oracle", h, name, c, t), gate("reference", h, name, c, t))
     for axis in "XYZ":
         for c in [0, nq - 1, int(rng.integers(nq))]:
-            for angle in [0.0, 0.7, -2.2, math.pi]:
+            for angle in [0.0, 0.7, -2.2, math.pi, 3.430420739766125]:
                 a, b = cpu("oracle", h), cpu("reference", h)
                 a.apply_rot(axis, c, angle)
                 b.apply_rot(axis, c, angle)
