@@ -155,6 +155,27 @@ int plh_batch_multiply_canonical_each(int njobs, const plh_sum* const* lhs, cons
 /* commutator / i_commutator (include/PauliSumOperations.h:348-356), the
  * uncombined terms in (i over a, j over b) order; num_qubits = max. */
 int plh_commutator(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The raw commutator of plh_commutator -- [a, b], or i[a, b]
+ * with multiply_by_i, one term per anticommuting pair, (i over a, j over b)
+ * order, coefficient (2 a_i) b_j -- from caller-held packed planes (a: stride
+ * n, b: stride m) into caller-allocated output planes of stride `capacity`,
+ * in the argument order of the reference's launch_gpu_commutator_soa
+ * (include/gpu/paulib_gpu_api.h:43-53).  *d_count (device memory) receives
+ * the number of anticommuting pairs (it need not be zeroed); terms from
+ * index `capacity` on are not written, so a count above capacity means the
+ * output was cut.  Stream-ordered on `stream` (the calling thread's current
+ * device must be the stream's); returns without synchronising.  Allocates
+ * nothing: its scratch is a fixed per-device array of ~12 MiB, so
+ * n*m <= 2^32 (PLH_ERR_SIZE beyond); calls that overlap on one device race
+ * on that array. */
+int plh_launch_commutator_soa_noalloc(int array_size, int multiply_by_i, const uint64_t* a_x,
+                                      const uint64_t* a_z, const uint8_t* a_flags, const double* a_val,
+                                      const uint64_t* b_x, const uint64_t* b_z, const uint8_t* b_flags,
+                                      const double* b_val, uint64_t* out_x, uint64_t* out_z,
+                                      uint8_t* out_flags, double* out_val, unsigned long long* d_count,
+                                      int64_t n, int64_t m, int64_t capacity, void* stream);
 /* canonicalize([a, b]) (or of i[a, b]) fused, without the uncombined list. */
 int plh_commutator_canonical(const plh_sum* a, const plh_sum* b, int multiply_by_i, plh_sum* out);
 /* PauliSum::sort_and_combine (include/PauliSumOperations.h:368-461). */

@@ -72,6 +72,8 @@ SIGNATURES = {
     "plh_batch_multiply_canonical": (_int, [_int, C.POINTER(_p), C.POINTER(_p), _p, _i64p]),
     "plh_batch_multiply_canonical_each": (_int, [_int, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), _i64]),
     "plh_commutator": (_int, [_p, _p, _int, _p]),
+    "plh_launch_commutator_soa_noalloc": (_int, [_int, _int, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                                 _i64, _i64, _i64, _p]),
     "plh_commutator_canonical": (_int, [_p, _p, _int, _p]),
     "plh_sort_and_combine": (_int, [_p]),
     "plh_canonicalize": (_int, [_p]),

@@ -246,6 +246,11 @@ void sum_take_storage(plh_sum* dst, plh_sum* src);
 // is written to *d_total (device).  Stream-ordered.
 int scan_u32_to_u64(plh_ctx* ctx, const uint32_t* d_in, uint64_t* d_out, uint64_t n,
                     uint64_t* d_total);
+// The same on a given stream with a caller-provided buffer of
+// scan_block_sums(n) u64 block sums (nothing drawn from a workspace).
+int scan_u32_to_u64_scratch(hipStream_t st, const uint32_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total,
+                            uint64_t* bsum);
+uint64_t scan_block_sums(uint64_t n);
 // Same for u64 inputs.
 int scan_u64(plh_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total);
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "plh_block.hpp"
 #include "plh_engine.hpp"
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_comm_write(int as, SumView A, SumVie
                                                        const uint64_t* __restrict__ boff,
                                                        uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
                                                        uint8_t* __restrict__ of, double* __restrict__ ov,
-                                                       int64_t os) {
+                                                       int64_t os, uint64_t limit) {
     __shared__ uint32_t red[kWavesPerBlock];
     const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
     uint32_t mask = 0;
@@ -120,6 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_comm_write(int as, SumView A, SumVie
     uint64_t o = boff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(mask), tot, red);
     for (int k = 0; k < kOpsIpt; ++k) {
         if (!(mask >> k & 1u)) continue;
+        if (o >= limit) break;  // output slots from `limit` on are not the caller's
         const uint64_t t = base + k;
         const uint64_t i = t / m, j = t - i * m;
         uint64_t c1 = 0, c2 = 0;
@@ -500,7 +502,7 @@ int op_commutator_raw(plh_ctx* ctx, const plh_sum* a, const plh_sum* b, int by_i
     case ASV:                                                                                         \
         k_comm_write<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(a->as, A, B, m, total, (uint32_t)by_i, \
                                                                 off, out->x, out->z, out->f, out->v,   \
-                                                                out->cap);                            \
+                                                                out->cap, k);                         \
         break;
     switch (padded(a->as)) {
         PLH_CW(1) PLH_CW(2) PLH_CW(4) PLH_CW(8) PLH_CW(16)
@@ -1019,3 +1021,123 @@ int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out) {
 
 
 }  // namespace plh
+
+// ---------------------------------------------- commutator without allocation
+// plh_launch_commutator_soa_noalloc: the raw commutator of plh_commutator
+// (same kernels, same deterministic (i over a, j over b) order) into
+// caller-allocated planes, for callers that cannot allocate device memory
+// (graph capture, serving loops, harnesses that forbid it).  Its scratch --
+// the per-tile anticommuting counts, their exclusive scan and the scan's
+// block sums -- is the file-scope array `scratch` below, one per device,
+// sized for kNoAllocMaxPairs pairs.  Host code reaches the array only
+// through the per-device address from hipGetSymbolAddress (the array's name
+// is a host-side stand-in, not a device address).  Every call on a device
+// shares its array: calls that overlap on one device race.
+namespace plh {
+namespace {
+
+constexpr uint64_t kNoAllocMaxPairs = 1ull << 32;
+constexpr uint64_t kNoAllocTiles = kNoAllocMaxPairs / kOpsTile;  // 2^20 tiles of 4096 pairs
+constexpr size_t kNoAllocCntBytes = kNoAllocTiles * 4;
+constexpr size_t kNoAllocOffBytes = kNoAllocTiles * 8;
+constexpr size_t kNoAllocScratchBytes = kNoAllocCntBytes + kNoAllocOffBytes + (64u << 10);  // ~12 MiB
+
+}  // namespace
+}  // namespace plh
+
+static __device__ __align__(256) unsigned char scratch[plh::kNoAllocScratchBytes];
+
+namespace plh {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int device_scratch(void** out) {
+    int dev = 0;
+    PLH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("plh_launch_commutator_soa_noalloc: device index beyond " + std::to_string(kMaxDevices));
+        return PLH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        PLH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("plh_launch_commutator_soa_noalloc: no device address for the scratch array");
+            return PLH_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PLH_OK;
+}
+
+__global__ void k_store_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+}  // namespace
+}  // namespace plh
+
+extern "C" int plh_launch_commutator_soa_noalloc(int array_size, int multiply_by_i, const uint64_t* a_x,
+                                                 const uint64_t* a_z, const uint8_t* a_flags, const double* a_val,
+                                                 const uint64_t* b_x, const uint64_t* b_z, const uint8_t* b_flags,
+                                                 const double* b_val, uint64_t* out_x, uint64_t* out_z,
+                                                 uint8_t* out_flags, double* out_val, unsigned long long* d_count,
+                                                 int64_t n, int64_t m, int64_t capacity, void* stream) {
+    using namespace plh;
+    if (array_size < 1 || array_size > 16) {
+        set_error("plh_launch_commutator_soa_noalloc: unsupported array_size " + std::to_string(array_size) +
+                  " (supported: 1..16)");
+        return PLH_ERR_AS;
+    }
+    if (n < 0 || m < 0 || capacity < 0 || !d_count) {
+        set_error("plh_launch_commutator_soa_noalloc: negative size or null count");
+        return PLH_ERR_ARG;
+    }
+    if (n > 0 && m > 0 && (uint64_t)n * (uint64_t)m > kNoAllocMaxPairs) {
+        set_error("plh_launch_commutator_soa_noalloc: n*m = " + std::to_string((uint64_t)n * (uint64_t)m) +
+                  " pairs exceeds the scratch's limit of 2^32 pairs");
+        return PLH_ERR_SIZE;
+    }
+    if ((n > 0 && (!a_x || !a_z || !a_flags || !a_val)) || (m > 0 && (!b_x || !b_z || !b_flags || !b_val)) ||
+        (capacity > 0 && (!out_x || !out_z || !out_flags || !out_val))) {
+        set_error("plh_launch_commutator_soa_noalloc: null plane");
+        return PLH_ERR_ARG;
+    }
+    void* sp = nullptr;
+    PLH_TRY(device_scratch(&sp));
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t total = (uint64_t)n * (uint64_t)m;
+    if (total == 0) {
+        k_store_u64<<<1, 1, 0, st>>>(d_count, 0ull);
+        PLH_HIP(hipGetLastError());
+        return PLH_OK;
+    }
+    const uint64_t nb = (total + kOpsTile - 1) / kOpsTile;
+    uint32_t* cnt = (uint32_t*)sp;
+    uint64_t* off = (uint64_t*)((char*)sp + kNoAllocCntBytes);
+    uint64_t* bsum = (uint64_t*)((char*)sp + kNoAllocCntBytes + kNoAllocOffBytes);
+    static_assert(kNoAllocTiles <= 1ull << 20 && (64u << 10) >= ((1ull << 20) / 2048 + 1) * 8,
+                  "block sums of the scan fit the scratch's tail");
+    const SumView A{a_x, a_z, a_flags, a_val, n}, B{b_x, b_z, b_flags, b_val, m};
+    const uint32_t by_i = multiply_by_i ? 1u : 0u;
+#define PLH_NA(ASV)                                                                                              \
+    case ASV:                                                                                                    \
+        k_comm_count<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(array_size, A, B, (uint64_t)m, total, cnt);     \
+        PLH_HIP(hipGetLastError());                                                                              \
+        PLH_TRY(scan_u32_to_u64_scratch(st, cnt, off, nb, (uint64_t*)d_count, bsum));                            \
+        k_comm_write<ASV><<<dim3((unsigned)nb), kBlock, 0, st>>>(array_size, A, B, (uint64_t)m, total, by_i, off, \
+                                                                 out_x, out_z, out_flags, out_val, capacity,     \
+                                                                 (uint64_t)capacity);                           \
+        break;
+    switch (padded(array_size)) {
+        PLH_NA(1) PLH_NA(2) PLH_NA(4) PLH_NA(8) PLH_NA(16)
+        default: return PLH_ERR_AS;
+    }
+#undef PLH_NA
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}

@@ -152,6 +152,23 @@ __global__ __launch_bounds__(kWave) void k_post(const uint64_t* __restrict__ src
 
 }  // namespace
 
+int scan_u32_to_u64_scratch(hipStream_t st, const uint32_t* d_in, uint64_t* d_out, uint64_t n, uint64_t* d_total,
+                            uint64_t* bsum) {
+    if (n <= kScanSmall) {
+        k_scan_small<uint32_t><<<1, kBlock, 0, st>>>(d_in, d_out, n, d_total, nullptr, 0ull);
+        PLH_HIP(hipGetLastError());
+        return PLH_OK;
+    }
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    k_scan_reduce<uint32_t><<<dim3((unsigned)nb), dim3(kBlock), 0, st>>>(d_in, n, bsum);
+    k_scan_bsum<<<1, kBlock, 0, st>>>(bsum, nb, d_total);
+    k_scan_apply<uint32_t><<<dim3((unsigned)nb), dim3(kBlock), 0, st>>>(d_in, d_out, n, bsum);
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
+uint64_t scan_block_sums(uint64_t n) { return n <= kScanSmall ? 0 : (n + kScanTile - 1) / kScanTile; }
+
 int scan_u32_to_u64(plh_ctx* ctx, const uint32_t* d_in, uint64_t* d_out, uint64_t n,
                     uint64_t* d_total) {
     return scan_impl<uint32_t>(ctx, d_in, d_out, n, d_total);
