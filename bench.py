@@ -123,10 +123,12 @@ def serving(ctx, n_jobs: int, rate: float, n_plain: int, max_jobs: int, sustain_
     log(f"serving at {rate:g}/s: {over['jobs_per_s']:.2f} jobs/s, p50 {over['p50_ms']:.0f} ms, "
         f"p99 {over['p99_ms']:.0f} ms")
     rate_s = sustain_frac * over["jobs_per_s"]
-    jobs_s = W.serving_jobs(sustain_jobs, rate_s, seed=10)
-    sus, _ = serve_stream(ctx, jobs_s, max_jobs, "sustained")
-    log(f"serving at {rate_s:.2f}/s: {sus['jobs_per_s']:.2f} jobs/s, p50 {sus['p50_ms']:.0f} ms, "
-        f"p99 {sus['p99_ms']:.0f} ms")
+    sus = {}
+    if sustain_jobs > 0:
+        jobs_s = W.serving_jobs(sustain_jobs, rate_s, seed=10)
+        sus, _ = serve_stream(ctx, jobs_s, max_jobs, "sustained")
+        log(f"serving at {rate_s:.2f}/s: {sus['jobs_per_s']:.2f} jobs/s, p50 {sus['p50_ms']:.0f} ms, "
+            f"p99 {sus['p99_ms']:.0f} ms")
 
     rng = np.random.Generator(np.random.PCG64(2024))
     sample = sorted(rng.choice(n_jobs, size=min(n_plain, n_jobs), replace=False).tolist())

@@ -94,11 +94,16 @@ int rotation_step_core(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8
         // (append + sort_and_combine, and with distinct input keys also the
         // truncation's canonicalize: every key then has at most two addends)
         const uint32_t psf = (uint32_t)(pf ^ 1u);
+        // a sorted canonical input rotates without a sort (rotate_sorted); one
+        // with distinct keys in another order (after Clifford gates) takes
+        // the single fused collation
+        if (truncate && s->sorted) return rotate_sorted(s->ctx, s, px, pz, psf, c, sn, trunc_spec(max_weight, cutoff));
         if (truncate && was_canonical)
             return collate_rotation(s->ctx, s, px, pz, psf, c, sn, trunc_spec(max_weight, cutoff), s);
         CollateSpec sc;
         PLH_TRY(collate_rotation(s->ctx, s, px, pz, psf, c, sn, sc, s));
         s->canonical = false;
+        s->sorted = false;
         if (!truncate) return PLH_OK;
         return collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s);
     }
@@ -109,6 +114,7 @@ int rotation_step_core(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8
             CollateSpec sc;
             PLH_TRY(collate_terms(s->ctx, s, sc, s));
             s->canonical = false;
+            s->sorted = false;
         }
         return PLH_OK;
     }
@@ -426,6 +432,7 @@ int plh_sum_upload(plh_sum* s, int64_t n, const uint64_t* x, const uint64_t* z, 
     PLH_HIP(hipStreamSynchronize(ctx->stream));
     s->n = n;
     s->canonical = false;
+    s->sorted = false;
     return PLH_OK;
 }
 
@@ -488,6 +495,7 @@ int plh_sum_copy(const plh_sum* src, plh_sum* dst) {
     PLH_TRY(op_append(dst->ctx, dst, src));
     dst->nq = src->nq;
     dst->canonical = src->canonical;
+    dst->sorted = src->sorted;
     return PLH_OK;
 }
 
@@ -534,6 +542,7 @@ int plh_multiply(plh_sum* lhs, const plh_sum* rhs) {
     sum_take_storage(lhs, &out);
     lhs->n = n * m;
     lhs->canonical = false;
+    lhs->sorted = false;
     return PLH_OK;
 }
 
@@ -690,6 +699,7 @@ int plh_sort_and_combine(plh_sum* s) {
     CollateSpec sc;
     PLH_TRY(collate_terms(s->ctx, s, sc, s));
     s->canonical = false;
+    s->sorted = false;
     return plh_ctx_synchronize(s->ctx);
 }
 
@@ -818,6 +828,7 @@ static int rotation_axis(plh_sum* s, int axis, int64_t q, double angle) {
     CollateSpec sc;
     PLH_TRY(collate_rotation(s->ctx, s, px, pz, 3u, ch, -sh, sc, s));
     s->canonical = false;
+    s->sorted = false;
     return plh_ctx_synchronize(s->ctx);
 }
 

@@ -2392,6 +2392,7 @@ int collate_rotation(plh_ctx* ctx, plh_sum* in, const uint64_t* px, const uint64
     });
     if (rc != PLH_OK) return rc;
     out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    out->sorted = out->canonical;
     return PLH_OK;
 }
 
@@ -2414,6 +2415,7 @@ int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, con
     });
     if (rc != PLH_OK) return rc;
     out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    out->sorted = out->canonical;
     return PLH_OK;
 }
 
@@ -2456,6 +2458,7 @@ int collate_products(plh_ctx* ctx, int njobs, const plh_sum* const* lhs, const p
         return PLH_ERR_HIP;
     }
     out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    out->sorted = out->canonical;
     void* p;
     PLH_TRY(ctx->ws.get(kWsTmpSum, ((size_t)njobs + 1) * 8 + 64, &p));
     int64_t* doff = (int64_t*)p;
@@ -2519,6 +2522,7 @@ int collate_terms(plh_ctx* ctx, plh_sum* in, const CollateSpec& spec, plh_sum* o
         in->n = n;
     }
     out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    out->sorted = out->canonical;
     return PLH_OK;
 }
 
@@ -2544,6 +2548,7 @@ int split_jobs(plh_ctx* ctx, const plh_sum* batch, int njobs, const int64_t* off
         PLH_TRY(sum_reserve(o, cnt, 0));
         o->n = cnt;
         o->canonical = batch->canonical;
+        o->sorted = batch->sorted;
         d[(size_t)k] = SplitDst{o->x, o->z, o->f, o->v, o->cap, offsets[k]};
     }
     if (batch->n == 0) return PLH_OK;
@@ -2590,6 +2595,7 @@ int collate_pairs(plh_ctx* ctx, const plh_sum* L, const plh_sum* R, int kind, in
     if (rc != PLH_OK) return rc;
     out->as = L->as;
     out->canonical = (spec.mode & kFold) && (spec.mode & kDropZero);
+    out->sorted = out->canonical;
     return PLH_OK;
 }
 

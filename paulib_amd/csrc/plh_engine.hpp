@@ -153,7 +153,9 @@ struct plh_sum {
     bool has_job = false;
     int block_as = 0;        // layout of the block: words per plane and job plane
     bool block_job = false;
-    bool canonical = false;  // (x, z, imag) keys distinct, flags in {0,1}, no zeros
+    bool canonical = false;  // (x, z, imag) keys distinct (the fused rotation collation relies on it)
+    bool sorted = false;     // sorted by (x, z, imag) with distinct keys and no zeros, as a folding
+                             // collation leaves it (the sort-free rotation relies on it)
 };
 
 namespace plh {
@@ -232,6 +234,9 @@ enum : int {
     kWsPassB,   // per-pass arrays over the big buckets
     kWsChild,   // children of the big buckets of a pass
     kWsJobs,    // product batch: job table and id-chunk table
+    kWsRotA,    // sort-free rotation: anticommuting flags and their scan
+    kWsRotN,    // sort-free rotation: the new terms, sorted
+    kWsRotM,    // sort-free rotation: merge flags, lower bounds, sums, output scan
     kWsCount
 };
 
@@ -294,6 +299,14 @@ int collate_rotation(plh_ctx* ctx, plh_sum* in, const uint64_t* px, const uint64
 struct JobRotTables;
 int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, const uint64_t* p_or,
                           const CollateSpec& spec, plh_sum* out);
+// In place: canonicalize(evolve(s)) for one general-path rotation followed
+// by spec's filters, for a canonical s (sorted by (x, z, imag), distinct
+// keys), without a sort (plh_rotate.hip); psf = flags of ps.
+int rotate_sorted(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint32_t psf, double c,
+                  double sn, const CollateSpec& spec);
+// The same for a canonical batched sum, each job by its own table entry
+// (jobs whose kind is not 4 pass through, under their own filters).
+int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, const CollateSpec& spec);
 // out = canonicalize(lhs[k] * rhs[k]) for k < njobs as one batched sum
 // (job plane = k, jobs in order); offsets[k] .. offsets[k + 1] is job k's
 // range (njobs + 1 entries, host).  Sum of n_k * m_k <= 2^32.

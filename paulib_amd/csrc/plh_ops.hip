@@ -511,6 +511,7 @@ int op_commutator_raw(plh_ctx* ctx, const plh_sum* a, const plh_sum* b, int by_i
 #undef PLH_CW
     PLH_HIP(hipGetLastError());
     out->canonical = false;
+    out->sorted = false;
     return PLH_OK;
 }
 
@@ -525,6 +526,7 @@ int op_clifford(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1) {
     k_clifford<<<nblk((uint64_t)s->n, kBlock), kBlock, 0, ctx->stream>>>(gate, s->x, s->z, s->f, s->cap,
                                                                           s->n, wc, mc, wt, mt);
     PLH_HIP(hipGetLastError());
+    s->sorted = false;  // keys stay distinct (canonical holds) but their order does not
     return PLH_OK;
 }
 
@@ -671,6 +673,7 @@ int op_clifford_table(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1
         s->x, s->z, s->f, s->cap, s->n, wc, mc, wt, mt, t.tbl, t.sgn, two ? 1 : 0, sign_only ? 0 : 1);
     PLH_HIP(hipGetLastError());
     s->canonical = false;
+    s->sorted = false;
     return PLH_OK;
 }
 
@@ -694,6 +697,7 @@ int op_elementwise(plh_ctx* ctx, plh_sum* s, const plh_sum* r) {
 #undef PLH_EW
     PLH_HIP(hipGetLastError());
     s->canonical = false;
+    s->sorted = false;
     return PLH_OK;
 }
 
@@ -733,7 +737,8 @@ int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz
         switch (pa) { PLH_RI(1) PLH_RI(2) PLH_RI(4) PLH_RI(8) PLH_RI(16) }
 #undef PLH_RI
         PLH_HIP(hipGetLastError());
-        s->canonical = s->canonical && false;
+        s->canonical = false;
+        s->sorted = false;
         return PLH_OK;
     }
     const uint64_t nb = nblk((uint64_t)s->n, kOpsTile);
@@ -765,6 +770,7 @@ int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz
     PLH_HIP(hipGetLastError());
     s->n += (int64_t)k;
     s->canonical = false;
+    s->sorted = false;
     *appended_general = 1;
     return PLH_OK;
 }
@@ -813,10 +819,11 @@ int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, doubl
 #undef PLH_FW
         PLH_HIP(hipGetLastError());
     }
-    const bool canon = s->canonical;
+    const bool canon = s->canonical, srt = s->sorted;
     sum_take_storage(s, &tmp);
     s->n = (int64_t)k;
     s->canonical = canon;  // a filter of a canonical sum stays canonical
+    s->sorted = srt;
     return PLH_OK;
 }
 
@@ -990,10 +997,11 @@ int op_trim(plh_ctx* ctx, plh_sum* s, int64_t max_terms) {
                                                             s->j, tmp.j);
         PLH_HIP(hipGetLastError());
     }
-    const bool canon = s->canonical;
+    const bool canon = s->canonical, srt = s->sorted;
     sum_take_storage(s, &tmp);
     s->n = (int64_t)k;
     s->canonical = canon;  // a subset in term order of a canonical sum stays canonical
+    s->sorted = srt;
     return PLH_OK;
 }
 

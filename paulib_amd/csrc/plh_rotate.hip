@@ -1,0 +1,544 @@
+// plh_rotate.hip -- a rotation of a sorted canonical sum without a sort.
+//
+// What it computes: evolve's general path (include/PauliSumOperations.h:
+// 148-180) for one generator P followed by apply_truncation's canonicalize
+// and filters (include/PauliPropagation.h:35-47, include/PauliSumUtils.h:
+// 314-328, 377-393): every term Q that anticommutes with P keeps cos * v
+// and gains Q * ps (ps = P with the i bit flipped) with (-sin) * v; equal
+// (x, z, imag) keys combine as acc = 0; acc += old; acc += new; exact zeros,
+// overweight terms and |c| < eps are dropped.  It requires the input to be
+// sorted by (x, z, imag) with distinct keys (the output of any folding
+// collation, kept by filters and trim), so every key has at most one old and
+// one new addend and the order of combining is the reference's.
+//
+// How (no radix pass, no leaf):
+//   1. anticommuting flags a_i, and their exclusive scan A.
+//   2. a new term's key is k_i ^ M (M: P's bits; a real term times iP stays
+//      real, so the imaginary bit is kept).  Its rank among the new terms is
+//      A_i corrected at each set bit b of M: inside the group of keys that
+//      share k_i's bits above b (a contiguous range of the sorted input, with
+//      bit b = 0 first), XOR with M swaps the two halves, so the rank gains
+//      the anticommuting count of the bit-1 half (k_i has bit b = 0) or loses
+//      that of the bit-0 half (k_i has bit b = 1).  The halves come from
+//      binary searches on the sorted input; the new terms are written in
+//      sorted order.
+//   3. old and new terms are merged: each finds its lower bound in the other
+//      list (binary search); an old term whose key is also a new one takes
+//      the sum; per-term keep flags (filters) are scanned once over both
+//      lists, and each kept term's output position is its kept rank in its
+//      own list plus the kept rank of its lower bound in the other.
+// About seven launches and one read-back (the output count) per rotation.
+//
+// Batched sums (a job plane, the job id leading the key) rotate each term by
+// its own job's generator when that job's step is a general rotation; the
+// groups of step 2 never cross jobs because the job word lies above every
+// bit of M.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "plh_block.hpp"
+#include "plh_engine.hpp"
+#include "plh_sources.hpp"
+
+namespace plh {
+
+namespace {
+
+inline unsigned blocks_of(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// A sorted sum's keys on its planes: words W[0..KW) = (job,) x[0..AS),
+// z[0..AS), W[KW] = (flags & 1) << 56; global bit g counts from the most
+// significant bit of W[0].  Values and flags are read with the sign bit
+// folded in (a sign flip of a fast-path rotation keeps the order).
+template <int AS, bool JOB>
+struct Sorted {
+    static constexpr int XO = JOB ? 1 : 0;
+    static constexpr int KW = 2 * AS + XO;
+    const uint64_t* x;
+    const uint64_t* z;
+    const uint8_t* f;
+    const double* v;
+    const uint32_t* j;
+    int64_t stride;
+    int as;
+
+    PLH_DEV uint64_t word(uint32_t i, int w) const {
+        if (JOB && w == 0) return j[i];
+        const int k = w - XO;
+        if (k < AS) return k < as ? x[(int64_t)k * stride + i] : 0ull;
+        if (k < 2 * AS) return (k - AS) < as ? z[(int64_t)(k - AS) * stride + i] : 0ull;
+        return (uint64_t)(f[i] & 1u) << 56;
+    }
+    PLH_DEV void key(uint32_t i, uint64_t* kw) const {
+#pragma unroll
+        for (int w = 0; w <= KW; ++w) kw[w] = word(i, w);
+    }
+    PLH_DEV double value(uint32_t i) const { return (f[i] & 2u) ? -v[i] : v[i]; }
+    PLH_DEV uint32_t job(uint32_t i) const { return JOB ? j[i] : 0u; }
+};
+
+template <int KW>
+PLH_DEV bool key_less(const uint64_t* a, const uint64_t* b) {
+#pragma unroll
+    for (int w = 0; w <= KW; ++w)
+        if (a[w] != b[w]) return a[w] < b[w];
+    return false;
+}
+
+template <int KW>
+PLH_DEV bool key_eq(const uint64_t* a, const uint64_t* b) {
+#pragma unroll
+    for (int w = 0; w <= KW; ++w)
+        if (a[w] != b[w]) return false;
+    return true;
+}
+
+PLH_DEV uint32_t bit_at(const uint64_t* kw, int g) { return (uint32_t)(kw[g >> 6] >> (63 - (g & 63))) & 1u; }
+
+// Compares bits [0, g) of the key of item t with those of kw: -1, 0, 1.
+template <int AS, bool JOB>
+PLH_DEV int prefix_cmp(const Sorted<AS, JOB>& S, uint32_t t, const uint64_t* kw, int g) {
+    const int wg = g >> 6, o = g & 63;
+    for (int w = 0; w < wg; ++w) {
+        const uint64_t a = S.word(t, w);
+        if (a != kw[w]) return a < kw[w] ? -1 : 1;
+    }
+    if (o) {
+        const uint64_t a = S.word(t, wg) >> (64 - o), b = kw[wg] >> (64 - o);
+        if (a != b) return a < b ? -1 : 1;
+    }
+    return 0;
+}
+
+// The rotation of item i: false when none applies (a batched job whose step
+// is not a general rotation).
+template <int AS, bool JOB>
+PLH_DEV bool rot_of(const RotParams<AS>& rp, const JobRotTables& jt, uint32_t job, const uint64_t*& px,
+                    const uint64_t*& pz, uint32_t& psf, double& c, double& sn) {
+    if (!JOB) {
+        px = rp.px;
+        pz = rp.pz;
+        psf = rp.psf;
+        c = rp.c;
+        sn = rp.s;
+        return true;
+    }
+    if (jt.kind[job] != 4) return false;
+    px = jt.p + (size_t)job * 2 * AS;
+    pz = px + AS;
+    psf = jt.psf[job];
+    c = jt.c[job];
+    sn = jt.s[job];
+    return true;
+}
+
+template <int AS>
+PLH_DEV bool anticommutes(const uint64_t* kw, int xo, const uint64_t* px, const uint64_t* pz) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) acc ^= (kw[xo + k] & pz[k]) ^ (kw[xo + AS + k] & px[k]);
+    return popc(acc) & 1;
+}
+
+// Filters of an output term (canonicalize's zero drop, truncate_weight,
+// cutoff; per job for batched sums, whose drained jobs keep nothing).
+struct Filters {
+    uint32_t mode;
+    int64_t max_weight;
+    double eps;
+    const double* job_eps;
+    const int64_t* job_maxw;
+    const uint8_t* job_alive;
+};
+
+template <int AS, int XO>
+PLH_DEV bool keep_of(const Filters& fl, const uint64_t* kw, double acc) {
+    if ((fl.mode & kDropZero) && acc == 0.0) return false;
+    const uint32_t job = XO ? (uint32_t)kw[0] : 0u;
+    if (XO && fl.job_alive && !fl.job_alive[job]) return false;
+    if (fl.mode & kMaxWeight) {
+        int64_t wgt = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) wgt += popc(kw[XO + k] | kw[XO + AS + k]);
+        const int64_t mw = (XO && fl.job_maxw) ? fl.job_maxw[job] : fl.max_weight;
+        if (mw >= 0 && wgt > mw) return false;
+    }
+    if (fl.mode & kCutoff) {
+        const double eps = (XO && fl.job_eps) ? fl.job_eps[job] : fl.eps;
+        if (eps > 0.0 && !(fabs(acc) >= eps)) return false;
+    }
+    return true;
+}
+
+// New terms, sorted: planes of stride `cap`.
+struct NewTerms {
+    uint64_t* x;
+    uint64_t* z;
+    uint8_t* f;
+    double* v;
+    uint32_t* j;
+    int64_t cap;
+};
+
+// ---- 1. anticommuting flags
+template <int AS, bool JOB>
+__global__ __launch_bounds__(kBlock) void k_rs_anti(Sorted<AS, JOB> S, uint32_t n, RotParams<AS> rp, JobRotTables jt,
+                                                    uint32_t* __restrict__ a) {
+    constexpr int KW = Sorted<AS, JOB>::KW;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t kw[KW + 1];
+    S.key(i, kw);
+    const uint64_t *px, *pz;
+    uint32_t psf;
+    double c, sn;
+    const bool r = rot_of<AS, JOB>(rp, jt, S.job(i), px, pz, psf, c, sn);
+    a[i] = (r && anticommutes<AS>(kw, S.XO, px, pz)) ? 1u : 0u;
+}
+
+// ---- 2. new terms at their ranks (A: exclusive scan of a, A[n] = count)
+template <int AS, bool JOB>
+__global__ __launch_bounds__(kBlock) void k_rs_new(Sorted<AS, JOB> S, uint32_t n, RotParams<AS> rp, JobRotTables jt,
+                                                   const uint32_t* __restrict__ a, const uint64_t* __restrict__ A,
+                                                   NewTerms N) {
+    constexpr int KW = Sorted<AS, JOB>::KW;
+    constexpr int XO = Sorted<AS, JOB>::XO;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || !a[i]) return;
+    uint64_t kw[KW + 1];
+    S.key(i, kw);
+    const uint64_t *px, *pz;
+    uint32_t psf;
+    double c, sn;
+    rot_of<AS, JOB>(rp, jt, S.job(i), px, pz, psf, c, sn);
+    int64_t r = (int64_t)A[i];
+    // each set bit b of M: x bits then z bits, any order
+#pragma unroll
+    for (int h = 0; h < 2 * AS; ++h) {
+        uint64_t m = h < AS ? px[h] : pz[h - AS];
+        const int wbase = 64 * (XO + h);
+        while (m) {
+            const int bit = 63 - __clzll(m);  // bit index within the word
+            m &= ~(1ull << bit);
+            const int g = wbase + (63 - bit);
+            if (bit_at(kw, g)) {
+                // group start: first t in [0, i] with the same bits above g
+                uint32_t lo = 0, cnt = i;
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = lo + half;
+                    if (prefix_cmp(S, t, kw, g) < 0) {
+                        lo = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                // first t in [lo, i] with bit g set (the group's bit-1 half)
+                uint32_t mid = lo;
+                cnt = i - lo;
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = mid + half;
+                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
+                        mid = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                r -= (int64_t)(A[mid] - A[lo]);
+            } else {
+                // group end: first t in (i, n) with larger bits above g
+                uint32_t hi = i + 1, cnt = n - (i + 1);
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = hi + half;
+                    if (prefix_cmp(S, t, kw, g) <= 0) {
+                        hi = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                // first t in (i, hi) with bit g set
+                uint32_t mid = i + 1;
+                cnt = hi - (i + 1);
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = mid + half;
+                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
+                        mid = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                r += (int64_t)(A[hi] - A[mid]);
+            }
+        }
+    }
+    if (r < 0 || (uint64_t)r >= A[n]) return;  // never with sorted distinct input; keeps writes in bounds
+    // the new term Q * ps (PauliSum::multiply(i, ps), :185-209), sign folded
+    uint64_t c1 = 0, c2 = 0;
+    const uint32_t f0 = S.f[i] & 1u;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        phase_word(kw[XO + k], kw[XO + AS + k], px[k], pz[k], c1, c2);
+        N.x[(int64_t)k * N.cap + r] = kw[XO + k] ^ px[k];
+        N.z[(int64_t)k * N.cap + r] = kw[XO + AS + k] ^ pz[k];
+    }
+    const uint32_t raw = phase_of(f0, psf, c1, c2);
+    const double nv = (-sn) * S.value(i);
+    N.v[r] = (raw & 2u) ? -nv : nv;
+    N.f[r] = (uint8_t)(raw & 1u);
+    if (JOB) N.j[r] = S.job(i);
+}
+
+template <int AS, bool JOB>
+PLH_DEV void new_key(const NewTerms& N, uint32_t r, uint64_t* kw) {
+    constexpr int XO = JOB ? 1 : 0;
+    if (JOB) kw[0] = N.j[r];
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        kw[XO + k] = N.x[(int64_t)k * N.cap + r];
+        kw[XO + AS + k] = N.z[(int64_t)k * N.cap + r];
+    }
+    kw[2 * AS + XO] = (uint64_t)(N.f[r] & 1u) << 56;
+}
+
+// ---- 3a. merge: lower bounds in the other list, sums, keep flags.  Threads
+// [0, n) take the old terms, [n, 2n) the new ones (A[n] of them).
+template <int AS, bool JOB>
+__global__ __launch_bounds__(kBlock) void k_rs_merge(Sorted<AS, JOB> S, uint32_t n, RotParams<AS> rp, JobRotTables jt,
+                                                     const uint32_t* __restrict__ a, const uint64_t* __restrict__ A,
+                                                     NewTerms N, Filters fl, uint32_t* __restrict__ keep,
+                                                     uint32_t* __restrict__ lb_out, double* __restrict__ val) {
+    constexpr int KW = Sorted<AS, JOB>::KW;
+    constexpr int XO = Sorted<AS, JOB>::XO;
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= 2ull * n) return;
+    const uint32_t nn = (uint32_t)A[n];
+    uint64_t kw[KW + 1], ow[KW + 1];
+    if (gid < n) {
+        const uint32_t i = (uint32_t)gid;
+        S.key(i, kw);
+        double vo = S.value(i);
+        if (a[i]) {
+            const uint64_t *px, *pz;
+            uint32_t psf;
+            double c, sn;
+            rot_of<AS, JOB>(rp, jt, S.job(i), px, pz, psf, c, sn);
+            vo = vo * c;
+        }
+        uint32_t lo = 0, cnt = nn;  // first new term >= k_i
+        while (cnt > 0) {
+            const uint32_t half = cnt / 2, t = lo + half;
+            new_key<AS, JOB>(N, t, ow);
+            if (key_less<KW>(ow, kw)) {
+                lo = t + 1;
+                cnt -= half + 1;
+            } else {
+                cnt = half;
+            }
+        }
+        double acc = 0.0;
+        acc += vo;
+        if (lo < nn) {
+            new_key<AS, JOB>(N, lo, ow);
+            if (key_eq<KW>(ow, kw)) acc += N.v[lo];
+        }
+        keep[i] = keep_of<AS, XO>(fl, kw, acc) ? 1u : 0u;
+        lb_out[i] = lo;
+        val[i] = acc;
+        return;
+    }
+    const uint32_t r = (uint32_t)(gid - n);
+    if (r >= nn) {
+        keep[gid] = 0u;
+        return;
+    }
+    new_key<AS, JOB>(N, r, kw);
+    uint32_t lo = 0, cnt = n;  // first old term >= n_r
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = lo + half;
+        S.key(t, ow);
+        if (key_less<KW>(ow, kw)) {
+            lo = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    bool dup = false;
+    if (lo < n) {
+        S.key(lo, ow);
+        dup = key_eq<KW>(ow, kw);
+    }
+    keep[gid] = (!dup && keep_of<AS, XO>(fl, kw, N.v[r])) ? 1u : 0u;
+    lb_out[gid] = lo;
+}
+
+// ---- 3b. kept terms to their output positions (C: exclusive scan of keep
+// over both lists).
+template <int AS, bool JOB>
+__global__ __launch_bounds__(kBlock) void k_rs_write(Sorted<AS, JOB> S, uint32_t n, const uint64_t* __restrict__ A,
+                                                     NewTerms N, const uint32_t* __restrict__ keep,
+                                                     const uint32_t* __restrict__ lb, const double* __restrict__ val,
+                                                     const uint64_t* __restrict__ C, int out_as, uint64_t* __restrict__ ox,
+                                                     uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
+                                                     double* __restrict__ ov, uint32_t* __restrict__ oj, int64_t os) {
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= 2ull * n || !keep[gid]) return;
+    const uint64_t Cn = C[n];
+    if (gid < n) {
+        const uint32_t i = (uint32_t)gid;
+        const uint64_t o = C[i] + (C[(uint64_t)n + lb[i]] - Cn);
+        if (o >= C[2ull * n]) return;  // never for consistent lower bounds; keeps writes in bounds
+        for (int k = 0; k < out_as; ++k) {
+            ox[(int64_t)k * os + o] = S.x[(int64_t)k * S.stride + i];
+            oz[(int64_t)k * os + o] = S.z[(int64_t)k * S.stride + i];
+        }
+        of[o] = S.f[i] & 1u;
+        ov[o] = val[i];
+        if (JOB) oj[o] = S.j[i];
+        return;
+    }
+    const uint32_t r = (uint32_t)(gid - n);
+    const uint64_t o = (C[gid] - Cn) + C[lb[gid]];
+    if (o >= C[2ull * n]) return;
+    for (int k = 0; k < out_as; ++k) {
+        ox[(int64_t)k * os + o] = N.x[(int64_t)k * N.cap + r];
+        oz[(int64_t)k * os + o] = N.z[(int64_t)k * N.cap + r];
+    }
+    of[o] = N.f[r];
+    ov[o] = N.v[r];
+    if (JOB) oj[o] = N.j[r];
+}
+
+size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+template <int AS, bool JOB>
+int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const JobRotTables& jt, const Filters& fl) {
+    hipStream_t st = ctx->stream;
+    const uint64_t n = (uint64_t)s->n;
+    Sorted<AS, JOB> S{s->x, s->z, s->f, s->v, s->j, s->cap, s->as};
+    // scratch: a (u32 n) and A (u64 n + 1); the new terms (n each plane);
+    // keep (u32 2n), lower bounds (u32 2n), old sums (f64 n), C (u64 2n + 1)
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsRotA, up256(n * 4) + (n + 1) * 8, &p));
+    uint32_t* a = (uint32_t*)p;
+    uint64_t* A = (uint64_t*)((char*)p + up256(n * 4));
+    const size_t nb_planes = up256(n * 8 * AS);
+    PLH_TRY(ctx->ws.get(kWsRotN, 2 * nb_planes + up256(n * 8) + up256(n) + up256(n * 4), &p));
+    NewTerms N;
+    N.x = (uint64_t*)p;
+    N.z = (uint64_t*)((char*)p + nb_planes);
+    N.v = (double*)((char*)p + 2 * nb_planes);
+    N.f = (uint8_t*)((char*)N.v + up256(n * 8));
+    N.j = (uint32_t*)((char*)N.f + up256(n));
+    N.cap = (int64_t)n;
+    PLH_TRY(ctx->ws.get(kWsRotM, up256(n * 8) + up256(n * 8) + up256(n * 8) + (2 * n + 1) * 8, &p));
+    uint32_t* keep = (uint32_t*)p;
+    uint32_t* lb = (uint32_t*)((char*)p + up256(n * 8));
+    double* val = (double*)((char*)p + 2 * up256(n * 8));
+    uint64_t* C = (uint64_t*)((char*)p + 3 * up256(n * 8));
+    const unsigned g1 = blocks_of(n), g2 = blocks_of(2 * n);
+    {
+        ProfScope ps(ctx, "rot_anti");
+        k_rs_anti<AS, JOB><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a);
+    }
+    PLH_HIP(hipGetLastError());
+    PLH_TRY(scan_u32_to_u64(ctx, a, A, n, A + n));
+    {
+        ProfScope ps(ctx, "rot_new");
+        k_rs_new<AS, JOB><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a, A, N);
+    }
+    {
+        ProfScope ps(ctx, "rot_merge");
+        k_rs_merge<AS, JOB><<<g2, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a, A, N, fl, keep, lb, val);
+    }
+    PLH_HIP(hipGetLastError());
+    uint64_t total = 0;
+    PLH_TRY(scan_u32_to_u64_read(ctx, keep, C, 2 * n, C + 2 * n, &total));
+    plh_sum tmp;
+    tmp.ctx = ctx;
+    tmp.as = s->as;
+    tmp.nq = s->nq;
+    tmp.has_job = s->has_job;
+    PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(total, 1), 0));
+    if (total) {
+        ProfScope ps(ctx, "rot_write", (double)total * (16.0 * s->as + 9.0));
+        k_rs_write<AS, JOB><<<g2, kBlock, 0, st>>>(S, (uint32_t)n, A, N, keep, lb, val, C, s->as, tmp.x, tmp.z,
+                                                   tmp.f, tmp.v, tmp.j, tmp.cap);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            sum_free_storage(&tmp);
+            PLH_HIP(e);
+        }
+    }
+    const int64_t nq = s->nq;
+    sum_take_storage(s, &tmp);
+    s->n = (int64_t)total;
+    s->nq = nq;
+    s->canonical = true;
+    s->sorted = true;
+    return PLH_OK;
+}
+
+Filters filters_of(const CollateSpec& spec) {
+    return Filters{spec.mode, spec.max_weight, spec.eps, spec.job_eps, spec.job_maxw, spec.job_alive};
+}
+
+}  // namespace
+
+int rotate_sorted(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint32_t psf, double c,
+                  double sn, const CollateSpec& spec) {
+    if (!s->sorted || s->has_job || (spec.mode & (kFold | kDropZero)) != (kFold | kDropZero)) {
+        set_error("rotate_sorted: needs a sorted canonical single sum and a canonicalizing spec");
+        return PLH_ERR_ARG;
+    }
+    if (s->n > (int64_t)0x7FFFFFFFll) {
+        set_error("rotate_sorted: at most 2^31 - 1 terms");
+        return PLH_ERR_SIZE;
+    }
+    if (s->n == 0) return PLH_OK;
+    const Filters fl = filters_of(spec);
+    switch (s->as <= 1 ? 1 : s->as == 2 ? 2 : s->as <= 4 ? 4 : s->as <= 8 ? 8 : 16) {
+#define PLH_RS(ASV)                                                 \
+    case ASV: {                                                     \
+        RotParams<ASV> rp{};                                        \
+        for (int k = 0; k < s->as; ++k) {                           \
+            rp.px[k] = px[k];                                       \
+            rp.pz[k] = pz[k];                                       \
+        }                                                           \
+        rp.psf = psf;                                               \
+        rp.c = c;                                                   \
+        rp.s = sn;                                                  \
+        return rotate_sorted_as<ASV, false>(ctx, s, rp, JobRotTables{}, fl); \
+    }
+        PLH_RS(1)
+        PLH_RS(2)
+        PLH_RS(4)
+        PLH_RS(8)
+        PLH_RS(16)
+#undef PLH_RS
+        default: return PLH_ERR_AS;
+    }
+}
+
+int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, const CollateSpec& spec) {
+    if (!s->sorted || !s->has_job || (spec.mode & (kFold | kDropZero)) != (kFold | kDropZero)) {
+        set_error("rotate_sorted_jobs: needs a sorted canonical batched sum and a canonicalizing spec");
+        return PLH_ERR_ARG;
+    }
+    if (s->n > (int64_t)0x7FFFFFFFll) {
+        set_error("rotate_sorted_jobs: at most 2^31 - 1 terms");
+        return PLH_ERR_SIZE;
+    }
+    if (s->n == 0) return PLH_OK;
+    const Filters fl = filters_of(spec);
+    if (s->as == 1) return rotate_sorted_as<1, true>(ctx, s, RotParams<1>{}, jt, fl);
+    if (s->as == 2) return rotate_sorted_as<2, true>(ctx, s, RotParams<2>{}, jt, fl);
+    set_error("rotate_sorted_jobs: batched sums support up to 128 qubits");
+    return PLH_ERR_AS;
+}
+
+}  // namespace plh

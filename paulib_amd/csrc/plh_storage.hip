@@ -64,6 +64,7 @@ int sum_reserve(plh_sum* s, int64_t cap, int64_t keep) {
         sum_free_storage(s);
         s->n = 0;
         s->canonical = false;
+        s->sorted = false;
     }
     if (!s->ctx) {
         set_error("the sum's context has been destroyed");
@@ -120,6 +121,7 @@ int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src) {
         PLH_HIP(hipMemcpyAsync(dst->j + n0, src->j, (size_t)src->n * 4, hipMemcpyDeviceToDevice, st));
     dst->n = n0 + src->n;
     dst->canonical = false;
+    dst->sorted = false;
     return PLH_OK;
 }
 
