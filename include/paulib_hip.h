@@ -297,12 +297,15 @@ int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t
 /* ---- Batched propagation server (continuous batching).
  * No reference counterpart: the reference runs one propagate() per call
  * (include/PauliPropagation.h:53-71).  A server keeps every admitted job's
- * terms in one device-resident sum tagged by job slot; each plh_server_step
- * advances every running job by its Clifford gates up to and including its
- * next rotation (evolve + apply_truncation with the job's own max_weight and
- * cutoff, exactly as plh_run_program), in a fixed number of launches for all
- * jobs together.  Jobs are admitted at step boundaries while a slot is free
- * and the batch fits the HBM budget; a finished job's result is its
+ * terms in one device-resident sum tagged by job slot, sorted by (job, key).
+ * A full plh_server_step advances every running job by its Clifford gates up
+ * to and including its next rotation (evolve + apply_truncation with the
+ * job's own max_weight and cutoff, exactly as plh_run_program) and collates
+ * the batch; a sorted step advances, without a sort, only the jobs whose next
+ * op is a rotation, the others waiting (at most 24 steps) for the next full
+ * step; either takes a fixed number of launches for all jobs together.  Jobs
+ * are admitted at full steps while a slot is free and the batch fits the HBM
+ * budget; a finished job's result is its
  * zero_state_expectation (include/PauliSumUtils.h:162-177) and, with
  * keep_terms, its final sum in canonical order.  A job's result does not
  * depend on which other jobs share its steps. */

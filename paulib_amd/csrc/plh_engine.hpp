@@ -304,9 +304,10 @@ int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, con
 // keys), without a sort (plh_rotate.hip); psf = flags of ps.
 int rotate_sorted(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint32_t psf, double c,
                   double sn, const CollateSpec& spec);
-// The same for a canonical batched sum, each job by its own table entry
-// (jobs whose kind is not 4 pass through, under their own filters).
-int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, const CollateSpec& spec);
+// The same for a canonical batched sum of job slots 0 .. njobs - 1, each job
+// by its own table entry (jobs whose kind is not 4 pass through, under their
+// own filters).
+int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, int njobs, const CollateSpec& spec);
 // out = canonicalize(lhs[k] * rhs[k]) for k < njobs as one batched sum
 // (job plane = k, jobs in order); offsets[k] .. offsets[k + 1] is job k's
 // range (njobs + 1 entries, host).  Sum of n_k * m_k <= 2^32.

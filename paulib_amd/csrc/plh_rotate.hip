@@ -181,6 +181,24 @@ struct NewTerms {
     int64_t cap;
 };
 
+// Job ranges of a batched sum sorted by job: off[k] = first term of job k
+// (k <= njobs).  Searches of the rotation stay inside an item's own job.
+__global__ void k_rs_bounds(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs, uint32_t* __restrict__ off) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > njobs) return;
+    uint32_t lo = 0, cnt = n;
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = lo + half;
+        if (J[t] < k) {
+            lo = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    off[k] = lo;
+}
+
 // ---- 1. anticommuting flags
 template <int AS, bool JOB>
 __global__ __launch_bounds__(kBlock) void k_rs_anti(Sorted<AS, JOB> S, uint32_t n, RotParams<AS> rp, JobRotTables jt,
@@ -201,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_anti(Sorted<AS, JOB> S, uint32_t 
 template <int AS, bool JOB>
 __global__ __launch_bounds__(kBlock) void k_rs_new(Sorted<AS, JOB> S, uint32_t n, RotParams<AS> rp, JobRotTables jt,
                                                    const uint32_t* __restrict__ a, const uint64_t* __restrict__ A,
-                                                   NewTerms N) {
+                                                   const uint32_t* __restrict__ joff, NewTerms N) {
     constexpr int KW = Sorted<AS, JOB>::KW;
     constexpr int XO = Sorted<AS, JOB>::XO;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -211,7 +229,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_new(Sorted<AS, JOB> S, uint32_t n
     const uint64_t *px, *pz;
     uint32_t psf;
     double c, sn;
-    rot_of<AS, JOB>(rp, jt, S.job(i), px, pz, psf, c, sn);
+    const uint32_t job = S.job(i);
+    rot_of<AS, JOB>(rp, jt, job, px, pz, psf, c, sn);
+    // the item's job range (the groups below never leave it)
+    const uint32_t base = JOB ? joff[job] : 0u, lim = JOB ? joff[job + 1] : n;
     int64_t r = (int64_t)A[i];
     // each set bit b of M: x bits then z bits, any order
 #pragma unroll
@@ -223,8 +244,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_new(Sorted<AS, JOB> S, uint32_t n
             m &= ~(1ull << bit);
             const int g = wbase + (63 - bit);
             if (bit_at(kw, g)) {
-                // group start: first t in [0, i] with the same bits above g
-                uint32_t lo = 0, cnt = i;
+                // group start: first t in [base, i] with the same bits above g
+                uint32_t lo = base, cnt = i - base;
                 while (cnt > 0) {
                     const uint32_t half = cnt / 2, t = lo + half;
                     if (prefix_cmp(S, t, kw, g) < 0) {
@@ -248,8 +269,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_new(Sorted<AS, JOB> S, uint32_t n
                 }
                 r -= (int64_t)(A[mid] - A[lo]);
             } else {
-                // group end: first t in (i, n) with larger bits above g
-                uint32_t hi = i + 1, cnt = n - (i + 1);
+                // group end: first t in (i, lim) with larger bits above g
+                uint32_t hi = i + 1, cnt = lim - (i + 1);
                 while (cnt > 0) {
                     const uint32_t half = cnt / 2, t = hi + half;
                     if (prefix_cmp(S, t, kw, g) <= 0) {
@@ -309,8 +330,9 @@ PLH_DEV void new_key(const NewTerms& N, uint32_t r, uint64_t* kw) {
 template <int AS, bool JOB>
 __global__ __launch_bounds__(kBlock) void k_rs_merge(Sorted<AS, JOB> S, uint32_t n, RotParams<AS> rp, JobRotTables jt,
                                                      const uint32_t* __restrict__ a, const uint64_t* __restrict__ A,
-                                                     NewTerms N, Filters fl, uint32_t* __restrict__ keep,
-                                                     uint32_t* __restrict__ lb_out, double* __restrict__ val) {
+                                                     const uint32_t* __restrict__ joff, NewTerms N, Filters fl,
+                                                     uint32_t* __restrict__ keep, uint32_t* __restrict__ lb_out,
+                                                     double* __restrict__ val) {
     constexpr int KW = Sorted<AS, JOB>::KW;
     constexpr int XO = Sorted<AS, JOB>::XO;
     const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -328,7 +350,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_merge(Sorted<AS, JOB> S, uint32_t
             rot_of<AS, JOB>(rp, jt, S.job(i), px, pz, psf, c, sn);
             vo = vo * c;
         }
-        uint32_t lo = 0, cnt = nn;  // first new term >= k_i
+        // first new term >= k_i, inside the job's new terms
+        const uint32_t job = S.job(i);
+        uint32_t lo = JOB ? (uint32_t)A[joff[job]] : 0u;
+        uint32_t cnt = (JOB ? (uint32_t)A[joff[job + 1]] : nn) - lo;
         while (cnt > 0) {
             const uint32_t half = cnt / 2, t = lo + half;
             new_key<AS, JOB>(N, t, ow);
@@ -341,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_merge(Sorted<AS, JOB> S, uint32_t
         }
         double acc = 0.0;
         acc += vo;
-        if (lo < nn) {
+        if (lo < (JOB ? (uint32_t)A[joff[job + 1]] : nn)) {
             new_key<AS, JOB>(N, lo, ow);
             if (key_eq<KW>(ow, kw)) acc += N.v[lo];
         }
@@ -356,7 +381,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_merge(Sorted<AS, JOB> S, uint32_t
         return;
     }
     new_key<AS, JOB>(N, r, kw);
-    uint32_t lo = 0, cnt = n;  // first old term >= n_r
+    // first old term >= n_r, inside the job's old terms
+    const uint32_t job = JOB ? N.j[r] : 0u;
+    uint32_t lo = JOB ? joff[job] : 0u;
+    uint32_t cnt = (JOB ? joff[job + 1] : n) - lo;
     while (cnt > 0) {
         const uint32_t half = cnt / 2, t = lo + half;
         S.key(t, ow);
@@ -368,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_merge(Sorted<AS, JOB> S, uint32_t
         }
     }
     bool dup = false;
-    if (lo < n) {
+    if (lo < (JOB ? joff[job + 1] : n)) {
         S.key(lo, ow);
         dup = key_eq<KW>(ow, kw);
     }
@@ -416,7 +444,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_write(Sorted<AS, JOB> S, uint32_t
 size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 template <int AS, bool JOB>
-int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const JobRotTables& jt, const Filters& fl) {
+int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const JobRotTables& jt, const Filters& fl,
+                     uint32_t njobs) {
     hipStream_t st = ctx->stream;
     const uint64_t n = (uint64_t)s->n;
     Sorted<AS, JOB> S{s->x, s->z, s->f, s->v, s->j, s->cap, s->as};
@@ -435,12 +464,18 @@ int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const Jo
     N.f = (uint8_t*)((char*)N.v + up256(n * 8));
     N.j = (uint32_t*)((char*)N.f + up256(n));
     N.cap = (int64_t)n;
-    PLH_TRY(ctx->ws.get(kWsRotM, up256(n * 8) + up256(n * 8) + up256(n * 8) + (2 * n + 1) * 8, &p));
+    PLH_TRY(ctx->ws.get(kWsRotM, up256(n * 8) + up256(n * 8) + up256(n * 8) + up256((2 * n + 1) * 8) +
+                                     ((size_t)njobs + 1) * 4, &p));
     uint32_t* keep = (uint32_t*)p;
     uint32_t* lb = (uint32_t*)((char*)p + up256(n * 8));
     double* val = (double*)((char*)p + 2 * up256(n * 8));
     uint64_t* C = (uint64_t*)((char*)p + 3 * up256(n * 8));
+    uint32_t* joff = (uint32_t*)((char*)C + up256((2 * n + 1) * 8));
     const unsigned g1 = blocks_of(n), g2 = blocks_of(2 * n);
+    if (JOB) {
+        k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, joff);
+        PLH_HIP(hipGetLastError());
+    }
     {
         ProfScope ps(ctx, "rot_anti");
         k_rs_anti<AS, JOB><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a);
@@ -449,11 +484,11 @@ int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const Jo
     PLH_TRY(scan_u32_to_u64(ctx, a, A, n, A + n));
     {
         ProfScope ps(ctx, "rot_new");
-        k_rs_new<AS, JOB><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a, A, N);
+        k_rs_new<AS, JOB><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a, A, joff, N);
     }
     {
         ProfScope ps(ctx, "rot_merge");
-        k_rs_merge<AS, JOB><<<g2, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a, A, N, fl, keep, lb, val);
+        k_rs_merge<AS, JOB><<<g2, kBlock, 0, st>>>(S, (uint32_t)n, rp, jt, a, A, joff, N, fl, keep, lb, val);
     }
     PLH_HIP(hipGetLastError());
     uint64_t total = 0;
@@ -512,7 +547,7 @@ int rotate_sorted(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* 
         rp.psf = psf;                                               \
         rp.c = c;                                                   \
         rp.s = sn;                                                  \
-        return rotate_sorted_as<ASV, false>(ctx, s, rp, JobRotTables{}, fl); \
+        return rotate_sorted_as<ASV, false>(ctx, s, rp, JobRotTables{}, fl, 0); \
     }
         PLH_RS(1)
         PLH_RS(2)
@@ -524,7 +559,7 @@ int rotate_sorted(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* 
     }
 }
 
-int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, const CollateSpec& spec) {
+int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, int njobs, const CollateSpec& spec) {
     if (!s->sorted || !s->has_job || (spec.mode & (kFold | kDropZero)) != (kFold | kDropZero)) {
         set_error("rotate_sorted_jobs: needs a sorted canonical batched sum and a canonicalizing spec");
         return PLH_ERR_ARG;
@@ -535,8 +570,9 @@ int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, const C
     }
     if (s->n == 0) return PLH_OK;
     const Filters fl = filters_of(spec);
-    if (s->as == 1) return rotate_sorted_as<1, true>(ctx, s, RotParams<1>{}, jt, fl);
-    if (s->as == 2) return rotate_sorted_as<2, true>(ctx, s, RotParams<2>{}, jt, fl);
+    if (njobs < 1) return PLH_ERR_ARG;
+    if (s->as == 1) return rotate_sorted_as<1, true>(ctx, s, RotParams<1>{}, jt, fl, (uint32_t)njobs);
+    if (s->as == 2) return rotate_sorted_as<2, true>(ctx, s, RotParams<2>{}, jt, fl, (uint32_t)njobs);
     set_error("rotate_sorted_jobs: batched sums support up to 128 qubits");
     return PLH_ERR_AS;
 }

@@ -53,6 +53,14 @@ struct Job {
 
 enum SlotState : int { kFree = 0, kRunning = 1, kDraining = 2 };
 
+// A full step advances every running job by its Clifford ops up to and
+// including its next rotation and collates the batch; a sorted step keeps
+// the batch sorted: only jobs whose next op is a rotation that keeps the
+// order (its truncation alone, or the general path) advance, by a sort-free
+// rotation, and the others (a Clifford op or a key-moving fast path next)
+// wait for the next full step with their terms untouched.
+enum StepKind : int { kFullStep = 0, kSortedStep = 1 };
+
 // One step's device tables (one entry per slot) and its effect on the jobs.
 struct StepPlan {
     std::vector<uint32_t> off, ops;  // ops[off[i] .. off[i+1]) are slot i's Clifford ops
@@ -123,8 +131,35 @@ class Scheduler {
         state_[(size_t)slot] = kRunning;
     }
 
-    // This step's tables; no job changes.
-    void plan(StepPlan* p) const {
+    // Whether a running job can advance in a sorted step.
+    bool sorted_ready(int slot) const {
+        const Job& j = active_[(size_t)slot];
+        if (state_[(size_t)slot] != kRunning || j.next < 0 || j.kind[(size_t)j.next] != 3) return false;
+        double c, sn;
+        angle_cos_sin(j.theta[(size_t)j.next], &c, &sn);
+        const uint8_t k = rotation_kind_of(c, sn);
+        return k == 0 || k == 4;
+    }
+    int sorted_ready_count() const {
+        int r = 0;
+        for (int i = 0; i < slots_; ++i) r += sorted_ready(i);
+        return r;
+    }
+
+    // The next step's kind: a sorted step while the batch is sorted and at
+    // least four in five running jobs can advance without a sort, a full
+    // step otherwise and after max_sorted sorted steps in a row (so a job
+    // waiting at a Clifford op, and a queued job, waits at most that long).
+    StepKind choose_step(bool batch_sorted, int64_t batch_terms, int since_full, int max_sorted) const {
+        const int run = running(), ready = sorted_ready_count();
+        const bool full = !batch_sorted || batch_terms == 0 || ready == 0 || 5 * (run - ready) >= run ||
+                          since_full >= max_sorted;
+        return full ? kFullStep : kSortedStep;
+    }
+
+    // This step's tables; no job changes.  In a sorted step the jobs that
+    // cannot advance keep their place: no ops, no rotation, no truncation.
+    void plan(StepPlan* p, StepKind kind = kFullStep) const {
         const size_t S = (size_t)slots_, A = (size_t)as_;
         p->off.assign(S + 1, 0);
         p->ops.clear();
@@ -146,6 +181,10 @@ class Scheduler {
             p->alive[i] = 1;
             const Job& j = active_[i];
             int64_t next = j.next;
+            if (kind == kSortedStep && !sorted_ready((int)i)) {
+                p->next[i] = next;  // waits for the next full step
+                continue;
+            }
             while (next >= 0 && j.kind[(size_t)next] != 3) {
                 const int64_t l = next--;
                 const uint32_t k = (uint32_t)j.kind[(size_t)l];

@@ -40,6 +40,9 @@ namespace {
 
 constexpr int kIpt = 16;
 constexpr uint64_t kTile = (uint64_t)kBlock * kIpt;
+// Sorted steps between full steps at most: a job waiting at a Clifford op
+// (or admitted) waits at most this many steps.
+constexpr int kMaxSortedSteps = 24;
 
 // Per-step job tables (device, one upload per step), one entry per slot.
 struct StepTables {
@@ -270,6 +273,7 @@ struct plh_server {
     std::vector<uint8_t> h_step;
     std::chrono::steady_clock::time_point t0;
     int64_t steps = 0;
+    int since_full = 0;  // sorted steps since the last full step
     // set when a step's device work failed part-way: the batch may hold
     // half-applied gates, so the server refuses further steps
     bool failed = false;
@@ -342,7 +346,7 @@ int admit(plh_server* s, const Job& j, int slot) {
 // The device part of a step: table upload, gates, one collation for every
 // job, and the finished jobs' expectations (and terms); their results go
 // to *res.  Nothing of the scheduler changes here.
-int step_device(plh_server* s, const sched::StepPlan& pl, std::vector<Result>* res) {
+int step_device(plh_server* s, const sched::StepPlan& pl, sched::StepKind kind, std::vector<Result>* res) {
     plh_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     const int AS = s->as;
@@ -388,7 +392,19 @@ int step_device(plh_server* s, const sched::StepPlan& pl, std::vector<Result>* r
     PLH_HIP(hipMemcpyAsync(s->d_step, s->h_step.data(), total, hipMemcpyHostToDevice, st));
 
     const int64_t n = B->n;
-    if (n > 0) {
+    if (n > 0 && kind == sched::kSortedStep) {
+        // ---- every advancing job's rotation without a sort; the waiting jobs
+        // pass through, drained jobs leave
+        CollateSpec sc;
+        sc.mode = kFold | kDropZero | kMaxWeight | kCutoff;
+        sc.max_weight = -1;
+        sc.eps = 0.0;
+        sc.job_eps = tb.eps;
+        sc.job_maxw = tb.maxw;
+        sc.job_alive = tb.alive;
+        JobRotTables jt{tb.rot_kind, tb.rot_p, tb.rot_psf, tb.rot_c, tb.rot_s};
+        PLH_TRY(rotate_sorted_jobs(ctx, B, jt, s->sched->slots(), sc));
+    } else if (n > 0) {
         // ---- gates: Cliffords + fast rotations in place
         const uint64_t nb = blocks_for((uint64_t)n, kTile);
         {
@@ -607,22 +623,29 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
     sched::Scheduler& sc = *s->sched;
     plh_sum* B = s->batch;
 
-    // ---- admission: a free slot, and the batch with the new job fits the HBM
-    // budget.  A job that cannot be appended stays queued; the server stays
-    // usable (the batch holds what it held).
-    int64_t in_flight = B->n;
-    for (int slot; (slot = sc.admission_slot(in_flight)) >= 0;) {
-        const int64_t n0 = sc.queue().front().n0;
-        PLH_TRY(admit(s, sc.queue().front(), slot));
-        sc.admitted(slot);
-        in_flight += n0;
+    // ---- the step's kind (Scheduler::choose_step)
+    const sched::StepKind kind = sc.choose_step(B->sorted, B->n, s->since_full, kMaxSortedSteps);
+    const bool full = kind == sched::kFullStep;
+
+    // ---- admission (full steps only: new terms are appended unsorted): a
+    // free slot, and the batch with the new job fits the HBM budget.  A job
+    // that cannot be appended stays queued; the server stays usable (the
+    // batch holds what it held).
+    if (full) {
+        int64_t in_flight = B->n;
+        for (int slot; (slot = sc.admission_slot(in_flight)) >= 0;) {
+            const int64_t n0 = sc.queue().front().n0;
+            PLH_TRY(admit(s, sc.queue().front(), slot));
+            sc.admitted(slot);
+            in_flight += n0;
+        }
     }
 
     // ---- the step: planned, run on the device, then committed
     sched::StepPlan pl;
-    sc.plan(&pl);
+    sc.plan(&pl, kind);
     std::vector<Result> res;
-    const int rc = step_device(s, pl, &res);
+    const int rc = step_device(s, pl, kind, &res);
     if (rc != PLH_OK) {
         s->failed = true;
         s->failure = last_error();
@@ -642,6 +665,7 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
         }
     }
     if (sc.commit(pl)) B->n = 0;  // nothing running: drained terms go without a collation
+    s->since_full = full ? 0 : s->since_full + 1;
     ++s->steps;
     if (active_out) *active_out = sc.running();
     if (queued_out) *queued_out = (int)sc.queue().size();

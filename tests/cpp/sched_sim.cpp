@@ -107,6 +107,9 @@ struct Replay {
     std::map<int64_t, int64_t> finished;  // handle -> final terms
     std::map<int64_t, int64_t> applied;   // handle -> next op expected (Heisenberg order)
     int admission_errors = 0;
+    int since_full = 0;
+    int sorted_steps = 0;
+    static constexpr int kMaxSorted = 24;  // plh_serve.hip kMaxSortedSteps
 
     Replay(int as, int slots, int64_t budget, uint64_t seed) : sc(as, slots, budget), rng(seed) {
         ctx.heap.set_device(0);
@@ -172,7 +175,7 @@ struct Replay {
 
     // step_device(): gates, one collation (with_output + collate_run's
     // storage), the finished jobs' ranges.
-    int device(const StepPlan& pl, std::map<int, int64_t>* after) {
+    int device(const StepPlan& pl, plh::sched::StepKind kind, std::map<int, int64_t>* after) {
         const int64_t n = batch.n;
         // the terms each slot holds once this step's collation has run
         after->clear();
@@ -188,7 +191,34 @@ struct Replay {
             out += c * scale;
             if (pl.rk[(size_t)slot] == 4) n_valid += kv.second * scale;  // candidate new terms
         }
-        if (n > 0) {
+        if (n > 0 && kind == plh::sched::kSortedStep) {
+            // plh_rotate.hip rotate_sorted_jobs: scratch, then the output block
+            void* p = nullptr;
+            PLH_TRY(ctx.ws.get(plh::kWsRotA, (size_t)n * 4 + 256 + (size_t)(n + 1) * 8, &p));
+            PLH_TRY(ctx.ws.get(plh::kWsRotN, (size_t)n * (16 * sc.as() + 13) + 1024, &p));
+            PLH_TRY(ctx.ws.get(plh::kWsRotM, (size_t)n * 24 + 1024 + (size_t)(2 * n + 1) * 8, &p));
+            if (injected()) return device_error();
+            sim::check_live(batch.x, (size_t)n * 8, "sorted rotation reads");
+            sim::check_live(batch.j, (size_t)n * 4, "sorted rotation reads");
+            plh_sum tmp;
+            tmp.ctx = &ctx;
+            tmp.as = batch.as;
+            tmp.nq = batch.nq;
+            tmp.has_job = true;
+            const int rc = plh::sum_reserve(&tmp, std::max<int64_t>(out, 1), 0);
+            if (rc != PLH_OK) {
+                plh::sum_free_storage(&tmp);
+                return rc;
+            }
+            if (out) sim::check_live(tmp.v, (size_t)out * 8, "sorted rotation writes");
+            if (injected()) {
+                plh::sum_free_storage(&tmp);
+                return device_error();
+            }
+            plh::sum_take_storage(&batch, &tmp);
+            batch.n = out;
+            batch.sorted = true;
+        } else if (n > 0) {
             if (injected()) return device_error();
             sim::check_live(batch.x, (size_t)n * 8, "gates");
             sim::check_live(batch.f, (size_t)n, "gates");
@@ -236,6 +266,7 @@ struct Replay {
             }
             plh::sum_take_storage(&batch, &tmp);
             batch.n = out;
+            batch.sorted = true;
         }
         if (pl.finishing.empty()) return PLH_OK;
         if (out > 0) {
@@ -259,8 +290,11 @@ struct Replay {
     // One plh_server_step: returns its status.
     int step() {
         if (failed) return PLH_ERR_STATE;
+        const plh::sched::StepKind kind = sc.choose_step(batch.sorted, batch.n, since_full, kMaxSorted);
+        const bool full = kind == plh::sched::kFullStep;
+        if (!full && !batch.sorted) fail("a sorted step on an unsorted batch");
         int64_t in_flight = batch.n / scale;  // the scheduler counts unscaled terms
-        for (int slot; (slot = sc.admission_slot(in_flight)) >= 0;) {
+        for (int slot; full && (slot = sc.admission_slot(in_flight)) >= 0;) {
             if (shadow.count(slot)) fail("slot %lld reused while its previous job's terms are in the batch", slot);
             const Job& q = sc.queue().front();
             const int64_t h = q.handle, n0 = q.n0, nb = batch.n;
@@ -279,11 +313,11 @@ struct Replay {
             in_flight += n0;
         }
         StepPlan pl;
-        sc.plan(&pl);
-        verify_plan(pl);
+        sc.plan(&pl, kind);
+        verify_plan(pl, kind);
         std::map<int, int64_t> after;
         const int64_t nb = batch.n;
-        const int rc = device(pl, &after);
+        const int rc = device(pl, kind, &after);
         if (rc != PLH_OK) {
             // before the collation replaced the batch it is unchanged;
             // after, it is the collation's result; consistent either way
@@ -305,13 +339,16 @@ struct Replay {
             batch.n = 0;
             shadow.clear();
         }
+        since_full = full ? 0 : since_full + 1;
+        sorted_steps += !full;
+        if (since_full > kMaxSorted) fail("more sorted steps in a row than the bound");
         check_batch("after a step");
         return PLH_OK;
     }
 
     // The plan's ops for each running job are its program's next ops, in
     // reverse order, up to and including one rotation.
-    void verify_plan(const StepPlan& pl) {
+    void verify_plan(const StepPlan& pl, plh::sched::StepKind kind) {
         for (int i = 0; i < sc.slots(); ++i) {
             if (sc.state(i) != plh::sched::kRunning) {
                 if (pl.off[(size_t)i] != pl.off[(size_t)i + 1] || pl.alive[(size_t)i]) fail("a slot that does not run has ops");
@@ -321,6 +358,18 @@ struct Replay {
             auto it = applied.find(j.handle);
             const int64_t expect_next = it == applied.end() ? (int64_t)j.kind.size() - 1 : it->second;
             if (j.next != expect_next) fail("job %lld resumes at op %lld", j.handle, j.next);
+            if (kind == plh::sched::kSortedStep) {
+                if (pl.off[(size_t)i] != pl.off[(size_t)i + 1]) fail("Clifford ops in a sorted step (job %lld)", j.handle);
+                if (!sc.sorted_ready(i)) {  // waits: untouched
+                    if (pl.rk[(size_t)i] != 0 || pl.maxw[(size_t)i] != -1 || pl.eps[(size_t)i] != 0.0 ||
+                        pl.next[(size_t)i] != j.next || !pl.alive[(size_t)i])
+                        fail("a waiting job is not left alone (job %lld)", j.handle);
+                    if (std::find(pl.finishing.begin(), pl.finishing.end(), i) != pl.finishing.end())
+                        fail("a waiting job finishes (job %lld)", j.handle);
+                    continue;
+                }
+                if (pl.rk[(size_t)i] != 0 && pl.rk[(size_t)i] != 4) fail("a key-moving rotation in a sorted step");
+            }
             int64_t l = j.next;
             for (uint32_t o = pl.off[(size_t)i]; o < pl.off[(size_t)i + 1]; ++o, --l) {
                 if (l < 0) fail("job %lld planned past its first op", j.handle);
@@ -398,6 +447,8 @@ std::vector<JobSpec> read_jobs(FILE* in, int* slots, int* as) {
     return jobs;
 }
 
+int g_sorted_steps = 0;
+
 // Submits every job, then steps until all are done or the server fails; a
 // failed admission is retried at the next step, as a caller would.
 void run(const std::vector<JobSpec>& jobs, int slots, int as, int64_t scale, size_t cap_chunks, double p_fail,
@@ -430,6 +481,7 @@ void run(const std::vector<JobSpec>& jobs, int slots, int as, int64_t scale, siz
         if (r.sc.running() == 0 && r.sc.queue().empty()) break;
     }
     *n_adm_errors += r.admission_errors;
+    g_sorted_steps += r.sorted_steps;
     if (!r.failed && !stuck) {
         if (r.finished.size() != jobs.size()) fail("%lld of %lld jobs finished", (long long)r.finished.size(), (long long)jobs.size());
         for (auto& kv : r.finished) {
@@ -455,7 +507,8 @@ int main() {
     // many more slots than jobs
     for (int s : {slots, 1, 64}) run(jobs, s, as, 1, 32, 0.0, 1, &failed_runs, &adm_errors);
     if (failed_runs || adm_errors) fail("failures without injection");
-    std::printf("clean replays ok (%zu jobs)\n", jobs.size());
+    if (g_sorted_steps == 0) fail("no sorted step in the clean replays");
+    std::printf("clean replays ok (%zu jobs, %d sorted steps)\n", jobs.size(), g_sorted_steps);
     // sizes scaled up against a small device, so that admissions and
     // collations run out of memory, and device errors at random phases
     const int64_t max_terms = [&] {
