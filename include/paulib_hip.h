@@ -184,6 +184,24 @@ int plh_sort_and_combine(plh_sum* s);
 int plh_canonicalize(plh_sum* s);
 /* operator+= (include/PauliSumCore.h:232-237): append, sort_and_combine. */
 int plh_add(plh_sum* lhs, const plh_sum* rhs);
+/* operator-= (include/PauliSumCore.h:221-250): append rhs with every
+ * coefficient times -1, then sort_and_combine. */
+int plh_sub(plh_sum* lhs, const plh_sum* rhs);
+/* PauliSum::scale (include/PauliSumUtils.h:395-398): every coefficient times
+ * c (operator*= with a scalar, include/PauliSumCore.h:191-207; unary minus
+ * is scale(-1), :258-262). */
+int plh_scale(plh_sum* s, double c);
+/* PauliSum::lookup (include/PauliSumUtils.h:253-268): the coefficient of the
+ * pattern (x, z) with phase flags, summed over every term with that
+ * pattern (phase relative to flags; terms off by +-i skipped); 0 if absent. */
+int plh_lookup(const plh_sum* s, int array_size, const uint64_t* x, const uint64_t* z, uint8_t flags,
+               double* out);
+/* PauliSum::trace (:231-243): 2^num_qubits * the sum of the identity terms'
+ * real coefficients. */
+int plh_trace(const plh_sum* s, double* out);
+/* PauliSum::frobenius_norm2 (:403-410): sum of squared coefficients (of a
+ * canonical sum). */
+int plh_frobenius_norm2(const plh_sum* s, double* out);
 /* PauliSum::zero_state_expectation (include/PauliSumUtils.h:162-177). */
 int plh_zero_state_expectation(const plh_sum* s, double* out);
 /* PauliSum::truncate_weight / cutoff (include/PauliSumUtils.h:377-393). */
@@ -274,11 +292,20 @@ int plh_evolve(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,
 /* apply_truncation (include/PauliPropagation.h:35-47); max_weight < 0,
  * cutoff <= 0, max_terms < 0 mean "not set". */
 int plh_apply_truncation(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms);
+/* apply_truncation(O, opts, keep) (include/PauliPropagation.h:35-47): the
+ * trim step is trim(max_terms, keep); keep may be NULL. */
+int plh_apply_truncation_keep(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms,
+                              const plh_sum* keep);
 /* PauliSum::trim(max_terms) (include/PauliSumUtils.h:330-343): keep the
  * max_terms terms of largest |c|, in term order.  Among equal magnitudes the
  * reference leaves the choice to std::nth_element; here the lower index
  * is kept. */
 int plh_trim(plh_sum* s, int64_t max_terms);
+/* PauliSum::trim(max_terms, keep) (include/PauliSumUtils.h:345-375): terms
+ * whose (x, z) pattern appears in keep are never dropped; the other terms
+ * keep the max(0, max_terms - protected) largest |c| (ties as plh_trim), in
+ * term order.  keep must share s's context and array size. */
+int plh_trim_keep(plh_sum* s, int64_t max_terms, const plh_sum* keep);
 /* propagate (include/PauliPropagation.h:53-63): gates in reverse order,
  * each a one-term evolve then apply_truncation. */
 int plh_propagate(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,

@@ -12,8 +12,10 @@
 //   * the PauliSum member operations of the path as DevicePauliSum methods
 //     with the reference's names (multiply, sort_and_combine, canonicalize,
 //     evolve, apply_H/S/CNOT and every other gate of PauliSumClifford.h,
-//     apply_RX/RY/RZ, elementwise_multiply, truncate_weight, cutoff,
-//     zero_state_expectation) and the free functions commutator,
+//     apply_RX/RY/RZ, elementwise_multiply, truncate_weight, cutoff, trim
+//     (with and without keep), scale, lookup, trace, frobenius_norm2,
+//     zero_state_expectation, +=, -=, *= c, binary and unary -, * c) and
+//     the free functions commutator,
 //     i_commutator, apply_truncation, propagate, propagate_expectation
 //     (include/PauliSumOperations.h:348-356, include/PauliPropagation.h).
 //   * gpu_multiply / gpu_commutator with the reference wrapper's signatures
@@ -173,6 +175,31 @@ class DevicePauliSum {
     void cutoff(T eps) { check(plh_cutoff(h_, eps), "plh_cutoff"); }
     // PauliSum::trim(max_terms) (include/PauliSumUtils.h:330-343); ties keep the lower index.
     void trim(std::size_t max_terms) { check(plh_trim(h_, (int64_t)max_terms), "plh_trim"); }
+    // trim(max_terms, keep) (:345-375): keep's (x, z) patterns are never dropped.
+    void trim(std::size_t max_terms, const DevicePauliSum& keep) {
+        check(plh_trim_keep(h_, (int64_t)max_terms, keep.h_), "plh_trim_keep");
+    }
+    void scale(T c) { check(plh_scale(h_, c), "plh_scale"); }  // :395-398
+    // lookup (:253-268), trace (:231-243), frobenius_norm2 (:403-410).
+    template <class PauliString>
+    T lookup(const PauliString& ps) const {
+        static_assert(sizeof(ps.x) == sizeof(uint64_t) * AS, "a Pauli string of this sum's array size");
+        double r = 0;
+        check(plh_lookup(h_, AS, reinterpret_cast<const uint64_t*>(ps.x.data()),
+                         reinterpret_cast<const uint64_t*>(ps.z.data()), ps.flags, &r),
+              "plh_lookup");
+        return r;
+    }
+    T trace() const {
+        double r = 0;
+        check(plh_trace(h_, &r), "plh_trace");
+        return r;
+    }
+    T frobenius_norm2() const {
+        double r = 0;
+        check(plh_frobenius_norm2(h_, &r), "plh_frobenius_norm2");
+        return r;
+    }
     void apply_H(std::size_t c) { check(plh_apply_H(h_, (int64_t)c), "plh_apply_H"); }
     void apply_S(std::size_t c) { check(plh_apply_S(h_, (int64_t)c), "plh_apply_S"); }
     void apply_CNOT(std::size_t c, std::size_t t) { check(plh_apply_CNOT(h_, (int64_t)c, (int64_t)t), "plh_apply_CNOT"); }
@@ -226,6 +253,20 @@ class DevicePauliSum {
         check(plh_add(h_, rhs.h_), "plh_add");
         return *this;
     }
+    // operator-= and scalar operator*= (include/PauliSumCore.h:191-250).
+    DevicePauliSum& operator-=(const DevicePauliSum& rhs) {
+        check(plh_sub(h_, rhs.h_), "plh_sub");
+        return *this;
+    }
+    DevicePauliSum& operator*=(T c) {
+        scale(c);
+        return *this;
+    }
+    DevicePauliSum copy() const {
+        DevicePauliSum out(*ctx_, std::max<int64_t>(size(), 16), num_qubits());
+        check(plh_sum_copy(h_, out.h_), "plh_sum_copy");
+        return out;
+    }
 
    private:
     Context* ctx_;
@@ -262,9 +303,36 @@ DevicePauliSum<N, T> gpu_commutator(const DevicePauliSum<N, T>& lhs, const Devic
     return multiply_by_i ? i_commutator(lhs, rhs) : commutator(lhs, rhs);
 }
 
+// operator- (binary and unary) and scalar operator* (include/PauliSumCore.h:197-262).
 template <std::size_t N, typename T>
-void apply_truncation(DevicePauliSum<N, T>& O, const TruncationOptions<T>& o) {
-    check(plh_apply_truncation(O.handle(), o.weight_arg(), o.coefficient_cutoff, o.terms_arg()), "plh_apply_truncation");
+DevicePauliSum<N, T> operator-(const DevicePauliSum<N, T>& a, const DevicePauliSum<N, T>& b) {
+    DevicePauliSum<N, T> out = a.copy();
+    out -= b;
+    return out;
+}
+template <std::size_t N, typename T>
+DevicePauliSum<N, T> operator-(const DevicePauliSum<N, T>& a) {
+    DevicePauliSum<N, T> out = a.copy();
+    out.scale(T{-1});
+    return out;
+}
+template <std::size_t N, typename T>
+DevicePauliSum<N, T> operator*(const DevicePauliSum<N, T>& a, T c) {
+    DevicePauliSum<N, T> out = a.copy();
+    out *= c;
+    return out;
+}
+template <std::size_t N, typename T>
+DevicePauliSum<N, T> operator*(T c, const DevicePauliSum<N, T>& a) {
+    return a * c;
+}
+
+template <std::size_t N, typename T>
+void apply_truncation(DevicePauliSum<N, T>& O, const TruncationOptions<T>& o,
+                      const DevicePauliSum<N, T>* keep = nullptr) {
+    check(plh_apply_truncation_keep(O.handle(), o.weight_arg(), o.coefficient_cutoff, o.terms_arg(),
+                                    keep ? keep->handle() : nullptr),
+          "plh_apply_truncation");
 }
 
 template <std::size_t N, typename T, class HostSum>

@@ -53,6 +53,13 @@ _SIG = {
     "elementwise_multiply": (C.c_int, [_p, _p]),
     "apply_truncation": (C.c_int, [_p, _i64, C.c_double, _i64]),
     "propagate": (C.c_int, [_p, _p, _i64, C.c_double, _i64]),
+    "trim_keep": (C.c_int, [_p, _i64, _p]),
+    "scale": (C.c_int, [_p, C.c_double]),
+    "sub": (C.c_int, [_p, _p]),
+    "lookup": (C.c_double, [_p, _u64p, _u64p, C.c_uint8]),
+    "trace": (C.c_double, [_p]),
+    "frobenius_norm2": (C.c_double, [_p]),
+    "apply_truncation_keep": (C.c_int, [_p, _i64, C.c_double, _i64, _p]),
     "run_program": (C.c_int, [_p, _i64, _i32p, _i64p, _i64p, _u64p, _u64p, _f64p, _i64, C.c_double, _i64]),
 }
 
@@ -175,6 +182,31 @@ class CpuSum:
 
     def trim(self, m):
         return self._rc(self._fn("trim")(self.handle, int(m)), "trim")
+
+    def trim_keep(self, m, keep):
+        return self._rc(self._fn("trim_keep")(self.handle, int(m), keep.handle), "trim_keep")
+
+    def scale(self, c):
+        return self._rc(self._fn("scale")(self.handle, float(c)), "scale")
+
+    def sub(self, rhs):
+        return self._rc(self._fn("sub")(self.handle, rhs.handle), "sub")
+
+    def lookup(self, x, z, flags) -> float:
+        xa = np.ascontiguousarray(np.asarray(x, np.uint64).reshape(self.array_size))
+        za = np.ascontiguousarray(np.asarray(z, np.uint64).reshape(self.array_size))
+        return float(self._fn("lookup")(self.handle, _ptr(xa, C.c_uint64), _ptr(za, C.c_uint64), int(flags) & 3))
+
+    def trace(self) -> float:
+        return float(self._fn("trace")(self.handle))
+
+    def frobenius_norm2(self) -> float:
+        return float(self._fn("frobenius_norm2")(self.handle))
+
+    def apply_truncation_keep(self, keep, max_weight=None, cutoff=0.0, max_terms=None):
+        return self._rc(self._fn("apply_truncation_keep")(
+            self.handle, -1 if max_weight is None else max_weight, cutoff,
+            -1 if max_terms is None else max_terms, keep.handle), "apply_truncation_keep")
 
     def apply_H(self, c):
         return self._rc(self._fn("apply_H")(self.handle, int(c)), "apply_H")

@@ -420,6 +420,92 @@ int orc_trim(orc_sum *s, int64_t max_terms) {
     return 0;
 }
 
+/* PauliSum::trim(max_terms, keep), include/PauliSumUtils.h:345-375: terms
+ * whose (x, z) pattern occurs in keep are always kept; the others compete
+ * for max(0, max_terms - protected) places by |c| (ties: lower index, as
+ * orc_trim; parity unpinned for ties).  Kept terms stay in term order. */
+static int same_pattern(const orc_sum *a, int64_t i, const orc_sum *b, int64_t j) {
+    for (int k = 0; k < a->as; ++k)
+        if (X(a, k, i) != X(b, k, j) || Z(a, k, i) != Z(b, k, j)) return 0;
+    return 1;
+}
+int orc_trim_keep(orc_sum *s, int64_t max_terms, const orc_sum *keep) {
+    if (keep && keep->as != s->as) return -2;
+    const int64_t n = s->n;
+    if (n <= max_terms) return 0;
+    uint8_t *mask = (uint8_t *)calloc((size_t)(n ? n : 1), 1);
+    int64_t *idx = (int64_t *)malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+    if (!mask || !idx) { free(mask); free(idx); return -1; }
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int hit = 0;
+        for (int64_t j = 0; keep && j < keep->n && !hit; ++j) hit = same_pattern(s, i, keep, j);
+        if (hit) mask[i] = 1;
+        else idx[m++] = i;
+    }
+    const int64_t prot = n - m;
+    const int64_t budget = max_terms > prot ? max_terms - prot : 0;
+    g_trim_sum = s;
+    qsort(idx, (size_t)m, sizeof(int64_t), trim_cmp);
+    for (int64_t i = 0; i < budget && i < m; ++i) mask[idx[i]] = 1;
+    compact(s, mask);
+    free(mask); free(idx);
+    return 0;
+}
+
+/* PauliSum::scale, include/PauliSumUtils.h:395-398. */
+int orc_scale(orc_sum *s, double c) {
+    for (int64_t i = 0; i < s->n; ++i) s->v[i] *= c;
+    return 0;
+}
+
+/* operator-=, include/PauliSumCore.h:245-250: append rhs with -1 * value
+ * (append_scaled, :221-232), then sort_and_combine. */
+int orc_sub(orc_sum *lhs, const orc_sum *rhs) {
+    if (lhs->as != rhs->as) return -2;
+    const int64_t m = rhs->n;  /* rhs may be lhs: take its length first */
+    uint64_t xw[16], zw[16];
+    for (int64_t t = 0; t < m; ++t) {
+        load_term(rhs, t, xw, zw);
+        if (push(lhs, xw, zw, rhs->f[t], -1.0 * rhs->v[t])) return -1;
+    }
+    return orc_sort_and_combine(lhs);
+}
+
+/* PauliSum::lookup, include/PauliSumUtils.h:253-268: every term with the
+ * pattern (x, z) in term order; phase relative to flags, +-i skipped. */
+double orc_lookup(const orc_sum *s, const uint64_t *x, const uint64_t *z, uint8_t flags) {
+    double r = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) {
+        int match = 1;
+        for (int k = 0; k < s->as && match; ++k) match = X(s, k, i) == x[k] && Z(s, k, i) == z[k];
+        if (!match) continue;
+        const unsigned rel = ((unsigned)s->f[i] - (unsigned)flags) & 3u;
+        if (rel & ORC_PHASE) continue;
+        r += (rel & ORC_SIGN) ? -s->v[i] : s->v[i];
+    }
+    return r;
+}
+
+/* PauliSum::trace, include/PauliSumUtils.h:229-243. */
+double orc_trace(const orc_sum *s) {
+    double r = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) {
+        int ident = 1;
+        for (int k = 0; k < s->as && ident; ++k) ident = (X(s, k, i) | Z(s, k, i)) == 0;
+        if (!ident || (s->f[i] & ORC_PHASE)) continue;
+        r += ((s->f[i] & ORC_SIGN) ? -1.0 : 1.0) * s->v[i];
+    }
+    return ldexp(r, (int)s->nq);
+}
+
+/* PauliSum::frobenius_norm2, include/PauliSumUtils.h:401-410. */
+double orc_frobenius_norm2(const orc_sum *s) {
+    double r = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) r += s->v[i] * s->v[i];
+    return r;
+}
+
 /* ------------------------------------------------------------ Clifford */
 
 /* PauliSum::apply_H, include/PauliSumClifford.h:43-62. */
@@ -553,12 +639,16 @@ int orc_evolve(orc_sum *s, const orc_sum *rot) {
 
 /* apply_truncation, include/PauliPropagation.h:35-47.  max_weight < 0,
  * cutoff <= 0 and max_terms < 0 mean "not set". */
-int orc_apply_truncation(orc_sum *s, int64_t max_weight, double cutoff, int64_t max_terms) {
+int orc_apply_truncation_keep(orc_sum *s, int64_t max_weight, double cutoff, int64_t max_terms,
+                              const orc_sum *keep) {
     int rc = orc_canonicalize(s);
     if (!rc && max_weight >= 0) rc = orc_truncate_weight(s, max_weight);
     if (!rc && cutoff > 0.0) rc = orc_cutoff(s, cutoff);
-    if (!rc && max_terms >= 0) rc = orc_trim(s, max_terms);
+    if (!rc && max_terms >= 0) rc = keep ? orc_trim_keep(s, max_terms, keep) : orc_trim(s, max_terms);
     return rc;
+}
+int orc_apply_truncation(orc_sum *s, int64_t max_weight, double cutoff, int64_t max_terms) {
+    return orc_apply_truncation_keep(s, max_weight, cutoff, max_terms, NULL);
 }
 
 /* propagate, include/PauliPropagation.h:53-63: gates in reverse order, each

@@ -202,6 +202,56 @@ int ref_cutoff(void* p, double eps) {
 int ref_trim(void* p, int64_t m) {
     return dispatch(static_cast<RefSumBase*>(p), [&](auto& s) { s.trim((std::size_t)m); return 0; });
 }
+int ref_trim_keep(void* p, int64_t m, const void* keep) {
+    auto* a = static_cast<RefSumBase*>(p);
+    auto* b = const_cast<RefSumBase*>(static_cast<const RefSumBase*>(keep));
+    if (a->as != b->as) return -2;
+    return dispatch(a, [&](auto& s) { s.trim((std::size_t)m, same<std::decay_t<decltype(s)>>(b)); return 0; });
+}
+int ref_scale(void* p, double c) {
+    return dispatch(static_cast<RefSumBase*>(p), [&](auto& s) { s.scale(c); return 0; });
+}
+int ref_sub(void* lhs, const void* rhs) {
+    auto* a = static_cast<RefSumBase*>(lhs);
+    auto* b = const_cast<RefSumBase*>(static_cast<const RefSumBase*>(rhs));
+    if (a->as != b->as) return -2;
+    return dispatch(a, [&](auto& s) { s -= same<std::decay_t<decltype(s)>>(b); return 0; });
+}
+double ref_lookup(const void* p, const uint64_t* x, const uint64_t* z, uint8_t flags) {
+    double r = 0.0;
+    dispatch(const_cast<RefSumBase*>(static_cast<const RefSumBase*>(p)), [&](auto& s) {
+        using S = std::decay_t<decltype(s)>;
+        paulib::PauliString<64 * S::ArraySize> ps;
+        for (std::size_t k = 0; k < S::ArraySize; ++k) {
+            ps.x[k] = x[k];
+            ps.z[k] = z[k];
+        }
+        ps.flags = flags;
+        r = s.lookup(ps);
+        return 0;
+    });
+    return r;
+}
+double ref_trace(const void* p) {
+    double r = 0.0;
+    dispatch(const_cast<RefSumBase*>(static_cast<const RefSumBase*>(p)), [&](auto& s) { r = s.trace(); return 0; });
+    return r;
+}
+double ref_frobenius_norm2(const void* p) {
+    double r = 0.0;
+    dispatch(const_cast<RefSumBase*>(static_cast<const RefSumBase*>(p)),
+             [&](auto& s) { r = s.frobenius_norm2(); return 0; });
+    return r;
+}
+int ref_apply_truncation_keep(void* p, int64_t max_weight, double cutoff, int64_t max_terms, const void* keep) {
+    auto* a = static_cast<RefSumBase*>(p);
+    auto* b = const_cast<RefSumBase*>(static_cast<const RefSumBase*>(keep));
+    if (a->as != b->as) return -2;
+    return dispatch(a, [&](auto& s) {
+        paulib::apply_truncation(s, opts_of<int>(max_weight, cutoff, max_terms), &same<std::decay_t<decltype(s)>>(b));
+        return 0;
+    });
+}
 int ref_apply_H(void* p, int64_t c) {
     return dispatch(static_cast<RefSumBase*>(p), [&](auto& s) { s.apply_H((std::size_t)c); return 0; });
 }

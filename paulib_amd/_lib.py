@@ -72,6 +72,13 @@ SIGNATURES = {
     "plh_batch_multiply_canonical": (_int, [_int, C.POINTER(_p), C.POINTER(_p), _p, _i64p]),
     "plh_batch_multiply_canonical_each": (_int, [_int, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), _i64]),
     "plh_commutator": (_int, [_p, _p, _int, _p]),
+    "plh_sub": (_int, [_p, _p]),
+    "plh_scale": (_int, [_p, C.c_double]),
+    "plh_lookup": (_int, [_p, _int, _p, _p, C.c_uint8, C.POINTER(C.c_double)]),
+    "plh_trace": (_int, [_p, C.POINTER(C.c_double)]),
+    "plh_frobenius_norm2": (_int, [_p, C.POINTER(C.c_double)]),
+    "plh_trim_keep": (_int, [_p, _i64, _p]),
+    "plh_apply_truncation_keep": (_int, [_p, _i64, C.c_double, _i64, _p]),
     "plh_launch_commutator_soa_noalloc": (_int, [_int, _int, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                                  _i64, _i64, _i64, _p]),
     "plh_commutator_canonical": (_int, [_p, _p, _int, _p]),

@@ -727,6 +727,54 @@ int plh_add(plh_sum* lhs, const plh_sum* rhs) {
     return plh_sort_and_combine(lhs);
 }
 
+int plh_sub(plh_sum* lhs, const plh_sum* rhs) {
+    PLH_TRY(check_pair(lhs, rhs));
+    // append_scaled(lhs, -1, rhs), then sort_and_combine (operator-=,
+    // include/PauliSumCore.h:221-250): rhs is copied, its copy negated
+    plh_sum tmp;
+    tmp.ctx = lhs->ctx;
+    tmp.as = lhs->as;
+    tmp.nq = lhs->nq;
+    int rc = op_append(lhs->ctx, &tmp, rhs);
+    if (rc == PLH_OK) rc = op_scale(lhs->ctx, &tmp, -1.0);
+    if (rc == PLH_OK) rc = op_append(lhs->ctx, lhs, &tmp);
+    (void)hipStreamSynchronize(lhs->ctx->stream);
+    sum_free_storage(&tmp);
+    PLH_TRY(rc);
+    return plh_sort_and_combine(lhs);
+}
+
+int plh_scale(plh_sum* s, double c) {
+    PLH_TRY(check_sum(s));
+    PLH_TRY(op_scale(s->ctx, s, c));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_lookup(const plh_sum* s, int array_size, const uint64_t* x, const uint64_t* z, uint8_t flags,
+               double* out) {
+    PLH_TRY(check_sum(s));
+    if (!out || array_size != s->as || (array_size > 0 && (!x || !z))) {
+        set_error("plh_lookup: null output or a pattern of another array size");
+        return PLH_ERR_ARG;
+    }
+    return op_select_sum(s->ctx, s, 0, x, z, flags, out);
+}
+
+int plh_trace(const plh_sum* s, double* out) {
+    PLH_TRY(check_sum(s));
+    if (!out) return PLH_ERR_ARG;
+    double sum = 0.0;
+    PLH_TRY(op_select_sum(s->ctx, s, 1, nullptr, nullptr, 0, &sum));
+    *out = std::ldexp(sum, (int)s->nq);
+    return PLH_OK;
+}
+
+int plh_frobenius_norm2(const plh_sum* s, double* out) {
+    PLH_TRY(check_sum(s));
+    if (!out) return PLH_ERR_ARG;
+    return op_select_sum(s->ctx, s, 2, nullptr, nullptr, 0, out);
+}
+
 int plh_zero_state_expectation(const plh_sum* s, double* out) {
     PLH_TRY(check_sum(s));
     if (!out) return PLH_ERR_ARG;
@@ -859,9 +907,15 @@ int plh_evolve(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* pz,
 }
 
 int plh_apply_truncation(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms) {
+    return plh_apply_truncation_keep(s, max_weight, cutoff, max_terms, nullptr);
+}
+
+int plh_apply_truncation_keep(plh_sum* s, int64_t max_weight, double cutoff, int64_t max_terms,
+                              const plh_sum* keep) {
     PLH_TRY(check_sum(s));
+    if (keep) PLH_TRY(check_pair(s, keep));
     PLH_TRY(collate_terms(s->ctx, s, trunc_spec(max_weight, cutoff), s));
-    if (max_terms >= 0) PLH_TRY(op_trim(s->ctx, s, max_terms));
+    if (max_terms >= 0) PLH_TRY(op_trim_keep(s->ctx, s, max_terms, keep));
     return plh_ctx_synchronize(s->ctx);
 }
 
@@ -872,6 +926,17 @@ int plh_trim(plh_sum* s, int64_t max_terms) {
         return PLH_ERR_ARG;
     }
     PLH_TRY(op_trim(s->ctx, s, max_terms));
+    return plh_ctx_synchronize(s->ctx);
+}
+
+int plh_trim_keep(plh_sum* s, int64_t max_terms, const plh_sum* keep) {
+    PLH_TRY(check_sum(s));
+    if (keep) PLH_TRY(check_pair(s, keep));
+    if (max_terms < 0) {
+        set_error("plh_trim_keep: max_terms must be >= 0");
+        return PLH_ERR_ARG;
+    }
+    PLH_TRY(op_trim_keep(s->ctx, s, max_terms, keep));
     return plh_ctx_synchronize(s->ctx);
 }
 

@@ -237,6 +237,7 @@ enum : int {
     kWsRotA,    // sort-free rotation: anticommuting flags and their scan
     kWsRotN,    // sort-free rotation: the new terms, sorted
     kWsRotM,    // sort-free rotation: merge flags, lower bounds, sums, output scan
+    kWsKeep,    // trim with keep: protected patterns and flags
     kWsCount
 };
 
@@ -347,6 +348,17 @@ int op_filter(plh_ctx* ctx, plh_sum* s, uint32_t mode, int64_t max_weight, doubl
 // PauliSum::trim(max_terms): keep the max_terms largest |c| in term order
 // (equal magnitudes: lower index first).
 int op_trim(plh_ctx* ctx, plh_sum* s, int64_t max_terms);
+// PauliSum::trim(max_terms, keep) (include/PauliSumUtils.h:348-375): terms
+// whose (x, z) pattern appears in keep are kept; the others compete for the
+// places left (keep null: op_trim).
+int op_trim_keep(plh_ctx* ctx, plh_sum* s, int64_t max_terms, const plh_sum* keep);
+// PauliSum::scale (include/PauliSumUtils.h:395-398): v *= c.
+int op_scale(plh_ctx* ctx, plh_sum* s, double c);
+// Fixed-order sums of a selection: mode 0 lookup of the pattern (qx, qz) with
+// phase flags qf, 1 trace's identity sum (before the 2^n factor), 2
+// frobenius_norm2.
+int op_select_sum(plh_ctx* ctx, const plh_sum* s, int mode, const uint64_t* qx, const uint64_t* qz, uint32_t qf,
+                  double* out);
 int op_zero_state_expectation(plh_ctx* ctx, const plh_sum* s, double* out);
 int op_append(plh_ctx* ctx, plh_sum* dst, const plh_sum* src);
 

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "plh_block.hpp"
 #include "plh_engine.hpp"
@@ -846,12 +847,13 @@ __global__ void k_trim_init(uint64_t* __restrict__ sel, uint32_t* __restrict__ g
 
 __global__ __launch_bounds__(kBlock) void k_trim_hist(const double* __restrict__ v, int64_t n,
                                                       const uint64_t* __restrict__ sel, int shift,
-                                                      uint32_t* __restrict__ ghist) {
+                                                      uint32_t* __restrict__ ghist, const uint8_t* __restrict__ prot) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t prefix = sel[0], mask = sel[1];
     for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        if (prot && prot[t]) continue;  // protected terms are kept outside the selection
         const uint64_t k = mag_key(v[t]);
         if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & 255u], 1u);
     }
@@ -880,7 +882,7 @@ __global__ void k_trim_pick(uint64_t* __restrict__ sel, uint32_t* __restrict__ g
 // Per block of kOpsTile terms: keys above the threshold, and keys equal to it.
 __global__ __launch_bounds__(kBlock) void k_trim_count(const double* __restrict__ v, int64_t n,
                                                        const uint64_t* __restrict__ sel, uint32_t* __restrict__ cgt,
-                                                       uint32_t* __restrict__ ctie) {
+                                                       uint32_t* __restrict__ ctie, const uint8_t* __restrict__ prot) {
     __shared__ uint32_t red[kWavesPerBlock];
     const uint64_t T = sel[0];
     const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
@@ -889,8 +891,9 @@ __global__ __launch_bounds__(kBlock) void k_trim_count(const double* __restrict_
         const uint64_t t = base + k;
         if (t >= (uint64_t)n) break;
         const uint64_t key = mag_key(v[t]);
-        g += key > T;
-        e += key == T;
+        const bool pr = prot && prot[t];
+        g += pr || key > T;
+        e += !pr && key == T;
     }
     const uint32_t tg = block_sum(g, red);
     const uint32_t te = block_sum(e, red);
@@ -917,7 +920,8 @@ __global__ __launch_bounds__(kBlock) void k_trim_write(int as, SumView s, int64_
                                                        const uint64_t* __restrict__ koff, uint64_t* __restrict__ ox,
                                                        uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
                                                        double* __restrict__ ov, int64_t os, uint64_t kmax,
-                                                       const uint32_t* __restrict__ sj, uint32_t* __restrict__ oj) {
+                                                       const uint32_t* __restrict__ sj, uint32_t* __restrict__ oj,
+                                                       const uint8_t* __restrict__ prot) {
     __shared__ uint32_t red[kWavesPerBlock];
     const uint64_t T = sel[0], kr = sel[2];
     const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
@@ -926,8 +930,9 @@ __global__ __launch_bounds__(kBlock) void k_trim_write(int as, SumView s, int64_
         const uint64_t t = base + k;
         if (t >= (uint64_t)n) break;
         const uint64_t key = mag_key(s.v[t]);
-        if (key > T) gt |= 1u << k;
-        if (key == T) tie |= 1u << k;
+        const bool pr = prot && prot[t];
+        if (pr || key > T) gt |= 1u << k;
+        if (!pr && key == T) tie |= 1u << k;
     }
     uint32_t tt;
     uint64_t r = toff[blockIdx.x] + block_exclusive_scan((uint32_t)__popc(tie), tt, red);  // rank of my first tie
@@ -954,13 +959,179 @@ __global__ __launch_bounds__(kBlock) void k_trim_write(int as, SumView s, int64_
     }
 }
 
+// prot[t] = 1 when term t's (x, z) pattern is one of the nk sorted distinct
+// patterns in keys (x words then z words, as words each); per-block counts.
+__global__ __launch_bounds__(kBlock) void k_trim_protect(int as, SumView s, int64_t n, const uint64_t* __restrict__ keys,
+                                                         uint32_t nk, uint8_t* __restrict__ prot,
+                                                         uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t hit = 0;
+    if (t < n) {
+        uint32_t lo = 0, c = nk;  // first key >= term t
+        while (c > 0) {
+            const uint32_t half = c / 2, m = lo + half;
+            const uint64_t* kk = keys + (size_t)m * 2 * as;
+            int cmp = 0;
+            for (int w = 0; w < 2 * as && cmp == 0; ++w) {
+                const uint64_t a = kk[w];
+                const uint64_t b = w < as ? s.x[(int64_t)w * s.stride + t] : s.z[(int64_t)(w - as) * s.stride + t];
+                cmp = a < b ? -1 : a > b ? 1 : 0;
+            }
+            if (cmp < 0) {
+                lo = m + 1;
+                c -= half + 1;
+            } else {
+                c = half;
+            }
+        }
+        if (lo < nk) {
+            const uint64_t* kk = keys + (size_t)lo * 2 * as;
+            bool eq = true;
+            for (int w = 0; w < 2 * as; ++w)
+                eq = eq && kk[w] == (w < as ? s.x[(int64_t)w * s.stride + t] : s.z[(int64_t)(w - as) * s.stride + t]);
+            hit = eq ? 1u : 0u;
+        }
+        prot[t] = (uint8_t)hit;
+    }
+    const uint32_t tot = block_sum(hit, red);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+// v[t] *= c (PauliSum::scale, include/PauliSumUtils.h:395-398).
+__global__ void k_scale(double* __restrict__ v, int64_t n, double c) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) v[t] *= c;
+}
+
+// Fixed-order partial sums of one selection per block: mode 0 lookup (terms
+// with the query's (x, z), real relative phase, sign folded;
+// include/PauliSumUtils.h:253-268), 1 trace (identity terms, real phase,
+// sign folded; :231-243), 2 frobenius_norm2 (v^2; :403-410).
+struct Query {
+    uint64_t x[16], z[16];
+    uint32_t f;
+};
+__global__ __launch_bounds__(kBlock) void k_select_partial(int mode, int as, SumView s, int64_t n, Query q,
+                                                           double* __restrict__ part) {
+    __shared__ double red[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kOpsTile + (uint64_t)threadIdx.x * kOpsIpt;
+    double acc = 0.0;
+    for (int k = 0; k < kOpsIpt; ++k) {
+        const uint64_t t = base + k;
+        if (t >= (uint64_t)n) break;
+        const double v = s.v[t];
+        if (mode == 2) {
+            acc += v * v;
+            continue;
+        }
+        bool match = true;
+        for (int w = 0; w < as; ++w) {
+            const uint64_t xw = s.x[(int64_t)w * s.stride + t], zw = s.z[(int64_t)w * s.stride + t];
+            match = match && (mode == 0 ? (xw == q.x[w] && zw == q.z[w]) : (xw == 0 && zw == 0));
+        }
+        if (!match) continue;
+        const uint32_t rel = mode == 0 ? ((uint32_t)s.f[t] - q.f) & 3u : (uint32_t)s.f[t] & 3u;
+        if (rel & 1u) continue;
+        acc += (rel & 2u) ? -v : v;
+    }
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
 }  // namespace
 
-int op_trim(plh_ctx* ctx, plh_sum* s, int64_t max_terms) {
+int op_trim(plh_ctx* ctx, plh_sum* s, int64_t max_terms) { return op_trim_keep(ctx, s, max_terms, nullptr); }
+
+int op_scale(plh_ctx* ctx, plh_sum* s, double c) {
+    if (s->n == 0) return PLH_OK;
+    k_scale<<<nblk((uint64_t)s->n, kBlock), kBlock, 0, ctx->stream>>>(s->v, s->n, c);
+    PLH_HIP(hipGetLastError());
+    // keys and order unchanged; a zero factor leaves zeros (as the reference)
+    if (c == 0.0) {
+        s->canonical = false;
+        s->sorted = false;
+    }
+    return PLH_OK;
+}
+
+int op_select_sum(plh_ctx* ctx, const plh_sum* s, int mode, const uint64_t* qx, const uint64_t* qz, uint32_t qf,
+                  double* out) {
+    if (s->n == 0) {
+        *out = 0.0;
+        return PLH_OK;
+    }
+    Query q{};
+    for (int w = 0; w < s->as && mode == 0; ++w) {
+        q.x[w] = qx[w];
+        q.z[w] = qz[w];
+    }
+    q.f = qf & 3u;
+    hipStream_t st = ctx->stream;
+    const uint64_t nb = nblk((uint64_t)s->n, kOpsTile);
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsScan1, nb * 8 + 64, &p));
+    double* part = (double*)p;
+    PLH_TRY(ctx->ws.get(kWsMisc, 4096, &p));
+    double* d_out = (double*)p + 41;
+    k_select_partial<<<dim3((unsigned)nb), kBlock, 0, st>>>(mode, s->as, SumView{s->x, s->z, s->f, s->v, s->cap},
+                                                            s->n, q, part);
+    k_zse_final<<<1, kBlock, 0, st>>>(part, nb, d_out);
+    PLH_HIP(hipGetLastError());
+    uint64_t bits = 0;
+    PLH_TRY(read_u64(ctx, (const uint64_t*)d_out, &bits));
+    std::memcpy(out, &bits, sizeof(double));
+    return PLH_OK;
+}
+
+int op_trim_keep(plh_ctx* ctx, plh_sum* s, int64_t max_terms, const plh_sum* keep) {
     if (max_terms < 0) return PLH_ERR_ARG;
     if (s->n <= max_terms) return PLH_OK;
     hipStream_t st = ctx->stream;
-    const uint64_t n = (uint64_t)s->n, k = (uint64_t)max_terms;
+    const uint64_t n = (uint64_t)s->n;
+    // trim(max_terms, keep) (include/PauliSumUtils.h:348-375): terms whose
+    // (x, z) pattern is in keep are never dropped; the others compete for
+    // the max_terms - protected places left
+    uint8_t* prot = nullptr;
+    uint64_t nprot = 0;
+    if (keep && keep->n > 0) {
+        const int as = s->as;
+        std::vector<uint64_t> kx((size_t)keep->as * keep->n), kz((size_t)keep->as * keep->n);
+        std::vector<uint8_t> kf((size_t)keep->n);
+        std::vector<double> kv((size_t)keep->n);
+        for (int w = 0; w < keep->as; ++w) {
+            PLH_HIP(hipMemcpyAsync(kx.data() + (size_t)w * keep->n, keep->x + (int64_t)w * keep->cap,
+                                   (size_t)keep->n * 8, hipMemcpyDeviceToHost, st));
+            PLH_HIP(hipMemcpyAsync(kz.data() + (size_t)w * keep->n, keep->z + (int64_t)w * keep->cap,
+                                   (size_t)keep->n * 8, hipMemcpyDeviceToHost, st));
+        }
+        PLH_HIP(hipStreamSynchronize(st));
+        std::vector<std::vector<uint64_t>> pats((size_t)keep->n, std::vector<uint64_t>((size_t)2 * as, 0));
+        for (int64_t t = 0; t < keep->n; ++t)
+            for (int w = 0; w < std::min(as, keep->as); ++w) {
+                pats[(size_t)t][(size_t)w] = kx[(size_t)w * keep->n + t];
+                pats[(size_t)t][(size_t)(as + w)] = kz[(size_t)w * keep->n + t];
+            }
+        std::sort(pats.begin(), pats.end());
+        pats.erase(std::unique(pats.begin(), pats.end()), pats.end());
+        std::vector<uint64_t> flat;
+        flat.reserve(pats.size() * 2 * as);
+        for (auto& v : pats) flat.insert(flat.end(), v.begin(), v.end());
+        const uint64_t nb = nblk(n, kBlock);
+        void* p;
+        PLH_TRY(ctx->ws.get(kWsKeep, flat.size() * 8 + n + 256 + nb * 4 + (nb + 1) * 8 + 256, &p));
+        uint64_t* dkeys = (uint64_t*)p;
+        prot = (uint8_t*)(dkeys + flat.size());
+        uint32_t* cnt = (uint32_t*)((char*)prot + ((n + 255) & ~(uint64_t)255));
+        uint64_t* coff = (uint64_t*)(cnt + nb + (nb & 1));
+        PLH_HIP(hipMemcpyAsync(dkeys, flat.data(), flat.size() * 8, hipMemcpyHostToDevice, st));
+        k_trim_protect<<<(unsigned)nb, kBlock, 0, st>>>(as, SumView{s->x, s->z, s->f, s->v, s->cap}, s->n, dkeys,
+                                                       (uint32_t)pats.size(), prot, cnt);
+        PLH_HIP(hipGetLastError());
+        PLH_TRY(scan_u32_to_u64_read(ctx, cnt, coff, nb, coff + nb, &nprot));
+    }
+    const uint64_t budget = (uint64_t)max_terms > nprot ? (uint64_t)max_terms - nprot : 0;
+    const uint64_t k = nprot + budget;  // kept terms (fewer than n: n > max_terms >= budget + ...)
     plh_sum tmp;
     tmp.ctx = ctx;
     tmp.as = s->as;
@@ -982,19 +1153,19 @@ int op_trim(plh_ctx* ctx, plh_sum* s, int64_t max_terms) {
         uint32_t* ghist = (uint32_t*)((uint64_t*)p + 128);  // 256 counters
         uint64_t* d_total = (uint64_t*)p + 40;
         const unsigned grid = (unsigned)std::min<uint64_t>(nblk(n, kBlock), 2048);
-        k_trim_init<<<1, 256, 0, st>>>(sel, ghist, k);
+        k_trim_init<<<1, 256, 0, st>>>(sel, ghist, budget);
         for (int shift = 56; shift >= 0; shift -= 8) {
-            k_trim_hist<<<grid, kBlock, 0, st>>>(s->v, s->n, sel, shift, ghist);
+            k_trim_hist<<<grid, kBlock, 0, st>>>(s->v, s->n, sel, shift, ghist, prot);
             k_trim_pick<<<1, 256, 0, st>>>(sel, ghist, shift);
         }
-        k_trim_count<<<dim3((unsigned)nb), kBlock, 0, st>>>(s->v, s->n, sel, cgt, ctie);
+        k_trim_count<<<dim3((unsigned)nb), kBlock, 0, st>>>(s->v, s->n, sel, cgt, ctie, prot);
         PLH_HIP(hipGetLastError());
         PLH_TRY(scan_u32_to_u64(ctx, ctie, toff, nb, d_total));
         k_trim_keepcnt<<<(unsigned)nblk(nb, 256), 256, 0, st>>>(cgt, ctie, toff, sel, nb, ckeep);
         PLH_TRY(scan_u32_to_u64(ctx, ckeep, koff, nb, d_total));
         k_trim_write<<<dim3((unsigned)nb), kBlock, 0, st>>>(s->as, SumView{s->x, s->z, s->f, s->v, s->cap}, s->n,
                                                             sel, toff, koff, tmp.x, tmp.z, tmp.f, tmp.v, tmp.cap, k,
-                                                            s->j, tmp.j);
+                                                            s->j, tmp.j, prot);
         PLH_HIP(hipGetLastError());
     }
     const bool canon = s->canonical, srt = s->sorted;

@@ -202,7 +202,23 @@ class PauliSum:
         return self
 
     def __imul__(self, rhs):
-        return self.multiply(rhs)
+        if isinstance(rhs, PauliSum):
+            return self.multiply(rhs)
+        return self.scale(float(rhs))
+
+    def __mul__(self, rhs):
+        """A * B (uncombined product) or A * c."""
+        out = self.copy()
+        out *= rhs
+        return out
+
+    def __rmul__(self, c):
+        if isinstance(c, PauliSum):
+            return NotImplemented
+        return self.copy().scale(float(c))
+
+    def __neg__(self):
+        return self.copy().scale(-1.0)
 
     def multiply_canonical(self, rhs: "PauliSum") -> "PauliSum":
         """canonicalize(self * rhs) via the fused product->collation path."""
@@ -222,6 +238,21 @@ class PauliSum:
         _lib.call("plh_add", self._h, rhs._h)
         return self
 
+    def __add__(self, rhs: "PauliSum") -> "PauliSum":
+        out = self.copy()
+        out += rhs
+        return out
+
+    def __isub__(self, rhs: "PauliSum"):
+        """operator-=: append -rhs, then sort_and_combine."""
+        _lib.call("plh_sub", self._h, rhs._h)
+        return self
+
+    def __sub__(self, rhs: "PauliSum") -> "PauliSum":
+        out = self.copy()
+        out -= rhs
+        return out
+
     def zero_state_expectation(self) -> float:
         out = C.c_double()
         _lib.call("plh_zero_state_expectation", self._h, C.byref(out))
@@ -235,10 +266,81 @@ class PauliSum:
         _lib.call("plh_cutoff", self._h, float(eps))
         return self
 
-    def trim(self, max_terms: int):
-        """PauliSum::trim: keep the max_terms largest |c| in term order (ties: lower index)."""
-        _lib.call("plh_trim", self._h, int(max_terms))
+    def trim(self, max_terms: int, keep: "PauliSum | None" = None):
+        """PauliSum::trim: keep the max_terms largest |c| in term order (ties:
+        lower index); with keep, terms whose (x, z) pattern is in keep are
+        never dropped and the rest fill max(0, max_terms - protected) places."""
+        if keep is None:
+            _lib.call("plh_trim", self._h, int(max_terms))
+        else:
+            _lib.call("plh_trim_keep", self._h, int(max_terms), keep._h)
         return self
+
+    def scale(self, c: float):
+        """PauliSum::scale: every coefficient times c."""
+        _lib.call("plh_scale", self._h, float(c))
+        return self
+
+    def collate(self):
+        """sort_and_combine under its Python name (python/bindings.cpp:170)."""
+        return self.sort_and_combine()
+
+    def lookup(self, pauli_string) -> float:
+        """Coefficient of pauli_string (a string such as '-XZ', or a tuple
+        (x words, z words, flags)); phase relative to it, +-i terms skipped."""
+        a = self.array_size
+        if isinstance(pauli_string, str):
+            xw, zw, f, q = HostSum.parse_string(pauli_string, a)
+        else:
+            xw, zw, f = pauli_string
+        x = np.ascontiguousarray(np.asarray(xw, np.uint64).reshape(a))
+        z = np.ascontiguousarray(np.asarray(zw, np.uint64).reshape(a))
+        out = C.c_double()
+        _lib.call("plh_lookup", self._h, a, _u64(x), _u64(z), int(f) & 3, C.byref(out))
+        return out.value
+
+    def trace(self) -> float:
+        """2^num_qubits times the sum of identity-term coefficients (real phase)."""
+        out = C.c_double()
+        _lib.call("plh_trace", self._h, C.byref(out))
+        return out.value
+
+    def frobenius_norm2(self) -> float:
+        """Sum of squared coefficients (tr(O^dagger O) / 2^n of a canonical sum)."""
+        out = C.c_double()
+        _lib.call("plh_frobenius_norm2", self._h, C.byref(out))
+        return out.value
+
+    def term_weight(self, i: int) -> int:
+        """Non-identity sites of term i."""
+        self._check_term(i)
+        h = self.range_to_host(int(i), 1)
+        return int(sum(int(a | b).bit_count() for a, b in zip(h.x[:, 0], h.z[:, 0])))
+
+    def _check_term(self, i: int):
+        if not 0 <= int(i) < len(self):
+            raise IndexError(f"term index {i} out of range for a sum of {len(self)} terms")
+
+    @property
+    def coefficients(self):
+        """The coefficients in term order (a host copy)."""
+        return self.to_host().value
+
+    def get_pauli_string(self, i: int) -> str:
+        self._check_term(i)
+        return self.range_to_host(int(i), 1).term_string(0)
+
+    def __getitem__(self, i: int):
+        """(Pauli string with its phase prefix, coefficient) of term i."""
+        self._check_term(i)
+        h = self.range_to_host(int(i), 1)
+        return h.term_string(0), float(h.value[0])
+
+    def get_str(self) -> str:
+        return self.to_host().get_str()
+
+    __str__ = get_str
+    __repr__ = get_str
 
     def apply_H(self, c: int):
         _lib.call("plh_apply_H", self._h, int(c))
@@ -305,8 +407,12 @@ def commutator_canonical(a: PauliSum, b: PauliSum, multiply_by_i: bool = False) 
     return out
 
 
-def apply_truncation(O: PauliSum, opts: TruncationOptions):
-    _lib.call("plh_apply_truncation", O._h, *opts.args())
+def apply_truncation(O: PauliSum, opts: TruncationOptions, keep: PauliSum | None = None):
+    """apply_truncation(O, opts[, keep]): canonicalize, weight, cutoff, trim."""
+    if keep is None:
+        _lib.call("plh_apply_truncation", O._h, *opts.args())
+    else:
+        _lib.call("plh_apply_truncation_keep", O._h, *opts.args(), keep._h)
     return O
 
 
@@ -393,5 +499,7 @@ def _named_gate(name, two):
 
 for _g in ("I", "X", "Y", "Z", "H_YZ", "H_XY", "sqrt_X", "sqrt_Y"):
     setattr(PauliSum, f"apply_{_g}", _named_gate(_g, False))
+PauliSum.apply_SX = PauliSum.apply_sqrt_X  # python/bindings.cpp names
+PauliSum.apply_SY = PauliSum.apply_sqrt_Y
 for _g in ("CZ", "CY", "XC_X", "XC_Z", "XC_Y", "YC_X", "YC_Z", "YC_Y", "SWAP", "iSWAP"):
     setattr(PauliSum, f"apply_{_g}", _named_gate(_g, True))

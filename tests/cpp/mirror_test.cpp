@@ -26,7 +26,20 @@ int orc_trim(void*, int64_t);
 int orc_apply_gate(void*, int, int64_t, int64_t);
 int orc_apply_rot_axis(void*, int, int64_t, double);
 int orc_elementwise_multiply(void*, const void*);
+int orc_trim_keep(void*, int64_t, const void*);
+int orc_sub(void*, const void*);
+int orc_scale(void*, double);
+double orc_lookup(const void*, const uint64_t*, const uint64_t*, uint8_t);
+double orc_trace(const void*);
+double orc_frobenius_norm2(const void*);
 }
+
+// The fields of paulib::PauliString<N> that lookup reads (include/PauliStringCore.h:24-26).
+template <std::size_t N>
+struct HostPauliString {
+    std::array<std::size_t, (N + 63) / 64> x{}, z{};
+    uint8_t flags = 0;
+};
 
 // The public fields of paulib::PauliSum<N,double> (include/PauliSumCore.h:20-45).
 template <std::size_t N>
@@ -157,6 +170,45 @@ bool run_all() {
     orc_apply_rot_axis(oc, 2, 5, 0.3);
     orc_apply_rot_axis(oc, 3, (int64_t)N - 1, -1.1);
     ok &= same(d.template download<HostPauliSum<N>>(), oc, "gates/RY/RZ");
+    // trim(max_terms, keep), operator-=, scalar *, lookup, trace, frobenius_norm2
+    {
+        auto K = random_sum<N>(40, 9);
+        for (std::size_t k = 0; k < K.ArraySize; ++k)
+            for (std::size_t t = 0; t < 20; ++t) {
+                K.x_data[k][t] = A.x_data[k][t * 7];
+                K.z_data[k][t] = A.z_data[k][t * 7];
+            }
+        paulib_hip::DevicePauliSum<N> t(A), dk(K);
+        t.trim(50, dk);
+        void* ot = to_oracle(A);
+        void* ok_ = to_oracle(K);
+        orc_trim_keep(ot, 50, ok_);
+        ok &= same(t.template download<HostPauliSum<N>>(), ot, "trim(max_terms, keep)");
+        paulib_hip::DevicePauliSum<N> u(A), w(B);
+        u -= w;
+        auto v2 = u * 2.5;
+        void* ou = to_oracle(A);
+        orc_sub(ou, ob);
+        orc_scale(ou, 2.5);
+        ok &= same(v2.template download<HostPauliSum<N>>(), ou, "operator-= and * c");
+        HostPauliString<N> q;
+        for (std::size_t k = 0; k < K.ArraySize; ++k) {
+            q.x[k] = A.x_data[k][5];
+            q.z[k] = A.z_data[k][5];
+        }
+        q.flags = A.flags[5];
+        void* oA = to_oracle(A);
+        const double lg = paulib_hip::DevicePauliSum<N>(A).lookup(q);
+        const double lw = orc_lookup(oA, reinterpret_cast<const uint64_t*>(q.x.data()),
+                                     reinterpret_cast<const uint64_t*>(q.z.data()), q.flags);
+        const double ng = paulib_hip::DevicePauliSum<N>(A).frobenius_norm2(), nw = orc_frobenius_norm2(oA);
+        const double tg = paulib_hip::DevicePauliSum<N>(A).trace(), tw = orc_trace(oA);
+        const bool qok = lg == lw && lw != 0.0 && std::fabs(ng - nw) <= 1e-12 * nw && tg == tw;
+        std::printf("%s lookup/frobenius_norm2/trace (%g %g, %g %g, %g %g)\n", qok ? "ok  " : "FAIL", lg, lw, ng,
+                    nw, tg, tw);
+        ok &= qok;
+        for (void* p : {ot, ok_, ou, oA}) orc_sum_free(p);
+    }
     auto B3 = random_sum<N>(300, 3);
     paulib_hip::DevicePauliSum<N> ea(A), eb(B3);
     ea.elementwise_multiply(eb);

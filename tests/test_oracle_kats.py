@@ -250,3 +250,87 @@ def test_filter_and_trim_kats(kind):
     assert len(s) == 1 and s.term_string(0) == "XII"
     out = run(kind, S([("XI", 0.1), ("ZI", -5.0), ("IZ", 0.3), ("YI", 2.0)]), ("trim", 2))
     assert {out.term_string(t): out.value[t] for t in range(len(out))} == {"ZI": -5.0, "YI": 2.0}
+
+
+def _lookup(kind, host, q):
+    xw, zw, f, _ = HostSum.parse_string(q, host.array_size)
+    return cpu(kind, host).lookup(xw, zw, f)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_trim_keep_kat(kind):
+    # :1125-1137 (TrimProtectsKeepPatterns): XI survives, the budget left goes to ZI
+    s = cpu(kind, S([("XI", 0.001), ("ZI", -5.0), ("IZ", 0.3), ("YI", 2.0)]))
+    s.trim_keep(2, cpu(kind, S([("XI", 1.0)])))
+    out = s.to_host()
+    assert {out.term_string(t): out.value[t] for t in range(len(out))} == {"XI": 0.001, "ZI": -5.0}
+    # the keep pattern ignores phase: keep '-iXI' protects 'XI'
+    s = cpu(kind, S([("XI", 0.001), ("ZI", -5.0), ("IZ", 0.3), ("YI", 2.0)]))
+    s.trim_keep(1, cpu(kind, S([("-iXI", 1.0)])))
+    out = s.to_host()
+    assert len(out) == 1 and out.term_string(0) == "XI"
+    # more protected terms than max_terms: only they survive, in term order
+    s = cpu(kind, S([("XI", 0.001), ("ZI", -5.0), ("IZ", 0.3), ("YI", 2.0)]))
+    s.trim_keep(1, cpu(kind, S([("IZ", 1.0), ("XI", 1.0)])))
+    out = s.to_host()
+    assert [out.term_string(t) for t in range(len(out))] == ["XI", "IZ"]
+    # n <= max_terms: untouched
+    s = cpu(kind, S([("XI", 0.001), ("ZI", -5.0)]))
+    s.trim_keep(2, cpu(kind, S([("YY", 1.0)])))
+    assert len(s) == 2
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_subtract_negate_scale_kats(kind):
+    # :608-616 DisjointTermsConcatenateNegatedSorted
+    s = cpu(kind, S([("XX", 1.0)]))
+    s.sub(cpu(kind, S([("ZZ", 2.0)])))
+    assert s.to_host().get_str() == "-2.000000 * ZZ\n1.000000 * XX\n"
+    # :618-633 OverlappingTermsCombine
+    s = cpu(kind, S([("XX", 1.0), ("ZZ", 2.0)]))
+    s.sub(cpu(kind, S([("XX", 3.0)])))
+    assert s.to_host().get_str() == "2.000000 * ZZ\n-2.000000 * XX\n"
+    # :638-645 SelfSubtractionZeroesCoefficients (zeros stay)
+    a = S([("XX", 1.0), ("ZZ", 2.0)])
+    s = cpu(kind, a)
+    s.sub(cpu(kind, a))
+    out = s.to_host()
+    assert len(out) == 2 and list(out.value) == [0.0, 0.0]
+    # :647-657 unary minus is scale(-1); :659-683 scalar products
+    s = cpu(kind, S([("XX", 1.0), ("ZZ", -2.5)]))
+    s.scale(-1.0)
+    assert s.to_host().get_str() == "-1.000000 * XX\n2.500000 * ZZ\n"
+    s = cpu(kind, S([("XX", 1.0), ("ZZ", -2.0)]))
+    s.scale(3.0)
+    assert s.to_host().get_str() == "3.000000 * XX\n-6.000000 * ZZ\n"
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_lookup_kats(kind):
+    # tests/PauliSumTests.cpp:768-817
+    s = S([("XXII", 1.0), ("IZZI", -0.5), ("IIXY", 0.25)])
+    assert [_lookup(kind, s, q) for q in ("XXII", "IZZI", "IIXY")] == [1.0, -0.5, 0.25]
+    s = S([("XX", 1.0), ("ZI", 2.0)])
+    assert _lookup(kind, s, "IZ") == 0.0 and _lookup(kind, s, "YY") == 0.0
+    assert _lookup(kind, S([], nq=2), "XX") == 0.0
+    assert _lookup(kind, S([("XZ", 1.0), ("IY", 4.0), ("XZ", 2.0)]), "XZ") == 3.0
+    s = S([("X", 1.0), ("-X", 0.25), ("iY", 2.0)])
+    assert [_lookup(kind, s, q) for q in ("X", "-X", "iY", "-iY")] == [0.75, -0.75, 2.0, -2.0]
+    s = S([("iZ", 3.0), ("Z", 1.0)])
+    assert _lookup(kind, s, "Z") == 1.0 and _lookup(kind, s, "iZ") == 3.0
+    a = ["I"] * 100
+    b = ["I"] * 100
+    a[3], a[97], b[97] = "X", "Z", "Z"
+    s = S([("".join(a), 1.5), ("".join(b), -2.0)], as_=2)
+    assert _lookup(kind, s, "".join(a)) == 1.5 and _lookup(kind, s, "".join(b)) == -2.0
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_trace_and_norm_kats(kind):
+    # :1042-1055 (trace), :1149-1155 (frobenius_norm2 = trace_product(s, s) / 2^n
+    # = 0.5^2 + 0.2^2 + 1.5^2 for the canonical sum)
+    assert cpu(kind, S([("IIII", 2.5), ("XIZI", 7.0), ("-IIII", 1.0), ("iIIII", 3.0)])).trace() == 16.0 * 1.5
+    assert cpu(kind, S([("XYZI", 1.0), ("ZZZZ", -3.0)])).trace() == 0.0
+    s = cpu(kind, S([("XII", 0.5), ("ZZI", -0.2), ("XYZ", 1.5)]))
+    s.canonicalize()
+    assert abs(s.frobenius_norm2() - (0.25 + 0.04 + 2.25)) <= 1e-14

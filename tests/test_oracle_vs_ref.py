@@ -126,3 +126,59 @@ def test_propagate_mixed_field_ising_trim_matches_reference(n, layers, log2m):
     a.propagate(circ, None, 0.0, 1 << log2m)
     b.propagate(circ, None, 0.0, 1 << log2m)
     assert_terms_equal(a.to_host(), b.to_host())
+
+
+def _distinct_magnitudes(host, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    host.value[:] = rng.permutation(np.arange(1, len(host) + 1)) * rng.choice([-1.0, 1.0], len(host)) / 7.0
+
+
+@pytest.mark.parametrize("nq,n,m,nkeep", [(12, 400, 50, 20), (70, 300, 10, 40), (30, 200, 150, 199)])
+def test_trim_keep_and_truncation_keep(nq, n, m, nkeep):
+    """trim(max_terms, keep) with distinct magnitudes (nth_element's tie
+    choice is left open), keep drawn partly from the sum itself."""
+    S = W.random_sum(n, nq, 0.5, "normal", n + nq)
+    _distinct_magnitudes(S, 3)
+    S.flags[:] = np.arange(n) % 4
+    pick = np.random.Generator(np.random.PCG64(nkeep)).choice(n, nkeep // 2, replace=False)
+    K = W.random_sum(nkeep - len(pick), nq, 0.5, "normal", 77)
+    K = HostSum(S.array_size, nq, np.concatenate([S.x[:, pick], K.x], axis=1),
+                np.concatenate([S.z[:, pick], K.z], axis=1),
+                np.concatenate([(S.flags[pick] + 1) % 4, K.flags]), np.ones(nkeep))
+    res = []
+    for kind in ("oracle", "reference"):
+        s = cpu(kind, S)
+        s.trim_keep(m, cpu(kind, K))
+        res.append(s.to_host())
+    assert_terms_equal(res[0], res[1])
+    res = []
+    for kind in ("oracle", "reference"):
+        s = cpu(kind, S)
+        s.apply_truncation_keep(cpu(kind, K), 6, 1e-3, m)
+        res.append(s.to_host())
+    assert_terms_equal(res[0], res[1])
+
+
+def test_sub_scale_lookup_trace_norm():
+    A = W.random_words_sum(300, 100, 5, array_size=2)
+    A.x[:, 10:14] = 0  # identity terms for trace
+    A.z[:, 10:14] = 0
+    B = W.random_words_sum(200, 100, 6, array_size=2)
+    B = HostSum(2, 100, np.concatenate([A.x[:, :50], B.x], axis=1), np.concatenate([A.z[:, :50], B.z], axis=1),
+                np.concatenate([A.flags[:50], B.flags]), np.concatenate([A.value[:50] * 0.5, B.value]))
+    res = []
+    for kind in ("oracle", "reference"):
+        s = cpu(kind, A)
+        s.sub(cpu(kind, B))
+        s.scale(-0.375)
+        res.append(s.to_host())
+    assert_terms_equal(res[0], res[1])
+    o, r = cpu("oracle", A), cpu("reference", A)
+    for t in [0, 7, 10, 49, 299]:
+        x, z = A.x[:, t], A.z[:, t]
+        for f in range(4):
+            assert o.lookup(x, z, f) == r.lookup(x, z, f)
+    assert o.trace() == r.trace() and o.trace() != 0.0
+    o.canonicalize()
+    r.canonicalize()
+    assert math.isclose(o.frobenius_norm2(), r.frobenius_norm2(), rel_tol=1e-12)
