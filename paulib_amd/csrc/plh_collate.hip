@@ -512,11 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_pass_hist(Src src, const IdKey* __re
         for (int u = 0; u < kHistU; ++u)
             if (dg[u] != kInvalid) atomicAdd(&h[dg[u]], 1u);
     }
-#pragma unroll
-    for (int s = kWave / 2; s > 0; s >>= 1) {
-        o |= __shfl_xor(o, s, kWave);
-        a &= __shfl_xor(a, s, kWave);
-    }
+    o = wave_or(o);
+    a = wave_and(a);
     if (lane_id() == 0) {
         atomicOr(&sor, (unsigned long long)o);
         atomicAnd(&sand, (unsigned long long)a);
@@ -684,7 +681,7 @@ PLH_DEV uint32_t group_children(const uint64_t (&st)[4], const uint64_t (&ss)[4]
         emit(o, st[k], g.P[nxt[k]] - g.P[d], g.NZ[nxt[k]] - g.NZ[d]);
         ++o;
     }
-    return (uint32_t)__shfl((int)hinc, kWave - 1, kWave);
+    return wave_last(hinc);
 }
 
 // Per big bucket (one workgroup each): the column scan of its tiles'
@@ -1325,9 +1322,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         }
 #pragma unroll
         for (int w = 0; w < KW + 1; ++w) {
-            uint64_t v = dl[w];
-#pragma unroll
-            for (int o = kWave / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
+            const uint64_t v = wave_or(dl[w]);
             if (lane_id() == 0 && v) atomicOr(&L.diff[w], (unsigned long long)v);
         }
     }
@@ -1609,8 +1604,7 @@ __global__ __launch_bounds__(kBlock) void k_live_or(SumView s, const uint32_t* _
             acc |= j[t];
         kw = 0;
     }
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, kWave);
+    acc = wave_or(acc);
     if (lane_id() == 0 && acc) atomicOr(&live[kw], (unsigned long long)acc);
 }
 
@@ -1993,19 +1987,13 @@ __global__ __launch_bounds__(kBlock) void k_pack(SumView s, const uint32_t* __re
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < AS; ++k) {
-        uint64_t a = ox[k], b = oz[k];
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            a |= __shfl_xor(a, o, kWave);
-            b |= __shfl_xor(b, o, kWave);
-        }
+        const uint64_t a = wave_or(ox[k]), b = wave_or(oz[k]);
         if (lane_id() == 0) {
             if (a) atomicOr(&acc[k], (unsigned long long)a);
             if (b) atomicOr(&acc[AS + k], (unsigned long long)b);
         }
     }
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) oj |= __shfl_xor(oj, o, kWave);
+    oj = wave_or(oj);
     if (lane_id() == 0 && oj) atomicOr(&acc[2 * AS], (unsigned long long)oj);
     __syncthreads();
     if (threadIdx.x < 2 * AS) {
@@ -2231,12 +2219,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_multi(const PackJob* __restrict
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < AS; ++k) {
-        uint64_t a = ox[k], b = oz[k];
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            a |= __shfl_xor(a, o, kWave);
-            b |= __shfl_xor(b, o, kWave);
-        }
+        const uint64_t a = wave_or(ox[k]), b = wave_or(oz[k]);
         if (lane_id() == 0) {
             if (a) atomicOr(&acc[k], (unsigned long long)a);
             if (b) atomicOr(&acc[AS + k], (unsigned long long)b);
