@@ -32,7 +32,9 @@
 // Batched sums (a job plane, the job id leading the key) rotate each term by
 // its own job's generator when that job's step is a general rotation; the
 // groups of step 2 never cross jobs because the job word lies above every
-// bit of M.
+// bit of M.  Their form (rotate_sorted_jobs_as) keeps the flags as bits with
+// per-tile counts and lets only the new terms search, so a step over a large
+// batch moves a fraction of the bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -443,6 +445,500 @@ __global__ __launch_bounds__(kBlock) void k_rs_write(Sorted<AS, JOB> S, uint32_t
 
 size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// ----------------------------------------------------------------------
+// Batched sums (the server's sorted steps): the same rotation with the
+// per-term state held as bit flags.  Old terms never search: the few new
+// terms find their lower bounds among the old ones, and an old term learns
+// the new terms below it (and its equal-key partner) from the sorted lower
+// bounds that fall in its tile.  Flags are bits with per-tile counts, so a
+// rank is a tile prefix, a word prefix and one popcount instead of a
+// full-length scan.  Per old term the passes read its job and the words
+// its job's P touches, then its value and flags, then the whole term
+// to write it: about 20 + 13 + 45 bytes read and 45 written at two words
+// per plane, against about 330 for the two-list merge above.
+//
+// Precondition (the server's sorted steps): every alive job's terms have
+// passed a truncation under the job's weight limit since its keys last
+// changed, so an old term, whose key a rotation keeps, is dropped only by
+// its coefficient (zero, cutoff) or by its job being drained.
+
+constexpr int kRjTile = 1024;              // terms per tile: 256 threads x 4
+constexpr int kRjWords = kRjTile / kWave;  // flag words per tile
+static_assert(kBlock * 4 == kRjTile, "a thread holds four terms of its tile");
+
+// Flags over terms t < cap (cap = tiles * kRjTile; flags past the live
+// count are 0): bits[w] = terms 64w .. 64w + 63, wpre[w] = set flags of
+// w's tile before word w, tpre[t] = exclusive scan of the tile counts
+// (tpre[tiles] = total).
+struct BitRank {
+    const uint64_t* bits;
+    const uint32_t* wpre;
+    const uint64_t* tpre;
+    uint64_t cap;
+    uint64_t tiles;
+    PLH_DEV bool test(uint64_t i) const { return (bits[i >> 6] >> (i & 63)) & 1ull; }
+    PLH_DEV uint64_t rank(uint64_t i) const {  // set flags before i
+        if (i >= cap) return tpre[tiles];
+        const uint64_t w = i >> 6;
+        const uint64_t below = bits[w] & ((1ull << (i & 63)) - 1ull);
+        return tpre[i / kRjTile] + wpre[w] + (uint64_t)popc(below);
+    }
+    PLH_DEV uint64_t total() const { return tpre[tiles]; }
+};
+
+// Where a tile's flags go: bits and in-tile word prefixes, the tile's
+// count (null: not counted).
+struct BitOut {
+    uint64_t* bits;
+    uint32_t* wpre;
+    uint32_t* tcnt;
+};
+
+// Term k (0..3) of this thread in tile t: t * 1024 + wave * 256 + k * 64 + lane,
+// so word t * 16 + wave * 4 + k holds it.
+PLH_DEV uint64_t rj_item(uint64_t tile, int k) {
+    return tile * kRjTile + (uint64_t)(threadIdx.x / kWave) * (4 * kWave) + (uint64_t)k * kWave + lane_id();
+}
+
+// Every thread of the block: writes the tile's flag words, their in-tile
+// prefixes and the tile count.  sh: kRjWords entries of LDS.
+PLH_DEV void rj_put(const bool (&fl)[4], uint64_t tile, const BitOut& o, uint32_t* sh) {
+    const int wv = (int)(threadIdx.x / kWave);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t b = ballot(fl[k]);
+        if (lane_id() == 0) {
+            o.bits[tile * kRjWords + (uint64_t)(wv * 4 + k)] = b;
+            sh[wv * 4 + k] = (uint32_t)popc(b);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)kRjWords) {
+        uint32_t s = 0;
+        for (unsigned w = 0; w < threadIdx.x; ++w) s += sh[w];
+        if (o.wpre) o.wpre[tile * kRjWords + threadIdx.x] = s;
+        if (o.tcnt && threadIdx.x == (unsigned)kRjWords - 1) o.tcnt[tile] = s + sh[kRjWords - 1];
+    }
+}
+
+// First index in [lo, hi) with v[t] >= key (gt: > key).
+PLH_DEV uint32_t u32_bound(const uint32_t* v, uint32_t lo, uint32_t hi, uint32_t key, bool gt) {
+    uint32_t cnt = hi - lo;
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = lo + half;
+        if (gt ? v[t] <= key : v[t] < key) {
+            lo = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    return lo;
+}
+
+// ---- 1. anticommuting flags: only the job and the words of its P are read
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_anti(Sorted<AS, true> S, uint32_t n, JobRotTables jt,
+                                                    const uint8_t* __restrict__ alive, BitOut ab) {
+    __shared__ uint32_t sh[kRjWords];
+    const uint64_t tile = blockIdx.x;
+    bool fl[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = rj_item(tile, k);
+        fl[k] = false;
+        if (i < n) {
+            const uint32_t job = S.j[i];
+            if (alive[job] && jt.kind[job] == 4) {
+                const uint64_t* p = jt.p + (size_t)job * 2 * AS;
+                uint64_t acc = 0;
+#pragma unroll
+                for (int w = 0; w < AS; ++w) {
+                    const uint64_t px = p[w], pz = p[AS + w];
+                    if (px | pz) acc ^= (S.x[(int64_t)w * S.stride + i] & pz) ^ (S.z[(int64_t)w * S.stride + i] & px);
+                }
+                fl[k] = (popc(acc) & 1) != 0;
+            }
+        }
+    }
+    rj_put(fl, tile, ab, sh);
+}
+
+// ---- 2. new terms at their ranks among the new terms (A: anticommuting
+// flags), as k_rs_new with ranks from the flag words
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
+                                                   const uint32_t* __restrict__ joff, NewTerms N) {
+    constexpr int KW = Sorted<AS, true>::KW;
+    constexpr int XO = 1;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || !A.test(i)) return;
+    uint64_t kw[KW + 1];
+    S.key(i, kw);
+    const uint32_t job = S.j[i];
+    const uint64_t* px = jt.p + (size_t)job * 2 * AS;
+    const uint64_t* pz = px + AS;
+    const uint32_t psf = jt.psf[job];
+    const double sn = jt.s[job];
+    const uint32_t base = joff[job], lim = joff[job + 1];
+    int64_t r = (int64_t)A.rank(i);
+#pragma unroll
+    for (int h = 0; h < 2 * AS; ++h) {
+        uint64_t m = h < AS ? px[h] : pz[h - AS];
+        const int wbase = 64 * (XO + h);
+        while (m) {
+            const int bit = 63 - __clzll(m);
+            m &= ~(1ull << bit);
+            const int g = wbase + (63 - bit);
+            if (bit_at(kw, g)) {
+                uint32_t lo = base, cnt = i - base;
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = lo + half;
+                    if (prefix_cmp(S, t, kw, g) < 0) {
+                        lo = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                uint32_t mid = lo;
+                cnt = i - lo;
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = mid + half;
+                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
+                        mid = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                r -= (int64_t)(A.rank(mid) - A.rank(lo));
+            } else {
+                uint32_t hi = i + 1, cnt = lim - (i + 1);
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = hi + half;
+                    if (prefix_cmp(S, t, kw, g) <= 0) {
+                        hi = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                uint32_t mid = i + 1;
+                cnt = hi - (i + 1);
+                while (cnt > 0) {
+                    const uint32_t half = cnt / 2, t = mid + half;
+                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
+                        mid = t + 1;
+                        cnt -= half + 1;
+                    } else {
+                        cnt = half;
+                    }
+                }
+                r += (int64_t)(A.rank(hi) - A.rank(mid));
+            }
+        }
+    }
+    if (r < 0 || (uint64_t)r >= A.total()) return;  // never with sorted distinct input; keeps writes in bounds
+    uint64_t c1 = 0, c2 = 0;
+    const uint32_t f0 = S.f[i] & 1u;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        phase_word(kw[XO + k], kw[XO + AS + k], px[k], pz[k], c1, c2);
+        N.x[(int64_t)k * N.cap + r] = kw[XO + k] ^ px[k];
+        N.z[(int64_t)k * N.cap + r] = kw[XO + AS + k] ^ pz[k];
+    }
+    const uint32_t raw = phase_of(f0, psf, c1, c2);
+    const double nv = (-sn) * S.value(i);
+    N.v[r] = (raw & 2u) ? -nv : nv;
+    N.f[r] = (uint8_t)(raw & 1u);
+    N.j[r] = job;
+}
+
+// ---- 3. each new term's lower bound among the old terms (inside its job),
+// whether it equals that old term (D), and whether it is kept as a term of
+// its own (KN: not equal to an old term, and passing the filters)
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, BitRank A, const uint32_t* __restrict__ joff,
+                                                  NewTerms N, Filters fl, uint32_t* __restrict__ lb, BitOut dout,
+                                                  BitOut knout) {
+    constexpr int KW = Sorted<AS, true>::KW;
+    __shared__ uint32_t sh[kRjWords];
+    const uint64_t tile = blockIdx.x;
+    const uint64_t m = A.total();
+    bool dup[4], kn[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t r = rj_item(tile, k);
+        dup[k] = false;
+        kn[k] = false;
+        if (r < m) {
+            uint64_t kw[KW + 1], ow[KW + 1];
+            new_key<AS, true>(N, (uint32_t)r, kw);
+            const uint32_t job = (uint32_t)kw[0];
+            uint32_t lo = joff[job];
+            const uint32_t hi = joff[job + 1];
+            uint32_t cnt = hi - lo;
+            while (cnt > 0) {
+                const uint32_t half = cnt / 2, t = lo + half;
+                S.key(t, ow);
+                if (key_less<KW>(ow, kw)) {
+                    lo = t + 1;
+                    cnt -= half + 1;
+                } else {
+                    cnt = half;
+                }
+            }
+            if (lo < hi) {
+                S.key(lo, ow);
+                dup[k] = key_eq<KW>(ow, kw);
+            }
+            kn[k] = !dup[k] && keep_of<AS, 1>(fl, kw, N.v[r]);
+            lb[r] = lo;
+        }
+    }
+    rj_put(dup, tile, dout, sh);
+    __syncthreads();  // sh is reused
+    rj_put(kn, tile, knout, sh);
+}
+
+// An old term's coefficient after the rotation: cos * v when it
+// anticommutes, plus the equal new term's (-sin) * v when there is one
+// (acc = 0; acc += old; acc += new, the reference's order).  u = the first
+// new term whose lower bound lies above i.
+template <int AS>
+PLH_DEV double rj_old_value(const Sorted<AS, true>& S, uint32_t i, uint32_t job, const JobRotTables& jt,
+                            const BitRank& A, const uint32_t* lb, const BitRank& D, const NewTerms& N, uint32_t r0,
+                            uint32_t r1, uint32_t* u_out) {
+    double vo = S.value(i);
+    if (A.test(i)) vo = vo * jt.c[job];
+    const uint32_t u = u32_bound(lb, r0, r1, i, true);
+    double acc = 0.0;
+    acc += vo;
+    if (u > r0 && lb[u - 1] == i && D.test(u - 1)) acc += N.v[u - 1];
+    *u_out = u;
+    return acc;
+}
+
+PLH_DEV bool rj_keep_old(const Filters& fl, uint32_t job, double acc) {
+    if ((fl.mode & kDropZero) && acc == 0.0) return false;
+    if (fl.mode & kCutoff) {
+        const double eps = fl.job_eps ? fl.job_eps[job] : fl.eps;
+        if (eps > 0.0 && !(fabs(acc) >= eps)) return false;
+    }
+    return true;
+}
+
+// The new terms whose lower bounds fall in old tile [i0, i0 + 1024).
+PLH_DEV void rj_tile_range(const uint32_t* lb, uint32_t m, uint64_t i0, uint32_t* rr) {
+    if (threadIdx.x == 0) {
+        rr[0] = u32_bound(lb, 0, m, (uint32_t)i0, false);
+        const uint64_t e = i0 + kRjTile;
+        rr[1] = u32_bound(lb, rr[0], m, e > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)e, false);
+    }
+    __syncthreads();
+}
+
+// ---- 4. old terms' keep flags (KO)
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
+                                                    const uint32_t* __restrict__ lb, BitRank D, NewTerms N, Filters fl,
+                                                    BitOut koout) {
+    __shared__ uint32_t sh[kRjWords];
+    __shared__ uint32_t rr[2];
+    const uint64_t tile = blockIdx.x;
+    rj_tile_range(lb, (uint32_t)A.total(), tile * kRjTile, rr);
+    bool ko[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = rj_item(tile, k);
+        ko[k] = false;
+        if (i < n) {
+            const uint32_t job = S.j[i];
+            if (fl.job_alive[job]) {
+                uint32_t u;
+                const double acc = rj_old_value<AS>(S, (uint32_t)i, job, jt, A, lb, D, N, rr[0], rr[1], &u);
+                ko[k] = rj_keep_old(fl, job, acc);
+            }
+        }
+    }
+    rj_put(ko, tile, koout, sh);
+}
+
+// The output count KO + KN, posted to the host.
+__global__ void k_rj_total(const uint64_t* __restrict__ ko_total, const uint64_t* __restrict__ kn_total,
+                           uint64_t* __restrict__ out, uint64_t* mbox, uint64_t seq) {
+    if (threadIdx.x != 0) return;
+    const uint64_t t = *ko_total + *kn_total;
+    *out = t;
+    post_write(mbox, seq, 1, &t);
+}
+
+// ---- 5. kept terms to their places: old term i after the kept old terms
+// before it and the kept new terms below it; new term r after the kept new
+// terms before it and the kept old terms below its lower bound.  Blocks
+// [0, tiles) take old tiles, [tiles, 2 tiles) new ones.
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_t n, uint64_t tiles, JobRotTables jt,
+                                                     BitRank A, const uint32_t* __restrict__ lb, BitRank D, BitRank KO,
+                                                     BitRank KN, NewTerms N, const uint64_t* __restrict__ d_total,
+                                                     uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
+                                                     uint8_t* __restrict__ of, double* __restrict__ ov,
+                                                     uint32_t* __restrict__ oj, int64_t os) {
+    __shared__ uint32_t rr[2];
+    const uint64_t total = *d_total;
+    const uint32_t m = (uint32_t)A.total();
+    if (blockIdx.x < tiles) {
+        const uint64_t tile = blockIdx.x;
+        rj_tile_range(lb, m, tile * kRjTile, rr);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t i = rj_item(tile, k);
+            if (i >= n || !KO.test(i)) continue;
+            const uint32_t job = S.j[i];
+            uint32_t u;
+            const double acc = rj_old_value<AS>(S, (uint32_t)i, job, jt, A, lb, D, N, rr[0], rr[1], &u);
+            const uint64_t o = KO.rank(i) + KN.rank(u);
+            if (o >= total) continue;  // never for consistent ranks; keeps writes in bounds
+#pragma unroll
+            for (int w = 0; w < AS; ++w) {
+                ox[(int64_t)w * os + o] = S.x[(int64_t)w * S.stride + i];
+                oz[(int64_t)w * os + o] = S.z[(int64_t)w * S.stride + i];
+            }
+            of[o] = S.f[i] & 1u;
+            ov[o] = acc;
+            oj[o] = job;
+        }
+        return;
+    }
+    const uint64_t tile = blockIdx.x - tiles;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t r = rj_item(tile, k);
+        if (r >= m || !KN.test(r)) continue;
+        const uint64_t o = KN.rank(r) + KO.rank(lb[r]);
+        if (o >= total) continue;
+#pragma unroll
+        for (int w = 0; w < AS; ++w) {
+            ox[(int64_t)w * os + o] = N.x[(int64_t)w * N.cap + r];
+            oz[(int64_t)w * os + o] = N.z[(int64_t)w * N.cap + r];
+        }
+        of[o] = N.f[r];
+        ov[o] = N.v[r];
+        oj[o] = N.j[r];
+    }
+}
+
+// Flag arrays of `tiles` tiles: bits, in-tile prefixes, tile counts and
+// their scan, carved from one workspace block.
+struct FlagBuf {
+    uint64_t* bits;
+    uint32_t* wpre;
+    uint32_t* tcnt;
+    uint64_t* tpre;
+    static size_t bytes(uint64_t tiles) {
+        return up256(tiles * kRjWords * 8) + up256(tiles * kRjWords * 4) + up256(tiles * 4) + up256((tiles + 1) * 8);
+    }
+    void carve(char*& p, uint64_t tiles) {
+        bits = (uint64_t*)p;
+        p += up256(tiles * kRjWords * 8);
+        wpre = (uint32_t*)p;
+        p += up256(tiles * kRjWords * 4);
+        tcnt = (uint32_t*)p;
+        p += up256(tiles * 4);
+        tpre = (uint64_t*)p;
+        p += up256((tiles + 1) * 8);
+    }
+    BitOut out(bool counted = true) const { return BitOut{bits, counted ? wpre : nullptr, counted ? tcnt : nullptr}; }
+    BitRank rank(uint64_t tiles) const { return BitRank{bits, wpre, tpre, tiles * kRjTile, tiles}; }
+};
+
+template <int AS>
+int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, const Filters& fl, uint32_t njobs) {
+    hipStream_t st = ctx->stream;
+    const uint64_t n = (uint64_t)s->n;
+    const uint64_t tiles = (n + kRjTile - 1) / kRjTile;
+    Sorted<AS, true> S{s->x, s->z, s->f, s->v, s->j, s->cap, s->as};
+    // flags: anticommuting (A), new term equals an old one (D), new term kept
+    // (KN), old term kept (KO); the new terms (at most n) and their lower bounds
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + up256(((size_t)njobs + 1) * 4) + up256(16), &p));
+    char* c = (char*)p;
+    FlagBuf fa, fd, fkn, fko;
+    fa.carve(c, tiles);
+    fd.carve(c, tiles);
+    fkn.carve(c, tiles);
+    fko.carve(c, tiles);
+    uint32_t* joff = (uint32_t*)c;
+    c += up256(((size_t)njobs + 1) * 4);
+    uint64_t* d_total = (uint64_t*)c;
+    const size_t nb_planes = up256(n * 8 * AS);
+    PLH_TRY(ctx->ws.get(kWsRotN, 2 * nb_planes + up256(n * 8) + up256(n) + up256(n * 4) + up256(n * 4), &p));
+    NewTerms N;
+    N.x = (uint64_t*)p;
+    N.z = (uint64_t*)((char*)p + nb_planes);
+    N.v = (double*)((char*)p + 2 * nb_planes);
+    N.f = (uint8_t*)((char*)N.v + up256(n * 8));
+    N.j = (uint32_t*)((char*)N.f + up256(n));
+    N.cap = (int64_t)n;
+    uint32_t* lb = (uint32_t*)((char*)N.j + up256(n * 4));
+    const unsigned gt = (unsigned)tiles, g1 = blocks_of(n);
+    k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, joff);
+    PLH_HIP(hipGetLastError());
+    {
+        ProfScope ps(ctx, "rot_anti");
+        k_rj_anti<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, fl.job_alive, fa.out());
+    }
+    PLH_HIP(hipGetLastError());
+    PLH_TRY(scan_u32_to_u64(ctx, fa.tcnt, fa.tpre, tiles, fa.tpre + tiles));
+    const BitRank A = fa.rank(tiles);
+    {
+        ProfScope ps(ctx, "rot_new");
+        k_rj_new<AS><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, joff, N);
+        PLH_HIP(hipGetLastError());
+        k_rj_lb<AS><<<gt, kBlock, 0, st>>>(S, A, joff, N, fl, lb, fd.out(false), fkn.out());
+        PLH_HIP(hipGetLastError());
+    }
+    PLH_TRY(scan_u32_to_u64(ctx, fkn.tcnt, fkn.tpre, tiles, fkn.tpre + tiles));
+    const BitRank D = fd.rank(tiles), KN = fkn.rank(tiles);
+    {
+        ProfScope ps(ctx, "rot_merge");
+        k_rj_keep<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, lb, D, N, fl, fko.out());
+        PLH_HIP(hipGetLastError());
+    }
+    PLH_TRY(scan_u32_to_u64(ctx, fko.tcnt, fko.tpre, tiles, fko.tpre + tiles));
+    const BitRank KO = fko.rank(tiles);
+    Post post;
+    PLH_TRY(post_prepare(ctx, &post));
+    k_rj_total<<<1, kWave, 0, st>>>(fko.tpre + tiles, fkn.tpre + tiles, d_total, post.mbox, post.seq);
+    PLH_HIP(hipGetLastError());
+    uint64_t total = 0;
+    PLH_TRY(post_wait(ctx, post, 1, &total));
+    plh_sum tmp;
+    tmp.ctx = ctx;
+    tmp.as = s->as;
+    tmp.nq = s->nq;
+    tmp.has_job = true;
+    PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(total, 1), 0));
+    if (total) {
+        ProfScope ps(ctx, "rot_write", (double)total * (16.0 * s->as + 13.0));
+        k_rj_write<AS><<<2 * gt, kBlock, 0, st>>>(S, (uint32_t)n, tiles, jt, A, lb, D, KO, KN, N, d_total, tmp.x,
+                                                  tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            sum_free_storage(&tmp);
+            PLH_HIP(e);
+        }
+    }
+    const int64_t nq = s->nq;
+    sum_take_storage(s, &tmp);
+    s->n = (int64_t)total;
+    s->nq = nq;
+    s->canonical = true;
+    s->sorted = true;
+    return PLH_OK;
+}
+
 template <int AS, bool JOB>
 int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const JobRotTables& jt, const Filters& fl,
                      uint32_t njobs) {
@@ -570,9 +1066,12 @@ int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, int njo
     }
     if (s->n == 0) return PLH_OK;
     const Filters fl = filters_of(spec);
-    if (njobs < 1) return PLH_ERR_ARG;
-    if (s->as == 1) return rotate_sorted_as<1, true>(ctx, s, RotParams<1>{}, jt, fl, (uint32_t)njobs);
-    if (s->as == 2) return rotate_sorted_as<2, true>(ctx, s, RotParams<2>{}, jt, fl, (uint32_t)njobs);
+    if (njobs < 1 || !spec.job_alive) {
+        set_error("rotate_sorted_jobs: needs at least one job slot and the job-alive table");
+        return PLH_ERR_ARG;
+    }
+    if (s->as == 1) return rotate_sorted_jobs_as<1>(ctx, s, jt, fl, (uint32_t)njobs);
+    if (s->as == 2) return rotate_sorted_jobs_as<2>(ctx, s, jt, fl, (uint32_t)njobs);
     set_error("rotate_sorted_jobs: batched sums support up to 128 qubits");
     return PLH_ERR_AS;
 }
