@@ -134,6 +134,43 @@ int rotation_step(plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pf
     return PLH_OK;
 }
 
+// Rotations l, l - 1, ... (Heisenberg order: the last op first) through the
+// window of rotate_sorted_window when the sum qualifies: sorted, at most two
+// words per plane, and `clean` (it passed this truncation since its keys last
+// changed).  rot_at(l, &px, &pz, &pf, &theta) describes op l, false when it
+// is not a rotation.  The window holds the rotations from l down whose path
+// is general or no change (up to 16).  *used = the rotations consumed, 0 when
+// the window does not apply (the caller runs op l as before).
+template <class RotAt>
+int rotation_window_step(plh_sum* s, bool clean, int64_t l, RotAt rot_at, int64_t max_weight, double cutoff,
+                         int64_t max_terms, int64_t* used) {
+    *used = 0;
+    if (!clean || !s->sorted || s->as > 2 || s->n > (int64_t)0x7FFFFFFFll) return PLH_OK;
+    constexpr int K = 16;
+    const uint64_t* px[K];
+    const uint64_t* pz[K];
+    uint8_t kind[K];
+    uint32_t psf[K];
+    double c[K], sn[K];
+    int w = 0;
+    for (int64_t t = l; t >= 0 && w < K; --t, ++w) {
+        uint8_t pf = 0;
+        double theta = 0.0;
+        if (!rot_at(t, &px[w], &pz[w], &pf, &theta)) break;
+        angle_cos_sin(theta, &c[w], &sn[w]);
+        const int k = rotation_kind(c[w], sn[w]);
+        if (k != 0 && k != 4) break;
+        kind[w] = (uint8_t)k;
+        psf[w] = (uint32_t)(pf ^ 1u);
+    }
+    if (w == 0) return PLH_OK;
+    int consumed = 0;
+    PLH_TRY(rotate_sorted_window(s->ctx, s, w, px, pz, kind, psf, c, sn, trunc_spec(max_weight, cutoff), &consumed));
+    if (max_terms >= 0) PLH_TRY(op_trim(s->ctx, s, max_terms));  // a no-op unless a rotation added terms
+    *used = consumed;
+    return PLH_OK;
+}
+
 // Host copy of len bytes spread over up to 8 threads (the destination's
 // first-touch page faults are spread with it).
 void parallel_memcpy(void* dst, const void* src, size_t len) {
@@ -944,9 +981,24 @@ int plh_propagate(plh_sum* s, int64_t nrot, const uint64_t* px, const uint64_t* 
                   const double* theta, int64_t max_weight, double cutoff, int64_t max_terms) {
     PLH_TRY(check_sum(s));
     if (nrot < 0 || (nrot > 0 && (!px || !pz || !theta))) return PLH_ERR_ARG;
-    for (int64_t l = nrot; l-- > 0;) {
-        PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, pflags ? pflags[l] : 0, theta[l], true,
-                              max_weight, cutoff, max_terms));
+    auto rot_at = [&](int64_t t, const uint64_t** rx, const uint64_t** rz, uint8_t* pf, double* th) {
+        *rx = px + t * s->as;
+        *rz = pz + t * s->as;
+        *pf = pflags ? pflags[t] : 0;
+        *th = theta[t];
+        return true;
+    };
+    bool clean = false;  // truncated since the keys last changed
+    for (int64_t l = nrot - 1; l >= 0;) {
+        int64_t used = 0;
+        PLH_TRY(rotation_window_step(s, clean, l, rot_at, max_weight, cutoff, max_terms, &used));
+        if (used == 0) {
+            PLH_TRY(rotation_step(s, px + l * s->as, pz + l * s->as, pflags ? pflags[l] : 0, theta[l], true,
+                                  max_weight, cutoff, max_terms));
+            used = 1;
+        }
+        clean = true;
+        l -= used;
     }
     return plh_ctx_synchronize(s->ctx);
 }
@@ -956,7 +1008,27 @@ int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t
                     double cutoff, int64_t max_terms) {
     PLH_TRY(check_sum(s));
     if (nops < 0 || (nops > 0 && (!kind || !q0 || !q1 || !px || !pz || !theta))) return PLH_ERR_ARG;
+    auto rot_at = [&](int64_t t, const uint64_t** rx, const uint64_t** rz, uint8_t* pf, double* th) {
+        if (kind[t] != 3) return false;
+        *rx = px + t * s->as;
+        *rz = pz + t * s->as;
+        *pf = 0;
+        *th = theta[t];
+        return true;
+    };
+    bool clean = false;  // truncated since the keys last changed (Clifford ops change them)
     for (int64_t l = nops; l-- > 0;) {
+        if (kind[l] == 3) {
+            int64_t used = 0;
+            PLH_TRY(rotation_window_step(s, clean, l, rot_at, max_weight, cutoff, max_terms, &used));
+            clean = true;
+            if (used > 0) {
+                l -= used - 1;
+                continue;
+            }
+        } else {
+            clean = false;
+        }
         switch (kind[l]) {
             case 0: PLH_TRY(op_clifford(s->ctx, s, 0, q0[l], 0)); break;
             case 1: PLH_TRY(op_clifford(s->ctx, s, 1, q0[l], 0)); break;

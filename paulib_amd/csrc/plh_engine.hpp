@@ -305,10 +305,26 @@ int collate_rotation_jobs(plh_ctx* ctx, plh_sum* in, const JobRotTables& jt, con
 // keys), without a sort (plh_rotate.hip); psf = flags of ps.
 int rotate_sorted(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint32_t psf, double c,
                   double sn, const CollateSpec& spec);
+// Rotations e = 0 .. w - 1 (w <= 16; kind 0 no change or 4 general, psf =
+// flags of ps) on a sorted canonical sum of at most two words per plane that
+// has passed spec's truncation since its keys last changed: the first
+// rotation a term anticommutes with is applied as rotate_sorted, the ones
+// before it are no-ops; *consumed = its index + 1, or w when none changes a
+// term (the sum is then untouched).
+int rotate_sorted_window(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const* px, const uint64_t* const* pz,
+                         const uint8_t* kind, const uint32_t* psf, const double* c, const double* sn,
+                         const CollateSpec& spec, int* consumed);
 // The same for a canonical batched sum of job slots 0 .. njobs - 1, each job
-// by its own table entry (jobs whose kind is not 4 pass through, under their
-// own filters).
-int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, int njobs, const CollateSpec& spec);
+// over its own window of rotations (JobWindows): a job applies the first
+// rotation of its window that changes a term and consumes the rotations up
+// to it (all of them when none does); jobs with an empty window pass
+// through, under their own filters.  adv_host (device address of host-pinned
+// memory, njobs bytes) receives the rotations each job consumed; it is
+// complete when the call returns.  Every job's terms must have passed its
+// truncation since its keys last changed.
+struct JobWindows;
+int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, int njobs, const CollateSpec& spec,
+                       uint8_t* adv_host);
 // out = canonicalize(lhs[k] * rhs[k]) for k < njobs as one batched sum
 // (job plane = k, jobs in order); offsets[k] .. offsets[k + 1] is job k's
 // range (njobs + 1 entries, host).  Sum of n_k * m_k <= 2^32.

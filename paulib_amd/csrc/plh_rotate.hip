@@ -217,6 +217,98 @@ __global__ __launch_bounds__(kBlock) void k_rs_anti(Sorted<AS, JOB> S, uint32_t 
     a[i] = (r && anticommutes<AS>(kw, S.XO, px, pz)) ? 1u : 0u;
 }
 
+// Galloping searches over a sorted range: the cost grows with the log of
+// the distance to the answer, not of the range, and the groups of the low
+// key bits a rotation flips hold few terms.
+// pred holds on [x, i] and fails on [base, x) (pred(i) holds); returns x.
+template <class P>
+PLH_DEV uint32_t gallop_back(uint32_t base, uint32_t i, P pred) {
+    uint32_t t_true = i, step = 1, L = base;
+    while (t_true - base >= step) {
+        const uint32_t t = t_true - step;
+        if (!pred(t)) {
+            L = t + 1;
+            break;
+        }
+        t_true = t;
+        step <<= 1;
+    }
+    uint32_t cnt = t_true - L;  // the first true lies in [L, t_true]
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = L + half;
+        if (!pred(t)) {
+            L = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    return L;
+}
+
+// pred holds on [i, x) and fails on [x, lim); returns x (lim when it holds to the end).
+template <class P>
+PLH_DEV uint32_t gallop_fwd(uint32_t i, uint32_t lim, P pred) {
+    uint32_t t_end = i, step = 1, hi = lim;
+    while (lim - t_end >= step) {
+        const uint32_t t = t_end + step - 1;
+        if (!pred(t)) {
+            hi = t;
+            break;
+        }
+        t_end = t + 1;
+        step <<= 1;
+    }
+    uint32_t L = t_end, cnt = hi - L;  // the first false lies in [t_end, hi]
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = L + half;
+        if (pred(t)) {
+            L = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    return L;
+}
+
+// Rank of item i's new term k_i ^ M among the new terms: rank(i) (the
+// anticommuting items before i) corrected at each set bit g of M: inside
+// the group of items sharing i's bits above g ([base, lim) bounds the
+// search), XOR with M swaps the bit-0 and bit-1 halves.  rank(t) = the
+// anticommuting items before t.
+template <int AS, bool JOB, class Rank>
+PLH_DEV int64_t new_rank(const Sorted<AS, JOB>& S, const uint64_t* kw, uint32_t i, uint32_t base, uint32_t lim,
+                         const uint64_t* px, const uint64_t* pz, Rank rank) {
+    constexpr int XO = Sorted<AS, JOB>::XO;
+    int64_t r = (int64_t)rank(i);
+#pragma unroll
+    for (int h = 0; h < 2 * AS; ++h) {
+        uint64_t m = h < AS ? px[h] : pz[h - AS];
+        const int wbase = 64 * (XO + h);
+        while (m) {
+            const int bit = 63 - __clzll(m);  // bit index within the word
+            m &= ~(1ull << bit);
+            const int g = wbase + (63 - bit);
+            const int gw = g >> 6, gs = 63 - (g & 63);
+            auto same_group = [&](uint32_t t) { return prefix_cmp(S, t, kw, g) == 0; };
+            auto bit_set = [&](uint32_t t) { return ((S.word(t, gw) >> gs) & 1ull) != 0; };
+            if (bit_at(kw, g)) {
+                // group start, then its bit-1 half's start (both at or before i)
+                const uint32_t lo = gallop_back(base, i, same_group);
+                const uint32_t mid = gallop_back(lo, i, bit_set);
+                r -= (int64_t)(rank(mid) - rank(lo));
+            } else {
+                // group end, then its bit-1 half's start (both after i)
+                const uint32_t hi = gallop_fwd(i + 1, lim, same_group);
+                const uint32_t mid = gallop_fwd(i + 1, hi, [&](uint32_t t) { return !bit_set(t); });
+                r += (int64_t)(rank(hi) - rank(mid));
+            }
+        }
+    }
+    return r;
+}
+
 // ---- 2. new terms at their ranks (A: exclusive scan of a, A[n] = count)
 template <int AS, bool JOB>
 __global__ __launch_bounds__(kBlock) void k_rs_new(Sorted<AS, JOB> S, uint32_t n, RotParams<AS> rp, JobRotTables jt,
@@ -235,69 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_new(Sorted<AS, JOB> S, uint32_t n
     rot_of<AS, JOB>(rp, jt, job, px, pz, psf, c, sn);
     // the item's job range (the groups below never leave it)
     const uint32_t base = JOB ? joff[job] : 0u, lim = JOB ? joff[job + 1] : n;
-    int64_t r = (int64_t)A[i];
-    // each set bit b of M: x bits then z bits, any order
-#pragma unroll
-    for (int h = 0; h < 2 * AS; ++h) {
-        uint64_t m = h < AS ? px[h] : pz[h - AS];
-        const int wbase = 64 * (XO + h);
-        while (m) {
-            const int bit = 63 - __clzll(m);  // bit index within the word
-            m &= ~(1ull << bit);
-            const int g = wbase + (63 - bit);
-            if (bit_at(kw, g)) {
-                // group start: first t in [base, i] with the same bits above g
-                uint32_t lo = base, cnt = i - base;
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = lo + half;
-                    if (prefix_cmp(S, t, kw, g) < 0) {
-                        lo = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                // first t in [lo, i] with bit g set (the group's bit-1 half)
-                uint32_t mid = lo;
-                cnt = i - lo;
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = mid + half;
-                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
-                        mid = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                r -= (int64_t)(A[mid] - A[lo]);
-            } else {
-                // group end: first t in (i, lim) with larger bits above g
-                uint32_t hi = i + 1, cnt = lim - (i + 1);
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = hi + half;
-                    if (prefix_cmp(S, t, kw, g) <= 0) {
-                        hi = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                // first t in (i, hi) with bit g set
-                uint32_t mid = i + 1;
-                cnt = hi - (i + 1);
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = mid + half;
-                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
-                        mid = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                r += (int64_t)(A[hi] - A[mid]);
-            }
-        }
-    }
+    const int64_t r = new_rank<AS, JOB>(S, kw, i, base, lim, px, pz, [&](uint32_t t) { return A[t]; });
     if (r < 0 || (uint64_t)r >= A[n]) return;  // never with sorted distinct input; keeps writes in bounds
     // the new term Q * ps (PauliSum::multiply(i, ps), :185-209), sign folded
     uint64_t c1 = 0, c2 = 0;
@@ -536,10 +566,143 @@ PLH_DEV uint32_t u32_bound(const uint32_t* v, uint32_t lo, uint32_t hi, uint32_t
     return lo;
 }
 
-// ---- 1. anticommuting flags: only the job and the words of its P are read
+// ---- 0. job ranges (as k_rs_bounds) and the per-job minimum reset
+__global__ void k_rj_bounds(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs, uint32_t* __restrict__ off,
+                            uint32_t* __restrict__ jmin) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > njobs) return;
+    if (k < njobs) jmin[k] = 0xFFu;
+    uint32_t lo = 0, cnt = n;
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = lo + half;
+        if (J[t] < k) {
+            lo = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    off[k] = lo;
+}
+
+// The first rotation of job's window that the term (x, z words) anticommutes
+// with, 255 when none; entries of kind 0 (no change) never count.
 template <int AS>
-__global__ __launch_bounds__(kBlock) void k_rj_anti(Sorted<AS, true> S, uint32_t n, JobRotTables jt,
-                                                    const uint8_t* __restrict__ alive, BitOut ab) {
+PLH_DEV uint32_t window_first(const JobWindows& jw, uint32_t job, const uint64_t* xw, const uint64_t* zw) {
+    const int w = jw.wn[job];
+    uint32_t f = 0xFFu;
+    for (int e = w - 1; e >= 0; --e) {  // descending: the last hit is the first index
+        const size_t ent = (size_t)job * kJobWindow + (size_t)e;
+        if (jw.kind[ent] != 4) continue;
+        const uint64_t* p = jw.p + ent * 2 * AS;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) acc ^= (xw[k] & p[AS + k]) ^ (zw[k] & p[k]);
+        if (popc(acc) & 1) f = (uint32_t)e;
+    }
+    return f;
+}
+
+template <class T>
+PLH_DEV T wave_min(T v) {
+    return wave_last(wave_scan_op(v, (T)~T(0), [](T a, T b) { return a < b ? a : b; }));
+}
+
+// ---- 1. per term the first rotation of its job's window it anticommutes
+// with (fk, one byte), and per job the least of them (jmin).  A wave of 64
+// consecutive terms of one job (the common case) reads its window through
+// scalar loads and only the words the window touches.
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_t n, JobWindows jw,
+                                                     const uint8_t* __restrict__ alive, uint8_t* __restrict__ fk,
+                                                     uint32_t* __restrict__ jmin) {
+    const uint64_t tile = blockIdx.x;
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = rj_item(tile, k);
+        if (rj_item(tile, k) - lane_id() >= n) break;  // the word's first term: uniform
+        const bool valid = i < n;
+        const uint32_t job = valid ? S.j[i] : 0xFFFFFFFFu;
+        const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);  // lane 0 is valid
+        const bool uni = ballot(valid && job != j0) == 0ull;
+        uint32_t f = 0xFFu;
+        if (uni) {
+            if (alive[j0] && jw.wn[j0] > 0) {
+                // the words the window's rotations read: x where P has z, z where P has x
+                uint64_t nx[AS], nz[AS];
+#pragma unroll
+                for (int w = 0; w < AS; ++w) nx[w] = nz[w] = 0;
+                for (int e = 0; e < (int)jw.wn[j0]; ++e) {
+                    const uint64_t* p = jw.p + ((size_t)j0 * kJobWindow + (size_t)e) * 2 * AS;
+#pragma unroll
+                    for (int w = 0; w < AS; ++w) {
+                        nx[w] |= p[AS + w];
+                        nz[w] |= p[w];
+                    }
+                }
+                uint64_t xw[AS], zw[AS];
+#pragma unroll
+                for (int w = 0; w < AS; ++w) {
+                    xw[w] = (nx[w] && valid) ? S.x[(int64_t)w * S.stride + i] : 0ull;
+                    zw[w] = (nz[w] && valid) ? S.z[(int64_t)w * S.stride + i] : 0ull;
+                }
+                if (valid) f = window_first<AS>(jw, j0, xw, zw);
+            }
+            const uint32_t m = wave_min(f);
+            if (lane_id() == 0 && m < 0xFFu) atomicMin(&jmin[j0], m);
+        } else if (valid && alive[job] && jw.wn[job] > 0) {
+            uint64_t xw[AS], zw[AS];
+#pragma unroll
+            for (int w = 0; w < AS; ++w) {
+                xw[w] = S.x[(int64_t)w * S.stride + i];
+                zw[w] = S.z[(int64_t)w * S.stride + i];
+            }
+            f = window_first<AS>(jw, job, xw, zw);
+            if (f < 0xFFu) atomicMin(&jmin[job], f);
+        }
+        if (valid) fk[i] = (uint8_t)f;
+    }
+}
+
+// ---- 1b. per job the rotation it applies (the first of its window that
+// changes a term: jmin) as this step's rotation table, and the rotations it
+// consumes (adv: up to and including that one; the whole window when no
+// rotation of it changes a term).
+template <int AS>
+__global__ void k_rj_select(JobWindows jw, uint32_t njobs, const uint32_t* __restrict__ jmin,
+                            uint8_t* __restrict__ kind, uint64_t* __restrict__ p, uint8_t* __restrict__ psf,
+                            double* __restrict__ c, double* __restrict__ sn, uint8_t* __restrict__ kstar,
+                            uint8_t* __restrict__ adv) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= njobs) return;
+    const uint32_t w = jw.wn[t], m = jmin[t];
+    kind[t] = 0;
+    kstar[t] = 0xFFu;
+    if (w == 0) {
+        adv[t] = 0;
+        return;
+    }
+    if (m >= w) {
+        adv[t] = (uint8_t)w;
+        return;
+    }
+    const size_t ent = (size_t)t * kJobWindow + m;
+    adv[t] = (uint8_t)(m + 1);
+    kind[t] = jw.kind[ent];
+    kstar[t] = (uint8_t)m;
+#pragma unroll
+    for (int k = 0; k < 2 * AS; ++k) p[(size_t)t * 2 * AS + k] = jw.p[ent * 2 * AS + k];
+    psf[t] = jw.psf[ent];
+    c[t] = jw.c[ent];
+    sn[t] = jw.s[ent];
+}
+
+// ---- 1c. anticommuting flags for the applied rotations: a term's first
+// hit in its window is its job's least one (every term's is at least that)
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__ J, uint32_t n,
+                                                    const uint8_t* __restrict__ fk, const uint8_t* __restrict__ kstar,
+                                                    BitOut ab) {
     __shared__ uint32_t sh[kRjWords];
     const uint64_t tile = blockIdx.x;
     bool fl[4];
@@ -548,17 +711,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_anti(Sorted<AS, true> S, uint32_t
         const uint64_t i = rj_item(tile, k);
         fl[k] = false;
         if (i < n) {
-            const uint32_t job = S.j[i];
-            if (alive[job] && jt.kind[job] == 4) {
-                const uint64_t* p = jt.p + (size_t)job * 2 * AS;
-                uint64_t acc = 0;
-#pragma unroll
-                for (int w = 0; w < AS; ++w) {
-                    const uint64_t px = p[w], pz = p[AS + w];
-                    if (px | pz) acc ^= (S.x[(int64_t)w * S.stride + i] & pz) ^ (S.z[(int64_t)w * S.stride + i] & px);
-                }
-                fl[k] = (popc(acc) & 1) != 0;
-            }
+            const uint32_t ks = kstar[J[i]];
+            fl[k] = ks != 0xFFu && fk[i] == ks;
         }
     }
     rj_put(fl, tile, ab, sh);
@@ -581,64 +735,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t 
     const uint32_t psf = jt.psf[job];
     const double sn = jt.s[job];
     const uint32_t base = joff[job], lim = joff[job + 1];
-    int64_t r = (int64_t)A.rank(i);
-#pragma unroll
-    for (int h = 0; h < 2 * AS; ++h) {
-        uint64_t m = h < AS ? px[h] : pz[h - AS];
-        const int wbase = 64 * (XO + h);
-        while (m) {
-            const int bit = 63 - __clzll(m);
-            m &= ~(1ull << bit);
-            const int g = wbase + (63 - bit);
-            if (bit_at(kw, g)) {
-                uint32_t lo = base, cnt = i - base;
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = lo + half;
-                    if (prefix_cmp(S, t, kw, g) < 0) {
-                        lo = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                uint32_t mid = lo;
-                cnt = i - lo;
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = mid + half;
-                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
-                        mid = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                r -= (int64_t)(A.rank(mid) - A.rank(lo));
-            } else {
-                uint32_t hi = i + 1, cnt = lim - (i + 1);
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = hi + half;
-                    if (prefix_cmp(S, t, kw, g) <= 0) {
-                        hi = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                uint32_t mid = i + 1;
-                cnt = hi - (i + 1);
-                while (cnt > 0) {
-                    const uint32_t half = cnt / 2, t = mid + half;
-                    if (!((S.word(t, g >> 6) >> (63 - (g & 63))) & 1ull)) {
-                        mid = t + 1;
-                        cnt -= half + 1;
-                    } else {
-                        cnt = half;
-                    }
-                }
-                r += (int64_t)(A.rank(hi) - A.rank(mid));
-            }
-        }
-    }
+    const int64_t r = new_rank<AS, true>(S, kw, i, base, lim, px, pz, [&](uint32_t t) { return A.rank(t); });
     if (r < 0 || (uint64_t)r >= A.total()) return;  // never with sorted distinct input; keeps writes in bounds
     uint64_t c1 = 0, c2 = 0;
     const uint32_t f0 = S.f[i] & 1u;
@@ -765,9 +862,17 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
     rj_put(ko, tile, koout, sh);
 }
 
-// The output count KO + KN, posted to the host.
-__global__ void k_rj_total(const uint64_t* __restrict__ ko_total, const uint64_t* __restrict__ kn_total,
-                           uint64_t* __restrict__ out, uint64_t* mbox, uint64_t seq) {
+// The output count KO + KN and each job's consumed rotations, posted to the
+// host: every thread fences its stores to host memory before the block's
+// barrier, and thread 0 then posts the count (the host polls its sequence).
+__global__ __launch_bounds__(kBlock) void k_rj_total(const uint64_t* __restrict__ ko_total,
+                                                     const uint64_t* __restrict__ kn_total,
+                                                     uint64_t* __restrict__ out, const uint8_t* __restrict__ adv,
+                                                     uint32_t njobs, uint8_t* adv_host, uint64_t* mbox,
+                                                     uint64_t seq) {
+    for (uint32_t t = threadIdx.x; t < njobs; t += blockDim.x) adv_host[t] = adv[t];
+    __threadfence_system();
+    __syncthreads();
     if (threadIdx.x != 0) return;
     const uint64_t t = *ko_total + *kn_total;
     *out = t;
@@ -854,15 +959,21 @@ struct FlagBuf {
 };
 
 template <int AS>
-int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, const Filters& fl, uint32_t njobs) {
+int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const Filters& fl, uint32_t njobs,
+                          uint8_t* adv_host) {
     hipStream_t st = ctx->stream;
     const uint64_t n = (uint64_t)s->n;
     const uint64_t tiles = (n + kRjTile - 1) / kRjTile;
     Sorted<AS, true> S{s->x, s->z, s->f, s->v, s->j, s->cap, s->as};
     // flags: anticommuting (A), new term equals an old one (D), new term kept
-    // (KN), old term kept (KO); the new terms (at most n) and their lower bounds
+    // (KN), old term kept (KO); per job: its range, least window hit, applied
+    // rotation and consumed count; per term its first window hit; the new
+    // terms (at most n) and their lower bounds
+    const size_t nj = njobs;
+    const size_t job_bytes = up256((nj + 1) * 4) + up256(nj * 4) + up256(nj) + up256(nj * 2 * AS * 8) + up256(nj) +
+                             2 * up256(nj * 8) + up256(nj) + up256(nj) + up256(16);
     void* p;
-    PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + up256(((size_t)njobs + 1) * 4) + up256(16), &p));
+    PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n), &p));
     char* c = (char*)p;
     FlagBuf fa, fd, fkn, fko;
     fa.carve(c, tiles);
@@ -870,8 +981,27 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, cons
     fkn.carve(c, tiles);
     fko.carve(c, tiles);
     uint32_t* joff = (uint32_t*)c;
-    c += up256(((size_t)njobs + 1) * 4);
+    c += up256((nj + 1) * 4);
+    uint32_t* jmin = (uint32_t*)c;
+    c += up256(nj * 4);
+    uint8_t* e_kind = (uint8_t*)c;
+    c += up256(nj);
+    uint64_t* e_p = (uint64_t*)c;
+    c += up256(nj * 2 * AS * 8);
+    uint8_t* e_psf = (uint8_t*)c;
+    c += up256(nj);
+    double* e_c = (double*)c;
+    c += up256(nj * 8);
+    double* e_s = (double*)c;
+    c += up256(nj * 8);
+    uint8_t* kstar = (uint8_t*)c;
+    c += up256(nj);
+    uint8_t* adv = (uint8_t*)c;
+    c += up256(nj);
     uint64_t* d_total = (uint64_t*)c;
+    c += up256(16);
+    uint8_t* fk = (uint8_t*)c;
+    const JobRotTables jt{e_kind, e_p, e_psf, e_c, e_s};
     const size_t nb_planes = up256(n * 8 * AS);
     PLH_TRY(ctx->ws.get(kWsRotN, 2 * nb_planes + up256(n * 8) + up256(n) + up256(n * 4) + up256(n * 4), &p));
     NewTerms N;
@@ -883,13 +1013,17 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, cons
     N.cap = (int64_t)n;
     uint32_t* lb = (uint32_t*)((char*)N.j + up256(n * 4));
     const unsigned gt = (unsigned)tiles, g1 = blocks_of(n);
-    k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, joff);
+    k_rj_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, joff, jmin);
     PLH_HIP(hipGetLastError());
     {
         ProfScope ps(ctx, "rot_anti");
-        k_rj_anti<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, fl.job_alive, fa.out());
+        k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jw, fl.job_alive, fk, jmin);
+        PLH_HIP(hipGetLastError());
+        k_rj_select<AS><<<blocks_of(nj), kBlock, 0, st>>>(jw, njobs, jmin, e_kind, e_p, e_psf, e_c, e_s, kstar, adv);
+        PLH_HIP(hipGetLastError());
+        k_rj_anti<AS><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, fk, kstar, fa.out());
+        PLH_HIP(hipGetLastError());
     }
-    PLH_HIP(hipGetLastError());
     PLH_TRY(scan_u32_to_u64(ctx, fa.tcnt, fa.tpre, tiles, fa.tpre + tiles));
     const BitRank A = fa.rank(tiles);
     {
@@ -910,7 +1044,8 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, cons
     const BitRank KO = fko.rank(tiles);
     Post post;
     PLH_TRY(post_prepare(ctx, &post));
-    k_rj_total<<<1, kWave, 0, st>>>(fko.tpre + tiles, fkn.tpre + tiles, d_total, post.mbox, post.seq);
+    k_rj_total<<<1, kBlock, 0, st>>>(fko.tpre + tiles, fkn.tpre + tiles, d_total, adv, njobs, adv_host, post.mbox,
+                                     post.seq);
     PLH_HIP(hipGetLastError());
     uint64_t total = 0;
     PLH_TRY(post_wait(ctx, post, 1, &total));
@@ -1014,6 +1149,74 @@ int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const Jo
     return PLH_OK;
 }
 
+// ----------------------------------------------------------------------
+// A window of rotations on a single sorted sum (propagate's rotation list,
+// a program's run of rotations): the first one that a term anticommutes
+// with is found in one pass, and the rotations before it are skipped.  They
+// are exact no-ops when the sum has passed the same truncation since its
+// keys last changed: a general rotation that no term anticommutes with
+// changes no term, a rotation of kind 0 changes nothing, and the
+// truncation after each finds nothing to drop.
+constexpr int kSumWindow = 16;
+
+template <int AS>
+struct WindowParams {
+    uint64_t px[kSumWindow][AS], pz[kSumWindow][AS];
+    uint8_t kind[kSumWindow];  // 0 no change, 4 general
+    int w;
+};
+
+__global__ void k_la_init(uint32_t* kmin) { *kmin = 0xFFu; }
+
+// The least window index any term anticommutes with (0xFF: none).
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_la_first(Sorted<AS, false> S, uint32_t n, WindowParams<AS> wp,
+                                                     uint32_t* __restrict__ kmin) {
+    __shared__ uint32_t sh[kWavesPerBlock];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t f = 0xFFu;
+    if (i < n) {
+        uint64_t xw[AS], zw[AS];
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            xw[k] = k < S.as ? S.x[(int64_t)k * S.stride + i] : 0ull;
+            zw[k] = k < S.as ? S.z[(int64_t)k * S.stride + i] : 0ull;
+        }
+        for (int e = wp.w - 1; e >= 0; --e) {  // descending: the last hit is the first index
+            if (wp.kind[e] != 4) continue;
+            uint64_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < AS; ++k) acc ^= (xw[k] & wp.pz[e][k]) ^ (zw[k] & wp.px[e][k]);
+            if (popc(acc) & 1) f = (uint32_t)e;
+        }
+    }
+    const uint32_t m = wave_min(f);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = sh[0];
+        for (int k = 1; k < kWavesPerBlock; ++k) b = sh[k] < b ? sh[k] : b;
+        if (b < 0xFFu) atomicMin(kmin, b);
+    }
+}
+
+template <int AS>
+int window_first_sorted(plh_ctx* ctx, const plh_sum* s, const WindowParams<AS>& wp, int* first) {
+    hipStream_t st = ctx->stream;
+    void* p;
+    PLH_TRY(ctx->ws.get(kWsMisc, 256, &p));  // scratch of one launch sequence, as elsewhere
+    uint32_t* kmin = (uint32_t*)p;
+    k_la_init<<<1, 1, 0, st>>>(kmin);
+    Sorted<AS, false> S{s->x, s->z, s->f, s->v, nullptr, s->cap, s->as};
+    k_la_first<AS><<<blocks_of((uint64_t)s->n), kBlock, 0, st>>>(S, (uint32_t)s->n, wp, kmin);
+    PLH_HIP(hipGetLastError());
+    // the count is the low half of the u64 word read back
+    uint64_t v = 0;
+    PLH_TRY(read_u64n(ctx, (const uint64_t*)p, 1, &v));
+    *first = (int)(v & 0xFFFFFFFFull);
+    return PLH_OK;
+}
+
 Filters filters_of(const CollateSpec& spec) {
     return Filters{spec.mode, spec.max_weight, spec.eps, spec.job_eps, spec.job_maxw, spec.job_alive};
 }
@@ -1055,7 +1258,54 @@ int rotate_sorted(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* 
     }
 }
 
-int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, int njobs, const CollateSpec& spec) {
+int rotate_sorted_window(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const* px, const uint64_t* const* pz,
+                         const uint8_t* kind, const uint32_t* psf, const double* c, const double* sn,
+                         const CollateSpec& spec, int* consumed) {
+    if (w < 1 || w > kSumWindow || !s->sorted || s->has_job || s->as > 2) {
+        set_error("rotate_sorted_window: 1..16 rotations on a sorted single sum of at most two words per plane");
+        return PLH_ERR_ARG;
+    }
+    if (s->n > (int64_t)0x7FFFFFFFll) {
+        set_error("rotate_sorted_window: at most 2^31 - 1 terms");
+        return PLH_ERR_SIZE;
+    }
+    if (s->n == 0) {
+        *consumed = w;
+        return PLH_OK;
+    }
+    int first = 0xFF;
+    if (s->as == 1) {
+        WindowParams<1> wp{};
+        wp.w = w;
+        for (int e = 0; e < w; ++e) {
+            wp.px[e][0] = px[e][0];
+            wp.pz[e][0] = pz[e][0];
+            wp.kind[e] = kind[e];
+        }
+        PLH_TRY(window_first_sorted<1>(ctx, s, wp, &first));
+    } else {
+        WindowParams<2> wp{};
+        wp.w = w;
+        for (int e = 0; e < w; ++e) {
+            for (int k = 0; k < 2; ++k) {
+                wp.px[e][k] = px[e][k];
+                wp.pz[e][k] = pz[e][k];
+            }
+            wp.kind[e] = kind[e];
+        }
+        PLH_TRY(window_first_sorted<2>(ctx, s, wp, &first));
+    }
+    if (first >= w) {
+        *consumed = w;
+        return PLH_OK;
+    }
+    PLH_TRY(rotate_sorted(ctx, s, px[first], pz[first], psf[first], c[first], sn[first], spec));
+    *consumed = first + 1;
+    return PLH_OK;
+}
+
+int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, int njobs, const CollateSpec& spec,
+                       uint8_t* adv_host) {
     if (!s->sorted || !s->has_job || (spec.mode & (kFold | kDropZero)) != (kFold | kDropZero)) {
         set_error("rotate_sorted_jobs: needs a sorted canonical batched sum and a canonicalizing spec");
         return PLH_ERR_ARG;
@@ -1064,14 +1314,14 @@ int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobRotTables& jt, int njo
         set_error("rotate_sorted_jobs: at most 2^31 - 1 terms");
         return PLH_ERR_SIZE;
     }
-    if (s->n == 0) return PLH_OK;
-    const Filters fl = filters_of(spec);
-    if (njobs < 1 || !spec.job_alive) {
-        set_error("rotate_sorted_jobs: needs at least one job slot and the job-alive table");
+    if (njobs < 1 || !spec.job_alive || !adv_host) {
+        set_error("rotate_sorted_jobs: needs at least one job slot, the job-alive table and the report buffer");
         return PLH_ERR_ARG;
     }
-    if (s->as == 1) return rotate_sorted_jobs_as<1>(ctx, s, jt, fl, (uint32_t)njobs);
-    if (s->as == 2) return rotate_sorted_jobs_as<2>(ctx, s, jt, fl, (uint32_t)njobs);
+    if (s->n == 0) return PLH_ERR_ARG;  // the caller reports consumed rotations itself for an empty batch
+    const Filters fl = filters_of(spec);
+    if (s->as == 1) return rotate_sorted_jobs_as<1>(ctx, s, jw, fl, (uint32_t)njobs, adv_host);
+    if (s->as == 2) return rotate_sorted_jobs_as<2>(ctx, s, jw, fl, (uint32_t)njobs, adv_host);
     set_error("rotate_sorted_jobs: batched sums support up to 128 qubits");
     return PLH_ERR_AS;
 }

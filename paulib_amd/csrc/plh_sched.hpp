@@ -61,6 +61,15 @@ enum SlotState : int { kFree = 0, kRunning = 1, kDraining = 2 };
 // wait for the next full step with their terms untouched.
 enum StepKind : int { kFullStep = 0, kSortedStep = 1 };
 
+// A sorted step hands each advancing job a window of up to kWindow
+// consecutive order-keeping rotations.  The device applies the first one
+// that changes the job's terms (a general rotation with an anticommuting
+// term) and reports how many rotations the job consumed: the ones before it
+// are exact no-ops, since the job's terms already passed the truncation
+// that follows each of them, and a rotation that no term anticommutes with
+// changes no term.
+constexpr int kWindow = 16;
+
 // One step's device tables (one entry per slot) and its effect on the jobs.
 struct StepPlan {
     std::vector<uint32_t> off, ops;  // ops[off[i] .. off[i+1]) are slot i's Clifford ops
@@ -72,6 +81,13 @@ struct StepPlan {
     std::vector<int64_t> next;    // per slot: the job's next op after this step
     std::vector<int> finishing;   // slots whose job ends with this step
     bool any_general = false;
+    // sorted steps: per slot its window (wn rotations, entries slot * kWindow + k:
+    // rotation kind, ps flags, cos, sin, P as x[AS] then z[AS]); next and
+    // finishing are set by Scheduler::settle once the device has reported
+    // how far each job advanced
+    std::vector<uint8_t> wn, wk, wpsf;
+    std::vector<double> wc, ws;
+    std::vector<uint64_t> wp;
 };
 
 inline bool close_one(double v) { return std::fabs(v - 1.0) <= 1e-14; }  // isCloseOne, :19-21
@@ -175,14 +191,42 @@ class Scheduler {
         p->next.assign(S, -1);
         p->finishing.clear();
         p->any_general = false;
+        const size_t SW = kind == kSortedStep ? S * (size_t)kWindow : 0;
+        p->wn.assign(kind == kSortedStep ? S : 0, 0);
+        p->wk.assign(SW, 0);
+        p->wpsf.assign(SW, 1);
+        p->wc.assign(SW, 0.0);
+        p->ws.assign(SW, 0.0);
+        p->wp.assign(SW * 2 * A, 0);
         for (size_t i = 0; i < S; ++i) {
             p->off[i] = (uint32_t)p->ops.size();
             if (state_[i] != kRunning) continue;
             p->alive[i] = 1;
             const Job& j = active_[i];
             int64_t next = j.next;
-            if (kind == kSortedStep && !sorted_ready((int)i)) {
-                p->next[i] = next;  // waits for the next full step
+            if (kind == kSortedStep) {
+                p->next[i] = next;  // waits for the next full step, or settled after the device
+                if (!sorted_ready((int)i)) continue;
+                // the window: rotations next, next - 1, ... that keep the order
+                int w = 0;
+                const size_t ja = (size_t)j.job_as;
+                for (int64_t l = next; w < kWindow && l >= 0 && j.kind[(size_t)l] == 3; --l, ++w) {
+                    double c, sn;
+                    angle_cos_sin(j.theta[(size_t)l], &c, &sn);
+                    const uint8_t rk = rotation_kind_of(c, sn);
+                    if (rk != 0 && rk != 4) break;
+                    const size_t e = i * (size_t)kWindow + (size_t)w;
+                    p->wk[e] = rk;
+                    p->wc[e] = c;
+                    p->ws[e] = sn;
+                    for (size_t k = 0; k < ja; ++k) {
+                        p->wp[e * 2 * A + k] = j.px[(size_t)l * ja + k];
+                        p->wp[e * 2 * A + A + k] = j.pz[(size_t)l * ja + k];
+                    }
+                }
+                p->wn[i] = (uint8_t)w;
+                p->maxw[i] = j.max_weight;
+                p->eps[i] = j.cutoff;
                 continue;
             }
             while (next >= 0 && j.kind[(size_t)next] != 3) {
@@ -224,6 +268,21 @@ class Scheduler {
             if (next < 0) p->finishing.push_back((int)i);
         }
         p->off[S] = (uint32_t)p->ops.size();
+    }
+
+    // A sorted step's device work reported adv[slot] rotations consumed per
+    // slot (1 .. wn[slot] for an advancing job, 0 otherwise): the jobs' next
+    // ops and the finishing jobs follow.  False when a report is out of range.
+    bool settle(StepPlan* p, const uint8_t* adv) const {
+        p->finishing.clear();
+        for (size_t i = 0; i < (size_t)slots_; ++i) {
+            if (state_[i] != kRunning) continue;
+            const int a = adv[i], w = i < p->wn.size() ? p->wn[i] : 0;
+            if ((w == 0 && a != 0) || (w > 0 && (a < 1 || a > w))) return false;
+            p->next[i] = active_[i].next - a;
+            if (w > 0 && p->next[i] < 0) p->finishing.push_back((int)i);
+        }
+        return true;
     }
 
     // The step's device work is done: draining slots (their terms left in

@@ -271,6 +271,10 @@ struct plh_server {
     void* d_step = nullptr;
     size_t step_cap = 0;
     std::vector<uint8_t> h_step;
+    // per slot, the rotations a sorted step consumed (host-pinned, written
+    // by the device; adv_dev is its device address)
+    uint8_t* adv_host = nullptr;
+    uint8_t* adv_dev = nullptr;
     std::chrono::steady_clock::time_point t0;
     int64_t steps = 0;
     int since_full = 0;  // sorted steps since the last full step
@@ -345,8 +349,9 @@ int admit(plh_server* s, const Job& j, int slot) {
 
 // The device part of a step: table upload, gates, one collation for every
 // job, and the finished jobs' expectations (and terms); their results go
-// to *res.  Nothing of the scheduler changes here.
-int step_device(plh_server* s, const sched::StepPlan& pl, sched::StepKind kind, std::vector<Result>* res) {
+// to *res.  Nothing of the scheduler changes here; a sorted step settles the
+// plan (the jobs' next ops) from the rotations the device consumed.
+int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::vector<Result>* res) {
     plh_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     const int AS = s->as;
@@ -355,11 +360,16 @@ int step_device(plh_server* s, const sched::StepPlan& pl, sched::StepKind kind, 
         const void* src;
         size_t bytes;
     };
-    const Part parts[10] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
-                            {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
-                            {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
-                            {pl.rp.data(), pl.rp.size() * 8},   {pl.eps.data(), pl.eps.size() * 8},
-                            {pl.maxw.data(), pl.maxw.size() * 8}, {pl.alive.data(), pl.alive.size()}};
+    constexpr int kParts = 16;
+    const Part parts[kParts] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
+                                {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
+                                {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
+                                {pl.rp.data(), pl.rp.size() * 8},   {pl.eps.data(), pl.eps.size() * 8},
+                                {pl.maxw.data(), pl.maxw.size() * 8}, {pl.alive.data(), pl.alive.size()},
+                                {pl.wn.data(), pl.wn.size()},       {pl.wk.data(), pl.wk.size()},
+                                {pl.wp.data(), pl.wp.size() * 8},   {pl.wpsf.data(), pl.wpsf.size()},
+                                {pl.wc.data(), pl.wc.size() * 8},   {pl.ws.data(), pl.ws.size() * 8}};
+    static_assert(sched::kWindow == kJobWindow, "scheduler and engine agree on the window depth");
     size_t total = 0;
     for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
     PLH_HIP(hipStreamSynchronize(st));  // the previous step's upload has been consumed
@@ -371,9 +381,9 @@ int step_device(plh_server* s, const sched::StepPlan& pl, sched::StepKind kind, 
         s->step_cap = total * 2;
     }
     s->h_step.assign(total, 0);
-    void* dptr[10];
+    void* dptr[kParts];
     size_t o = 0;
-    for (int i = 0; i < 10; ++i) {
+    for (int i = 0; i < kParts; ++i) {
         if (parts[i].bytes) std::memcpy(s->h_step.data() + o, parts[i].src, parts[i].bytes);
         dptr[i] = (char*)s->d_step + o;
         o += align256(std::max<size_t>(parts[i].bytes, 1));
@@ -402,8 +412,13 @@ int step_device(plh_server* s, const sched::StepPlan& pl, sched::StepKind kind, 
         sc.job_eps = tb.eps;
         sc.job_maxw = tb.maxw;
         sc.job_alive = tb.alive;
-        JobRotTables jt{tb.rot_kind, tb.rot_p, tb.rot_psf, tb.rot_c, tb.rot_s};
-        PLH_TRY(rotate_sorted_jobs(ctx, B, jt, s->sched->slots(), sc));
+        const JobWindows jw{(const uint8_t*)dptr[10], (const uint8_t*)dptr[11], (const uint64_t*)dptr[12],
+                            (const uint8_t*)dptr[13], (const double*)dptr[14], (const double*)dptr[15]};
+        PLH_TRY(rotate_sorted_jobs(ctx, B, jw, s->sched->slots(), sc, s->adv_dev));
+        if (!s->sched->settle(&pl, s->adv_host)) {
+            set_error("plh_server_step: the device reported an out-of-range rotation count");
+            return PLH_ERR_HIP;
+        }
     } else if (n > 0) {
         // ---- gates: Cliffords + fast rotations in place
         const uint64_t nb = blocks_for((uint64_t)n, kTile);
@@ -524,7 +539,14 @@ int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_bu
     s->keep_terms = keep_terms != 0;
     s->sched = new sched::Scheduler(array_size, max_jobs,
                                     hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)(free_b / 2));
-    int rc = plh_sum_create(ctx, array_size, 64 * array_size, 0, &s->batch);
+    int rc = PLH_OK;
+    if (hipHostMalloc((void**)&s->adv_host, (size_t)max_jobs, hipHostMallocCoherent | hipHostMallocMapped) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&s->adv_dev, s->adv_host, 0) != hipSuccess) {
+        set_error("plh_server_create: pinned report buffer");
+        rc = PLH_ERR_HIP;
+    }
+    if (rc == PLH_OK) rc = plh_sum_create(ctx, array_size, 64 * array_size, 0, &s->batch);
     if (rc == PLH_OK) {
         s->batch->has_job = true;
         sum_free_storage(s->batch);
@@ -532,6 +554,7 @@ int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_bu
     }
     if (rc != PLH_OK) {
         if (s->batch) plh_sum_destroy(s->batch);
+        if (s->adv_host) (void)hipHostFree(s->adv_host);
         delete s->sched;
         delete s;
         return rc;
@@ -546,6 +569,7 @@ int plh_server_destroy(plh_server* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     plh_sum_destroy(s->batch);
     (void)hipFree(s->d_step);
+    if (s->adv_host) (void)hipHostFree(s->adv_host);
     delete s->sched;
     delete s;
     return PLH_OK;

@@ -123,6 +123,19 @@ struct JobRotTables {
     const double* s;
 };
 
+// Per-job rotation windows of a sorted server step (device tables by job
+// slot): wn[slot] rotations, entry slot * kJobWindow + k with its rotation
+// kind (0 no change, 4 general), ps flags, cos, sin and P (x[AS] then z[AS]).
+constexpr int kJobWindow = 16;
+struct JobWindows {
+    const uint8_t* wn;
+    const uint8_t* kind;
+    const uint64_t* p;
+    const uint8_t* psf;
+    const double* c;
+    const double* s;
+};
+
 // A sum after evolve's general path, without materialising the new terms.
 // id < n: term id, its value times c when it anticommutes with P; id >= n:
 // the new term (term id - n) * ps with value (-s) * v, valid only when that

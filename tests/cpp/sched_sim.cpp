@@ -28,6 +28,7 @@
 // -fsanitize=address,undefined by tests/test_sched_sim.py.
 #include <cinttypes>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <random>
@@ -175,8 +176,18 @@ struct Replay {
 
     // step_device(): gates, one collation (with_output + collate_run's
     // storage), the finished jobs' ranges.
-    int device(const StepPlan& pl, plh::sched::StepKind kind, std::map<int, int64_t>* after) {
+    int device(StepPlan& pl, plh::sched::StepKind kind, std::map<int, int64_t>* after) {
         const int64_t n = batch.n;
+        if (kind == plh::sched::kSortedStep) {
+            // the device's report: each advancing job consumes 1 .. wn of its
+            // window (here drawn at random: any count is a valid report)
+            std::vector<uint8_t> adv((size_t)sc.slots(), 0);
+            for (int i = 0; i < sc.slots(); ++i)
+                if (pl.wn[(size_t)i] > 0)
+                    adv[(size_t)i] = (uint8_t)(1 + std::uniform_int_distribution<int>(0, pl.wn[(size_t)i] - 1)(rng));
+            if (!sc.settle(&pl, adv.data())) fail("settle refused a valid report");
+            verify_settled(pl, adv);
+        }
         // the terms each slot holds once this step's collation has run
         after->clear();
         int64_t n_valid = n, out = 0;
@@ -194,9 +205,8 @@ struct Replay {
         if (n > 0 && kind == plh::sched::kSortedStep) {
             // plh_rotate.hip rotate_sorted_jobs: scratch, then the output block
             void* p = nullptr;
-            PLH_TRY(ctx.ws.get(plh::kWsRotA, (size_t)n * 4 + 256 + (size_t)(n + 1) * 8, &p));
-            PLH_TRY(ctx.ws.get(plh::kWsRotN, (size_t)n * (16 * sc.as() + 13) + 1024, &p));
-            PLH_TRY(ctx.ws.get(plh::kWsRotM, (size_t)n * 24 + 1024 + (size_t)(2 * n + 1) * 8, &p));
+            PLH_TRY(ctx.ws.get(plh::kWsRotA, (size_t)n + (size_t)n / 2 + 4096 + (size_t)sc.slots() * 64, &p));
+            PLH_TRY(ctx.ws.get(plh::kWsRotN, (size_t)n * (16 * sc.as() + 17) + 1536, &p));
             if (injected()) return device_error();
             sim::check_live(batch.x, (size_t)n * 8, "sorted rotation reads");
             sim::check_live(batch.j, (size_t)n * 4, "sorted rotation reads");
@@ -346,6 +356,24 @@ struct Replay {
         return PLH_OK;
     }
 
+    // After a sorted step's report: each advancing job moved by its count,
+    // waiting jobs stayed, and exactly the jobs that passed op 0 finish.
+    void verify_settled(const StepPlan& pl, const std::vector<uint8_t>& adv) {
+        for (int i = 0; i < sc.slots(); ++i) {
+            if (sc.state(i) != plh::sched::kRunning) continue;
+            const Job& j = sc.job(i);
+            if (pl.next[(size_t)i] != j.next - (int64_t)adv[(size_t)i]) fail("job %lld settled at the wrong op", j.handle);
+            const bool fin = std::find(pl.finishing.begin(), pl.finishing.end(), i) != pl.finishing.end();
+            if (fin != (pl.wn[(size_t)i] > 0 && pl.next[(size_t)i] < 0)) fail("finishing flag of job %lld", j.handle);
+        }
+        std::vector<uint8_t> bad = adv;
+        for (size_t i = 0; i < bad.size(); ++i)
+            if (sc.state((int)i) == plh::sched::kRunning) bad[i] = (uint8_t)(pl.wn[i] + 1);
+        StepPlan copy = pl;
+        if (std::any_of(pl.wn.begin(), pl.wn.end(), [](uint8_t w) { return w > 0; }) && sc.settle(&copy, bad.data()))
+            fail("settle accepted a count past a window");
+    }
+
     // The plan's ops for each running job are its program's next ops, in
     // reverse order, up to and including one rotation.
     void verify_plan(const StepPlan& pl, plh::sched::StepKind kind) {
@@ -360,15 +388,40 @@ struct Replay {
             if (j.next != expect_next) fail("job %lld resumes at op %lld", j.handle, j.next);
             if (kind == plh::sched::kSortedStep) {
                 if (pl.off[(size_t)i] != pl.off[(size_t)i + 1]) fail("Clifford ops in a sorted step (job %lld)", j.handle);
+                if (pl.rk[(size_t)i] != 0 || pl.next[(size_t)i] != j.next || !pl.alive[(size_t)i] || !pl.finishing.empty())
+                    fail("a sorted step's plan decides for job %lld before the device", j.handle);
                 if (!sc.sorted_ready(i)) {  // waits: untouched
-                    if (pl.rk[(size_t)i] != 0 || pl.maxw[(size_t)i] != -1 || pl.eps[(size_t)i] != 0.0 ||
-                        pl.next[(size_t)i] != j.next || !pl.alive[(size_t)i])
+                    if (pl.wn[(size_t)i] != 0 || pl.maxw[(size_t)i] != -1 || pl.eps[(size_t)i] != 0.0)
                         fail("a waiting job is not left alone (job %lld)", j.handle);
-                    if (std::find(pl.finishing.begin(), pl.finishing.end(), i) != pl.finishing.end())
-                        fail("a waiting job finishes (job %lld)", j.handle);
                     continue;
                 }
-                if (pl.rk[(size_t)i] != 0 && pl.rk[(size_t)i] != 4) fail("a key-moving rotation in a sorted step");
+                // the window: the job's next rotations in order, each keeping the
+                // order, up to the window depth or the first op that does not
+                const int wn = pl.wn[(size_t)i];
+                if (wn < 1 || wn > plh::sched::kWindow) fail("window of job %lld holds %lld rotations", j.handle, wn);
+                const size_t A = (size_t)sc.as(), ja = (size_t)j.job_as;
+                int64_t l = j.next;
+                for (int e = 0; e < wn; ++e, --l) {
+                    if (l < 0 || j.kind[(size_t)l] != 3) fail("window entry %lld of job %lld is not a rotation", e, j.handle);
+                    double c, sn;
+                    plh::angle_cos_sin(j.theta[(size_t)l], &c, &sn);
+                    const uint8_t want = plh::sched::rotation_kind_of(c, sn);
+                    const size_t ent = (size_t)i * plh::sched::kWindow + (size_t)e;
+                    if (pl.wk[ent] != want || (want != 0 && want != 4)) fail("window entry %lld of job %lld: kind", e, j.handle);
+                    if (pl.wc[ent] != c || pl.ws[ent] != sn) fail("window entry %lld of job %lld: angle", e, j.handle);
+                    for (size_t k = 0; k < A; ++k)
+                        if (pl.wp[ent * 2 * A + k] != (k < ja ? j.px[(size_t)l * ja + k] : 0ull) ||
+                            pl.wp[ent * 2 * A + A + k] != (k < ja ? j.pz[(size_t)l * ja + k] : 0ull))
+                            fail("window entry %lld of job %lld: generator", e, j.handle);
+                }
+                if (wn < plh::sched::kWindow && l >= 0 && j.kind[(size_t)l] == 3) {
+                    double c, sn;
+                    plh::angle_cos_sin(j.theta[(size_t)l], &c, &sn);
+                    const uint8_t k = plh::sched::rotation_kind_of(c, sn);
+                    if (k == 0 || k == 4) fail("window of job %lld stops before an order-keeping rotation", j.handle);
+                }
+                if (pl.maxw[(size_t)i] != j.max_weight || pl.eps[(size_t)i] != j.cutoff) fail("truncation of job %lld", j.handle);
+                continue;
             }
             int64_t l = j.next;
             for (uint32_t o = pl.off[(size_t)i]; o < pl.off[(size_t)i + 1]; ++o, --l) {
