@@ -585,6 +585,39 @@ __global__ void k_rj_bounds(const uint32_t* __restrict__ J, uint32_t n, uint32_t
     off[k] = lo;
 }
 
+// Word k of an AS-word register array by a (mostly uniform) index.
+template <int AS>
+PLH_DEV uint64_t word_of(const uint64_t* w, int k) {
+    uint64_t v = w[0];
+#pragma unroll
+    for (int i = 1; i < AS; ++i)
+        if (k == i) v = w[i];
+    return v;
+}
+
+// Whether the term (x, z words) anticommutes with a window entry's
+// generator, from its code (the generator's words for a wide one).
+template <int AS>
+PLH_DEV bool code_anti(uint32_t code, const uint64_t* p, const uint64_t* xw, const uint64_t* zw) {
+    if (code & (1u << 30)) {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) acc ^= (xw[k] & p[AS + k]) ^ (zw[k] & p[k]);
+        return (popc(acc) & 1) != 0;
+    }
+    uint32_t par = 0;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint32_t f = code >> (10 * u);
+        const uint32_t t = (f >> 8) & 3u, q = f & 0xFFu;
+        if (!t) continue;
+        const int wd = (int)(q >> 6), b = (int)(q & 63);
+        const uint32_t xb = (uint32_t)(word_of<AS>(xw, wd) >> b) & 1u, zb = (uint32_t)(word_of<AS>(zw, wd) >> b) & 1u;
+        par ^= ((t & 1u) ? zb : 0u) ^ ((t & 2u) ? xb : 0u);  // X tests z, Z tests x, Y both
+    }
+    return par != 0;
+}
+
 // The first rotation of job's window that the term (x, z words) anticommutes
 // with, 255 when none; entries of kind 0 (no change) never count.
 template <int AS>
@@ -593,12 +626,9 @@ PLH_DEV uint32_t window_first(const JobWindows& jw, uint32_t job, const uint64_t
     uint32_t f = 0xFFu;
     for (int e = w - 1; e >= 0; --e) {  // descending: the last hit is the first index
         const size_t ent = (size_t)job * kJobWindow + (size_t)e;
-        if (jw.kind[ent] != 4) continue;
-        const uint64_t* p = jw.p + ent * 2 * AS;
-        uint64_t acc = 0;
-#pragma unroll
-        for (int k = 0; k < AS; ++k) acc ^= (xw[k] & p[AS + k]) ^ (zw[k] & p[k]);
-        if (popc(acc) & 1) f = (uint32_t)e;
+        const uint32_t code = jw.code[ent];
+        if (!(code >> 31)) continue;
+        if (code_anti<AS>(code, jw.p + ent * 2 * AS, xw, zw)) f = (uint32_t)e;
     }
     return f;
 }
@@ -608,57 +638,106 @@ PLH_DEV T wave_min(T v) {
     return wave_last(wave_scan_op(v, (T)~T(0), [](T a, T b) { return a < b ? a : b; }));
 }
 
+// Whether some term whose x and z words have ux and uz as their OR could
+// anticommute with a window entry (a scalar test of its code).
+template <int AS>
+PLH_DEV bool code_may_hit(uint32_t code, const uint64_t* ux, const uint64_t* uz) {
+    if (code & (1u << 30)) return true;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint32_t f = code >> (10 * u);
+        const uint32_t t = (f >> 8) & 3u, q = f & 0xFFu;
+        if (!t) continue;
+        const int wd = (int)(q >> 6), b = (int)(q & 63);
+        const uint64_t xb = (word_of<AS>(ux, wd) >> b) & 1ull, zb = (word_of<AS>(uz, wd) >> b) & 1ull;
+        if (((t & 1u) && zb) || ((t & 2u) && xb)) return true;
+    }
+    return false;
+}
+
 // ---- 1. per term the first rotation of its job's window it anticommutes
-// with (fk, one byte), and per job the least of them (jmin).  A wave of 64
-// consecutive terms of one job (the common case) reads its window through
-// scalar loads and only the words the window touches.
+// with (fk, one byte), and per job the least of them (jmin).  A block stages
+// the windows of the jobs its tile spans (at most kStageJobs) in LDS, and
+// issues every term's loads before the tests.  A wave of one job first ORs
+// its terms' words: an entry whose qubits none of them touches in an
+// anticommuting way is skipped by a scalar test (most rotations of a window
+// lie outside a low-weight observable's support).
+constexpr int kStageJobs = 8;
+
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_t n, JobWindows jw,
                                                      const uint8_t* __restrict__ alive, uint8_t* __restrict__ fk,
                                                      uint32_t* __restrict__ jmin) {
+    __shared__ uint32_t s_code[kStageJobs * kJobWindow];
+    __shared__ uint32_t s_wn[kStageJobs];
+    __shared__ uint32_t s_j0, s_nj;
     const uint64_t tile = blockIdx.x;
-#pragma unroll 1
+    const uint64_t i0 = tile * kRjTile;
+    const uint64_t i1 = (i0 + kRjTile < (uint64_t)n ? i0 + kRjTile : (uint64_t)n) - 1;
+    if (threadIdx.x == 0) {
+        s_j0 = S.j[i0];
+        s_nj = S.j[i1] - s_j0 + 1;
+    }
+    __syncthreads();
+    const uint32_t j0 = s_j0, nj = s_nj;
+    const bool staged = nj <= (uint32_t)kStageJobs;
+    if (staged) {
+        for (uint32_t t = threadIdx.x; t < nj * kJobWindow; t += kBlock) {
+            const uint32_t jb = j0 + t / kJobWindow;
+            s_code[t] = alive[jb] ? jw.code[(size_t)jb * kJobWindow + t % kJobWindow] : 0u;
+        }
+        if (threadIdx.x < nj) s_wn[threadIdx.x] = alive[j0 + threadIdx.x] ? jw.wn[j0 + threadIdx.x] : 0u;
+    }
+    __syncthreads();
+    uint32_t job[4];
+    uint64_t xw[4][AS], zw[4][AS];
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint64_t i = rj_item(tile, k);
-        if (rj_item(tile, k) - lane_id() >= n) break;  // the word's first term: uniform
+        job[k] = i < n ? S.j[i] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int w = 0; w < AS; ++w) {
+            xw[k][w] = i < n ? S.x[(int64_t)w * S.stride + i] : 0ull;
+            zw[k][w] = i < n ? S.z[(int64_t)w * S.stride + i] : 0ull;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = rj_item(tile, k);
         const bool valid = i < n;
-        const uint32_t job = valid ? S.j[i] : 0xFFFFFFFFu;
-        const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);  // lane 0 is valid
-        const bool uni = ballot(valid && job != j0) == 0ull;
         uint32_t f = 0xFFu;
-        if (uni) {
-            if (alive[j0] && jw.wn[j0] > 0) {
-                // the words the window's rotations read: x where P has z, z where P has x
-                uint64_t nx[AS], nz[AS];
-#pragma unroll
-                for (int w = 0; w < AS; ++w) nx[w] = nz[w] = 0;
-                for (int e = 0; e < (int)jw.wn[j0]; ++e) {
-                    const uint64_t* p = jw.p + ((size_t)j0 * kJobWindow + (size_t)e) * 2 * AS;
-#pragma unroll
-                    for (int w = 0; w < AS; ++w) {
-                        nx[w] |= p[AS + w];
-                        nz[w] |= p[w];
-                    }
-                }
-                uint64_t xw[AS], zw[AS];
+        const uint32_t jf = (uint32_t)__builtin_amdgcn_readfirstlane((int)job[k]);
+        if (staged && ballot(valid && job[k] != jf) == 0ull) {
+            if (jf != 0xFFFFFFFFu) {
+                uint64_t ux[AS], uz[AS];
 #pragma unroll
                 for (int w = 0; w < AS; ++w) {
-                    xw[w] = (nx[w] && valid) ? S.x[(int64_t)w * S.stride + i] : 0ull;
-                    zw[w] = (nz[w] && valid) ? S.z[(int64_t)w * S.stride + i] : 0ull;
+                    ux[w] = wave_or(xw[k][w]);
+                    uz[w] = wave_or(zw[k][w]);
                 }
-                if (valid) f = window_first<AS>(jw, j0, xw, zw);
+                const uint32_t* codes = s_code + (jf - j0) * kJobWindow;
+                const int wn = (int)s_wn[jf - j0];
+                for (int e = wn - 1; e >= 0; --e) {  // descending: the last hit is the first index
+                    const uint32_t code = (uint32_t)__builtin_amdgcn_readfirstlane((int)codes[e]);
+                    if (!(code >> 31) || !code_may_hit<AS>(code, ux, uz)) continue;
+                    const bool hit = code_anti<AS>(code, jw.p + ((size_t)jf * kJobWindow + (size_t)e) * 2 * AS,
+                                                   xw[k], zw[k]);
+                    if (hit && valid) f = (uint32_t)e;
+                }
+                const uint32_t m = wave_min(f);
+                if (lane_id() == 0 && m < 0xFFu) atomicMin(&jmin[jf], m);
             }
-            const uint32_t m = wave_min(f);
-            if (lane_id() == 0 && m < 0xFFu) atomicMin(&jmin[j0], m);
-        } else if (valid && alive[job] && jw.wn[job] > 0) {
-            uint64_t xw[AS], zw[AS];
-#pragma unroll
-            for (int w = 0; w < AS; ++w) {
-                xw[w] = S.x[(int64_t)w * S.stride + i];
-                zw[w] = S.z[(int64_t)w * S.stride + i];
+        } else if (valid) {
+            // a wave that spans jobs (or a tile of many small jobs): per lane
+            const uint32_t jb = job[k];
+            const int wn = alive[jb] ? (int)jw.wn[jb] : 0;
+            for (int e = wn - 1; e >= 0; --e) {
+                const size_t ent = (size_t)jb * kJobWindow + (size_t)e;
+                const uint32_t code = jw.code[ent];
+                if (!(code >> 31)) continue;
+                if (code_anti<AS>(code, jw.p + ent * 2 * AS, xw[k], zw[k])) f = (uint32_t)e;
             }
-            f = window_first<AS>(jw, job, xw, zw);
-            if (f < 0xFFu) atomicMin(&jmin[job], f);
+            if (f < 0xFFu) atomicMin(&jmin[jb], f);
         }
         if (valid) fk[i] = (uint8_t)f;
     }
@@ -826,25 +905,27 @@ PLH_DEV bool rj_keep_old(const Filters& fl, uint32_t job, double acc) {
     return true;
 }
 
-// The new terms whose lower bounds fall in old tile [i0, i0 + 1024).
-PLH_DEV void rj_tile_range(const uint32_t* lb, uint32_t m, uint64_t i0, uint32_t* rr) {
-    if (threadIdx.x == 0) {
-        rr[0] = u32_bound(lb, 0, m, (uint32_t)i0, false);
-        const uint64_t e = i0 + kRjTile;
-        rr[1] = u32_bound(lb, rr[0], m, e > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)e, false);
-    }
-    __syncthreads();
+// The first new term of each old tile: tr0[t] = the first r whose lower
+// bound lies in tile t or later (t <= tiles; tr0[tiles] = m).  Thread r
+// fills the tiles between its predecessor's lower bound and its own.
+__global__ void k_rj_tiles(const uint32_t* __restrict__ lb, BitRank A, uint64_t tiles, uint32_t n,
+                           uint32_t* __restrict__ tr0) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t m = A.total();
+    if (r > m || r > n) return;
+    const int64_t prev = r > 0 ? (int64_t)(lb[r - 1] / kRjTile) : -1;
+    const int64_t cur = r < m ? (int64_t)(lb[r] / kRjTile) : (int64_t)tiles;
+    for (int64_t t = prev + 1; t <= cur; ++t) tr0[t] = (uint32_t)r;
 }
 
 // ---- 4. old terms' keep flags (KO)
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
-                                                    const uint32_t* __restrict__ lb, BitRank D, NewTerms N, Filters fl,
-                                                    BitOut koout) {
+                                                    const uint32_t* __restrict__ lb, const uint32_t* __restrict__ tr0,
+                                                    BitRank D, NewTerms N, Filters fl, BitOut koout) {
     __shared__ uint32_t sh[kRjWords];
-    __shared__ uint32_t rr[2];
     const uint64_t tile = blockIdx.x;
-    rj_tile_range(lb, (uint32_t)A.total(), tile * kRjTile, rr);
+    const uint32_t rr[2] = {tr0[tile], tr0[tile + 1]};
     bool ko[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -885,17 +966,17 @@ __global__ __launch_bounds__(kBlock) void k_rj_total(const uint64_t* __restrict_
 // [0, tiles) take old tiles, [tiles, 2 tiles) new ones.
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_t n, uint64_t tiles, JobRotTables jt,
-                                                     BitRank A, const uint32_t* __restrict__ lb, BitRank D, BitRank KO,
+                                                     BitRank A, const uint32_t* __restrict__ lb,
+                                                     const uint32_t* __restrict__ tr0, BitRank D, BitRank KO,
                                                      BitRank KN, NewTerms N, const uint64_t* __restrict__ d_total,
                                                      uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
                                                      uint8_t* __restrict__ of, double* __restrict__ ov,
                                                      uint32_t* __restrict__ oj, int64_t os) {
-    __shared__ uint32_t rr[2];
     const uint64_t total = *d_total;
     const uint32_t m = (uint32_t)A.total();
     if (blockIdx.x < tiles) {
         const uint64_t tile = blockIdx.x;
-        rj_tile_range(lb, m, tile * kRjTile, rr);
+        const uint32_t rr[2] = {tr0[tile], tr0[tile + 1]};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint64_t i = rj_item(tile, k);
@@ -973,7 +1054,7 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const size_t job_bytes = up256((nj + 1) * 4) + up256(nj * 4) + up256(nj) + up256(nj * 2 * AS * 8) + up256(nj) +
                              2 * up256(nj * 8) + up256(nj) + up256(nj) + up256(16);
     void* p;
-    PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n), &p));
+    PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n) + up256((tiles + 1) * 4), &p));
     char* c = (char*)p;
     FlagBuf fa, fd, fkn, fko;
     fa.carve(c, tiles);
@@ -1001,6 +1082,8 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     uint64_t* d_total = (uint64_t*)c;
     c += up256(16);
     uint8_t* fk = (uint8_t*)c;
+    c += up256(n);
+    uint32_t* tr0 = (uint32_t*)c;
     const JobRotTables jt{e_kind, e_p, e_psf, e_c, e_s};
     const size_t nb_planes = up256(n * 8 * AS);
     PLH_TRY(ctx->ws.get(kWsRotN, 2 * nb_planes + up256(n * 8) + up256(n) + up256(n * 4) + up256(n * 4), &p));
@@ -1032,12 +1115,14 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
         PLH_HIP(hipGetLastError());
         k_rj_lb<AS><<<gt, kBlock, 0, st>>>(S, A, joff, N, fl, lb, fd.out(false), fkn.out());
         PLH_HIP(hipGetLastError());
+        k_rj_tiles<<<blocks_of(n + 1), kBlock, 0, st>>>(lb, A, tiles, (uint32_t)n, tr0);
+        PLH_HIP(hipGetLastError());
     }
     PLH_TRY(scan_u32_to_u64(ctx, fkn.tcnt, fkn.tpre, tiles, fkn.tpre + tiles));
     const BitRank D = fd.rank(tiles), KN = fkn.rank(tiles);
     {
         ProfScope ps(ctx, "rot_merge");
-        k_rj_keep<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, lb, D, N, fl, fko.out());
+        k_rj_keep<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, lb, tr0, D, N, fl, fko.out());
         PLH_HIP(hipGetLastError());
     }
     PLH_TRY(scan_u32_to_u64(ctx, fko.tcnt, fko.tpre, tiles, fko.tpre + tiles));
@@ -1057,7 +1142,7 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(total, 1), 0));
     if (total) {
         ProfScope ps(ctx, "rot_write", (double)total * (16.0 * s->as + 13.0));
-        k_rj_write<AS><<<2 * gt, kBlock, 0, st>>>(S, (uint32_t)n, tiles, jt, A, lb, D, KO, KN, N, d_total, tmp.x,
+        k_rj_write<AS><<<2 * gt, kBlock, 0, st>>>(S, (uint32_t)n, tiles, jt, A, lb, tr0, D, KO, KN, N, d_total, tmp.x,
                                                   tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) {

@@ -1,13 +1,17 @@
 // plh_serve.hip -- batched propagation: many jobs in one device-resident sum.
 //
 // A batched sum is a plh_sum with a job plane: term t belongs to job slot
-// j[t].  The slot is the leading key word of the collation, so one
-// collation per step keeps every job's terms contiguous and ordered exactly
-// as that job alone would be, under its own cutoff and weight limit.  One
-// step advances every job in flight by one rotation (and the Clifford gates
-// before it, in Heisenberg order) with a fixed number of launches, so the
-// per-gate launch and synchronisation cost of the plain path is paid once
-// per step for all jobs instead of once per job.
+// j[t].  The slot is the leading key word of every collation, so the batch
+// keeps each job's terms contiguous and ordered exactly as that job alone
+// would be, under its own cutoff and weight limit.  A full step advances
+// every job in flight by its Clifford gates up to and including its next
+// rotation (Heisenberg order) and collates the batch; a sorted step keeps
+// the order and advances each job whose next op is an order-keeping
+// rotation through a window of up to 16 of them without a sort (the first
+// one that changes a term is applied, the ones before it are no-ops).
+// Either takes a fixed number of launches, so the per-gate launch and
+// synchronisation cost of the plain path is paid once per step for all jobs
+// instead of once per job.
 //
 // Per job the semantics are those of a Clifford + rotation program run by
 // propagate() (include/PauliPropagation.h:53-63) with the Clifford methods
@@ -42,7 +46,7 @@ constexpr int kIpt = 16;
 constexpr uint64_t kTile = (uint64_t)kBlock * kIpt;
 // Sorted steps between full steps at most: a job waiting at a Clifford op
 // (or admitted) waits at most this many steps.
-constexpr int kMaxSortedSteps = 24;
+constexpr int kMaxSortedSteps = 48;
 
 // Per-step job tables (device, one upload per step), one entry per slot.
 struct StepTables {
@@ -360,7 +364,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
         const void* src;
         size_t bytes;
     };
-    constexpr int kParts = 16;
+    constexpr int kParts = 17;
     const Part parts[kParts] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
                                 {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
                                 {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
@@ -368,7 +372,8 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
                                 {pl.maxw.data(), pl.maxw.size() * 8}, {pl.alive.data(), pl.alive.size()},
                                 {pl.wn.data(), pl.wn.size()},       {pl.wk.data(), pl.wk.size()},
                                 {pl.wp.data(), pl.wp.size() * 8},   {pl.wpsf.data(), pl.wpsf.size()},
-                                {pl.wc.data(), pl.wc.size() * 8},   {pl.ws.data(), pl.ws.size() * 8}};
+                                {pl.wc.data(), pl.wc.size() * 8},   {pl.ws.data(), pl.ws.size() * 8},
+                                {pl.wcode.data(), pl.wcode.size() * 4}};
     static_assert(sched::kWindow == kJobWindow, "scheduler and engine agree on the window depth");
     size_t total = 0;
     for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
@@ -413,7 +418,8 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
         sc.job_maxw = tb.maxw;
         sc.job_alive = tb.alive;
         const JobWindows jw{(const uint8_t*)dptr[10], (const uint8_t*)dptr[11], (const uint64_t*)dptr[12],
-                            (const uint8_t*)dptr[13], (const double*)dptr[14], (const double*)dptr[15]};
+                            (const uint8_t*)dptr[13], (const double*)dptr[14], (const double*)dptr[15],
+                            (const uint32_t*)dptr[16]};
         PLH_TRY(rotate_sorted_jobs(ctx, B, jw, s->sched->slots(), sc, s->adv_dev));
         if (!s->sched->settle(&pl, s->adv_host)) {
             set_error("plh_server_step: the device reported an out-of-range rotation count");
