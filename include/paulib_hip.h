@@ -344,13 +344,16 @@ int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_bu
 int plh_server_destroy(plh_server* s);
 /* Queue a job: observable (n terms, planes x[k*n+t] for k < ceil(nq/64),
  * flags, values; its (x, z, imag) keys must be distinct, as after
- * canonicalize) and a program as for plh_run_program.  max_weight < 0 and
- * cutoff <= 0 disable those truncations. */
+ * canonicalize) and a program as for plh_run_program.  max_weight < 0,
+ * cutoff <= 0 and max_terms < 0 disable those truncations (TruncationOptions,
+ * include/PauliPropagation.h:26-47: after each rotation the job is
+ * canonicalized, filtered by weight and cutoff, then trimmed to its max_terms
+ * largest |c| in term order, equal magnitudes lower index first). */
 int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_t* z,
                       const uint8_t* flags, const double* values, int64_t num_qubits, int64_t nops,
                       const int32_t* kind, const int64_t* q0, const int64_t* q1,
                       const uint64_t* px, const uint64_t* pz, const double* theta,
-                      int64_t max_weight, double cutoff, int64_t* handle);
+                      int64_t max_weight, double cutoff, int64_t max_terms, int64_t* handle);
 /* One step: admit, advance every running job, finish jobs whose program is
  * done.  Returns the running and queued job counts.  A job that cannot be
  * admitted (its upload fails) stays queued and the call returns the error;

@@ -108,7 +108,7 @@ SIGNATURES = {
     "plh_server_create": (_int, [_p, _int, _int, _i64, _int, C.POINTER(_p)]),
     "plh_server_destroy": (_int, [_p]),
     "plh_server_submit": (_int, [_p, _i64, _u64p, _u64p, _u8p, _f64p, _i64, _i64, _i32p, _i64p, _i64p,
-                                 _u64p, _u64p, _f64p, _i64, C.c_double, _i64p]),
+                                 _u64p, _u64p, _f64p, _i64, C.c_double, _i64, _i64p]),
     "plh_server_step": (_int, [_p, C.POINTER(_int), C.POINTER(_int)]),
     "plh_server_poll": (_int, [_p, _int, _i64p, _f64p, _i64p, _f64p, C.POINTER(_int)]),
     "plh_server_take_terms": (_int, [_p, _i64, _u64p, _u64p, _u8p, _f64p]),

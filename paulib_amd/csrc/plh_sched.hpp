@@ -48,6 +48,7 @@ struct Job {
     int64_t next = -1;  // next op to apply (Heisenberg order: last to first)
     int64_t max_weight = -1;
     double cutoff = 0.0;
+    int64_t max_terms = -1;  // trim after each truncation (< 0: none)
     double t_submit = 0.0;
 };
 
@@ -76,6 +77,8 @@ struct StepPlan {
     std::vector<uint8_t> rk, rpsf, alive;
     std::vector<double> rc, rs, eps;
     std::vector<int64_t> maxw;
+    std::vector<int64_t> maxt;    // per slot: max_terms when its step truncates and it trims, else -1
+    bool any_trim = false;
     std::vector<uint64_t> rp;     // per slot: x[AS] then z[AS] of its rotation
     std::vector<uint64_t> p_or;   // OR of the general-path rotations' words (x then z)
     std::vector<int64_t> next;    // per slot: the job's next op after this step
@@ -211,6 +214,8 @@ class Scheduler {
         p->rs.assign(S, 0.0);
         p->eps.assign(S, 0.0);
         p->maxw.assign(S, -1);
+        p->maxt.assign(S, -1);
+        p->any_trim = false;
         p->rp.assign(S * 2 * A, 0);
         p->p_or.assign(2 * A, 0);
         p->next.assign(S, -1);
@@ -254,6 +259,8 @@ class Scheduler {
                 p->wn[i] = (uint8_t)w;
                 p->maxw[i] = j.max_weight;
                 p->eps[i] = j.cutoff;
+                p->maxt[i] = j.max_terms;
+                p->any_trim |= j.max_terms >= 0;
                 continue;
             }
             while (next >= 0 && j.kind[(size_t)next] != 3) {
@@ -290,6 +297,8 @@ class Scheduler {
                 }
                 p->maxw[i] = j.max_weight;  // apply_truncation follows every rotation
                 p->eps[i] = j.cutoff;
+                p->maxt[i] = j.max_terms;
+                p->any_trim |= j.max_terms >= 0;
             }
             p->next[i] = next;
             if (next < 0) p->finishing.push_back((int)i);

@@ -308,6 +308,199 @@ bool keys_distinct(const Job& j) {
     return std::adjacent_find(keys.begin(), keys.end()) == keys.end();
 }
 
+// ---- per-job trim (apply_truncation's last step, include/PauliPropagation.h:
+// 43-46; PauliSum::trim, include/PauliSumUtils.h:330-343): a job whose step
+// truncated and that holds more than its max_terms keeps its max_terms terms
+// of largest |c|, in term order; among equal magnitudes the lower index
+// first (as the engine's single-sum trim).  One workgroup per job selects
+// the max_terms-th largest |c| by its IEEE bits (non-negative doubles order
+// as their bits), eight 8-bit digits, then finds the tie cut in term order.
+constexpr int kTrimThreads = 1024;
+
+__global__ void k_job_offsets(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs,
+                              uint32_t* __restrict__ off) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > njobs) return;
+    uint32_t lo = 0, cnt = n;
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = lo + half;
+        if (J[t] < k) {
+            lo = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    off[k] = lo;
+}
+
+PLH_DEV uint64_t mag_bits(double v) { return (uint64_t)__double_as_longlong(fabs(v)); }
+
+__global__ __launch_bounds__(kTrimThreads) void k_job_trim_select(const double* __restrict__ v,
+                                                                  const uint32_t* __restrict__ joff,
+                                                                  const int64_t* __restrict__ maxt,
+                                                                  uint8_t* __restrict__ trim,
+                                                                  uint64_t* __restrict__ thr,
+                                                                  uint32_t* __restrict__ cut) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t s_prefix, s_mask, s_need;
+    __shared__ uint32_t red[kTrimThreads / kWave];
+    __shared__ uint32_t s_cut;
+    const uint32_t job = blockIdx.x, tid = threadIdx.x;
+    const uint32_t a = joff[job], b = joff[job + 1];
+    const int64_t m = maxt[job];
+    if (m < 0 || (int64_t)(b - a) <= m) {
+        if (tid == 0) trim[job] = 0;
+        return;
+    }
+    if (tid == 0) {
+        s_prefix = 0;
+        s_mask = 0;
+        s_need = (uint64_t)m;  // the m-th largest magnitude (m >= 0; m == 0 keeps nothing)
+    }
+    __syncthreads();
+    if (m > 0) {
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            for (uint32_t d = tid; d < 256; d += kTrimThreads) hist[d] = 0;
+            __syncthreads();
+            const uint64_t prefix = s_prefix, mask = s_mask;
+            for (uint32_t t = a + tid; t < b; t += kTrimThreads) {
+                const uint64_t key = mag_bits(v[t]);
+                if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                uint64_t need = s_need, cum = 0;
+                int d = 255;
+                for (; d > 0; --d) {
+                    if (cum + hist[d] >= need) break;
+                    cum += hist[d];
+                }
+                s_need = need - cum;
+                s_prefix = prefix | ((uint64_t)d << shift);
+                s_mask = mask | (255ull << shift);
+            }
+            __syncthreads();
+        }
+    }
+    // ties: the s_need lowest-index terms of magnitude T are kept
+    const uint64_t T = s_prefix;
+    const uint64_t need = m > 0 ? s_need : 0;
+    if (tid == 0) s_cut = a;
+    __syncthreads();
+    if (need > 0) {
+        uint32_t run = 0;
+        for (uint32_t c0 = a; c0 < b; c0 += kTrimThreads) {
+            const uint32_t t = c0 + tid;
+            const uint32_t eq = (t < b && mag_bits(v[t]) == T) ? 1u : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_exclusive_scan<kTrimThreads / kWave>(eq, tot, red);
+            if (eq && run + ex + 1 == need) s_cut = t + 1;
+            run += tot;
+            __syncthreads();
+            if (run >= need) break;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        trim[job] = 1;
+        thr[job] = m > 0 ? T : ~0ull;  // m == 0: nothing passes
+        cut[job] = s_cut;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_job_trim_flags(const double* __restrict__ v,
+                                                           const uint32_t* __restrict__ J, uint32_t n,
+                                                           const uint8_t* __restrict__ trim,
+                                                           const uint64_t* __restrict__ thr,
+                                                           const uint32_t* __restrict__ cut,
+                                                           uint32_t* __restrict__ keep) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t job = J[t];
+    bool k = true;
+    if (trim[job]) {
+        const uint64_t key = mag_bits(v[t]), T = thr[job];
+        k = T != ~0ull && (key > T || (key == T && t < cut[job]));
+    }
+    keep[t] = k ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_job_trim_write(SumView s, const uint32_t* __restrict__ J, int as,
+                                                           uint32_t n, const uint32_t* __restrict__ keep,
+                                                           const uint64_t* __restrict__ pos, uint64_t* __restrict__ ox,
+                                                           uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
+                                                           double* __restrict__ ov, uint32_t* __restrict__ oj,
+                                                           int64_t os) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n || !keep[t]) return;
+    const uint64_t o = pos[t];
+    for (int w = 0; w < as; ++w) {
+        ox[(int64_t)w * os + o] = s.x[(int64_t)w * s.stride + t];
+        oz[(int64_t)w * os + o] = s.z[(int64_t)w * s.stride + t];
+    }
+    of[o] = s.f[t];
+    ov[o] = s.v[t];
+    oj[o] = J[t];
+}
+
+// Trims the jobs of the batch whose entry of maxt is set and that hold more
+// terms than it (the batch stays job-ordered and sorted).
+int trim_jobs(plh_ctx* ctx, plh_sum* B, const int64_t* d_maxt, uint32_t slots) {
+    hipStream_t st = ctx->stream;
+    const uint64_t n = (uint64_t)B->n;
+    if (n == 0) return PLH_OK;
+    if (n > 0xFFFFFFFFull) {
+        set_error("plh_server_step: batch too large to trim");
+        return PLH_ERR_SIZE;
+    }
+    void* p;
+    const size_t sj = ((size_t)slots + 1 + 63) & ~(size_t)63;
+    PLH_TRY(ctx->ws.get(kWsKeep, sj * (4 + 1 + 8 + 4) + 256 + ((n * 4 + 255) & ~(size_t)255) + (n + 1) * 8, &p));
+    char* c = (char*)p;
+    uint32_t* joff = (uint32_t*)c;
+    c += sj * 4;
+    uint8_t* trim = (uint8_t*)c;
+    c += sj;
+    uint64_t* thr = (uint64_t*)(((uintptr_t)c + 7) & ~(uintptr_t)7);
+    c = (char*)(thr + sj);
+    uint32_t* cut = (uint32_t*)c;
+    c += sj * 4 + 256;
+    uint32_t* keep = (uint32_t*)(((uintptr_t)c + 255) & ~(uintptr_t)255);
+    uint64_t* pos = (uint64_t*)((char*)keep + ((n * 4 + 255) & ~(size_t)255));
+    k_job_offsets<<<blocks_for((uint64_t)slots + 1, 256), 256, 0, st>>>(B->j, (uint32_t)n, slots, joff);
+    k_job_trim_select<<<dim3(slots), kTrimThreads, 0, st>>>(B->v, joff, d_maxt, trim, thr, cut);
+    k_job_trim_flags<<<blocks_for(n, kBlock), kBlock, 0, st>>>(B->v, B->j, (uint32_t)n, trim, thr, cut, keep);
+    PLH_HIP(hipGetLastError());
+    uint64_t total = 0;
+    PLH_TRY(scan_u32_to_u64_read(ctx, keep, pos, n, pos + n, &total));
+    if (total == n) return PLH_OK;
+    plh_sum tmp;
+    tmp.ctx = ctx;
+    tmp.as = B->as;
+    tmp.nq = B->nq;
+    tmp.has_job = true;
+    PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(total, 1), 0));
+    if (total) {
+        k_job_trim_write<<<blocks_for(n, kBlock), kBlock, 0, st>>>(SumView{B->x, B->z, B->f, B->v, B->cap}, B->j,
+                                                                    B->as, (uint32_t)n, keep, pos, tmp.x, tmp.z,
+                                                                    tmp.f, tmp.v, tmp.j, tmp.cap);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            sum_free_storage(&tmp);
+            PLH_HIP(e);
+        }
+    }
+    const int64_t nq = B->nq;
+    const bool sorted = B->sorted, canonical = B->canonical;
+    sum_take_storage(B, &tmp);
+    B->n = (int64_t)total;
+    B->nq = nq;
+    B->sorted = sorted;
+    B->canonical = canonical;
+    return PLH_OK;
+}
+
 // Append a job's observable to the batch under its slot.  On failure the
 // batch holds what it held before (its storage may have grown).
 int admit(plh_server* s, const Job& j, int slot) {
@@ -364,7 +557,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
         const void* src;
         size_t bytes;
     };
-    constexpr int kParts = 17;
+    constexpr int kParts = 18;
     const Part parts[kParts] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
                                 {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
                                 {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
@@ -373,7 +566,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
                                 {pl.wn.data(), pl.wn.size()},       {pl.wk.data(), pl.wk.size()},
                                 {pl.wp.data(), pl.wp.size() * 8},   {pl.wpsf.data(), pl.wpsf.size()},
                                 {pl.wc.data(), pl.wc.size() * 8},   {pl.ws.data(), pl.ws.size() * 8},
-                                {pl.wcode.data(), pl.wcode.size() * 4}};
+                                {pl.wcode.data(), pl.wcode.size() * 4}, {pl.maxt.data(), pl.maxt.size() * 8}};
     static_assert(sched::kWindow == kJobWindow, "scheduler and engine agree on the window depth");
     size_t total = 0;
     for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
@@ -452,6 +645,9 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
             PLH_TRY(collate_terms(ctx, B, sc, B));
         }
     }
+
+    // ---- apply_truncation's trim for the jobs that have max_terms
+    if (pl.any_trim && B->n > 0) PLH_TRY(trim_jobs(ctx, B, (const int64_t*)dptr[17], (uint32_t)s->sched->slots()));
 
     // ---- finished jobs: expectation on the job-ordered batch
     if (pl.finishing.empty()) return PLH_OK;
@@ -584,7 +780,7 @@ int plh_server_destroy(plh_server* s) {
 int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_t* z, const uint8_t* f,
                       const double* v, int64_t num_qubits, int64_t nops, const int32_t* kind, const int64_t* q0,
                       const int64_t* q1, const uint64_t* px, const uint64_t* pz, const double* theta,
-                      int64_t max_weight, double cutoff, int64_t* handle) {
+                      int64_t max_weight, double cutoff, int64_t max_terms, int64_t* handle) {
     if (!s || n < 0 || nops < 0 || !handle || (n > 0 && (!x || !z || !f || !v)) ||
         (nops > 0 && (!kind || !q0 || !q1 || !px || !pz || !theta))) {
         set_error("plh_server_submit: null pointer or negative size");
@@ -637,6 +833,7 @@ int plh_server_submit(plh_server* s, int64_t n, const uint64_t* x, const uint64_
     j.next = nops - 1;
     j.max_weight = max_weight;
     j.cutoff = cutoff;
+    j.max_terms = max_terms;
     j.handle = s->next_handle++;
     j.t_submit = now_s(s);
     *handle = j.handle;

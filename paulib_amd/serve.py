@@ -63,8 +63,6 @@ class Server:
     def submit(self, O: HostSum, prog, opts: TruncationOptions | None = None) -> int:
         """Queue a job; O must have distinct (x, z, imag) keys (canonical)."""
         opts = opts or TruncationOptions()
-        if opts.max_terms is not None:
-            raise ValueError("max_terms truncation is not supported")
         nq = int(O.num_qubits)
         a = words_for(nq)
         if prog.num_qubits != nq or prog.array_size != a:
@@ -81,10 +79,10 @@ class Server:
         px = np.ascontiguousarray(prog.px, dtype=np.uint64)
         pz = np.ascontiguousarray(prog.pz, dtype=np.uint64)
         th = np.ascontiguousarray(prog.theta, dtype=np.float64)
-        mw, cut, _ = opts.args()
+        mw, cut, mt = opts.args()
         h = C.c_int64()
         _lib.call("plh_server_submit", self._h, len(f), _u64(x), _u64(z), _u8(f), _f64(v), nq, len(kind),
-                  _i32(kind), _i64(q0), _i64(q1), _u64(px), _u64(pz), _f64(th), mw, cut, C.byref(h))
+                  _i32(kind), _i64(q0), _i64(q1), _u64(px), _u64(pz), _f64(th), mw, cut, mt, C.byref(h))
         self._nq[int(h.value)] = nq
         return int(h.value)
 

@@ -46,7 +46,7 @@ def _mixed_jobs(count, seed, qubits=(6, 20, 33, 64, 70, 100, 128), depths=(1, 4)
 
 def _oracle(O, prog, opts):
     o = cpu("oracle", O)
-    o.run_program(prog, opts.max_weight, opts.coefficient_cutoff)
+    o.run_program(prog, opts.max_weight, opts.coefficient_cutoff, opts.max_terms)
     e = o.zero_state_expectation()
     o.canonicalize()
     return e, o.to_host()
@@ -123,6 +123,36 @@ def test_server_matches_gpu_run_program(ctx):
         assert abs(r.expectation - e_orc) <= 1e-12 * max(1.0, np.abs(orc.value).sum())
 
 
+def test_server_max_terms_matches_oracle_and_alone(ctx):
+    """Jobs with a max_terms budget (trim after every truncation, include/
+    PauliPropagation.h:43-46) that binds: terms bit for bit against the
+    oracle's run_program with the same budget, and each job alone equals it in
+    the mix.  Some budgets are 0 and some exceed what the job reaches."""
+    rng = np.random.Generator(np.random.PCG64(31))
+    base = _mixed_jobs(8, seed=31, qubits=(12, 20, 33, 64, 100), depths=(2, 4))
+    jobs = []
+    for k, (O, prog, opts) in enumerate(base):
+        mt = [0, 3, 17, 60, 200, 1000, 5, 40][k]
+        jobs.append((O, prog, TruncationOptions(max_weight=opts.max_weight,
+                                                coefficient_cutoff=float(opts.coefficient_cutoff) * rng.uniform(0.01, 1),
+                                                max_terms=mt)))
+    mixed = _serve(ctx, jobs, max_jobs=16, stagger=3)
+    trimmed = 0
+    for k, job in enumerate(jobs):
+        e_want, want = _oracle(*job)
+        rm, tm = mixed[k]
+        assert rm.terms == len(want), f"job {k}"
+        assert_terms_equal(tm, want)
+        assert abs(rm.expectation - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum()), f"job {k}"
+        (ra, ta), = _serve(ctx, [job], max_jobs=1)
+        assert np.float64(ra.expectation).view(np.uint64) == np.float64(rm.expectation).view(np.uint64), f"job {k}"
+        assert_terms_equal(ta, tm)
+        full_e, full = _oracle(job[0], job[1], TruncationOptions(max_weight=job[2].max_weight,
+                                                                 coefficient_cutoff=job[2].coefficient_cutoff))
+        trimmed += len(full) > job[2].max_terms
+    assert trimmed >= 4  # the budgets bind for most jobs
+
+
 def test_server_budget_admits_one_at_a_time(ctx):
     jobs = _mixed_jobs(5, seed=5, qubits=(20, 40), depths=(1, 2))
     with Server(ctx, array_size=1, max_jobs=8, keep_terms=False, hbm_budget_bytes=1) as srv:
@@ -171,9 +201,6 @@ def test_server_rejects_bad_jobs(ctx):
         big = W.observable(100, [{0: "Z"}])
         with pytest.raises(EngineError):
             srv.submit(big, W.brickwork_program(100, 1, 1))
-        with pytest.raises(ValueError):
-            srv.submit(W.observable(8, [{0: "Z"}]), W.brickwork_program(8, 1, 1),
-                       TruncationOptions(max_terms=10))
 
 
 def test_server_gate_set_matches_run_program(ctx):
