@@ -272,10 +272,27 @@ def batched_products(ctx, count: int, budget_pairs: int, solo_pairs: int):
         ctx.synchronize()
         return time.perf_counter() - t0
 
+    # the products below the solo threshold (the ones batching can change):
+    # their share of the plain loop, and the batch against the loop on them
+    small = sorted((k for k in range(count) if pairs[k] < solo_pairs and L[k].array_size <= 2),
+                   key=lambda k: L[k].array_size)
+
+    def run_small_batched():
+        for a in (1, 2):
+            ks = [k for k in small if L[k].array_size == a]
+            for g0 in range(0, len(ks), 4096):
+                o = batch_multiply_canonical_each([L[k] for k in ks[g0:g0 + 4096]], [R[k] for k in ks[g0:g0 + 4096]],
+                                                  solo_pairs=solo_pairs)
+                for x in o:
+                    x.close()
+
     run_each()  # warm-up: grows the context's heap to the workload's working set
     t_each = timed(run_each)
     t_one = timed(run_one)
     t_plain = timed(run_plain, sorted(range(count), key=lambda k: L[k].array_size))
+    run_small_batched()
+    t_small_batch = timed(run_small_batched)
+    t_small_plain = timed(run_plain, small)
     for s_ in L + R:
         s_.close()
     ctx.trim()
@@ -284,6 +301,8 @@ def batched_products(ctx, count: int, budget_pairs: int, solo_pairs: int):
     log(f"batch: {len(each_groups)} groups ({solo} products alone, {count - solo} batched), {t_each:.3f} s "
         f"({total / t_each:.4g} pairs/s); one collation per group {t_one:.3f} s; plain loop {t_plain:.3f} s "
         f"({total / t_plain:.4g} pairs/s)")
+    log(f"batch: the {len(small)} products below {solo_pairs} pairs take {t_small_plain:.3f} s of the plain loop "
+        f"({100 * t_small_plain / t_plain:.1f}%); batched {t_small_batch:.3f} s ({t_small_plain / t_small_batch:.2f}x)")
     return {
         "workload": (f"{count} independent collated products, terms per sum log-uniform in [64, 65536], "
                      "qubits from {16, 32, 64, 128}, sites non-I w.p. 0.25, U[-1,1] FP64"),
@@ -292,6 +311,9 @@ def batched_products(ctx, count: int, budget_pairs: int, solo_pairs: int):
         "batch_s": t_each, "pairs_per_s": total / t_each, "jobs_per_s": count / t_each,
         "one_collation_per_group_s": t_one, "one_collation_groups": len(one_groups),
         "plain_loop_s": t_plain, "plain_pairs_per_s": total / t_plain, "speedup_vs_plain": t_plain / t_each,
+        "small_products": {"jobs": len(small), "pairs": int(sum(pairs[k] for k in small)),
+                           "plain_loop_s": t_small_plain, "share_of_plain_loop": t_small_plain / t_plain,
+                           "batch_s": t_small_batch, "speedup_vs_plain": t_small_plain / t_small_batch},
     }
 
 
@@ -628,11 +650,16 @@ def main():
     per_launch_s = dom["ms"] / dom["launches"] / 1e3
     per_launch_bytes = dom["bytes"] / dom["launches"]
     achieved = per_launch_bytes / per_launch_s / 1e9 if per_launch_bytes else 0.0
-    traffic = None
+    # HBM bytes per launch of the dominant kernel: a profile constant from the
+    # FETCH_SIZE / WRITE_SIZE counter runs (rocprofv3 cannot run inside the
+    # bench), only for the workload it was measured on
+    traffic, traffic_src = None, None
     tp = Path(args.traffic)
-    if tp.exists():
+    if tp.exists() and args.n == 65536 and args.m == 65536:
         try:
-            traffic = json.loads(tp.read_text()).get(dom_name)
+            tj = json.loads(tp.read_text())
+            traffic = tj.get(dom_name)
+            traffic_src = f"{tp.relative_to(ROOT) if tp.is_relative_to(ROOT) else tp}: {tj.get('_about', '')}"
         except Exception:
             traffic = None
     b_alg_job = (args.n + args.m) * 25 + t_out * 25
@@ -653,6 +680,7 @@ def main():
                    "collated_terms": t_out, "qubits": 64, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "launch_ms": per_launch_s * 1e3, "alg_bytes_per_launch": per_launch_bytes},
         "job_roofline": {"alg_bytes": b_alg_job, "frac": b_alg_job / dt / HBM_PEAK},
         "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()},
