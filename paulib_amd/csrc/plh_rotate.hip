@@ -585,91 +585,66 @@ __global__ void k_rj_bounds(const uint32_t* __restrict__ J, uint32_t n, uint32_t
     off[k] = lo;
 }
 
-// Word k of an AS-word register array by a (mostly uniform) index.
-template <int AS>
-PLH_DEV uint64_t word_of(const uint64_t* w, int k) {
-    uint64_t v = w[0];
-#pragma unroll
-    for (int i = 1; i < AS; ++i)
-        if (k == i) v = w[i];
-    return v;
-}
-
-// Whether the term (x, z words) anticommutes with a window entry's
-// generator, from its code (the generator's words for a wide one).
-template <int AS>
-PLH_DEV bool code_anti(uint32_t code, const uint64_t* p, const uint64_t* xw, const uint64_t* zw) {
-    if (code & (1u << 30)) {
-        uint64_t acc = 0;
-#pragma unroll
-        for (int k = 0; k < AS; ++k) acc ^= (xw[k] & p[AS + k]) ^ (zw[k] & p[k]);
-        return (popc(acc) & 1) != 0;
-    }
-    uint32_t par = 0;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const uint32_t f = code >> (10 * u);
-        const uint32_t t = (f >> 8) & 3u, q = f & 0xFFu;
-        if (!t) continue;
-        const int wd = (int)(q >> 6), b = (int)(q & 63);
-        const uint32_t xb = (uint32_t)(word_of<AS>(xw, wd) >> b) & 1u, zb = (uint32_t)(word_of<AS>(zw, wd) >> b) & 1u;
-        par ^= ((t & 1u) ? zb : 0u) ^ ((t & 2u) ? xb : 0u);  // X tests z, Z tests x, Y both
-    }
-    return par != 0;
-}
-
-// The first rotation of job's window that the term (x, z words) anticommutes
-// with, 255 when none; entries of kind 0 (no change) never count.
-template <int AS>
-PLH_DEV uint32_t window_first(const JobWindows& jw, uint32_t job, const uint64_t* xw, const uint64_t* zw) {
-    const int w = jw.wn[job];
-    uint32_t f = 0xFFu;
-    for (int e = w - 1; e >= 0; --e) {  // descending: the last hit is the first index
-        const size_t ent = (size_t)job * kJobWindow + (size_t)e;
-        const uint32_t code = jw.code[ent];
-        if (!(code >> 31)) continue;
-        if (code_anti<AS>(code, jw.p + ent * 2 * AS, xw, zw)) f = (uint32_t)e;
-    }
-    return f;
-}
-
 template <class T>
 PLH_DEV T wave_min(T v) {
     return wave_last(wave_scan_op(v, (T)~T(0), [](T a, T b) { return a < b ? a : b; }));
 }
 
-// Whether some term whose x and z words have ux and uz as their OR could
-// anticommute with a window entry (a scalar test of its code).
+// Per job and qubit, the window entries a set bit at that qubit anticommutes
+// with: low 16 bits for the term's z bit (entries whose generator has X or Y
+// there), high 16 bits for its x bit (Z or Y).  A term's anticommutation
+// with entry e is the parity of bit e over its set bits, so one pass over a
+// term's (few) set bits tests the whole window.  Entries of kind 0 (no
+// change) are left out.  One thread per (job, qubit).
 template <int AS>
-PLH_DEV bool code_may_hit(uint32_t code, const uint64_t* ux, const uint64_t* uz) {
-    if (code & (1u << 30)) return true;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const uint32_t f = code >> (10 * u);
-        const uint32_t t = (f >> 8) & 3u, q = f & 0xFFu;
-        if (!t) continue;
-        const int wd = (int)(q >> 6), b = (int)(q & 63);
-        const uint64_t xb = (word_of<AS>(ux, wd) >> b) & 1ull, zb = (word_of<AS>(uz, wd) >> b) & 1ull;
-        if (((t & 1u) && zb) || ((t & 2u) && xb)) return true;
+__global__ void k_rj_tables(JobWindows jw, uint32_t njobs, const uint8_t* __restrict__ alive,
+                            uint32_t* __restrict__ tbl) {
+    constexpr uint32_t Q = 64u * AS;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)njobs * Q) return;
+    const uint32_t job = (uint32_t)(g / Q), q = (uint32_t)(g % Q);
+    const int wd = (int)(q >> 6), b = (int)(q & 63);
+    uint32_t mz = 0, mx = 0;
+    const int wn = alive[job] ? (int)jw.wn[job] : 0;
+    for (int e = 0; e < wn; ++e) {
+        const size_t ent = (size_t)job * kJobWindow + (size_t)e;
+        if (jw.kind[ent] != 4) continue;
+        const uint64_t* p = jw.p + ent * 2 * AS;
+        if ((p[wd] >> b) & 1ull) mz |= 1u << e;       // X or Y at q: the term's z bit counts
+        if ((p[AS + wd] >> b) & 1ull) mx |= 1u << e;  // Z or Y at q: the term's x bit counts
     }
-    return false;
+    tbl[g] = mz | (mx << 16);
 }
 
 // ---- 1. per term the first rotation of its job's window it anticommutes
 // with (fk, one byte), and per job the least of them (jmin).  A block stages
-// the windows of the jobs its tile spans (at most kStageJobs) in LDS, and
-// issues every term's loads before the tests.  A wave of one job first ORs
-// its terms' words: an entry whose qubits none of them touches in an
-// anticommuting way is skipped by a scalar test (most rotations of a window
-// lie outside a low-weight observable's support).
+// the qubit tables of the jobs its tile spans (at most kStageJobs) in LDS;
+// each term walks its set bits.
 constexpr int kStageJobs = 8;
 
 template <int AS>
-__global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_t n, JobWindows jw,
-                                                     const uint8_t* __restrict__ alive, uint8_t* __restrict__ fk,
-                                                     uint32_t* __restrict__ jmin) {
-    __shared__ uint32_t s_code[kStageJobs * kJobWindow];
-    __shared__ uint32_t s_wn[kStageJobs];
+PLH_DEV uint32_t first_hit(const uint32_t* t, const uint64_t* xw, const uint64_t* zw) {
+    uint32_t par = 0;
+#pragma unroll
+    for (int w = 0; w < AS; ++w) {
+        uint64_t m = xw[w] | zw[w];
+        while (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t e = t[64 * w + b];
+            if ((zw[w] >> b) & 1ull) par ^= e & 0xFFFFu;
+            if ((xw[w] >> b) & 1ull) par ^= e >> 16;
+        }
+    }
+    return par ? (uint32_t)__builtin_ctz(par) : 0xFFu;
+}
+
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_t n, const uint32_t* __restrict__ tbl,
+                                                     uint8_t* __restrict__ fk, uint32_t* __restrict__ jmin) {
+    constexpr uint32_t Q = 64u * AS;
+    __shared__ uint32_t s_tbl[kStageJobs * Q];
+    __shared__ uint32_t s_min[kStageJobs];
     __shared__ uint32_t s_j0, s_nj;
     const uint64_t tile = blockIdx.x;
     const uint64_t i0 = tile * kRjTile;
@@ -681,13 +656,9 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
     __syncthreads();
     const uint32_t j0 = s_j0, nj = s_nj;
     const bool staged = nj <= (uint32_t)kStageJobs;
-    if (staged) {
-        for (uint32_t t = threadIdx.x; t < nj * kJobWindow; t += kBlock) {
-            const uint32_t jb = j0 + t / kJobWindow;
-            s_code[t] = alive[jb] ? jw.code[(size_t)jb * kJobWindow + t % kJobWindow] : 0u;
-        }
-        if (threadIdx.x < nj) s_wn[threadIdx.x] = alive[j0 + threadIdx.x] ? jw.wn[j0 + threadIdx.x] : 0u;
-    }
+    if (staged)
+        for (uint32_t t = threadIdx.x; t < nj * Q; t += kBlock) s_tbl[t] = tbl[(size_t)j0 * Q + t];
+    if (threadIdx.x < (unsigned)kStageJobs) s_min[threadIdx.x] = 0xFFu;
     __syncthreads();
     uint32_t job[4];
     uint64_t xw[4][AS], zw[4][AS];
@@ -704,43 +675,34 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint64_t i = rj_item(tile, k);
-        const bool valid = i < n;
         uint32_t f = 0xFFu;
-        const uint32_t jf = (uint32_t)__builtin_amdgcn_readfirstlane((int)job[k]);
-        if (staged && ballot(valid && job[k] != jf) == 0ull) {
-            if (jf != 0xFFFFFFFFu) {
-                uint64_t ux[AS], uz[AS];
-#pragma unroll
-                for (int w = 0; w < AS; ++w) {
-                    ux[w] = wave_or(xw[k][w]);
-                    uz[w] = wave_or(zw[k][w]);
-                }
-                const uint32_t* codes = s_code + (jf - j0) * kJobWindow;
-                const int wn = (int)s_wn[jf - j0];
-                for (int e = wn - 1; e >= 0; --e) {  // descending: the last hit is the first index
-                    const uint32_t code = (uint32_t)__builtin_amdgcn_readfirstlane((int)codes[e]);
-                    if (!(code >> 31) || !code_may_hit<AS>(code, ux, uz)) continue;
-                    const bool hit = code_anti<AS>(code, jw.p + ((size_t)jf * kJobWindow + (size_t)e) * 2 * AS,
-                                                   xw[k], zw[k]);
-                    if (hit && valid) f = (uint32_t)e;
-                }
-                const uint32_t m = wave_min(f);
-                if (lane_id() == 0 && m < 0xFFu) atomicMin(&jmin[jf], m);
-            }
-        } else if (valid) {
-            // a wave that spans jobs (or a tile of many small jobs): per lane
+        if (i < n) {
             const uint32_t jb = job[k];
-            const int wn = alive[jb] ? (int)jw.wn[jb] : 0;
-            for (int e = wn - 1; e >= 0; --e) {
-                const size_t ent = (size_t)jb * kJobWindow + (size_t)e;
-                const uint32_t code = jw.code[ent];
-                if (!(code >> 31)) continue;
-                if (code_anti<AS>(code, jw.p + ent * 2 * AS, xw[k], zw[k])) f = (uint32_t)e;
-            }
-            if (f < 0xFFu) atomicMin(&jmin[jb], f);
+            f = staged ? first_hit<AS>(s_tbl + (jb - j0) * Q, xw[k], zw[k])
+                       : first_hit<AS>(tbl + (size_t)jb * Q, xw[k], zw[k]);
+            fk[i] = (uint8_t)f;
         }
-        if (valid) fk[i] = (uint8_t)f;
+        // per-job minimum, reduced in LDS first (one global atomic per job
+        // and block: the per-job words would otherwise take an atomic from
+        // every wave, all in a few L2 channels)
+        const uint32_t jf = (uint32_t)__builtin_amdgcn_readfirstlane((int)job[k]);
+        if (ballot(i < n && job[k] != jf) == 0ull) {
+            const uint32_t m = wave_min(f);
+            if (lane_id() == 0 && m < 0xFFu && jf != 0xFFFFFFFFu) {
+                if (staged)
+                    atomicMin(&s_min[jf - j0], m);
+                else
+                    atomicMin(&jmin[jf], m);
+            }
+        } else if (f < 0xFFu) {
+            if (staged)
+                atomicMin(&s_min[job[k] - j0], f);
+            else
+                atomicMin(&jmin[job[k]], f);
+        }
     }
+    __syncthreads();
+    if (staged && threadIdx.x < nj && s_min[threadIdx.x] < 0xFFu) atomicMin(&jmin[j0 + threadIdx.x], s_min[threadIdx.x]);
 }
 
 // ---- 1b. per job the rotation it applies (the first of its window that
@@ -1054,7 +1016,8 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const size_t job_bytes = up256((nj + 1) * 4) + up256(nj * 4) + up256(nj) + up256(nj * 2 * AS * 8) + up256(nj) +
                              2 * up256(nj * 8) + up256(nj) + up256(nj) + up256(16);
     void* p;
-    PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n) + up256((tiles + 1) * 4), &p));
+    PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n) + up256((tiles + 1) * 4) +
+                                     up256(nj * 64 * AS * 4), &p));
     char* c = (char*)p;
     FlagBuf fa, fd, fkn, fko;
     fa.carve(c, tiles);
@@ -1084,6 +1047,8 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     uint8_t* fk = (uint8_t*)c;
     c += up256(n);
     uint32_t* tr0 = (uint32_t*)c;
+    c += up256((tiles + 1) * 4);
+    uint32_t* qtbl = (uint32_t*)c;
     const JobRotTables jt{e_kind, e_p, e_psf, e_c, e_s};
     const size_t nb_planes = up256(n * 8 * AS);
     PLH_TRY(ctx->ws.get(kWsRotN, 2 * nb_planes + up256(n * 8) + up256(n) + up256(n * 4) + up256(n * 4), &p));
@@ -1098,9 +1063,11 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const unsigned gt = (unsigned)tiles, g1 = blocks_of(n);
     k_rj_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, joff, jmin);
     PLH_HIP(hipGetLastError());
+    k_rj_tables<AS><<<blocks_of((uint64_t)nj * 64 * AS), kBlock, 0, st>>>(jw, njobs, fl.job_alive, qtbl);
+    PLH_HIP(hipGetLastError());
     {
         ProfScope ps(ctx, "rot_anti");
-        k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jw, fl.job_alive, fk, jmin);
+        k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, fk, jmin);
         PLH_HIP(hipGetLastError());
         k_rj_select<AS><<<blocks_of(nj), kBlock, 0, st>>>(jw, njobs, jmin, e_kind, e_p, e_psf, e_c, e_s, kstar, adv);
         PLH_HIP(hipGetLastError());
@@ -1246,14 +1213,13 @@ constexpr int kSumWindow = 16;
 
 template <int AS>
 struct WindowParams {
-    uint64_t px[kSumWindow][AS], pz[kSumWindow][AS];
-    uint8_t kind[kSumWindow];  // 0 no change, 4 general
-    int w;
+    uint32_t tbl[64 * AS];  // per qubit: low 16 bits entries tested by its z bit, high 16 by its x bit
 };
 
 __global__ void k_la_init(uint32_t* kmin) { *kmin = 0xFFu; }
 
-// The least window index any term anticommutes with (0xFF: none).
+// The least window index any term anticommutes with (0xFF: none), from the
+// qubit table: a term XORs the entry masks of its set bits.
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_la_first(Sorted<AS, false> S, uint32_t n, WindowParams<AS> wp,
                                                      uint32_t* __restrict__ kmin) {
@@ -1267,13 +1233,7 @@ __global__ __launch_bounds__(kBlock) void k_la_first(Sorted<AS, false> S, uint32
             xw[k] = k < S.as ? S.x[(int64_t)k * S.stride + i] : 0ull;
             zw[k] = k < S.as ? S.z[(int64_t)k * S.stride + i] : 0ull;
         }
-        for (int e = wp.w - 1; e >= 0; --e) {  // descending: the last hit is the first index
-            if (wp.kind[e] != 4) continue;
-            uint64_t acc = 0;
-#pragma unroll
-            for (int k = 0; k < AS; ++k) acc ^= (xw[k] & wp.pz[e][k]) ^ (zw[k] & wp.px[e][k]);
-            if (popc(acc) & 1) f = (uint32_t)e;
-        }
+        f = first_hit<AS>(wp.tbl, xw, zw);
     }
     const uint32_t m = wave_min(f);
     if (lane_id() == 0) sh[threadIdx.x / kWave] = m;
@@ -1358,26 +1318,24 @@ int rotate_sorted_window(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const*
         *consumed = w;
         return PLH_OK;
     }
+    // the qubit table of the window: entries of kind 0 (no change) never count
+    uint32_t tbl[128] = {};
+    for (int e = 0; e < w; ++e) {
+        if (kind[e] != 4) continue;
+        for (int k = 0; k < s->as; ++k)
+            for (int b = 0; b < 64; ++b) {
+                if ((px[e][k] >> b) & 1ull) tbl[64 * k + b] |= 1u << e;          // X or Y: the term's z bit
+                if ((pz[e][k] >> b) & 1ull) tbl[64 * k + b] |= 1u << (16 + e);   // Z or Y: the term's x bit
+            }
+    }
     int first = 0xFF;
     if (s->as == 1) {
         WindowParams<1> wp{};
-        wp.w = w;
-        for (int e = 0; e < w; ++e) {
-            wp.px[e][0] = px[e][0];
-            wp.pz[e][0] = pz[e][0];
-            wp.kind[e] = kind[e];
-        }
+        for (int q = 0; q < 64; ++q) wp.tbl[q] = tbl[q];
         PLH_TRY(window_first_sorted<1>(ctx, s, wp, &first));
     } else {
         WindowParams<2> wp{};
-        wp.w = w;
-        for (int e = 0; e < w; ++e) {
-            for (int k = 0; k < 2; ++k) {
-                wp.px[e][k] = px[e][k];
-                wp.pz[e][k] = pz[e][k];
-            }
-            wp.kind[e] = kind[e];
-        }
+        for (int q = 0; q < 128; ++q) wp.tbl[q] = tbl[q];
         PLH_TRY(window_first_sorted<2>(ctx, s, wp, &first));
     }
     if (first >= w) {

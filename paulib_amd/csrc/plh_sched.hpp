@@ -91,32 +91,7 @@ struct StepPlan {
     std::vector<uint8_t> wn, wk, wpsf;
     std::vector<double> wc, ws;
     std::vector<uint64_t> wp;
-    std::vector<uint32_t> wcode;  // window_code of each entry
 };
-
-// A window entry's anticommutation test in one word: bit 31 set for a
-// general-path rotation (a no-change rotation never changes a term); for a
-// generator on at most two qubits, qubit and Pauli of each (bits 0-7 and
-// 8-9, 10-17 and 18-19; Pauli 1 X, 2 Z, 3 Y, 0 none); bit 30 set when the
-// generator has more qubits (the test then reads its words).
-inline uint32_t window_code(uint8_t rk, const uint64_t* px, const uint64_t* pz, int as) {
-    if (rk != 4) return 0u;
-    uint32_t code = 1u << 31;
-    int used = 0;
-    for (int w = 0; w < as; ++w) {
-        uint64_t m = px[w] | pz[w];
-        while (m) {
-            const int b = __builtin_ctzll(m);
-            m &= m - 1;
-            if (used == 2) return (1u << 31) | (1u << 30);
-            const uint32_t q = (uint32_t)(64 * w + b);
-            const uint32_t t = (uint32_t)((px[w] >> b) & 1u) | ((uint32_t)((pz[w] >> b) & 1u) << 1);
-            code |= (q | (t << 8)) << (10 * used);
-            ++used;
-        }
-    }
-    return code;
-}
 
 inline bool close_one(double v) { return std::fabs(v - 1.0) <= 1e-14; }  // isCloseOne, :19-21
 
@@ -228,7 +203,6 @@ class Scheduler {
         p->wc.assign(SW, 0.0);
         p->ws.assign(SW, 0.0);
         p->wp.assign(SW * 2 * A, 0);
-        p->wcode.assign(SW, 0);
         for (size_t i = 0; i < S; ++i) {
             p->off[i] = (uint32_t)p->ops.size();
             if (state_[i] != kRunning) continue;
@@ -254,7 +228,6 @@ class Scheduler {
                         p->wp[e * 2 * A + k] = j.px[(size_t)l * ja + k];
                         p->wp[e * 2 * A + A + k] = j.pz[(size_t)l * ja + k];
                     }
-                    p->wcode[e] = window_code(rk, &p->wp[e * 2 * A], &p->wp[e * 2 * A + A], (int)A);
                 }
                 p->wn[i] = (uint8_t)w;
                 p->maxw[i] = j.max_weight;

@@ -557,7 +557,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
         const void* src;
         size_t bytes;
     };
-    constexpr int kParts = 18;
+    constexpr int kParts = 17;
     const Part parts[kParts] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
                                 {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
                                 {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
@@ -566,7 +566,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
                                 {pl.wn.data(), pl.wn.size()},       {pl.wk.data(), pl.wk.size()},
                                 {pl.wp.data(), pl.wp.size() * 8},   {pl.wpsf.data(), pl.wpsf.size()},
                                 {pl.wc.data(), pl.wc.size() * 8},   {pl.ws.data(), pl.ws.size() * 8},
-                                {pl.wcode.data(), pl.wcode.size() * 4}, {pl.maxt.data(), pl.maxt.size() * 8}};
+                                {pl.maxt.data(), pl.maxt.size() * 8}};
     static_assert(sched::kWindow == kJobWindow, "scheduler and engine agree on the window depth");
     size_t total = 0;
     for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
@@ -611,8 +611,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
         sc.job_maxw = tb.maxw;
         sc.job_alive = tb.alive;
         const JobWindows jw{(const uint8_t*)dptr[10], (const uint8_t*)dptr[11], (const uint64_t*)dptr[12],
-                            (const uint8_t*)dptr[13], (const double*)dptr[14], (const double*)dptr[15],
-                            (const uint32_t*)dptr[16]};
+                            (const uint8_t*)dptr[13], (const double*)dptr[14], (const double*)dptr[15]};
         PLH_TRY(rotate_sorted_jobs(ctx, B, jw, s->sched->slots(), sc, s->adv_dev));
         if (!s->sched->settle(&pl, s->adv_host)) {
             set_error("plh_server_step: the device reported an out-of-range rotation count");
@@ -647,7 +646,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
     }
 
     // ---- apply_truncation's trim for the jobs that have max_terms
-    if (pl.any_trim && B->n > 0) PLH_TRY(trim_jobs(ctx, B, (const int64_t*)dptr[17], (uint32_t)s->sched->slots()));
+    if (pl.any_trim && B->n > 0) PLH_TRY(trim_jobs(ctx, B, (const int64_t*)dptr[16], (uint32_t)s->sched->slots()));
 
     // ---- finished jobs: expectation on the job-ordered batch
     if (pl.finishing.empty()) return PLH_OK;

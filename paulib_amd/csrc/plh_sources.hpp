@@ -127,9 +127,6 @@ struct JobRotTables {
 // slot): wn[slot] rotations, entry slot * kJobWindow + k with its rotation
 // kind (0 no change, 4 general), ps flags, cos, sin and P (x[AS] then z[AS]).
 constexpr int kJobWindow = 16;
-// code: each entry's anticommutation test in one word (bit 31 general
-// path, bit 30 a generator on more than two qubits, else qubit and Pauli of
-// each of its at most two qubits: bits 0-7 and 8-9, 10-17 and 18-19).
 struct JobWindows {
     const uint8_t* wn;
     const uint8_t* kind;
@@ -137,7 +134,6 @@ struct JobWindows {
     const uint8_t* psf;
     const double* c;
     const double* s;
-    const uint32_t* code;
 };
 
 // A sum after evolve's general path, without materialising the new terms.

@@ -356,34 +356,6 @@ struct Replay {
         return PLH_OK;
     }
 
-    // A window code answers the anticommutation test as the generator's words
-    // do (symplectic form parity), on random terms.
-    void check_code(uint32_t code, const uint64_t* px, const uint64_t* pz, int as, bool general) {
-        if (((code >> 31) != 0) != general) fail("window code's general bit");
-        if (!general) return;
-        for (int r = 0; r < 64; ++r) {
-            uint64_t x[4] = {}, z[4] = {};
-            int full = 0;
-            for (int w = 0; w < as; ++w) {
-                x[w] = rng() & rng();
-                z[w] = rng() & rng();
-                full += __builtin_popcountll((x[w] & pz[w]) ^ (z[w] & px[w]));
-            }
-            int par = 0;
-            if (code & (1u << 30)) {
-                par = full;
-            } else {
-                for (int u = 0; u < 2; ++u) {
-                    const uint32_t f = code >> (10 * u), t = (f >> 8) & 3u, q = f & 0xFFu;
-                    if (!t) continue;
-                    const int xb = (int)((x[q >> 6] >> (q & 63)) & 1u), zb = (int)((z[q >> 6] >> (q & 63)) & 1u);
-                    par += ((t & 1u) ? zb : 0) + ((t & 2u) ? xb : 0);
-                }
-            }
-            if ((par & 1) != (full & 1)) fail("window code %lld disagrees with the generator's words", (long long)code);
-        }
-    }
-
     // After a sorted step's report: each advancing job moved by its count,
     // waiting jobs stayed, and exactly the jobs that passed op 0 finish.
     void verify_settled(const StepPlan& pl, const std::vector<uint8_t>& adv) {
@@ -437,9 +409,6 @@ struct Replay {
                     const size_t ent = (size_t)i * plh::sched::kWindow + (size_t)e;
                     if (pl.wk[ent] != want || (want != 0 && want != 4)) fail("window entry %lld of job %lld: kind", e, j.handle);
                     if (pl.wc[ent] != c || pl.ws[ent] != sn) fail("window entry %lld of job %lld: angle", e, j.handle);
-                    if (pl.wcode[ent] != plh::sched::window_code(want, &pl.wp[ent * 2 * A], &pl.wp[ent * 2 * A + A], (int)A))
-                        fail("window entry %lld of job %lld: code", e, j.handle);
-                    check_code(pl.wcode[ent], &pl.wp[ent * 2 * A], &pl.wp[ent * 2 * A + A], (int)A, want == 4);
                     for (size_t k = 0; k < A; ++k)
                         if (pl.wp[ent * 2 * A + k] != (k < ja ? j.px[(size_t)l * ja + k] : 0ull) ||
                             pl.wp[ent * 2 * A + A + k] != (k < ja ? j.pz[(size_t)l * ja + k] : 0ull))
