@@ -321,10 +321,12 @@ int rotate_sorted_window(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const*
 // through, under their own filters.  adv_host (device address of host-pinned
 // memory, njobs bytes) receives the rotations each job consumed; it is
 // complete when the call returns.  Every job's terms must have passed its
-// truncation since its keys last changed.
+// truncation since its keys last changed.  place (or null): a sorted batched
+// sum of other jobs (not alive under spec) whose terms go into the output
+// between the rotated jobs', in job order.
 struct JobWindows;
 int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, int njobs, const CollateSpec& spec,
-                       uint8_t* adv_host);
+                       uint8_t* adv_host, const plh_sum* place = nullptr);
 // out = canonicalize(lhs[k] * rhs[k]) for k < njobs as one batched sum
 // (job plane = k, jobs in order); offsets[k] .. offsets[k + 1] is job k's
 // range (njobs + 1 entries, host).  Sum of n_k * m_k <= 2^32.

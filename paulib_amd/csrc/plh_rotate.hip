@@ -909,7 +909,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
 // host: every thread fences its stores to host memory before the block's
 // barrier, and thread 0 then posts the count (the host polls its sequence).
 __global__ __launch_bounds__(kBlock) void k_rj_total(const uint64_t* __restrict__ ko_total,
-                                                     const uint64_t* __restrict__ kn_total,
+                                                     const uint64_t* __restrict__ kn_total, uint64_t extra,
                                                      uint64_t* __restrict__ out, const uint8_t* __restrict__ adv,
                                                      uint32_t njobs, uint8_t* adv_host, uint64_t* mbox,
                                                      uint64_t seq) {
@@ -917,20 +917,22 @@ __global__ __launch_bounds__(kBlock) void k_rj_total(const uint64_t* __restrict_
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const uint64_t t = *ko_total + *kn_total;
+    const uint64_t t = *ko_total + *kn_total + extra;
     *out = t;
     post_write(mbox, seq, 1, &t);
 }
 
 // ---- 5. kept terms to their places: old term i after the kept old terms
 // before it and the kept new terms below it; new term r after the kept new
-// terms before it and the kept old terms below its lower bound.  Blocks
+// terms before it and the kept old terms below its lower bound; both after
+// the placed terms (toff, when set) of the jobs before their own.  Blocks
 // [0, tiles) take old tiles, [tiles, 2 tiles) new ones.
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_t n, uint64_t tiles, JobRotTables jt,
                                                      BitRank A, const uint32_t* __restrict__ lb,
                                                      const uint32_t* __restrict__ tr0, BitRank D, BitRank KO,
                                                      BitRank KN, NewTerms N, const uint64_t* __restrict__ d_total,
+                                                     const uint32_t* __restrict__ toff,
                                                      uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
                                                      uint8_t* __restrict__ of, double* __restrict__ ov,
                                                      uint32_t* __restrict__ oj, int64_t os) {
@@ -946,7 +948,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
             const uint32_t job = S.j[i];
             uint32_t u;
             const double acc = rj_old_value<AS>(S, (uint32_t)i, job, jt, A, lb, D, N, rr[0], rr[1], &u);
-            const uint64_t o = KO.rank(i) + KN.rank(u);
+            const uint64_t o = KO.rank(i) + KN.rank(u) + (toff ? toff[job] : 0u);
             if (o >= total) continue;  // never for consistent ranks; keeps writes in bounds
 #pragma unroll
             for (int w = 0; w < AS; ++w) {
@@ -964,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
     for (int k = 0; k < 4; ++k) {
         const uint64_t r = rj_item(tile, k);
         if (r >= m || !KN.test(r)) continue;
-        const uint64_t o = KN.rank(r) + KO.rank(lb[r]);
+        const uint64_t o = KN.rank(r) + KO.rank(lb[r]) + (toff ? toff[N.j[r]] : 0u);
         if (o >= total) continue;
 #pragma unroll
         for (int w = 0; w < AS; ++w) {
@@ -975,6 +977,33 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
         ov[o] = N.v[r];
         oj[o] = N.j[r];
     }
+}
+
+// ---- 5b. a job-ordered sum of other jobs' terms (placed) into the output:
+// term t of job j after the kept terms of the rotated jobs before j (their
+// kept old terms before joff[j] and kept new terms before the first new
+// term of job j) and after the placed terms before it.
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_rj_place(SumView P, const uint32_t* __restrict__ PJ, uint32_t pn,
+                                                     const uint32_t* __restrict__ joff, BitRank A, BitRank KO,
+                                                     BitRank KN, const uint64_t* __restrict__ d_total,
+                                                     uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
+                                                     uint8_t* __restrict__ of, double* __restrict__ ov,
+                                                     uint32_t* __restrict__ oj, int64_t os) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= pn) return;
+    const uint32_t job = PJ[t];
+    const uint64_t b = joff[job];
+    const uint64_t o = KO.rank(b) + KN.rank(A.rank(b)) + t;
+    if (o >= *d_total) return;  // never for consistent ranks; keeps writes in bounds
+#pragma unroll
+    for (int w = 0; w < AS; ++w) {
+        ox[(int64_t)w * os + o] = P.x[(int64_t)w * P.stride + t];
+        oz[(int64_t)w * os + o] = P.z[(int64_t)w * P.stride + t];
+    }
+    of[o] = P.f[t];
+    ov[o] = P.v[t];
+    oj[o] = job;
 }
 
 // Flag arrays of `tiles` tiles: bits, in-tile prefixes, tile counts and
@@ -1003,7 +1032,7 @@ struct FlagBuf {
 
 template <int AS>
 int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const Filters& fl, uint32_t njobs,
-                          uint8_t* adv_host) {
+                          uint8_t* adv_host, const plh_sum* place) {
     hipStream_t st = ctx->stream;
     const uint64_t n = (uint64_t)s->n;
     const uint64_t tiles = (n + kRjTile - 1) / kRjTile;
@@ -1017,7 +1046,7 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
                              2 * up256(nj * 8) + up256(nj) + up256(nj) + up256(16);
     void* p;
     PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n) + up256((tiles + 1) * 4) +
-                                     up256(nj * 64 * AS * 4), &p));
+                                     up256(nj * 64 * AS * 4) + up256((nj + 1) * 4), &p));
     char* c = (char*)p;
     FlagBuf fa, fd, fkn, fko;
     fa.carve(c, tiles);
@@ -1049,6 +1078,9 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     uint32_t* tr0 = (uint32_t*)c;
     c += up256((tiles + 1) * 4);
     uint32_t* qtbl = (uint32_t*)c;
+    c += up256(nj * 64 * AS * 4);
+    uint32_t* toff = (uint32_t*)c;  // job offsets of the placed terms
+    const uint64_t pn = place ? (uint64_t)place->n : 0;
     const JobRotTables jt{e_kind, e_p, e_psf, e_c, e_s};
     const size_t nb_planes = up256(n * 8 * AS);
     PLH_TRY(ctx->ws.get(kWsRotN, 2 * nb_planes + up256(n * 8) + up256(n) + up256(n * 4) + up256(n * 4), &p));
@@ -1063,6 +1095,10 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const unsigned gt = (unsigned)tiles, g1 = blocks_of(n);
     k_rj_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, joff, jmin);
     PLH_HIP(hipGetLastError());
+    if (pn) {
+        k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(place->j, (uint32_t)pn, njobs, toff);
+        PLH_HIP(hipGetLastError());
+    }
     k_rj_tables<AS><<<blocks_of((uint64_t)nj * 64 * AS), kBlock, 0, st>>>(jw, njobs, fl.job_alive, qtbl);
     PLH_HIP(hipGetLastError());
     {
@@ -1096,8 +1132,8 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const BitRank KO = fko.rank(tiles);
     Post post;
     PLH_TRY(post_prepare(ctx, &post));
-    k_rj_total<<<1, kBlock, 0, st>>>(fko.tpre + tiles, fkn.tpre + tiles, d_total, adv, njobs, adv_host, post.mbox,
-                                     post.seq);
+    k_rj_total<<<1, kBlock, 0, st>>>(fko.tpre + tiles, fkn.tpre + tiles, pn, d_total, adv, njobs, adv_host,
+                                     post.mbox, post.seq);
     PLH_HIP(hipGetLastError());
     uint64_t total = 0;
     PLH_TRY(post_wait(ctx, post, 1, &total));
@@ -1109,8 +1145,12 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(total, 1), 0));
     if (total) {
         ProfScope ps(ctx, "rot_write", (double)total * (16.0 * s->as + 13.0));
-        k_rj_write<AS><<<2 * gt, kBlock, 0, st>>>(S, (uint32_t)n, tiles, jt, A, lb, tr0, D, KO, KN, N, d_total, tmp.x,
-                                                  tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
+        k_rj_write<AS><<<2 * gt, kBlock, 0, st>>>(S, (uint32_t)n, tiles, jt, A, lb, tr0, D, KO, KN, N, d_total,
+                                                  pn ? toff : nullptr, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
+        if (pn)
+            k_rj_place<AS><<<blocks_of(pn), kBlock, 0, st>>>(
+                SumView{place->x, place->z, place->f, place->v, place->cap}, place->j, (uint32_t)pn, joff, A, KO, KN,
+                d_total, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             sum_free_storage(&tmp);
@@ -1348,7 +1388,7 @@ int rotate_sorted_window(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const*
 }
 
 int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, int njobs, const CollateSpec& spec,
-                       uint8_t* adv_host) {
+                       uint8_t* adv_host, const plh_sum* place) {
     if (!s->sorted || !s->has_job || (spec.mode & (kFold | kDropZero)) != (kFold | kDropZero)) {
         set_error("rotate_sorted_jobs: needs a sorted canonical batched sum and a canonicalizing spec");
         return PLH_ERR_ARG;
@@ -1363,8 +1403,12 @@ int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, int njobs
     }
     if (s->n == 0) return PLH_ERR_ARG;  // the caller reports consumed rotations itself for an empty batch
     const Filters fl = filters_of(spec);
-    if (s->as == 1) return rotate_sorted_jobs_as<1>(ctx, s, jw, fl, (uint32_t)njobs, adv_host);
-    if (s->as == 2) return rotate_sorted_jobs_as<2>(ctx, s, jw, fl, (uint32_t)njobs, adv_host);
+    if (place && place->n > 0 && (!place->has_job || place->as != s->as || place->n > (int64_t)0x7FFFFFFFll)) {
+        set_error("rotate_sorted_jobs: the placed terms must be a batched sum of the same array size");
+        return PLH_ERR_ARG;
+    }
+    if (s->as == 1) return rotate_sorted_jobs_as<1>(ctx, s, jw, fl, (uint32_t)njobs, adv_host, place);
+    if (s->as == 2) return rotate_sorted_jobs_as<2>(ctx, s, jw, fl, (uint32_t)njobs, adv_host, place);
     set_error("rotate_sorted_jobs: batched sums support up to 128 qubits");
     return PLH_ERR_AS;
 }

@@ -55,11 +55,13 @@ struct Job {
 enum SlotState : int { kFree = 0, kRunning = 1, kDraining = 2 };
 
 // A full step advances every running job by its Clifford ops up to and
-// including its next rotation and collates the batch; a sorted step keeps
-// the batch sorted: only jobs whose next op is a rotation that keeps the
-// order (its truncation alone, or the general path) advance, by a sort-free
-// rotation, and the others (a Clifford op or a key-moving fast path next)
-// wait for the next full step with their terms untouched.
+// including its next rotation and collates the batch.  A mixed step keeps
+// the batch sorted and advances every running job too: a job whose next op
+// is a rotation that keeps the order (its truncation alone, or the general
+// path) and whose terms are in the sorted batch advances by a sort-free
+// rotation window (an R job); the others (a Clifford op or a key-moving fast
+// path next, or admitted this step) take the full step's ops on their own
+// terms, which are collated apart and placed between the R jobs' (C jobs).
 enum StepKind : int { kFullStep = 0, kSortedStep = 1 };
 
 // A sorted step hands each advancing job a window of up to kWindow
@@ -79,6 +81,9 @@ struct StepPlan {
     std::vector<int64_t> maxw;
     std::vector<int64_t> maxt;    // per slot: max_terms when its step truncates and it trims, else -1
     bool any_trim = false;
+    // mixed steps: per slot 1 for a C job (collated apart), and whether any
+    std::vector<uint8_t> isc;
+    bool any_c = false;
     std::vector<uint64_t> rp;     // per slot: x[AS] then z[AS] of its rotation
     std::vector<uint64_t> p_or;   // OR of the general-path rotations' words (x then z)
     std::vector<int64_t> next;    // per slot: the job's next op after this step
@@ -148,7 +153,11 @@ class Scheduler {
         active_[(size_t)slot] = std::move(queue_.front());
         queue_.pop_front();
         state_[(size_t)slot] = kRunning;
+        fresh_[(size_t)slot] = 1;
     }
+    // Whether the slot's job was admitted for the coming step (its terms are
+    // not in the sorted batch yet).
+    bool fresh(int slot) const { return fresh_[(size_t)slot] != 0; }
 
     // Whether a running job can advance in a sorted step.
     bool sorted_ready(int slot) const {
@@ -165,20 +174,38 @@ class Scheduler {
         return r;
     }
 
-    // The next step's kind: a sorted step while the batch is sorted and more
-    // than half of the running jobs can advance without a sort, a full
-    // step otherwise and after max_sorted sorted steps in a row (so a job
-    // waiting at a Clifford op, and a queued job, waits at most that long).
-    StepKind choose_step(bool batch_sorted, int64_t batch_terms, int since_full, int max_sorted) const {
-        const int run = running(), ready = sorted_ready_count();
-        const bool full = !batch_sorted || batch_terms == 0 || ready == 0 || 2 * (run - ready) >= run ||
-                          since_full >= max_sorted;
+    // The next step's kind: a mixed step while the batch is sorted and some
+    // running job can advance without a sort, a full step otherwise.
+    StepKind choose_step(bool batch_sorted, int64_t batch_terms) const {
+        const bool full = !batch_sorted || batch_terms == 0 || sorted_ready_count() == 0;
         return full ? kFullStep : kSortedStep;
     }
 
-    // This step's tables; no job changes.  In a sorted step the jobs that
-    // cannot advance keep their place: no ops, no rotation, no truncation.
+    // Whether this mixed step takes its C jobs (their Clifford layers and the
+    // admitted jobs): when at least an eighth of the running jobs are C, or
+    // one of them has waited kMaxWait steps; else they wait (their terms stay
+    // in the batch untouched) and only R jobs advance, which saves a small
+    // collation's launches and read-backs per step.
+    static constexpr int kMaxWait = 8;
+    bool take_c() const {
+        int c = 0, run = 0, waited = 0;
+        for (int i = 0; i < slots_; ++i) {
+            if (state_[(size_t)i] != kRunning) continue;
+            ++run;
+            if (fresh(i)) return true;  // admitted jobs' terms are outside the batch
+            if (!sorted_ready(i)) {
+                ++c;
+                waited = std::max(waited, (int)wait_[(size_t)i]);
+            }
+        }
+        return c > 0 && (8 * c >= run || waited >= kMaxWait);
+    }
+
+    // This step's tables; no job changes.  In a mixed step the R jobs get
+    // their windows and the C jobs the full step's tables (or, when the step
+    // does not take them, nothing: they wait).
     void plan(StepPlan* p, StepKind kind = kFullStep) const {
+        const bool with_c = kind == kFullStep || take_c();
         const size_t S = (size_t)slots_, A = (size_t)as_;
         p->off.assign(S + 1, 0);
         p->ops.clear();
@@ -191,6 +218,8 @@ class Scheduler {
         p->maxw.assign(S, -1);
         p->maxt.assign(S, -1);
         p->any_trim = false;
+        p->isc.assign(S, 0);
+        p->any_c = false;
         p->rp.assign(S * 2 * A, 0);
         p->p_or.assign(2 * A, 0);
         p->next.assign(S, -1);
@@ -209,9 +238,8 @@ class Scheduler {
             p->alive[i] = 1;
             const Job& j = active_[i];
             int64_t next = j.next;
-            if (kind == kSortedStep) {
-                p->next[i] = next;  // waits for the next full step, or settled after the device
-                if (!sorted_ready((int)i)) continue;
+            if (kind == kSortedStep && sorted_ready((int)i) && !fresh((int)i)) {
+                p->next[i] = next;  // settled after the device
                 // the window: rotations next, next - 1, ... that keep the order
                 int w = 0;
                 const size_t ja = (size_t)j.job_as;
@@ -235,6 +263,14 @@ class Scheduler {
                 p->maxt[i] = j.max_terms;
                 p->any_trim |= j.max_terms >= 0;
                 continue;
+            }
+            if (kind == kSortedStep) {
+                if (!with_c) {
+                    p->next[i] = next;  // waits, terms untouched
+                    continue;
+                }
+                p->isc[i] = 1;
+                p->any_c = true;
             }
             while (next >= 0 && j.kind[(size_t)next] != 3) {
                 const int64_t l = next--;
@@ -283,13 +319,13 @@ class Scheduler {
     // slot (1 .. wn[slot] for an advancing job, 0 otherwise): the jobs' next
     // ops and the finishing jobs follow.  False when a report is out of range.
     bool settle(StepPlan* p, const uint8_t* adv) const {
-        p->finishing.clear();
         for (size_t i = 0; i < (size_t)slots_; ++i) {
             if (state_[i] != kRunning) continue;
             const int a = adv[i], w = i < p->wn.size() ? p->wn[i] : 0;
             if ((w == 0 && a != 0) || (w > 0 && (a < 1 || a > w))) return false;
+            if (w == 0) continue;  // a C job: planned
             p->next[i] = active_[i].next - a;
-            if (w > 0 && p->next[i] < 0) p->finishing.push_back((int)i);
+            if (p->next[i] < 0) p->finishing.push_back((int)i);
         }
         return true;
     }
@@ -299,6 +335,12 @@ class Scheduler {
     // planned next op, and finishing jobs' slots start draining.  Returns
     // true when nothing runs any more (the batch can then be emptied).
     bool commit(const StepPlan& p) {
+        for (size_t i = 0; i < (size_t)slots_; ++i) {
+            const bool waited = state_[i] == kRunning && i < p.wn.size() && p.wn[i] == 0 &&
+                                !(i < p.isc.size() && p.isc[i]) && p.next[i] == active_[i].next && !p.wn.empty();
+            wait_[i] = waited ? (uint8_t)std::min(255, wait_[i] + 1) : 0;
+        }
+        for (auto& f : fresh_) f = 0;
         for (auto& s : state_)
             if (s == kDraining) s = kFree;
         for (size_t i = 0; i < (size_t)slots_; ++i)
@@ -320,6 +362,8 @@ class Scheduler {
     std::deque<Job> queue_;
     std::vector<Job> active_;  // by slot
     std::vector<int> state_;
+    std::vector<uint8_t> fresh_ = std::vector<uint8_t>((size_t)slots_, 0);
+    std::vector<uint8_t> wait_ = std::vector<uint8_t>((size_t)slots_, 0);  // mixed steps waited at a C op
 };
 
 }  // namespace sched

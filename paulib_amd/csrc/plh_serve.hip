@@ -44,10 +44,6 @@ namespace {
 
 constexpr int kIpt = 16;
 constexpr uint64_t kTile = (uint64_t)kBlock * kIpt;
-// Sorted steps between full steps at most: a job waiting at a Clifford op
-// (or admitted) waits at most this many steps.
-constexpr int kMaxSortedSteps = 48;
-
 // Per-step job tables (device, one upload per step), one entry per slot.
 struct StepTables {
     const uint32_t* cl_off;   // [slots + 1] range of the slot's Clifford ops
@@ -60,6 +56,8 @@ struct StepTables {
     double* eps;              // collation filters of this step
     int64_t* maxw;
     uint8_t* alive;
+    const uint8_t* isc;      // mixed steps: 1 for a C job (collated apart)
+    const uint8_t* alive_r;  // mixed steps: alive and an R job
 };
 
 // Tables of the gate set (16 nibbles of new local bits, 16 sign bits per
@@ -168,9 +166,9 @@ PLH_DEV uint32_t times_p(uint64_t* x, uint64_t* z, uint32_t f, const uint64_t* p
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_batch_gates(uint64_t* __restrict__ X, uint64_t* __restrict__ Z,
                                                         uint8_t* __restrict__ F, const uint32_t* __restrict__ J,
-                                                        int64_t stride, int64_t n, StepTables tb) {
-    const uint64_t base = (uint64_t)blockIdx.x * kTile;
-    for (int k = 0; k < kIpt; ++k) {
+                                                        int64_t stride, int64_t n, StepTables tb, int ipt) {
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * (uint64_t)ipt;
+    for (int k = 0; k < ipt; ++k) {
         const uint64_t t = base + (uint64_t)k * kBlock + threadIdx.x;
         if (t >= (uint64_t)n) break;
         const uint32_t job = J[t];
@@ -267,7 +265,9 @@ struct plh_server {
     plh_ctx* ctx = nullptr;
     int as = 2;
     bool keep_terms = false;
-    plh_sum* batch = nullptr;  // job-plane sum holding every job in flight
+    plh_sum* batch = nullptr;     // job-plane sum holding every job in flight (sorted after each step)
+    plh_sum* incoming = nullptr;  // observables admitted for the coming step
+    plh_sum* side = nullptr;      // a mixed step's C jobs, collated apart
     plh::sched::Scheduler* sched = nullptr;
     std::deque<plh::Result> done;
     std::unordered_map<int64_t, plh::Result> kept;
@@ -281,7 +281,6 @@ struct plh_server {
     uint8_t* adv_dev = nullptr;
     std::chrono::steady_clock::time_point t0;
     int64_t steps = 0;
-    int since_full = 0;  // sorted steps since the last full step
     // set when a step's device work failed part-way: the batch may hold
     // half-applied gates, so the server refuses further steps
     bool failed = false;
@@ -501,8 +500,83 @@ int trim_jobs(plh_ctx* ctx, plh_sum* B, const int64_t* d_maxt, uint32_t slots) {
     return PLH_OK;
 }
 
-// Append a job's observable to the batch under its slot.  On failure the
-// batch holds what it held before (its storage may have grown).
+// ---- a mixed step's C jobs: their terms gathered out of the batch (job
+// order kept).  cpre = exclusive scan over slots of the C jobs' term counts
+// (one workgroup), its total posted to the host.
+__global__ __launch_bounds__(kBlock) void k_c_sizes(const uint32_t* __restrict__ joff,
+                                                    const uint8_t* __restrict__ isc, uint32_t slots,
+                                                    uint64_t* __restrict__ cpre, uint64_t* mbox, uint64_t seq) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    uint64_t carry = 0;
+    for (uint32_t c0 = 0; c0 < slots; c0 += kBlock) {
+        const uint32_t t = c0 + threadIdx.x;
+        const uint64_t v = (t < slots && isc[t]) ? (uint64_t)(joff[t + 1] - joff[t]) : 0ull;
+        uint64_t tot;
+        const uint64_t ex = block_exclusive_scan(v, tot, sh);
+        if (t < slots) cpre[t] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        cpre[slots] = carry;
+        post_write(mbox, seq, 1, &carry);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_c_gather(SumView b, const uint32_t* __restrict__ J, int as, uint32_t n,
+                                                     const uint32_t* __restrict__ joff,
+                                                     const uint8_t* __restrict__ isc,
+                                                     const uint64_t* __restrict__ cpre, uint64_t* __restrict__ ox,
+                                                     uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
+                                                     double* __restrict__ ov, uint32_t* __restrict__ oj, int64_t os,
+                                                     uint64_t cap) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t job = J[i];
+    if (!isc[job]) return;
+    const uint64_t o = cpre[job] + (i - joff[job]);
+    if (o >= cap) return;  // never: cpre counts these terms; keeps writes in bounds
+    for (int w = 0; w < as; ++w) {
+        ox[(int64_t)w * os + o] = b.x[(int64_t)w * b.stride + i];
+        oz[(int64_t)w * os + o] = b.z[(int64_t)w * b.stride + i];
+    }
+    of[o] = b.f[i];
+    ov[o] = b.v[i];
+    oj[o] = job;
+}
+
+// T = the terms of the batch's C jobs (isc, device), in job order, with room
+// for `extra` more terms.
+int gather_jobs(plh_ctx* ctx, const plh_sum* B, const uint8_t* isc, uint32_t slots, int64_t extra, plh_sum* T) {
+    hipStream_t st = ctx->stream;
+    const uint64_t n = (uint64_t)B->n;
+    void* p;
+    const size_t sj = ((size_t)slots + 1 + 63) & ~(size_t)63;
+    PLH_TRY(ctx->ws.get(kWsTmpSum, sj * 4 + sj * 8 + 256, &p));
+    uint32_t* joff = (uint32_t*)p;
+    uint64_t* cpre = (uint64_t*)((char*)p + sj * 4);
+    k_job_offsets<<<blocks_for((uint64_t)slots + 1, 256), 256, 0, st>>>(B->j, (uint32_t)n, slots, joff);
+    Post post;
+    PLH_TRY(post_prepare(ctx, &post));
+    k_c_sizes<<<1, kBlock, 0, st>>>(joff, isc, slots, cpre, post.mbox, post.seq);
+    PLH_HIP(hipGetLastError());
+    uint64_t total = 0;
+    PLH_TRY(post_wait(ctx, post, 1, &total));
+    T->n = 0;
+    PLH_TRY(sum_reserve(T, std::max<int64_t>((int64_t)total + extra, 1), 0));
+    if (total) {
+        k_c_gather<<<blocks_for(n, kBlock), kBlock, 0, st>>>(SumView{B->x, B->z, B->f, B->v, B->cap}, B->j, B->as,
+                                                              (uint32_t)n, joff, isc, cpre, T->x, T->z, T->f, T->v,
+                                                              T->j, T->cap, (uint64_t)T->cap);
+        PLH_HIP(hipGetLastError());
+    }
+    T->n = (int64_t)total;
+    T->canonical = false;
+    T->sorted = false;
+    return PLH_OK;
+}
+
+// Append a job's observable to the incoming sum under its slot.  On failure
+// it holds what it held before (its storage may have grown).
 int admit(plh_server* s, const Job& j, int slot) {
     const int64_t n = j.n0;
     if (n == 0) return PLH_OK;
@@ -537,7 +611,7 @@ int admit(plh_server* s, const Job& j, int slot) {
         rc = PLH_ERR_HIP;
     } else {
         tmp.n = n;
-        rc = op_append(s->ctx, s->batch, &tmp);
+        rc = op_append(s->ctx, s->incoming, &tmp);
     }
     if (hipStreamSynchronize(st) != hipSuccess && rc == PLH_OK) rc = PLH_ERR_HIP;  // staging is host memory of j
     sum_free_storage(&tmp);
@@ -557,7 +631,9 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
         const void* src;
         size_t bytes;
     };
-    constexpr int kParts = 17;
+    constexpr int kParts = 19;
+    std::vector<uint8_t> alive_r(pl.alive.size(), 0);
+    for (size_t i = 0; i < alive_r.size(); ++i) alive_r[i] = pl.alive[i] && !(i < pl.isc.size() && pl.isc[i]);
     const Part parts[kParts] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
                                 {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
                                 {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
@@ -566,7 +642,8 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
                                 {pl.wn.data(), pl.wn.size()},       {pl.wk.data(), pl.wk.size()},
                                 {pl.wp.data(), pl.wp.size() * 8},   {pl.wpsf.data(), pl.wpsf.size()},
                                 {pl.wc.data(), pl.wc.size() * 8},   {pl.ws.data(), pl.ws.size() * 8},
-                                {pl.maxt.data(), pl.maxt.size() * 8}};
+                                {pl.maxt.data(), pl.maxt.size() * 8}, {pl.isc.data(), pl.isc.size()},
+                                {alive_r.data(), alive_r.size()}};
     static_assert(sched::kWindow == kJobWindow, "scheduler and engine agree on the window depth");
     size_t total = 0;
     for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
@@ -597,51 +674,72 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
     tb.eps = (double*)dptr[7];
     tb.maxw = (int64_t*)dptr[8];
     tb.alive = (uint8_t*)dptr[9];
+    tb.isc = (const uint8_t*)dptr[17];
+    tb.alive_r = (const uint8_t*)dptr[18];
     PLH_HIP(hipMemcpyAsync(s->d_step, s->h_step.data(), total, hipMemcpyHostToDevice, st));
 
-    const int64_t n = B->n;
-    if (n > 0 && kind == sched::kSortedStep) {
-        // ---- every advancing job's rotation without a sort; the waiting jobs
-        // pass through, drained jobs leave
-        CollateSpec sc;
-        sc.mode = kFold | kDropZero | kMaxWeight | kCutoff;
-        sc.max_weight = -1;
-        sc.eps = 0.0;
-        sc.job_eps = tb.eps;
-        sc.job_maxw = tb.maxw;
-        sc.job_alive = tb.alive;
+    CollateSpec sc;
+    sc.mode = kFold | kDropZero | kMaxWeight | kCutoff;
+    sc.max_weight = -1;
+    sc.eps = 0.0;
+    sc.job_eps = tb.eps;
+    sc.job_maxw = tb.maxw;
+    sc.job_alive = tb.alive;
+    // gates (Cliffords and fast-path rotations in place) and one collation
+    // (general rotations' new terms, fold, combine, filters, drained jobs
+    // out) over a job-plane sum
+    auto gates_and_collate = [&](plh_sum* X) -> int {
+        // up to kIpt terms per thread, fewer while that leaves under 8192
+        // workgroups (a small sum of C jobs would not fill the device)
+        int ipt = kIpt;
+        while (ipt > 1 && blocks_for((uint64_t)X->n, (uint64_t)kBlock * ipt) < 8192) ipt /= 2;
+        const uint64_t nb = blocks_for((uint64_t)X->n, (uint64_t)kBlock * ipt);
+        {
+            ProfScope ps(ctx, "batch_gates", (double)X->n * (2.0 * (16.0 * AS + 1.0) + 4.0));
+            if (AS == 1)
+                k_batch_gates<1><<<dim3((unsigned)nb), kBlock, 0, st>>>(X->x, X->z, X->f, X->j, X->cap, X->n, tb, ipt);
+            else
+                k_batch_gates<2><<<dim3((unsigned)nb), kBlock, 0, st>>>(X->x, X->z, X->f, X->j, X->cap, X->n, tb, ipt);
+            PLH_HIP(hipGetLastError());
+        }
+        if (pl.any_general) {
+            JobRotTables jt{tb.rot_kind, tb.rot_p, tb.rot_psf, tb.rot_c, tb.rot_s};
+            return collate_rotation_jobs(ctx, X, jt, pl.p_or.data(), sc, X);
+        }
+        return collate_terms(ctx, X, sc, X);
+    };
+    if (kind == sched::kFullStep) {
+        // ---- every job: the admitted observables join the batch, then gates
+        // and one collation
+        if (s->incoming->n > 0) {
+            PLH_TRY(op_append(ctx, B, s->incoming));
+            s->incoming->n = 0;
+        }
+        if (B->n > 0) PLH_TRY(gates_and_collate(B));
+    } else {
+        // ---- C jobs (a Clifford op next, or admitted): their terms apart,
+        // gates and a collation
+        plh_sum* T = s->side;
+        T->n = 0;
+        if (pl.any_c) {
+            PLH_TRY(gather_jobs(ctx, B, tb.isc, (uint32_t)s->sched->slots(), s->incoming->n, T));
+            if (s->incoming->n > 0) {
+                PLH_TRY(op_append(ctx, T, s->incoming));
+                s->incoming->n = 0;
+            }
+            if (T->n > 0) PLH_TRY(gates_and_collate(T));
+        }
+        // ---- R jobs: each one's rotation window without a sort; the C jobs'
+        // collated terms go between them (the C and drained jobs' terms in the
+        // batch are dropped)
+        CollateSpec sr = sc;
+        sr.job_alive = tb.alive_r;
         const JobWindows jw{(const uint8_t*)dptr[10], (const uint8_t*)dptr[11], (const uint64_t*)dptr[12],
                             (const uint8_t*)dptr[13], (const double*)dptr[14], (const double*)dptr[15]};
-        PLH_TRY(rotate_sorted_jobs(ctx, B, jw, s->sched->slots(), sc, s->adv_dev));
+        PLH_TRY(rotate_sorted_jobs(ctx, B, jw, s->sched->slots(), sr, s->adv_dev, T->n > 0 ? T : nullptr));
         if (!s->sched->settle(&pl, s->adv_host)) {
             set_error("plh_server_step: the device reported an out-of-range rotation count");
             return PLH_ERR_HIP;
-        }
-    } else if (n > 0) {
-        // ---- gates: Cliffords + fast rotations in place
-        const uint64_t nb = blocks_for((uint64_t)n, kTile);
-        {
-            ProfScope ps(ctx, "batch_gates", (double)n * (2.0 * (16.0 * AS + 1.0) + 4.0));
-            if (AS == 1)
-                k_batch_gates<1><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb);
-            else
-                k_batch_gates<2><<<dim3((unsigned)nb), kBlock, 0, st>>>(B->x, B->z, B->f, B->j, B->cap, n, tb);
-            PLH_HIP(hipGetLastError());
-        }
-        // ---- one collation for every job: the general rotations' new terms,
-        // fold, combine, zero-drop, per-job filters, drained jobs out
-        CollateSpec sc;
-        sc.mode = kFold | kDropZero | kMaxWeight | kCutoff;
-        sc.max_weight = -1;
-        sc.eps = 0.0;
-        sc.job_eps = tb.eps;
-        sc.job_maxw = tb.maxw;
-        sc.job_alive = tb.alive;
-        if (pl.any_general) {
-            JobRotTables jt{tb.rot_kind, tb.rot_p, tb.rot_psf, tb.rot_c, tb.rot_s};
-            PLH_TRY(collate_rotation_jobs(ctx, B, jt, pl.p_or.data(), sc, B));
-        } else {
-            PLH_TRY(collate_terms(ctx, B, sc, B));
         }
     }
 
@@ -747,14 +845,17 @@ int plh_server_create(plh_ctx* ctx, int array_size, int max_jobs, int64_t hbm_bu
         set_error("plh_server_create: pinned report buffer");
         rc = PLH_ERR_HIP;
     }
-    if (rc == PLH_OK) rc = plh_sum_create(ctx, array_size, 64 * array_size, 0, &s->batch);
-    if (rc == PLH_OK) {
-        s->batch->has_job = true;
-        sum_free_storage(s->batch);
-        rc = sum_reserve(s->batch, 4096, 0);
+    for (plh_sum** ps : {&s->batch, &s->incoming, &s->side}) {
+        if (rc == PLH_OK) rc = plh_sum_create(ctx, array_size, 64 * array_size, 0, ps);
+        if (rc == PLH_OK) {
+            (*ps)->has_job = true;
+            sum_free_storage(*ps);
+            rc = sum_reserve(*ps, 4096, 0);
+        }
     }
     if (rc != PLH_OK) {
-        if (s->batch) plh_sum_destroy(s->batch);
+        for (plh_sum* q : {s->batch, s->incoming, s->side})
+            if (q) plh_sum_destroy(q);
         if (s->adv_host) (void)hipHostFree(s->adv_host);
         delete s->sched;
         delete s;
@@ -769,6 +870,8 @@ int plh_server_destroy(plh_server* s) {
     if (!s) return PLH_OK;
     (void)hipStreamSynchronize(s->ctx->stream);
     plh_sum_destroy(s->batch);
+    plh_sum_destroy(s->incoming);
+    plh_sum_destroy(s->side);
     (void)hipFree(s->d_step);
     if (s->adv_host) (void)hipHostFree(s->adv_host);
     delete s->sched;
@@ -849,16 +952,13 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
     sched::Scheduler& sc = *s->sched;
     plh_sum* B = s->batch;
 
-    // ---- the step's kind (Scheduler::choose_step)
-    const sched::StepKind kind = sc.choose_step(B->sorted, B->n, s->since_full, kMaxSortedSteps);
-    const bool full = kind == sched::kFullStep;
-
-    // ---- admission (full steps only: new terms are appended unsorted): a
-    // free slot, and the batch with the new job fits the HBM budget.  A job
-    // that cannot be appended stays queued; the server stays usable (the
-    // batch holds what it held).
-    if (full) {
-        int64_t in_flight = B->n;
+    // ---- admission: a free slot, and the batch with the new job fits the
+    // HBM budget.  The job's observable goes to the incoming sum, which the
+    // step consumes (a full step appends it to the batch, a mixed step
+    // collates it with the C jobs).  A job that cannot be staged stays
+    // queued; the server stays usable.
+    {
+        int64_t in_flight = B->n + s->incoming->n;
         for (int slot; (slot = sc.admission_slot(in_flight)) >= 0;) {
             const int64_t n0 = sc.queue().front().n0;
             PLH_TRY(admit(s, sc.queue().front(), slot));
@@ -866,6 +966,8 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
             in_flight += n0;
         }
     }
+    // ---- the step's kind (Scheduler::choose_step)
+    const sched::StepKind kind = sc.choose_step(B->sorted, B->n);
 
     // ---- the step: planned, run on the device, then committed
     sched::StepPlan pl;
@@ -891,7 +993,6 @@ int plh_server_step(plh_server* s, int* active_out, int* queued_out) {
         }
     }
     if (sc.commit(pl)) B->n = 0;  // nothing running: drained terms go without a collation
-    s->since_full = full ? 0 : s->since_full + 1;
     ++s->steps;
     if (active_out) *active_out = sc.running();
     if (queued_out) *queued_out = (int)sc.queue().size();

@@ -103,23 +103,30 @@ struct Replay {
     std::mt19937_64 rng;
     double p_fail = 0.0;  // probability of a device error at each phase
     std::map<int, int64_t> shadow;       // slot -> terms of its job in the batch
+    std::map<int, int64_t> pending;      // slot -> terms of its admitted job in the incoming sum
+    plh_sum incoming, side;              // plh_serve.hip: admitted observables; a mixed step's C jobs
     std::map<int, int64_t> slot_handle;  // slot -> handle of the job whose terms those are
     std::map<int64_t, const JobSpec*> spec;
     std::map<int64_t, int64_t> finished;  // handle -> final terms
     std::map<int64_t, int64_t> applied;   // handle -> next op expected (Heisenberg order)
     int admission_errors = 0;
-    int since_full = 0;
     int sorted_steps = 0;
-    static constexpr int kMaxSorted = 48;  // plh_serve.hip kMaxSortedSteps
 
     Replay(int as, int slots, int64_t budget, uint64_t seed) : sc(as, slots, budget), rng(seed) {
         ctx.heap.set_device(0);
         batch.ctx = &ctx;
-        batch.as = as;
-        batch.nq = 64 * as;
-        batch.has_job = true;
+        for (plh_sum* q : {&batch, &incoming, &side}) {
+            q->ctx = &ctx;
+            q->as = as;
+            q->nq = 64 * as;
+            q->has_job = true;
+        }
     }
-    ~Replay() { plh::sum_free_storage(&batch); }
+    ~Replay() {
+        plh::sum_free_storage(&batch);
+        plh::sum_free_storage(&incoming);
+        plh::sum_free_storage(&side);
+    }
 
     bool injected() { return p_fail > 0.0 && std::uniform_real_distribution<double>(0, 1)(rng) < p_fail; }
     int device_error() {
@@ -148,7 +155,7 @@ struct Replay {
     }
 
     // plh_serve.hip admit(): a staging sum, uploads and the slot fill into
-    // it, then op_append onto the batch.
+    // it, then op_append onto the incoming sum.
     int admit(const Job& j, int slot) {
         const int64_t n = j.n0 * scale;
         if (n == 0) return PLH_OK;
@@ -168,19 +175,74 @@ struct Replay {
             rc = device_error();
         } else {
             tmp.n = n;
-            rc = plh::op_append(&ctx, &batch, &tmp);
+            rc = plh::op_append(&ctx, &incoming, &tmp);
         }
         plh::sum_free_storage(&tmp);
         return rc;
     }
 
-    // step_device(): gates, one collation (with_output + collate_run's
-    // storage), the finished jobs' ranges.
+    // plh_serve.hip step_device(): a full step appends the incoming sum to
+    // the batch, then gates and one collation; a mixed step gathers the C
+    // jobs' terms with the incoming ones into the side sum and collates it,
+    // then rotates the R jobs without a sort, the side sum placed between
+    // them.  The finished jobs' ranges are read last.
+    // One collation's memory (with_output + collate_run's storage): X is
+    // replaced by `out` terms.
+    int collate_model(plh_sum& X, int64_t n_valid, int64_t out) {
+        const int64_t n = X.n;
+        if (injected()) return device_error();
+        sim::check_live(X.x, (size_t)n * 8, "gates");
+        sim::check_live(X.f, (size_t)n, "gates");
+        sim::check_live(X.j, (size_t)n * 4, "gates");
+        void* p = nullptr;
+        PLH_TRY(ctx.ws.get(plh::kWsMisc, 4096, &p));
+        PLH_TRY(ctx.ws.get(plh::kWsPackL, (size_t)n * (2 * sc.as() + 2) * 8, &p));
+        PLH_TRY(ctx.ws.get(plh::kWsIds0, (size_t)n_valid * 8, &p));
+        PLH_TRY(ctx.ws.get(plh::kWsIds1, (size_t)n_valid * 8, &p));
+        if (injected()) return device_error();
+        plh_sum tmp;
+        tmp.ctx = &ctx;
+        tmp.as = X.as;
+        tmp.nq = X.nq;
+        tmp.has_job = true;
+        auto run = [&]() -> int {
+            PLH_TRY(plh::sum_reserve(&tmp, std::max<int64_t>(n_valid, 1), 0));
+            sim::check_live(X.x, (size_t)n * 8, "leaf reads");
+            if (n_valid) sim::check_live(tmp.x, (size_t)n_valid * 8, "leaf writes");
+            if (n_valid) sim::check_live(tmp.j, (size_t)n_valid * 4, "leaf writes");
+            if (injected()) return device_error();
+            if (out != n_valid) {  // collapsed runs or dropped terms: compaction
+                plh_sum c2;
+                c2.ctx = &ctx;
+                c2.as = tmp.as;
+                c2.nq = tmp.nq;
+                c2.has_job = true;
+                const int rc = plh::sum_reserve(&c2, std::max<int64_t>(out, 1), 0);
+                if (rc != PLH_OK) {
+                    plh::sum_free_storage(&c2);
+                    return rc;
+                }
+                if (out) sim::check_live(c2.v, (size_t)out * 8, "compaction writes");
+                plh::sum_take_storage(&tmp, &c2);
+            }
+            tmp.n = out;
+            return PLH_OK;
+        };
+        const int rc = run();
+        if (rc != PLH_OK) {
+            plh::sum_free_storage(&tmp);
+            return rc;
+        }
+        plh::sum_take_storage(&X, &tmp);
+        X.n = out;
+        X.sorted = true;
+        return PLH_OK;
+    }
+
     int device(StepPlan& pl, plh::sched::StepKind kind, std::map<int, int64_t>* after) {
-        const int64_t n = batch.n;
         if (kind == plh::sched::kSortedStep) {
-            // the device's report: each advancing job consumes 1 .. wn of its
-            // window (here drawn at random: any count is a valid report)
+            // the device's report: each R job consumes 1 .. wn of its window
+            // (here drawn at random: any count is a valid report)
             std::vector<uint8_t> adv((size_t)sc.slots(), 0);
             for (int i = 0; i < sc.slots(); ++i)
                 if (pl.wn[(size_t)i] > 0)
@@ -188,11 +250,10 @@ struct Replay {
             if (!sc.settle(&pl, adv.data())) fail("settle refused a valid report");
             verify_settled(pl, adv);
         }
-        // the terms each slot holds once this step's collation has run
+        // the terms each running slot holds once this step has run
         after->clear();
-        int64_t n_valid = n, out = 0;
-        for (auto& kv : shadow) {
-            const int slot = kv.first;
+        int64_t out = 0, out_c = 0, in_c = 0, valid_c = 0;
+        for (int slot = 0; slot < sc.slots(); ++slot) {
             if (sc.state(slot) != plh::sched::kRunning) continue;  // draining: dropped
             const Job& j = sc.job(slot);
             const JobSpec* sp = spec.at(j.handle);
@@ -200,12 +261,42 @@ struct Replay {
             if (c < 0) fail("the step of a job ends at op %lld, which is neither a rotation nor op 0", pl.next[(size_t)slot] + 1);
             (*after)[slot] = c;
             out += c * scale;
-            if (pl.rk[(size_t)slot] == 4) n_valid += kv.second * scale;  // candidate new terms
+            const bool is_c = kind == plh::sched::kFullStep || pl.isc[(size_t)slot];
+            if (is_c) {
+                const int64_t held = (shadow.count(slot) ? shadow.at(slot) : 0) + (pending.count(slot) ? pending.at(slot) : 0);
+                in_c += held * scale;
+                out_c += c * scale;
+                valid_c += (pl.rk[(size_t)slot] == 4 ? 2 : 1) * held * scale;  // candidate new terms
+            }
         }
-        if (n > 0 && kind == plh::sched::kSortedStep) {
-            // plh_rotate.hip rotate_sorted_jobs: scratch, then the output block
+        if (kind == plh::sched::kFullStep) {
+            PLH_TRY(plh::op_append(&ctx, &batch, &incoming));
+            incoming.n = 0;
+            pending.clear();
+            if (batch.n > 0) {
+                const int64_t nv = batch.n + (valid_c - in_c);
+                PLH_TRY(collate_model(batch, nv, out));
+            }
+        } else {
+            // C jobs apart: gather, the incoming terms, collation
+            side.n = 0;
+            if (pl.any_c) {
+                void* p = nullptr;
+                PLH_TRY(ctx.ws.get(plh::kWsTmpSum, (size_t)sc.slots() * 16 + 1024, &p));
+                if (injected()) return device_error();
+                const int64_t gathered = in_c - incoming.n;
+                PLH_TRY(plh::sum_reserve(&side, std::max<int64_t>(gathered + incoming.n, 1), 0));
+                if (gathered) sim::check_live(side.x, (size_t)gathered * 8, "gather writes");
+                side.n = gathered;
+                PLH_TRY(plh::op_append(&ctx, &side, &incoming));
+                incoming.n = 0;
+                pending.clear();
+                if (side.n > 0) PLH_TRY(collate_model(side, valid_c, out_c));
+            }
+            // R jobs: plh_rotate.hip rotate_sorted_jobs, the side sum placed
+            const int64_t n = batch.n;
             void* p = nullptr;
-            PLH_TRY(ctx.ws.get(plh::kWsRotA, (size_t)n + (size_t)n / 2 + 4096 + (size_t)sc.slots() * 64, &p));
+            PLH_TRY(ctx.ws.get(plh::kWsRotA, (size_t)n + (size_t)n / 2 + 4096 + (size_t)sc.slots() * 600, &p));
             PLH_TRY(ctx.ws.get(plh::kWsRotN, (size_t)n * (16 * sc.as() + 17) + 1536, &p));
             if (injected()) return device_error();
             sim::check_live(batch.x, (size_t)n * 8, "sorted rotation reads");
@@ -221,58 +312,10 @@ struct Replay {
                 return rc;
             }
             if (out) sim::check_live(tmp.v, (size_t)out * 8, "sorted rotation writes");
+            if (side.n) sim::check_live(side.v, (size_t)side.n * 8, "placed terms");
             if (injected()) {
                 plh::sum_free_storage(&tmp);
                 return device_error();
-            }
-            plh::sum_take_storage(&batch, &tmp);
-            batch.n = out;
-            batch.sorted = true;
-        } else if (n > 0) {
-            if (injected()) return device_error();
-            sim::check_live(batch.x, (size_t)n * 8, "gates");
-            sim::check_live(batch.f, (size_t)n, "gates");
-            sim::check_live(batch.j, (size_t)n * 4, "gates");
-            // collation scratch (plh_collate.hip collate_run and pack_sum)
-            void* p = nullptr;
-            PLH_TRY(ctx.ws.get(plh::kWsMisc, 4096, &p));
-            PLH_TRY(ctx.ws.get(plh::kWsPackL, (size_t)n * (2 * sc.as() + 2) * 8, &p));
-            PLH_TRY(ctx.ws.get(plh::kWsIds0, (size_t)n_valid * 8, &p));
-            PLH_TRY(ctx.ws.get(plh::kWsIds1, (size_t)n_valid * 8, &p));
-            if (injected()) return device_error();
-            // with_output: the result goes to a scratch sum that replaces the batch
-            plh_sum tmp;
-            tmp.ctx = &ctx;
-            tmp.as = batch.as;
-            tmp.nq = batch.nq;
-            tmp.has_job = true;
-            auto run = [&]() -> int {
-                PLH_TRY(plh::sum_reserve(&tmp, n_valid, 0));
-                sim::check_live(batch.x, (size_t)n * 8, "leaf reads");
-                sim::check_live(tmp.x, (size_t)n_valid * 8, "leaf writes");
-                sim::check_live(tmp.j, (size_t)n_valid * 4, "leaf writes");
-                if (injected()) return device_error();
-                if (out != n_valid) {  // collapsed runs or dropped terms: compaction
-                    plh_sum c2;
-                    c2.ctx = &ctx;
-                    c2.as = tmp.as;
-                    c2.nq = tmp.nq;
-                    c2.has_job = true;
-                    const int rc = plh::sum_reserve(&c2, out, 0);
-                    if (rc != PLH_OK) {
-                        plh::sum_free_storage(&c2);
-                        return rc;
-                    }
-                    if (out) sim::check_live(c2.v, (size_t)out * 8, "compaction writes");
-                    plh::sum_take_storage(&tmp, &c2);
-                }
-                tmp.n = out;
-                return PLH_OK;
-            };
-            const int rc = run();
-            if (rc != PLH_OK) {
-                plh::sum_free_storage(&tmp);
-                return rc;
             }
             plh::sum_take_storage(&batch, &tmp);
             batch.n = out;
@@ -300,28 +343,29 @@ struct Replay {
     // One plh_server_step: returns its status.
     int step() {
         if (failed) return PLH_ERR_STATE;
-        const plh::sched::StepKind kind = sc.choose_step(batch.sorted, batch.n, since_full, kMaxSorted);
-        const bool full = kind == plh::sched::kFullStep;
-        if (!full && !batch.sorted) fail("a sorted step on an unsorted batch");
-        int64_t in_flight = batch.n / scale;  // the scheduler counts unscaled terms
-        for (int slot; full && (slot = sc.admission_slot(in_flight)) >= 0;) {
-            if (shadow.count(slot)) fail("slot %lld reused while its previous job's terms are in the batch", slot);
+        int64_t in_flight = (batch.n + incoming.n) / scale;  // the scheduler counts unscaled terms
+        for (int slot; (slot = sc.admission_slot(in_flight)) >= 0;) {
+            if (shadow.count(slot) || pending.count(slot))
+                fail("slot %lld reused while its previous job's terms are in the batch", slot);
             const Job& q = sc.queue().front();
-            const int64_t h = q.handle, n0 = q.n0, nb = batch.n;
+            const int64_t h = q.handle, n0 = q.n0, nb = batch.n, ni = incoming.n;
             const int rc = admit(q, slot);
             if (rc != PLH_OK) {
-                // the job stays queued, the batch keeps its terms
+                // the job stays queued, the batch and the incoming sum keep their terms
                 if (sc.queue().empty() || sc.queue().front().handle != h) fail("a failed admission lost the job");
-                if (batch.n != nb) fail("a failed admission changed the batch count");
+                if (batch.n != nb || incoming.n != ni) fail("a failed admission changed the batch or incoming count");
                 check_batch("after a failed admission");
                 ++admission_errors;
                 return rc;
             }
             sc.admitted(slot);
-            shadow[slot] = n0;
+            pending[slot] = n0;
             slot_handle[slot] = h;
             in_flight += n0;
         }
+        const plh::sched::StepKind kind = sc.choose_step(batch.sorted, batch.n);
+        const bool full = kind == plh::sched::kFullStep;
+        if (!full && !batch.sorted) fail("a mixed step on an unsorted batch");
         StepPlan pl;
         sc.plan(&pl, kind);
         verify_plan(pl, kind);
@@ -349,9 +393,8 @@ struct Replay {
             batch.n = 0;
             shadow.clear();
         }
-        since_full = full ? 0 : since_full + 1;
         sorted_steps += !full;
-        if (since_full > kMaxSorted) fail("more sorted steps in a row than the bound");
+        if (incoming.n != 0 || !pending.empty()) fail("a step left admitted terms unconsumed");
         check_batch("after a step");
         return PLH_OK;
     }
@@ -362,9 +405,10 @@ struct Replay {
         for (int i = 0; i < sc.slots(); ++i) {
             if (sc.state(i) != plh::sched::kRunning) continue;
             const Job& j = sc.job(i);
-            if (pl.next[(size_t)i] != j.next - (int64_t)adv[(size_t)i]) fail("job %lld settled at the wrong op", j.handle);
+            if (pl.wn[(size_t)i] > 0 && pl.next[(size_t)i] != j.next - (int64_t)adv[(size_t)i])
+                fail("job %lld settled at the wrong op", j.handle);
             const bool fin = std::find(pl.finishing.begin(), pl.finishing.end(), i) != pl.finishing.end();
-            if (fin != (pl.wn[(size_t)i] > 0 && pl.next[(size_t)i] < 0)) fail("finishing flag of job %lld", j.handle);
+            if (fin != (pl.next[(size_t)i] < 0)) fail("finishing flag of job %lld", j.handle);
         }
         std::vector<uint8_t> bad = adv;
         for (size_t i = 0; i < bad.size(); ++i)
@@ -386,15 +430,22 @@ struct Replay {
             auto it = applied.find(j.handle);
             const int64_t expect_next = it == applied.end() ? (int64_t)j.kind.size() - 1 : it->second;
             if (j.next != expect_next) fail("job %lld resumes at op %lld", j.handle, j.next);
-            if (kind == plh::sched::kSortedStep) {
-                if (pl.off[(size_t)i] != pl.off[(size_t)i + 1]) fail("Clifford ops in a sorted step (job %lld)", j.handle);
-                if (pl.rk[(size_t)i] != 0 || pl.next[(size_t)i] != j.next || !pl.alive[(size_t)i] || !pl.finishing.empty())
-                    fail("a sorted step's plan decides for job %lld before the device", j.handle);
-                if (!sc.sorted_ready(i)) {  // waits: untouched
-                    if (pl.wn[(size_t)i] != 0 || pl.maxw[(size_t)i] != -1 || pl.eps[(size_t)i] != 0.0)
-                        fail("a waiting job is not left alone (job %lld)", j.handle);
-                    continue;
-                }
+            const bool r_job = kind == plh::sched::kSortedStep && sc.sorted_ready(i) && !sc.fresh(i);
+            if (kind == plh::sched::kSortedStep && pl.isc[(size_t)i] != (r_job ? 0 : (pl.any_c ? 1 : 0)))
+                fail("job %lld is %lld in a mixed step's C set", j.handle, pl.isc[(size_t)i]);
+            if (kind == plh::sched::kSortedStep && !r_job && pl.wn[(size_t)i] != 0) fail("a C job has a window (job %lld)", j.handle);
+            if (kind == plh::sched::kSortedStep && !r_job && !pl.any_c) {  // waits: untouched
+                if (sc.fresh(i)) fail("an admitted job waits (job %lld)", j.handle);
+                if (pl.off[(size_t)i] != pl.off[(size_t)i + 1] || pl.rk[(size_t)i] != 0 || pl.maxw[(size_t)i] != -1 ||
+                    pl.eps[(size_t)i] != 0.0 || pl.next[(size_t)i] != j.next || !pl.alive[(size_t)i])
+                    fail("a waiting job is not left alone (job %lld)", j.handle);
+                continue;
+            }
+            if (r_job) {
+                if (pl.off[(size_t)i] != pl.off[(size_t)i + 1]) fail("Clifford ops for an R job (job %lld)", j.handle);
+                if (pl.rk[(size_t)i] != 0 || pl.next[(size_t)i] != j.next || !pl.alive[(size_t)i] ||
+                    std::find(pl.finishing.begin(), pl.finishing.end(), i) != pl.finishing.end())
+                    fail("a mixed step's plan decides for R job %lld before the device", j.handle);
                 // the window: the job's next rotations in order, each keeping the
                 // order, up to the window depth or the first op that does not
                 const int wn = pl.wn[(size_t)i];
@@ -545,6 +596,8 @@ void run(const std::vector<JobSpec>& jobs, int slots, int as, int64_t scale, siz
         if (n_complete) ++*n_complete;
     }
     plh::sum_free_storage(&r.batch);
+    plh::sum_free_storage(&r.incoming);
+    plh::sum_free_storage(&r.side);
     r.ctx.ws.release();
     if (r.ctx.heap.bytes_live() != 0) fail("%lld bytes still live after the server went", (long long)r.ctx.heap.bytes_live());
     sim::heap = nullptr;
