@@ -566,10 +566,113 @@ PLH_DEV uint32_t u32_bound(const uint32_t* v, uint32_t lo, uint32_t hi, uint32_t
     return lo;
 }
 
-// ---- 0. job ranges (as k_rs_bounds) and the per-job minimum reset
+// Common key prefixes of neighbours in a sorted batch: l0[t] = the leading
+// key bits (job word, x words, z words; at most 64 * (2 AS + 1)) that term t
+// shares with term t - 1, 0 for t = 0 and for t >= n.  The terms sharing
+// term i's first g bits (g >= 64: inside its job) are the range from the
+// last t <= i with l0[t] < g to the first t > i with l0[t] < g, and inside
+// it the terms with bit g set start at the t with l0[t] == g, if any.  Both
+// are nearest-smaller queries on l0, answered through minima per 64 terms
+// (l1), per tile (l2) and per 16 tiles (l3): a few contiguous loads near i
+// instead of key compares at galloping distances.  Arrays hold whole 8-entry
+// chunks (16-byte loads); l2 reads stay below `tiles`.
+PLH_DEV int64_t i64min(int64_t a, int64_t b) { return a < b ? a : b; }
+
+struct LcpIndex {
+    const uint16_t* l0;
+    const uint16_t* l1;
+    const uint16_t* l2;
+    const uint32_t* l3;
+    uint64_t n;
+    uint64_t tiles;
+
+    PLH_DEV static uint32_t at(const uint4& q, int k) {
+        const uint32_t w = k < 2 ? q.x : k < 4 ? q.y : k < 6 ? q.z : q.w;
+        return (w >> (16 * (k & 1))) & 0xFFFFu;
+    }
+    // last t in [lo, hi] with v[t] < thr, or -1
+    PLH_DEV static int64_t last_lt(const uint16_t* v, int64_t lo, int64_t hi, uint32_t thr) {
+        if (hi < lo) return -1;
+        for (int64_t c = hi >> 3; c >= (lo >> 3); --c) {
+            const uint4 q = reinterpret_cast<const uint4*>(v)[c];
+#pragma unroll
+            for (int k = 7; k >= 0; --k) {
+                const int64_t t = 8 * c + k;
+                if (t <= hi && t >= lo && at(q, k) < thr) return t;
+            }
+        }
+        return -1;
+    }
+    // first t in [lo, hi] with v[t] < thr, or -1
+    PLH_DEV static int64_t first_lt(const uint16_t* v, int64_t lo, int64_t hi, uint32_t thr) {
+        if (hi < lo) return -1;
+        for (int64_t c = lo >> 3; c <= (hi >> 3); ++c) {
+            const uint4 q = reinterpret_cast<const uint4*>(v)[c];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t t = 8 * c + k;
+                if (t <= hi && t >= lo && at(q, k) < thr) return t;
+            }
+        }
+        return -1;
+    }
+    // the last t <= i with l0[t] < thr (thr >= 1: l0[0] = 0 ends the search)
+    PLH_DEV uint64_t prev_lt(uint64_t i, uint32_t thr) const {
+        const int64_t t = last_lt(l0, (int64_t)(i & ~63ull), (int64_t)i, thr);
+        if (t >= 0) return (uint64_t)t;
+        const int64_t w = (int64_t)(i >> 6);
+        int64_t ww = last_lt(l1, w & ~15ll, w - 1, thr);
+        if (ww < 0) {
+            const int64_t T = (int64_t)(i >> 10);
+            int64_t tt = last_lt(l2, T & ~15ll, T - 1, thr);
+            if (tt < 0) {
+                int64_t u = (T >> 4) - 1;
+                while (u > 0 && l3[u] >= thr) --u;
+                if (u < 0) return 0;  // i lies in the first 16 tiles and no earlier tile qualifies: never (l0[0] = 0)
+                tt = last_lt(l2, 16 * u, i64min(16 * u + 15, (int64_t)tiles - 1), thr);
+                if (tt < 0) return 0;
+            }
+            ww = last_lt(l1, 16 * tt, 16 * tt + 15, thr);
+            if (ww < 0) return 0;
+        }
+        const int64_t r = last_lt(l0, 64 * ww, 64 * ww + 63, thr);
+        return r < 0 ? 0 : (uint64_t)r;
+    }
+    // the first t > i with l0[t] < thr, n when there is none (l0 is 0 from n on)
+    PLH_DEV uint64_t next_lt(uint64_t i, uint32_t thr) const {
+        const uint64_t s = i + 1;
+        if (s >= n) return n;
+        int64_t t = first_lt(l0, (int64_t)s, (int64_t)(s | 63ull), thr);
+        if (t < 0) {
+            const int64_t w = (int64_t)(s >> 6);
+            int64_t ww = first_lt(l1, w + 1, w | 15ll, thr);
+            if (ww < 0) {
+                const int64_t T = (int64_t)(s >> 10);
+                int64_t tt = first_lt(l2, T + 1, i64min(T | 15ll, (int64_t)tiles - 1), thr);
+                if (tt < 0) {
+                    const int64_t nu = (int64_t)((tiles + 15) >> 4);
+                    int64_t u = (T >> 4) + 1;
+                    while (u < nu && l3[u] >= thr) ++u;
+                    if (u >= nu) return n;
+                    tt = first_lt(l2, 16 * u, i64min(16 * u + 15, (int64_t)tiles - 1), thr);
+                    if (tt < 0) return n;
+                }
+                ww = first_lt(l1, 16 * tt, 16 * tt + 15, thr);
+                if (ww < 0) return n;
+            }
+            t = first_lt(l0, 64 * ww, 64 * ww + 63, thr);
+            if (t < 0) return n;
+        }
+        return (uint64_t)t < n ? (uint64_t)t : n;
+    }
+};
+
+// ---- 0. job ranges (as k_rs_bounds), the per-job minimum reset and the
+// l3 minima reset
 __global__ void k_rj_bounds(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs, uint32_t* __restrict__ off,
-                            uint32_t* __restrict__ jmin) {
+                            uint32_t* __restrict__ jmin, uint32_t* __restrict__ l3, uint32_t nl3) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nl3) l3[k] = 0xFFFFu;
     if (k > njobs) return;
     if (k < njobs) jmin[k] = 0xFFu;
     uint32_t lo = 0, cnt = n;
@@ -641,10 +744,13 @@ PLH_DEV uint32_t first_hit(const uint32_t* t, const uint64_t* xw, const uint64_t
 
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_t n, const uint32_t* __restrict__ tbl,
-                                                     uint8_t* __restrict__ fk, uint32_t* __restrict__ jmin) {
+                                                     uint8_t* __restrict__ fk, uint32_t* __restrict__ jmin,
+                                                     uint16_t* __restrict__ l0, uint16_t* __restrict__ l1,
+                                                     uint16_t* __restrict__ l2, uint32_t* __restrict__ l3) {
     constexpr uint32_t Q = 64u * AS;
     __shared__ uint32_t s_tbl[kStageJobs * Q];
     __shared__ uint32_t s_min[kStageJobs];
+    __shared__ uint32_t s_lw[kRjWords];
     __shared__ uint32_t s_j0, s_nj;
     const uint64_t tile = blockIdx.x;
     const uint64_t i0 = tile * kRjTile;
@@ -670,6 +776,42 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
         for (int w = 0; w < AS; ++w) {
             xw[k][w] = i < n ? S.x[(int64_t)w * S.stride + i] : 0ull;
             zw[k][w] = i < n ? S.z[(int64_t)w * S.stride + i] : 0ull;
+        }
+    }
+    // common prefix with the previous term (the LcpIndex arrays): the first
+    // differing key word decides; l0 is 0 at t = 0 and past n
+    {
+        const int wv = (int)(threadIdx.x / kWave);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t i = rj_item(tile, k);
+            uint32_t v = 0;
+            if (i < n && i > 0) {
+                uint64_t d[2 * AS + 1];
+                d[0] = (uint64_t)(job[k] ^ S.j[i - 1]);
+#pragma unroll
+                for (int w = 0; w < AS; ++w) {
+                    d[1 + w] = xw[k][w] ^ S.x[(int64_t)w * S.stride + i - 1];
+                    d[1 + AS + w] = zw[k][w] ^ S.z[(int64_t)w * S.stride + i - 1];
+                }
+                v = 64u * (2 * AS + 1);
+#pragma unroll
+                for (int w = 2 * AS; w >= 0; --w)
+                    if (d[w]) v = 64u * (uint32_t)w + (uint32_t)__clzll(d[w]);
+            }
+            l0[i] = (uint16_t)v;
+            const uint32_t m = wave_min(v);
+            if (lane_id() == 0) {
+                l1[i >> 6] = (uint16_t)m;
+                s_lw[wv * 4 + k] = m;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t m = s_lw[0];
+            for (int w = 1; w < kRjWords; ++w) m = s_lw[w] < m ? s_lw[w] : m;
+            l2[tile] = (uint16_t)m;
+            atomicMin(&l3[tile >> 4], m);
         }
     }
 #pragma unroll
@@ -759,25 +901,198 @@ __global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__
     rj_put(fl, tile, ab, sh);
 }
 
+// Wave-wide searches on the common-prefix index (every lane calls them with
+// the same arguments): one coalesced load and a ballot per level.
+PLH_DEV uint64_t hi_bit(uint64_t b) { return 63u - (uint64_t)__clzll(b); }
+PLH_DEV uint64_t lo_bit(uint64_t b) { return (uint64_t)__builtin_ctzll(b); }
+
+// the last t in word w with l0[t] < thr (one exists)
+PLH_DEV uint64_t wv_word_last(const LcpIndex& L, uint64_t w, uint32_t thr) {
+    const uint64_t b = ballot(L.l0[64 * w + lane_id()] < thr);
+    return 64 * w + (b ? hi_bit(b) : 0);
+}
+PLH_DEV uint64_t wv_tile_last(const LcpIndex& L, uint64_t T, uint32_t thr) {
+    const uint32_t k = lane_id();
+    const uint64_t b = ballot(k < 16 && L.l1[16 * T + k] < thr);
+    return wv_word_last(L, 16 * T + (b ? hi_bit(b) : 0), thr);
+}
+PLH_DEV uint64_t wv_block_last(const LcpIndex& L, uint64_t u, uint32_t thr) {
+    const uint32_t k = lane_id();
+    const uint64_t T = 16 * u + k;
+    const uint64_t b = ballot(k < 16 && T < L.tiles && L.l2[T] < thr);
+    return wv_tile_last(L, 16 * u + (b ? hi_bit(b) : 0), thr);
+}
+// the last t < p with l0[t] < thr (p >= 1; l0[0] = 0 ends every search)
+PLH_DEV uint64_t wv_prev_lt(const LcpIndex& L, uint64_t p, uint32_t thr) {
+    const uint32_t k = lane_id();
+    const uint64_t q = p - 1, w = q >> 6, T = w >> 4;
+    uint64_t b = ballot(64 * w + k <= q && L.l0[64 * w + k] < thr);
+    if (b) return 64 * w + hi_bit(b);
+    b = ballot(k < 16 && (w & ~15ull) + k < w && L.l1[(w & ~15ull) + k] < thr);
+    if (b) return wv_word_last(L, (w & ~15ull) + hi_bit(b), thr);
+    b = ballot(k < 16 && (T & ~15ull) + k < T && L.l2[(T & ~15ull) + k] < thr);
+    if (b) return wv_tile_last(L, (T & ~15ull) + hi_bit(b), thr);
+    uint64_t U = T >> 4;  // blocks [0, U) remain
+    while (U > 0) {
+        const uint64_t u0 = U >= 64 ? U - 64 : 0;
+        b = ballot(u0 + k < U && L.l3[u0 + k] < thr);
+        if (b) return wv_block_last(L, u0 + hi_bit(b), thr);
+        U = u0;
+    }
+    return 0;
+}
+PLH_DEV uint64_t wv_word_first(const LcpIndex& L, uint64_t w, uint32_t thr) {
+    const uint64_t b = ballot(L.l0[64 * w + lane_id()] < thr);
+    return 64 * w + (b ? lo_bit(b) : 63);
+}
+PLH_DEV uint64_t wv_tile_first(const LcpIndex& L, uint64_t T, uint32_t thr) {
+    const uint32_t k = lane_id();
+    const uint64_t b = ballot(k < 16 && L.l1[16 * T + k] < thr);
+    return wv_word_first(L, 16 * T + (b ? lo_bit(b) : 15), thr);
+}
+PLH_DEV uint64_t wv_block_first(const LcpIndex& L, uint64_t u, uint32_t thr) {
+    const uint32_t k = lane_id();
+    const uint64_t T = 16 * u + k;
+    const uint64_t b = ballot(k < 16 && T < L.tiles && L.l2[T] < thr);
+    return wv_tile_first(L, b ? 16 * u + lo_bit(b) : L.tiles - 1, thr);
+}
+// the first t > p with l0[t] < thr, n when there is none (l0 is 0 from n on)
+PLH_DEV uint64_t wv_next_lt(const LcpIndex& L, uint64_t p, uint32_t thr) {
+    const uint64_t s = p + 1;
+    if (s >= L.n) return L.n;
+    const uint32_t k = lane_id();
+    const uint64_t w = s >> 6, T = w >> 4;
+    auto clamp = [&](uint64_t t) { return t < L.n ? t : L.n; };
+    uint64_t b = ballot(64 * w + k >= s && L.l0[64 * w + k] < thr);
+    if (b) return clamp(64 * w + lo_bit(b));
+    b = ballot(k < 16 && (w & ~15ull) + k > w && L.l1[(w & ~15ull) + k] < thr);
+    if (b) return clamp(wv_word_first(L, (w & ~15ull) + lo_bit(b), thr));
+    b = ballot(k < 16 && (T & ~15ull) + k > T && (T & ~15ull) + k < L.tiles && L.l2[(T & ~15ull) + k] < thr);
+    if (b) return clamp(wv_tile_first(L, (T & ~15ull) + lo_bit(b), thr));
+    const uint64_t nu = (L.tiles + 15) >> 4;
+    for (uint64_t u0 = (T >> 4) + 1; u0 < nu; u0 += 64) {
+        b = ballot(u0 + k < nu && L.l3[u0 + k] < thr);
+        if (b) return clamp(wv_block_first(L, u0 + lo_bit(b), thr));
+    }
+    return L.n;
+}
+
 // ---- 2. new terms at their ranks among the new terms (A: anticommuting
-// flags), as k_rs_new with ranks from the flag words
+// flags).  The rank is new_rank's (the anticommuting count before the term,
+// corrected at each set bit g of M by the halves of its group); the group
+// bounds come from the common-prefix index, a wave at a time: each wave
+// holds 64 consecutive terms, whose group starts (l0 < g) and bit-1 half
+// starts (l0 == g) are two ballots, and a group that crosses the wave's edge
+// takes one wave-wide search past that edge.  A group never crosses a job
+// (l0 < 64 at a job's first term), so every lane tests its own job's g.
+// Per new term r, hb[2r], hb[2r + 1] bound its lower bound among the old
+// terms: the other half of its first set bit's group (the new key keeps the
+// bits above that bit and flips it).
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
-                                                   const uint32_t* __restrict__ joff, NewTerms N) {
+                                                   LcpIndex L, NewTerms N, uint32_t* __restrict__ hb) {
     constexpr int KW = Sorted<AS, true>::KW;
     constexpr int XO = 1;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n || !A.test(i)) return;
-    uint64_t kw[KW + 1];
-    S.key(i, kw);
-    const uint32_t job = S.j[i];
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t i0 = i - lane_id();
+    const uint32_t lane = lane_id();
+    const bool in = i < n;
+    const bool act = in && A.test(i);
+    if (ballot(act) == 0ull) return;  // wave-uniform
+    const uint32_t lz = in ? L.l0[i] : 0u;
+    uint64_t kw[KW + 1] = {};
+    if (act) S.key((uint32_t)i, kw);
+    // this lane's job generator (every term of a rotated job, for the group
+    // structure; none for other jobs)
+    const uint32_t job = in ? S.j[i] : 0u;
+    const bool rot = in && jt.kind[job] == 4;
+    uint64_t m[2 * AS];
+#pragma unroll
+    for (int h = 0; h < 2 * AS; ++h) m[h] = rot ? jt.p[(size_t)job * 2 * AS + h] : 0ull;
+    uint32_t w = 0;
+#pragma unroll
+    for (int h = 0; h < 2 * AS; ++h) w += (uint32_t)popc(m[h]);
+    const uint32_t wmax = wave_last(wave_scan_op(w, 0u, [](uint32_t x, uint32_t y) { return x > y ? x : y; }));
+    int64_t r = act ? (int64_t)A.rank(i) : 0;
+    uint64_t h0 = 0, h1 = 0;
+    const uint64_t below = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);  // lanes <= this one
+    for (uint32_t k = 0; k < wmax; ++k) {
+        const bool has = k < w;
+        // the k-th set bit of M, most significant first (key bit g)
+        uint32_t g = 0, kk = k;
+        bool found = !has;
+#pragma unroll
+        for (int h = 0; h < 2 * AS; ++h) {
+            const uint32_t c = (uint32_t)popc(m[h]);
+            if (!found && kk < c) {
+                uint64_t v = m[h];
+                for (uint32_t t = 0; t < kk; ++t) v &= ~(1ull << (63 - __clzll(v)));
+                g = (uint32_t)(64 * (XO + h) + __clzll(v));
+                found = true;
+            } else if (!found) {
+                kk -= c;
+            }
+        }
+        const uint32_t tlt = has ? g : 64u, tle = has ? g + 1 : 64u;  // no set bit: job starts only
+        const uint64_t s_lt = ballot(lz < tlt), s_le = ballot(lz < tle);
+        const bool one = has && act && bit_at(kw, (int)g);
+        // searches past the wave's edges, for the lanes whose group crosses
+        // them (lane 0's and lane 63's jobs)
+        uint64_t p_le = 0, p_lt = 0, n_le = n, n_lt = n;
+        if (ballot(act && has && (s_lt & below) == 0ull)) {
+            const uint32_t t_le = __builtin_amdgcn_readfirstlane(tle), t_lt = __builtin_amdgcn_readfirstlane(tlt);
+            p_le = wv_prev_lt(L, i0, t_le);
+            p_lt = L.l0[p_le] < t_lt ? p_le : wv_prev_lt(L, p_le, t_lt);
+        }
+        if (ballot(act && has && (s_lt & ~below) == 0ull)) {
+            const uint32_t t_le = (uint32_t)__builtin_amdgcn_readlane((int)tle, 63);
+            const uint32_t t_lt = (uint32_t)__builtin_amdgcn_readlane((int)tlt, 63);
+            n_le = wv_next_lt(L, i0 + 63, t_le);
+            n_lt = (n_le >= n || L.l0[n_le] < t_lt) ? n_le : wv_next_lt(L, n_le, t_lt);
+        }
+        if (!(act && has)) continue;
+        uint64_t lo, mid, hi;
+        if (one) {
+            // the bit-1 half starts at mid (the last t <= i with l0 <= g);
+            // the group at lo (the last t < mid with l0 < g, or mid)
+            const uint64_t bm = s_le & below;
+            mid = bm ? i0 + hi_bit(bm) : p_le;
+            if (bm) {
+                const uint64_t ml = hi_bit(bm);
+                const bool split = ((s_lt >> ml) & 1ull) == 0ull;
+                const uint64_t bl = ml ? (s_lt & ((1ull << ml) - 1ull)) : 0ull;
+                lo = split ? (bl ? i0 + hi_bit(bl) : p_lt) : mid;
+            } else {
+                lo = p_lt;
+            }
+            r -= (int64_t)(A.rank(mid) - A.rank(lo));
+            if (k == 0) {
+                h0 = lo;
+                h1 = mid;
+            }
+        } else {
+            // the bit-1 half starts at mid (the first t > i with l0 <= g)
+            // when l0 there is g; the group ends at hi
+            const uint64_t bm = s_le & ~below;
+            mid = bm ? i0 + lo_bit(bm) : n_le;
+            if (bm) {
+                const uint64_t ml = lo_bit(bm);
+                const bool split = ((s_lt >> ml) & 1ull) == 0ull && mid < n;
+                const uint64_t bh = ml == 63 ? 0ull : (s_lt & ~((2ull << ml) - 1ull));
+                hi = split ? (bh ? i0 + lo_bit(bh) : n_lt) : mid;
+            } else {
+                hi = n_lt;
+            }
+            r += (int64_t)(A.rank(hi) - A.rank(mid));
+            if (k == 0) {
+                h0 = mid;
+                h1 = hi;
+            }
+        }
+    }
+    if (!act || w == 0 || r < 0 || (uint64_t)r >= A.total()) return;  // never with sorted distinct input
     const uint64_t* px = jt.p + (size_t)job * 2 * AS;
     const uint64_t* pz = px + AS;
-    const uint32_t psf = jt.psf[job];
-    const double sn = jt.s[job];
-    const uint32_t base = joff[job], lim = joff[job + 1];
-    const int64_t r = new_rank<AS, true>(S, kw, i, base, lim, px, pz, [&](uint32_t t) { return A.rank(t); });
-    if (r < 0 || (uint64_t)r >= A.total()) return;  // never with sorted distinct input; keeps writes in bounds
     uint64_t c1 = 0, c2 = 0;
     const uint32_t f0 = S.f[i] & 1u;
 #pragma unroll
@@ -786,21 +1101,26 @@ __global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t 
         N.x[(int64_t)k * N.cap + r] = kw[XO + k] ^ px[k];
         N.z[(int64_t)k * N.cap + r] = kw[XO + AS + k] ^ pz[k];
     }
-    const uint32_t raw = phase_of(f0, psf, c1, c2);
-    const double nv = (-sn) * S.value(i);
+    const uint32_t raw = phase_of(f0, jt.psf[job], c1, c2);
+    const double nv = (-jt.s[job]) * S.value((uint32_t)i);
     N.v[r] = (raw & 2u) ? -nv : nv;
     N.f[r] = (uint8_t)(raw & 1u);
     N.j[r] = job;
+    hb[2 * r] = (uint32_t)h0;
+    hb[2 * r + 1] = (uint32_t)h1;
 }
 
-// ---- 3. each new term's lower bound among the old terms (inside its job),
-// whether it equals that old term (D), and whether it is kept as a term of
-// its own (KN: not equal to an old term, and passing the filters)
+// ---- 3. each new term's lower bound among the old terms, inside the range
+// hb[2r] .. hb[2r + 1] (the other half of its first set bit's group, whose
+// key words before that bit's word equal the new key's), whether it equals
+// that old term (D), and whether it is kept as a term of its own (KN: not
+// equal to an old term, and passing the filters)
 template <int AS>
-__global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, BitRank A, const uint32_t* __restrict__ joff,
-                                                  NewTerms N, Filters fl, uint32_t* __restrict__ lb, BitOut dout,
-                                                  BitOut knout) {
+__global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, uint32_t n, BitRank A, const uint32_t* __restrict__ hb,
+                                                  JobRotTables jt, NewTerms N, Filters fl, uint32_t* __restrict__ lb,
+                                                  BitOut dout, BitOut knout) {
     constexpr int KW = Sorted<AS, true>::KW;
+    constexpr int XO = 1;
     __shared__ uint32_t sh[kRjWords];
     const uint64_t tile = blockIdx.x;
     const uint64_t m = A.total();
@@ -811,16 +1131,27 @@ __global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, BitRank A,
         dup[k] = false;
         kn[k] = false;
         if (r < m) {
-            uint64_t kw[KW + 1], ow[KW + 1];
+            uint64_t kw[KW + 1];
             new_key<AS, true>(N, (uint32_t)r, kw);
             const uint32_t job = (uint32_t)kw[0];
-            uint32_t lo = joff[job];
-            const uint32_t hi = joff[job + 1];
+            int w0 = KW;  // the word of the generator's first set bit
+            for (int h = 2 * AS - 1; h >= 0; --h)
+                if (jt.p[(size_t)job * 2 * AS + h]) w0 = XO + h;
+            // clamped into the batch, so that no read leaves it
+            const uint32_t hi = hb[2 * r + 1] < n ? hb[2 * r + 1] : n;
+            uint32_t lo = hb[2 * r] < hi ? hb[2 * r] : hi;
             uint32_t cnt = hi - lo;
             while (cnt > 0) {
                 const uint32_t half = cnt / 2, t = lo + half;
-                S.key(t, ow);
-                if (key_less<KW>(ow, kw)) {
+                bool less = false;
+                for (int w = w0; w <= KW; ++w) {
+                    const uint64_t a = S.word(t, w);
+                    if (a != kw[w]) {
+                        less = a < kw[w];
+                        break;
+                    }
+                }
+                if (less) {
                     lo = t + 1;
                     cnt -= half + 1;
                 } else {
@@ -828,8 +1159,9 @@ __global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, BitRank A,
                 }
             }
             if (lo < hi) {
-                S.key(lo, ow);
-                dup[k] = key_eq<KW>(ow, kw);
+                bool eq = true;
+                for (int w = w0; w <= KW; ++w) eq = eq && S.word(lo, w) == kw[w];
+                dup[k] = eq;
             }
             kn[k] = !dup[k] && keep_of<AS, 1>(fl, kw, N.v[r]);
             lb[r] = lo;
@@ -875,8 +1207,10 @@ __global__ void k_rj_tiles(const uint32_t* __restrict__ lb, BitRank A, uint64_t 
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t m = A.total();
     if (r > m || r > n) return;
-    const int64_t prev = r > 0 ? (int64_t)(lb[r - 1] / kRjTile) : -1;
-    const int64_t cur = r < m ? (int64_t)(lb[r] / kRjTile) : (int64_t)tiles;
+    // lower bounds are at most n (clamped so that the fill stays in tr0)
+    auto tile_of = [&](uint32_t b) { return (int64_t)((b < n ? b : n) / kRjTile); };
+    const int64_t prev = r > 0 ? tile_of(lb[r - 1]) : -1;
+    const int64_t cur = r < m ? tile_of(lb[r]) : (int64_t)tiles;
     for (int64_t t = prev + 1; t <= cur; ++t) tr0[t] = (uint32_t)r;
 }
 
@@ -1044,9 +1378,15 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const size_t nj = njobs;
     const size_t job_bytes = up256((nj + 1) * 4) + up256(nj * 4) + up256(nj) + up256(nj * 2 * AS * 8) + up256(nj) +
                              2 * up256(nj * 8) + up256(nj) + up256(nj) + up256(16);
+    // the common-prefix index: l0 per term (whole tiles), l1 per 64 terms,
+    // l2 per tile, l3 per 16 tiles
+    const uint64_t nl3 = (tiles + 15) / 16;
+    const size_t lcp_bytes = up256(tiles * kRjTile * 2) + up256(tiles * kRjWords * 2) + up256(tiles * 2 + 16) +
+                             up256(nl3 * 4);
     void* p;
     PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n) + up256((tiles + 1) * 4) +
-                                     up256(nj * 64 * AS * 4) + up256((nj + 1) * 4), &p));
+                                     up256(nj * 64 * AS * 4) + up256((nj + 1) * 4) + lcp_bytes + up256(n * 8),
+                        &p));
     char* c = (char*)p;
     FlagBuf fa, fd, fkn, fko;
     fa.carve(c, tiles);
@@ -1080,6 +1420,17 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     uint32_t* qtbl = (uint32_t*)c;
     c += up256(nj * 64 * AS * 4);
     uint32_t* toff = (uint32_t*)c;  // job offsets of the placed terms
+    c += up256((nj + 1) * 4);
+    uint16_t* l0 = (uint16_t*)c;
+    c += up256(tiles * kRjTile * 2);
+    uint16_t* l1 = (uint16_t*)c;
+    c += up256(tiles * kRjWords * 2);
+    uint16_t* l2 = (uint16_t*)c;
+    c += up256(tiles * 2 + 16);
+    uint32_t* l3 = (uint32_t*)c;
+    c += up256(nl3 * 4);
+    uint32_t* hb = (uint32_t*)c;  // per new term: the range of its lower bound
+    const LcpIndex L{l0, l1, l2, l3, n, tiles};
     const uint64_t pn = place ? (uint64_t)place->n : 0;
     const JobRotTables jt{e_kind, e_p, e_psf, e_c, e_s};
     const size_t nb_planes = up256(n * 8 * AS);
@@ -1093,7 +1444,8 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     N.cap = (int64_t)n;
     uint32_t* lb = (uint32_t*)((char*)N.j + up256(n * 4));
     const unsigned gt = (unsigned)tiles, g1 = blocks_of(n);
-    k_rj_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, joff, jmin);
+    k_rj_bounds<<<blocks_of(std::max<uint64_t>((uint64_t)njobs + 1, nl3)), kBlock, 0, st>>>(
+        s->j, (uint32_t)n, njobs, joff, jmin, l3, (uint32_t)nl3);
     PLH_HIP(hipGetLastError());
     if (pn) {
         k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(place->j, (uint32_t)pn, njobs, toff);
@@ -1103,7 +1455,7 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     PLH_HIP(hipGetLastError());
     {
         ProfScope ps(ctx, "rot_anti");
-        k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, fk, jmin);
+        k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, fk, jmin, l0, l1, l2, l3);
         PLH_HIP(hipGetLastError());
         k_rj_select<AS><<<blocks_of(nj), kBlock, 0, st>>>(jw, njobs, jmin, e_kind, e_p, e_psf, e_c, e_s, kstar, adv);
         PLH_HIP(hipGetLastError());
@@ -1114,9 +1466,9 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const BitRank A = fa.rank(tiles);
     {
         ProfScope ps(ctx, "rot_new");
-        k_rj_new<AS><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, joff, N);
+        k_rj_new<AS><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, L, N, hb);
         PLH_HIP(hipGetLastError());
-        k_rj_lb<AS><<<gt, kBlock, 0, st>>>(S, A, joff, N, fl, lb, fd.out(false), fkn.out());
+        k_rj_lb<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, A, hb, jt, N, fl, lb, fd.out(false), fkn.out());
         PLH_HIP(hipGetLastError());
         k_rj_tiles<<<blocks_of(n + 1), kBlock, 0, st>>>(lb, A, tiles, (uint32_t)n, tr0);
         PLH_HIP(hipGetLastError());
