@@ -66,7 +66,7 @@ struct Sorted {
     int as;
 
     PLH_DEV uint64_t word(uint32_t i, int w) const {
-        if (JOB && w == 0) return j[i];
+        if (JOB && w == 0) return j ? j[i] : 0u;
         const int k = w - XO;
         if (k < AS) return k < as ? x[(int64_t)k * stride + i] : 0ull;
         if (k < 2 * AS) return (k - AS) < as ? z[(int64_t)(k - AS) * stride + i] : 0ull;
@@ -77,7 +77,7 @@ struct Sorted {
         for (int w = 0; w <= KW; ++w) kw[w] = word(i, w);
     }
     PLH_DEV double value(uint32_t i) const { return (f[i] & 2u) ? -v[i] : v[i]; }
-    PLH_DEV uint32_t job(uint32_t i) const { return JOB ? j[i] : 0u; }
+    PLH_DEV uint32_t job(uint32_t i) const { return JOB && j ? j[i] : 0u; }
 };
 
 template <int KW>
@@ -574,8 +574,8 @@ PLH_DEV uint32_t u32_bound(const uint32_t* v, uint32_t lo, uint32_t hi, uint32_t
 // it the terms with bit g set start at the t with l0[t] == g, if any.  Both
 // are nearest-smaller queries on l0, answered through minima per 64 terms
 // (l1), per tile (l2) and per 16 tiles (l3): a few contiguous loads near i
-// instead of key compares at galloping distances.  Arrays hold whole 8-entry
-// chunks (16-byte loads); l2 reads stay below `tiles`.
+// instead of key compares at galloping distances.  A sum without a job plane
+// reads as one job (job word 0), so its l0 is 64 + the common x/z prefix.
 PLH_DEV int64_t i64min(int64_t a, int64_t b) { return a < b ? a : b; }
 
 struct LcpIndex {
@@ -585,130 +585,55 @@ struct LcpIndex {
     const uint32_t* l3;
     uint64_t n;
     uint64_t tiles;
-
-    PLH_DEV static uint32_t at(const uint4& q, int k) {
-        const uint32_t w = k < 2 ? q.x : k < 4 ? q.y : k < 6 ? q.z : q.w;
-        return (w >> (16 * (k & 1))) & 0xFFFFu;
-    }
-    // last t in [lo, hi] with v[t] < thr, or -1
-    PLH_DEV static int64_t last_lt(const uint16_t* v, int64_t lo, int64_t hi, uint32_t thr) {
-        if (hi < lo) return -1;
-        for (int64_t c = hi >> 3; c >= (lo >> 3); --c) {
-            const uint4 q = reinterpret_cast<const uint4*>(v)[c];
-#pragma unroll
-            for (int k = 7; k >= 0; --k) {
-                const int64_t t = 8 * c + k;
-                if (t <= hi && t >= lo && at(q, k) < thr) return t;
-            }
-        }
-        return -1;
-    }
-    // first t in [lo, hi] with v[t] < thr, or -1
-    PLH_DEV static int64_t first_lt(const uint16_t* v, int64_t lo, int64_t hi, uint32_t thr) {
-        if (hi < lo) return -1;
-        for (int64_t c = lo >> 3; c <= (hi >> 3); ++c) {
-            const uint4 q = reinterpret_cast<const uint4*>(v)[c];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int64_t t = 8 * c + k;
-                if (t <= hi && t >= lo && at(q, k) < thr) return t;
-            }
-        }
-        return -1;
-    }
-    // the last t <= i with l0[t] < thr (thr >= 1: l0[0] = 0 ends the search)
-    PLH_DEV uint64_t prev_lt(uint64_t i, uint32_t thr) const {
-        const int64_t t = last_lt(l0, (int64_t)(i & ~63ull), (int64_t)i, thr);
-        if (t >= 0) return (uint64_t)t;
-        const int64_t w = (int64_t)(i >> 6);
-        int64_t ww = last_lt(l1, w & ~15ll, w - 1, thr);
-        if (ww < 0) {
-            const int64_t T = (int64_t)(i >> 10);
-            int64_t tt = last_lt(l2, T & ~15ll, T - 1, thr);
-            if (tt < 0) {
-                int64_t u = (T >> 4) - 1;
-                while (u > 0 && l3[u] >= thr) --u;
-                if (u < 0) return 0;  // i lies in the first 16 tiles and no earlier tile qualifies: never (l0[0] = 0)
-                tt = last_lt(l2, 16 * u, i64min(16 * u + 15, (int64_t)tiles - 1), thr);
-                if (tt < 0) return 0;
-            }
-            ww = last_lt(l1, 16 * tt, 16 * tt + 15, thr);
-            if (ww < 0) return 0;
-        }
-        const int64_t r = last_lt(l0, 64 * ww, 64 * ww + 63, thr);
-        return r < 0 ? 0 : (uint64_t)r;
-    }
-    // the first t > i with l0[t] < thr, n when there is none (l0 is 0 from n on)
-    PLH_DEV uint64_t next_lt(uint64_t i, uint32_t thr) const {
-        const uint64_t s = i + 1;
-        if (s >= n) return n;
-        int64_t t = first_lt(l0, (int64_t)s, (int64_t)(s | 63ull), thr);
-        if (t < 0) {
-            const int64_t w = (int64_t)(s >> 6);
-            int64_t ww = first_lt(l1, w + 1, w | 15ll, thr);
-            if (ww < 0) {
-                const int64_t T = (int64_t)(s >> 10);
-                int64_t tt = first_lt(l2, T + 1, i64min(T | 15ll, (int64_t)tiles - 1), thr);
-                if (tt < 0) {
-                    const int64_t nu = (int64_t)((tiles + 15) >> 4);
-                    int64_t u = (T >> 4) + 1;
-                    while (u < nu && l3[u] >= thr) ++u;
-                    if (u >= nu) return n;
-                    tt = first_lt(l2, 16 * u, i64min(16 * u + 15, (int64_t)tiles - 1), thr);
-                    if (tt < 0) return n;
-                }
-                ww = first_lt(l1, 16 * tt, 16 * tt + 15, thr);
-                if (ww < 0) return n;
-            }
-            t = first_lt(l0, 64 * ww, 64 * ww + 63, thr);
-            if (t < 0) return n;
-        }
-        return (uint64_t)t < n ? (uint64_t)t : n;
-    }
 };
-
-// ---- 0. job ranges (as k_rs_bounds), the per-job minimum reset and the
-// l3 minima reset
-__global__ void k_rj_bounds(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs, uint32_t* __restrict__ off,
-                            uint32_t* __restrict__ jmin, uint32_t* __restrict__ l3, uint32_t nl3) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nl3) l3[k] = 0xFFFFu;
-    if (k > njobs) return;
-    if (k < njobs) jmin[k] = 0xFFu;
-    uint32_t lo = 0, cnt = n;
-    while (cnt > 0) {
-        const uint32_t half = cnt / 2, t = lo + half;
-        if (J[t] < k) {
-            lo = t + 1;
-            cnt -= half + 1;
-        } else {
-            cnt = half;
-        }
-    }
-    off[k] = lo;
-}
 
 template <class T>
 PLH_DEV T wave_min(T v) {
     return wave_last(wave_scan_op(v, (T)~T(0), [](T a, T b) { return a < b ? a : b; }));
 }
 
-// Per job and qubit, the window entries a set bit at that qubit anticommutes
-// with: low 16 bits for the term's z bit (entries whose generator has X or Y
-// there), high 16 bits for its x bit (Z or Y).  A term's anticommutation
-// with entry e is the parity of bit e over its set bits, so one pass over a
-// term's (few) set bits tests the whole window.  Entries of kind 0 (no
-// change) are left out.  One thread per (job, qubit).
+// ---- 0. set-up: per job and qubit, the window entries a set bit at that
+// qubit anticommutes with: low 16 bits for the term's z bit (entries whose
+// generator has X or Y there), high 16 bits for its x bit (Z or Y).  A
+// term's anticommutation with entry e is the parity of bit e over its set
+// bits, so one pass over a term's (few) set bits tests the whole window.
+// Entries of kind 0 (no change) are left out.  One thread per (job, qubit);
+// the first threads also find the job ranges (off, as k_rs_bounds; a sum
+// without a job plane is one job) and reset the per-job minima and the l3
+// minima.
 template <int AS>
-__global__ void k_rj_tables(JobWindows jw, uint32_t njobs, const uint8_t* __restrict__ alive,
-                            uint32_t* __restrict__ tbl) {
+__global__ void k_rj_setup(JobWindows jw, uint32_t njobs, const uint8_t* __restrict__ alive,
+                           uint32_t* __restrict__ tbl, const uint32_t* __restrict__ J, uint32_t n,
+                           uint32_t* __restrict__ off, uint32_t* __restrict__ jmin, uint32_t* __restrict__ l3,
+                           uint32_t nl3) {
     constexpr uint32_t Q = 64u * AS;
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nl3) l3[g] = 0xFFFFu;
+    if (g < njobs) jmin[g] = 0xFFu;
+    if (g <= njobs) {
+        const uint32_t k = (uint32_t)g;
+        uint32_t lo = 0;
+        if (!J) {
+            lo = k == 0 ? 0u : n;
+        } else {
+            uint32_t cnt = n;
+            while (cnt > 0) {
+                const uint32_t half = cnt / 2, t = lo + half;
+                if (J[t] < k) {
+                    lo = t + 1;
+                    cnt -= half + 1;
+                } else {
+                    cnt = half;
+                }
+            }
+        }
+        off[k] = lo;
+    }
     if (g >= (uint64_t)njobs * Q) return;
     const uint32_t job = (uint32_t)(g / Q), q = (uint32_t)(g % Q);
     const int wd = (int)(q >> 6), b = (int)(q & 63);
     uint32_t mz = 0, mx = 0;
-    const int wn = alive[job] ? (int)jw.wn[job] : 0;
+    const int wn = (!alive || alive[job]) ? (int)jw.wn[job] : 0;
     for (int e = 0; e < wn; ++e) {
         const size_t ent = (size_t)job * kJobWindow + (size_t)e;
         if (jw.kind[ent] != 4) continue;
@@ -756,8 +681,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
     const uint64_t i0 = tile * kRjTile;
     const uint64_t i1 = (i0 + kRjTile < (uint64_t)n ? i0 + kRjTile : (uint64_t)n) - 1;
     if (threadIdx.x == 0) {
-        s_j0 = S.j[i0];
-        s_nj = S.j[i1] - s_j0 + 1;
+        s_j0 = S.job((uint32_t)i0);
+        s_nj = S.job((uint32_t)i1) - s_j0 + 1;
     }
     __syncthreads();
     const uint32_t j0 = s_j0, nj = s_nj;
@@ -771,7 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint64_t i = rj_item(tile, k);
-        job[k] = i < n ? S.j[i] : 0xFFFFFFFFu;
+        job[k] = i < n ? S.job((uint32_t)i) : 0xFFFFFFFFu;
 #pragma unroll
         for (int w = 0; w < AS; ++w) {
             xw[k][w] = i < n ? S.x[(int64_t)w * S.stride + i] : 0ull;
@@ -788,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
             uint32_t v = 0;
             if (i < n && i > 0) {
                 uint64_t d[2 * AS + 1];
-                d[0] = (uint64_t)(job[k] ^ S.j[i - 1]);
+                d[0] = (uint64_t)(job[k] ^ S.job((uint32_t)(i - 1)));
 #pragma unroll
                 for (int w = 0; w < AS; ++w) {
                     d[1 + w] = xw[k][w] ^ S.x[(int64_t)w * S.stride + i - 1];
@@ -848,53 +773,51 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
 }
 
 // ---- 1b. per job the rotation it applies (the first of its window that
-// changes a term: jmin) as this step's rotation table, and the rotations it
-// consumes (adv: up to and including that one; the whole window when no
-// rotation of it changes a term).
-template <int AS>
-__global__ void k_rj_select(JobWindows jw, uint32_t njobs, const uint32_t* __restrict__ jmin,
-                            uint8_t* __restrict__ kind, uint64_t* __restrict__ p, uint8_t* __restrict__ psf,
-                            double* __restrict__ c, double* __restrict__ sn, uint8_t* __restrict__ kstar,
-                            uint8_t* __restrict__ adv) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= njobs) return;
-    const uint32_t w = jw.wn[t], m = jmin[t];
-    kind[t] = 0;
-    kstar[t] = 0xFFu;
-    if (w == 0) {
-        adv[t] = 0;
-        return;
-    }
-    if (m >= w) {
-        adv[t] = (uint8_t)w;
-        return;
-    }
-    const size_t ent = (size_t)t * kJobWindow + m;
-    adv[t] = (uint8_t)(m + 1);
-    kind[t] = jw.kind[ent];
-    kstar[t] = (uint8_t)m;
-#pragma unroll
-    for (int k = 0; k < 2 * AS; ++k) p[(size_t)t * 2 * AS + k] = jw.p[ent * 2 * AS + k];
-    psf[t] = jw.psf[ent];
-    c[t] = jw.c[ent];
-    sn[t] = jw.s[ent];
+// changes a term: jmin) and the rotations it consumes (adv: up to and
+// including that one; the whole window when no rotation of it changes a
+// term); the first workgroup writes them as this step's rotation table.
+// Every term is anticommuting when its first hit is its job's applied entry
+// (every term's first hit is at least that one).
+PLH_DEV uint32_t rj_applied(const JobWindows& jw, const uint32_t* jmin, uint32_t job) {
+    const uint32_t w = jw.wn[job], m = jmin[job];
+    return (w != 0 && m < w) ? m : 0xFFu;
 }
 
-// ---- 1c. anticommuting flags for the applied rotations: a term's first
-// hit in its window is its job's least one (every term's is at least that)
 template <int AS>
-__global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__ J, uint32_t n,
-                                                    const uint8_t* __restrict__ fk, const uint8_t* __restrict__ kstar,
-                                                    BitOut ab) {
+__global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs,
+                                                    JobWindows jw, const uint32_t* __restrict__ jmin,
+                                                    const uint8_t* __restrict__ fk, JobRotTables jt,
+                                                    uint8_t* __restrict__ kstar, uint8_t* __restrict__ adv, BitOut ab) {
     __shared__ uint32_t sh[kRjWords];
     const uint64_t tile = blockIdx.x;
+    if (tile == 0) {
+        uint8_t* kind = const_cast<uint8_t*>(jt.kind);
+        uint64_t* p = const_cast<uint64_t*>(jt.p);
+        uint8_t* psf = const_cast<uint8_t*>(jt.psf);
+        double* c = const_cast<double*>(jt.c);
+        double* sn = const_cast<double*>(jt.s);
+        for (uint32_t t = threadIdx.x; t < njobs; t += kBlock) {
+            const uint32_t w = jw.wn[t], m = rj_applied(jw, jmin, t);
+            kind[t] = 0;
+            kstar[t] = (uint8_t)m;
+            adv[t] = (uint8_t)(m == 0xFFu ? w : m + 1);
+            if (m == 0xFFu) continue;
+            const size_t ent = (size_t)t * kJobWindow + m;
+            kind[t] = jw.kind[ent];
+#pragma unroll
+            for (int k = 0; k < 2 * AS; ++k) p[(size_t)t * 2 * AS + k] = jw.p[ent * 2 * AS + k];
+            psf[t] = jw.psf[ent];
+            c[t] = jw.c[ent];
+            sn[t] = jw.s[ent];
+        }
+    }
     bool fl[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint64_t i = rj_item(tile, k);
         fl[k] = false;
         if (i < n) {
-            const uint32_t ks = kstar[J[i]];
+            const uint32_t ks = rj_applied(jw, jmin, J ? J[i] : 0u);
             fl[k] = ks != 0xFFu && fk[i] == ks;
         }
     }
@@ -1004,7 +927,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t 
     if (act) S.key((uint32_t)i, kw);
     // this lane's job generator (every term of a rotated job, for the group
     // structure; none for other jobs)
-    const uint32_t job = in ? S.j[i] : 0u;
+    const uint32_t job = in ? S.job((uint32_t)i) : 0u;
     const bool rot = in && jt.kind[job] == 4;
     uint64_t m[2 * AS];
 #pragma unroll
@@ -1143,12 +1066,17 @@ __global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, uint32_t n
             uint32_t cnt = hi - lo;
             while (cnt > 0) {
                 const uint32_t half = cnt / 2, t = lo + half;
-                bool less = false;
-                for (int w = w0; w <= KW; ++w) {
-                    const uint64_t a = S.word(t, w);
-                    if (a != kw[w]) {
-                        less = a < kw[w];
-                        break;
+                // the probe's words from w0 on, loaded together (one latency
+                // per probe), then compared
+                uint64_t ow[KW + 1];
+#pragma unroll
+                for (int w = 0; w <= KW; ++w) ow[w] = w >= w0 ? S.word(t, w) : kw[w];
+                bool less = false, decided = false;
+#pragma unroll
+                for (int w = 0; w <= KW; ++w) {
+                    if (!decided && ow[w] != kw[w]) {
+                        less = ow[w] < kw[w];
+                        decided = true;
                     }
                 }
                 if (less) {
@@ -1159,9 +1087,10 @@ __global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, uint32_t n
                 }
             }
             if (lo < hi) {
-                bool eq = true;
-                for (int w = w0; w <= KW; ++w) eq = eq && S.word(lo, w) == kw[w];
-                dup[k] = eq;
+                uint64_t ow[KW + 1];
+#pragma unroll
+                for (int w = 0; w <= KW; ++w) ow[w] = w >= w0 ? S.word(lo, w) : kw[w];
+                dup[k] = key_eq<KW>(ow, kw);
             }
             kn[k] = !dup[k] && keep_of<AS, 1>(fl, kw, N.v[r]);
             lb[r] = lo;
@@ -1228,8 +1157,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
         const uint64_t i = rj_item(tile, k);
         ko[k] = false;
         if (i < n) {
-            const uint32_t job = S.j[i];
-            if (fl.job_alive[job]) {
+            const uint32_t job = S.job((uint32_t)i);
+            if (!fl.job_alive || fl.job_alive[job]) {
                 uint32_t u;
                 const double acc = rj_old_value<AS>(S, (uint32_t)i, job, jt, A, lb, D, N, rr[0], rr[1], &u);
                 ko[k] = rj_keep_old(fl, job, acc);
@@ -1239,21 +1168,60 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
     rj_put(ko, tile, koout, sh);
 }
 
-// The output count KO + KN and each job's consumed rotations, posted to the
+// One workgroup: exclusive scan of n tile counts (out[n] = total), in
+// chunks of kBlock * 8.
+PLH_DEV void rj_block_scan(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, uint64_t* sh) {
+    constexpr int kIpt = 8;
+    uint64_t carry = 0;
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kBlock * kIpt) {
+        const uint64_t base = c0 + (uint64_t)threadIdx.x * kIpt;
+        uint32_t vals[kIpt];
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < kIpt; ++k) {
+            vals[k] = base + k < n ? in[base + k] : 0u;
+            acc += vals[k];
+        }
+        uint64_t tot;
+        uint64_t run = carry + block_exclusive_scan(acc, tot, sh);
+#pragma unroll
+        for (int k = 0; k < kIpt; ++k) {
+            if (base + k < n) out[base + k] = run;
+            run += vals[k];
+        }
+        carry += tot;
+        __syncthreads();  // sh is reused by the next chunk
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+// ---- 4b. the kept new (KN) and kept old (KO) terms' tile scans, then the
+// output count KO + KN and each job's consumed rotations, posted to the
 // host: every thread fences its stores to host memory before the block's
 // barrier, and thread 0 then posts the count (the host polls its sequence).
-__global__ __launch_bounds__(kBlock) void k_rj_total(const uint64_t* __restrict__ ko_total,
-                                                     const uint64_t* __restrict__ kn_total, uint64_t extra,
-                                                     uint64_t* __restrict__ out, const uint8_t* __restrict__ adv,
-                                                     uint32_t njobs, uint8_t* adv_host, uint64_t* mbox,
-                                                     uint64_t seq) {
-    for (uint32_t t = threadIdx.x; t < njobs; t += blockDim.x) adv_host[t] = adv[t];
+// A single sum (no adv_host) posts its count, consumed rotations and applied
+// window entry (0xFF: none) together.
+__global__ __launch_bounds__(kBlock) void k_rj_counts(const uint32_t* __restrict__ kn_cnt, uint64_t* __restrict__ kn_pre,
+                                                      const uint32_t* __restrict__ ko_cnt, uint64_t* __restrict__ ko_pre,
+                                                      uint64_t tiles, uint64_t extra, uint64_t* __restrict__ out,
+                                                      const uint8_t* __restrict__ adv, const uint8_t* __restrict__ kstar,
+                                                      uint32_t njobs, uint8_t* adv_host, uint64_t* mbox, uint64_t seq) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    rj_block_scan(kn_cnt, kn_pre, tiles, sh);
+    rj_block_scan(ko_cnt, ko_pre, tiles, sh);
+    if (adv_host)
+        for (uint32_t t = threadIdx.x; t < njobs; t += blockDim.x) adv_host[t] = adv[t];
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const uint64_t t = *ko_total + *kn_total + extra;
+    const uint64_t t = ko_pre[tiles] + kn_pre[tiles] + extra;
     *out = t;
-    post_write(mbox, seq, 1, &t);
+    if (adv_host) {
+        post_write(mbox, seq, 1, &t);
+    } else {
+        const uint64_t v[3] = {t, adv[0], kstar[0]};
+        post_write(mbox, seq, 3, v);
+    }
 }
 
 // ---- 5. kept terms to their places: old term i after the kept old terms
@@ -1279,7 +1247,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
         for (int k = 0; k < 4; ++k) {
             const uint64_t i = rj_item(tile, k);
             if (i >= n || !KO.test(i)) continue;
-            const uint32_t job = S.j[i];
+            const uint32_t job = S.job((uint32_t)i);
             uint32_t u;
             const double acc = rj_old_value<AS>(S, (uint32_t)i, job, jt, A, lb, D, N, rr[0], rr[1], &u);
             const uint64_t o = KO.rank(i) + KN.rank(u) + (toff ? toff[job] : 0u);
@@ -1291,7 +1259,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
             }
             of[o] = S.f[i] & 1u;
             ov[o] = acc;
-            oj[o] = job;
+            if (oj) oj[o] = job;
         }
         return;
     }
@@ -1309,7 +1277,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
         }
         of[o] = N.f[r];
         ov[o] = N.v[r];
-        oj[o] = N.j[r];
+        if (oj) oj[o] = N.j[r];
     }
 }
 
@@ -1364,9 +1332,11 @@ struct FlagBuf {
     BitRank rank(uint64_t tiles) const { return BitRank{bits, wpre, tpre, tiles * kRjTile, tiles}; }
 };
 
+// A batched sum (adv_host: each job's consumed rotations), or a single sum
+// as one job (no job plane, adv_host null, *consumed set).
 template <int AS>
 int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const Filters& fl, uint32_t njobs,
-                          uint8_t* adv_host, const plh_sum* place) {
+                          uint8_t* adv_host, const plh_sum* place, int* consumed = nullptr) {
     hipStream_t st = ctx->stream;
     const uint64_t n = (uint64_t)s->n;
     const uint64_t tiles = (n + kRjTile - 1) / kRjTile;
@@ -1444,22 +1414,18 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     N.cap = (int64_t)n;
     uint32_t* lb = (uint32_t*)((char*)N.j + up256(n * 4));
     const unsigned gt = (unsigned)tiles, g1 = blocks_of(n);
-    k_rj_bounds<<<blocks_of(std::max<uint64_t>((uint64_t)njobs + 1, nl3)), kBlock, 0, st>>>(
-        s->j, (uint32_t)n, njobs, joff, jmin, l3, (uint32_t)nl3);
-    PLH_HIP(hipGetLastError());
     if (pn) {
         k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(place->j, (uint32_t)pn, njobs, toff);
         PLH_HIP(hipGetLastError());
     }
-    k_rj_tables<AS><<<blocks_of((uint64_t)nj * 64 * AS), kBlock, 0, st>>>(jw, njobs, fl.job_alive, qtbl);
+    k_rj_setup<AS><<<blocks_of(std::max<uint64_t>(std::max<uint64_t>((uint64_t)nj * 64 * AS, nj + 1), nl3)), kBlock, 0,
+                     st>>>(jw, njobs, fl.job_alive, qtbl, s->j, (uint32_t)n, joff, jmin, l3, (uint32_t)nl3);
     PLH_HIP(hipGetLastError());
     {
         ProfScope ps(ctx, "rot_anti");
         k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, fk, jmin, l0, l1, l2, l3);
         PLH_HIP(hipGetLastError());
-        k_rj_select<AS><<<blocks_of(nj), kBlock, 0, st>>>(jw, njobs, jmin, e_kind, e_p, e_psf, e_c, e_s, kstar, adv);
-        PLH_HIP(hipGetLastError());
-        k_rj_anti<AS><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, fk, kstar, fa.out());
+        k_rj_anti<AS><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, jw, jmin, fk, jt, kstar, adv, fa.out());
         PLH_HIP(hipGetLastError());
     }
     PLH_TRY(scan_u32_to_u64(ctx, fa.tcnt, fa.tpre, tiles, fa.tpre + tiles));
@@ -1473,27 +1439,30 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
         k_rj_tiles<<<blocks_of(n + 1), kBlock, 0, st>>>(lb, A, tiles, (uint32_t)n, tr0);
         PLH_HIP(hipGetLastError());
     }
-    PLH_TRY(scan_u32_to_u64(ctx, fkn.tcnt, fkn.tpre, tiles, fkn.tpre + tiles));
     const BitRank D = fd.rank(tiles), KN = fkn.rank(tiles);
     {
         ProfScope ps(ctx, "rot_merge");
         k_rj_keep<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, lb, tr0, D, N, fl, fko.out());
         PLH_HIP(hipGetLastError());
     }
-    PLH_TRY(scan_u32_to_u64(ctx, fko.tcnt, fko.tpre, tiles, fko.tpre + tiles));
     const BitRank KO = fko.rank(tiles);
     Post post;
     PLH_TRY(post_prepare(ctx, &post));
-    k_rj_total<<<1, kBlock, 0, st>>>(fko.tpre + tiles, fkn.tpre + tiles, pn, d_total, adv, njobs, adv_host,
-                                     post.mbox, post.seq);
+    k_rj_counts<<<1, kBlock, 0, st>>>(fkn.tcnt, fkn.tpre, fko.tcnt, fko.tpre, tiles, pn, d_total, adv, kstar, njobs,
+                                      adv_host, post.mbox, post.seq);
     PLH_HIP(hipGetLastError());
-    uint64_t total = 0;
-    PLH_TRY(post_wait(ctx, post, 1, &total));
+    uint64_t rb[3] = {0, 0, 0xFFu};
+    PLH_TRY(post_wait(ctx, post, adv_host ? 1 : 3, rb));
+    const uint64_t total = rb[0];
+    if (!adv_host) {
+        *consumed = (int)rb[1];
+        if (rb[2] == 0xFFu) return PLH_OK;  // no rotation of the window changes a term: the sum stays as it is
+    }
     plh_sum tmp;
     tmp.ctx = ctx;
     tmp.as = s->as;
     tmp.nq = s->nq;
-    tmp.has_job = true;
+    tmp.has_job = s->has_job;
     PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(total, 1), 0));
     if (total) {
         ProfScope ps(ctx, "rot_write", (double)total * (16.0 * s->as + 13.0));
@@ -1596,62 +1565,74 @@ int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const Jo
 // ----------------------------------------------------------------------
 // A window of rotations on a single sorted sum (propagate's rotation list,
 // a program's run of rotations): the first one that a term anticommutes
-// with is found in one pass, and the rotations before it are skipped.  They
-// are exact no-ops when the sum has passed the same truncation since its
-// keys last changed: a general rotation that no term anticommutes with
-// changes no term, a rotation of kind 0 changes nothing, and the
-// truncation after each finds nothing to drop.
+// with is applied, and the rotations before it are skipped.  They are exact
+// no-ops when the sum has passed the same truncation since its keys last
+// changed: a general rotation that no term anticommutes with changes no
+// term, a rotation of kind 0 changes nothing, and the truncation after each
+// finds nothing to drop.  The sum goes through the batched rotation as one
+// job (its job word reads as 0).
 constexpr int kSumWindow = 16;
 
+// One sum's window, passed by value to the kernel that lays it out as a
+// one-job window table.
 template <int AS>
-struct WindowParams {
-    uint32_t tbl[64 * AS];  // per qubit: low 16 bits entries tested by its z bit, high 16 by its x bit
+struct WindowArg {
+    uint8_t wn;
+    uint8_t kind[kJobWindow];
+    uint8_t psf[kJobWindow];
+    uint64_t p[kJobWindow * 2 * AS];
+    double c[kJobWindow];
+    double s[kJobWindow];
 };
 
-__global__ void k_la_init(uint32_t* kmin) { *kmin = 0xFFu; }
-
-// The least window index any term anticommutes with (0xFF: none), from the
-// qubit table: a term XORs the entry masks of its set bits.
 template <int AS>
-__global__ __launch_bounds__(kBlock) void k_la_first(Sorted<AS, false> S, uint32_t n, WindowParams<AS> wp,
-                                                     uint32_t* __restrict__ kmin) {
-    __shared__ uint32_t sh[kWavesPerBlock];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t f = 0xFFu;
-    if (i < n) {
-        uint64_t xw[AS], zw[AS];
-#pragma unroll
-        for (int k = 0; k < AS; ++k) {
-            xw[k] = k < S.as ? S.x[(int64_t)k * S.stride + i] : 0ull;
-            zw[k] = k < S.as ? S.z[(int64_t)k * S.stride + i] : 0ull;
-        }
-        f = first_hit<AS>(wp.tbl, xw, zw);
+__global__ void k_rj_window1(WindowArg<AS> a, uint8_t* __restrict__ wn, uint8_t* __restrict__ kind,
+                             uint64_t* __restrict__ p, uint8_t* __restrict__ psf, double* __restrict__ c,
+                             double* __restrict__ sn) {
+    const int t = (int)threadIdx.x;
+    if (t == 0) *wn = a.wn;
+    if (t < kJobWindow) {
+        kind[t] = a.kind[t];
+        psf[t] = a.psf[t];
+        c[t] = a.c[t];
+        sn[t] = a.s[t];
     }
-    const uint32_t m = wave_min(f);
-    if (lane_id() == 0) sh[threadIdx.x / kWave] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t b = sh[0];
-        for (int k = 1; k < kWavesPerBlock; ++k) b = sh[k] < b ? sh[k] : b;
-        if (b < 0xFFu) atomicMin(kmin, b);
-    }
+    for (int k = t; k < kJobWindow * 2 * AS; k += (int)blockDim.x) p[k] = a.p[k];
 }
 
+// A window on a single sorted sum through the batched rotation as one job:
+// the window test, the applied rotation and its output count in one pass
+// with one read-back (the host learns the consumed count with the count).
 template <int AS>
-int window_first_sorted(plh_ctx* ctx, const plh_sum* s, const WindowParams<AS>& wp, int* first) {
-    hipStream_t st = ctx->stream;
+int rotate_window_single(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const* px, const uint64_t* const* pz,
+                         const uint8_t* kind, const uint32_t* psf, const double* c, const double* sn,
+                         const Filters& fl, int* consumed) {
+    WindowArg<AS> a{};
+    a.wn = (uint8_t)w;
+    for (int e = 0; e < w; ++e) {
+        a.kind[e] = kind[e];
+        a.psf[e] = (uint8_t)psf[e];
+        a.c[e] = c[e];
+        a.s[e] = sn[e];
+        for (int k = 0; k < s->as; ++k) {
+            a.p[e * 2 * AS + k] = px[e][k];
+            a.p[e * 2 * AS + AS + k] = pz[e][k];
+        }
+    }
     void* p;
-    PLH_TRY(ctx->ws.get(kWsMisc, 256, &p));  // scratch of one launch sequence, as elsewhere
-    uint32_t* kmin = (uint32_t*)p;
-    k_la_init<<<1, 1, 0, st>>>(kmin);
-    Sorted<AS, false> S{s->x, s->z, s->f, s->v, nullptr, s->cap, s->as};
-    k_la_first<AS><<<blocks_of((uint64_t)s->n), kBlock, 0, st>>>(S, (uint32_t)s->n, wp, kmin);
+    PLH_TRY(ctx->ws.get(kWsRotM, 8 * 256, &p));
+    char* b = (char*)p;
+    uint8_t* d_wn = (uint8_t*)b;
+    uint8_t* d_kind = (uint8_t*)(b + 256);
+    uint8_t* d_psf = (uint8_t*)(b + 512);
+    double* d_c = (double*)(b + 768);
+    double* d_s = (double*)(b + 1024);
+    uint64_t* d_p = (uint64_t*)(b + 1280);  // 16 * 2 * AS words: up to 512 bytes
+    static_assert(kJobWindow * 2 * 2 * 8 <= 768, "window generators fit their slot");
+    k_rj_window1<AS><<<1, kBlock, 0, ctx->stream>>>(a, d_wn, d_kind, d_p, d_psf, d_c, d_s);
     PLH_HIP(hipGetLastError());
-    // the count is the low half of the u64 word read back
-    uint64_t v = 0;
-    PLH_TRY(read_u64n(ctx, (const uint64_t*)p, 1, &v));
-    *first = (int)(v & 0xFFFFFFFFull);
-    return PLH_OK;
+    const JobWindows jw{d_wn, d_kind, d_p, d_psf, d_c, d_s};
+    return rotate_sorted_jobs_as<AS>(ctx, s, jw, fl, 1, nullptr, nullptr, consumed);
 }
 
 Filters filters_of(const CollateSpec& spec) {
@@ -1710,33 +1691,9 @@ int rotate_sorted_window(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const*
         *consumed = w;
         return PLH_OK;
     }
-    // the qubit table of the window: entries of kind 0 (no change) never count
-    uint32_t tbl[128] = {};
-    for (int e = 0; e < w; ++e) {
-        if (kind[e] != 4) continue;
-        for (int k = 0; k < s->as; ++k)
-            for (int b = 0; b < 64; ++b) {
-                if ((px[e][k] >> b) & 1ull) tbl[64 * k + b] |= 1u << e;          // X or Y: the term's z bit
-                if ((pz[e][k] >> b) & 1ull) tbl[64 * k + b] |= 1u << (16 + e);   // Z or Y: the term's x bit
-            }
-    }
-    int first = 0xFF;
-    if (s->as == 1) {
-        WindowParams<1> wp{};
-        for (int q = 0; q < 64; ++q) wp.tbl[q] = tbl[q];
-        PLH_TRY(window_first_sorted<1>(ctx, s, wp, &first));
-    } else {
-        WindowParams<2> wp{};
-        for (int q = 0; q < 128; ++q) wp.tbl[q] = tbl[q];
-        PLH_TRY(window_first_sorted<2>(ctx, s, wp, &first));
-    }
-    if (first >= w) {
-        *consumed = w;
-        return PLH_OK;
-    }
-    PLH_TRY(rotate_sorted(ctx, s, px[first], pz[first], psf[first], c[first], sn[first], spec));
-    *consumed = first + 1;
-    return PLH_OK;
+    const Filters fl = filters_of(spec);
+    if (s->as == 1) return rotate_window_single<1>(ctx, s, w, px, pz, kind, psf, c, sn, fl, consumed);
+    return rotate_window_single<2>(ctx, s, w, px, pz, kind, psf, c, sn, fl, consumed);
 }
 
 int rotate_sorted_jobs(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, int njobs, const CollateSpec& spec,

@@ -43,7 +43,6 @@ namespace plh {
 namespace {
 
 constexpr int kIpt = 16;
-constexpr uint64_t kTile = (uint64_t)kBlock * kIpt;
 // Per-step job tables (device, one upload per step), one entry per slot.
 struct StepTables {
     const uint32_t* cl_off;   // [slots + 1] range of the slot's Clifford ops
@@ -200,29 +199,39 @@ __global__ __launch_bounds__(kBlock) void k_batch_gates(uint64_t* __restrict__ X
 }
 
 // <0|O_job|0> (include/PauliSumUtils.h:162-177) for finished jobs of a
-// job-ordered batch: one workgroup per job, fixed-order reduction over the
-// job's contiguous range, whose bounds are also returned.
-__global__ __launch_bounds__(kBlock) void k_batch_zse(int as, SumView s, const uint32_t* __restrict__ J, int64_t n,
-                                                      const uint32_t* __restrict__ jobs, double* __restrict__ out,
-                                                      int64_t* __restrict__ range) {
+// job-ordered batch, in fixed order: kZseParts workgroups per job each sum a
+// fixed slice of the job's contiguous range (a slice depends only on the
+// job's length, so a job's sum is the same alone or in any mix), then one
+// thread per job adds the slices in order.  The first slice's workgroup also
+// returns the job's range.
+constexpr int kZseParts = 64;
+
+__global__ __launch_bounds__(kBlock) void k_batch_zse_part(int as, SumView s, const uint32_t* __restrict__ J,
+                                                           int64_t n, const uint32_t* __restrict__ jobs,
+                                                           double* __restrict__ part, int64_t* __restrict__ range) {
     __shared__ double red[kWavesPerBlock];
-    const uint32_t job = jobs[blockIdx.x];
-    int64_t lo = 0, hi = n;  // first index with J >= job
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (J[mid] < job) lo = mid + 1; else hi = mid;
+    __shared__ int64_t s_ab[2];
+    const uint32_t f = blockIdx.x / kZseParts, q = blockIdx.x % kZseParts;
+    if (threadIdx.x == 0) {
+        const uint32_t job = jobs[f];
+        int64_t lo = 0, hi = n;  // first index with J >= job
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (J[mid] < job) lo = mid + 1; else hi = mid;
+        }
+        s_ab[0] = lo;
+        hi = n;
+        while (lo < hi) {  // first index with J > job
+            const int64_t mid = (lo + hi) >> 1;
+            if (J[mid] <= job) lo = mid + 1; else hi = mid;
+        }
+        s_ab[1] = lo;
     }
-    const int64_t a = lo;
-    hi = n;
-    while (lo < hi) {  // first index with J > job
-        const int64_t mid = (lo + hi) >> 1;
-        if (J[mid] <= job) lo = mid + 1; else hi = mid;
-    }
-    const int64_t b = lo;
+    __syncthreads();
+    const int64_t a = s_ab[0], b = s_ab[1], len = b - a;
+    const int64_t i0 = a + len * q / kZseParts, i1 = a + len * (q + 1) / kZseParts;
     double acc = 0.0;
-    const int64_t per = (b - a + kBlock - 1) / kBlock;
-    const int64_t i0 = a + (int64_t)threadIdx.x * per, i1 = min(b, i0 + per);
-    for (int64_t i = i0; i < i1; ++i) {
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
         bool has_xy = false;
         for (int w = 0; w < as; ++w) has_xy |= s.x[(int64_t)w * s.stride + i] != 0;
         const uint32_t fl = s.f[i];
@@ -231,10 +240,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_zse(int as, SumView s, const u
     }
     const double tot = block_sum(acc, red);
     if (threadIdx.x == 0) {
-        out[blockIdx.x] = tot;
-        range[2 * blockIdx.x] = a;
-        range[2 * blockIdx.x + 1] = b;
+        part[blockIdx.x] = tot;
+        if (q == 0) {
+            range[2 * f] = a;
+            range[2 * f + 1] = b;
+        }
     }
+}
+
+__global__ void k_batch_zse_final(const double* __restrict__ part, uint32_t F, double* __restrict__ out) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    double acc = 0.0;
+    for (int q = 0; q < kZseParts; ++q) acc += part[(size_t)f * kZseParts + q];
+    out[f] = acc;
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
@@ -753,15 +772,18 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
     std::vector<int64_t> hr(2 * F, 0);
     if (B->n > 0) {
         void* p;
-        PLH_TRY(ctx->ws.get(kWsTmpSum, align256(F * 4) + F * 24 + 256, &p));
+        PLH_TRY(ctx->ws.get(kWsTmpSum, align256(F * 4) + F * 24 + align256(F * kZseParts * 8) + 256, &p));
         uint32_t* d_jobs = (uint32_t*)p;
         double* d_e = (double*)((char*)p + align256(F * 4));
         int64_t* d_rng = (int64_t*)(d_e + F);
+        double* d_part = (double*)((char*)p + align256(F * 4) + align256(F * 24));
         std::vector<uint32_t> hj(F);
         for (size_t i = 0; i < F; ++i) hj[i] = (uint32_t)pl.finishing[i];
         PLH_HIP(hipMemcpyAsync(d_jobs, hj.data(), F * 4, hipMemcpyHostToDevice, st));
-        k_batch_zse<<<dim3((unsigned)F), kBlock, 0, st>>>(B->as, SumView{B->x, B->z, B->f, B->v, B->cap}, B->j,
-                                                           B->n, d_jobs, d_e, d_rng);
+        k_batch_zse_part<<<dim3((unsigned)(F * kZseParts)), kBlock, 0, st>>>(
+            B->as, SumView{B->x, B->z, B->f, B->v, B->cap}, B->j, B->n, d_jobs, d_part, d_rng);
+        PLH_HIP(hipGetLastError());
+        k_batch_zse_final<<<dim3((unsigned)((F + kBlock - 1) / kBlock)), kBlock, 0, st>>>(d_part, (uint32_t)F, d_e);
         PLH_HIP(hipGetLastError());
         PLH_HIP(hipMemcpyAsync(he.data(), d_e, F * 8, hipMemcpyDeviceToHost, st));
         PLH_HIP(hipMemcpyAsync(hr.data(), d_rng, F * 16, hipMemcpyDeviceToHost, st));
