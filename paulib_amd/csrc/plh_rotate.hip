@@ -592,6 +592,18 @@ PLH_DEV T wave_min(T v) {
     return wave_last(wave_scan_op(v, (T)~T(0), [](T a, T b) { return a < b ? a : b; }));
 }
 
+// One sum's window, passed by value to the set-up kernel, which lays it out
+// as a one-job window table.
+template <int AS>
+struct WindowArg {
+    uint8_t wn;
+    uint8_t kind[kJobWindow];
+    uint8_t psf[kJobWindow];
+    uint64_t p[kJobWindow * 2 * AS];
+    double c[kJobWindow];
+    double s[kJobWindow];
+};
+
 // ---- 0. set-up: per job and qubit, the window entries a set bit at that
 // qubit anticommutes with: low 16 bits for the term's z bit (entries whose
 // generator has X or Y there), high 16 bits for its x bit (Z or Y).  A
@@ -601,13 +613,25 @@ PLH_DEV T wave_min(T v) {
 // the first threads also find the job ranges (off, as k_rs_bounds; a sum
 // without a job plane is one job) and reset the per-job minima and the l3
 // minima.
-template <int AS>
-__global__ void k_rj_setup(JobWindows jw, uint32_t njobs, const uint8_t* __restrict__ alive,
+// A single sum (SINGLE) brings its window by value (a), which the set-up
+// also stores as the one-job window table jw for the later kernels.
+template <int AS, bool SINGLE>
+__global__ void k_rj_setup(JobWindows jw, WindowArg<AS> a, uint32_t njobs, const uint8_t* __restrict__ alive,
                            uint32_t* __restrict__ tbl, const uint32_t* __restrict__ J, uint32_t n,
                            uint32_t* __restrict__ off, uint32_t* __restrict__ jmin, uint32_t* __restrict__ l3,
                            uint32_t nl3) {
     constexpr uint32_t Q = 64u * AS;
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (SINGLE) {
+        if (g == 0) *const_cast<uint8_t*>(jw.wn) = a.wn;
+        if (g < (uint64_t)kJobWindow) {
+            const_cast<uint8_t*>(jw.kind)[g] = a.kind[g];
+            const_cast<uint8_t*>(jw.psf)[g] = a.psf[g];
+            const_cast<double*>(jw.c)[g] = a.c[g];
+            const_cast<double*>(jw.s)[g] = a.s[g];
+        }
+        if (g < (uint64_t)kJobWindow * 2 * AS) const_cast<uint64_t*>(jw.p)[g] = a.p[g];
+    }
     if (g < nl3) l3[g] = 0xFFFFu;
     if (g < njobs) jmin[g] = 0xFFu;
     if (g <= njobs) {
@@ -633,11 +657,11 @@ __global__ void k_rj_setup(JobWindows jw, uint32_t njobs, const uint8_t* __restr
     const uint32_t job = (uint32_t)(g / Q), q = (uint32_t)(g % Q);
     const int wd = (int)(q >> 6), b = (int)(q & 63);
     uint32_t mz = 0, mx = 0;
-    const int wn = (!alive || alive[job]) ? (int)jw.wn[job] : 0;
+    const int wn = (!alive || alive[job]) ? (int)(SINGLE ? a.wn : jw.wn[job]) : 0;
     for (int e = 0; e < wn; ++e) {
         const size_t ent = (size_t)job * kJobWindow + (size_t)e;
-        if (jw.kind[ent] != 4) continue;
-        const uint64_t* p = jw.p + ent * 2 * AS;
+        if ((SINGLE ? a.kind[e] : jw.kind[ent]) != 4) continue;
+        const uint64_t* p = SINGLE ? a.p + (size_t)e * 2 * AS : jw.p + ent * 2 * AS;
         if ((p[wd] >> b) & 1ull) mz |= 1u << e;       // X or Y at q: the term's z bit counts
         if ((p[AS + wd] >> b) & 1ull) mx |= 1u << e;  // Z or Y at q: the term's x bit counts
     }
@@ -1336,7 +1360,8 @@ struct FlagBuf {
 // as one job (no job plane, adv_host null, *consumed set).
 template <int AS>
 int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const Filters& fl, uint32_t njobs,
-                          uint8_t* adv_host, const plh_sum* place, int* consumed = nullptr) {
+                          uint8_t* adv_host, const plh_sum* place, int* consumed = nullptr,
+                          const WindowArg<AS>* single = nullptr) {
     hipStream_t st = ctx->stream;
     const uint64_t n = (uint64_t)s->n;
     const uint64_t tiles = (n + kRjTile - 1) / kRjTile;
@@ -1418,9 +1443,17 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
         k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(place->j, (uint32_t)pn, njobs, toff);
         PLH_HIP(hipGetLastError());
     }
-    k_rj_setup<AS><<<blocks_of(std::max<uint64_t>(std::max<uint64_t>((uint64_t)nj * 64 * AS, nj + 1), nl3)), kBlock, 0,
-                     st>>>(jw, njobs, fl.job_alive, qtbl, s->j, (uint32_t)n, joff, jmin, l3, (uint32_t)nl3);
-    PLH_HIP(hipGetLastError());
+    {
+        const uint64_t threads = std::max<uint64_t>(std::max<uint64_t>((uint64_t)nj * 64 * AS, nj + 1), nl3);
+        if (single)
+            k_rj_setup<AS, true><<<blocks_of(threads), kBlock, 0, st>>>(jw, *single, njobs, fl.job_alive, qtbl, s->j,
+                                                                        (uint32_t)n, joff, jmin, l3, (uint32_t)nl3);
+        else
+            k_rj_setup<AS, false><<<blocks_of(threads), kBlock, 0, st>>>(jw, WindowArg<AS>{}, njobs, fl.job_alive, qtbl,
+                                                                         s->j, (uint32_t)n, joff, jmin, l3,
+                                                                         (uint32_t)nl3);
+        PLH_HIP(hipGetLastError());
+    }
     {
         ProfScope ps(ctx, "rot_anti");
         k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, fk, jmin, l0, l1, l2, l3);
@@ -1573,33 +1606,6 @@ int rotate_sorted_as(plh_ctx* ctx, plh_sum* s, const RotParams<AS>& rp, const Jo
 // job (its job word reads as 0).
 constexpr int kSumWindow = 16;
 
-// One sum's window, passed by value to the kernel that lays it out as a
-// one-job window table.
-template <int AS>
-struct WindowArg {
-    uint8_t wn;
-    uint8_t kind[kJobWindow];
-    uint8_t psf[kJobWindow];
-    uint64_t p[kJobWindow * 2 * AS];
-    double c[kJobWindow];
-    double s[kJobWindow];
-};
-
-template <int AS>
-__global__ void k_rj_window1(WindowArg<AS> a, uint8_t* __restrict__ wn, uint8_t* __restrict__ kind,
-                             uint64_t* __restrict__ p, uint8_t* __restrict__ psf, double* __restrict__ c,
-                             double* __restrict__ sn) {
-    const int t = (int)threadIdx.x;
-    if (t == 0) *wn = a.wn;
-    if (t < kJobWindow) {
-        kind[t] = a.kind[t];
-        psf[t] = a.psf[t];
-        c[t] = a.c[t];
-        sn[t] = a.s[t];
-    }
-    for (int k = t; k < kJobWindow * 2 * AS; k += (int)blockDim.x) p[k] = a.p[k];
-}
-
 // A window on a single sorted sum through the batched rotation as one job:
 // the window test, the applied rotation and its output count in one pass
 // with one read-back (the host learns the consumed count with the count).
@@ -1629,10 +1635,8 @@ int rotate_window_single(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const*
     double* d_s = (double*)(b + 1024);
     uint64_t* d_p = (uint64_t*)(b + 1280);  // 16 * 2 * AS words: up to 512 bytes
     static_assert(kJobWindow * 2 * 2 * 8 <= 768, "window generators fit their slot");
-    k_rj_window1<AS><<<1, kBlock, 0, ctx->stream>>>(a, d_wn, d_kind, d_p, d_psf, d_c, d_s);
-    PLH_HIP(hipGetLastError());
     const JobWindows jw{d_wn, d_kind, d_p, d_psf, d_c, d_s};
-    return rotate_sorted_jobs_as<AS>(ctx, s, jw, fl, 1, nullptr, nullptr, consumed);
+    return rotate_sorted_jobs_as<AS>(ctx, s, jw, fl, 1, nullptr, nullptr, consumed, &a);
 }
 
 Filters filters_of(const CollateSpec& spec) {
