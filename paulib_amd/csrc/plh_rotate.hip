@@ -848,6 +848,58 @@ __global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__
     rj_put(fl, tile, ab, sh);
 }
 
+// One workgroup: exclusive scan of n tile counts (out[n] = total), in
+// chunks of kBlock * 8.
+PLH_DEV void rj_block_scan(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, uint64_t* sh) {
+    constexpr int kIpt = 8;
+    uint64_t carry = 0;
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kBlock * kIpt) {
+        const uint64_t base = c0 + (uint64_t)threadIdx.x * kIpt;
+        uint32_t vals[kIpt];
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < kIpt; ++k) {
+            vals[k] = base + k < n ? in[base + k] : 0u;
+            acc += vals[k];
+        }
+        uint64_t tot;
+        uint64_t run = carry + block_exclusive_scan(acc, tot, sh);
+#pragma unroll
+        for (int k = 0; k < kIpt; ++k) {
+            if (base + k < n) out[base + k] = run;
+            run += vals[k];
+        }
+        carry += tot;
+        __syncthreads();  // sh is reused by the next chunk
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+// Sums of at most kLocalTiles tiles (2M terms) skip two launches: the new-
+// term kernel scans the anticommuting tile counts itself, and the keep
+// kernel finds its tile's new terms by a wave-wide search of the lower
+// bounds.
+constexpr uint64_t kLocalTiles = 2048;
+
+// A wave's lower bound of v in the sorted a[0, m): the first r with
+// a[r] >= v (m when none).  Every lane calls it; each round samples 64 evenly
+// spaced entries and keeps the gap that holds the answer.
+PLH_DEV uint32_t wave_lower_bound(const uint32_t* __restrict__ a, uint32_t m, uint64_t v) {
+    uint32_t lo = 0, hi = m;  // the answer lies in [lo, hi]
+    const uint32_t k = lane_id();
+    while (hi - lo > (uint32_t)kWave) {
+        const uint64_t len = hi - lo;
+        const uint32_t pk = lo + (uint32_t)(len * k / kWave);
+        const uint32_t c = (uint32_t)popc(ballot((uint64_t)a[pk] < v));  // samples below v: a prefix of k
+        if (c == 0) return lo;
+        const uint32_t plast = lo + (uint32_t)(len * (c - 1) / kWave);
+        const uint32_t pnext = c < (uint32_t)kWave ? lo + (uint32_t)(len * c / kWave) : hi;
+        lo = plast + 1;
+        hi = pnext;
+    }
+    return lo + (uint32_t)popc(ballot(lo + k < hi && (uint64_t)a[lo + k] < v));
+}
+
 // Wave-wide searches on the common-prefix index (every lane calls them with
 // the same arguments): one coalesced load and a ballot per level.
 PLH_DEV uint64_t hi_bit(uint64_t b) { return 63u - (uint64_t)__clzll(b); }
@@ -935,11 +987,24 @@ PLH_DEV uint64_t wv_next_lt(const LcpIndex& L, uint64_t p, uint32_t thr) {
 // Per new term r, hb[2r], hb[2r + 1] bound its lower bound among the old
 // terms: the other half of its first set bit's group (the new key keeps the
 // bits above that bit and flips it).
+// With a_cnt set (at most kLocalTiles tiles), every workgroup first scans
+// the anticommuting tile counts into LDS for its ranks, and workgroup 0
+// stores the scan as A's tile prefixes for the later kernels.
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
-                                                   LcpIndex L, NewTerms N, uint32_t* __restrict__ hb) {
+                                                   LcpIndex L, NewTerms N, uint32_t* __restrict__ hb,
+                                                   const uint32_t* __restrict__ a_cnt) {
     constexpr int KW = Sorted<AS, true>::KW;
     constexpr int XO = 1;
+    __shared__ uint64_t s_pre[kLocalTiles + 1];
+    __shared__ uint64_t s_red[kWavesPerBlock];
+    if (a_cnt) {
+        rj_block_scan(a_cnt, s_pre, A.tiles, s_red);
+        __syncthreads();
+        if (blockIdx.x == 0)
+            for (uint64_t t = threadIdx.x; t <= A.tiles; t += kBlock) const_cast<uint64_t*>(A.tpre)[t] = s_pre[t];
+        A.tpre = s_pre;
+    }
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint64_t i0 = i - lane_id();
     const uint32_t lane = lane_id();
@@ -1168,13 +1233,32 @@ __global__ void k_rj_tiles(const uint32_t* __restrict__ lb, BitRank A, uint64_t 
 }
 
 // ---- 4. old terms' keep flags (KO)
+// With search set (at most kLocalTiles tiles), the first wave finds the
+// tile's new terms itself and stores tr0[tile] (and tr0[tiles]) for the write
+// kernel; otherwise k_rj_tiles has filled tr0.
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
-                                                    const uint32_t* __restrict__ lb, const uint32_t* __restrict__ tr0,
-                                                    BitRank D, NewTerms N, Filters fl, BitOut koout) {
+                                                    const uint32_t* __restrict__ lb, uint32_t* __restrict__ tr0,
+                                                    BitRank D, NewTerms N, Filters fl, BitOut koout, bool search) {
     __shared__ uint32_t sh[kRjWords];
+    __shared__ uint32_t s_rr[2];
     const uint64_t tile = blockIdx.x;
-    const uint32_t rr[2] = {tr0[tile], tr0[tile + 1]};
+    if (search) {
+        if (threadIdx.x < (unsigned)kWave) {
+            const uint64_t tiles = A.tiles;
+            const uint32_t m = (uint32_t)A.total();
+            const uint32_t r0 = wave_lower_bound(lb, m, tile * kRjTile);
+            const uint32_t r1 = wave_lower_bound(lb, m, (tile + 1) * kRjTile);
+            if (threadIdx.x == 0) {
+                s_rr[0] = r0;
+                s_rr[1] = r1;
+                tr0[tile] = r0;
+                if (tile + 1 == tiles) tr0[tiles] = r1;
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t rr[2] = {search ? s_rr[0] : tr0[tile], search ? s_rr[1] : tr0[tile + 1]};
     bool ko[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1190,33 +1274,6 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
         }
     }
     rj_put(ko, tile, koout, sh);
-}
-
-// One workgroup: exclusive scan of n tile counts (out[n] = total), in
-// chunks of kBlock * 8.
-PLH_DEV void rj_block_scan(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, uint64_t* sh) {
-    constexpr int kIpt = 8;
-    uint64_t carry = 0;
-    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kBlock * kIpt) {
-        const uint64_t base = c0 + (uint64_t)threadIdx.x * kIpt;
-        uint32_t vals[kIpt];
-        uint64_t acc = 0;
-#pragma unroll
-        for (int k = 0; k < kIpt; ++k) {
-            vals[k] = base + k < n ? in[base + k] : 0u;
-            acc += vals[k];
-        }
-        uint64_t tot;
-        uint64_t run = carry + block_exclusive_scan(acc, tot, sh);
-#pragma unroll
-        for (int k = 0; k < kIpt; ++k) {
-            if (base + k < n) out[base + k] = run;
-            run += vals[k];
-        }
-        carry += tot;
-        __syncthreads();  // sh is reused by the next chunk
-    }
-    if (threadIdx.x == 0) out[n] = carry;
 }
 
 // ---- 4b. the kept new (KN) and kept old (KO) terms' tile scans, then the
@@ -1461,21 +1518,25 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
         k_rj_anti<AS><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, jw, jmin, fk, jt, kstar, adv, fa.out());
         PLH_HIP(hipGetLastError());
     }
-    PLH_TRY(scan_u32_to_u64(ctx, fa.tcnt, fa.tpre, tiles, fa.tpre + tiles));
+    // small sums: the tile scan inside k_rj_new, the tile ranges inside k_rj_keep
+    const bool local = tiles <= kLocalTiles;
+    if (!local) PLH_TRY(scan_u32_to_u64(ctx, fa.tcnt, fa.tpre, tiles, fa.tpre + tiles));
     const BitRank A = fa.rank(tiles);
     {
         ProfScope ps(ctx, "rot_new");
-        k_rj_new<AS><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, L, N, hb);
+        k_rj_new<AS><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, L, N, hb, local ? fa.tcnt : nullptr);
         PLH_HIP(hipGetLastError());
         k_rj_lb<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, A, hb, jt, N, fl, lb, fd.out(false), fkn.out());
         PLH_HIP(hipGetLastError());
-        k_rj_tiles<<<blocks_of(n + 1), kBlock, 0, st>>>(lb, A, tiles, (uint32_t)n, tr0);
-        PLH_HIP(hipGetLastError());
+        if (!local) {
+            k_rj_tiles<<<blocks_of(n + 1), kBlock, 0, st>>>(lb, A, tiles, (uint32_t)n, tr0);
+            PLH_HIP(hipGetLastError());
+        }
     }
     const BitRank D = fd.rank(tiles), KN = fkn.rank(tiles);
     {
         ProfScope ps(ctx, "rot_merge");
-        k_rj_keep<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, lb, tr0, D, N, fl, fko.out());
+        k_rj_keep<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, lb, tr0, D, N, fl, fko.out(), local);
         PLH_HIP(hipGetLastError());
     }
     const BitRank KO = fko.rank(tiles);
