@@ -1282,34 +1282,55 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
 // barrier, and thread 0 then posts the count (the host polls its sequence).
 // A single sum (no adv_host) posts its count, consumed rotations and applied
 // window entry (0xFF: none) together.
-__global__ __launch_bounds__(kBlock) void k_rj_counts(const uint32_t* __restrict__ kn_cnt, uint64_t* __restrict__ kn_pre,
-                                                      const uint32_t* __restrict__ ko_cnt, uint64_t* __restrict__ ko_pre,
-                                                      uint64_t tiles, uint64_t extra, uint64_t* __restrict__ out,
-                                                      const uint8_t* __restrict__ adv, const uint8_t* __restrict__ kstar,
-                                                      uint32_t njobs, uint8_t* adv_host, uint64_t* mbox, uint64_t seq) {
-    __shared__ uint64_t sh[kWavesPerBlock];
-    rj_block_scan(kn_cnt, kn_pre, tiles, sh);
-    rj_block_scan(ko_cnt, ko_pre, tiles, sh);
-    if (adv_host)
-        for (uint32_t t = threadIdx.x; t < njobs; t += blockDim.x) adv_host[t] = adv[t];
+struct RjPost {
+    const uint32_t* kn_cnt;
+    uint64_t* kn_pre;
+    const uint32_t* ko_cnt;
+    uint64_t* ko_pre;
+    uint64_t tiles;
+    uint64_t extra;
+    uint64_t* out;
+    const uint8_t* adv;
+    const uint8_t* kstar;
+    uint32_t njobs;
+    uint8_t* adv_host;
+    uint64_t* mbox;
+    uint64_t seq;
+};
+
+// One workgroup, after the scans: the count to *out and the host.
+PLH_DEV void rj_post(const RjPost& rp, uint64_t total) {
+    if (rp.adv_host)
+        for (uint32_t t = threadIdx.x; t < rp.njobs; t += blockDim.x) rp.adv_host[t] = rp.adv[t];
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const uint64_t t = ko_pre[tiles] + kn_pre[tiles] + extra;
-    *out = t;
-    if (adv_host) {
-        post_write(mbox, seq, 1, &t);
+    *rp.out = total;
+    if (rp.adv_host) {
+        post_write(rp.mbox, rp.seq, 1, &total);
     } else {
-        const uint64_t v[3] = {t, adv[0], kstar[0]};
-        post_write(mbox, seq, 3, v);
+        const uint64_t v[3] = {total, rp.adv[0], rp.kstar[0]};
+        post_write(rp.mbox, rp.seq, 3, v);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rj_counts(RjPost rp) {
+    __shared__ uint64_t sh[kWavesPerBlock];
+    rj_block_scan(rp.kn_cnt, rp.kn_pre, rp.tiles, sh);
+    rj_block_scan(rp.ko_cnt, rp.ko_pre, rp.tiles, sh);
+    __syncthreads();
+    rj_post(rp, rp.ko_pre[rp.tiles] + rp.kn_pre[rp.tiles] + rp.extra);
 }
 
 // ---- 5. kept terms to their places: old term i after the kept old terms
 // before it and the kept new terms below it; new term r after the kept new
 // terms before it and the kept old terms below its lower bound; both after
 // the placed terms (toff, when set) of the jobs before their own.  Blocks
-// [0, tiles) take old tiles, [tiles, 2 tiles) new ones.
+// [0, tiles) take old tiles, [tiles, 2 tiles) new ones.  With local set (at
+// most kLocalTiles tiles; the output allocated for the most terms it can
+// take) every workgroup scans the KO and KN tile counts into dynamic LDS
+// itself, and workgroup 0 stores them and posts the count, so the host's
+// wait for it overlaps the writes.
 template <int AS>
 __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_t n, uint64_t tiles, JobRotTables jt,
                                                      BitRank A, const uint32_t* __restrict__ lb,
@@ -1318,8 +1339,29 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
                                                      const uint32_t* __restrict__ toff,
                                                      uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
                                                      uint8_t* __restrict__ of, double* __restrict__ ov,
-                                                     uint32_t* __restrict__ oj, int64_t os) {
-    const uint64_t total = *d_total;
+                                                     uint32_t* __restrict__ oj, int64_t os, RjPost rp, bool local) {
+    extern __shared__ uint64_t dyn[];
+    uint64_t total;
+    if (local) {
+        uint64_t* s_ko = dyn;
+        uint64_t* s_kn = dyn + (tiles + 1);
+        uint64_t* s_red = dyn + 2 * (tiles + 1);
+        rj_block_scan(rp.ko_cnt, s_ko, tiles, s_red);
+        rj_block_scan(rp.kn_cnt, s_kn, tiles, s_red);
+        __syncthreads();
+        KO.tpre = s_ko;
+        KN.tpre = s_kn;
+        total = s_ko[tiles] + s_kn[tiles] + rp.extra;
+        if (blockIdx.x == 0) {
+            for (uint64_t t = threadIdx.x; t <= tiles; t += kBlock) {
+                rp.ko_pre[t] = s_ko[t];
+                rp.kn_pre[t] = s_kn[t];
+            }
+            rj_post(rp, total);
+        }
+    } else {
+        total = *d_total;
+    }
     const uint32_t m = (uint32_t)A.total();
     if (blockIdx.x < tiles) {
         const uint64_t tile = blockIdx.x;
@@ -1542,26 +1584,20 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     const BitRank KO = fko.rank(tiles);
     Post post;
     PLH_TRY(post_prepare(ctx, &post));
-    k_rj_counts<<<1, kBlock, 0, st>>>(fkn.tcnt, fkn.tpre, fko.tcnt, fko.tpre, tiles, pn, d_total, adv, kstar, njobs,
-                                      adv_host, post.mbox, post.seq);
-    PLH_HIP(hipGetLastError());
-    uint64_t rb[3] = {0, 0, 0xFFu};
-    PLH_TRY(post_wait(ctx, post, adv_host ? 1 : 3, rb));
-    const uint64_t total = rb[0];
-    if (!adv_host) {
-        *consumed = (int)rb[1];
-        if (rb[2] == 0xFFu) return PLH_OK;  // no rotation of the window changes a term: the sum stays as it is
-    }
+    const RjPost rp{fkn.tcnt, fkn.tpre, fko.tcnt, fko.tpre, tiles, pn, d_total, adv, kstar, njobs, adv_host,
+                    post.mbox, post.seq};
     plh_sum tmp;
     tmp.ctx = ctx;
     tmp.as = s->as;
     tmp.nq = s->nq;
     tmp.has_job = s->has_job;
-    PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(total, 1), 0));
-    if (total) {
-        ProfScope ps(ctx, "rot_write", (double)total * (16.0 * s->as + 13.0));
-        k_rj_write<AS><<<2 * gt, kBlock, 0, st>>>(S, (uint32_t)n, tiles, jt, A, lb, tr0, D, KO, KN, N, d_total,
-                                                  pn ? toff : nullptr, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
+    uint64_t rb[3] = {0, 0, 0xFFu};
+    auto launch_write = [&](bool lw) -> int {
+        const size_t dyn = lw ? (2 * (tiles + 1) + kWavesPerBlock) * sizeof(uint64_t) : 0;
+        ProfScope ps(ctx, "rot_write", (double)n * (16.0 * s->as + 13.0));
+        k_rj_write<AS><<<2 * gt, kBlock, dyn, st>>>(S, (uint32_t)n, tiles, jt, A, lb, tr0, D, KO, KN, N, d_total,
+                                                    pn ? toff : nullptr, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap,
+                                                    rp, lw);
         if (pn)
             k_rj_place<AS><<<blocks_of(pn), kBlock, 0, st>>>(
                 SumView{place->x, place->z, place->f, place->v, place->cap}, place->j, (uint32_t)pn, joff, A, KO, KN,
@@ -1571,7 +1607,37 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
             sum_free_storage(&tmp);
             PLH_HIP(e);
         }
+        return PLH_OK;
+    };
+    if (local) {
+        // the output sized for every old term, one new term each and the
+        // placed terms; the write posts the count
+        PLH_TRY(sum_reserve(&tmp, (int64_t)(2 * n + pn), 0));
+        PLH_TRY(launch_write(true));
+        const int rc = post_wait(ctx, post, adv_host ? 1 : 3, rb);
+        if (rc != PLH_OK) {
+            sum_free_storage(&tmp);
+            return rc;
+        }
+        if (!adv_host) {
+            *consumed = (int)rb[1];
+            if (rb[2] == 0xFFu) {  // no rotation of the window changes a term: the sum stays as it is
+                sum_free_storage(&tmp);
+                return PLH_OK;
+            }
+        }
+    } else {
+        k_rj_counts<<<1, kBlock, 0, st>>>(rp);
+        PLH_HIP(hipGetLastError());
+        PLH_TRY(post_wait(ctx, post, adv_host ? 1 : 3, rb));
+        if (!adv_host) {
+            *consumed = (int)rb[1];
+            if (rb[2] == 0xFFu) return PLH_OK;  // no rotation of the window changes a term: the sum stays as it is
+        }
+        PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(rb[0], 1), 0));
+        if (rb[0]) PLH_TRY(launch_write(false));
     }
+    const uint64_t total = rb[0];
     const int64_t nq = s->nq;
     sum_take_storage(s, &tmp);
     s->n = (int64_t)total;
