@@ -98,7 +98,9 @@ class Workspace {
    public:
     explicit Workspace(DeviceHeap* heap) : heap_(heap) {}
     ~Workspace() { release(); }
-    int get(int slot, size_t bytes, void** out);
+    // moved (optional): set when the slot was (re)allocated by this call, so
+    // its contents are not what the last caller left there
+    int get(int slot, size_t bytes, void** out, bool* moved = nullptr);
     void release();
     size_t bytes_held() const;
 
@@ -193,6 +195,10 @@ struct plh_ctx {
     std::vector<plh_sum*> sums;  // live sums; orphaned (ctx = null) when the context goes
     std::vector<plh::ProfRec> prof_recs;
     std::vector<hipEvent_t> prof_pool;
+    // sort-free rotation passes: a pass number tags the per-job and 16-tile
+    // minima, which then need no reset (kWsRotE; capacities of its layout)
+    uint64_t rot_epoch = 0;
+    uint64_t rot_jcap = 0, rot_l3cap = 0;
 };
 
 namespace plh {
@@ -238,6 +244,7 @@ enum : int {
     kWsRotN,    // sort-free rotation: the new terms, sorted
     kWsRotM,    // sort-free rotation: merge flags, lower bounds, sums, output scan
     kWsKeep,    // trim with keep: protected patterns and flags
+    kWsRotE,    // sort-free rotation: per-job minima and 16-tile minima, epoch-tagged, kept between passes
     kWsCount
 };
 

@@ -240,7 +240,8 @@ void DeviceHeap::release() {
 
 // ------------------------------------------------------------ workspace
 
-int Workspace::get(int slot, size_t bytes, void** out) {
+int Workspace::get(int slot, size_t bytes, void** out, bool* moved) {
+    if (moved) *moved = false;
     if (slot < 0) return PLH_ERR_ARG;
     if ((size_t)slot >= bufs_.size()) bufs_.resize((size_t)slot + 1);
     Buf& b = bufs_[(size_t)slot];
@@ -267,6 +268,7 @@ int Workspace::get(int slot, size_t bytes, void** out) {
         }
         b.p = np;
         b.n = want;
+        if (moved) *moved = true;
     }
     *out = b.p;
     return PLH_OK;

@@ -582,9 +582,10 @@ struct LcpIndex {
     const uint16_t* l0;
     const uint16_t* l1;
     const uint16_t* l2;
-    const uint32_t* l3;
+    const uint64_t* l3;  // tagged with the pass (bits 16 and up); the minimum in the low 16 bits
     uint64_t n;
     uint64_t tiles;
+    PLH_DEV uint32_t l3v(uint64_t u) const { return (uint32_t)(l3[u] & 0xFFFFull); }
 };
 
 template <class T>
@@ -604,68 +605,77 @@ struct WindowArg {
     double s[kJobWindow];
 };
 
-// ---- 0. set-up: per job and qubit, the window entries a set bit at that
-// qubit anticommutes with: low 16 bits for the term's z bit (entries whose
-// generator has X or Y there), high 16 bits for its x bit (Z or Y).  A
-// term's anticommutation with entry e is the parity of bit e over its set
-// bits, so one pass over a term's (few) set bits tests the whole window.
-// Entries of kind 0 (no change) are left out.  One thread per (job, qubit);
-// the first threads also find the job ranges (off, as k_rs_bounds; a sum
-// without a job plane is one job) and reset the per-job minima and the l3
-// minima.
-// A single sum (SINGLE) brings its window by value (a), which the set-up
-// also stores as the one-job window table jw for the later kernels.
+// The windows of a pass: per job from the step tables (jw), or a single
+// sum's one window by value (a).
 template <int AS, bool SINGLE>
-__global__ void k_rj_setup(JobWindows jw, WindowArg<AS> a, uint32_t njobs, const uint8_t* __restrict__ alive,
+struct WinView {
+    JobWindows jw;
+    WindowArg<AS> a;
+    PLH_DEV uint32_t wn(uint32_t job) const { return SINGLE ? a.wn : jw.wn[job]; }
+    PLH_DEV uint32_t kind(uint32_t job, uint32_t e) const {
+        return SINGLE ? a.kind[e] : jw.kind[(size_t)job * kJobWindow + e];
+    }
+    PLH_DEV uint64_t p(uint32_t job, uint32_t e, int k) const {
+        return SINGLE ? a.p[e * 2 * AS + k] : jw.p[((size_t)job * kJobWindow + e) * 2 * AS + k];
+    }
+    PLH_DEV uint32_t psf(uint32_t job, uint32_t e) const {
+        return SINGLE ? a.psf[e] : jw.psf[(size_t)job * kJobWindow + e];
+    }
+    PLH_DEV double c(uint32_t job, uint32_t e) const { return SINGLE ? a.c[e] : jw.c[(size_t)job * kJobWindow + e]; }
+    PLH_DEV double s(uint32_t job, uint32_t e) const { return SINGLE ? a.s[e] : jw.s[(size_t)job * kJobWindow + e]; }
+};
+
+// Qubit table entry (job, q): the window entries a set bit at qubit q
+// anticommutes with, low 16 bits for the term's z bit (entries whose
+// generator has X or Y there), high 16 bits for its x bit (Z or Y).  Entries
+// of kind 0 (no change) are left out.
+template <int AS, bool SINGLE>
+PLH_DEV uint32_t qubit_entry(const WinView<AS, SINGLE>& win, uint32_t job, uint32_t q, bool alive) {
+    const int wd = (int)(q >> 6), b = (int)(q & 63);
+    uint32_t mz = 0, mx = 0;
+    const int wn = alive ? (int)win.wn(job) : 0;
+    for (int e = 0; e < wn; ++e) {
+        if (win.kind(job, (uint32_t)e) != 4) continue;
+        if ((win.p(job, (uint32_t)e, wd) >> b) & 1ull) mz |= 1u << e;       // X or Y at q: the term's z bit counts
+        if ((win.p(job, (uint32_t)e, AS + wd) >> b) & 1ull) mx |= 1u << e;  // Z or Y at q: the term's x bit counts
+    }
+    return mz | (mx << 16);
+}
+
+// The pass tag of the per-job minima (jmin: tag << 8 | entry) and the
+// 16-tile minima (l3: tag << 16 | value): it falls with every pass, so this
+// pass's atomic minima win over whatever earlier passes left, and no reset
+// is needed; the buffer is filled with ones once, when it is (re)allocated.
+constexpr uint64_t kPassTagTop = (1ull << 47) - 1;
+
+// ---- 0. set-up (batched sums): the qubit tables, one thread per (job,
+// qubit); the first threads also find the job ranges (off, as k_rs_bounds).
+// A term's anticommutation with entry e is the parity of bit e over its set
+// bits, so one pass over a term's (few) set bits tests the whole window.
+template <int AS>
+__global__ void k_rj_setup(JobWindows jw, uint32_t njobs, const uint8_t* __restrict__ alive,
                            uint32_t* __restrict__ tbl, const uint32_t* __restrict__ J, uint32_t n,
-                           uint32_t* __restrict__ off, uint32_t* __restrict__ jmin, uint32_t* __restrict__ l3,
-                           uint32_t nl3) {
+                           uint32_t* __restrict__ off) {
     constexpr uint32_t Q = 64u * AS;
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (SINGLE) {
-        if (g == 0) *const_cast<uint8_t*>(jw.wn) = a.wn;
-        if (g < (uint64_t)kJobWindow) {
-            const_cast<uint8_t*>(jw.kind)[g] = a.kind[g];
-            const_cast<uint8_t*>(jw.psf)[g] = a.psf[g];
-            const_cast<double*>(jw.c)[g] = a.c[g];
-            const_cast<double*>(jw.s)[g] = a.s[g];
-        }
-        if (g < (uint64_t)kJobWindow * 2 * AS) const_cast<uint64_t*>(jw.p)[g] = a.p[g];
-    }
-    if (g < nl3) l3[g] = 0xFFFFu;
-    if (g < njobs) jmin[g] = 0xFFu;
     if (g <= njobs) {
         const uint32_t k = (uint32_t)g;
-        uint32_t lo = 0;
-        if (!J) {
-            lo = k == 0 ? 0u : n;
-        } else {
-            uint32_t cnt = n;
-            while (cnt > 0) {
-                const uint32_t half = cnt / 2, t = lo + half;
-                if (J[t] < k) {
-                    lo = t + 1;
-                    cnt -= half + 1;
-                } else {
-                    cnt = half;
-                }
+        uint32_t lo = 0, cnt = n;
+        while (cnt > 0) {
+            const uint32_t half = cnt / 2, t = lo + half;
+            if (J[t] < k) {
+                lo = t + 1;
+                cnt -= half + 1;
+            } else {
+                cnt = half;
             }
         }
         off[k] = lo;
     }
     if (g >= (uint64_t)njobs * Q) return;
-    const uint32_t job = (uint32_t)(g / Q), q = (uint32_t)(g % Q);
-    const int wd = (int)(q >> 6), b = (int)(q & 63);
-    uint32_t mz = 0, mx = 0;
-    const int wn = (!alive || alive[job]) ? (int)(SINGLE ? a.wn : jw.wn[job]) : 0;
-    for (int e = 0; e < wn; ++e) {
-        const size_t ent = (size_t)job * kJobWindow + (size_t)e;
-        if ((SINGLE ? a.kind[e] : jw.kind[ent]) != 4) continue;
-        const uint64_t* p = SINGLE ? a.p + (size_t)e * 2 * AS : jw.p + ent * 2 * AS;
-        if ((p[wd] >> b) & 1ull) mz |= 1u << e;       // X or Y at q: the term's z bit counts
-        if ((p[AS + wd] >> b) & 1ull) mx |= 1u << e;  // Z or Y at q: the term's x bit counts
-    }
-    tbl[g] = mz | (mx << 16);
+    const uint32_t job = (uint32_t)(g / Q);
+    const WinView<AS, false> win{jw, {}};
+    tbl[g] = qubit_entry<AS, false>(win, job, (uint32_t)(g % Q), !alive || alive[job]);
 }
 
 // ---- 1. per term the first rotation of its job's window it anticommutes
@@ -691,11 +701,13 @@ PLH_DEV uint32_t first_hit(const uint32_t* t, const uint64_t* xw, const uint64_t
     return par ? (uint32_t)__builtin_ctz(par) : 0xFFu;
 }
 
-template <int AS>
+// A single sum (SINGLE) builds its one qubit table in LDS from the window.
+template <int AS, bool SINGLE>
 __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_t n, const uint32_t* __restrict__ tbl,
-                                                     uint8_t* __restrict__ fk, uint32_t* __restrict__ jmin,
-                                                     uint16_t* __restrict__ l0, uint16_t* __restrict__ l1,
-                                                     uint16_t* __restrict__ l2, uint32_t* __restrict__ l3) {
+                                                     WinView<AS, SINGLE> win, uint8_t* __restrict__ fk,
+                                                     uint64_t* __restrict__ jmin, uint16_t* __restrict__ l0,
+                                                     uint16_t* __restrict__ l1, uint16_t* __restrict__ l2,
+                                                     uint64_t* __restrict__ l3, uint64_t tag) {
     constexpr uint32_t Q = 64u * AS;
     __shared__ uint32_t s_tbl[kStageJobs * Q];
     __shared__ uint32_t s_min[kStageJobs];
@@ -711,8 +723,11 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
     __syncthreads();
     const uint32_t j0 = s_j0, nj = s_nj;
     const bool staged = nj <= (uint32_t)kStageJobs;
-    if (staged)
+    if constexpr (SINGLE) {
+        for (uint32_t t = threadIdx.x; t < Q; t += kBlock) s_tbl[t] = qubit_entry<AS, true>(win, 0, t, true);
+    } else if (staged) {
         for (uint32_t t = threadIdx.x; t < nj * Q; t += kBlock) s_tbl[t] = tbl[(size_t)j0 * Q + t];
+    }
     if (threadIdx.x < (unsigned)kStageJobs) s_min[threadIdx.x] = 0xFFu;
     __syncthreads();
     uint32_t job[4];
@@ -760,7 +775,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
             uint32_t m = s_lw[0];
             for (int w = 1; w < kRjWords; ++w) m = s_lw[w] < m ? s_lw[w] : m;
             l2[tile] = (uint16_t)m;
-            atomicMin(&l3[tile >> 4], m);
+            atomicMin((unsigned long long*)&l3[tile >> 4], (unsigned long long)((tag << 16) | m));
         }
     }
 #pragma unroll
@@ -783,17 +798,18 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
                 if (staged)
                     atomicMin(&s_min[jf - j0], m);
                 else
-                    atomicMin(&jmin[jf], m);
+                    atomicMin((unsigned long long*)&jmin[jf], (unsigned long long)((tag << 8) | m));
             }
         } else if (f < 0xFFu) {
             if (staged)
                 atomicMin(&s_min[job[k] - j0], f);
             else
-                atomicMin(&jmin[job[k]], f);
+                atomicMin((unsigned long long*)&jmin[job[k]], (unsigned long long)((tag << 8) | f));
         }
     }
     __syncthreads();
-    if (staged && threadIdx.x < nj && s_min[threadIdx.x] < 0xFFu) atomicMin(&jmin[j0 + threadIdx.x], s_min[threadIdx.x]);
+    if (staged && threadIdx.x < nj && s_min[threadIdx.x] < 0xFFu)
+        atomicMin((unsigned long long*)&jmin[j0 + threadIdx.x], (unsigned long long)((tag << 8) | s_min[threadIdx.x]));
 }
 
 // ---- 1b. per job the rotation it applies (the first of its window that
@@ -802,15 +818,18 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
 // term); the first workgroup writes them as this step's rotation table.
 // Every term is anticommuting when its first hit is its job's applied entry
 // (every term's first hit is at least that one).
-PLH_DEV uint32_t rj_applied(const JobWindows& jw, const uint32_t* jmin, uint32_t job) {
-    const uint32_t w = jw.wn[job], m = jmin[job];
+template <int AS, bool SINGLE>
+PLH_DEV uint32_t rj_applied(const WinView<AS, SINGLE>& win, const uint64_t* jmin, uint64_t tag, uint32_t job) {
+    const uint32_t w = win.wn(job);
+    const uint64_t v = jmin[job];
+    const uint32_t m = (v >> 8) == tag ? (uint32_t)(v & 0xFFu) : 0xFFu;  // no hit this pass: an older tag
     return (w != 0 && m < w) ? m : 0xFFu;
 }
 
-template <int AS>
+template <int AS, bool SINGLE>
 __global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs,
-                                                    JobWindows jw, const uint32_t* __restrict__ jmin,
-                                                    const uint8_t* __restrict__ fk, JobRotTables jt,
+                                                    WinView<AS, SINGLE> win, const uint64_t* __restrict__ jmin,
+                                                    uint64_t tag, const uint8_t* __restrict__ fk, JobRotTables jt,
                                                     uint8_t* __restrict__ kstar, uint8_t* __restrict__ adv, BitOut ab) {
     __shared__ uint32_t sh[kRjWords];
     const uint64_t tile = blockIdx.x;
@@ -821,18 +840,17 @@ __global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__
         double* c = const_cast<double*>(jt.c);
         double* sn = const_cast<double*>(jt.s);
         for (uint32_t t = threadIdx.x; t < njobs; t += kBlock) {
-            const uint32_t w = jw.wn[t], m = rj_applied(jw, jmin, t);
+            const uint32_t w = win.wn(t), m = rj_applied<AS, SINGLE>(win, jmin, tag, t);
             kind[t] = 0;
             kstar[t] = (uint8_t)m;
             adv[t] = (uint8_t)(m == 0xFFu ? w : m + 1);
             if (m == 0xFFu) continue;
-            const size_t ent = (size_t)t * kJobWindow + m;
-            kind[t] = jw.kind[ent];
+            kind[t] = (uint8_t)win.kind(t, m);
 #pragma unroll
-            for (int k = 0; k < 2 * AS; ++k) p[(size_t)t * 2 * AS + k] = jw.p[ent * 2 * AS + k];
-            psf[t] = jw.psf[ent];
-            c[t] = jw.c[ent];
-            sn[t] = jw.s[ent];
+            for (int k = 0; k < 2 * AS; ++k) p[(size_t)t * 2 * AS + k] = win.p(t, m, k);
+            psf[t] = (uint8_t)win.psf(t, m);
+            c[t] = win.c(t, m);
+            sn[t] = win.s(t, m);
         }
     }
     bool fl[4];
@@ -841,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__
         const uint64_t i = rj_item(tile, k);
         fl[k] = false;
         if (i < n) {
-            const uint32_t ks = rj_applied(jw, jmin, J ? J[i] : 0u);
+            const uint32_t ks = rj_applied<AS, SINGLE>(win, jmin, tag, J ? J[i] : 0u);
             fl[k] = ks != 0xFFu && fk[i] == ks;
         }
     }
@@ -934,7 +952,7 @@ PLH_DEV uint64_t wv_prev_lt(const LcpIndex& L, uint64_t p, uint32_t thr) {
     uint64_t U = T >> 4;  // blocks [0, U) remain
     while (U > 0) {
         const uint64_t u0 = U >= 64 ? U - 64 : 0;
-        b = ballot(u0 + k < U && L.l3[u0 + k] < thr);
+        b = ballot(u0 + k < U && L.l3v(u0 + k) < thr);
         if (b) return wv_block_last(L, u0 + hi_bit(b), thr);
         U = u0;
     }
@@ -970,7 +988,7 @@ PLH_DEV uint64_t wv_next_lt(const LcpIndex& L, uint64_t p, uint32_t thr) {
     if (b) return clamp(wv_tile_first(L, (T & ~15ull) + lo_bit(b), thr));
     const uint64_t nu = (L.tiles + 15) >> 4;
     for (uint64_t u0 = (T >> 4) + 1; u0 < nu; u0 += 64) {
-        b = ballot(u0 + k < nu && L.l3[u0 + k] < thr);
+        b = ballot(u0 + k < nu && L.l3v(u0 + k) < thr);
         if (b) return clamp(wv_block_first(L, u0 + lo_bit(b), thr));
     }
     return L.n;
@@ -1455,6 +1473,38 @@ struct FlagBuf {
     BitRank rank(uint64_t tiles) const { return BitRank{bits, wpre, tpre, tiles * kRjTile, tiles}; }
 };
 
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// This pass's tag and the pass-tagged per-job minima (jmin, at least njobs)
+// and 16-tile minima (l3, at least nl3), kept in their own workspace slot:
+// filled with ones whenever the slot or its layout changes, after which
+// every pass's tag is below every earlier value.
+int rot_pass_state(plh_ctx* ctx, uint64_t njobs, uint64_t nl3, uint64_t** jmin, uint64_t** l3, uint64_t* tag) {
+    const uint64_t jcap = std::max<uint64_t>(ctx->rot_jcap, std::max<uint64_t>(256, njobs));
+    const uint64_t l3cap = std::max<uint64_t>(ctx->rot_l3cap, std::max<uint64_t>(1ull << 17, nl3));
+    const size_t bytes = up256(jcap * 8) + up256(l3cap * 8);
+    void* p;
+    bool moved = false;
+    PLH_TRY(ctx->ws.get(kWsRotE, bytes, &p, &moved));
+    if (moved || jcap != ctx->rot_jcap || l3cap != ctx->rot_l3cap) {
+        k_fill_u64<<<blocks_of(bytes / 8), kBlock, 0, ctx->stream>>>((uint64_t*)p, bytes / 8, ~0ull);
+        PLH_HIP(hipGetLastError());
+        ctx->rot_jcap = jcap;
+        ctx->rot_l3cap = l3cap;
+    }
+    if (ctx->rot_epoch + 2 >= kPassTagTop) {
+        set_error("sort-free rotation: pass tags exhausted");
+        return PLH_ERR_STATE;
+    }
+    *tag = kPassTagTop - (++ctx->rot_epoch);
+    *jmin = (uint64_t*)p;
+    *l3 = (uint64_t*)((char*)p + up256(jcap * 8));
+    return PLH_OK;
+}
+
 // A batched sum (adv_host: each job's consumed rotations), or a single sum
 // as one job (no job plane, adv_host null, *consumed set).
 template <int AS>
@@ -1470,13 +1520,15 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     // rotation and consumed count; per term its first window hit; the new
     // terms (at most n) and their lower bounds
     const size_t nj = njobs;
-    const size_t job_bytes = up256((nj + 1) * 4) + up256(nj * 4) + up256(nj) + up256(nj * 2 * AS * 8) + up256(nj) +
+    const size_t job_bytes = up256((nj + 1) * 4) + up256(nj) + up256(nj * 2 * AS * 8) + up256(nj) +
                              2 * up256(nj * 8) + up256(nj) + up256(nj) + up256(16);
     // the common-prefix index: l0 per term (whole tiles), l1 per 64 terms,
-    // l2 per tile, l3 per 16 tiles
+    // l2 per tile, l3 per 16 tiles (with the per-job minima, in the
+    // pass-tagged buffer)
     const uint64_t nl3 = (tiles + 15) / 16;
-    const size_t lcp_bytes = up256(tiles * kRjTile * 2) + up256(tiles * kRjWords * 2) + up256(tiles * 2 + 16) +
-                             up256(nl3 * 4);
+    const size_t lcp_bytes = up256(tiles * kRjTile * 2) + up256(tiles * kRjWords * 2) + up256(tiles * 2 + 16);
+    uint64_t *jmin, *l3, tag;
+    PLH_TRY(rot_pass_state(ctx, nj, nl3, &jmin, &l3, &tag));
     void* p;
     PLH_TRY(ctx->ws.get(kWsRotA, 4 * FlagBuf::bytes(tiles) + job_bytes + up256(n) + up256((tiles + 1) * 4) +
                                      up256(nj * 64 * AS * 4) + up256((nj + 1) * 4) + lcp_bytes + up256(n * 8),
@@ -1489,8 +1541,6 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     fko.carve(c, tiles);
     uint32_t* joff = (uint32_t*)c;
     c += up256((nj + 1) * 4);
-    uint32_t* jmin = (uint32_t*)c;
-    c += up256(nj * 4);
     uint8_t* e_kind = (uint8_t*)c;
     c += up256(nj);
     uint64_t* e_p = (uint64_t*)c;
@@ -1521,8 +1571,6 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
     c += up256(tiles * kRjWords * 2);
     uint16_t* l2 = (uint16_t*)c;
     c += up256(tiles * 2 + 16);
-    uint32_t* l3 = (uint32_t*)c;
-    c += up256(nl3 * 4);
     uint32_t* hb = (uint32_t*)c;  // per new term: the range of its lower bound
     const LcpIndex L{l0, l1, l2, l3, n, tiles};
     const uint64_t pn = place ? (uint64_t)place->n : 0;
@@ -1543,21 +1591,24 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
         PLH_HIP(hipGetLastError());
     }
     {
-        const uint64_t threads = std::max<uint64_t>(std::max<uint64_t>((uint64_t)nj * 64 * AS, nj + 1), nl3);
-        if (single)
-            k_rj_setup<AS, true><<<blocks_of(threads), kBlock, 0, st>>>(jw, *single, njobs, fl.job_alive, qtbl, s->j,
-                                                                        (uint32_t)n, joff, jmin, l3, (uint32_t)nl3);
-        else
-            k_rj_setup<AS, false><<<blocks_of(threads), kBlock, 0, st>>>(jw, WindowArg<AS>{}, njobs, fl.job_alive, qtbl,
-                                                                         s->j, (uint32_t)n, joff, jmin, l3,
-                                                                         (uint32_t)nl3);
-        PLH_HIP(hipGetLastError());
-    }
-    {
         ProfScope ps(ctx, "rot_anti");
-        k_rj_first<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, fk, jmin, l0, l1, l2, l3);
-        PLH_HIP(hipGetLastError());
-        k_rj_anti<AS><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, jw, jmin, fk, jt, kstar, adv, fa.out());
+        if (single) {
+            // one sum: its qubit table is built by each workgroup of the test
+            const WinView<AS, true> win{JobWindows{}, *single};
+            k_rj_first<AS, true><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, nullptr, win, fk, jmin, l0, l1, l2, l3, tag);
+            PLH_HIP(hipGetLastError());
+            k_rj_anti<AS, true><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, win, jmin, tag, fk, jt, kstar, adv,
+                                                       fa.out());
+        } else {
+            k_rj_setup<AS><<<blocks_of(std::max<uint64_t>((uint64_t)nj * 64 * AS, nj + 1)), kBlock, 0, st>>>(
+                jw, njobs, fl.job_alive, qtbl, s->j, (uint32_t)n, joff);
+            PLH_HIP(hipGetLastError());
+            const WinView<AS, false> win{jw, {}};
+            k_rj_first<AS, false><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, win, fk, jmin, l0, l1, l2, l3, tag);
+            PLH_HIP(hipGetLastError());
+            k_rj_anti<AS, false><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, win, jmin, tag, fk, jt, kstar, adv,
+                                                        fa.out());
+        }
         PLH_HIP(hipGetLastError());
     }
     // small sums: the tile scan inside k_rj_new, the tile ranges inside k_rj_keep
@@ -1752,18 +1803,7 @@ int rotate_window_single(plh_ctx* ctx, plh_sum* s, int w, const uint64_t* const*
             a.p[e * 2 * AS + AS + k] = pz[e][k];
         }
     }
-    void* p;
-    PLH_TRY(ctx->ws.get(kWsRotM, 8 * 256, &p));
-    char* b = (char*)p;
-    uint8_t* d_wn = (uint8_t*)b;
-    uint8_t* d_kind = (uint8_t*)(b + 256);
-    uint8_t* d_psf = (uint8_t*)(b + 512);
-    double* d_c = (double*)(b + 768);
-    double* d_s = (double*)(b + 1024);
-    uint64_t* d_p = (uint64_t*)(b + 1280);  // 16 * 2 * AS words: up to 512 bytes
-    static_assert(kJobWindow * 2 * 2 * 8 <= 768, "window generators fit their slot");
-    const JobWindows jw{d_wn, d_kind, d_p, d_psf, d_c, d_s};
-    return rotate_sorted_jobs_as<AS>(ctx, s, jw, fl, 1, nullptr, nullptr, consumed, &a);
+    return rotate_sorted_jobs_as<AS>(ctx, s, JobWindows{}, fl, 1, nullptr, nullptr, consumed, &a);
 }
 
 Filters filters_of(const CollateSpec& spec) {
