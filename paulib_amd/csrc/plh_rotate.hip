@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "plh_block.hpp"
 #include "plh_engine.hpp"
@@ -524,22 +525,28 @@ struct BitOut {
     uint32_t* tcnt;
 };
 
-// Term k (0..3) of this thread in tile t: t * 1024 + wave * 256 + k * 64 + lane,
-// so word t * 16 + wave * 4 + k holds it.
+// A tile of 1024 terms is taken by a workgroup of 1024 / IPT threads, each
+// holding IPT terms: 4 per thread (256 threads) for large sums, 1 per
+// thread (1024 threads) for small ones, whose passes have too few waves to
+// hide their loads' latency.  Term k (0..IPT-1) of this thread in tile t:
+// t * 1024 + wave * 64 IPT + k * 64 + lane, so word t * 16 + wave * IPT + k
+// holds it.
+template <int IPT>
 PLH_DEV uint64_t rj_item(uint64_t tile, int k) {
-    return tile * kRjTile + (uint64_t)(threadIdx.x / kWave) * (4 * kWave) + (uint64_t)k * kWave + lane_id();
+    return tile * kRjTile + (uint64_t)(threadIdx.x / kWave) * (IPT * kWave) + (uint64_t)k * kWave + lane_id();
 }
 
 // Every thread of the block: writes the tile's flag words, their in-tile
 // prefixes and the tile count.  sh: kRjWords entries of LDS.
-PLH_DEV void rj_put(const bool (&fl)[4], uint64_t tile, const BitOut& o, uint32_t* sh) {
+template <int IPT>
+PLH_DEV void rj_put(const bool (&fl)[IPT], uint64_t tile, const BitOut& o, uint32_t* sh) {
     const int wv = (int)(threadIdx.x / kWave);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < IPT; ++k) {
         const uint64_t b = ballot(fl[k]);
         if (lane_id() == 0) {
-            o.bits[tile * kRjWords + (uint64_t)(wv * 4 + k)] = b;
-            sh[wv * 4 + k] = (uint32_t)popc(b);
+            o.bits[tile * kRjWords + (uint64_t)(wv * IPT + k)] = b;
+            sh[wv * IPT + k] = (uint32_t)popc(b);
         }
     }
     __syncthreads();
@@ -702,8 +709,8 @@ PLH_DEV uint32_t first_hit(const uint32_t* t, const uint64_t* xw, const uint64_t
 }
 
 // A single sum (SINGLE) builds its one qubit table in LDS from the window.
-template <int AS, bool SINGLE>
-__global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_t n, const uint32_t* __restrict__ tbl,
+template <int AS, bool SINGLE, int IPT>
+__global__ __launch_bounds__(kRjTile / IPT) void k_rj_first(Sorted<AS, true> S, uint32_t n, const uint32_t* __restrict__ tbl,
                                                      WinView<AS, SINGLE> win, uint8_t* __restrict__ fk,
                                                      uint64_t* __restrict__ jmin, uint16_t* __restrict__ l0,
                                                      uint16_t* __restrict__ l1, uint16_t* __restrict__ l2,
@@ -724,17 +731,17 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
     const uint32_t j0 = s_j0, nj = s_nj;
     const bool staged = nj <= (uint32_t)kStageJobs;
     if constexpr (SINGLE) {
-        for (uint32_t t = threadIdx.x; t < Q; t += kBlock) s_tbl[t] = qubit_entry<AS, true>(win, 0, t, true);
+        for (uint32_t t = threadIdx.x; t < Q; t += kRjTile / IPT) s_tbl[t] = qubit_entry<AS, true>(win, 0, t, true);
     } else if (staged) {
-        for (uint32_t t = threadIdx.x; t < nj * Q; t += kBlock) s_tbl[t] = tbl[(size_t)j0 * Q + t];
+        for (uint32_t t = threadIdx.x; t < nj * Q; t += kRjTile / IPT) s_tbl[t] = tbl[(size_t)j0 * Q + t];
     }
     if (threadIdx.x < (unsigned)kStageJobs) s_min[threadIdx.x] = 0xFFu;
     __syncthreads();
-    uint32_t job[4];
-    uint64_t xw[4][AS], zw[4][AS];
+    uint32_t job[IPT];
+    uint64_t xw[IPT][AS], zw[IPT][AS];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint64_t i = rj_item(tile, k);
+    for (int k = 0; k < IPT; ++k) {
+        const uint64_t i = rj_item<IPT>(tile, k);
         job[k] = i < n ? S.job((uint32_t)i) : 0xFFFFFFFFu;
 #pragma unroll
         for (int w = 0; w < AS; ++w) {
@@ -747,8 +754,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
     {
         const int wv = (int)(threadIdx.x / kWave);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t i = rj_item(tile, k);
+        for (int k = 0; k < IPT; ++k) {
+            const uint64_t i = rj_item<IPT>(tile, k);
             uint32_t v = 0;
             if (i < n && i > 0) {
                 uint64_t d[2 * AS + 1];
@@ -767,7 +774,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
             const uint32_t m = wave_min(v);
             if (lane_id() == 0) {
                 l1[i >> 6] = (uint16_t)m;
-                s_lw[wv * 4 + k] = m;
+                s_lw[wv * IPT + k] = m;
             }
         }
         __syncthreads();
@@ -779,8 +786,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_first(Sorted<AS, true> S, uint32_
         }
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint64_t i = rj_item(tile, k);
+    for (int k = 0; k < IPT; ++k) {
+        const uint64_t i = rj_item<IPT>(tile, k);
         uint32_t f = 0xFFu;
         if (i < n) {
             const uint32_t jb = job[k];
@@ -826,8 +833,8 @@ PLH_DEV uint32_t rj_applied(const WinView<AS, SINGLE>& win, const uint64_t* jmin
     return (w != 0 && m < w) ? m : 0xFFu;
 }
 
-template <int AS, bool SINGLE>
-__global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs,
+template <int AS, bool SINGLE, int IPT>
+__global__ __launch_bounds__(kRjTile / IPT) void k_rj_anti(const uint32_t* __restrict__ J, uint32_t n, uint32_t njobs,
                                                     WinView<AS, SINGLE> win, const uint64_t* __restrict__ jmin,
                                                     uint64_t tag, const uint8_t* __restrict__ fk, JobRotTables jt,
                                                     uint8_t* __restrict__ kstar, uint8_t* __restrict__ adv, BitOut ab) {
@@ -839,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__
         uint8_t* psf = const_cast<uint8_t*>(jt.psf);
         double* c = const_cast<double*>(jt.c);
         double* sn = const_cast<double*>(jt.s);
-        for (uint32_t t = threadIdx.x; t < njobs; t += kBlock) {
+        for (uint32_t t = threadIdx.x; t < njobs; t += kRjTile / IPT) {
             const uint32_t w = win.wn(t), m = rj_applied<AS, SINGLE>(win, jmin, tag, t);
             kind[t] = 0;
             kstar[t] = (uint8_t)m;
@@ -853,25 +860,26 @@ __global__ __launch_bounds__(kBlock) void k_rj_anti(const uint32_t* __restrict__
             sn[t] = win.s(t, m);
         }
     }
-    bool fl[4];
+    bool fl[IPT];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint64_t i = rj_item(tile, k);
+    for (int k = 0; k < IPT; ++k) {
+        const uint64_t i = rj_item<IPT>(tile, k);
         fl[k] = false;
         if (i < n) {
             const uint32_t ks = rj_applied<AS, SINGLE>(win, jmin, tag, J ? J[i] : 0u);
             fl[k] = ks != 0xFFu && fk[i] == ks;
         }
     }
-    rj_put(fl, tile, ab, sh);
+    rj_put<IPT>(fl, tile, ab, sh);
 }
 
-// One workgroup: exclusive scan of n tile counts (out[n] = total), in
-// chunks of kBlock * 8.
+// One workgroup of NT threads: exclusive scan of n tile counts (out[n] =
+// total), in chunks of NT * 8.  sh: NT / 64 entries of LDS.
+template <int NT>
 PLH_DEV void rj_block_scan(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, uint64_t* sh) {
     constexpr int kIpt = 8;
     uint64_t carry = 0;
-    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kBlock * kIpt) {
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)NT * kIpt) {
         const uint64_t base = c0 + (uint64_t)threadIdx.x * kIpt;
         uint32_t vals[kIpt];
         uint64_t acc = 0;
@@ -881,7 +889,7 @@ PLH_DEV void rj_block_scan(const uint32_t* __restrict__ in, uint64_t* __restrict
             acc += vals[k];
         }
         uint64_t tot;
-        uint64_t run = carry + block_exclusive_scan(acc, tot, sh);
+        uint64_t run = carry + block_exclusive_scan<NT / kWave>(acc, tot, sh);
 #pragma unroll
         for (int k = 0; k < kIpt; ++k) {
             if (base + k < n) out[base + k] = run;
@@ -1017,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t 
     __shared__ uint64_t s_pre[kLocalTiles + 1];
     __shared__ uint64_t s_red[kWavesPerBlock];
     if (a_cnt) {
-        rj_block_scan(a_cnt, s_pre, A.tiles, s_red);
+        rj_block_scan<kBlock>(a_cnt, s_pre, A.tiles, s_red);
         __syncthreads();
         if (blockIdx.x == 0)
             for (uint64_t t = threadIdx.x; t <= A.tiles; t += kBlock) const_cast<uint64_t*>(A.tpre)[t] = s_pre[t];
@@ -1145,8 +1153,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_new(Sorted<AS, true> S, uint32_t 
 // key words before that bit's word equal the new key's), whether it equals
 // that old term (D), and whether it is kept as a term of its own (KN: not
 // equal to an old term, and passing the filters)
-template <int AS>
-__global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, uint32_t n, BitRank A, const uint32_t* __restrict__ hb,
+template <int AS, int IPT>
+__global__ __launch_bounds__(kRjTile / IPT) void k_rj_lb(Sorted<AS, true> S, uint32_t n, BitRank A, const uint32_t* __restrict__ hb,
                                                   JobRotTables jt, NewTerms N, Filters fl, uint32_t* __restrict__ lb,
                                                   BitOut dout, BitOut knout) {
     constexpr int KW = Sorted<AS, true>::KW;
@@ -1154,10 +1162,10 @@ __global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, uint32_t n
     __shared__ uint32_t sh[kRjWords];
     const uint64_t tile = blockIdx.x;
     const uint64_t m = A.total();
-    bool dup[4], kn[4];
+    bool dup[IPT], kn[IPT];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint64_t r = rj_item(tile, k);
+    for (int k = 0; k < IPT; ++k) {
+        const uint64_t r = rj_item<IPT>(tile, k);
         dup[k] = false;
         kn[k] = false;
         if (r < m) {
@@ -1203,9 +1211,9 @@ __global__ __launch_bounds__(kBlock) void k_rj_lb(Sorted<AS, true> S, uint32_t n
             lb[r] = lo;
         }
     }
-    rj_put(dup, tile, dout, sh);
+    rj_put<IPT>(dup, tile, dout, sh);
     __syncthreads();  // sh is reused
-    rj_put(kn, tile, knout, sh);
+    rj_put<IPT>(kn, tile, knout, sh);
 }
 
 // An old term's coefficient after the rotation: cos * v when it
@@ -1254,8 +1262,8 @@ __global__ void k_rj_tiles(const uint32_t* __restrict__ lb, BitRank A, uint64_t 
 // With search set (at most kLocalTiles tiles), the first wave finds the
 // tile's new terms itself and stores tr0[tile] (and tr0[tiles]) for the write
 // kernel; otherwise k_rj_tiles has filled tr0.
-template <int AS>
-__global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
+template <int AS, int IPT>
+__global__ __launch_bounds__(kRjTile / IPT) void k_rj_keep(Sorted<AS, true> S, uint32_t n, JobRotTables jt, BitRank A,
                                                     const uint32_t* __restrict__ lb, uint32_t* __restrict__ tr0,
                                                     BitRank D, NewTerms N, Filters fl, BitOut koout, bool search) {
     __shared__ uint32_t sh[kRjWords];
@@ -1277,10 +1285,10 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
         __syncthreads();
     }
     const uint32_t rr[2] = {search ? s_rr[0] : tr0[tile], search ? s_rr[1] : tr0[tile + 1]};
-    bool ko[4];
+    bool ko[IPT];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint64_t i = rj_item(tile, k);
+    for (int k = 0; k < IPT; ++k) {
+        const uint64_t i = rj_item<IPT>(tile, k);
         ko[k] = false;
         if (i < n) {
             const uint32_t job = S.job((uint32_t)i);
@@ -1291,7 +1299,7 @@ __global__ __launch_bounds__(kBlock) void k_rj_keep(Sorted<AS, true> S, uint32_t
             }
         }
     }
-    rj_put(ko, tile, koout, sh);
+    rj_put<IPT>(ko, tile, koout, sh);
 }
 
 // ---- 4b. the kept new (KN) and kept old (KO) terms' tile scans, then the
@@ -1334,8 +1342,8 @@ PLH_DEV void rj_post(const RjPost& rp, uint64_t total) {
 
 __global__ __launch_bounds__(kBlock) void k_rj_counts(RjPost rp) {
     __shared__ uint64_t sh[kWavesPerBlock];
-    rj_block_scan(rp.kn_cnt, rp.kn_pre, rp.tiles, sh);
-    rj_block_scan(rp.ko_cnt, rp.ko_pre, rp.tiles, sh);
+    rj_block_scan<kBlock>(rp.kn_cnt, rp.kn_pre, rp.tiles, sh);
+    rj_block_scan<kBlock>(rp.ko_cnt, rp.ko_pre, rp.tiles, sh);
     __syncthreads();
     rj_post(rp, rp.ko_pre[rp.tiles] + rp.kn_pre[rp.tiles] + rp.extra);
 }
@@ -1349,8 +1357,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_counts(RjPost rp) {
 // take) every workgroup scans the KO and KN tile counts into dynamic LDS
 // itself, and workgroup 0 stores them and posts the count, so the host's
 // wait for it overlaps the writes.
-template <int AS>
-__global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_t n, uint64_t tiles, JobRotTables jt,
+template <int AS, int IPT>
+__global__ __launch_bounds__(kRjTile / IPT) void k_rj_write(Sorted<AS, true> S, uint32_t n, uint64_t tiles, JobRotTables jt,
                                                      BitRank A, const uint32_t* __restrict__ lb,
                                                      const uint32_t* __restrict__ tr0, BitRank D, BitRank KO,
                                                      BitRank KN, NewTerms N, const uint64_t* __restrict__ d_total,
@@ -1364,14 +1372,14 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
         uint64_t* s_ko = dyn;
         uint64_t* s_kn = dyn + (tiles + 1);
         uint64_t* s_red = dyn + 2 * (tiles + 1);
-        rj_block_scan(rp.ko_cnt, s_ko, tiles, s_red);
-        rj_block_scan(rp.kn_cnt, s_kn, tiles, s_red);
+        rj_block_scan<kRjTile / IPT>(rp.ko_cnt, s_ko, tiles, s_red);
+        rj_block_scan<kRjTile / IPT>(rp.kn_cnt, s_kn, tiles, s_red);
         __syncthreads();
         KO.tpre = s_ko;
         KN.tpre = s_kn;
         total = s_ko[tiles] + s_kn[tiles] + rp.extra;
         if (blockIdx.x == 0) {
-            for (uint64_t t = threadIdx.x; t <= tiles; t += kBlock) {
+            for (uint64_t t = threadIdx.x; t <= tiles; t += kRjTile / IPT) {
                 rp.ko_pre[t] = s_ko[t];
                 rp.kn_pre[t] = s_kn[t];
             }
@@ -1385,8 +1393,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
         const uint64_t tile = blockIdx.x;
         const uint32_t rr[2] = {tr0[tile], tr0[tile + 1]};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t i = rj_item(tile, k);
+        for (int k = 0; k < IPT; ++k) {
+            const uint64_t i = rj_item<IPT>(tile, k);
             if (i >= n || !KO.test(i)) continue;
             const uint32_t job = S.job((uint32_t)i);
             uint32_t u;
@@ -1406,8 +1414,8 @@ __global__ __launch_bounds__(kBlock) void k_rj_write(Sorted<AS, true> S, uint32_
     }
     const uint64_t tile = blockIdx.x - tiles;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint64_t r = rj_item(tile, k);
+    for (int k = 0; k < IPT; ++k) {
+        const uint64_t r = rj_item<IPT>(tile, k);
         if (r >= m || !KN.test(r)) continue;
         const uint64_t o = KN.rank(r) + KO.rank(lb[r]) + (toff ? toff[N.j[r]] : 0u);
         if (o >= total) continue;
@@ -1590,112 +1598,118 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
         k_rs_bounds<<<blocks_of((uint64_t)njobs + 1), kBlock, 0, st>>>(place->j, (uint32_t)pn, njobs, toff);
         PLH_HIP(hipGetLastError());
     }
-    {
-        ProfScope ps(ctx, "rot_anti");
-        if (single) {
-            // one sum: its qubit table is built by each workgroup of the test
-            const WinView<AS, true> win{JobWindows{}, *single};
-            k_rj_first<AS, true><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, nullptr, win, fk, jmin, l0, l1, l2, l3, tag);
-            PLH_HIP(hipGetLastError());
-            k_rj_anti<AS, true><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, win, jmin, tag, fk, jt, kstar, adv,
-                                                       fa.out());
-        } else {
-            k_rj_setup<AS><<<blocks_of(std::max<uint64_t>((uint64_t)nj * 64 * AS, nj + 1)), kBlock, 0, st>>>(
-                jw, njobs, fl.job_alive, qtbl, s->j, (uint32_t)n, joff);
-            PLH_HIP(hipGetLastError());
-            const WinView<AS, false> win{jw, {}};
-            k_rj_first<AS, false><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, qtbl, win, fk, jmin, l0, l1, l2, l3, tag);
-            PLH_HIP(hipGetLastError());
-            k_rj_anti<AS, false><<<gt, kBlock, 0, st>>>(s->j, (uint32_t)n, njobs, win, jmin, tag, fk, jt, kstar, adv,
-                                                        fa.out());
-        }
-        PLH_HIP(hipGetLastError());
-    }
-    // small sums: the tile scan inside k_rj_new, the tile ranges inside k_rj_keep
+    // small sums (at most kLocalTiles tiles) take one term per thread in
+    // 1024-thread workgroups, large ones four per thread in 256
     const bool local = tiles <= kLocalTiles;
-    if (!local) PLH_TRY(scan_u32_to_u64(ctx, fa.tcnt, fa.tpre, tiles, fa.tpre + tiles));
-    const BitRank A = fa.rank(tiles);
-    {
-        ProfScope ps(ctx, "rot_new");
-        k_rj_new<AS><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, L, N, hb, local ? fa.tcnt : nullptr);
-        PLH_HIP(hipGetLastError());
-        k_rj_lb<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, A, hb, jt, N, fl, lb, fd.out(false), fkn.out());
-        PLH_HIP(hipGetLastError());
-        if (!local) {
-            k_rj_tiles<<<blocks_of(n + 1), kBlock, 0, st>>>(lb, A, tiles, (uint32_t)n, tr0);
+    auto pass = [&](auto ipt) -> int {
+        constexpr int IPT = decltype(ipt)::value;
+        {
+            ProfScope ps(ctx, "rot_anti");
+            if (single) {
+                // one sum: its qubit table is built by each workgroup of the test
+                const WinView<AS, true> win{JobWindows{}, *single};
+                k_rj_first<AS, true, IPT><<<gt, kRjTile / IPT, 0, st>>>(S, (uint32_t)n, nullptr, win, fk, jmin, l0, l1, l2, l3, tag);
+                PLH_HIP(hipGetLastError());
+                k_rj_anti<AS, true, IPT><<<gt, kRjTile / IPT, 0, st>>>(s->j, (uint32_t)n, njobs, win, jmin, tag, fk, jt, kstar, adv,
+                                                           fa.out());
+            } else {
+                k_rj_setup<AS><<<blocks_of(std::max<uint64_t>((uint64_t)nj * 64 * AS, nj + 1)), kBlock, 0, st>>>(
+                    jw, njobs, fl.job_alive, qtbl, s->j, (uint32_t)n, joff);
+                PLH_HIP(hipGetLastError());
+                const WinView<AS, false> win{jw, {}};
+                k_rj_first<AS, false, IPT><<<gt, kRjTile / IPT, 0, st>>>(S, (uint32_t)n, qtbl, win, fk, jmin, l0, l1, l2, l3, tag);
+                PLH_HIP(hipGetLastError());
+                k_rj_anti<AS, false, IPT><<<gt, kRjTile / IPT, 0, st>>>(s->j, (uint32_t)n, njobs, win, jmin, tag, fk, jt, kstar, adv,
+                                                            fa.out());
+            }
             PLH_HIP(hipGetLastError());
         }
-    }
-    const BitRank D = fd.rank(tiles), KN = fkn.rank(tiles);
-    {
-        ProfScope ps(ctx, "rot_merge");
-        k_rj_keep<AS><<<gt, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, lb, tr0, D, N, fl, fko.out(), local);
-        PLH_HIP(hipGetLastError());
-    }
-    const BitRank KO = fko.rank(tiles);
-    Post post;
-    PLH_TRY(post_prepare(ctx, &post));
-    const RjPost rp{fkn.tcnt, fkn.tpre, fko.tcnt, fko.tpre, tiles, pn, d_total, adv, kstar, njobs, adv_host,
-                    post.mbox, post.seq};
-    plh_sum tmp;
-    tmp.ctx = ctx;
-    tmp.as = s->as;
-    tmp.nq = s->nq;
-    tmp.has_job = s->has_job;
-    uint64_t rb[3] = {0, 0, 0xFFu};
-    auto launch_write = [&](bool lw) -> int {
-        const size_t dyn = lw ? (2 * (tiles + 1) + kWavesPerBlock) * sizeof(uint64_t) : 0;
-        ProfScope ps(ctx, "rot_write", (double)n * (16.0 * s->as + 13.0));
-        k_rj_write<AS><<<2 * gt, kBlock, dyn, st>>>(S, (uint32_t)n, tiles, jt, A, lb, tr0, D, KO, KN, N, d_total,
-                                                    pn ? toff : nullptr, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap,
-                                                    rp, lw);
-        if (pn)
-            k_rj_place<AS><<<blocks_of(pn), kBlock, 0, st>>>(
-                SumView{place->x, place->z, place->f, place->v, place->cap}, place->j, (uint32_t)pn, joff, A, KO, KN,
-                d_total, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) {
-            sum_free_storage(&tmp);
-            PLH_HIP(e);
-        }
-        return PLH_OK;
-    };
-    if (local) {
-        // the output sized for every old term, one new term each and the
-        // placed terms; the write posts the count
-        PLH_TRY(sum_reserve(&tmp, (int64_t)(2 * n + pn), 0));
-        PLH_TRY(launch_write(true));
-        const int rc = post_wait(ctx, post, adv_host ? 1 : 3, rb);
-        if (rc != PLH_OK) {
-            sum_free_storage(&tmp);
-            return rc;
-        }
-        if (!adv_host) {
-            *consumed = (int)rb[1];
-            if (rb[2] == 0xFFu) {  // no rotation of the window changes a term: the sum stays as it is
-                sum_free_storage(&tmp);
-                return PLH_OK;
+        // small sums: the tile scan inside k_rj_new, the tile ranges inside k_rj_keep
+        if (!local) PLH_TRY(scan_u32_to_u64(ctx, fa.tcnt, fa.tpre, tiles, fa.tpre + tiles));
+        const BitRank A = fa.rank(tiles);
+        {
+            ProfScope ps(ctx, "rot_new");
+            k_rj_new<AS><<<g1, kBlock, 0, st>>>(S, (uint32_t)n, jt, A, L, N, hb, local ? fa.tcnt : nullptr);
+            PLH_HIP(hipGetLastError());
+            k_rj_lb<AS, IPT><<<gt, kRjTile / IPT, 0, st>>>(S, (uint32_t)n, A, hb, jt, N, fl, lb, fd.out(false), fkn.out());
+            PLH_HIP(hipGetLastError());
+            if (!local) {
+                k_rj_tiles<<<blocks_of(n + 1), kBlock, 0, st>>>(lb, A, tiles, (uint32_t)n, tr0);
+                PLH_HIP(hipGetLastError());
             }
         }
-    } else {
-        k_rj_counts<<<1, kBlock, 0, st>>>(rp);
-        PLH_HIP(hipGetLastError());
-        PLH_TRY(post_wait(ctx, post, adv_host ? 1 : 3, rb));
-        if (!adv_host) {
-            *consumed = (int)rb[1];
-            if (rb[2] == 0xFFu) return PLH_OK;  // no rotation of the window changes a term: the sum stays as it is
+        const BitRank D = fd.rank(tiles), KN = fkn.rank(tiles);
+        {
+            ProfScope ps(ctx, "rot_merge");
+            k_rj_keep<AS, IPT><<<gt, kRjTile / IPT, 0, st>>>(S, (uint32_t)n, jt, A, lb, tr0, D, N, fl, fko.out(), local);
+            PLH_HIP(hipGetLastError());
         }
-        PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(rb[0], 1), 0));
-        if (rb[0]) PLH_TRY(launch_write(false));
-    }
-    const uint64_t total = rb[0];
-    const int64_t nq = s->nq;
-    sum_take_storage(s, &tmp);
-    s->n = (int64_t)total;
-    s->nq = nq;
-    s->canonical = true;
-    s->sorted = true;
-    return PLH_OK;
+        const BitRank KO = fko.rank(tiles);
+        Post post;
+        PLH_TRY(post_prepare(ctx, &post));
+        const RjPost rp{fkn.tcnt, fkn.tpre, fko.tcnt, fko.tpre, tiles, pn, d_total, adv, kstar, njobs, adv_host,
+                        post.mbox, post.seq};
+        plh_sum tmp;
+        tmp.ctx = ctx;
+        tmp.as = s->as;
+        tmp.nq = s->nq;
+        tmp.has_job = s->has_job;
+        uint64_t rb[3] = {0, 0, 0xFFu};
+        auto launch_write = [&](bool lw) -> int {
+            const size_t dyn = lw ? (2 * (tiles + 1) + kRjTile / IPT / kWave) * sizeof(uint64_t) : 0;
+            ProfScope ps(ctx, "rot_write", (double)n * (16.0 * s->as + 13.0));
+            k_rj_write<AS, IPT><<<2 * gt, kRjTile / IPT, dyn, st>>>(S, (uint32_t)n, tiles, jt, A, lb, tr0, D, KO, KN, N, d_total,
+                                                        pn ? toff : nullptr, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap,
+                                                        rp, lw);
+            if (pn)
+                k_rj_place<AS><<<blocks_of(pn), kBlock, 0, st>>>(
+                    SumView{place->x, place->z, place->f, place->v, place->cap}, place->j, (uint32_t)pn, joff, A, KO, KN,
+                    d_total, tmp.x, tmp.z, tmp.f, tmp.v, tmp.j, tmp.cap);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) {
+                sum_free_storage(&tmp);
+                PLH_HIP(e);
+            }
+            return PLH_OK;
+        };
+        if (local) {
+            // the output sized for every old term, one new term each and the
+            // placed terms; the write posts the count
+            PLH_TRY(sum_reserve(&tmp, (int64_t)(2 * n + pn), 0));
+            PLH_TRY(launch_write(true));
+            const int rc = post_wait(ctx, post, adv_host ? 1 : 3, rb);
+            if (rc != PLH_OK) {
+                sum_free_storage(&tmp);
+                return rc;
+            }
+            if (!adv_host) {
+                *consumed = (int)rb[1];
+                if (rb[2] == 0xFFu) {  // no rotation of the window changes a term: the sum stays as it is
+                    sum_free_storage(&tmp);
+                    return PLH_OK;
+                }
+            }
+        } else {
+            k_rj_counts<<<1, kBlock, 0, st>>>(rp);
+            PLH_HIP(hipGetLastError());
+            PLH_TRY(post_wait(ctx, post, adv_host ? 1 : 3, rb));
+            if (!adv_host) {
+                *consumed = (int)rb[1];
+                if (rb[2] == 0xFFu) return PLH_OK;  // no rotation of the window changes a term: the sum stays as it is
+            }
+            PLH_TRY(sum_reserve(&tmp, (int64_t)std::max<uint64_t>(rb[0], 1), 0));
+            if (rb[0]) PLH_TRY(launch_write(false));
+        }
+        const uint64_t total = rb[0];
+        const int64_t nq = s->nq;
+        sum_take_storage(s, &tmp);
+        s->n = (int64_t)total;
+        s->nq = nq;
+        s->canonical = true;
+        s->sorted = true;
+        return PLH_OK;
+    };
+    return local ? pass(std::integral_constant<int, 1>{}) : pass(std::integral_constant<int, 4>{});
 }
 
 template <int AS, bool JOB>
