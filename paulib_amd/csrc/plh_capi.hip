@@ -1028,6 +1028,38 @@ int plh_run_program(plh_sum* s, int64_t nops, const int32_t* kind, const int64_t
             }
         } else {
             clean = false;
+            // a run of consecutive gate ops (in the order they act) in one pass
+            if (s->as <= 16) {
+                CliffordRun run;
+                run.nops = 0;
+                bool any_table = false;
+                int64_t t = l;
+                const int64_t nqb = 64 * (int64_t)s->as;
+                for (; t >= 0 && run.nops < kCliffordRunMax && kind[t] != 3; --t) {
+                    const int32_t kd = kind[t];
+                    const bool table = kd >= kProgramGateBase && kd < kProgramGateBase + kNumGates;
+                    if (!(kd == 0 || kd == 1 || kd == 2 || table)) break;  // reported by the single-op path below
+                    const int gate = table ? kd - kProgramGateBase : -1;
+                    const bool two = kd == 2 || (table && gate >= kFirstTwoQubitGate);
+                    if (q0[t] < 0 || q0[t] >= nqb || (two && (q1[t] < 0 || q1[t] >= nqb)) || (two && q0[t] == q1[t]))
+                        break;  // reported by the single-op path below
+                    if (table && gate == 0) continue;  // apply_I
+                    uint32_t k4 = kd <= 2 ? (uint32_t)kd : (two ? 4u : 3u);
+                    const int o = run.nops++;
+                    run.op[o] = (k4 << 28) | ((uint32_t)q0[t] << 14) | (two ? (uint32_t)q1[t] : 0u);
+                    run.tbl[o] = 0;
+                    run.sgn[o] = 0;
+                    if (table) {
+                        PLH_TRY(clifford_table_of(gate, &run.tbl[o], &run.sgn[o]));
+                        any_table = true;
+                    }
+                }
+                if (t < l) {  // ops l .. t + 1 taken (apply_I ones only skipped)
+                    if (run.nops > 0) PLH_TRY(op_clifford_run(s->ctx, s, run, any_table));
+                    l = t + 1;
+                    continue;
+                }
+            }
         }
         switch (kind[l]) {
             case 0: PLH_TRY(op_clifford(s->ctx, s, 0, q0[l], 0)); break;

@@ -363,6 +363,19 @@ constexpr int kFirstTwoQubitGate = 10;
 constexpr int kProgramGateBase = 16;
 int op_clifford_table(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1);
 int clifford_table_of(int gate, uint64_t* tbl, uint32_t* sgn);
+// A run of up to kCliffordRunMax Clifford ops in one pass over the terms
+// (each thread applies the whole run to its term in registers).  op[k]:
+// kind << 28 | q0 << 14 | q1, kind 0 H, 1 S, 2 CNOT(q0 -> q1), 3 a one-qubit
+// table gate, 4 a two-qubit table gate (tbl[k], sgn[k] as clifford_table_of
+// gives them); qubits below 16384.
+constexpr int kCliffordRunMax = 64;
+struct CliffordRun {
+    int nops;
+    uint32_t op[kCliffordRunMax];
+    uint64_t tbl[kCliffordRunMax];
+    uint32_t sgn[kCliffordRunMax];
+};
+int op_clifford_run(plh_ctx* ctx, plh_sum* s, const CliffordRun& run, bool any_table);
 int op_elementwise(plh_ctx* ctx, plh_sum* s, const plh_sum* rhs);
 int op_rotation(plh_ctx* ctx, plh_sum* s, const uint64_t* px, const uint64_t* pz, uint8_t pflags,
                 double cos_t, double sin_t, int* appended_general);

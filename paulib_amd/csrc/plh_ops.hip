@@ -171,6 +171,91 @@ __global__ void k_clifford(int gate, uint64_t* __restrict__ x, uint64_t* __restr
     }
 }
 
+// A run of Clifford ops in one pass: each thread holds its term (AS words
+// per plane and the flags) in registers and applies the ops in order, H, S
+// and CNOT as k_clifford, table gates as k_clifford_table.  Words are picked
+// by compare so the term stays in registers.
+template <int AS>
+__global__ void k_clifford_run(uint64_t* __restrict__ x, uint64_t* __restrict__ z, uint8_t* __restrict__ f,
+                               int64_t stride, int64_t n, int as, CliffordRun run) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t xw[AS], zw[AS];
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        xw[k] = k < as ? x[(int64_t)k * stride + i] : 0ull;
+        zw[k] = k < as ? z[(int64_t)k * stride + i] : 0ull;
+    }
+    uint32_t fl = f[i];
+    for (int o = 0; o < run.nops; ++o) {
+        const uint32_t op = run.op[o];
+        const uint32_t kind = op >> 28, q0 = (op >> 14) & 0x3FFFu, q1 = op & 0x3FFFu;
+        const int wc = (int)(q0 >> 6), wt = (int)(q1 >> 6);
+        const uint64_t mc = 1ull << (q0 & 63), mt = 1ull << (q1 & 63);
+        uint64_t xcw = 0, zcw = 0, xtw = 0, ztw = 0;
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            if (k == wc) {
+                xcw = xw[k];
+                zcw = zw[k];
+            }
+            if (k == wt) {
+                xtw = xw[k];
+                ztw = zw[k];
+            }
+        }
+        const uint64_t xc = (xcw & mc) != 0, zc = (zcw & mc) != 0;
+        if (kind == 0) {  // H: X <-> Z, Y -> -Y
+            fl ^= (uint32_t)((xc & zc) << 1);
+            const uint64_t d = (xcw ^ zcw) & mc;
+#pragma unroll
+            for (int k = 0; k < AS; ++k)
+                if (k == wc) {
+                    xw[k] ^= d;
+                    zw[k] ^= d;
+                }
+        } else if (kind == 1) {  // S
+            fl ^= (uint32_t)((xc & zc) << 1);
+#pragma unroll
+            for (int k = 0; k < AS; ++k)
+                if (k == wc) zw[k] = zcw ^ (xcw & mc);
+        } else if (kind == 2) {  // CNOT(q0 -> q1)
+            const uint64_t xt = (xtw & mt) != 0, zt = (ztw & mt) != 0;
+            fl ^= (uint32_t)((xc & zt & (1ull ^ xt ^ zc)) << 1);
+#pragma unroll
+            for (int k = 0; k < AS; ++k) {
+                if (k == wt) xw[k] ^= (0ull - xc) & mt;
+                if (k == wc) zw[k] ^= (0ull - zt) & mc;
+            }
+        } else {  // a table gate on q0 (and q1 when kind 4)
+            const bool two = kind == 4;
+            uint32_t b = (uint32_t)xc | ((uint32_t)zc << 1);
+            if (two) b |= ((uint32_t)((xtw & mt) != 0) << 2) | ((uint32_t)((ztw & mt) != 0) << 3);
+            fl ^= ((run.sgn[o] >> b) & 1u) << 1;
+            const uint32_t e = (uint32_t)(run.tbl[o] >> (4 * b)) & 15u;
+#pragma unroll
+            for (int k = 0; k < AS; ++k) {
+                if (k == wc) {
+                    xw[k] = (xw[k] & ~mc) | ((e & 1u) ? mc : 0ull);
+                    zw[k] = (zw[k] & ~mc) | ((e & 2u) ? mc : 0ull);
+                }
+                if (two && k == wt) {
+                    xw[k] = (xw[k] & ~mt) | ((e & 4u) ? mt : 0ull);
+                    zw[k] = (zw[k] & ~mt) | ((e & 8u) ? mt : 0ull);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        if (k < as) {
+            x[(int64_t)k * stride + i] = xw[k];
+            z[(int64_t)k * stride + i] = zw[k];
+        }
+    }
+    f[i] = (uint8_t)fl;
+}
+
 // ------------------------------------------------ table-driven Clifford
 // Every gate of include/PauliSumClifford.h acts on a term only through the
 // bits of its one or two qubits, b = xc | zc<<1 | xt<<2 | zt<<3.  A 16-entry
@@ -675,6 +760,26 @@ int op_clifford_table(plh_ctx* ctx, plh_sum* s, int gate, int64_t q0, int64_t q1
     PLH_HIP(hipGetLastError());
     s->canonical = false;
     s->sorted = false;
+    return PLH_OK;
+}
+
+int op_clifford_run(plh_ctx* ctx, plh_sum* s, const CliffordRun& run, bool any_table) {
+    if (run.nops < 1 || run.nops > kCliffordRunMax) return PLH_ERR_ARG;
+    if (s->n > 0) {
+        const unsigned g = nblk((uint64_t)s->n, kBlock);
+        switch (s->as <= 1 ? 1 : s->as == 2 ? 2 : s->as <= 4 ? 4 : s->as <= 8 ? 8 : 16) {
+            case 1: k_clifford_run<1><<<g, kBlock, 0, ctx->stream>>>(s->x, s->z, s->f, s->cap, s->n, s->as, run); break;
+            case 2: k_clifford_run<2><<<g, kBlock, 0, ctx->stream>>>(s->x, s->z, s->f, s->cap, s->n, s->as, run); break;
+            case 4: k_clifford_run<4><<<g, kBlock, 0, ctx->stream>>>(s->x, s->z, s->f, s->cap, s->n, s->as, run); break;
+            case 8: k_clifford_run<8><<<g, kBlock, 0, ctx->stream>>>(s->x, s->z, s->f, s->cap, s->n, s->as, run); break;
+            default:
+                k_clifford_run<16><<<g, kBlock, 0, ctx->stream>>>(s->x, s->z, s->f, s->cap, s->n, s->as, run);
+                break;
+        }
+        PLH_HIP(hipGetLastError());
+    }
+    if (any_table) s->canonical = false;
+    s->sorted = false;  // keys stay distinct but their order does not
     return PLH_OK;
 }
 

@@ -127,10 +127,12 @@ def test_elementwise_length_mismatch(ctx):
         d.elementwise_multiply(dev(ctx, W.random_words_sum(9, 64, 2)))
 
 
-@pytest.mark.parametrize("nq,depth,seed", [(12, 5, 1), (64, 3, 2), (100, 2, 3)])
+@pytest.mark.parametrize("nq,depth,seed", [(12, 5, 1), (64, 3, 2), (100, 2, 3), (300, 2, 4)])
 def test_run_program_with_gate_set(ctx, nq, depth, seed):
     """run_program with the whole gate set (kinds 16 + id) and truncated
-    rotations against the oracle's run_program."""
+    rotations against the oracle's run_program.  A layer's gates go through
+    the fused Clifford-run kernel in runs of at most 64 ops; 300 qubits take
+    several runs per layer at five words per plane."""
     from paulib_amd import TruncationOptions, run_program
 
     prog = W.mixed_gate_program(nq, depth, seed)
