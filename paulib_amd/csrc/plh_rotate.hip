@@ -1340,10 +1340,11 @@ PLH_DEV void rj_post(const RjPost& rp, uint64_t total) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_rj_counts(RjPost rp) {
-    __shared__ uint64_t sh[kWavesPerBlock];
-    rj_block_scan<kBlock>(rp.kn_cnt, rp.kn_pre, rp.tiles, sh);
-    rj_block_scan<kBlock>(rp.ko_cnt, rp.ko_pre, rp.tiles, sh);
+// One 1024-thread workgroup (the scans cover up to 2^21 tiles).
+__global__ __launch_bounds__(kRjTile) void k_rj_counts(RjPost rp) {
+    __shared__ uint64_t sh[kRjTile / kWave];
+    rj_block_scan<kRjTile>(rp.kn_cnt, rp.kn_pre, rp.tiles, sh);
+    rj_block_scan<kRjTile>(rp.ko_cnt, rp.ko_pre, rp.tiles, sh);
     __syncthreads();
     rj_post(rp, rp.ko_pre[rp.tiles] + rp.kn_pre[rp.tiles] + rp.extra);
 }
@@ -1690,7 +1691,7 @@ int rotate_sorted_jobs_as(plh_ctx* ctx, plh_sum* s, const JobWindows& jw, const 
                 }
             }
         } else {
-            k_rj_counts<<<1, kBlock, 0, st>>>(rp);
+            k_rj_counts<<<1, kRjTile, 0, st>>>(rp);
             PLH_HIP(hipGetLastError());
             PLH_TRY(post_wait(ctx, post, adv_host ? 1 : 3, rb));
             if (!adv_host) {

@@ -541,19 +541,30 @@ __global__ __launch_bounds__(kBlock) void k_c_sizes(const uint32_t* __restrict__
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_c_gather(SumView b, const uint32_t* __restrict__ J, int as, uint32_t n,
+// One thread per gathered term o: its job is the slot t with cpre[t] <= o <
+// cpre[t + 1] (a search of the slots' prefix; a job that is not gathered
+// has an empty range), its source joff[t] + (o - cpre[t]).
+__global__ __launch_bounds__(kBlock) void k_c_gather(SumView b, int as, uint32_t slots,
                                                      const uint32_t* __restrict__ joff,
-                                                     const uint8_t* __restrict__ isc,
-                                                     const uint64_t* __restrict__ cpre, uint64_t* __restrict__ ox,
-                                                     uint64_t* __restrict__ oz, uint8_t* __restrict__ of,
-                                                     double* __restrict__ ov, uint32_t* __restrict__ oj, int64_t os,
-                                                     uint64_t cap) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t job = J[i];
-    if (!isc[job]) return;
-    const uint64_t o = cpre[job] + (i - joff[job]);
-    if (o >= cap) return;  // never: cpre counts these terms; keeps writes in bounds
+                                                     const uint64_t* __restrict__ cpre, uint64_t total,
+                                                     uint64_t* __restrict__ ox, uint64_t* __restrict__ oz,
+                                                     uint8_t* __restrict__ of, double* __restrict__ ov,
+                                                     uint32_t* __restrict__ oj, int64_t os) {
+    const uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (o >= total) return;
+    uint32_t lo = 0, cnt = slots;  // the last t with cpre[t] <= o
+    while (cnt > 0) {
+        const uint32_t half = cnt / 2, t = lo + half;
+        if (cpre[t + 1] <= o) {
+            lo = t + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    const uint32_t job = lo;
+    if (job >= slots) return;  // never: o < cpre[slots]
+    const uint64_t i = (uint64_t)joff[job] + (o - cpre[job]);
     for (int w = 0; w < as; ++w) {
         ox[(int64_t)w * os + o] = b.x[(int64_t)w * b.stride + i];
         oz[(int64_t)w * os + o] = b.z[(int64_t)w * b.stride + i];
@@ -583,9 +594,9 @@ int gather_jobs(plh_ctx* ctx, const plh_sum* B, const uint8_t* isc, uint32_t slo
     T->n = 0;
     PLH_TRY(sum_reserve(T, std::max<int64_t>((int64_t)total + extra, 1), 0));
     if (total) {
-        k_c_gather<<<blocks_for(n, kBlock), kBlock, 0, st>>>(SumView{B->x, B->z, B->f, B->v, B->cap}, B->j, B->as,
-                                                              (uint32_t)n, joff, isc, cpre, T->x, T->z, T->f, T->v,
-                                                              T->j, T->cap, (uint64_t)T->cap);
+        k_c_gather<<<blocks_for(total, kBlock), kBlock, 0, st>>>(SumView{B->x, B->z, B->f, B->v, B->cap}, B->as, slots,
+                                                                  joff, cpre, total, T->x, T->z, T->f, T->v, T->j,
+                                                                  T->cap);
         PLH_HIP(hipGetLastError());
     }
     T->n = (int64_t)total;
