@@ -46,7 +46,9 @@ constexpr int kIpt = 16;
 // Per-step job tables (device, one upload per step), one entry per slot.
 struct StepTables {
     const uint32_t* cl_off;   // [slots + 1] range of the slot's Clifford ops
-    const uint32_t* cl_op;    // kind << 28 | q0 << 14 | q1 (kind 0 H, 1 S, 2 CNOT); kind 3: apply_table_op
+    const uint32_t* cl_op;    // kind << 28 | q0 << 14 | q1 (kind 0 H, 1 S, 2 CNOT); kind 3: apply_table_op;
+                              // kind 5: gate layer (low 28 bits: its index in layers)
+    const uint64_t* layers;   // kLayerWords words per gate layer (GateLayer)
     const uint8_t* rot_kind;  // 0 none, 1 sign flip, 2 flip + multiply, 3 multiply, 4 general
     const uint8_t* rot_psf;   // flags of ps = P with the i bit flipped
     const double* rot_c;
@@ -94,12 +96,71 @@ PLH_DEV void apply_table_op(uint32_t op, uint64_t* x, uint64_t* z, uint32_t& f) 
     }
 }
 
+// A gate layer: H, S and CNOT ops on pairwise distinct qubits, as masks
+// (qubits below 128): H on h, S on s, CNOT(c -> c + d) for each control
+// bit c of cm, one shift d (|d| < 64) per layer.  Its ops commute (each
+// reads and writes only its own qubits, and sign flips add), so the layer
+// is all of them at once, from the term's bits before the layer.
+constexpr int kLayerWords = 7;  // h[2], s[2], cm[2], d
+
+// v's bits moved d (1..63) qubits up (out bit q + d = v bit q) or down.
+template <int AS>
+PLH_DEV void bits_up(const uint64_t* v, int d, uint64_t* out) {
+#pragma unroll
+    for (int k = AS - 1; k >= 0; --k) out[k] = (v[k] << d) | (k > 0 ? v[k - 1] >> (64 - d) : 0ull);
+}
+template <int AS>
+PLH_DEV void bits_down(const uint64_t* v, int d, uint64_t* out) {
+#pragma unroll
+    for (int k = 0; k < AS; ++k) out[k] = (v[k] >> d) | (k + 1 < AS ? v[k + 1] << (64 - d) : 0ull);
+}
+
+template <int AS>
+PLH_DEV void apply_gate_layer(const uint64_t* L, uint64_t* x, uint64_t* z, uint32_t& f) {
+    const int64_t d = (int64_t)L[6];
+    const int ad = (int)(d < 0 ? -d : d);
+    uint64_t xt[AS], zt[AS], xu[AS], xc[AS];
+    // the target qubits' bits at their controls' positions
+    if (d > 0) {
+        bits_down<AS>(x, ad, xt);
+        bits_down<AS>(z, ad, zt);
+    } else if (d < 0) {
+        bits_up<AS>(x, ad, xt);
+        bits_up<AS>(z, ad, zt);
+    }
+    uint32_t par = 0;
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        const uint64_t h = L[k], sm = L[2 + k], cm = d != 0 ? L[4 + k] : 0ull;
+        par += (uint32_t)popc(x[k] & z[k] & (h | sm));                         // H and S: Y gets -1
+        xc[k] = x[k] & cm;
+        if (d != 0) par += (uint32_t)popc(xc[k] & zt[k] & ~(xt[k] ^ z[k]) & cm);  // CNOT: xc zt (1 ^ xt ^ zc)
+    }
+    // the controls' x bits at their targets
+    if (d > 0) bits_up<AS>(xc, ad, xu);
+    else if (d < 0) bits_down<AS>(xc, ad, xu);
+#pragma unroll
+    for (int k = 0; k < AS; ++k) {
+        const uint64_t h = L[k], sm = L[2 + k], cm = d != 0 ? L[4 + k] : 0ull;
+        const uint64_t dh = (x[k] ^ z[k]) & h;
+        const uint64_t nx = x[k] ^ dh ^ (d != 0 ? xu[k] : 0ull);
+        const uint64_t nz = z[k] ^ dh ^ (x[k] & sm) ^ (zt[k] & cm);
+        x[k] = nx;
+        z[k] = nz;
+    }
+    f ^= (par & 1u) << 1;
+}
+
 // One Clifford conjugation of a term held in registers: the update of the
 // plain k_clifford kernel (H, S, CNOT of include/PauliSumClifford.h), or a
 // table gate.  Words are selected by compare so the term stays in registers.
 template <int AS>
-PLH_DEV void apply_clifford_op(uint32_t op, uint64_t* x, uint64_t* z, uint32_t& f) {
+PLH_DEV void apply_clifford_op(uint32_t op, uint64_t* x, uint64_t* z, uint32_t& f, const uint64_t* layers) {
     const uint32_t kind = op >> 28, q0 = (op >> 14) & 0x3FFFu, q1 = op & 0x3FFFu;
+    if (kind == 5) {
+        apply_gate_layer<AS>(layers + (size_t)(op & 0x0FFFFFFFu) * kLayerWords, x, z, f);
+        return;
+    }
     if (kind == 3) {
         apply_table_op<AS>(op, x, z, f);
         return;
@@ -181,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_gates(uint64_t* __restrict__ X
             z[w] = Z[(int64_t)w * stride + t];
         }
         uint32_t f = F[t];
-        for (uint32_t o = o0; o < o1; ++o) apply_clifford_op<AS>(tb.cl_op[o], x, z, f);
+        for (uint32_t o = o0; o < o1; ++o) apply_clifford_op<AS>(tb.cl_op[o], x, z, f, tb.layers);
         if (kind >= 1 && kind <= 3) {
             const uint64_t* p = tb.rot_p + (size_t)job * 2 * AS;
             if (anticommutes_p<AS>(x, z, p)) {
@@ -652,6 +713,66 @@ int admit(plh_server* s, const Job& j, int slot) {
 // job, and the finished jobs' expectations (and terms); their results go
 // to *res.  Nothing of the scheduler changes here; a sorted step settles the
 // plan (the jobs' next ops) from the rotations the device consumed.
+// The step's Clifford ops per slot, with each maximal run of consecutive H,
+// S and CNOT ops on pairwise distinct qubits (CNOTs of one shift q1 - q0)
+// made into one gate layer; table gates, and layers of one op, stay single
+// ops.  off2/ops2 replace the plan's off/ops for the device.
+void compile_layers(const sched::StepPlan& pl, std::vector<uint32_t>* off2, std::vector<uint32_t>* ops2,
+                    std::vector<uint64_t>* layers) {
+    const size_t S = pl.off.empty() ? 0 : pl.off.size() - 1;
+    off2->assign(S + 1, 0);
+    ops2->clear();
+    layers->clear();
+    for (size_t i = 0; i < S; ++i) {
+        (*off2)[i] = (uint32_t)ops2->size();
+        uint64_t L[kLayerWords] = {};
+        uint64_t used[2] = {0, 0};
+        int nin = 0;
+        uint32_t first = 0;
+        auto flush = [&]() {
+            if (nin == 1) {
+                ops2->push_back(first);
+            } else if (nin > 1) {
+                ops2->push_back((5u << 28) | (uint32_t)(layers->size() / kLayerWords));
+                layers->insert(layers->end(), L, L + kLayerWords);
+            }
+            for (int k = 0; k < kLayerWords; ++k) L[k] = 0;
+            used[0] = used[1] = 0;
+            nin = 0;
+        };
+        for (uint32_t o = pl.off[i]; o < pl.off[i + 1]; ++o) {
+            const uint32_t op = pl.ops[o];
+            const uint32_t kind = op >> 28, q0 = (op >> 14) & 0x3FFFu, q1 = op & 0x3FFFu;
+            const bool two = kind == 2;
+            if (kind > 2 || q0 >= 128 || (two && (q1 >= 128 || q1 == q0))) {
+                flush();
+                ops2->push_back(op);
+                continue;
+            }
+            const int64_t d = two ? (int64_t)q1 - (int64_t)q0 : 0;
+            const bool free0 = !((used[q0 >> 6] >> (q0 & 63)) & 1ull);
+            const bool free1 = !two || !((used[q1 >> 6] >> (q1 & 63)) & 1ull);
+            const bool shift_ok = !two || d == (int64_t)L[6] || (int64_t)L[6] == 0;
+            if (!(free0 && free1 && shift_ok && (!two || (d > -64 && d < 64)))) flush();
+            if (two && (d <= -64 || d >= 64)) {  // not a layer CNOT
+                ops2->push_back(op);
+                continue;
+            }
+            used[q0 >> 6] |= 1ull << (q0 & 63);
+            if (kind == 0) L[q0 >> 6] |= 1ull << (q0 & 63);
+            if (kind == 1) L[2 + (q0 >> 6)] |= 1ull << (q0 & 63);
+            if (two) {
+                used[q1 >> 6] |= 1ull << (q1 & 63);
+                L[4 + (q0 >> 6)] |= 1ull << (q0 & 63);
+                L[6] = (uint64_t)d;
+            }
+            if (nin++ == 0) first = op;
+        }
+        flush();
+    }
+    (*off2)[S] = (uint32_t)ops2->size();
+}
+
 int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::vector<Result>* res) {
     plh_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
@@ -661,10 +782,13 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
         const void* src;
         size_t bytes;
     };
-    constexpr int kParts = 19;
+    constexpr int kParts = 20;
     std::vector<uint8_t> alive_r(pl.alive.size(), 0);
     for (size_t i = 0; i < alive_r.size(); ++i) alive_r[i] = pl.alive[i] && !(i < pl.isc.size() && pl.isc[i]);
-    const Part parts[kParts] = {{pl.off.data(), pl.off.size() * 4}, {pl.ops.data(), pl.ops.size() * 4},
+    std::vector<uint32_t> off2, ops2;
+    std::vector<uint64_t> layers;
+    compile_layers(pl, &off2, &ops2, &layers);
+    const Part parts[kParts] = {{off2.data(), off2.size() * 4}, {ops2.data(), ops2.size() * 4},
                                 {pl.rk.data(), pl.rk.size()},       {pl.rpsf.data(), pl.rpsf.size()},
                                 {pl.rc.data(), pl.rc.size() * 8},   {pl.rs.data(), pl.rs.size() * 8},
                                 {pl.rp.data(), pl.rp.size() * 8},   {pl.eps.data(), pl.eps.size() * 8},
@@ -673,7 +797,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
                                 {pl.wp.data(), pl.wp.size() * 8},   {pl.wpsf.data(), pl.wpsf.size()},
                                 {pl.wc.data(), pl.wc.size() * 8},   {pl.ws.data(), pl.ws.size() * 8},
                                 {pl.maxt.data(), pl.maxt.size() * 8}, {pl.isc.data(), pl.isc.size()},
-                                {alive_r.data(), alive_r.size()}};
+                                {alive_r.data(), alive_r.size()},   {layers.data(), layers.size() * 8}};
     static_assert(sched::kWindow == kJobWindow, "scheduler and engine agree on the window depth");
     size_t total = 0;
     for (const Part& p : parts) total += align256(std::max<size_t>(p.bytes, 1));
@@ -706,6 +830,7 @@ int step_device(plh_server* s, sched::StepPlan& pl, sched::StepKind kind, std::v
     tb.alive = (uint8_t*)dptr[9];
     tb.isc = (const uint8_t*)dptr[17];
     tb.alive_r = (const uint8_t*)dptr[18];
+    tb.layers = (const uint64_t*)dptr[19];
     PLH_HIP(hipMemcpyAsync(s->d_step, s->h_step.data(), total, hipMemcpyHostToDevice, st));
 
     CollateSpec sc;

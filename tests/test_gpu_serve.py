@@ -222,3 +222,51 @@ def test_server_gate_set_matches_run_program(ctx):
         assert_terms_equal(got, d.to_host())
         e_want, want = _oracle(O, prog, opts)
         assert abs(r.expectation - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum())
+
+
+def _layered_program(nq, depth, seed):
+    """Clifford layers the server turns into gate layers: H and S on random
+    qubits, CNOTs in both orientations between neighbours (across the word
+    boundary at 63/64 too) and between far qubits (more than 63 apart: single
+    ops), then a rotation on every few qubits."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = W.ProgramBuilder(nq)
+    for layer in range(depth):
+        for q in range(nq):
+            r = rng.random()
+            if r < 0.3:
+                b.clifford(0, q)
+            elif r < 0.55:
+                b.clifford(1, q)
+        for q in range(layer % 2, nq - 1, 2):
+            if rng.random() < 0.7:
+                c, t = (q, q + 1) if rng.random() < 0.5 else (q + 1, q)
+                b.clifford(2, c, t)
+        if nq > 70:
+            b.clifford(2, 3, nq - 2)
+            b.clifford(2, nq - 1, 1)
+        for q in range(nq):
+            b.rotation({q: "XYZ"[int(rng.integers(0, 3))]}, float(rng.random() * 2 * np.pi))
+        b.rotation({62: "Z", 65: "X"} if nq > 65 else {0: "Z", 1: "Z"}, float(rng.random() * 2 * np.pi))
+    return b.build()
+
+
+def test_server_clifford_layers_match_oracle(ctx):
+    """Jobs whose Clifford steps become gate layers (H and S masks, CNOTs of
+    one shift in either direction, across the 64-qubit word boundary) and
+    single ops (far CNOTs): terms bit for bit against the oracle, and each
+    job alone equals it in the mix."""
+    jobs = []
+    for k, nq in enumerate((16, 64, 100, 128)):
+        O = W.observable(nq, [{0: "Z", nq // 2: "X"}, {nq - 1: "Y"}], [0.7, -0.4])
+        jobs.append((O, _layered_program(nq, 5, 40 + k), TruncationOptions(max_weight=8, coefficient_cutoff=1e-5)))
+    mixed = _serve(ctx, jobs, max_jobs=8)
+    for k, job in enumerate(jobs):
+        e_want, want = _oracle(*job)
+        rm, tm = mixed[k]
+        assert rm.terms == len(want), f"job {k}"
+        assert_terms_equal(tm, want)
+        assert abs(rm.expectation - e_want) <= 1e-12 * max(1.0, np.abs(want.value).sum()), f"job {k}"
+        (ra, ta), = _serve(ctx, [job], max_jobs=1)
+        assert np.float64(ra.expectation).view(np.uint64) == np.float64(rm.expectation).view(np.uint64), f"job {k}"
+        assert_terms_equal(ta, tm)
