@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "plh_block.hpp"
@@ -372,6 +373,182 @@ __global__ __launch_bounds__(kBlock) void k_root_scatter(Src src, IdKey* __restr
         }
         const uint32_t total = msplit_round(sh, dig, rank);
         msplit_write(sh, total, dig, rank, idv, base, ids_out);
+    }
+}
+
+// ---------------------------------------------- root pass of a product by groups
+// Each x/z word of a product key is the XOR of its operands' words, so when
+// the root digit lies in the x/z words, the digit of pair (rhs i, lhs j) is
+// dl(j) ^ dr(i), and so are the 32 carried bits when they lie there too.
+// Root bucket d then holds, for each rhs row i in order, the lhs terms of
+// digit d ^ dr(i) in index order: the ids in ascending order, exactly what
+// the scatter writes.  The lhs are grouped by digit once (stably); a block of
+// rhs rows counts its pairs per bucket from the group sizes alone, and its
+// ids are written group by group, with no key gather and no ranking.
+
+// The 40-bit window [pos, pos + 40) of one operand record's x/z words (words
+// from 2*AS on read 0; callers use it only where the window lies in them).
+template <int AS>
+PLH_DEV uint64_t rec_window(const uint64_t* rec, int pos) {
+    constexpr int W = kDigitBits + kCarryBits;
+    const int w = pos >> 6, o = pos & 63;
+    uint64_t hi = (w < 2 * AS ? rec[w] : 0ull) << o;
+    if (o + W > 64) hi |= (w + 1 < 2 * AS ? rec[w + 1] : 0ull) >> (64 - o);
+    return hi >> (64 - W);
+}
+
+template <class Src>
+struct GroupRoot : std::false_type {};
+template <int AS>
+struct GroupRoot<PairSource<AS, 0>> : std::true_type {};
+
+constexpr int kGrpThreads = 1024;  // the grouping workgroup
+constexpr uint32_t kGrpRows = 32;  // rhs rows per block of the generator
+
+// Stable grouping of the n lhs terms by root digit, in one workgroup:
+// gcnt[0..255] the group sizes, gcnt[256..511] their starts; position p of
+// the grouped order holds lrec[p] = carried bits << 32 | j and lgrp[p] = the
+// digit.  A chunk of 1024 terms is ranked by 8 ballots per wave and per-wave
+// digit counts.
+template <int AS>
+__global__ __launch_bounds__(kGrpThreads) void k_group_lhs(PackedView<AS> L, uint32_t n,
+                                                          const uint64_t* __restrict__ live, int kb,
+                                                          uint64_t* __restrict__ lrec, uint8_t* __restrict__ lgrp,
+                                                          uint32_t* __restrict__ gcnt) {
+    constexpr int kW = kGrpThreads / kWave;
+    __shared__ uint32_t h[kDigits];
+    __shared__ uint32_t run[kDigits];
+    __shared__ uint32_t wc[kW][kDigits];
+    const int pos = root_pos(live, kb);
+    if (threadIdx.x < (unsigned)kDigits) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < n; j += kGrpThreads)
+        atomicAdd(&h[(uint32_t)(rec_window<AS>(L.rec(j), pos) >> kCarryBits)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0;
+        for (int d = 0; d < kDigits; ++d) {
+            run[d] = r;
+            r += h[d];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)kDigits) {
+        gcnt[threadIdx.x] = h[threadIdx.x];
+        gcnt[kDigits + threadIdx.x] = run[threadIdx.x];
+    }
+    const int w = (int)threadIdx.x / kWave;
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t c = 0; c < n; c += kGrpThreads) {
+        for (int i = threadIdx.x; i < kW * kDigits; i += kGrpThreads) (&wc[0][0])[i] = 0;
+        __syncthreads();
+        const uint32_t j = c + threadIdx.x;
+        const bool valid = j < n;
+        const uint64_t win = valid ? rec_window<AS>(L.rec(j), pos) : 0ull;
+        const uint32_t d = (uint32_t)(win >> kCarryBits);
+        uint64_t m = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kDigitBits; ++b) {
+            const bool bit = valid && ((d >> b) & 1u);
+            const uint64_t bb = ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t lower = (uint32_t)popc(m & lt);
+        if (valid && lower == 0) wc[w][d] = (uint32_t)popc(m);
+        __syncthreads();
+        if (threadIdx.x < (unsigned)kDigits) {
+            const int dd = (int)threadIdx.x;
+            uint32_t r = run[dd];
+#pragma unroll
+            for (int ww = 0; ww < kW; ++ww) {
+                const uint32_t t = wc[ww][dd];
+                wc[ww][dd] = r;
+                r += t;
+            }
+            run[dd] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t p = wc[w][d] + lower;
+            lrec[p] = (uint64_t)(uint32_t)win << 32 | j;
+            lgrp[p] = (uint8_t)d;
+        }
+        __syncthreads();  // before the next chunk clears wc
+    }
+}
+
+// Pairs per root bucket of each block of kGrpRows rhs rows: hist[b][d] =
+// sum over the block's rows i of the size of lhs group d ^ dr(i).
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_group_hist(PackedView<AS> R, uint32_t m, const uint64_t* __restrict__ live,
+                                                       int kb, const uint32_t* __restrict__ gcnt,
+                                                       uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kDigits];
+    __shared__ uint32_t dr[kGrpRows];
+    const int pos = root_pos(live, kb);
+    const uint32_t i0 = blockIdx.x * kGrpRows;
+    const uint32_t nr = min(kGrpRows, m - i0);
+    cnt[threadIdx.x] = gcnt[threadIdx.x];
+    if (threadIdx.x < nr) dr[threadIdx.x] = (uint32_t)(rec_window<AS>(R.rec(i0 + threadIdx.x), pos) >> kCarryBits);
+    __syncthreads();
+    uint32_t s = 0;
+    for (uint32_t r = 0; r < nr; ++r) s += cnt[threadIdx.x ^ dr[r]];
+    hist[(uint64_t)blockIdx.x * kDigits + threadIdx.x] = s;
+}
+
+// The root ids of one block of rhs rows (blockIdx.x) for one chunk of the
+// grouped lhs positions (blockIdx.y).  Row r's pairs with the lhs of group g
+// start at bucket g ^ dr(r)'s next free slot; adj[r][g] is that slot minus
+// the group's start, so position p goes to adj[r][g(p)] + p (mod 2^32: every
+// target is below n_dense <= 2^32).  A wave's 64 positions fall in one group
+// or a few, so each row's stores are runs of consecutive ids.
+template <int AS>
+__global__ __launch_bounds__(kBlock) void k_group_gen(PairSource<AS, 0> src, uint32_t m, uint32_t n, uint32_t chunk,
+                                                      const uint64_t* __restrict__ live, int kb,
+                                                      const uint64_t* __restrict__ lrec,
+                                                      const uint8_t* __restrict__ lgrp,
+                                                      const uint32_t* __restrict__ gcnt,
+                                                      const uint32_t* __restrict__ hist,
+                                                      const uint64_t* __restrict__ sub_start,
+                                                      IdKey* __restrict__ ids_out) {
+    __shared__ uint32_t adj[kGrpRows][kDigits];
+    __shared__ uint32_t cnt[kDigits];
+    __shared__ uint32_t gst[kDigits];
+    __shared__ uint32_t dr[kGrpRows];
+    __shared__ uint32_t rw[kGrpRows];
+    const int pos = root_pos(live, kb);
+    const bool xw = pos + kDigitBits + kCarryBits <= 2 * AS * 64;  // carried bits lie in the x/z words
+    const uint32_t i0 = blockIdx.x * kGrpRows;
+    const uint32_t nr = min(kGrpRows, m - i0);
+    cnt[threadIdx.x] = gcnt[threadIdx.x];
+    gst[threadIdx.x] = gcnt[kDigits + threadIdx.x];
+    if (threadIdx.x < nr) {
+        const uint64_t win = rec_window<AS>(src.R.rec(i0 + threadIdx.x), pos);
+        dr[threadIdx.x] = (uint32_t)(win >> kCarryBits);
+        rw[threadIdx.x] = (uint32_t)win;
+    }
+    __syncthreads();
+    {
+        const uint32_t d = threadIdx.x;
+        uint64_t next = sub_start[d] + hist[(uint64_t)blockIdx.x * kDigits + d];
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t g = d ^ dr[r];
+            adj[r][g] = (uint32_t)(next - gst[g]);
+            next += cnt[g];
+        }
+    }
+    __syncthreads();
+    const uint32_t p0 = blockIdx.y * chunk;
+    const uint32_t p1 = min(n, p0 + chunk);
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        const uint64_t lr = lrec[p];
+        const uint32_t g = lgrp[p];
+        const uint32_t j = (uint32_t)lr, lw = (uint32_t)(lr >> 32);
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t id = ((i0 + r) << src.lo_bits) | j;
+            const uint32_t bits = xw ? (lw ^ rw[r]) : (uint32_t)key_bits<kDigitBits + kCarryBits>(src, id, pos);
+            ids_out[(size_t)(uint32_t)(adj[r][g] + p)] = IdKey{id, bits};
+        }
     }
 }
 
@@ -1670,6 +1847,66 @@ int get_lists(plh_ctx* ctx, int slot, uint64_t cap, Lists* out) {
 
 inline unsigned grid1(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// The root pass by lhs groups applies to products of at least 2^22 pairs
+// with 2^11..2^22 lhs terms and 2^9 rhs rows or more (below that the
+// grouping and its read of the live mask cost more than the scatter saves),
+// and only when the root digit lies in the x/z words: some x/z bit before
+// the words' last 8 is live.
+template <int AS>
+int root_by_groups_ok(plh_ctx* ctx, const PairSource<AS, 0>& src, uint64_t n_dense, const uint64_t* d_live,
+                      bool* ok) {
+    *ok = false;
+    if (n_dense < (1ull << 22) || src.n_lo < (1u << 11) || src.n_lo > (1u << 22) || src.n_hi < (1u << 9))
+        return PLH_OK;
+    uint64_t w[2 * AS];
+    PLH_TRY(read_u64n(ctx, d_live, 2 * AS, w));
+    for (int k = 0; k < 2 * AS; ++k)
+        if (w[k]) {
+            *ok = 64 * k + __builtin_clzll(w[k]) + kDigitBits <= 2 * AS * 64;
+            break;
+        }
+    return PLH_OK;
+}
+
+template <int AS>
+int root_by_groups(plh_ctx* ctx, const PairSource<AS, 0>& src, const uint64_t* d_live, int kb, IdKey* ids0,
+                   uint64_t* sub_start, uint64_t* sub_size) {
+    hipStream_t st = ctx->stream;
+    const uint32_t n = src.n_lo, m = src.n_hi;
+    const size_t lbytes = ((size_t)n * 8 + 255) & ~(size_t)255;
+    const size_t gbytes = ((size_t)n + 255) & ~(size_t)255;
+    void* p = nullptr;
+    PLH_TRY(ctx->ws.get(kWsGroup, lbytes + gbytes + 2 * kDigits * 4, &p));
+    uint64_t* lrec = (uint64_t*)p;
+    uint8_t* lgrp = (uint8_t*)p + lbytes;
+    uint32_t* gcnt = (uint32_t*)((char*)p + lbytes + gbytes);
+    const uint64_t nb = (m + kGrpRows - 1) / kGrpRows;
+    PLH_TRY(ctx->ws.get(kWsHist, nb * kDigits * 4, &p));
+    uint32_t* hist = (uint32_t*)p;
+    // chunks of the lhs positions: about 4096 workgroups, at least 1024
+    // positions each
+    uint64_t nc = std::max<uint64_t>(1, (4096 + nb - 1) / nb);
+    nc = std::min<uint64_t>(nc, std::max<uint64_t>(1, n / 1024));
+    const uint32_t chunk = (uint32_t)(((n + nc - 1) / nc + kBlock - 1) / kBlock * kBlock);
+    nc = (n + chunk - 1) / chunk;
+    {
+        ProfScope ps(ctx, "msd_hist_first");
+        k_group_lhs<AS><<<1, kGrpThreads, 0, st>>>(src.L, n, d_live, kb, lrec, lgrp, gcnt);
+        k_group_hist<AS><<<dim3((unsigned)nb), kBlock, 0, st>>>(src.R, m, d_live, kb, gcnt, hist);
+    }
+    {
+        ProfScope ps(ctx, "colscan");
+        k_root_colscan<<<1, kColThreads, 0, st>>>(nb, hist, sub_start, sub_size);
+    }
+    {
+        ProfScope ps(ctx, "msd_scatter_first", 8.0 * (double)n * (double)m);
+        k_group_gen<AS><<<dim3((unsigned)nb, (unsigned)nc), kBlock, 0, st>>>(src, m, n, chunk, d_live, kb, lrec, lgrp,
+                                                                            gcnt, hist, sub_start, ids0);
+    }
+    PLH_HIP(hipGetLastError());
+    return PLH_OK;
+}
+
 template <int AS, class Src>
 int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpec& spec, plh_sum* out,
                 int out_as) {
@@ -1719,25 +1956,34 @@ int collate_run(plh_ctx* ctx, const Src& src, uint64_t n_dense, const CollateSpe
 
     // ---- pass 0: implicit ids, one root bucket (its digit position is the
     // first live bit, found by each kernel from the live mask).
-    const uint32_t tile0 = pick_tile(n_dense, ctx->tune.tile_root_max ? ctx->tune.tile_root_max : kTileMaxImplicit);
-    const uint64_t t0 = (n_dense + tile0 - 1) / tile0;
-    PLH_TRY(ctx->ws.get(kWsHist, t0 * kDigits * 4, &p));
-    uint32_t* hist = (uint32_t*)p;
     PLH_TRY(ctx->ws.get(kWsSub, 2 * kDigits * 8, &p));
     uint64_t* sub_start = (uint64_t*)p;
     uint64_t* sub_size = sub_start + kDigits;
-    {
-        ProfScope ps(ctx, "msd_hist_first");
-        k_root_hist<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, n_dense, d_live, kb, tile0, hist);
+    uint32_t* hist = nullptr;  // per-tile digit histograms (the later passes reuse the slot)
+    bool grouped = false;
+    if constexpr (GroupRoot<Src>::value) {
+        PLH_TRY(root_by_groups_ok<AS>(ctx, src, n_dense, d_live, &grouped));
+        if (grouped) PLH_TRY(root_by_groups<AS>(ctx, src, d_live, kb, ids0, sub_start, sub_size));
     }
-    {
-        ProfScope ps(ctx, "colscan");
-        k_root_colscan<<<1, kColThreads, 0, st>>>(t0, hist, sub_start, sub_size);
-    }
-    {
-        ProfScope ps(ctx, "msd_scatter_first", 8.0 * (double)n_dense);
-        k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, d_live, kb, tile0, hist,
-                                                                    sub_start);
+    if (!grouped) {
+        const uint32_t tile0 =
+            pick_tile(n_dense, ctx->tune.tile_root_max ? ctx->tune.tile_root_max : kTileMaxImplicit);
+        const uint64_t t0 = (n_dense + tile0 - 1) / tile0;
+        PLH_TRY(ctx->ws.get(kWsHist, t0 * kDigits * 4, &p));
+        hist = (uint32_t*)p;
+        {
+            ProfScope ps(ctx, "msd_hist_first");
+            k_root_hist<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, n_dense, d_live, kb, tile0, hist);
+        }
+        {
+            ProfScope ps(ctx, "colscan");
+            k_root_colscan<<<1, kColThreads, 0, st>>>(t0, hist, sub_start, sub_size);
+        }
+        {
+            ProfScope ps(ctx, "msd_scatter_first", 8.0 * (double)n_dense);
+            k_root_scatter<Src><<<dim3((unsigned)t0), kBlock, 0, st>>>(src, ids0, n_dense, d_live, kb, tile0, hist,
+                                                                        sub_start);
+        }
     }
     PLH_HIP(hipGetLastError());
     // list arrays (isbig u32, cnt u32, bigpos u64, bigsz u64, newpos u64) for a

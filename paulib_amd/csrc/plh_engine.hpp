@@ -245,6 +245,7 @@ enum : int {
     kWsRotM,    // sort-free rotation: merge flags, lower bounds, sums, output scan
     kWsKeep,    // trim with keep: protected patterns and flags
     kWsRotE,    // sort-free rotation: per-job minima and 16-tile minima, epoch-tagged, kept between passes
+    kWsGroup,   // collated product: the lhs grouped by root digit and the group sizes
     kWsCount
 };
 
