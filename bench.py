@@ -180,7 +180,7 @@ def serving(ctx, n_jobs: int, rate: float, n_plain: int, max_jobs: int, sustain_
         "speedup_vs_plain": over["steady_jobs_per_s"] / plain_rate_stream,
         "speedup_vs_plain_whole_stream": over["jobs_per_s"] / plain_rate_stream,
         "served_vs_plain": {"max_abs_expectation_diff": max_diff, "term_count_mismatches": term_mismatch},
-    }
+    }, served
 
 
 def batched_products(ctx, count: int, budget_pairs: int, solo_pairs: int):
@@ -568,6 +568,89 @@ def cpu_baseline(A, B, target_s: float, gpu_c2=None, gpu_trim=None):
     }
 
 
+def _cpu_kind():
+    import oracle
+
+    return "reference" if oracle.available("reference") else "oracle"
+
+
+def cpu_batch_sample(count: int, target_s: float, gpu):
+    """C3 on the host cores: products of the same batch in a seeded random
+    order (those of at most 2^24 pairs, so that a sample holds many of
+    them), multiply + canonicalize each, until target_s has passed."""
+    import oracle
+    from paulib_amd import workloads as W
+
+    kind = _cpu_kind()
+    spec = W.batch_products(count)
+    rng = np.random.Generator(np.random.PCG64(77))
+    order = [int(k) for k in rng.permutation(len(spec)) if spec[k][1] * spec[k][2] <= 1 << 24]
+    pairs, t, jobs = 0, 0.0, 0
+    for k in order:
+        nq, n, m, seed = spec[k]
+        a = oracle.CpuSum.from_host(kind, W.random_sum(n, nq, 0.75, "uniform", seed))
+        b = oracle.CpuSum.from_host(kind, W.random_sum(m, nq, 0.75, "uniform", seed + 1))
+        t0 = time.perf_counter()
+        a.multiply(b)
+        a.canonicalize()
+        t += time.perf_counter() - t0
+        pairs += n * m
+        jobs += 1
+        del a, b
+        if t >= target_s:
+            break
+    rate = pairs / t
+    log(f"cpu baseline: C3 sample {jobs} products, {pairs:.4g} pairs in {t:.1f} s ({rate:.3g} pairs/s)")
+    return {"jobs": jobs, "pairs": pairs, "s": t, "pairs_per_s": rate,
+            "sample": (f"{jobs} of the {count} products in a seeded random order (those of <= 2^24 pairs), "
+                       "multiply + canonicalize each"),
+            "gpu_batch_pairs_per_s": gpu["pairs_per_s"], "speedup_gpu_batch": gpu["pairs_per_s"] / rate}
+
+
+def cpu_serving_sample(n_jobs: int, rate_hz: float, target_s: float, gpu, served):
+    """C4 on the host cores: jobs of the same stream (the plain loop's seeded
+    sample order) through the reference's run_program and
+    zero_state_expectation, until target_s has passed; the rate is scaled to
+    the stream's mean rotations like the plain GPU loop's, and each job's
+    expectation and term count are compared with the served ones."""
+    import oracle
+    from paulib_amd import workloads as W
+
+    kind = _cpu_kind()
+    jobs = W.serving_jobs(n_jobs, rate_hz, seed=9)
+    rng = np.random.Generator(np.random.PCG64(2024))
+    order = rng.permutation(n_jobs).tolist()
+    t, done, rots, max_diff, mism = 0.0, 0, [], 0.0, 0
+    for k in order:
+        _, O, prog, cut = jobs[k]
+        o = oracle.CpuSum.from_host(kind, O)
+        t0 = time.perf_counter()
+        o.run_program(prog, 8, cut)
+        e = o.zero_state_expectation()
+        t += time.perf_counter() - t0
+        done += 1
+        rots.append(prog.num_rotations())
+        if k in served:
+            e_s, n_s = served[k]
+            max_diff = max(max_diff, abs(e - e_s))
+            mism += int(n_s != len(o))
+        del o
+        if t >= target_s:
+            break
+    rate = done / t
+    rots_all = [p.num_rotations() for _, _, p, _ in jobs]
+    rate_stream = rate * float(np.mean(rots)) / float(np.mean(rots_all))
+    log(f"cpu baseline: C4 sample {done} jobs in {t:.1f} s ({rate:.3f} jobs/s, {rate_stream:.3f} scaled to the "
+        f"stream); served vs cpu: max |<O> diff| {max_diff:.2e}, {mism} term-count mismatches")
+    return {"jobs": done, "s": t, "jobs_per_s": rate, "sample_rotations_mean": float(np.mean(rots)),
+            "jobs_per_s_stream": rate_stream,
+            "sample": (f"{done} jobs of the {n_jobs}-job stream in a seeded random order, run_program + "
+                       "zero_state_expectation each (max_weight 8, the job's cutoff)"),
+            "served_steady_jobs_per_s": gpu["steady_jobs_per_s"],
+            "speedup_served": gpu["steady_jobs_per_s"] / rate_stream,
+            "served_vs_cpu": {"max_abs_expectation_diff": max_diff, "term_count_mismatches": mism}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -788,12 +871,18 @@ def main():
     if args.batch_jobs > 0:
         result["batched_products"] = batched_products(ctx, args.batch_jobs, args.batch_pairs, args.solo_pairs)
     if args.serve_jobs > 0:
-        result["serving"] = serving(ctx, args.serve_jobs, args.serve_rate, args.serve_plain, args.serve_slots,
-                                    args.serve_sustain, args.serve_sustain_jobs)
+        result["serving"], served = serving(ctx, args.serve_jobs, args.serve_rate, args.serve_plain,
+                                            args.serve_slots, args.serve_sustain, args.serve_sustain_jobs)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds, result.get("propagation"),
                                               result.get("propagation_trim"))
+        if args.batch_jobs > 0:
+            result["cpu_baseline"]["c3"] = cpu_batch_sample(args.batch_jobs, args.cpu_seconds,
+                                                            result["batched_products"])
+        if args.serve_jobs > 0:
+            result["cpu_baseline"]["c4"] = cpu_serving_sample(args.serve_jobs, args.serve_rate,
+                                                              1.5 * args.cpu_seconds, result["serving"], served)
     print(json.dumps(result), flush=True)
 
 
