@@ -1247,6 +1247,7 @@ struct LeafLds {
     int pos;
     uint32_t nvtot;                    // varying bits in the sort key (<= 64)
     uint32_t pcs;                      // set sort-key bits, summed over the leaf
+    uint32_t dup;                      // nonzero when two items share a sort key
     uint32_t red[kLeafWaves];
 };
 
@@ -1510,6 +1511,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             if (L.diff[w]) pos = 64 * w + __clzll(L.diff[w]);
         L.pos = pos;
         L.pcs = 0;
+        L.dup = 0;
         int tot = 0;
         for (int w = 0; w < KW + 1; ++w) tot += popc(L.diff[w]);
         L.nvtot = (uint32_t)min(tot, 64);
@@ -1549,6 +1551,9 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         // key, and the slot (= id order) fills the padding: every sort key
         // is then distinct and already carries the (key, id) order.
         const bool complete = !sparse && L.nvtot + D::kSlotBits <= 64;
+        // complete sort keys carry the slot, so they never tie even where
+        // the keys do: run heads must then be found from the keys
+        if (complete && tid == 0) L.dup = 1u;
         // counting sort on the kDl leading bits of the sort key ...
         uint32_t dig[kPer], slot[kPer];
         uint64_t S[kPer];
@@ -1625,6 +1630,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
                     }
                     r += lt;
                     if (eq > 1) {  // other items share this sort key (eq counts this one too)
+                        L.dup = 1u;
                         for (t = s0; t < s1; ++t)
                             if (t != q && L.bkey[t] == ms && slot_less<AS, KW>(L, c.fmask, L.perm[t], me)) ++r;
                     }
@@ -1638,14 +1644,20 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     }
     // ---- values in sorted order and run heads, by all threads (the sort
     // keys and the second permutation are free until the output step)
+    // When no two items share a sort key (a sort key is a function of the
+    // key and the masked flags), every item is a run of its own: the run
+    // heads and the sorted values are not needed.
+    const bool distinct = pos >= 0 && L.dup == 0u;
     double* vs = reinterpret_cast<double*>(L.bkey);
     uint16_t* head = L.perm2;
-    for (uint32_t q = tid; q < n; q += kLeafThreads) {
-        const int sq = L.perm[q];
-        vs[q] = L.val[sq];
-        head[q] = (q == 0 || !slot_key_eq<AS, KW>(L, c.fmask, sq, L.perm[q - 1])) ? 1 : 0;
+    if (!distinct) {
+        for (uint32_t q = tid; q < n; q += kLeafThreads) {
+            const int sq = L.perm[q];
+            vs[q] = L.val[sq];
+            head[q] = (q == 0 || !slot_key_eq<AS, KW>(L, c.fmask, sq, L.perm[q - 1])) ? 1 : 0;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // ---- runs: thread t owns sorted positions [t*kPer, t*kPer + kPer);
     // each run is summed acc = 0; acc += v in (key, id) order
     // Slots are indexed by k (compile-time after unrolling), never by a
@@ -1654,7 +1666,26 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     bool kept[kPer];
     uint32_t nk = 0;
     const uint32_t q0 = (uint32_t)tid * kPer;
-    {
+    if (distinct) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t q = q0 + k;
+            kept[k] = false;
+            sums[k] = 0.0;
+            if (q >= n) continue;
+            const int s = L.perm[q];
+            double acc = 0.0;
+            acc += L.val[s];
+            uint64_t xz[KW];
+#pragma unroll
+            for (int w = 0; w < KW; ++w) xz[w] = L.key[w][s];
+            if (keep_term<AS, XO>(c, xz, acc)) {
+                sums[k] = acc;
+                kept[k] = true;
+                ++nk;
+            }
+        }
+    } else {
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t q = q0 + k;
