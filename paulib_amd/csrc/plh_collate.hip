@@ -1236,8 +1236,8 @@ struct LeafLds {
     uint64_t key[KW][D::kCap];         // key words (job word first when present, then x, z)
     double val[D::kCap];               // folded coefficients; run sums at head slots
     uint64_t bkey[D::kCap];            // sort keys, in bin order
-    uint16_t perm[D::kCap];            // sorted position -> slot
-    uint16_t perm2[D::kCap];           // bin order -> (key, id) order; later rslot
+    uint16_t perm[D::kCap];            // bin order -> slot (identity when all keys are equal)
+    uint16_t perm2[D::kCap];           // (key, id) order -> slot; the spare array holds run heads, then rslot
     uint8_t fl[D::kCap];               // raw flags
     uint32_t bin[D::kBins + 1];        // counting-sort bin starts
     unsigned long long diff[KW + 1];   // varying bits of each key word
@@ -1526,6 +1526,10 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     for (int b = tid; b < D::kBins; b += kLeafThreads) L.bin[b] = 0;
     __syncthreads();
     const int pos = L.pos;
+    // sp: sorted position -> slot (perm when all keys are equal, else
+    // perm2); spare: the other array, free once the order is known
+    uint16_t* sp = L.perm;
+    uint16_t* spare = L.perm2;
     if (pos < 0) {
         // all keys equal; ids ascend -> identity order
         for (uint32_t p = tid; p < n; p += kLeafThreads) L.perm[p] = (uint16_t)p;
@@ -1639,8 +1643,8 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             }
         }
         __syncthreads();
-        for (uint32_t p = tid; p < n; p += kLeafThreads) L.perm[p] = L.perm2[p];
-        __syncthreads();
+        sp = L.perm2;
+        spare = L.perm;
     }
     // ---- values in sorted order and run heads, by all threads (the sort
     // keys and the second permutation are free until the output step)
@@ -1649,12 +1653,12 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     // heads and the sorted values are not needed.
     const bool distinct = pos >= 0 && L.dup == 0u;
     double* vs = reinterpret_cast<double*>(L.bkey);
-    uint16_t* head = L.perm2;
+    uint16_t* head = spare;
     if (!distinct) {
         for (uint32_t q = tid; q < n; q += kLeafThreads) {
-            const int sq = L.perm[q];
+            const int sq = sp[q];
             vs[q] = L.val[sq];
-            head[q] = (q == 0 || !slot_key_eq<AS, KW>(L, c.fmask, sq, L.perm[q - 1])) ? 1 : 0;
+            head[q] = (q == 0 || !slot_key_eq<AS, KW>(L, c.fmask, sq, sp[q - 1])) ? 1 : 0;
         }
         __syncthreads();
     }
@@ -1673,7 +1677,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             kept[k] = false;
             sums[k] = 0.0;
             if (q >= n) continue;
-            const int s = L.perm[q];
+            const int s = sp[q];
             double acc = 0.0;
             acc += L.val[s];
             uint64_t xz[KW];
@@ -1714,7 +1718,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
                 }
             }
             uint64_t xz[KW];
-            const int s = L.perm[q];
+            const int s = sp[q];
 #pragma unroll
             for (int w = 0; w < KW; ++w) xz[w] = L.key[w][s];
             if (keep_term<AS, XO>(c, xz, acc)) {
@@ -1727,14 +1731,14 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     uint32_t total;
     const uint32_t r0 = block_exclusive_scan<kLeafWaves>(nk, total, L.red);
     // the scan's barriers order the run sums (and their reads of head[])
-    // before these writes; perm is unchanged since the sort
-    uint16_t* rslot = L.perm2;
+    // before these writes; sp is unchanged since the sort
+    uint16_t* rslot = spare;
     {
         uint32_t r = r0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             if (!kept[k]) continue;
-            const int s = L.perm[q0 + k];
+            const int s = sp[q0 + k];
             rslot[r++] = (uint16_t)s;
             L.val[s] = sums[k];
         }
