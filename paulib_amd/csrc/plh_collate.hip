@@ -1240,7 +1240,9 @@ struct LeafLds {
     uint16_t perm2[D::kCap];           // (key, id) order -> slot; the spare array holds run heads, then rslot
     uint8_t fl[D::kCap];               // raw flags
     uint32_t bin[D::kBins + 1];        // counting-sort bin starts
-    unsigned long long diff[KW + 1];   // varying bits of each key word
+    unsigned long long diff[KW + 1];   // OR of each key word over the leaf
+    unsigned long long dand[KW + 1];   // AND of each key word over the leaf
+    uint64_t vmask[KW + 1];            // varying bits of each key word: diff & ~dand
     uint64_t cmask[KW + 1][6];         // compress masks of diff
     int nvary[KW + 1];                 // varying bits per word
     int crun[KW + 1];                  // 1 + shift when a word's varying bits are one run, else 0
@@ -1308,15 +1310,15 @@ PLH_HD uint64_t compress_bits(uint64_t x, uint64_t m, const uint64_t* mv) {
 // left-aligned.  Bits that do not vary inside a leaf do not order it, so
 // this is an order-preserving key that is complete when at most 64 bits vary.
 template <int AS, int KW>
-PLH_DEV uint64_t varying_key(const LeafLds<AS, KW>& L, uint32_t fmask, int i) {
+PLH_DEV uint64_t varying_key(const LeafLds<AS, KW>& L, const uint64_t (&kw)[KW + 1]) {
     uint64_t K = 0;
     int used = 0;
 #pragma unroll
     for (int w = 0; w < KW + 1; ++w) {
         const int nv = L.nvary[w];
         if (nv == 0 || used >= 64) continue;
-        const uint64_t m = L.diff[w];
-        const uint64_t x = lds_word(L, fmask, i, w);
+        const uint64_t m = L.vmask[w];
+        const uint64_t x = kw[w];
         // a contiguous run of varying bits needs only a mask and a shift
         const uint64_t c = L.crun[w] ? (x & m) >> (L.crun[w] - 1) : compress_bits(x, m, L.cmask[w]);
         const int room = 64 - used;
@@ -1454,8 +1456,15 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         return;
     }
     const uint32_t n = (uint32_t)size;
-    if (tid < KW + 1) L.diff[tid] = 0;
-    // ---- load: keys, raw flags, folded values (loads of kPer items batched)
+    if (tid < KW + 1) {
+        L.diff[tid] = 0;
+        L.dand[tid] = ~0ull;
+    }
+    // ---- load: keys, raw flags, folded values (loads of kPer items batched).
+    // Each thread keeps its items' key words and flags in registers for the
+    // varying-bit masks and the sort keys.
+    uint64_t xz[kPer][KW];
+    uint32_t fl[kPer];
     {
         uint32_t idv[kPer];
 #pragma unroll
@@ -1465,8 +1474,6 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
         }
         // every gather into registers first, then the LDS stores: the
         // stores cannot then hold back the next item's loads
-        uint64_t xz[kPer][KW];
-        uint32_t fl[kPer];
         double bv[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
@@ -1484,40 +1491,56 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
             }
         }
     }
-    __syncthreads();
-    // ---- first bit where the keys differ
+    // ---- bits where the keys differ: set in some item and clear in another
     {
-        uint64_t dl[KW + 1];
+        uint64_t vo[KW + 1], va[KW + 1];
 #pragma unroll
-        for (int w = 0; w < KW + 1; ++w) dl[w] = 0;
+        for (int w = 0; w < KW + 1; ++w) {
+            vo[w] = 0;
+            va[w] = ~0ull;
+        }
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const uint32_t p = tid + k * kLeafThreads;
             if (p < n) {
 #pragma unroll
-                for (int w = 0; w < KW + 1; ++w) dl[w] |= lds_word(L, c.fmask, p, w) ^ lds_word(L, c.fmask, 0, w);
+                for (int w = 0; w < KW; ++w) {
+                    vo[w] |= xz[k][w];
+                    va[w] &= xz[k][w];
+                }
+                const uint64_t fw = (uint64_t)(fl[k] & 0xFFu & c.fmask) << 56;
+                vo[KW] |= fw;
+                va[KW] &= fw;
             }
         }
+        __syncthreads();  // the accumulators are set; the LDS stores are done
 #pragma unroll
         for (int w = 0; w < KW + 1; ++w) {
-            const uint64_t v = wave_or(dl[w]);
-            if (lane_id() == 0 && v) atomicOr(&L.diff[w], (unsigned long long)v);
+            const uint64_t o = wave_or(vo[w]);
+            const uint64_t a = wave_and(va[w]);
+            if (lane_id() == 0) {
+                if (o) atomicOr(&L.diff[w], (unsigned long long)o);
+                if (~a) atomicAnd(&L.dand[w], (unsigned long long)a);
+            }
         }
     }
     __syncthreads();
     if (tid == 0) {
         int pos = -1;
-        for (int w = 0; w < KW + 1 && pos < 0; ++w)
-            if (L.diff[w]) pos = 64 * w + __clzll(L.diff[w]);
+        for (int w = 0; w < KW + 1 && pos < 0; ++w) {
+            const uint64_t m = L.diff[w] & ~L.dand[w];
+            if (m) pos = 64 * w + __clzll(m);
+        }
         L.pos = pos;
         L.pcs = 0;
         L.dup = 0;
         int tot = 0;
-        for (int w = 0; w < KW + 1; ++w) tot += popc(L.diff[w]);
+        for (int w = 0; w < KW + 1; ++w) tot += popc(L.diff[w] & ~L.dand[w]);
         L.nvtot = (uint32_t)min(tot, 64);
     }
     if (tid < KW + 1) {
-        const uint64_t m = L.diff[tid];
+        const uint64_t m = L.diff[tid] & ~L.dand[tid];
+        L.vmask[tid] = m;
         L.nvary[tid] = popc(m);
         compress_masks(m, L.cmask[tid]);
         const int tz = m ? __ffsll((long long)m) - 1 : 0;
@@ -1543,7 +1566,14 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const uint32_t p = tid + k * kLeafThreads;
-                K[k] = p < n ? varying_key<AS, KW>(L, c.fmask, (int)p) : 0ull;
+                K[k] = 0ull;
+                if (p < n) {
+                    uint64_t kw[KW + 1];
+#pragma unroll
+                    for (int w = 0; w < KW; ++w) kw[w] = xz[k][w];
+                    kw[KW] = (uint64_t)(fl[k] & 0xFFu & c.fmask) << 56;
+                    K[k] = varying_key<AS, KW>(L, kw);
+                }
                 pc += (uint32_t)popc(K[k]);
             }
             pc = wave_sum(pc);
