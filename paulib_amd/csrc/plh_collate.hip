@@ -1701,23 +1701,70 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     uint32_t nk = 0;
     const uint32_t q0 = (uint32_t)tid * kPer;
     if (distinct) {
+        // Items are owned strided (q = tid + k * kLeafThreads), so that when
+        // every item is kept, item q goes to out_base + q straight from
+        // registers and a wave's stores are consecutive.
+        double dsum[kPer];
+        bool dkeep[kPer];
+        uint64_t dw[kPer][KW];
+        uint32_t dfl[kPer];
+        uint32_t dn = 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            const uint32_t q = q0 + k;
-            kept[k] = false;
-            sums[k] = 0.0;
+            const uint32_t q = tid + k * kLeafThreads;
+            dkeep[k] = false;
+            dsum[k] = 0.0;
+            dfl[k] = 0;
             if (q >= n) continue;
             const int s = sp[q];
             double acc = 0.0;
             acc += L.val[s];
-            uint64_t xz[KW];
 #pragma unroll
-            for (int w = 0; w < KW; ++w) xz[w] = L.key[w][s];
-            if (keep_term<AS, XO>(c, xz, acc)) {
-                sums[k] = acc;
-                kept[k] = true;
-                ++nk;
+            for (int w = 0; w < KW; ++w) dw[k][w] = L.key[w][s];
+            dfl[k] = L.fl[s] & c.fmask;
+            if (keep_term<AS, XO>(c, dw[k], acc)) {
+                dsum[k] = acc;
+                dkeep[k] = true;
+                ++dn;
             }
+        }
+        uint32_t dtotal;
+        (void)block_exclusive_scan<kLeafWaves>(dn, dtotal, L.red);
+        if (dtotal == n) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const uint32_t q = tid + k * kLeafThreads;
+                if (q >= n) continue;
+                const uint64_t o = out_base + q;
+                for (int w = 0; w < c.out_as; ++w) {
+                    __builtin_nontemporal_store(dw[k][XO + w], &c.ox[(int64_t)w * c.ostride + o]);
+                    __builtin_nontemporal_store(dw[k][XO + AS + w], &c.oz[(int64_t)w * c.ostride + o]);
+                }
+                if (XO && c.oj) __builtin_nontemporal_store((uint32_t)dw[k][0], &c.oj[o]);
+                __builtin_nontemporal_store((uint8_t)dfl[k], &c.of[o]);
+                __builtin_nontemporal_store(dsum[k], &c.ov[o]);
+            }
+            if (tid == 0) kept_out[e] = n;
+            return;
+        }
+        // Some items are dropped: keep marks by sorted position (in the free
+        // sort-key array) and the sums by slot, then the kept items of the
+        // contiguous positions below, as for runs.
+        uint8_t* kmark = reinterpret_cast<uint8_t*>(L.bkey);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t q = tid + k * kLeafThreads;
+            if (q >= n) continue;
+            kmark[q] = dkeep[k] ? 1 : 0;
+            if (dkeep[k]) L.val[sp[q]] = dsum[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t q = q0 + k;
+            kept[k] = q < n && kmark[q] != 0;
+            sums[k] = kept[k] ? L.val[sp[q]] : 0.0;
+            nk += kept[k] ? 1u : 0u;
         }
     } else {
 #pragma unroll
