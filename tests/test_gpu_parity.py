@@ -106,6 +106,27 @@ def test_multiply_canonical_config_1000x1000_q32(ctx):
     assert_terms_equal(got, oracle_product(A, B, True))
 
 
+@pytest.mark.parametrize("as_,nq,n,m,zero_frac", [(1, 64, 3000, 2000, 0.1), (1, 64, 700, 900, 0.9),
+                                                  (2, 128, 1500, 1200, 0.3), (1, 40, 2000, 2000, 1.0)])
+def test_multiply_canonical_zero_coefficients(ctx, as_, nq, n, m, zero_frac):
+    """Operands with zero (and negative-zero) coefficients: on many qubits
+    almost every key is distinct, so leaves keep their items as runs of one
+    and drop the zero products; the kept ones are compacted in order.  Some
+    all-zero operands drop every term."""
+    A = W.random_sum(n, nq, 0.75, "uniform", 11, array_size=as_)
+    B = W.random_sum(m, nq, 0.75, "uniform", 12, array_size=as_)
+    rng = np.random.Generator(np.random.PCG64(13))
+    for S in (A, B):
+        z = rng.random(len(S.value)) < zero_frac
+        neg = rng.random(len(S.value)) < 0.5
+        S.value[z] = np.where(neg[z], -0.0, 0.0)
+    want = oracle_product(A, B, True)
+    got = dev(ctx, A).multiply_canonical(dev(ctx, B)).to_host()
+    assert len(want) < n * m or zero_frac == 0.0
+    assert_terms_equal(got, want)
+    assert not (got.value == 0.0).any()
+
+
 @pytest.mark.parametrize("nq,n,m", [(3, 200, 300), (5, 2000, 900), (2, 64, 64), (8, 4096, 1500)])
 def test_multiply_canonical_heavy_collisions(ctx, nq, n, m):
     """Few qubits: long runs of equal keys, sums of many addends in order."""
