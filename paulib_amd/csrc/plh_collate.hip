@@ -939,6 +939,16 @@ __global__ __launch_bounds__(kDigits * G) void k_pass_colscan(PassBuckets B, uin
     if (lane == 0) C.count[bi] = n;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8), and each XCD has its own L2.  This maps workgroup b < nb to a tile
+// so that XCD x takes one contiguous range of the nb tiles (a bijection on
+// [0, nb)): neighbouring tiles, whose runs per digit meet in shared cache
+// lines, then mostly write through the same L2.
+PLH_DEV uint64_t xcd_contiguous(uint64_t b, uint64_t nb) {
+    const uint64_t x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Stable scatter of a tile into its bucket's sub-buckets (other id buffer);
 // buckets whose digit is constant are left in place.
 template <class Src>
@@ -950,8 +960,9 @@ __global__ __launch_bounds__(kBlock) void k_pass_scatter(Src src, IdKey* __restr
                                                          const uint64_t* __restrict__ live, int kb) {
     __shared__ MsplitShared sh;
     __shared__ uint64_t base[kDigits];
-    const uint64_t t = blockIdx.x;
-    if (t >= *dT) return;
+    const uint64_t nt = *dT;
+    if (blockIdx.x >= nt) return;
+    const uint64_t t = xcd_contiguous(blockIdx.x, nt);
     const uint32_t bi = find_bucket(B.tbase, nbig, t);
     if (B.skip[bi]) return;
     const uint64_t beg = B.start[bi] + (t - B.tbase[bi]) * tile_items;
