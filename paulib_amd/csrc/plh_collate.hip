@@ -86,6 +86,16 @@ PLH_HD uint32_t pick_tile(uint64_t items, uint32_t cap) {
     return (uint32_t)t;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8), and each XCD has its own L2.  This maps workgroup b < nb to a tile
+// so that XCD x takes one contiguous range of the nb tiles (a bijection on
+// [0, nb)): neighbouring tiles, whose outputs meet in shared cache lines, then
+// mostly write through the same L2.
+PLH_DEV uint64_t xcd_contiguous(uint64_t b, uint64_t nb) {
+    const uint64_t x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Kernel-argument block shared by the sort and emit kernels.
 struct Cfg {
     const uint64_t* live;  // live mask, one u64 per key word (MSB-first bits)
@@ -518,7 +528,12 @@ __global__ __launch_bounds__(kBlock) void k_group_gen(PairSource<AS, 0> src, uin
     __shared__ uint32_t rw[kGrpRows];
     const int pos = root_pos(live, kb);
     const bool xw = pos + kDigitBits + kCarryBits <= 2 * AS * 64;  // carried bits lie in the x/z words
-    const uint32_t i0 = blockIdx.x * kGrpRows;
+    // row blocks of one chunk write next to each other in every bucket: an
+    // XCD takes a contiguous run of them (linear order x + y * gridDim.x)
+    const uint64_t lin = xcd_contiguous(blockIdx.x + (uint64_t)blockIdx.y * gridDim.x,
+                                        (uint64_t)gridDim.x * gridDim.y);
+    const uint32_t bx = (uint32_t)(lin % gridDim.x), by = (uint32_t)(lin / gridDim.x);
+    const uint32_t i0 = bx * kGrpRows;
     const uint32_t nr = min(kGrpRows, m - i0);
     cnt[threadIdx.x] = gcnt[threadIdx.x];
     gst[threadIdx.x] = gcnt[kDigits + threadIdx.x];
@@ -530,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_group_gen(PairSource<AS, 0> src, uin
     __syncthreads();
     {
         const uint32_t d = threadIdx.x;
-        uint64_t next = sub_start[d] + hist[(uint64_t)blockIdx.x * kDigits + d];
+        uint64_t next = sub_start[d] + hist[(uint64_t)bx * kDigits + d];
         for (uint32_t r = 0; r < nr; ++r) {
             const uint32_t g = d ^ dr[r];
             adj[r][g] = (uint32_t)(next - gst[g]);
@@ -538,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_group_gen(PairSource<AS, 0> src, uin
         }
     }
     __syncthreads();
-    const uint32_t p0 = blockIdx.y * chunk;
+    const uint32_t p0 = by * chunk;
     const uint32_t p1 = min(n, p0 + chunk);
     for (uint32_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
         const uint64_t lr = lrec[p];
@@ -937,16 +952,6 @@ __global__ __launch_bounds__(kDigits * G) void k_pass_colscan(PassBuckets B, uin
             parts == 1 ? make_meta((uint32_t)child_depth, buf, child_uniform) : make_meta(pos, buf, 0u);
     });
     if (lane == 0) C.count[bi] = n;
-}
-
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
-// b % 8), and each XCD has its own L2.  This maps workgroup b < nb to a tile
-// so that XCD x takes one contiguous range of the nb tiles (a bijection on
-// [0, nb)): neighbouring tiles, whose runs per digit meet in shared cache
-// lines, then mostly write through the same L2.
-PLH_DEV uint64_t xcd_contiguous(uint64_t b, uint64_t nb) {
-    const uint64_t x = b & 7u, q = nb >> 3, r = nb & 7u;
-    return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // Stable scatter of a tile into its bucket's sub-buckets (other id buffer);
@@ -1434,7 +1439,9 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(Src src, Cfg c, const uin
     constexpr int kPer = D::kPer;
     __shared__ LeafLds<AS, KW> L;
     const int tid = threadIdx.x;
-    const uint64_t e = blockIdx.x;
+    // one leaf per workgroup (grid = the leaf count); neighbouring leaves'
+    // outputs are adjacent, so an XCD takes a contiguous run of leaves
+    const uint64_t e = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint64_t start = Lstart[e];
     const uint64_t size = Lsize[e];
     const uint32_t meta = Lmeta[e];
